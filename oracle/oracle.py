@@ -1,0 +1,386 @@
+"""ctypes front end of the CPU oracle (``_build/liboracle.so``) and of the
+reference build (``_ref/libcrtomo_ref.so``).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcrtomo_ref.so")
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+
+
+class _Grid(C.Structure):
+    _fields_ = [("sanz", _i), ("mb", _i), ("typanz", _i), ("typ", _p), ("nelanz", _p),
+                ("selanz", _p), ("maxn", _i), ("nrel", _p), ("x", _p), ("y", _p), ("rnr", _p),
+                ("elanz", _i), ("relanz", _i), ("eanz", _i), ("enr", _p)]
+
+
+class _CG(C.Structure):
+    _fields_ = [("nanz", _i), ("manz", _i), ("ltri", _i), ("nx", _i), ("maxn", _i), ("ncgmax", _i),
+                ("lam", _d), ("eps", _d), ("sens", _p), ("wmatd", _p), ("wdfak", _p), ("dat", _p),
+                ("sigmaa", _p), ("par", _p), ("smatm", _p), ("nachbar", _p)]
+
+
+def build(force: bool = False) -> None:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        for name in ("orc_bessk0", "orc_bessk1", "orc_bessi0", "orc_bessi1", "orc_gammln"):
+            getattr(_lib, name).restype = _d
+            getattr(_lib, name).argtypes = [_d]
+        _lib.orc_gauleg.argtypes = [_d, _d, _p, _p, _i]
+        _lib.orc_gaulag.argtypes = [_p, _p, _i, _d]
+        _lib.orc_bsytop.restype = _d
+    return _lib
+
+
+def ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+class OracleGrid:
+    """Keeps the numpy arrays alive for the C struct."""
+
+    def __init__(self, g):
+        self.g = g
+        self._keep = [g.typ, g.nelanz, g.selanz, g.nrel, g.x, g.y, g.rnr, g.enr]
+        self.s = _Grid(g.sanz, g.mb, g.typanz, ptr(g.typ), ptr(g.nelanz), ptr(g.selanz), g.maxn,
+                       ptr(g.nrel), ptr(g.x), ptr(g.y), ptr(g.rnr), g.elanz, g.relanz, g.eanz,
+                       ptr(g.enr))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _chk(err: int, what: str) -> None:
+    if err != 0:
+        raise RuntimeError(f"oracle {what}: errnr {err}")
+
+
+# ----------------------------------------------------------- scalar helpers
+def bessel(name: str, x: float) -> float:
+    return getattr(lib(), "orc_" + name)(float(x))
+
+
+def gauleg(n: int, x1: float = 0.0, x2: float = 1.0):
+    x = np.zeros(n); w = np.zeros(n)
+    lib().orc_gauleg(x1, x2, ptr(x), ptr(w), n)
+    return x, w
+
+
+def gaulag(n: int, alf: float = 0.0):
+    x = np.zeros(n); w = np.zeros(n)
+    lib().orc_gaulag(ptr(x), ptr(w), n, alf)
+    return x, w
+
+
+# ----------------------------------------------------------- path functions
+def rwaven(grid, survey):
+    og = OracleGrid(grid)
+    kwn = np.zeros(30); kwi = np.zeros(30)
+    n = _i(); amin = _d(); amax = _d()
+    err = lib().orc_rwaven(og.ref, survey.nanz, ptr(survey.strnr), ptr(survey.vnr), C.byref(n),
+                           ptr(kwn), ptr(kwi), C.byref(amin), C.byref(amax))
+    _chk(err, "rwaven")
+    return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
+
+
+def precal(grid, kwn, lsytop: bool = False):
+    og = OracleGrid(grid)
+    kwn = np.ascontiguousarray(kwn, np.float64)
+    K = kwn.size
+    elbg = np.zeros((K, grid.npk, grid.elanz))
+    relbg = np.zeros((grid.npk, max(grid.relanz, 1)))
+    kg = np.zeros((K, grid.eanz, max(grid.relanz, 1)))
+    lbeta = _i(); sytop = _d()
+    err = lib().orc_precal(og.ref, K, ptr(kwn), int(lsytop), ptr(elbg), ptr(relbg), ptr(kg),
+                           C.byref(lbeta), C.byref(sytop))
+    _chk(err, "precal")
+    return elbg, relbg, kg, bool(lbeta.value)
+
+
+def forward(grid, sigma, elbg, relbg, kg, lbeta, systems=None):
+    """kpot (kwnanz, eanz, sanz) = the reference's kpot(sanz, eanz, kwnanz)."""
+    og = OracleGrid(grid)
+    sigma = np.ascontiguousarray(sigma, np.complex128)
+    K = elbg.shape[0]
+    kpot = np.zeros((K, grid.eanz, grid.sanz), np.complex128)
+    if systems is None:
+        n, sl, sk = 0, None, None
+    else:
+        sl = np.ascontiguousarray([s[0] for s in systems], np.int32)
+        sk = np.ascontiguousarray([s[1] for s in systems], np.int32)
+        n = sl.size
+    err = lib().orc_forward(og.ref, ptr(sigma), K, ptr(elbg), ptr(relbg), ptr(kg), int(lbeta),
+                            ptr(kpot), n, ptr(sl), ptr(sk))
+    _chk(err, "forward")
+    return kpot
+
+
+def rtrafo(kpot, kwnwi):
+    K, E, S = kpot.shape
+    hpot = np.zeros((E, S), np.complex128)
+    kwnwi = np.ascontiguousarray(kwnwi, np.float64)
+    lib().orc_rtrafo(S, E, K, ptr(kwnwi), ptr(np.ascontiguousarray(kpot)), ptr(hpot))
+    return hpot
+
+
+def bvolti(grid, survey, hpot):
+    volt = np.zeros(survey.nanz, np.complex128); sa = np.zeros_like(volt)
+    err = lib().orc_bvolti(survey.nanz, ptr(survey.strnr), ptr(survey.vnr), ptr(grid.enr), grid.sanz,
+                           ptr(np.ascontiguousarray(hpot)), ptr(volt), ptr(sa))
+    _chk(err, "bvolti")
+    return volt, sa
+
+
+def bvolt(grid, survey, hpot, kfak=None):
+    volt = np.zeros(survey.nanz, np.complex128); sa = np.zeros_like(volt)
+    kf = None if kfak is None else np.ascontiguousarray(kfak, np.float64)
+    lib().orc_bvolt(survey.nanz, ptr(survey.strnr), ptr(survey.vnr), ptr(grid.enr), grid.sanz,
+                    ptr(np.ascontiguousarray(hpot)), ptr(kf), ptr(volt), ptr(sa))
+    return volt, sa
+
+
+def bsensi(grid, survey, kwnwi, kpot, elbg, sigma, volt):
+    """sens (manz, nanz) = the reference's sens(nanz, manz)."""
+    og = OracleGrid(grid)
+    sens = np.zeros((grid.elanz, survey.nanz), np.complex128)
+    lib().orc_bsensi(og.ref, survey.nanz, ptr(survey.strnr), ptr(survey.vnr), kpot.shape[0],
+                     ptr(np.ascontiguousarray(kwnwi, np.float64)), ptr(np.ascontiguousarray(kpot)),
+                     ptr(elbg), ptr(np.ascontiguousarray(sigma, np.complex128)),
+                     ptr(np.ascontiguousarray(volt, np.complex128)), ptr(sens))
+    return sens
+
+
+def bsens(grid, survey, kwnwi, kpot, elbg, strom=None):
+    og = OracleGrid(grid)
+    st = np.ones(survey.nanz) if strom is None else np.ascontiguousarray(strom, np.float64)
+    sens = np.zeros((grid.elanz, survey.nanz), np.complex128)
+    lib().orc_bsens(og.ref, survey.nanz, ptr(survey.strnr), ptr(survey.vnr), ptr(st), kpot.shape[0],
+                    ptr(np.ascontiguousarray(kwnwi, np.float64)), ptr(np.ascontiguousarray(kpot)),
+                    ptr(elbg), ptr(sens))
+    return sens
+
+
+def kompab(grid, sigma, elbg, relbg, kg, l, k):
+    og = OracleGrid(grid)
+    a = np.zeros((grid.mb + 1) * grid.sanz, np.complex128)
+    b = np.zeros(grid.sanz, np.complex128)
+    err = lib().orc_kompab(og.ref, ptr(np.ascontiguousarray(sigma, np.complex128)), ptr(elbg), ptr(relbg),
+                           ptr(kg), elbg.shape[0], l, k, ptr(a), ptr(b))
+    _chk(err, "kompab")
+    return a, b
+
+
+def scalab(sanz, mb, a, b):
+    fak = np.zeros(sanz)
+    err = lib().orc_scalab(sanz, mb, ptr(a), ptr(b), ptr(fak))
+    _chk(err, "scalab")
+    return fak
+
+
+def chol(sanz, mb, a):
+    return lib().orc_chol(sanz, mb, ptr(a))
+
+
+def vre(sanz, mb, a, b):
+    pot = np.zeros(sanz, np.complex128)
+    lib().orc_vre(sanz, mb, ptr(a), ptr(b), ptr(pot))
+    return pot
+
+
+def cholz(a_colmajor: np.ndarray):
+    n = a_colmajor.shape[0]
+    a = np.asfortranarray(a_colmajor.astype(np.complex128)).copy(order="F")
+    p = np.zeros(n, np.complex128)
+    err = lib().orc_cholz(a.ctypes.data_as(_p), ptr(p), n)
+    return err, a, p
+
+
+def bsmatmreg(grid, nx, nz, alfx, alfz):
+    og = OracleGrid(grid)
+    sm = np.zeros((3, grid.elanz))
+    lib().orc_bsmatmreg(og.ref, nx, nz, _d(alfx), _d(alfz), ptr(sm))
+    return sm
+
+
+def bnachbar(grid):
+    og = OracleGrid(grid)
+    nb = np.zeros((grid.maxn + 1, grid.elanz), np.int32)
+    lib().orc_bnachbar(og.ref, ptr(nb))
+    return nb
+
+
+def bsmatmtri(grid, nachbar, alfx, alfz):
+    og = OracleGrid(grid)
+    sm = np.zeros((grid.maxn + 1, grid.elanz))
+    lib().orc_bsmatmtri(og.ref, ptr(np.ascontiguousarray(nachbar, np.int32)), _d(alfx), _d(alfz), ptr(sm))
+    return sm
+
+
+class CGProblem:
+    """Arrays in the reference layouts: sens (manz, nanz) row-major = sens(nanz, manz)."""
+
+    def __init__(self, sens, wmatd, wdfak, dat, sigmaa, par, smatm, ltri, nx, lam, eps, ncgmax,
+                 nachbar=None, maxn=4):
+        self.arrs = dict(
+            sens=np.ascontiguousarray(sens, np.complex128), wmatd=np.ascontiguousarray(wmatd, np.float64),
+            wdfak=np.ascontiguousarray(wdfak, np.int32), dat=np.ascontiguousarray(dat, np.complex128),
+            sigmaa=np.ascontiguousarray(sigmaa, np.complex128), par=np.ascontiguousarray(par, np.complex128),
+            smatm=np.ascontiguousarray(smatm, np.float64),
+            nachbar=None if nachbar is None else np.ascontiguousarray(nachbar, np.int32))
+        manz, nanz = self.arrs["sens"].shape
+        a = self.arrs
+        self.s = _CG(nanz, manz, ltri, nx, maxn, ncgmax, lam, eps, ptr(a["sens"]), ptr(a["wmatd"]),
+                     ptr(a["wdfak"]), ptr(a["dat"]), ptr(a["sigmaa"]), ptr(a["par"]), ptr(a["smatm"]),
+                     ptr(a["nachbar"]))
+        self.manz, self.nanz = manz, nanz
+
+
+def update_rhs(p: CGProblem):
+    cgfac = np.zeros(p.manz); bvec = np.zeros(p.manz, np.complex128)
+    lib().orc_update_rhs(C.byref(p.s), ptr(cgfac), ptr(bvec))
+    return cgfac, bvec
+
+
+def cjggra(p: CGProblem, cgfac, bvec):
+    dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1)
+    ncg = _i()
+    lib().orc_cjggra(C.byref(p.s), ptr(cgfac), ptr(np.ascontiguousarray(bvec)), ptr(dpar), ptr(cgres),
+                     C.byref(ncg))
+    return dpar, cgres, ncg.value
+
+
+def cg_matvec(p: CGProblem, cgfac, pvec):
+    ap = np.zeros(p.nanz, np.complex128); out = np.zeros(p.manz, np.complex128)
+    lib().orc_cg_matvec(C.byref(p.s), ptr(cgfac), ptr(np.ascontiguousarray(pvec, np.complex128)), ptr(ap),
+                        ptr(out))
+    return ap, out
+
+
+def cutmck(grid):
+    perm = np.zeros(grid.sanz, np.int32); st = _i()
+    mb = lib().orc_cutmck(grid.sanz, grid.typanz, ptr(grid.nelanz), ptr(grid.selanz), grid.maxn,
+                          ptr(grid.nrel), grid.elanz, ptr(perm), C.byref(st))
+    if mb < 0:
+        raise RuntimeError("cutmck: disconnected graph")
+    return perm, mb, st.value
+
+
+# ----------------------------------------------------------- reference build
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own routines (amdflang build); raises if not built."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise FileNotFoundError(REF_PATH)
+        _ref = C.CDLL(REF_PATH)
+    return _ref
+
+
+class RefSession:
+    """Fills the reference modules and drives its routines (oracle/ref_harness.f90)."""
+
+    def __init__(self, grid, survey):
+        self.grid, self.survey = grid, survey
+        r = ref()
+        self._keep = [grid.typ, grid.nelanz, grid.selanz, grid.nrel, grid.x, grid.y, grid.rnr, grid.enr]
+        rnr = grid.rnr if grid.rnr.size else np.zeros(1, np.int32)
+        self._keep.append(rnr)
+        _chk(r.ref_set_grid(grid.sanz, grid.mb, grid.typanz, ptr(grid.typ), ptr(grid.nelanz),
+                            ptr(grid.selanz), grid.maxn, ptr(grid.nrel), ptr(grid.x), ptr(grid.y),
+                            ptr(rnr), grid.eanz, ptr(grid.enr)), "ref_set_grid")
+        _chk(r.ref_set_data(survey.nanz, ptr(survey.strnr), ptr(survey.vnr)), "ref_set_data")
+
+    def rwaven(self):
+        kwn = np.zeros(30); kwi = np.zeros(30); n = _i(); amin = _d(); amax = _d()
+        _chk(ref().ref_rwaven(C.byref(n), ptr(kwn), ptr(kwi), C.byref(amin), C.byref(amax)), "ref_rwaven")
+        self.K = n.value
+        return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
+
+    def precal(self, lsytop=False):
+        g = self.grid
+        elbg = np.zeros((self.K, g.npk, g.elanz))
+        relbg = np.zeros((g.npk, max(g.relanz, 1)))
+        kg = np.zeros((self.K, g.eanz, max(g.relanz, 1)))
+        lb = _i()
+        _chk(ref().ref_precal(int(lsytop), ptr(elbg), ptr(relbg), ptr(kg), C.byref(lb)), "ref_precal")
+        return elbg, relbg, kg, bool(lb.value)
+
+    def forward(self, sigma, nthreads=1):
+        g = self.grid
+        kpot = np.zeros((self.K, g.eanz, g.sanz), np.complex128)
+        s = np.ascontiguousarray(sigma, np.complex128)
+        _chk(ref().ref_forward(ptr(s), int(nthreads), ptr(kpot)), "ref_forward")
+        return kpot
+
+    def voltages(self, complex_inv=True):
+        g = self.grid
+        hpot = np.zeros((g.eanz, g.sanz), np.complex128)
+        volt = np.zeros(self.survey.nanz, np.complex128); sa = np.zeros_like(volt)
+        err = ref().ref_voltages(int(complex_inv), ptr(hpot), ptr(volt), ptr(sa))
+        _chk(err, "ref_voltages")
+        return hpot, volt, sa
+
+    def sens(self, inversion=True):
+        sens = np.zeros((self.grid.elanz, self.survey.nanz), np.complex128)
+        ref().ref_sens(int(inversion), ptr(sens))
+        return sens
+
+    def bsmatm(self, ltri, nx=0, nz=0, alfx=1.0, alfz=1.0):
+        g = self.grid
+        ncol = 3 if ltri == 0 else g.maxn + 1
+        sm = np.zeros((ncol, g.elanz))
+        nb = np.zeros((g.maxn + 1, g.elanz), np.int32)
+        ref().ref_bsmatm(ltri, nx, nz, _d(alfx), _d(alfz), ptr(sm), ptr(nb))
+        return sm, nb
+
+    def update(self, p: CGProblem):
+        a = p.arrs
+        cgfac = np.zeros(p.manz); bvec = np.zeros(p.manz, np.complex128)
+        dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1); ncg = _i()
+        _chk(ref().ref_update(p.s.ncgmax, _d(p.s.eps), _d(p.s.lam), ptr(a["sens"]), ptr(a["wmatd"]),
+                              ptr(a["wdfak"]), ptr(a["dat"]), ptr(a["sigmaa"]), ptr(a["par"]), ptr(cgfac),
+                              ptr(bvec), ptr(dpar), ptr(cgres), C.byref(ncg)), "ref_update")
+        return cgfac, bvec, dpar, cgres, ncg.value
+
+
+def ref_chol(sanz, mb, a):
+    return ref().ref_chol(sanz, mb, ptr(a))
+
+
+def ref_vre(sanz, mb, a, b):
+    pot = np.zeros(sanz, np.complex128)
+    ref().ref_vre(sanz, mb, ptr(a), ptr(b), ptr(pot))
+    return pot
