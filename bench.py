@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Throughput of the CRTomo forward + Jacobian path on one MI355X.
+
+One step = the complex forward solve for every (electrode, wavenumber) system
+(on-the-fly assembly, scaling, band Cholesky, substitution), the inverse
+Fourier back-transform with the voltages (bvolti) and the full Jacobian
+(bsensi), on the seeded C1 workload: ~20k-triangle Delaunay mesh over
+100 x 30 m with 2 m sinusoidal topography, 64 electrodes at 1.5 m, 2000
+dipole-dipole + Wenner quadrupoles, complex GRF model.  Grid, element matrices
+and conductivities are resident in HBM before the timed region.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def device_sync():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipDeviceSynchronize()
+
+
+def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
+    """Reference OpenMP Fortran build (oracle/_ref) on a bounded sample of the same
+    workload; falls back to the C restatement when the reference build is absent."""
+    import numpy as np
+    from oracle import oracle as O
+
+    g, s = w.grid, w.survey
+    if O.ref_available():
+        R = O.RefSession(g, s)
+        kwn, kwi, _, _ = R.rwaven()
+        R.precal()
+        r = O.ref()
+        r.ref_forward_sample.restype = ctypes.c_double
+        r.ref_sens_sample.restype = ctypes.c_double
+        K, E = kwn.size, g.eanz
+        nk = min(K, threads)
+        sig = np.ascontiguousarray(w.sigma, np.complex128)
+        t_f = r.ref_forward_sample(O.ptr(sig), threads, nk, fwd_electrodes)
+        # the reference spreads the forward over wavenumbers only (src/fem.f90:257-277)
+        waves = -(-K // min(K, threads))
+        fwd_full = waves * E * t_f / fwd_electrodes
+        t_s = r.ref_sens_sample(threads, sens_meas)
+        sens_full = t_s * s.nanz / sens_meas
+        kind = "reference"
+        sample = (f"reference OpenMP Fortran (amdflang -O2 -fopenmp): forward of {nk} wavenumbers x "
+                  f"{fwd_electrodes} electrodes ({t_f:.2f} s), bsensi of {sens_meas} of {s.nanz} "
+                  f"measurements ({t_s:.2f} s), scaled to the full step")
+    else:
+        kwn, kwi, _, _ = O.rwaven(g, s)
+        elbg, relbg, kg, lb = O.precal(g, kwn)
+        K, E = kwn.size, g.eanz
+        systems = [(l, 1) for l in range(1, fwd_electrodes + 1)]
+        t0 = time.perf_counter()
+        O.forward(g, w.sigma, elbg, relbg, kg, lb, systems=systems)
+        t_f = time.perf_counter() - t0
+        fwd_full = t_f * K * E / fwd_electrodes
+        fwd_full_threads = fwd_full
+        threads = 1
+        sens_full = float("nan")
+        kind = "port"
+        sample = f"C restatement, 1 thread: forward of {fwd_electrodes} systems scaled to {K * E}"
+        return dict(value=1.0 / fwd_full_threads, unit="solves/s", cores=1, kind=kind, sample=sample,
+                    forward_s=fwd_full, sens_s=None)
+    return dict(value=1.0 / (fwd_full + sens_full), unit="solves/s", cores=threads, kind=kind, sample=sample,
+                forward_s=fwd_full, sens_s=sens_full)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-sens-meas", type=int, default=96)
+    ap.add_argument("--cpu-fwd-electrodes", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in "
+              "DESIGN.md (shard the (electrode, wavenumber) systems, all-reduce the CG vectors)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from crtomo_stable_amd import configs, lib
+    if not os.path.exists(lib.LIB_PATH):
+        from crtomo_stable_amd import build
+        build.build()
+
+    w = configs.c1() if args.workload == "c1" else configs.c0()
+    g, s = w.grid, w.survey
+    ctx = lib.Context(0)
+    ctx.set_grid(g)
+    ctx.set_survey(s)
+    kwn, kwi, _, _ = ctx.rwaven()
+    lbeta = ctx.precal()
+    ctx.set_sigma(w.sigma)
+    K, E = kwn.size, g.eanz
+    nsys = K * E if lbeta else K
+
+    def step():
+        ctx.forward()
+        ctx.voltages(1)
+        ctx.sens(1)
+
+    for _ in range(args.warmup):
+        step()
+    device_sync()
+    acc = dict(diag=0.0, factor=0.0, solve=0.0, volt=0.0, sens=0.0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = ctx.times()
+        for key in acc:
+            acc[key] += t[key]
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / args.steps
+    phase = {k: v / args.steps for k, v in acc.items()}
+
+    # band Cholesky: 8 sanz mb(mb+1)/2 real FLOP per (electrode, wavenumber) system
+    flop_sys = 8.0 * g.sanz * g.mb * (g.mb + 1) / 2.0
+    fac_ms = phase["factor"]
+    achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
+                    kernel="crt_factor_kernel", flop_per_system=flop_sys, systems_per_launch=nsys,
+                    launch_ms=round(fac_ms, 4))
+
+    out = {
+        "metric": "forward+Jacobian solves/s",
+        "value": round(1e3 / ms_step, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": ("c1: seeded Delaunay mesh 100x30 m, 2 m sinusoidal topography, complex GRF model, "
+                         "complex forward + bsensi Jacobian" if args.workload == "c1" else
+                         "c0: 60x20 quads, 30 electrodes, 188 dipole-dipole, forward + bsensi"),
+            "elements": g.elanz, "nodes": g.sanz, "mb": g.mb, "electrodes": E, "measurements": s.nanz,
+            "wavenumbers": K, "systems_per_step": nsys, "parallelism": "1 GPU",
+        },
+        "forward_systems_per_s": round(nsys * 1e3 / ms_step, 1),
+        "phase_ms": {k: round(v, 4) for k, v in phase.items()},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            cb = cpu_baseline(w, args.cpu_threads, args.cpu_sens_meas, args.cpu_fwd_electrodes)
+            out["cpu_baseline"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cb.items()}
+        except Exception as e:  # the baseline is reported, never the target
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,999 @@
+// crt_api.hip -- host side of the CRTomo HIP library: context, device-resident
+// state, assembly map, wavenumber bookkeeping, CutMcK and the C-ABI.
+#include "crt_common.hpp"
+#include "../../include/crtomo_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+// kernel launchers (crt_precal.hip, crt_band.hip, crt_volt_sens.hip)
+int crt_launch_precal(hipStream_t, int, int, int, int, int, int, const int *, const int *, const int *,
+                      const double *, const double *, const int *, const double *, double, double *, double *,
+                      double *, int *);
+int crt_launch_diag(hipStream_t, const DevGrid &, int, const int2 *, double *, double *, int *);
+int crt_launch_factor(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *, zd *,
+                      int *);
+size_t crt_factor_lds_bytes(int D);
+int crt_launch_solve(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
+                     const zd *, const zd *, zd *, zd *);
+int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
+int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
+int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
+int crt_launch_sens(hipStream_t, int, int, int, int, int, int, int, const int *, const int *, const int4 *,
+                    const double *, const zd *, const double *, const zd *, const zd *, const double *, zd *);
+int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
+
+// CG (crt_cg.hip)
+struct CGScalars {
+    double dr, dr0, dr1, alpha, beta;
+    int k, done, ncg, pad;
+};
+struct CGArgs {
+    int nanz, manz, ltri, nx, maxn, ncgmax;
+    double lam, eps;
+    const zd *J;
+    const double *w;
+    const double *smatm;
+    const int *nachbar;
+    const double *cgfac;
+    zd *r, *p, *q, *bv, *dpar;
+    zd *ap;
+    zd *part_ap;
+    double *part_dr1;
+    double *part_rr;
+    int n_dr1, n_rr, ncb, cb;
+    const CGScalars *sc;
+    CGScalars *scn;
+    double *cgres;
+};
+int crt_cg_geometry(int, int, int *, int *, int *, int *);
+int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const zd *, const zd *,
+                      const double *, double *, zd *);
+int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
+int crt_launch_cg_init(hipStream_t, const CGArgs &, const zd *);
+int crt_launch_cg_iter(hipStream_t, const CGArgs &);
+int crt_launch_cg_final(hipStream_t, int, double, const double *, zd *, zd *);
+
+namespace {
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int ensure(size_t count) {
+        if (count <= n && p) return 0;
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) { p = nullptr; return CRT_ERR_ALLOC; }
+        n = count;
+        return 0;
+    }
+    int upload(const T *h, size_t count, hipStream_t s) {
+        int e = ensure(count);
+        if (e) return e;
+        if (count && hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return CRT_ERR_HIP;
+        return 0;
+    }
+};
+
+enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_N };
+
+}  // namespace
+
+struct crt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0[T_N], ev1[T_N];
+    double times[T_N] = {0};
+    // grid
+    bool have_grid = false, have_survey = false, have_kwn = false, have_precal = false, have_sigma = false,
+         have_kpot = false, have_volt = false, have_sens = false;
+    int sanz = 0, mb = 0, typanz = 0, maxn = 0, elanz = 0, relanz = 0, eanz = 0, npk = 0, nblk = 0, D = 0;
+    std::vector<int> typ, nelanz, selanz, nrel, rnr, enr0, etyp, btyp, nkel;
+    std::vector<double> x, y;
+    int nanz = 0;
+    std::vector<int> strnr, vnr;
+    int kwnanz = 0;
+    std::vector<double> kwn, kwnwi;
+    int lbeta = 0;
+    // device state
+    DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_cptr, d_contrib, d_nkel, d_status;
+    DBuf<int4> d_meas, d_rhs;
+    DBuf<int2> d_sys;
+    DBuf<double> d_x, d_y, d_kwn, d_kwnwi, d_elbg, d_relbg, d_kg, d_fak, d_dga, d_kfak, d_strom;
+    DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp;
+    // normal-equation state
+    int ltri = -1, rnx = 0, rncol = 0, sens_nanz = 0, sens_manz = 0;
+    DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
+    DBuf<int> d_nachbar;
+    DBuf<zd> d_res, d_rm, d_bvec, d_r, d_p, d_q, d_bv, d_dpar, d_ap, d_part_ap, d_par;
+    DBuf<CGScalars> d_sc;
+    int cg_check_every = 16;
+    size_t panel_budget = (size_t)96 << 30;  // bytes of factor panels kept in flight
+
+    DevGrid devgrid() const {
+        DevGrid g;
+        g.sanz = sanz; g.mb = mb; g.elanz = elanz; g.relanz = relanz; g.eanz = eanz; g.maxn = maxn;
+        g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D;
+        g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
+        g.rnr = d_rnr.p; g.enr = d_enr.p; g.elbg = d_elbg.p; g.relbg = d_relbg.p; g.kg = d_kg.p;
+        g.sigma = d_sigma.p;
+        return g;
+    }
+    void tic(int i) { (void)hipEventRecord(ev0[i], stream); }
+    void toc(int i) { (void)hipEventRecord(ev1[i], stream); }
+    void collect(int i) {
+        float ms = 0.f;
+        if (hipEventSynchronize(ev1[i]) == hipSuccess && hipEventElapsedTime(&ms, ev0[i], ev1[i]) == hipSuccess)
+            times[i] = ms;
+    }
+    int status_check() {
+        int st = 0;
+        if (hipMemcpy(&st, d_status.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CRT_ERR_HIP;
+        return st;
+    }
+    int status_reset() {
+        int e = d_status.ensure(1);
+        if (e) return e;
+        return hipMemsetAsync(d_status.p, 0, sizeof(int), stream) == hipSuccess ? 0 : CRT_ERR_HIP;
+    }
+};
+
+// ------------------------------------------------------------------ helpers
+namespace {
+
+// host restatement of the wavenumber set (src/rwaven.f90:52-151)
+void gauleg_h(double x1, double x2, double *xo, double *w, int n) {
+    int m = (n + 1) / 2;
+    double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+    for (int i = 1; i <= m; ++i) {
+        double z = std::cos(3.141592653589793 * (i - 0.25) / (n + 0.5)), z1, pp;
+        do {
+            double p1 = 1.0, p2 = 0.0, p3;
+            for (int j = 1; j <= n; ++j) {
+                p3 = p2; p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (std::fabs(z - z1) > 3e-14);
+        xo[i - 1] = xm - xl * z;
+        xo[n - i] = xm + xl * z;
+        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+        w[n - i] = w[i - 1];
+    }
+}
+double gammln_h(double xx) {
+    static const double cof[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                  -1.231739572450155, .1208650973866179e-2, -.5395239384953e-5};
+    double x = xx, yv = x, tmp = x + 5.5, ser = 1.000000000190015;
+    tmp = (x + 0.5) * std::log(tmp) - tmp;
+    for (int j = 0; j < 6; ++j) { yv += 1.0; ser += cof[j] / yv; }
+    return tmp + std::log(2.5066282746310005 * ser / x);
+}
+void gaulag_h(double *xo, double *w, int n, double alf) {
+    double z = 0, z1, pp = 0, p1, p2 = 0, p3;
+    for (int i = 1; i <= n; ++i) {
+        if (i == 1) z = (1.0 + alf) * (3.0 + 0.92 * alf) / (1.0 + 2.4 * n + 1.8 * alf);
+        else if (i == 2) z += (15.0 + 6.25 * alf) / (1.0 + 0.9 * alf + 2.5 * n);
+        else {
+            double ai = i - 2;
+            z += ((1.0 + 2.55 * ai) / (1.9 * ai) + 1.26 * ai * alf / (1.0 + 3.5 * ai)) * (z - xo[i - 3]) /
+                 (1.0 + 0.3 * alf);
+        }
+        for (int its = 1; its <= 10; ++its) {
+            p1 = 1.0; p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                p3 = p2; p2 = p1;
+                p1 = (((double)(2 * j - 1) + alf - z) * p2 - ((double)(j - 1) + alf) * p3) / (double)j;
+            }
+            pp = (n * p1 - (n + alf) * p2) / z;
+            z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= 3e-14) break;
+        }
+        xo[i - 1] = z;
+        w[i - 1] = -std::exp(gammln_h(alf + n) - gammln_h((double)n)) / (pp * n * p2);
+    }
+}
+
+// mean height of the no-flow boundary (src/bsytop.f90:23-66)
+double bsytop_h(const crt_ctx *c) {
+    double sytop = 0.0;
+    int iel = 0;
+    for (int t = 0; t < c->typanz; ++t) {
+        if (c->typ[t] == 12) {
+            int nk = c->selanz[t];
+            sytop = 0.0;
+            for (int j = 1; j <= c->nelanz[t]; ++j, ++iel) {
+                double sp = 0.0;
+                for (int ik = 0; ik < nk; ++ik) sp += c->y[c->nrel[(size_t)iel * c->maxn + ik] - 1];
+                sp = sp / (double)nk;
+                sytop = (j == 1) ? sp : (sytop * (double)(j - 1) + sp) / (double)j;
+            }
+        } else {
+            iel += c->nelanz[t];
+        }
+    }
+    return sytop;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+extern "C" {
+
+const char *crt_version(void) { return "crtomo_hip 0.1 (gfx950, FP64 band path)"; }
+
+int crt_create(crt_ctx **out, int device) {
+    if (!out) return CRT_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return CRT_ERR_HIP;
+    CRT_HIP(hipSetDevice(device));
+    crt_ctx *c = new crt_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CRT_ERR_HIP; }
+    for (int i = 0; i < T_N; ++i) {
+        (void)hipEventCreate(&c->ev0[i]);
+        (void)hipEventCreate(&c->ev1[i]);
+    }
+    if (const char *b = getenv("CRT_PANEL_BUDGET_GB")) c->panel_budget = (size_t)(atof(b) * (double)(1ull << 30));
+    if (c->d_status.ensure(1)) { crt_destroy(c); return CRT_ERR_ALLOC; }
+    *out = c;
+    return 0;
+}
+
+void crt_destroy(crt_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < T_N; ++i) {
+        (void)hipEventDestroy(c->ev0[i]);
+        (void)hipEventDestroy(c->ev1[i]);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const int *nelanz, const int *selanz,
+                 int maxn, const int *nrel, const double *x, const double *y, const int *rnr, int eanz,
+                 const int *enr) {
+    if (!c || sanz <= 0 || mb < 0 || typanz <= 0 || maxn <= 0 || maxn > 4) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    c->have_grid = c->have_precal = c->have_kpot = c->have_volt = c->have_sens = false;
+    c->sanz = sanz; c->mb = mb; c->typanz = typanz; c->maxn = maxn; c->eanz = eanz;
+    c->npk = maxn * (maxn + 1) / 2;
+    c->typ.assign(typ, typ + typanz);
+    c->nelanz.assign(nelanz, nelanz + typanz);
+    c->selanz.assign(selanz, selanz + typanz);
+    c->elanz = c->relanz = 0;
+    bool boundary_seen = false;
+    for (int t = 0; t < typanz; ++t) {
+        if (typ[t] == 13) return CRT_ERR_UNSUPPORTED;  // Dirichlet boundaries are not on this path
+        if (typ[t] > 10) { c->relanz += nelanz[t]; boundary_seen = true; }
+        else {
+            if (boundary_seen) return CRT_ERR_ARG;  // area groups precede boundary groups
+            if (typ[t] != 3 && typ[t] != 5 && typ[t] != 8) return CRT_ERR_TYPE;
+            c->elanz += nelanz[t];
+        }
+    }
+    const int neltot = c->elanz + c->relanz;
+    c->nrel.assign(nrel, nrel + (size_t)neltot * maxn);
+    c->x.assign(x, x + sanz);
+    c->y.assign(y, y + sanz);
+    c->rnr.assign(rnr, rnr + c->relanz);
+    for (int i = 0; i < c->relanz; ++i)
+        if (rnr[i] < 1 || rnr[i] > c->elanz) return CRT_ERR_ARG;
+    c->enr0.resize(eanz);
+    for (int i = 0; i < eanz; ++i) {
+        if (enr[i] < 1 || enr[i] > sanz) return CRT_ERR_ARG;
+        c->enr0[i] = enr[i] - 1;
+    }
+    c->etyp.resize(c->elanz);
+    c->nkel.resize(c->elanz);
+    c->btyp.resize(std::max(c->relanz, 1));
+    {
+        int iel = 0;
+        for (int t = 0; t < typanz; ++t)
+            for (int j = 0; j < nelanz[t]; ++j, ++iel) {
+                if (typ[t] < 10) { c->etyp[iel] = typ[t]; c->nkel[iel] = selanz[t]; }
+                else c->btyp[iel - c->elanz] = typ[t];
+            }
+    }
+    // assembly map: contributions to each lower-triangle entry in the element order
+    // of src/kompab.f90:80-138 (area elements, then type-11 boundary elements)
+    struct Ent { int i, j, code; };
+    std::vector<Ent> ents;
+    ents.reserve((size_t)neltot * 10 + sanz);
+    {
+        int iel = 0;
+        for (int t = 0; t < typanz; ++t) {
+            const int nk = selanz[t];
+            for (int j = 0; j < nelanz[t]; ++j, ++iel) {
+                if (typ[t] > 11) continue;
+                int ikl = 0;
+                for (int kk = 0; kk < nk; ++kk) {
+                    int nzp = nrel[(size_t)iel * maxn + kk];
+                    for (int ll = 0; ll <= kk; ++ll, ++ikl) {
+                        int nnp = nrel[(size_t)iel * maxn + ll];
+                        if (nzp < 1 || nnp < 1 || nzp > sanz || nnp > sanz) return CRT_ERR_ARG;
+                        if (std::abs(nzp - nnp) > mb) return CRT_ERR_BAND;
+                        int i = std::max(nzp, nnp) - 1, jj = std::min(nzp, nnp) - 1;
+                        int code = (typ[t] == 11) ? -((iel - c->elanz) * c->npk + ikl) - 1 : iel * c->npk + ikl;
+                        ents.push_back({i, jj, code});
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < sanz; ++i) ents.push_back({i, i, INT32_MIN});  // guarantees a diagonal entry
+    }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        return a.i != b.i ? a.i < b.i : a.j < b.j;
+    });
+    std::vector<int> row_ptr(sanz + 1, 0), col, cptr, contrib;
+    col.reserve(ents.size()); cptr.reserve(ents.size() + 1); contrib.reserve(ents.size());
+    for (size_t q = 0; q < ents.size();) {
+        size_t q1 = q;
+        while (q1 < ents.size() && ents[q1].i == ents[q].i && ents[q1].j == ents[q].j) ++q1;
+        col.push_back(ents[q].j);
+        cptr.push_back((int)contrib.size());
+        for (size_t r = q; r < q1; ++r)
+            if (ents[r].code != INT32_MIN) contrib.push_back(ents[r].code);
+        row_ptr[ents[q].i + 1]++;
+        q = q1;
+    }
+    cptr.push_back((int)contrib.size());
+    for (int i = 0; i < sanz; ++i) row_ptr[i + 1] += row_ptr[i];
+    if (contrib.empty()) contrib.push_back(0);
+
+    c->nblk = (sanz + CRT_NB - 1) / CRT_NB;
+    c->D = std::max(1, (mb + CRT_NB - 1) / CRT_NB);
+    hipStream_t s = c->stream;
+    std::vector<int> rnr0(std::max(c->relanz, 1), 0);
+    for (int i = 0; i < c->relanz; ++i) rnr0[i] = rnr[i] - 1;
+    int e = 0;
+    e = e ? e : c->d_nrel.upload(c->nrel.data(), c->nrel.size(), s);
+    e = e ? e : c->d_etyp.upload(c->etyp.data(), std::max((size_t)1, c->etyp.size()), s);
+    e = e ? e : c->d_nkel.upload(c->nkel.data(), std::max((size_t)1, c->nkel.size()), s);
+    e = e ? e : c->d_btyp.upload(c->btyp.data(), c->btyp.size(), s);
+    e = e ? e : c->d_rnr.upload(rnr0.data(), rnr0.size(), s);
+    e = e ? e : c->d_enr.upload(c->enr0.data(), std::max((size_t)1, c->enr0.size()), s);
+    e = e ? e : c->d_x.upload(c->x.data(), sanz, s);
+    e = e ? e : c->d_y.upload(c->y.data(), sanz, s);
+    e = e ? e : c->d_row_ptr.upload(row_ptr.data(), row_ptr.size(), s);
+    e = e ? e : c->d_col.upload(col.data(), col.size(), s);
+    e = e ? e : c->d_cptr.upload(cptr.data(), cptr.size(), s);
+    e = e ? e : c->d_contrib.upload(contrib.data(), contrib.size(), s);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(s));
+    c->have_grid = true;
+    return 0;
+}
+
+int crt_set_survey(crt_ctx *c, int nanz, const int *strnr, const int *vnr) {
+    if (!c || nanz <= 0) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    c->nanz = nanz;
+    c->strnr.assign(strnr, strnr + nanz);
+    c->vnr.assign(vnr, vnr + nanz);
+    std::vector<int4> meas(nanz);
+    for (int i = 0; i < nanz; ++i) {
+        int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+        int e3 = vnr[i] % 10000, e4 = (vnr[i] - e3) / 10000;
+        int ee[4] = {e1, e2, e3, e4};
+        for (int q = 0; q < 4; ++q)
+            if (ee[q] < 0 || ee[q] > c->eanz) return CRT_ERR_ARG;
+        meas[i] = int4{e1, e2, e3, e4};
+    }
+    int e = c->d_meas.upload(meas.data(), nanz, c->stream);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->have_survey = true;
+    c->have_volt = c->have_sens = false;
+    return 0;
+}
+
+int crt_rwaven(crt_ctx *c, int *kwnanz, double *kwn, double *kwnwi, double *amin_o, double *amax_o) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_survey) return CRT_ERR_STATE;
+    double amin = 1e9, amax = 0.0;
+    for (int i = 0; i < c->nanz; ++i) {
+        int e[4];
+        double xe[4] = {0, 0, 0, 0}, ye[4] = {0, 0, 0, 0};
+        e[0] = c->strnr[i] % 10000; e[1] = (c->strnr[i] - e[0]) / 10000;
+        e[2] = c->vnr[i] % 10000;   e[3] = (c->vnr[i] - e[2]) / 10000;
+        for (int j = 0; j < 4; ++j)
+            if (e[j] > 0) { int nd = c->enr0[e[j] - 1]; xe[j] = c->x[nd]; ye[j] = c->y[nd]; }
+        for (int m = 2; m < 4; ++m) {
+            if (e[m] <= 0) continue;
+            for (int j = 0; j < 2; ++j) {
+                if (e[j] <= 0) continue;
+                double x21 = xe[m] - xe[j], y21 = ye[m] - ye[j];
+                double d = std::sqrt(x21 * x21 + y21 * y21);
+                amin = std::fmin(d, amin);
+                amax = std::fmax(d, amax);
+            }
+        }
+    }
+    if (amin_o) *amin_o = amin;
+    if (amax_o) *amax_o = amax;
+    if (amin < 1e-12 || amax < 1e-12) return CRT_ERR_ELEC;
+    const int lanz = 4;
+    int ganz = (int)(float)(6.0 * std::log10(amax / amin));  // INT(REAL(...)) of src/rwaven.f90:105
+    ganz = std::min(std::max(ganz, 2), 30 - lanz);
+    int n = ganz + lanz;
+    std::vector<double> k(n), w(n);
+    double k0 = 1.0 / (2.0 * amin);
+    gauleg_h(0.0, 1.0, k.data(), w.data(), ganz);
+    for (int i = 0; i < ganz; ++i) {
+        w[i] = 2.0 * k0 * w[i] * k[i];
+        k[i] = k0 * k[i] * k[i];
+    }
+    gaulag_h(k.data() + ganz, w.data() + ganz, lanz, 0.0);
+    for (int i = ganz; i < n; ++i) {
+        w[i] = k0 * w[i] * std::exp(k[i]);
+        k[i] = k0 * (k[i] + 1.0);
+    }
+    if (kwnanz) *kwnanz = n;
+    if (kwn) std::copy(k.begin(), k.end(), kwn);
+    if (kwnwi) std::copy(w.begin(), w.end(), kwnwi);
+    return crt_set_wavenumbers(c, n, k.data(), w.data());
+}
+
+int crt_set_wavenumbers(crt_ctx *c, int kwnanz, const double *kwn, const double *kwnwi) {
+    if (!c || kwnanz <= 0 || kwnanz > 30) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    c->kwnanz = kwnanz;
+    c->kwn.assign(kwn, kwn + kwnanz);
+    c->kwnwi.assign(kwnwi, kwnwi + kwnanz);
+    int e = c->d_kwn.upload(c->kwn.data(), kwnanz, c->stream);
+    e = e ? e : c->d_kwnwi.upload(c->kwnwi.data(), kwnanz, c->stream);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->have_kwn = true;
+    c->have_precal = c->have_kpot = false;
+    return 0;
+}
+
+int crt_precal(crt_ctx *c, int lsytop, int *lbeta) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_grid || !c->have_kwn) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int K = c->kwnanz;
+    int e = c->d_elbg.ensure((size_t)c->elanz * c->npk * K);
+    e = e ? e : c->d_relbg.ensure((size_t)std::max(c->relanz, 1) * c->npk);
+    e = e ? e : c->d_kg.ensure((size_t)std::max(c->relanz, 1) * std::max(c->eanz, 1) * K);
+    if (e) return e;
+    CRT_HIP(hipMemsetAsync(c->d_elbg.p, 0, c->d_elbg.n * sizeof(double), c->stream));
+    CRT_HIP(hipMemsetAsync(c->d_relbg.p, 0, c->d_relbg.n * sizeof(double), c->stream));
+    CRT_HIP(hipMemsetAsync(c->d_kg.p, 0, c->d_kg.n * sizeof(double), c->stream));
+    if ((e = c->status_reset())) return e;
+    double sytop = lsytop ? bsytop_h(c) : 0.0;
+    c->tic(T_PRECAL);
+    e = crt_launch_precal(c->stream, c->elanz, c->relanz, c->maxn, c->npk, c->eanz, K, c->d_etyp.p, c->d_btyp.p,
+                          c->d_nrel.p, c->d_x.p, c->d_y.p, c->d_enr.p, c->d_kwn.p, sytop, c->d_elbg.p, c->d_relbg.p,
+                          c->d_kg.p, c->d_status.p);
+    c->toc(T_PRECAL);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_PRECAL);
+    int st = c->status_check();
+    if (st) return st;
+    c->lbeta = 0;
+    for (int t = 0; t < c->typanz; ++t)
+        if (c->typ[t] == 11) c->lbeta = 1;
+    if (lbeta) *lbeta = c->lbeta;
+    c->have_precal = true;
+    return 0;
+}
+
+int crt_get_precal(crt_ctx *c, double *elbg, double *relbg, double *kg) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_precal) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int K = c->kwnanz;
+    if (elbg) CRT_HIP(hipMemcpy(elbg, c->d_elbg.p, (size_t)c->elanz * c->npk * K * sizeof(double), hipMemcpyDeviceToHost));
+    if (relbg && c->relanz)
+        CRT_HIP(hipMemcpy(relbg, c->d_relbg.p, (size_t)c->relanz * c->npk * sizeof(double), hipMemcpyDeviceToHost));
+    if (kg && c->relanz)
+        CRT_HIP(hipMemcpy(kg, c->d_kg.p, (size_t)c->relanz * c->eanz * K * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int crt_set_sigma(crt_ctx *c, const double *sigma) {
+    if (!c || !sigma) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    int e = c->d_sigma.upload(reinterpret_cast<const zd *>(sigma), c->elanz, c->stream);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->have_sigma = true;
+    c->have_kpot = c->have_volt = c->have_sens = false;
+    return 0;
+}
+
+int crt_forward(crt_ctx *c) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    if (crt_factor_lds_bytes(c->D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    const int K = c->kwnanz, E = c->eanz, nblk = c->nblk, D = c->D;
+    // systems to factorise: every (l,k) with mixed boundaries, else l = 1 per k
+    // (src/fem.f90:355: lbeta.or.l.eq.1)
+    std::vector<int2> sys;
+    std::vector<int4> rhs;
+    for (int k = 0; k < K; ++k) {
+        if (c->lbeta) {
+            for (int l = 0; l < E; ++l) {
+                rhs.push_back(int4{l, k, (int)sys.size(), 1});
+                sys.push_back(int2{l, k});
+            }
+        } else {
+            int fs = (int)sys.size();
+            sys.push_back(int2{0, k});
+            for (int l = 0; l < E; ++l) rhs.push_back(int4{l, k, fs, l == 0 ? 1 : 0});
+        }
+    }
+    const size_t panel_elems = (size_t)nblk * (D + 1) * CRT_NB * CRT_NB;
+    size_t per_sys = panel_elems * sizeof(zd) + (size_t)nblk * CRT_NB * (sizeof(zd) + 2 * sizeof(double));
+    size_t batch = std::max<size_t>(1, std::min(sys.size(), c->panel_budget / per_sys));
+    int e = c->d_panels.ensure(batch * panel_elems);
+    e = e ? e : c->d_yf.ensure(batch * nblk * CRT_NB);
+    e = e ? e : c->d_fak.ensure(batch * nblk * CRT_NB);
+    e = e ? e : c->d_dga.ensure(batch * nblk * CRT_NB);
+    e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
+    e = e ? e : c->d_sys.ensure(sys.size());
+    e = e ? e : c->d_rhs.ensure(rhs.size());
+    e = e ? e : c->d_status.ensure(sys.size() + 1);
+    if (e) return e;
+    size_t nyscr = c->lbeta ? 1 : rhs.size();
+    if ((e = c->d_yscr.ensure(nyscr * nblk * CRT_NB))) return e;
+    CRT_HIP(hipMemcpyAsync(c->d_sys.p, sys.data(), sys.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipMemsetAsync(c->d_status.p, 0, (sys.size() + 1) * sizeof(int), c->stream));
+    DevGrid g = c->devgrid();
+    double tdiag = 0, tfac = 0, tsol = 0;
+    for (size_t s0 = 0; s0 < sys.size(); s0 += batch) {
+        int nb = (int)std::min(batch, sys.size() - s0);
+        // right-hand sides of this batch, factor index relative to the batch
+        std::vector<int4> r;
+        for (auto q : rhs)
+            if ((size_t)q.z >= s0 && (size_t)q.z < s0 + nb) r.push_back(int4{q.x, q.y, q.z - (int)s0, q.w});
+        CRT_HIP(hipMemcpyAsync(c->d_rhs.p, r.data(), r.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        c->tic(T_DIAG);
+        if ((e = crt_launch_diag(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_status.p + 1 + s0)))
+            return e;
+        c->toc(T_DIAG);
+        c->tic(T_FACTOR);
+        if ((e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_panels.p,
+                                   c->d_yf.p, c->d_status.p + 1 + s0)))
+            return e;
+        c->toc(T_FACTOR);
+        c->tic(T_SOLVE);
+        if ((e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
+                                  c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p)))
+            return e;
+        c->toc(T_SOLVE);
+        CRT_HIP(hipStreamSynchronize(c->stream));
+        c->collect(T_DIAG); tdiag += c->times[T_DIAG];
+        c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
+        c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
+    }
+    c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol;
+    std::vector<int> st(sys.size() + 1);
+    CRT_HIP(hipMemcpy(st.data(), c->d_status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int worst = 0;
+    for (int v : st) worst = std::max(worst, v);
+    if (worst && getenv("CRT_DEBUG")) {
+        for (size_t q = 1; q < st.size(); ++q)
+            if (st[q]) {
+                fprintf(stderr, "crt_forward: system %zu (l=%d k=%d) status %d\n", q - 1, sys[q - 1].x, sys[q - 1].y,
+                        st[q]);
+                if (q - 1 < batch) {
+                    std::vector<double> dg(c->sanz);
+                    (void)hipMemcpy(dg.data(), c->d_dga.p + (q - 1) * nblk * CRT_NB, c->sanz * sizeof(double),
+                                    hipMemcpyDeviceToHost);
+                    for (int i = 0; i < c->sanz; ++i)
+                        if (!(dg[i] > 0.0) || dg[i] == 1.0) {
+                            fprintf(stderr, "  row %d |a_ii| = %g\n", i, dg[i]);
+                            break;
+                        }
+                }
+                break;
+            }
+    }
+    if (worst) return worst;
+    c->have_kpot = true;
+    c->have_volt = c->have_sens = false;
+    return 0;
+}
+
+int crt_get_kpot(crt_ctx *c, double *kpot) {
+    if (!c || !kpot) return CRT_ERR_ARG;
+    if (!c->have_kpot) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    size_t n = (size_t)c->sanz * c->eanz * c->kwnanz;
+    int e = c->d_tmp.ensure(n);
+    if (e) return e;
+    if ((e = crt_launch_kpot_ref(c->stream, c->sanz, c->eanz, c->kwnanz, c->d_kpotN.p, c->d_tmp.p))) return e;
+    CRT_HIP(hipMemcpyAsync(kpot, c->d_tmp.p, n * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int crt_voltages(crt_ctx *c, int mode, const double *kfak, double *volt, double *sigmaa) {
+    if (!c || (mode != 0 && mode != 1)) return CRT_ERR_ARG;
+    if (!c->have_kpot || !c->have_survey) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    int e = c->d_hpotE.ensure((size_t)c->eanz * c->eanz);
+    e = e ? e : c->d_volt.ensure(c->nanz);
+    e = e ? e : c->d_sigmaa.ensure(c->nanz);
+    if (e) return e;
+    const double *dk = nullptr;
+    if (kfak && mode == 0) {
+        if ((e = c->d_kfak.upload(kfak, c->nanz, c->stream))) return e;
+        dk = c->d_kfak.p;
+    }
+    if ((e = c->status_reset())) return e;
+    c->tic(T_VOLT);
+    e = crt_launch_hpot_elec(c->stream, c->eanz, c->kwnanz, c->d_enr.p, c->d_kwnwi.p, c->d_kpotN.p, c->d_hpotE.p);
+    e = e ? e : crt_launch_volt(c->stream, c->nanz, c->eanz, mode, c->d_meas.p, c->d_hpotE.p, dk, c->d_volt.p,
+                                c->d_sigmaa.p, c->d_status.p);
+    c->toc(T_VOLT);
+    if (e) return e;
+    if (volt) CRT_HIP(hipMemcpyAsync(volt, c->d_volt.p, c->nanz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    if (sigmaa) CRT_HIP(hipMemcpyAsync(sigmaa, c->d_sigmaa.p, c->nanz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_VOLT);
+    int st = c->status_check();
+    c->have_volt = true;
+    return st;
+}
+
+int crt_get_hpot(crt_ctx *c, double *hpot) {
+    if (!c || !hpot) return CRT_ERR_ARG;
+    if (!c->have_kpot) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    size_t n = (size_t)c->sanz * c->eanz;
+    int e = c->d_tmp.ensure(n);
+    if (e) return e;
+    if ((e = crt_launch_hpot_full(c->stream, c->sanz, c->eanz, c->kwnanz, c->d_kwnwi.p, c->d_kpotN.p, c->d_tmp.p)))
+        return e;
+    CRT_HIP(hipMemcpyAsync(hpot, c->d_tmp.p, n * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int crt_sens(crt_ctx *c, int mode, const double *strom) {
+    if (!c || (mode != 0 && mode != 1)) return CRT_ERR_ARG;
+    if (!c->have_kpot || !c->have_survey) return CRT_ERR_STATE;
+    if (mode == 1 && !c->have_volt) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    int e = c->d_sens.ensure((size_t)c->nanz * c->elanz);
+    if (e) return e;
+    const double *ds = nullptr;
+    if (strom && mode == 0) {
+        if ((e = c->d_strom.upload(strom, c->nanz, c->stream))) return e;
+        ds = c->d_strom.p;
+    }
+    c->tic(T_SENS);
+    e = crt_launch_sens(c->stream, c->elanz, c->nanz, c->eanz, c->kwnanz, c->maxn, c->npk, mode, c->d_nrel.p,
+                        c->d_nkel.p, c->d_meas.p, c->d_kwnwi.p, c->d_kpotN.p, c->d_elbg.p, c->d_sigma.p,
+                        c->d_volt.p ? c->d_volt.p : nullptr, ds, c->d_sens.p);
+    c->toc(T_SENS);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_SENS);
+    c->have_sens = true;
+    c->sens_nanz = c->nanz;
+    c->sens_manz = c->elanz;
+    return 0;
+}
+
+int crt_get_sens(crt_ctx *c, double *sens) {
+    if (!c || !sens) return CRT_ERR_ARG;
+    if (!c->have_sens) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    CRT_HIP(hipMemcpy(sens, c->d_sens.p, (size_t)c->nanz * c->elanz * sizeof(zd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ------------------------------------------------------------- normal equations
+int crt_set_sens(crt_ctx *c, int nanz, int manz, const double *sens) {
+    if (!c || !sens || nanz <= 0 || manz <= 0) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    int e = c->d_sens.upload(reinterpret_cast<const zd *>(sens), (size_t)nanz * manz, c->stream);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->sens_nanz = nanz;
+    c->sens_manz = manz;
+    c->have_sens = true;
+    return 0;
+}
+
+int crt_set_regularization(crt_ctx *c, int ltri, int nx, int manz, int maxn, const double *smatm,
+                           const int *nachbar) {
+    if (!c || !smatm || manz <= 0) return CRT_ERR_ARG;
+    if (ltri != 0 && ltri != 1 && ltri != 2) return CRT_ERR_UNSUPPORTED;
+    if ((ltri == 1 || ltri == 2) && (!nachbar || maxn <= 0)) return CRT_ERR_ARG;
+    if (ltri == 0 && nx <= 0) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    int ncol = ltri == 0 ? 3 : maxn + 1;
+    std::vector<double> rd(manz);
+    for (int j = 0; j < manz; ++j) rd[j] = smatm[(size_t)(ltri == 0 ? 0 : maxn) * manz + j];
+    int e = c->d_smatm.upload(smatm, (size_t)manz * ncol, c->stream);
+    e = e ? e : c->d_rdiag.upload(rd.data(), manz, c->stream);
+    if (!e && ltri != 0) e = c->d_nachbar.upload(nachbar, (size_t)manz * (maxn + 1), c->stream);
+    if (!e && ltri == 0) e = c->d_nachbar.ensure(1);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->ltri = ltri;
+    c->rnx = nx;
+    c->rncol = ncol;
+    c->sens_manz = manz;
+    return 0;
+}
+
+static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, double eps, CGArgs &a) {
+    int ncb, cb, n_dr1, n_rr;
+    crt_cg_geometry(nanz, manz, &ncb, &cb, &n_dr1, &n_rr);
+    int e = 0;
+    e = e ? e : c->d_r.ensure(manz);
+    e = e ? e : c->d_p.ensure(manz);
+    e = e ? e : c->d_q.ensure(manz);
+    e = e ? e : c->d_bv.ensure(manz);
+    e = e ? e : c->d_dpar.ensure(manz);
+    e = e ? e : c->d_ap.ensure(nanz);
+    e = e ? e : c->d_part_ap.ensure((size_t)ncb * nanz);
+    e = e ? e : c->d_part_dr1.ensure(n_dr1);
+    e = e ? e : c->d_part_rr.ensure(n_rr);
+    e = e ? e : c->d_cgres.ensure((size_t)ncgmax + 1);
+    e = e ? e : c->d_sc.ensure(2);
+    if (e) return e;
+    CRT_HIP(hipMemsetAsync(c->d_cgres.p, 0, ((size_t)ncgmax + 1) * sizeof(double), c->stream));
+    a = CGArgs{};
+    a.nanz = nanz; a.manz = manz; a.ltri = c->ltri; a.nx = c->rnx; a.maxn = c->rncol - 1; a.ncgmax = ncgmax;
+    a.lam = lam; a.eps = eps;
+    a.J = c->d_sens.p; a.w = c->d_w.p; a.smatm = c->d_smatm.p; a.nachbar = c->d_nachbar.p;
+    a.cgfac = c->d_cgfac.p;
+    a.r = c->d_r.p; a.p = c->d_p.p; a.q = c->d_q.p; a.bv = c->d_bv.p; a.dpar = c->d_dpar.p; a.ap = c->d_ap.p;
+    a.part_ap = c->d_part_ap.p; a.part_dr1 = c->d_part_dr1.p; a.part_rr = c->d_part_rr.p;
+    a.n_dr1 = n_dr1; a.n_rr = n_rr; a.ncb = ncb; a.cb = cb;
+    a.sc = c->d_sc.p; a.scn = c->d_sc.p; a.cgres = c->d_cgres.p;
+    return 0;
+}
+
+// Fletcher-Reeves iterations on the device; the host looks at the stop flag
+// every cg_check_every iterations.
+static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
+    int e;
+    CGArgs a0 = a, a1 = a;
+    a0.sc = c->d_sc.p; a0.scn = c->d_sc.p + 1;
+    a1.sc = c->d_sc.p + 1; a1.scn = c->d_sc.p;
+    CGArgs ai = a;
+    ai.scn = c->d_sc.p;  // init writes state 0
+    c->tic(T_CG);
+    if ((e = crt_launch_cg_init(c->stream, ai, bvec))) return e;
+    CGScalars sc{};
+    int it = 0;
+    double tmv = 0.0;
+    while (it < a.ncgmax) {
+        int n = std::min(c->cg_check_every, a.ncgmax - it);
+        c->tic(T_CGMV);
+        for (int q = 0; q < n; ++q, ++it)
+            if ((e = crt_launch_cg_iter(c->stream, (it & 1) ? a1 : a0))) return e;
+        c->toc(T_CGMV);
+        CRT_HIP(hipMemcpyAsync(&sc, c->d_sc.p + (it & 1), sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+        CRT_HIP(hipStreamSynchronize(c->stream));
+        c->collect(T_CGMV);
+        tmv += c->times[T_CGMV];
+        if (sc.done) break;
+    }
+    c->toc(T_CG);
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_CG);
+    int done_its = sc.done ? sc.ncg : a.ncgmax;
+    *ncg = done_its;
+    // per-iteration device time of the batches (the last batch may include no-op launches)
+    c->times[T_CGMV] = it > 0 ? tmv / it : 0.0;
+    double ncgd = (double)(float)(*ncg);
+    CRT_HIP(hipMemcpyAsync(c->d_cgres.p, &ncgd, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfak) {
+    std::vector<double> w(nanz);
+    for (int i = 0; i < nanz; ++i) w[i] = wmatd[i] * (double)wdfak[i];
+    return c->d_w.upload(w.data(), nanz, c->stream);
+}
+
+int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,
+            const double *cgfac, const double *bvec, double *dpar, double *cgres, int *ncg) {
+    if (!c || !wmatd || !wdfak || !cgfac || !bvec || ncgmax < 0) return CRT_ERR_ARG;
+    if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int e = cg_weights(c, nanz, wmatd, wdfak);
+    e = e ? e : c->d_cgfac.upload(cgfac, manz, c->stream);
+    e = e ? e : c->d_bvec.upload(reinterpret_cast<const zd *>(bvec), manz, c->stream);
+    if (e) return e;
+    CGArgs a;
+    if ((e = cg_prepare(c, nanz, manz, ncgmax, lam, eps, a))) return e;
+    int n = 0;
+    if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
+    if (dpar) CRT_HIP(hipMemcpy(dpar, c->d_dpar.p, manz * sizeof(zd), hipMemcpyDeviceToHost));
+    if (cgres) CRT_HIP(hipMemcpy(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (ncg) *ncg = n;
+    return 0;
+}
+
+int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, const double *wmatd, const int *wdfak,
+               const double *dat, const double *sigmaa, double *par, double *dpar, double *cgres, int *ncg,
+               double *cgfac, double *bvec0) {
+    if (!c || !wmatd || !wdfak || !dat || !par || ncgmax < 0) return CRT_ERR_ARG;
+    if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
+    if (!sigmaa && !c->have_volt) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int e = cg_weights(c, nanz, wmatd, wdfak);
+    // residual d - f(m)
+    std::vector<zd> res(nanz), sa(nanz);
+    if (sigmaa) std::memcpy(sa.data(), sigmaa, nanz * sizeof(zd));
+    else CRT_HIP(hipMemcpy(sa.data(), c->d_sigmaa.p, nanz * sizeof(zd), hipMemcpyDeviceToHost));
+    const zd *dd = reinterpret_cast<const zd *>(dat);
+    for (int i = 0; i < nanz; ++i) res[i] = zd{dd[i].re - sa[i].re, dd[i].im - sa[i].im};
+    e = e ? e : c->d_res.upload(res.data(), nanz, c->stream);
+    e = e ? e : c->d_par.upload(reinterpret_cast<const zd *>(par), manz, c->stream);
+    e = e ? e : c->d_rm.ensure(manz);
+    e = e ? e : c->d_cgfac.ensure(manz);
+    e = e ? e : c->d_bvec.ensure(manz);
+    if (e) return e;
+    CGArgs a;
+    if ((e = cg_prepare(c, nanz, manz, ncgmax, lam, eps, a))) return e;
+    if ((e = crt_launch_cg_rmul(c->stream, a, c->d_par.p, c->d_rm.p))) return e;
+    if ((e = crt_launch_cg_rhs(c->stream, nanz, manz, lam, c->d_sens.p, c->d_w.p, c->d_res.p, c->d_rm.p,
+                               c->d_rdiag.p, c->d_cgfac.p, c->d_bvec.p)))
+        return e;
+    if (cgfac) CRT_HIP(hipMemcpyAsync(cgfac, c->d_cgfac.p, manz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (bvec0) CRT_HIP(hipMemcpyAsync(bvec0, c->d_bvec.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    int n = 0;
+    if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
+    if ((e = crt_launch_cg_final(c->stream, manz, step, c->d_cgfac.p, c->d_dpar.p, c->d_par.p))) return e;
+    if (dpar) CRT_HIP(hipMemcpyAsync(dpar, c->d_dpar.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipMemcpyAsync(par, c->d_par.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    if (cgres) CRT_HIP(hipMemcpyAsync(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    if (ncg) *ncg = n;
+    return 0;
+}
+
+int crt_get_times(crt_ctx *c, double *ms, int n) {
+    if (!c || !ms) return CRT_ERR_ARG;
+    for (int i = 0; i < n && i < T_N; ++i) ms[i] = c->times[i];
+    return 0;
+}
+
+// Cuthill-McKee renumbering with minimal-degree start nodes; the start giving
+// the smallest band width wins (cutmckee/cutmck.f:126-300).
+int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel, int elanz,
+               int *perm, int *mb_out) {
+    if (sanz <= 0 || !perm) return CRT_ERR_ARG;
+    const int GRMAX = 500, SPMAX = 100;
+    // adjacency lists in insertion order (the order decides ties)
+    std::vector<int> graph((size_t)GRMAX * sanz, 0), grad(sanz, 0);
+    auto G = [&](int k, int n) -> int & { return graph[(size_t)(n - 1) * GRMAX + (k - 1)]; };
+    int idum = 0;
+    for (int l = 0; l < typanz; ++l) {
+        for (int m = 1; m <= nelanz[l]; ++m) {
+            const int *el = nrel + (size_t)(idum + m - 1) * maxn;
+            for (int i = 1; i <= selanz[l] - 1; ++i) {
+                int nzp = el[i - 1];
+                for (int j = i + 1; j <= selanz[l]; ++j) {
+                    int nnp = el[j - 1];
+                    if (nzp < 1 || nzp > sanz || nnp < 1 || nnp > sanz) return CRT_ERR_ARG;
+                    bool dup = false;
+                    for (int k = 1; k <= GRMAX; ++k) {
+                        if (G(k, nzp) == nnp) { dup = true; break; }
+                        if (G(k, nzp) > 0) continue;
+                        G(k, nzp) = nnp;
+                        grad[nzp - 1]++;
+                        break;
+                    }
+                    if (dup) continue;
+                    grad[nnp - 1]++;
+                    if (grad[nnp - 1] <= GRMAX) G(grad[nnp - 1], nnp) = nzp;
+                }
+            }
+        }
+        idum += nelanz[l];
+    }
+    for (int i = 0; i < sanz; ++i)
+        if (grad[i] > GRMAX) return CRT_ERR_UNSUPPORTED;
+    int mingd = *std::min_element(grad.begin(), grad.end());
+    if (mingd == 0) return CRT_ERR_ARG;
+    int spanz = std::min(std::max(elanz / 10, 1000), SPMAX);
+    std::vector<int> start(sanz + 1, 0);
+    int k = 0;
+    for (;;) {
+        bool done = false;
+        for (int i = 1; i <= sanz; ++i)
+            if (grad[i - 1] == mingd) {
+                start[++k] = i;
+                if (k >= spanz) { done = true; break; }
+            }
+        if (done || mingd > elanz / 10) break;
+        ++mingd;
+    }
+    spanz = k - 1;
+    std::vector<int> neu(sanz), neuin(sanz), level(sanz + 2);
+    std::vector<char> num(sanz);
+    int mmin = sanz;
+    bool found = false;
+    for (int is = 1; is <= spanz; ++is) {
+        int nstart = start[is];
+        neu[0] = nstart;
+        neuin[nstart - 1] = 1;
+        std::fill(num.begin(), num.end(), 0);
+        num[nstart - 1] = 1;
+        level[1] = 1;
+        int levs = 1, leve = 1, nlev = 1, l = 1;
+        for (;;) {
+            for (int j = levs; j <= leve; ++j) {
+                int nzp = neu[j - 1], gradzp = grad[nzp - 1];
+                for (;;) {
+                    int mingr = GRMAX, kk = 0;
+                    for (int i = 1; i <= gradzp; ++i) {
+                        int nnp = G(i, nzp);
+                        if (num[nnp - 1] || grad[nnp - 1] > mingr) continue;
+                        mingr = grad[nnp - 1];
+                        kk = nnp;
+                    }
+                    if (!kk) break;
+                    ++l;
+                    neu[l - 1] = kk;
+                    neuin[kk - 1] = l;
+                    num[kk - 1] = 1;
+                }
+            }
+            levs += level[nlev];
+            ++nlev;
+            if (nlev > sanz) return CRT_ERR_ARG;
+            level[nlev] = l - levs + 1;
+            leve += level[nlev];
+            if (level[nlev] == 0 && leve < sanz) return CRT_ERR_ARG;  // disconnected grid
+            if (leve >= sanz) break;
+        }
+        int mb = 0;
+        for (int i = 1; i <= sanz; ++i) {
+            int nzp = neuin[i - 1];
+            for (int j = 1; j <= grad[i - 1]; ++j) mb = std::max(mb, std::abs(neuin[G(j, i) - 1] - nzp));
+        }
+        if (mb < mmin) {
+            mmin = mb;
+            std::copy(neuin.begin(), neuin.end(), perm);
+            found = true;
+        }
+    }
+    if (!found) return CRT_ERR_ARG;
+    if (mb_out) *mb_out = mmin;
+    return 0;
+}
+
+int crt_chol(crt_ctx *, int, int, double *) { return CRT_ERR_UNSUPPORTED; }
+int crt_vre(crt_ctx *, int, int, const double *, const double *, double *) { return CRT_ERR_UNSUPPORTED; }
+int crt_cholz(crt_ctx *, int, double *, double *) { return CRT_ERR_UNSUPPORTED; }
+
+}  // extern "C"

@@ -1,0 +1,430 @@
+// crt_band.hip -- batched complex-symmetric band factorisation and substitution.
+//
+// One workgroup factorises one (electrode l, wavenumber k) system
+//     A_lk = sum_e sigma_e elbg(e,.,k) + sum_rel sigma_rnr(rel) relbg(rel,.) kg(rel,l,k)
+// (src/kompab.f90:80-138), scaled to a unit-modulus diagonal
+// (src/scalab.f90:52-76), factorised A = L L^T without conjugation
+// (src/chol.f90:46-78) and forward-substituted y = L^{-1} b with the unit
+// source b = -fak(enr(l)) e_enr(l) (src/kompab.f90:162, src/kompb.f90:58).
+// The back substitution L^T x = -y and kpot = x * fak (src/vre.f90:45-69,
+// src/fem.f90:383-394) run in crt_solve_kernel, one wavefront per right-hand
+// side.
+//
+// Storage.  Rows are grouped in blocks of NB = 8.  The factor is written as
+// block-column panels panel[J][a][r][c] = L(8(J+a)+r, 8J+c), a = 0..D with
+// D = ceil(mb/8), so a panel is one contiguous (D+1) KB record.  During the
+// factorisation the active window (block rows J..J+D+1, lower triangle) sits
+// in LDS, one 8x8 tile per slot; block (I,K) with d = I-K lives in slot
+// base[d] + K mod (D+2-d), so the window slides by one block per step with no
+// data movement.  The matrix is never materialised in HBM: each entering row
+// block is assembled from the CSR contribution lists straight into LDS.
+#include "crt_common.hpp"
+
+#define NB CRT_NB
+#define NB2 (NB * NB)
+
+namespace {
+
+__device__ __forceinline__ zd shfl_z(zd v, int src) {
+    return zd{__shfl(v.re, src, 64), __shfl(v.im, src, 64)};
+}
+__device__ __forceinline__ zd shfl_xor_z(zd v, int m) {
+    return zd{__shfl_xor(v.re, m, 64), __shfl_xor(v.im, m, 64)};
+}
+
+// Sum of the contributions to A(i,j) in the reference's element order.
+// Contraction is off so the sum is formed exactly as the Fortran loop does.
+__device__ __forceinline__ zd assemble_entry(const DevGrid &g, int e0, int e1, int l, int k) {
+#pragma clang fp contract(off)
+    zd s = zd{0.0, 0.0};
+    for (int q = e0; q < e1; ++q) {
+        int c = g.contrib[q];
+        double dum;
+        zd sg;
+        if (c >= 0) {
+            int e = c / g.npk, imn = c - e * g.npk;
+            dum = g.elbg[((size_t)k * g.npk + imn) * g.elanz + e];
+            sg = g.sigma[e];
+        } else {
+            int cc = -c - 1;
+            int rel = cc / g.npk, imn = cc - rel * g.npk;
+            dum = g.relbg[(size_t)imn * g.relanz + rel] * g.kg[((size_t)k * g.eanz + l) * g.relanz + rel];
+            sg = g.sigma[g.rnr[rel]];
+        }
+        s.re = s.re + dum * sg.re;
+        s.im = s.im + dum * sg.im;
+    }
+    return s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ diagonal
+// fak_i = 1/sqrt|a_ii|, dga_i = |a_ii| per system (src/scalab.f90:54-66)
+__global__ void __launch_bounds__(256) crt_diag_kernel(DevGrid g, int nsys, const int2 *__restrict__ sys,
+                                                       double *__restrict__ fak, double *__restrict__ dga,
+                                                       int *__restrict__ status) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nsys * g.sanz) return;
+    int s = (int)(t / g.sanz), i = (int)(t - (long)s * g.sanz);
+    int2 lk = sys[s];
+    int q = g.row_ptr[i + 1] - 1;  // diagonal is the last CSR entry of the row
+    zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
+    double dum = zabs(a);
+    size_t o = (size_t)s * g.nblk * NB + i;
+    if (!(dum > 0.0)) {
+        status[s] = CRT_ERR_ZERODIAG;  // plain store: any non-zero code fails the call
+        dum = 1.0;
+    }
+    fak[o] = 1.0 / sqrt(dum);
+    dga[o] = dum;
+}
+
+// --------------------------------------------------------------- factorise
+struct FactorArgs {
+    DevGrid g;
+    int nsys;
+    const int2 *sys;    // (l, k) per system, 0-based
+    const double *fak;  // [sys][nblk*NB]
+    const double *dga;
+    zd *panels;         // [sys][nblk][D+1][NB][NB]
+    zd *y;              // [sys][nblk*NB]
+    int *status;
+};
+
+__global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
+    extern __shared__ double smem_d[];
+    const DevGrid &g = A.g;
+    const int D = g.D, nblk = g.nblk, tid = threadIdx.x;
+    const int s = blockIdx.x;
+    if (s >= A.nsys) return;
+    const int l = A.sys[s].x, k = A.sys[s].y;
+    const int enode = g.enr[l];
+    const double *fak = A.fak + (size_t)s * nblk * NB;
+    const double *dga = A.dga + (size_t)s * nblk * NB;
+    zd *panel = A.panels + (size_t)s * nblk * (D + 1) * NB2;
+    zd *yout = A.y + (size_t)s * nblk * NB;
+
+    // LDS carve-up
+    const int nslot = (D + 1) * (D + 4) / 2;
+    zd *slots = reinterpret_cast<zd *>(smem_d);               // nslot * 64
+    zd *bring = slots + (size_t)nslot * NB2;                  // (D+2) * 8
+    zd *ljj = bring + (D + 2) * NB;                           // 64 (factored diag block)
+    zd *linv = ljj + NB2;                                     // 8 inverse pivots
+    zd *yj = linv + NB;                                       // 8
+    int *base = reinterpret_cast<int *>(yj + NB);             // D+1
+    int *jm = base + (D + 1);                                 // D+1 : J mod (D+2-d)
+    unsigned char *pairs = reinterpret_cast<unsigned char *>(jm + (D + 1));  // 2*npairs
+    const int npairs = D * (D + 1) / 2;
+
+    if (tid == 0) {
+        int b = 0;
+        for (int d = 0; d <= D; ++d) { base[d] = b; b += D + 2 - d; jm[d] = 0; }
+        int p = 0;
+        for (int a = 1; a <= D; ++a)
+            for (int bb = 1; bb <= a; ++bb) { pairs[2 * p] = (unsigned char)a; pairs[2 * p + 1] = (unsigned char)bb; ++p; }
+    }
+    __syncthreads();
+
+    auto slot_of = [&](int I, int K) -> zd * {
+        int d = I - K, m = D + 2 - d;
+        return slots + (size_t)(base[d] + K % m) * NB2;
+    };
+
+    // assemble row block I into its (zeroed) slots; CSR scatter, one thread per entry
+    auto scatter_rows = [&](int I, int t0, int nthr) {
+        for (int r = 0; r < NB; ++r) {
+            int i = I * NB + r;
+            if (i >= g.sanz) {
+                if (t0 == 0) slot_of(I, I)[r * NB + r] = zd{1.0, 0.0};
+                continue;
+            }
+            int q0 = g.row_ptr[i], q1 = g.row_ptr[i + 1];
+            double fi = fak[i];
+            for (int q = q0 + t0; q < q1; q += nthr) {
+                int j = g.col[q];
+                zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], l, k);
+                zd v;
+                if (j == i) {
+                    double dum = dga[i];
+                    v = zd{a.re / dum, a.im / dum};
+                } else {
+                    double f = fi * fak[j];
+                    v = zd{a.re * f, a.im * f};
+                }
+                slot_of(I, j / NB)[r * NB + (j % NB)] = v;
+            }
+        }
+        if (t0 < NB) {
+            int i = I * NB + t0;
+            zd bv = zd{0.0, 0.0};
+            if (i < g.sanz && i == enode) bv = zd{-fak[i], 0.0};
+            bring[(I % (D + 2)) * NB + t0] = bv;
+        }
+    };
+
+    // ---- initial window: row blocks 0..min(D, nblk-1)
+    for (int q = tid; q < nslot * NB2; q += blockDim.x) slots[q] = zd{0.0, 0.0};
+    __syncthreads();
+    for (int I = 0; I <= D && I < nblk; ++I) scatter_rows(I, tid, blockDim.x);
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63;
+    int err = 0;
+    for (int J = 0; J < nblk; ++J) {
+        const int Ie = J + D + 1;  // entering row block
+        // ---------------- phase 1: diag block (wave 0) | zero entering slots (others)
+        if (wave == 0) {
+            const int r = lane >> 3, c = lane & 7;
+            zd *dj = slot_of(J, J);
+            zd v = dj[lane];
+            zd yv = bring[(J % (D + 2)) * NB + (lane & 7)];
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc) {
+                zd pv = shfl_z(v, cc * NB + cc);
+                zd piv = zsqrt(pv);
+                if (zabs(pv) <= 0.0) err = CRT_ERR_PIVOT;
+                zd pinv = zinv(piv);
+                if (r == cc && c == cc) v = piv;
+                if (r > cc && c == cc) v = zmul(v, pinv);
+                zd lr = shfl_z(v, r * NB + cc);   // L(r, cc)
+                zd lc = shfl_z(v, c * NB + cc);   // L(c, cc)
+                if (r > cc && c > cc && c <= r) v = zfms(v, lr, lc);
+                // forward substitution on the RHS block (lanes 0..7 hold y(c))
+                zd ycc = shfl_z(yv, cc);
+                ycc = zmul(ycc, pinv);
+                if (lane == cc) yv = ycc;
+                zd lcol = shfl_z(v, (lane & 7) * NB + cc);  // L(lane, cc) after division
+                if ((lane & 7) > cc) yv = zfms(yv, lcol, ycc);
+                if (lane == 0) linv[cc] = pinv;
+            }
+            if (c <= r) dj[lane] = v;
+            ljj[lane] = (c <= r) ? v : zd{0.0, 0.0};
+            if (lane < NB) {
+                yj[lane] = yv;
+                bring[(J % (D + 2)) * NB + lane] = yv;
+                yout[(size_t)J * NB + lane] = yv;
+            }
+        } else if (Ie < nblk) {
+            // zero the slots of the entering row block (they held column J-1)
+            int nthr = blockDim.x - 64, t0 = tid - 64;
+            for (int q = t0; q < (D + 1) * NB2; q += nthr) {
+                int K = Ie - D + q / NB2;  // K = J+1 .. J+D+1
+                slot_of(Ie, K)[q % NB2] = zd{0.0, 0.0};
+            }
+        }
+        __syncthreads();
+        // ---------------- phase 2: TRSM + RHS update (rows 8..8D+7) | assemble entering block
+        {
+            int nrow = NB * D;
+            int ntrsm = ((nrow + 63) / 64) * 64;
+            if (tid < nrow) {
+                int a = 1 + tid / NB, r = tid % NB;
+                if (J + a < nblk) {
+                    zd *blk = slot_of(J + a, J);
+                    zd xr[NB];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) xr[c] = blk[r * NB + c];
+                    zd bacc = bring[((J + a) % (D + 2)) * NB + r];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        zd sacc = xr[c];
+#pragma unroll
+                        for (int cp = 0; cp < c; ++cp) sacc = zfms(sacc, xr[cp], ljj[c * NB + cp]);
+                        xr[c] = zmul(sacc, linv[c]);
+                        bacc = zfms(bacc, xr[c], yj[c]);
+                    }
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) blk[r * NB + c] = xr[c];
+                    bring[((J + a) % (D + 2)) * NB + r] = bacc;
+                }
+            } else if (tid >= ntrsm && Ie < nblk) {
+                scatter_rows(Ie, tid - ntrsm, blockDim.x - ntrsm);
+            }
+            if (ntrsm >= (int)blockDim.x && Ie < nblk) {
+                // not enough spare threads: assemble after the TRSM
+                __syncthreads();
+                scatter_rows(Ie, tid, blockDim.x);
+            }
+        }
+        __syncthreads();
+        // ---------------- phase 3: store panel J | rank-8 update of the trailing window
+        {
+            zd *pj = panel + (size_t)J * (D + 1) * NB2;
+            for (int q = tid; q < (D + 1) * NB2; q += blockDim.x) {
+                int a = q / NB2;
+                pj[q] = (J + a < nblk) ? slot_of(J + a, J)[q % NB2] : zd{0.0, 0.0};
+            }
+            int nitem = npairs * 16;
+            for (int it = tid; it < nitem; it += blockDim.x) {
+                int p = it >> 4, tile = it & 15;
+                int a = pairs[2 * p], b = pairs[2 * p + 1];
+                if (J + a >= nblk) continue;
+                int r0 = (tile >> 2) * 2, c0 = (tile & 3) * 2;
+                if (a == b && c0 > r0 + 1) continue;  // strictly upper tile of a diagonal block
+                const zd *La = slot_of(J + a, J);
+                const zd *Lb = slot_of(J + b, J);
+                zd *T = slot_of(J + a, J + b);
+                zd t00 = T[r0 * NB + c0], t01 = T[r0 * NB + c0 + 1];
+                zd t10 = T[(r0 + 1) * NB + c0], t11 = T[(r0 + 1) * NB + c0 + 1];
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    zd a0 = La[r0 * NB + q], a1 = La[(r0 + 1) * NB + q];
+                    zd b0 = Lb[c0 * NB + q], b1 = Lb[(c0 + 1) * NB + q];
+                    t00 = zfms(t00, a0, b0); t01 = zfms(t01, a0, b1);
+                    t10 = zfms(t10, a1, b0); t11 = zfms(t11, a1, b1);
+                }
+                T[r0 * NB + c0] = t00; T[r0 * NB + c0 + 1] = t01;
+                T[(r0 + 1) * NB + c0] = t10; T[(r0 + 1) * NB + c0 + 1] = t11;
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0 && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+
+size_t crt_factor_lds_bytes(int D) {
+    size_t nslot = (size_t)(D + 1) * (D + 4) / 2;
+    size_t b = nslot * NB2 * sizeof(zd) + (size_t)(D + 2) * NB * sizeof(zd) + NB2 * sizeof(zd) +
+               2 * NB * sizeof(zd) + 2 * (D + 1) * sizeof(int) + (size_t)D * (D + 1) + 16;
+    return (b + 15) & ~(size_t)15;
+}
+
+// ------------------------------------------------------------------- solve
+// One wavefront per right-hand side.  If have_y, y is the forward result of the
+// factorisation; otherwise the forward sweep is run from b = -fak(enode) e_enode.
+// Back substitution L^T x = -y, then kpot = x * fak written node-major:
+// kpotN[(node*K + k)*eanz + l] (src/vre.f90:59-69, src/fem.f90:383-394).
+struct SolveArgs {
+    int sanz, nblk, D, kwnanz, eanz;
+    int nrhs;
+    const int4 *rhs;     // (l, k, factor system index, have_y)
+    const int *enr;
+    const double *fak;   // [fsys][nblk*NB]
+    const zd *panels;    // [fsys][nblk][D+1][64]
+    const zd *y;         // [fsys][nblk*NB]
+    zd *yscr;            // [nrhs][nblk*NB] forward result of right-hand sides without have_y
+    zd *kpotN;
+};
+
+__global__ void __launch_bounds__(256) crt_solve_kernel(SolveArgs A) {
+    extern __shared__ double smem_s[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int rid = blockIdx.x * wpb + wave;
+    if (rid >= A.nrhs) return;
+    const int D = A.D, nblk = A.nblk;
+    const int4 rq = A.rhs[rid];
+    const int l = rq.x, k = rq.y, fs = rq.z;
+    const double *fak = A.fak + (size_t)fs * nblk * NB;
+    const zd *panel = A.panels + (size_t)fs * nblk * (D + 1) * NB2;
+    zd *ring = reinterpret_cast<zd *>(smem_s) + (size_t)wave * (D + 1) * NB;
+    const int r = lane >> 3, c = lane & 7;
+    const int enode = A.enr[l];
+    zd *yscr = A.yscr + (size_t)rid * nblk * NB;
+
+    if (!rq.w) {
+        // forward sweep: y_J = L_JJ^{-1} (b_J - sum_a L(J, J-a) y_{J-a})
+        for (int J = 0; J < nblk; ++J) {
+            zd acc = zd{0.0, 0.0};
+            for (int a = 1; a <= D && a <= J; ++a) {
+                zd Lv = panel[((size_t)(J - a) * (D + 1) + a) * NB2 + lane];  // L(J*8+r, (J-a)*8+c)
+                zd yv = ring[((J - a) % (D + 1)) * NB + c];
+                acc = zfma(acc, Lv, yv);
+            }
+            // reduce over c (lanes r*8 .. r*8+7)
+            acc = zadd(acc, shfl_xor_z(acc, 1));
+            acc = zadd(acc, shfl_xor_z(acc, 2));
+            acc = zadd(acc, shfl_xor_z(acc, 4));
+            int i = J * NB + r;
+            zd bv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+            zd rhs = zsub(bv, acc);  // valid on lanes with c == 0 for row r
+            zd Ld = panel[((size_t)J * (D + 1)) * NB2 + lane];
+            zd yv = shfl_z(rhs, (lane & 7) * NB);  // lane holds rhs of row (lane & 7)
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc) {
+                zd piv = shfl_z(Ld, cc * NB + cc);
+                zd ycc = zdiv(shfl_z(yv, cc), piv);
+                if ((lane & 7) == cc) yv = ycc;
+                zd lcol = shfl_z(Ld, (lane & 7) * NB + cc);
+                if ((lane & 7) > cc) yv = zfms(yv, lcol, ycc);
+            }
+            if (lane < NB) {
+                ring[(J % (D + 1)) * NB + lane] = yv;
+                yscr[(size_t)J * NB + lane] = yv;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // backward sweep: L^T x = -y ; the ring holds x of the D blocks below J
+    const zd *yglob = rq.w ? A.y + (size_t)fs * nblk * NB : yscr;
+    for (int J = nblk - 1; J >= 0; --J) {
+        zd acc = zd{0.0, 0.0};
+        for (int a = 1; a <= D && J + a < nblk; ++a) {
+            zd Lv = panel[((size_t)J * (D + 1) + a) * NB2 + lane];  // L((J+a)*8+r, J*8+c)
+            zd xv = ring[((J + a) % (D + 1)) * NB + r];
+            acc = zfma(acc, Lv, xv);
+        }
+        // reduce over r (lanes c, c+8, ..., c+56)
+        acc = zadd(acc, shfl_xor_z(acc, 8));
+        acc = zadd(acc, shfl_xor_z(acc, 16));
+        acc = zadd(acc, shfl_xor_z(acc, 32));
+        zd yJ = yglob[(size_t)J * NB + c];
+        zd rhs = zsub(zd{-yJ.re, -yJ.im}, acc);  // lanes 0..7 hold rhs(c)
+        zd Ld = panel[((size_t)J * (D + 1)) * NB2 + lane];
+        zd xv = rhs;
+#pragma unroll
+        for (int cc = NB - 1; cc >= 0; --cc) {
+            zd piv = shfl_z(Ld, cc * NB + cc);
+            zd xcc = zdiv(shfl_z(xv, cc), piv);
+            if ((lane & 7) == cc) xv = xcc;
+            zd lrow = shfl_z(Ld, cc * NB + (lane & 7));  // L(cc, lane&7): row cc, column < cc
+            if ((lane & 7) < cc) xv = zfms(xv, lrow, xcc);
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < NB) {
+            ring[(J % (D + 1)) * NB + lane] = xv;
+            int i = J * NB + lane;
+            if (i < A.sanz) {
+                double f = fak[i];
+                A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+size_t crt_solve_lds_bytes(int D, int waves) { return (size_t)waves * (D + 1) * NB * sizeof(zd); }
+
+int crt_launch_diag(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, double *fak, double *dga,
+                    int *status) {
+    long n = (long)nsys * g.sanz;
+    hipLaunchKernelGGL(crt_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, sys,
+                       fak, dga, status);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_factor(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,
+                      const double *dga, zd *panels, zd *y, int *status) {
+    FactorArgs a{g, nsys, sys, fak, dga, panels, y, status};
+    size_t lds = crt_factor_lds_bytes(g.D);
+    if (lds > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)crt_factor_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(crt_factor_kernel, dim3(nsys), dim3(256), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_solve(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs,
+                     const int4 *rhs, const int *enr, const double *fak, const zd *panels, const zd *y,
+                     zd *yscr, zd *kpotN) {
+    SolveArgs a{sanz, nblk, D, kwnanz, eanz, nrhs, rhs, enr, fak, panels, y, yscr, kpotN};
+    const int waves = 4;
+    size_t lds = crt_solve_lds_bytes(D, waves);
+    hipLaunchKernelGGL(crt_solve_kernel, dim3((nrhs + waves - 1) / waves), dim3(64 * waves), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

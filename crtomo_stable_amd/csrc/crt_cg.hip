@@ -1,0 +1,357 @@
+// crt_cg.hip -- Gauss-Newton normal-equation step on the device.
+//
+// RHS and Jacobi scaling (src/update.f90:63-136,179-231,236-293):
+//   cgfac_j = (sum_i |J_ij|^2 w_i + lam R_jj)^(-1/2),   w_i = wmatd_i * wdfak_i
+//   bvec_j  = cgfac_j (sum_i conj(J_ij) w_i (d_i - f_i) - lam (R m)_j)
+// Fletcher-Reeves CG on B = C (J^H W J + lam R) C (src/cg_mod.f90:426-518):
+//   ap = J (C p)                         bap  (src/cg_mod.f90:537-543)
+//   t  = R (C p)                         bp / bptri (src/cg_mod.f90:564-576, 600-611)
+//   Bp = C (J^H (W ap) + lam t)          bb   (src/cg_mod.f90:715-725)
+// with real alpha = dr / Re(p^H Bp), beta = dr / dr_prev and the stop test
+// dr <= eps * dr_0 on the squared residual.  Scalars live on the device; every
+// reduction has a fixed order, so the iteration is deterministic.  J is
+// column-major (nanz, manz) complex FP64, as written by the sensitivity kernel.
+#include "crt_common.hpp"
+
+#include <algorithm>
+
+#define CG_BLK 256
+
+struct CGScalars {
+    double dr, dr0, dr1, alpha, beta;
+    int k, done, ncg, pad;
+};
+
+struct CGArgs {
+    int nanz, manz, ltri, nx, maxn, ncgmax;
+    double lam, eps;
+    const zd *J;            // (nanz, manz) column-major
+    const double *w;        // nanz: wmatd * wdfak
+    const double *smatm;    // (manz, ncol) column-major
+    const int *nachbar;     // (manz, maxn+1) column-major, 1-based (ltri 1/2)
+    const double *cgfac;    // manz
+    zd *r, *p, *q, *bv, *dpar;  // manz
+    zd *ap;                 // nanz (W-weighted after reduction: wap)
+    zd *part_ap;            // [ncb][nanz]
+    double *part_dr1;       // per block of the J^H kernel
+    double *part_rr;        // per block of the update kernel
+    int n_dr1, n_rr, ncb, cb;
+    const CGScalars *sc;    // state after the previous iteration (read only)
+    CGScalars *scn;         // state of this iteration (written by block 0)
+    double *cgres;          // ncgmax+1
+};
+
+namespace {
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    // fixed-order tree reduction over a 256-thread block
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    return s;
+}
+
+// sum of a partial array, identical in every block
+__device__ __forceinline__ double reduce_partials(const double *part, int n, double *red) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    return block_sum(v, red);
+}
+
+// (R x)_i with optional per-cell scaling c on the neighbour values
+__device__ __forceinline__ zd rstencil(const CGArgs &A, const zd *x, int i) {
+    const int manz = A.manz, nx = A.nx;
+    const double *s = A.smatm;
+    zd cd = zd{0.0, 0.0};
+    if (A.ltri == 0) {
+        // 5-point stencil of the regular grid (src/cg_mod.f90:564-576)
+        if (i > 0) cd = zfma(cd, x[i - 1], zd{s[manz + i - 1], 0.0});
+        if (i < manz - 1) cd = zfma(cd, x[i + 1], zd{s[manz + i], 0.0});
+        if (i >= nx) cd = zfma(cd, x[i - nx], zd{s[2 * manz + i - nx], 0.0});
+        if (i < manz - nx) cd = zfma(cd, x[i + nx], zd{s[2 * manz + i], 0.0});
+        return zfma(cd, x[i], zd{s[i], 0.0});
+    }
+    // neighbour stencil of unstructured grids (src/cg_mod.f90:600-611)
+    for (int ij = 0; ij < A.maxn; ++ij) {
+        int in = A.nachbar[(size_t)ij * manz + i];
+        if (in != 0) cd = zfma(cd, x[in - 1], zd{s[(size_t)ij * manz + i], 0.0});
+    }
+    return zfma(cd, x[i], zd{s[(size_t)A.maxn * manz + i], 0.0});
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- update: RHS pass
+// One wave per column j: cgfac_j and bvec_j in a single pass over J.
+struct RhsArgs {
+    int nanz, manz;
+    double lam;
+    const zd *J;
+    const double *w;
+    const zd *res;       // d - f (nanz)
+    const zd *rm;        // R m (manz)
+    const double *rdiag; // R_jj (manz)
+    double *cgfac;
+    zd *bvec;
+};
+
+__global__ void __launch_bounds__(256) crt_cg_rhs_kernel(RhsArgs A) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (blockDim.x >> 6) + wv;
+    if (j >= A.manz) return;
+    const zd *col = A.J + (size_t)j * A.nanz;
+    double nrm = 0.0;
+    zd s = zd{0.0, 0.0};
+    for (int i = lane; i < A.nanz; i += 64) {
+        zd v = col[i];
+        double wi = A.w[i];
+        nrm += (v.re * v.re + v.im * v.im) * wi;
+        zd d = A.res[i];
+        // conj(v) * w * d
+        s.re += wi * (v.re * d.re + v.im * d.im);
+        s.im += wi * (v.re * d.im - v.im * d.re);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        nrm += __shfl_xor(nrm, o, 64);
+        s.re += __shfl_xor(s.re, o, 64);
+        s.im += __shfl_xor(s.im, o, 64);
+    }
+    if (lane == 0) {
+        double c = 1.0 / sqrt(nrm + A.lam * A.rdiag[j]);
+        A.cgfac[j] = c;
+        zd b = zd{s.re - A.lam * A.rm[j].re, s.im - A.lam * A.rm[j].im};
+        A.bvec[j] = zd{b.re * c, b.im * c};
+    }
+}
+
+// R m (unscaled) for the RHS
+__global__ void __launch_bounds__(256) crt_cg_rmul_kernel(CGArgs A, const zd *x, zd *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.manz) return;
+    out[i] = rstencil(A, x, i);
+}
+
+// ------------------------------------------------------------- CG iteration
+// init: r = bvec, p = 0, dpar = 0, |r|^2 partials
+__global__ void __launch_bounds__(CG_BLK) crt_cg_init_kernel(CGArgs A, const zd *bvec) {
+    __shared__ double red[8];
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (j < A.manz) {
+        zd b = bvec[j];
+        A.r[j] = b;
+        A.p[j] = zd{0.0, 0.0};
+        A.dpar[j] = zd{0.0, 0.0};
+        v = b.re * b.re + b.im * b.im;
+    }
+    double s = block_sum(v, red);
+    if (threadIdx.x == 0) A.part_rr[blockIdx.x] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CGScalars z{};
+        z.k = 0;
+        z.done = 0;
+        z.ncg = A.ncgmax;
+        *A.scn = z;
+    }
+}
+
+// K_c: dr, stop test, beta; p = r + beta p; q = C p
+// Reads the previous state A.sc, block 0 writes the new state A.scn.
+__global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
+    __shared__ double red[8];
+    const CGScalars cur = *A.sc;
+    if (cur.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *A.scn = cur;
+        return;
+    }
+    const int k = cur.k + 1;  // 1-based iteration number
+    double dr = reduce_partials(A.part_rr, A.n_rr, red);
+    double dr0, beta;
+    bool stop = false;
+    if (k == 1) {
+        dr0 = dr * A.eps;
+        beta = 0.0;
+    } else {
+        dr0 = cur.dr0;
+        stop = dr <= dr0;
+        beta = dr / cur.dr1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CGScalars nx = cur;
+        if (stop) {
+            nx.done = 1;
+            nx.ncg = k - 1;
+        } else {
+            nx.dr = dr;
+            nx.dr0 = dr0;
+            nx.beta = beta;
+            nx.k = k;
+            if (k == A.ncgmax) nx.ncg = A.ncgmax;
+            A.cgres[k] = (double)(float)(A.eps * dr / dr0);
+        }
+        *A.scn = nx;
+    }
+    if (stop) return;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < A.manz) {
+        zd r = A.r[j], p = A.p[j];
+        zd pn = zd{r.re + beta * p.re, r.im + beta * p.im};
+        A.p[j] = pn;
+        double c = A.cgfac[j];
+        A.q[j] = zd{pn.re * c, pn.im * c};
+    }
+}
+
+// K_d: partial ap over column chunks: part_ap[cb][i] = sum_{j in chunk} J(i,j) q_j
+__global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A) {
+    __shared__ zd qs[1024];
+    if (A.scn->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cb = blockIdx.y;
+    const int j0 = cb * A.cb, j1 = min(A.manz, j0 + A.cb);
+    for (int t = threadIdx.x; t < j1 - j0; t += blockDim.x) qs[t] = A.q[j0 + t];
+    __syncthreads();
+    if (i >= A.nanz) return;
+    zd acc0 = zd{0.0, 0.0}, acc1 = zd{0.0, 0.0};
+    const zd *col = A.J + (size_t)j0 * A.nanz + i;
+    int j = j0;
+    for (; j + 1 < j1; j += 2) {
+        zd v0 = col[0], v1 = col[A.nanz];
+        acc0 = zfma(acc0, v0, qs[j - j0]);
+        acc1 = zfma(acc1, v1, qs[j + 1 - j0]);
+        col += 2 * (size_t)A.nanz;
+    }
+    if (j < j1) acc0 = zfma(acc0, col[0], qs[j - j0]);
+    A.part_ap[(size_t)cb * A.nanz + i] = zadd(acc0, acc1);
+}
+
+// K_e: ap = sum over chunks, then wap = w * ap
+__global__ void __launch_bounds__(CG_BLK) crt_cg_apred_kernel(CGArgs A) {
+    if (A.scn->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.nanz) return;
+    zd s = zd{0.0, 0.0};
+    for (int cb = 0; cb < A.ncb; ++cb) s = zadd(s, A.part_ap[(size_t)cb * A.nanz + i]);
+    double wi = A.w[i];
+    A.ap[i] = zd{s.re * wi, s.im * wi};
+}
+
+// K_f: Bp_j = C_j (J^H wap + lam R q)_j, one wave per column, dr1 partials
+__global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A) {
+    __shared__ double red[8];
+    if (A.scn->done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cols_per_wave = 16;
+    const int jbase = (blockIdx.x * (blockDim.x >> 6) + wv) * cols_per_wave;
+    double dpart = 0.0;
+    for (int jj = 0; jj < cols_per_wave; ++jj) {
+        int j = jbase + jj;
+        if (j >= A.manz) break;
+        const zd *col = A.J + (size_t)j * A.nanz;
+        zd s = zd{0.0, 0.0};
+        for (int i = lane; i < A.nanz; i += 64) {
+            zd v = col[i], a = A.ap[i];
+            s.re += v.re * a.re + v.im * a.im;
+            s.im += v.re * a.im - v.im * a.re;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            s.re += __shfl_xor(s.re, o, 64);
+            s.im += __shfl_xor(s.im, o, 64);
+        }
+        if (lane == 0) {
+            zd t = rstencil(A, A.q, j);
+            double c = A.cgfac[j];
+            zd b = zd{(s.re + A.lam * t.re) * c, (s.im + A.lam * t.im) * c};
+            A.bv[j] = b;
+            zd p = A.p[j];
+            dpart += p.re * b.re + p.im * b.im;
+        }
+    }
+    double s = block_sum(dpart, red);
+    if (threadIdx.x == 0) A.part_dr1[blockIdx.x] = s;
+}
+
+// K_h: alpha; dpar += alpha p; r -= alpha Bp; |r|^2 partials
+__global__ void __launch_bounds__(CG_BLK) crt_cg_upd_kernel(CGArgs A) {
+    __shared__ double red[8];
+    if (A.scn->done) return;
+    double pbp = reduce_partials(A.part_dr1, A.n_dr1, red);
+    double dr = A.scn->dr;
+    double alpha = dr / pbp;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (j < A.manz) {
+        zd p = A.p[j], b = A.bv[j], r = A.r[j], d = A.dpar[j];
+        A.dpar[j] = zd{d.re + alpha * p.re, d.im + alpha * p.im};
+        zd rn = zd{r.re - alpha * b.re, r.im - alpha * b.im};
+        A.r[j] = rn;
+        v = rn.re * rn.re + rn.im * rn.im;
+    }
+    double s = block_sum(v, red);
+    if (threadIdx.x == 0) A.part_rr[blockIdx.x] = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.scn->alpha = alpha;
+        A.scn->dr1 = dr;
+    }
+}
+
+// final scaling dpar <- C dpar (src/update.f90:302-307) and par += step dpar
+__global__ void __launch_bounds__(256) crt_cg_final_kernel(int manz, double step, const double *cgfac, zd *dpar,
+                                                           zd *par) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= manz) return;
+    zd d = dpar[j];
+    double c = cgfac[j];
+    d = zd{d.re * c, d.im * c};
+    dpar[j] = d;
+    if (par) par[j] = zd{par[j].re + step * d.re, par[j].im + step * d.im};
+}
+
+// ---------------------------------------------------------------- launchers
+int crt_cg_geometry(int nanz, int manz, int *ncb, int *cb, int *n_dr1, int *n_rr) {
+    int rb = (nanz + CG_BLK - 1) / CG_BLK;
+    // aim for >= 2048 blocks in the J q pass, at most 1024 columns per chunk
+    int want = std::max(1, 2048 / std::max(rb, 1));
+    int c = std::max(16, std::min(1024, (manz + want - 1) / want));
+    *cb = c;
+    *ncb = (manz + c - 1) / c;
+    *n_dr1 = (manz + 4 * 16 - 1) / (4 * 16);
+    *n_rr = (manz + CG_BLK - 1) / CG_BLK;
+    return 0;
+}
+
+int crt_launch_cg_rhs(hipStream_t s, int nanz, int manz, double lam, const zd *J, const double *w, const zd *res,
+                      const zd *rm, const double *rdiag, double *cgfac, zd *bvec) {
+    RhsArgs a{nanz, manz, lam, J, w, res, rm, rdiag, cgfac, bvec};
+    hipLaunchKernelGGL(crt_cg_rhs_kernel, dim3((manz + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_cg_rmul(hipStream_t s, const CGArgs &a, const zd *x, zd *out) {
+    hipLaunchKernelGGL(crt_cg_rmul_kernel, dim3((a.manz + 255) / 256), dim3(256), 0, s, a, x, out);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_cg_init(hipStream_t s, const CGArgs &a, const zd *bvec) {
+    hipLaunchKernelGGL(crt_cg_init_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a, bvec);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
+    hipLaunchKernelGGL(crt_cg_dir_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+    hipLaunchKernelGGL(crt_cg_jq_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK, a.ncb), dim3(CG_BLK), 0, s, a);
+    hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK), dim3(CG_BLK), 0, s, a);
+    hipLaunchKernelGGL(crt_cg_jhw_kernel, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a);
+    hipLaunchKernelGGL(crt_cg_upd_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_cg_final(hipStream_t s, int manz, double step, const double *cgfac, zd *dpar, zd *par) {
+    hipLaunchKernelGGL(crt_cg_final_kernel, dim3((manz + 255) / 256), dim3(256), 0, s, manz, step, cgfac, dpar, par);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

@@ -1,0 +1,115 @@
+// crt_common.hpp -- shared types of the CRTomo HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+// Error numbers follow the reference's errmod codes (src/error.txt):
+//   19 band width exceeded, 26 element geometry, 27 zero diagonal,
+//   28 non-positive pivot, 35 beta geometry, 73 duplicate electrodes,
+//   82 zero voltage, 94/97 allocation.  Library-only codes start at 900.
+enum {
+    CRT_OK = 0,
+    CRT_ERR_BAND = 19,
+    CRT_ERR_GEOM = 26,
+    CRT_ERR_ZERODIAG = 27,
+    CRT_ERR_PIVOT = 28,
+    CRT_ERR_BETA = 35,
+    CRT_ERR_ELEC = 73,
+    CRT_ERR_ZEROVOLT = 82,
+    CRT_ERR_ALLOC = 97,
+    CRT_ERR_TYPE = 18,
+    CRT_ERR_HIP = 900,
+    CRT_ERR_STATE = 901,
+    CRT_ERR_ARG = 902,
+    CRT_ERR_UNSUPPORTED = 903,
+};
+
+#define CRT_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "crtomo_hip: %s failed at %s:%d: %s\n", #call, __FILE__, __LINE__, \
+                    hipGetErrorString(e_));                                                \
+            return CRT_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------- complex FP64
+struct zd {
+    double re, im;
+};
+
+__host__ __device__ __forceinline__ zd zmake(double r, double i) { return zd{r, i}; }
+__host__ __device__ __forceinline__ zd zadd(zd a, zd b) { return zd{a.re + b.re, a.im + b.im}; }
+__host__ __device__ __forceinline__ zd zsub(zd a, zd b) { return zd{a.re - b.re, a.im - b.im}; }
+__host__ __device__ __forceinline__ zd zmul(zd a, zd b) {
+    return zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ __forceinline__ zd zscale(zd a, double s) { return zd{a.re * s, a.im * s}; }
+// a - b*c
+__host__ __device__ __forceinline__ zd zfms(zd a, zd b, zd c) {
+    return zd{a.re - b.re * c.re + b.im * c.im, a.im - b.re * c.im - b.im * c.re};
+}
+// a + b*c
+__host__ __device__ __forceinline__ zd zfma(zd a, zd b, zd c) {
+    return zd{a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re};
+}
+__host__ __device__ __forceinline__ double zabs(zd a) { return hypot(a.re, a.im); }
+// 1/z
+__host__ __device__ __forceinline__ zd zinv(zd z) {
+    // Smith's algorithm for 1/(c+id)
+    if (fabs(z.re) >= fabs(z.im)) {
+        double r = z.im / z.re, den = z.re + z.im * r;
+        return zd{1.0 / den, -r / den};
+    } else {
+        double r = z.re / z.im, den = z.re * r + z.im;
+        return zd{r / den, -1.0 / den};
+    }
+}
+__host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
+    // Smith's algorithm for a/b
+    if (fabs(b.re) >= fabs(b.im)) {
+        double r = b.im / b.re, den = b.re + b.im * r;
+        return zd{(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+    } else {
+        double r = b.re / b.im, den = b.re * r + b.im;
+        return zd{(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+    }
+}
+// principal square root (branch cut on the negative real axis)
+__host__ __device__ __forceinline__ zd zsqrt(zd z) {
+    double r = hypot(z.re, z.im);
+    if (r == 0.0) return zd{0.0, z.im};
+    if (z.re >= 0.0) {
+        double t = sqrt(0.5 * (r + z.re));
+        return zd{t, z.im / (2.0 * t)};
+    }
+    double t = sqrt(0.5 * (r - z.re));
+    return zd{fabs(z.im) / (2.0 * t), copysign(t, z.im)};
+}
+
+// ---------------------------------------------------------------- geometry
+// Band tile used by the FP64 factorisation: rows are grouped into blocks of
+// CRT_NB rows; the factor is stored as block-column panels
+//   panel[J][a][r][c] = L(J*NB + a*NB + r, J*NB + c),  a = 0..D,
+// where D = ceil(mb / NB) is the number of sub-diagonal blocks.
+#ifndef CRT_NB
+#define CRT_NB 8
+#endif
+
+struct DevGrid {
+    int sanz, mb, elanz, relanz, eanz, maxn, npk, kwnanz, nanz;
+    int nblk, D;              // row blocks and sub-diagonal block count
+    // CSR of the lower triangle of A in node order (0-based), diagonal last
+    const int *row_ptr;       // sanz+1
+    const int *col;           // nnz
+    const int *cptr;          // nnz+1 contribution ranges
+    const int *contrib;       // >=0: e*npk+imn (area); <0: -(rel*npk+imn)-1 (mixed boundary)
+    const int *rnr;           // relanz (0-based area element)
+    const int *enr;           // eanz (0-based node)
+    const double *elbg;       // [k][imn][e]
+    const double *relbg;      // [imn][rel]
+    const double *kg;         // [k][l][rel]
+    const zd *sigma;          // elanz
+};

@@ -1,0 +1,210 @@
+// crt_volt_sens.hip -- inverse-Fourier back-transform, voltages and sensitivities.
+//
+// kpot is kept node-major on the device, kpotN[(node*K + k)*eanz + l], so the
+// values an element or an electrode needs are contiguous.
+//
+//  * hpot(j,l) = sum_k kpot(j,l,k) w_k / pi          (src/rtrafo.f90:73-88)
+//  * V = (hpot(e4,e2)-hpot(e4,e1)) - (hpot(e3,e2)-hpot(e3,e1)), pole
+//    electrodes (0) masked                          (src/bvolt.f90:44-63)
+//  * bvolt: sigmaa = kfak V; bvolti: sigmaa = -log V (src/bvolt.f90:77, src/bvolti.f90:80-88)
+//  * J(i,e) = [sum_k w_k/pi sum_{m,n} elbg(e,mn,k) (phiB-phiA)_n (phiN-phiM)_m] sigma_e / V_i
+//    (bsensi, src/bsensi.f90:98-187) or -(...) strom_i (bsens, src/bsens.f90:103-191)
+#include "crt_common.hpp"
+
+// hpotE[e][l]: back-transformed potential at electrode node e for source l
+__global__ void __launch_bounds__(256) crt_hpot_elec_kernel(int eanz, int K, const int *__restrict__ enr,
+                                                            const double *__restrict__ kwnwi,
+                                                            const zd *__restrict__ kpotN,
+                                                            zd *__restrict__ hpotE) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= eanz * eanz) return;
+    int l = t % eanz, e = t / eanz;
+    const double pi = acos(-1.0);
+    const zd *p = kpotN + (size_t)enr[e] * K * eanz + l;
+    zd s = zd{0.0, 0.0};
+    for (int k = 0; k < K; ++k) {
+        zd v = p[(size_t)k * eanz];
+        s.re = s.re + v.re * kwnwi[k];
+        s.im = s.im + v.im * kwnwi[k];
+    }
+    hpotE[t] = zd{s.re / pi, s.im / pi};
+}
+
+// full hpot in the reference layout hpot(sanz, eanz): out[l*sanz + j]
+__global__ void __launch_bounds__(256) crt_hpot_full_kernel(int sanz, int eanz, int K,
+                                                            const double *__restrict__ kwnwi,
+                                                            const zd *__restrict__ kpotN,
+                                                            zd *__restrict__ hpot) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)sanz * eanz) return;
+    int l = (int)(t % eanz), j = (int)(t / eanz);
+    const double pi = acos(-1.0);
+    const zd *p = kpotN + (size_t)j * K * eanz + l;
+    zd s = zd{0.0, 0.0};
+    for (int k = 0; k < K; ++k) {
+        zd v = p[(size_t)k * eanz];
+        s.re = s.re + v.re * kwnwi[k];
+        s.im = s.im + v.im * kwnwi[k];
+    }
+    hpot[(size_t)l * sanz + j] = zd{s.re / pi, s.im / pi};
+}
+
+// meas[i] = (e1, e2, e3, e4) 1-based electrode numbers (0 = pole / absent)
+__global__ void __launch_bounds__(256) crt_volt_kernel(int nanz, int eanz, int mode, const int4 *__restrict__ meas,
+                                                       const zd *__restrict__ hpotE,
+                                                       const double *__restrict__ kfak, zd *__restrict__ volt,
+                                                       zd *__restrict__ sigmaa, int *__restrict__ status) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nanz) return;
+    int4 m = meas[i];
+    auto hp = [&](int pot, int src) -> zd {
+        if (pot <= 0 || src <= 0) return zd{0.0, 0.0};
+        return hpotE[(size_t)(pot - 1) * eanz + (src - 1)];
+    };
+    zd d1 = hp(m.w, m.y), d2 = hp(m.w, m.x), d3 = hp(m.z, m.y), d4 = hp(m.z, m.x);
+    zd v = zsub(zsub(d1, d2), zsub(d3, d4));
+    volt[i] = v;
+    if (mode == 1) {
+        double a = zabs(v);
+        if (a == 0.0) {
+            *status = CRT_ERR_ZEROVOLT;
+            sigmaa[i] = zd{0.0, 0.0};
+        } else {
+            sigmaa[i] = zd{-log(a), -atan2(v.im, v.re)};
+        }
+    } else {
+        double f = kfak ? kfak[i] : 1.0;
+        sigmaa[i] = zd{f * v.re, f * v.im};
+    }
+}
+
+// One workgroup per area element; threads stride over measurements.
+// LDS: potentials of the element's nodes for all (k, l), and its elbg(., k).
+struct SensArgs {
+    int elanz, nanz, eanz, K, maxn, npk, mode;  // mode 1: bsensi, 0: bsens
+    const int *nrel;       // (neltot, maxn) 1-based
+    const int *nkel;       // nodes per area element
+    const int4 *meas;      // 1-based electrodes
+    const double *kwnwi;
+    const zd *kpotN;
+    const double *elbg;    // [k][imn][e]
+    const zd *sigma;
+    const zd *volt;
+    const double *strom;
+    zd *sens;              // (nanz, manz) column-major
+};
+
+__global__ void __launch_bounds__(256) crt_sens_kernel(SensArgs A) {
+    extern __shared__ double smem_v[];
+    const int e = blockIdx.x;
+    if (e >= A.elanz) return;
+    const int K = A.K, E = A.eanz, nk = A.nkel[e];
+    zd *pot = reinterpret_cast<zd *>(smem_v);                    // [m][k][l]
+    double *eb = reinterpret_cast<double *>(pot + (size_t)nk * K * E);  // [k][imn]
+    const int per_node = K * E;
+    for (int q = threadIdx.x; q < nk * per_node; q += blockDim.x) {
+        int m = q / per_node, r = q - m * per_node;
+        int node = A.nrel[(size_t)e * A.maxn + m] - 1;
+        pot[q] = A.kpotN[(size_t)node * per_node + r];
+    }
+    const int np = nk * (nk + 1) / 2;
+    for (int q = threadIdx.x; q < K * np; q += blockDim.x) {
+        int k = q / np, imn = q - k * np;
+        eb[q] = A.elbg[((size_t)k * A.npk + imn) * A.elanz + e];
+    }
+    __syncthreads();
+    const double pi = acos(-1.0);
+    for (int i = threadIdx.x; i < A.nanz; i += blockDim.x) {
+        int4 m = A.meas[i];  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
+        zd dum = zd{0.0, 0.0};
+        for (int k = 0; k < K; ++k) {
+            zd h = zd{0.0, 0.0};
+            const zd *pk = pot + (size_t)k * E;
+            for (int mi = 0; mi < nk; ++mi) {
+                const zd *pmi = pk + (size_t)mi * per_node;
+                zd s3 = m.z > 0 ? pmi[m.z - 1] : zd{0.0, 0.0};
+                zd s4 = m.w > 0 ? pmi[m.w - 1] : zd{0.0, 0.0};
+                zd dm = zsub(s4, s3);
+                for (int mj = 0; mj < nk; ++mj) {
+                    const zd *pmj = pk + (size_t)mj * per_node;
+                    zd s1 = m.x > 0 ? pmj[m.x - 1] : zd{0.0, 0.0};
+                    zd s2 = m.y > 0 ? pmj[m.y - 1] : zd{0.0, 0.0};
+                    int ia = mi > mj ? mi : mj, ib = mi > mj ? mj : mi;
+                    double w = eb[k * np + ia * (ia + 1) / 2 + ib];
+                    zd d = zmul(zsub(s2, s1), dm);
+                    h.re = h.re + w * d.re;
+                    h.im = h.im + w * d.im;
+                }
+            }
+            dum.re = dum.re + h.re * A.kwnwi[k];
+            dum.im = dum.im + h.im * A.kwnwi[k];
+        }
+        zd out;
+        if (A.mode == 1) {
+            zd t = zd{dum.re / pi, dum.im / pi};
+            out = zdiv(zmul(t, A.sigma[e]), A.volt[i]);
+        } else {
+            double f = A.strom ? A.strom[i] / pi : 1.0 / pi;
+            out = zd{-(dum.re * f), -(dum.im * f)};
+        }
+        A.sens[(size_t)e * A.nanz + i] = out;
+    }
+}
+
+// kpot in the reference layout kpot(sanz, eanz, kwnanz): out[(k*eanz + l)*sanz + j]
+__global__ void __launch_bounds__(256) crt_kpot_ref_kernel(int sanz, int eanz, int K, const zd *__restrict__ kpotN,
+                                                           zd *__restrict__ out) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)sanz * eanz * K) return;
+    int j = (int)(t % sanz);
+    long r = t / sanz;
+    int l = (int)(r % eanz), k = (int)(r / eanz);
+    out[t] = kpotN[((size_t)j * K + k) * eanz + l];
+}
+
+int crt_launch_hpot_elec(hipStream_t s, int eanz, int K, const int *enr, const double *kwnwi, const zd *kpotN,
+                         zd *hpotE) {
+    int n = eanz * eanz;
+    hipLaunchKernelGGL(crt_hpot_elec_kernel, dim3((n + 255) / 256), dim3(256), 0, s, eanz, K, enr, kwnwi, kpotN, hpotE);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_hpot_full(hipStream_t s, int sanz, int eanz, int K, const double *kwnwi, const zd *kpotN, zd *hpot) {
+    long n = (long)sanz * eanz;
+    hipLaunchKernelGGL(crt_hpot_full_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sanz, eanz, K, kwnwi,
+                       kpotN, hpot);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_volt(hipStream_t s, int nanz, int eanz, int mode, const int4 *meas, const zd *hpotE,
+                    const double *kfak, zd *volt, zd *sigmaa, int *status) {
+    hipLaunchKernelGGL(crt_volt_kernel, dim3((nanz + 255) / 256), dim3(256), 0, s, nanz, eanz, mode, meas, hpotE,
+                       kfak, volt, sigmaa, status);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+size_t crt_sens_lds_bytes(int maxn, int K, int eanz) {
+    return (size_t)maxn * K * eanz * sizeof(zd) + (size_t)K * 10 * sizeof(double);
+}
+
+int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int maxn, int npk, int mode,
+                    const int *nrel, const int *nkel, const int4 *meas, const double *kwnwi, const zd *kpotN,
+                    const double *elbg, const zd *sigma, const zd *volt, const double *strom, zd *sens) {
+    SensArgs a{elanz, nanz, eanz, K, maxn, npk, mode, nrel, nkel, meas, kwnwi, kpotN, elbg, sigma, volt, strom, sens};
+    size_t lds = crt_sens_lds_bytes(maxn, K, eanz);
+    if (lds > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)crt_sens_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(crt_sens_kernel, dim3(elanz), dim3(256), lds, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_kpot_ref(hipStream_t s, int sanz, int eanz, int K, const zd *kpotN, zd *out) {
+    long n = (long)sanz * eanz * K;
+    hipLaunchKernelGGL(crt_kpot_ref_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sanz, eanz, K, kpotN,
+                       out);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

@@ -1,0 +1,223 @@
+"""ctypes binding of the HIP library (``_lib/libcrtomo_hip.so``, C-ABI in
+``include/crtomo_hip.h``).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly if
+the shared object has not been built (``python -m crtomo_stable_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcrtomo_hip.so")
+
+EXPORTS = (
+    "crt_create", "crt_destroy", "crt_version", "crt_set_grid", "crt_set_survey", "crt_rwaven",
+    "crt_set_wavenumbers", "crt_precal", "crt_get_precal", "crt_set_sigma", "crt_forward", "crt_get_kpot",
+    "crt_voltages", "crt_get_hpot", "crt_sens", "crt_get_sens", "crt_cutmck", "crt_chol", "crt_vre",
+    "crt_cholz", "crt_get_times", "crt_set_sens", "crt_set_regularization", "crt_cjg", "crt_update",
+)
+
+ERRORS = {
+    18: "unknown element type", 19: "band width exceeded", 26: "element geometry (det <= 0)",
+    27: "zero diagonal", 28: "non-positive pivot in Cholesky", 35: "boundary geometry in beta",
+    73: "duplicate electrodes", 82: "zero voltage", 97: "allocation failed", 900: "HIP runtime error",
+    901: "call order (missing set-up call)", 902: "invalid argument", 903: "unsupported configuration",
+}
+
+_lib = None
+_p = C.c_void_p
+
+
+class CRTError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        super().__init__(f"{where}: errnr {code} ({ERRORS.get(code, 'unknown')})")
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python -m crtomo_stable_amd.build)")
+        lib = C.CDLL(LIB_PATH)
+        lib.crt_version.restype = C.c_char_p
+        lib.crt_destroy.restype = None
+        lib.crt_destroy.argtypes = [_p]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+def _chk(code: int, where: str) -> None:
+    if code != 0:
+        raise CRTError(code, where)
+
+
+def cutmck(grid):
+    """Cuthill-McKee renumbering on the host (``cutmckee/cutmck.f:126-337``)."""
+    lib = load()
+    perm = np.zeros(grid.sanz, np.int32)
+    mb = C.c_int()
+    _chk(lib.crt_cutmck(grid.sanz, grid.typanz, _ptr(grid.nelanz), _ptr(grid.selanz), grid.maxn,
+                        _ptr(grid.nrel), grid.elanz, _ptr(perm), C.byref(mb)), "crt_cutmck")
+    return perm, mb.value
+
+
+class Context:
+    """One device context holding grid, survey, element matrices, kpot and J."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = _p()
+        _chk(lib.crt_create(C.byref(h), device), "crt_create")
+        self._h = h
+        self.lib = lib
+        self.grid = None
+        self.survey = None
+        self.kwnanz = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.crt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- set-up
+    def set_grid(self, g):
+        g.validate()
+        rnr = g.rnr if g.rnr.size else np.zeros(1, np.int32)
+        _chk(self.lib.crt_set_grid(self._h, g.sanz, g.mb, g.typanz, _ptr(g.typ), _ptr(g.nelanz), _ptr(g.selanz),
+                                   g.maxn, _ptr(g.nrel), _ptr(g.x), _ptr(g.y), _ptr(rnr), g.eanz, _ptr(g.enr)),
+             "crt_set_grid")
+        self.grid = g
+
+    def set_survey(self, s):
+        _chk(self.lib.crt_set_survey(self._h, s.nanz, _ptr(s.strnr), _ptr(s.vnr)), "crt_set_survey")
+        self.survey = s
+
+    def rwaven(self):
+        kwn = np.zeros(30); kwi = np.zeros(30); n = C.c_int(); amin = C.c_double(); amax = C.c_double()
+        _chk(self.lib.crt_rwaven(self._h, C.byref(n), _ptr(kwn), _ptr(kwi), C.byref(amin), C.byref(amax)),
+             "crt_rwaven")
+        self.kwnanz = n.value
+        return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
+
+    def set_wavenumbers(self, kwn, kwnwi):
+        kwn = np.ascontiguousarray(kwn, np.float64); kwi = np.ascontiguousarray(kwnwi, np.float64)
+        _chk(self.lib.crt_set_wavenumbers(self._h, kwn.size, _ptr(kwn), _ptr(kwi)), "crt_set_wavenumbers")
+        self.kwnanz = kwn.size
+
+    def precal(self, lsytop: bool = False) -> bool:
+        lb = C.c_int()
+        _chk(self.lib.crt_precal(self._h, int(lsytop), C.byref(lb)), "crt_precal")
+        return bool(lb.value)
+
+    def get_precal(self):
+        g = self.grid
+        K = self.kwnanz
+        elbg = np.zeros((K, g.npk, g.elanz)); relbg = np.zeros((g.npk, max(g.relanz, 1)))
+        kg = np.zeros((K, g.eanz, max(g.relanz, 1)))
+        _chk(self.lib.crt_get_precal(self._h, _ptr(elbg), _ptr(relbg), _ptr(kg)), "crt_get_precal")
+        return elbg, relbg, kg
+
+    def set_sigma(self, sigma):
+        s = np.ascontiguousarray(sigma, np.complex128)
+        assert s.size == self.grid.elanz
+        _chk(self.lib.crt_set_sigma(self._h, _ptr(s)), "crt_set_sigma")
+
+    # --------------------------------------------------------------- compute
+    def forward(self):
+        _chk(self.lib.crt_forward(self._h), "crt_forward")
+
+    def get_kpot(self):
+        """kpot as (kwnanz, eanz, sanz) = reference kpot(sanz, eanz, kwnanz)."""
+        g = self.grid
+        out = np.zeros((self.kwnanz, g.eanz, g.sanz), np.complex128)
+        _chk(self.lib.crt_get_kpot(self._h, _ptr(out)), "crt_get_kpot")
+        return out
+
+    def voltages(self, mode: int = 1, kfak=None):
+        n = self.survey.nanz
+        volt = np.zeros(n, np.complex128); sa = np.zeros(n, np.complex128)
+        kf = None if kfak is None else np.ascontiguousarray(kfak, np.float64)
+        _chk(self.lib.crt_voltages(self._h, mode, _ptr(kf), _ptr(volt), _ptr(sa)), "crt_voltages")
+        return volt, sa
+
+    def get_hpot(self):
+        g = self.grid
+        out = np.zeros((g.eanz, g.sanz), np.complex128)
+        _chk(self.lib.crt_get_hpot(self._h, _ptr(out)), "crt_get_hpot")
+        return out
+
+    def sens(self, mode: int = 1, strom=None):
+        st = None if strom is None else np.ascontiguousarray(strom, np.float64)
+        _chk(self.lib.crt_sens(self._h, mode, _ptr(st)), "crt_sens")
+
+    def get_sens(self):
+        """J as (manz, nanz) = reference sens(nanz, manz)."""
+        out = np.zeros((self.grid.elanz, self.survey.nanz), np.complex128)
+        _chk(self.lib.crt_get_sens(self._h, _ptr(out)), "crt_get_sens")
+        return out
+
+    # ------------------------------------------------------- normal equations
+    def set_sens(self, sens):
+        """sens as (manz, nanz) = reference sens(nanz, manz)."""
+        s = np.ascontiguousarray(sens, np.complex128)
+        manz, nanz = s.shape
+        _chk(self.lib.crt_set_sens(self._h, nanz, manz, _ptr(s)), "crt_set_sens")
+
+    def set_regularization(self, ltri, smatm, nx=0, nachbar=None):
+        """smatm as (ncol, manz) = reference smatm(manz, ncol); nachbar (maxn+1, manz)."""
+        sm = np.ascontiguousarray(smatm, np.float64)
+        manz = sm.shape[1]
+        nb = None if nachbar is None else np.ascontiguousarray(nachbar, np.int32)
+        maxn = 0 if nb is None else nb.shape[0] - 1
+        _chk(self.lib.crt_set_regularization(self._h, int(ltri), int(nx), manz, maxn, _ptr(sm), _ptr(nb)),
+             "crt_set_regularization")
+        self.manz = manz
+
+    def cjg(self, lam, eps, ncgmax, wmatd, wdfak, cgfac, bvec):
+        manz = np.asarray(cgfac).size
+        w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)
+        cf = np.ascontiguousarray(cgfac, np.float64); bv = np.ascontiguousarray(bvec, np.complex128)
+        dpar = np.zeros(manz, np.complex128); cgres = np.zeros(ncgmax + 1); ncg = C.c_int()
+        _chk(self.lib.crt_cjg(self._h, C.c_double(lam), C.c_double(eps), int(ncgmax), _ptr(w), _ptr(wd), _ptr(cf),
+                              _ptr(bv), _ptr(dpar), _ptr(cgres), C.byref(ncg)), "crt_cjg")
+        return dpar, cgres, ncg.value
+
+    def update(self, lam, eps, ncgmax, wmatd, wdfak, dat, par, sigmaa=None, step=1.0):
+        par = np.array(par, np.complex128)
+        manz = par.size
+        w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)
+        d = np.ascontiguousarray(dat, np.complex128)
+        sa = None if sigmaa is None else np.ascontiguousarray(sigmaa, np.complex128)
+        dpar = np.zeros(manz, np.complex128); cgres = np.zeros(ncgmax + 1); ncg = C.c_int()
+        cgfac = np.zeros(manz); bvec = np.zeros(manz, np.complex128)
+        _chk(self.lib.crt_update(self._h, C.c_double(lam), C.c_double(eps), int(ncgmax), C.c_double(step), _ptr(w),
+                                 _ptr(wd), _ptr(d), _ptr(sa), _ptr(par), _ptr(dpar), _ptr(cgres), C.byref(ncg),
+                                 _ptr(cgfac), _ptr(bvec)), "crt_update")
+        return dict(par=par, dpar=dpar, cgres=cgres, ncg=ncg.value, cgfac=cgfac, bvec=bvec)
+
+    def times(self):
+        ms = np.zeros(8)
+        self.lib.crt_get_times(self._h, _ptr(ms), 8)
+        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec"), ms))

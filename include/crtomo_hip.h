@@ -1,0 +1,120 @@
+/*
+ * crtomo_hip.h -- C-ABI of the MI355X (gfx950) CRTomo forward / Jacobian / CG library.
+ *
+ * Conventions (identical to the reference's Fortran module data, so a Fortran
+ * shim passes its arrays unchanged):
+ *   - node, element, electrode and model numbers are 1-based;
+ *   - nrel is (neltot x maxn) element-major: nrel[iel*maxn + m] (the transpose
+ *     of the Fortran nrel(neltot, maxn) array, which a shim passes as
+ *     TRANSPOSE(nrel) or builds element-major);
+ *   - x[j], y[j] are the coordinates of node j (sx(snr(j)), sy(snr(j)));
+ *   - complex arrays are interleaved (re, im) doubles, the memory layout of
+ *     Fortran COMPLEX*16;
+ *   - multi-dimensional outputs use the Fortran column-major layouts of the
+ *     reference modules: elbg(elanz,npk,kwnanz), kg(relanz,eanz,kwnanz),
+ *     kpot(sanz,eanz,kwnanz), hpot(sanz,eanz), sens(nanz,manz).
+ * Every function returns 0 or the reference's errnr code (src/error.txt):
+ *   19 band width exceeded, 26 element geometry, 27 zero diagonal,
+ *   28 non-positive pivot, 35 boundary geometry, 73 duplicate electrodes,
+ *   82 zero voltage, 97 allocation; 900+ are library codes (900 HIP error,
+ *   901 call order, 902 argument, 903 unsupported configuration).
+ * A context is bound to one device and one HIP stream; calls on one context
+ * must not overlap.
+ */
+#ifndef CRTOMO_HIP_H
+#define CRTOMO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct crt_ctx crt_ctx;
+
+/* Create / destroy a context on HIP device `device`. */
+int crt_create(crt_ctx **ctx, int device);
+void crt_destroy(crt_ctx *ctx);
+const char *crt_version(void);
+
+/* Grid (replaces the elemmod/electrmod state read by src/relem.f90:60-151 and
+ * src/relectr.f90).  Builds the assembly map used by the forward solve. */
+int crt_set_grid(crt_ctx *ctx, int sanz, int mb, int typanz, const int *typ, const int *nelanz,
+                 const int *selanz, int maxn, const int *nrel, const double *x, const double *y,
+                 const int *rnr, int eanz, const int *enr);
+
+/* Measurements (datmod strnr/vnr, src/rdatm.f90). */
+int crt_set_survey(crt_ctx *ctx, int nanz, const int *strnr, const int *vnr);
+
+/* Wavenumbers and weights (src/rwaven.f90:22).  kwn/kwnwi need room for 30. */
+int crt_rwaven(crt_ctx *ctx, int *kwnanz, double *kwn, double *kwnwi, double *amin, double *amax);
+/* Or set them explicitly (2D: kwnanz = 1, kwn = 0, swrtr = 0 in the reference). */
+int crt_set_wavenumbers(crt_ctx *ctx, int kwnanz, const double *kwn, const double *kwnwi);
+
+/* Element matrices and mixed-boundary factors on the device (src/precal.f90:13). */
+int crt_precal(crt_ctx *ctx, int lsytop, int *lbeta);
+int crt_get_precal(crt_ctx *ctx, double *elbg, double *relbg, double *kg);
+
+/* Complex conductivity per area element (sigmamod sigma). */
+int crt_set_sigma(crt_ctx *ctx, const double *sigma);
+
+/* The forward loop of src/fem.f90:328-397 / src/inv.f90:415-484 for every
+ * (electrode, wavenumber): kompab + scalab + chol + vre (or kompb + vre).
+ * kpot stays on the device. */
+int crt_forward(crt_ctx *ctx);
+int crt_get_kpot(crt_ctx *ctx, double *kpot);
+
+/* rtrafo + bvolt (mode 0, kfak may be NULL = 1) or rtrafo + bvolti (mode 1)
+ * (src/rtrafo.f90:13, src/bvolt.f90:7, src/bvolti.f90:8). */
+int crt_voltages(crt_ctx *ctx, int mode, const double *kfak, double *volt, double *sigmaa);
+int crt_get_hpot(crt_ctx *ctx, double *hpot);
+
+/* Sensitivities: bsensi (mode 1, src/bsensi.f90:23) or bsens (mode 0,
+ * src/bsens.f90:23, strom may be NULL = 1).  J stays on the device. */
+int crt_sens(crt_ctx *ctx, int mode, const double *strom);
+int crt_get_sens(crt_ctx *ctx, double *sens);
+
+/* Upload a Jacobian sens(nanz, manz) (column-major complex) instead of crt_sens. */
+int crt_set_sens(crt_ctx *ctx, int nanz, int manz, const double *sens);
+
+/* Regularisation matrix for the CG (src/bsmatm_mod.f90:190-460):
+ * ltri 0: smatm(manz,3) 5-point stencil of an nx-wide regular grid;
+ * ltri 1/2: smatm(manz,maxn+1) with nachbar(manz,maxn+1) (1-based neighbours). */
+int crt_set_regularization(crt_ctx *ctx, int ltri, int nx, int manz, int maxn, const double *smatm,
+                           const int *nachbar);
+
+/* cjg (src/cg_mod.f90:84, complex case -> cjggra :426): Fletcher-Reeves CG on
+ * C (J^H W J + lam R) C y = bvec with W = wmatd*wdfak; writes the raw CG
+ * solution dpar (before the cgfac scaling of src/update.f90:302), the residual
+ * history cgres(ncgmax+1) (cgres(1) = ncg) and ncg. */
+int crt_cjg(crt_ctx *ctx, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,
+            const double *cgfac, const double *bvec, double *dpar, double *cgres, int *ncg);
+
+/* update (src/update.f90:27, complex case, no prior / reference model): RHS and
+ * cgfac from J on the device, cjg, dpar <- cgfac*dpar, par += step*dpar.
+ * sigmaa = NULL uses the forward values of the last crt_voltages call.
+ * cgfac and bvec0 (the scaled RHS) may be NULL. */
+int crt_update(crt_ctx *ctx, double lam, double eps, int ncgmax, double step, const double *wmatd,
+               const int *wdfak, const double *dat, const double *sigmaa, double *par, double *dpar,
+               double *cgres, int *ncg, double *cgfac, double *bvec0);
+
+/* Cuthill-McKee renumbering (cutmckee/cutmck.f:126-337): perm[old-1] = new
+ * number; returns the band width in *mb. */
+int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel,
+               int elanz, int *perm, int *mb);
+
+/* Single-system drop-ins for the named Fortran routines, host arrays in/out:
+ * chol(a) (src/chol.f90:7), vre(a,b,pot) (src/vre.f90:5), cholz(a,p,n,...)
+ * (src/cholz.f90:1).  Band layout: entry (i,j), j<=i, 1-based, at
+ * a[i*mb + j - 1], length (mb+1)*sanz. */
+int crt_chol(crt_ctx *ctx, int sanz, int mb, double *a);
+int crt_vre(crt_ctx *ctx, int sanz, int mb, const double *a, const double *b, double *pot);
+int crt_cholz(crt_ctx *ctx, int n, double *a, double *p);
+
+/* Device time of the last call of each phase, milliseconds:
+ * [0] diag, [1] factor, [2] solve, [3] voltages, [4] sensitivities,
+ * [5] precal, [6] cg total, [7] cg matvec (mean per iteration). */
+int crt_get_times(crt_ctx *ctx, double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

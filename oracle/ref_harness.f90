@@ -185,6 +185,75 @@ contains
     ref_forward = errnr
   end function ref_forward
 
+  ! Bounded sample of the same OpenMP loop (k = 1..nk, l = 1..nl) for timing;
+  ! returns the wall time in seconds.
+  real(c_double) function ref_forward_sample(sigma_, nthreads, nk, nl) bind(C, name='ref_forward_sample')
+    complex(c_double_complex), intent(in) :: sigma_(*)
+    integer(c_int), value :: nthreads, nk, nl
+    complex(kind(0d0)), allocatable :: la(:), lb(:), lpot(:)
+    real(kind(0d0)), allocatable :: lfak(:)
+    integer :: j, l, k
+    integer(kind=8) :: c0, c1, rate
+    external kompab, scalab, chol, kompb, vre
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = sigma_(1:elanz)
+    if (allocated(kpot)) deallocate(kpot)
+    allocate(kpot(sanz, eanz, kwnanz))
+    allocate(la((mb + 1) * sanz), lb(sanz), lpot(sanz), lfak(sanz))
+    call omp_set_num_threads(nthreads)
+    call system_clock(c0, rate)
+    !$OMP PARALLEL DEFAULT (none) &
+    !$OMP FIRSTPRIVATE (lfak,lpot,la,lb) &
+    !$OMP PRIVATE (j,l,k) &
+    !$OMP SHARED (nk,nl,lbeta,sanz,kpot)
+    !$OMP DO
+    do k = 1, nk
+       do l = 1, nl
+          if (lbeta .or. l == 1) then
+             call kompab(l, k, la, lb)
+             call scalab(la, lb, lfak)
+             call chol(la)
+          else
+             call kompb(l, lb, lfak)
+          end if
+          call vre(la, lb, lpot)
+          do j = 1, sanz
+             kpot(j, l, k) = lpot(j) * dcmplx(lfak(j))
+          end do
+       end do
+    end do
+    !$OMP END PARALLEL
+    call system_clock(c1)
+    ref_forward_sample = dble(c1 - c0) / dble(rate)
+  end function ref_forward_sample
+
+  ! bsensi over the first nmeas measurements (OpenMP over measurements as
+  ! src/bsensi.f90:90-96); returns the wall time in seconds.
+  real(c_double) function ref_sens_sample(nthreads, nmeas) bind(C, name='ref_sens_sample')
+    integer(c_int), value :: nthreads, nmeas
+    integer :: j, nsave
+    integer(kind=8) :: c0, c1, rate
+    external bsensi
+    if (allocated(mnr)) deallocate(mnr)
+    allocate(mnr(elanz))
+    do j = 1, elanz
+       mnr(j) = j
+    end do
+    nsave = nanz
+    nanz = nmeas
+    ! timing only: a sampled forward leaves most voltages unset
+    where (abs(volt) == 0d0) volt = dcmplx(1d0, 0d0)
+    if (allocated(sens)) deallocate(sens)
+    allocate(sens(nanz, manz))
+    call omp_set_num_threads(nthreads)
+    call system_clock(c0, rate)
+    call bsensi(.false.)
+    call system_clock(c1)
+    nanz = nsave
+    ref_sens_sample = dble(c1 - c0) / dble(rate)
+  end function ref_sens_sample
+
   integer(c_int) function ref_voltages(complex_inv, hpot_, volt_, sigmaa_) bind(C, name='ref_voltages')
     integer(c_int), value :: complex_inv
     complex(c_double_complex), intent(out) :: hpot_(*), volt_(*), sigmaa_(*)

@@ -1,0 +1,127 @@
+"""GPU parity of the Gauss-Newton normal-equation step (update RHS, cgfac, cjg CG
+iterations, dpar scaling) against the oracle restatement of src/update.f90 and
+src/cg_mod.f90.
+
+Finite-precision CG amplifies summation-order rounding along its trajectory: a
+numpy CG with a different summation order differs from the oracle (and the oracle
+from the reference build, which it matches bit for bit) by ~3e-6 after 19
+iterations on these systems (condition number ~1e3).  Parity is therefore pinned
+where it is well posed:
+  * cgfac and the scaled RHS: 1e-12 relative;
+  * the first iterations (3): dpar 1e-10 relative;
+  * the converged solve (eps = 1e-30): dpar 1e-10 relative, also against a dense
+    direct solve of the scaled normal equations;
+  * the reference's default stop (eps = 1e-4, ncgmax = manz/10) and eps = 1e-12:
+    ncg within 3 % (at least 3 iterations), dpar within the accuracy of a stop at
+    that residual (1e-2 and 1e-4 relative), cgres[0] = ncg, history within REAL*4
+    rounding for the first iterations.
+"""
+import numpy as np
+import pytest
+
+from crtomo_stable_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(O, w, ltri, seed=0):
+    g, s = w.grid, w.survey
+    kwn, kwi, _, _ = O.rwaven(g, s)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    kpot = O.forward(g, w.sigma, elbg, relbg, kg, lb)
+    hpot = O.rtrafo(kpot, kwi)
+    volt, sa = O.bvolti(g, s, hpot)
+    sens = O.bsensi(g, s, kwi, kpot, elbg, w.sigma, volt)
+    rng = np.random.default_rng(seed)
+    dat = sa + 0.02 * (rng.standard_normal(sa.size) + 1j * 0.1 * rng.standard_normal(sa.size))
+    wmatd = 1.0 / (0.03 + 0.01 * rng.random(sa.size)) ** 2
+    wdfak = np.ones(sa.size, np.int32)
+    par = np.log(w.sigma) + 0.05 * rng.standard_normal(w.sigma.size)
+    if ltri == 0:
+        smatm = O.bsmatmreg(g, g.nx, g.nz, 1.0, 1.0)
+        nachbar = None
+    else:
+        nachbar = O.bnachbar(g)
+        smatm = O.bsmatmtri(g, nachbar, 1.0, 1.0)
+    return dict(sens=sens, sa=sa, dat=dat, wmatd=wmatd, wdfak=wdfak, par=par, smatm=smatm, nachbar=nachbar)
+
+
+def _oracle_update(O, w, P, ltri, lam, eps, ncgmax):
+    cp = O.CGProblem(P["sens"], P["wmatd"], P["wdfak"], P["dat"], P["sa"], P["par"], P["smatm"], ltri,
+                     w.grid.nx, lam, eps, ncgmax, nachbar=P["nachbar"], maxn=w.grid.maxn)
+    cgfac, bvec = O.update_rhs(cp)
+    dpar, cgres, ncg = O.cjggra(cp, cgfac, bvec)
+    return cp, cgfac, bvec, dpar * cgfac, cgres, ncg
+
+
+def _gpu_update(ctx, w, P, ltri, lam, eps, ncgmax):
+    ctx.set_grid(w.grid)
+    ctx.set_sens(P["sens"])
+    ctx.set_regularization(ltri, P["smatm"], nx=w.grid.nx, nachbar=P["nachbar"])
+    return ctx.update(lam, eps, ncgmax, P["wmatd"], P["wdfak"], P["dat"], P["par"], sigmaa=P["sa"])
+
+
+def _workload(ltri):
+    return configs.small_quad() if ltri == 0 else configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("ltri", [0, 1])
+@pytest.mark.parametrize("eps,ncgmax,lam", [(1e-30, 3, 3.0), (1e-30, 3000, 3.0)])
+def test_update_parity(ctx, oracle_lib, ltri, eps, ncgmax, lam):
+    O = oracle_lib
+    w = _workload(ltri)
+    print(f"[cg parity] ltri={ltri} eps={eps} ncgmax={ncgmax} manz={w.grid.elanz}", flush=True)
+    P = _problem(O, w, ltri)
+    cp, cgfac_o, bvec_o, dpar_o, cgres_o, ncg_o = _oracle_update(O, w, P, ltri, lam, eps, ncgmax)
+    r = _gpu_update(ctx, w, P, ltri, lam, eps, ncgmax)
+    assert np.max(np.abs(r["cgfac"] - cgfac_o) / cgfac_o) <= 1e-12
+    assert _rel(r["bvec"], bvec_o) <= 1e-12
+    assert _rel(r["dpar"], dpar_o) <= 1e-10
+    assert _rel(r["par"] - P["par"], dpar_o) <= 1e-10
+    assert abs(r["ncg"] - ncg_o) <= max(3, 0.03 * ncg_o)
+    assert r["cgres"][0] == float(r["ncg"])
+    if ncgmax > 100:
+        # the converged update against a dense solve of C (J^H W J + lam R) C y = bvec
+        manz = cgfac_o.size
+        B = np.zeros((manz, manz), complex)
+        for j in range(manz):
+            e = np.zeros(manz, complex); e[j] = 1.0
+            B[:, j] = O.cg_matvec(cp, cgfac_o, e)[1]
+        exact = np.linalg.solve(B, bvec_o) * cgfac_o
+        assert _rel(r["dpar"], exact) <= 1e-10
+
+
+@pytest.mark.parametrize("ltri", [0, 1])
+@pytest.mark.parametrize("eps,lam", [(1e-4, 10.0), (1e-12, 3.0)])
+def test_update_reference_stop(ctx, oracle_lib, ltri, eps, lam):
+    O = oracle_lib
+    w = _workload(ltri)
+    ncgmax = w.grid.elanz // 10 if eps == 1e-4 else 400  # src/rall.f90:735-738
+    P = _problem(O, w, ltri)
+    _, _, _, dpar_o, cgres_o, ncg_o = _oracle_update(O, w, P, ltri, lam, eps, ncgmax)
+    r = _gpu_update(ctx, w, P, ltri, lam, eps, ncgmax)
+    assert abs(r["ncg"] - ncg_o) <= max(3, 0.03 * ncg_o)
+    assert _rel(r["dpar"], dpar_o) <= (1e-2 if eps > 1e-8 else 1e-4)
+    assert np.allclose(r["cgres"][1:4], cgres_o[1:4], rtol=2e-6, atol=0)
+
+
+def test_cjg_dropin(ctx, oracle_lib):
+    """crt_cjg takes cgfac/bvec from the host exactly as cjg reads them from cjgmod."""
+    O = oracle_lib
+    w = configs.small_quad()
+    P = _problem(O, w, 0, seed=3)
+    lam, eps, ncgmax = 5.0, 1e-30, 2000
+    cp = O.CGProblem(P["sens"], P["wmatd"], P["wdfak"], P["dat"], P["sa"], P["par"], P["smatm"], 0,
+                     w.grid.nx, lam, eps, ncgmax)
+    cgfac, bvec = O.update_rhs(cp)
+    dpar_o, cgres_o, ncg_o = O.cjggra(cp, cgfac, bvec)
+    ctx.set_grid(w.grid)
+    ctx.set_sens(P["sens"])
+    ctx.set_regularization(0, P["smatm"], nx=w.grid.nx)
+    dpar, cgres, ncg = ctx.cjg(lam, eps, ncgmax, P["wmatd"], P["wdfak"], cgfac, bvec)
+    assert abs(ncg - ncg_o) <= max(3, 0.03 * ncg_o)
+    assert np.max(np.abs(dpar - dpar_o)) <= 1e-10 * np.max(np.abs(dpar_o))

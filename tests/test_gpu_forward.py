@@ -1,0 +1,105 @@
+"""GPU parity of the forward path (precal, assembly+scaling+band Cholesky+substitution,
+rtrafo+bvolt/bvolti, bsens/bsensi) against the CPU oracle on the same seeded inputs.
+
+Tolerance: FP64 path, relative error <= 1e-10 (voltages per datum, potentials and
+sensitivities relative to the largest entry of each source / measurement row).
+Index and bookkeeping outputs (wavenumber count, CutMcK, lbeta) are compared exactly.
+"""
+import numpy as np
+import pytest
+
+from crtomo_stable_amd import configs, lib
+
+TOL = 1e-10
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_rows(a, b):
+    """max over rows of max|a-b| / max|b| (rows = last-but-one axis flattened)."""
+    a2 = a.reshape(-1, a.shape[-1]); b2 = b.reshape(-1, b.shape[-1])
+    num = np.abs(a2 - b2).max(axis=1)
+    den = np.abs(b2).max(axis=1)
+    return float((num / np.where(den > 0, den, 1.0)).max())
+
+
+def _run_gpu(ctx, w, mode=1):
+    ctx.set_grid(w.grid)
+    ctx.set_survey(w.survey)
+    kwn, kwi, amin, amax = ctx.rwaven()
+    lbeta = ctx.precal()
+    ctx.set_sigma(w.sigma)
+    ctx.forward()
+    volt, sa = ctx.voltages(mode)
+    ctx.sens(mode)
+    return dict(kwn=kwn, kwi=kwi, lbeta=lbeta, volt=volt, sa=sa)
+
+
+def _oracle(O, w, mode=1, full=True):
+    kwn, kwi, amin, amax = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn)
+    kpot = O.forward(w.grid, w.sigma, elbg, relbg, kg, lb)
+    hpot = O.rtrafo(kpot, kwi)
+    if mode == 1:
+        volt, sa = O.bvolti(w.grid, w.survey, hpot)
+        sens = O.bsensi(w.grid, w.survey, kwi, kpot, elbg, w.sigma, volt)
+    else:
+        volt, sa = O.bvolt(w.grid, w.survey, hpot)
+        sens = O.bsens(w.grid, w.survey, kwi, kpot, elbg)
+    return dict(kwn=kwn, kwi=kwi, elbg=elbg, kg=kg, lbeta=lb, kpot=kpot, hpot=hpot, volt=volt, sa=sa, sens=sens)
+
+
+@pytest.mark.parametrize("name", ["small_quad", "c0", "small_tri"])
+@pytest.mark.parametrize("mode", [1, 0])
+def test_forward_parity(ctx, oracle_lib, name, mode):
+    O = oracle_lib
+    if name == "small_quad":
+        w = configs.small_quad()
+    elif name == "c0":
+        w = configs.c0()
+    else:
+        w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7)
+    print(f"[forward parity] {name} mode={mode} sanz={w.grid.sanz} mb={w.grid.mb}", flush=True)
+    g = _run_gpu(ctx, w, mode)
+    r = _oracle(O, w, mode)
+    assert np.array_equal(g["kwn"], r["kwn"]) or np.allclose(g["kwn"], r["kwn"], rtol=1e-14, atol=0)
+    assert g["lbeta"] == r["lbeta"]
+    elbg, relbg, kg = ctx.get_precal()
+    assert np.abs(elbg - r["elbg"]).max() <= 1e-12 * np.abs(r["elbg"]).max()
+    assert np.abs(kg - r["kg"]).max() <= 1e-12 * np.abs(r["kg"]).max()
+    kpot = ctx.get_kpot()
+    assert _rel_rows(kpot, r["kpot"]) <= TOL
+    hpot = ctx.get_hpot()
+    assert _rel_rows(hpot, r["hpot"]) <= TOL
+    assert float(np.max(np.abs(g["volt"] - r["volt"]) / np.abs(r["volt"]))) <= TOL
+    assert float(np.max(np.abs(g["sa"] - r["sa"]) / np.maximum(np.abs(r["sa"]), 1e-300))) <= TOL
+    sens = ctx.get_sens()
+    assert _rel_rows(sens.T, r["sens"].T) <= TOL
+
+
+def test_c1_sampled_systems(ctx, oracle_lib):
+    """Full C1 (20k triangles): kpot of a few (electrode, wavenumber) systems against
+    the oracle, plus reciprocity of the voltages."""
+    O = oracle_lib
+    w = configs.c1()
+    print(f"[c1] sanz={w.grid.sanz} mb={w.grid.mb} nanz={w.survey.nanz}", flush=True)
+    g = _run_gpu(ctx, w, 1)
+    kpot = ctx.get_kpot()
+    kwn, kwi, _, _ = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn)
+    K, E = kwn.size, w.grid.eanz
+    picks = [(1, 1), (E // 2, K // 2), (E, K)]
+    ref = O.forward(w.grid, w.sigma, elbg, relbg, kg, lb, systems=picks)
+    for l, k in picks:
+        a = kpot[k - 1, l - 1]; b = ref[k - 1, l - 1]
+        assert np.abs(a - b).max() / np.abs(b).max() <= TOL, (l, k)
+    # reciprocity V(AB, MN) ~ V(MN, AB): exact for the continuous problem; the
+    # source-dependent mixed boundary (kg(rel,l,k)) makes it approximate here
+    from crtomo_stable_amd import grid as G
+    q = w.survey.electrodes()[:50]
+    swapped = G.Survey(strnr=q[:, 3] * 10000 + q[:, 2], vnr=q[:, 1] * 10000 + q[:, 0])
+    ctx.set_survey(swapped)
+    v2, _ = ctx.voltages(0)
+    ctx.set_survey(G.Survey(strnr=q[:, 1] * 10000 + q[:, 0], vnr=q[:, 3] * 10000 + q[:, 2]))
+    v1, _ = ctx.voltages(0)
+    assert np.max(np.abs(v1 - v2) / np.abs(v1)) < 5e-2
