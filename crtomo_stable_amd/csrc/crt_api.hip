@@ -14,11 +14,14 @@ int crt_launch_precal(hipStream_t, int, int, int, int, int, int, const int *, co
                       const double *, const double *, const int *, const double *, double, double *, double *,
                       double *, int *);
 int crt_launch_diag(hipStream_t, const DevGrid &, int, const int2 *, double *, double *, int *);
-int crt_launch_factor(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *, zd *,
-                      int *);
+int crt_launch_assemble(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *);
+int crt_launch_factor(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, const zd *,
+                      zd *, zd *, int *);
 size_t crt_factor_lds_bytes(int D);
 int crt_launch_solve(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
-                     const zd *, const zd *, zd *, zd *);
+                     const zd *, const zd *, zd *, zd *, const zd *, zd *);
+int crt_launch_panel_band(hipStream_t, int, int, int, int, int, zd *, zd *);
+int crt_launch_factor_band(hipStream_t, const DevGrid &, const int2 *, const zd *, zd *, zd *, int *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
@@ -106,11 +109,12 @@ struct crt_ctx {
     std::vector<double> kwn, kwnwi;
     int lbeta = 0;
     // device state
-    DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_cptr, d_contrib, d_nkel, d_status;
+    DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_crow, d_cptr, d_contrib, d_nkel, d_status;
+    int nnz = 0;
     DBuf<int4> d_meas, d_rhs;
     DBuf<int2> d_sys;
     DBuf<double> d_x, d_y, d_kwn, d_kwnwi, d_elbg, d_relbg, d_kg, d_fak, d_dga, d_kfak, d_strom;
-    DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp;
+    DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp, d_vals;
     // normal-equation state
     int ltri = -1, rnx = 0, rncol = 0, sens_nanz = 0, sens_manz = 0;
     DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
@@ -123,8 +127,8 @@ struct crt_ctx {
     DevGrid devgrid() const {
         DevGrid g;
         g.sanz = sanz; g.mb = mb; g.elanz = elanz; g.relanz = relanz; g.eanz = eanz; g.maxn = maxn;
-        g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D;
-        g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
+        g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D; g.nnz = nnz;
+        g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.crow = d_crow.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
         g.rnr = d_rnr.p; g.enr = d_enr.p; g.elbg = d_elbg.p; g.relbg = d_relbg.p; g.kg = d_kg.p;
         g.sigma = d_sigma.p;
         return g;
@@ -341,12 +345,13 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
         return a.i != b.i ? a.i < b.i : a.j < b.j;
     });
-    std::vector<int> row_ptr(sanz + 1, 0), col, cptr, contrib;
-    col.reserve(ents.size()); cptr.reserve(ents.size() + 1); contrib.reserve(ents.size());
+    std::vector<int> row_ptr(sanz + 1, 0), col, crow, cptr, contrib;
+    col.reserve(ents.size()); crow.reserve(ents.size()); cptr.reserve(ents.size() + 1); contrib.reserve(ents.size());
     for (size_t q = 0; q < ents.size();) {
         size_t q1 = q;
         while (q1 < ents.size() && ents[q1].i == ents[q].i && ents[q1].j == ents[q].j) ++q1;
         col.push_back(ents[q].j);
+        crow.push_back(ents[q].i);
         cptr.push_back((int)contrib.size());
         for (size_t r = q; r < q1; ++r)
             if (ents[r].code != INT32_MIN) contrib.push_back(ents[r].code);
@@ -373,6 +378,8 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     e = e ? e : c->d_y.upload(c->y.data(), sanz, s);
     e = e ? e : c->d_row_ptr.upload(row_ptr.data(), row_ptr.size(), s);
     e = e ? e : c->d_col.upload(col.data(), col.size(), s);
+    e = e ? e : c->d_crow.upload(crow.data(), crow.size(), s);
+    c->nnz = (int)col.size();
     e = e ? e : c->d_cptr.upload(cptr.data(), cptr.size(), s);
     e = e ? e : c->d_contrib.upload(contrib.data(), contrib.size(), s);
     if (e) return e;
@@ -547,12 +554,14 @@ int crt_forward(crt_ctx *c) {
         }
     }
     const size_t panel_elems = (size_t)nblk * (D + 1) * CRT_NB * CRT_NB;
-    size_t per_sys = panel_elems * sizeof(zd) + (size_t)nblk * CRT_NB * (sizeof(zd) + 2 * sizeof(double));
+    size_t per_sys = panel_elems * sizeof(zd) + (size_t)nblk * CRT_NB * (sizeof(zd) + 2 * sizeof(double)) +
+                     (size_t)c->nnz * sizeof(zd);
     size_t batch = std::max<size_t>(1, std::min(sys.size(), c->panel_budget / per_sys));
     int e = c->d_panels.ensure(batch * panel_elems);
     e = e ? e : c->d_yf.ensure(batch * nblk * CRT_NB);
     e = e ? e : c->d_fak.ensure(batch * nblk * CRT_NB);
     e = e ? e : c->d_dga.ensure(batch * nblk * CRT_NB);
+    e = e ? e : c->d_vals.ensure(batch * c->nnz);
     e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
     e = e ? e : c->d_sys.ensure(sys.size());
     e = e ? e : c->d_rhs.ensure(rhs.size());
@@ -574,15 +583,18 @@ int crt_forward(crt_ctx *c) {
         c->tic(T_DIAG);
         if ((e = crt_launch_diag(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_status.p + 1 + s0)))
             return e;
+        if ((e = crt_launch_assemble(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p)))
+            return e;
         c->toc(T_DIAG);
         c->tic(T_FACTOR);
-        if ((e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_panels.p,
-                                   c->d_yf.p, c->d_status.p + 1 + s0)))
+        if ((e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
+                                   c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0)))
             return e;
         c->toc(T_FACTOR);
         c->tic(T_SOLVE);
         if ((e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                  c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p)))
+                                  c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr,
+                                  nullptr)))
             return e;
         c->toc(T_SOLVE);
         CRT_HIP(hipStreamSynchronize(c->stream));
@@ -992,8 +1004,95 @@ int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int m
     return 0;
 }
 
-int crt_chol(crt_ctx *, int, int, double *) { return CRT_ERR_UNSUPPORTED; }
-int crt_vre(crt_ctx *, int, int, const double *, const double *, double *) { return CRT_ERR_UNSUPPORTED; }
-int crt_cholz(crt_ctx *, int, double *, double *) { return CRT_ERR_UNSUPPORTED; }
+// ------------------------------------------------------ single-system drop-ins
+static DevGrid band_grid(int sanz, int mb) {
+    DevGrid g{};
+    g.sanz = sanz;
+    g.mb = mb;
+    g.nblk = (sanz + CRT_NB - 1) / CRT_NB;
+    g.D = std::max(1, (mb + CRT_NB - 1) / CRT_NB);
+    return g;
+}
+
+// chol(a): in-place complex-symmetric band Cholesky of a scaled band matrix in
+// the reference layout (src/chol.f90:46-78); errnr 28 on a vanishing pivot.
+int crt_chol(crt_ctx *c, int sanz, int mb, double *a) {
+    if (!c || !a || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    DevGrid g = band_grid(sanz, mb);
+    if (crt_factor_lds_bytes(g.D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    size_t nband = (size_t)sanz * (mb + 1), npanel = (size_t)g.nblk * (g.D + 1) * CRT_NB * CRT_NB;
+    int e = c->d_tmp.upload(reinterpret_cast<const zd *>(a), nband, c->stream);
+    e = e ? e : c->d_panels.ensure(npanel);
+    e = e ? e : c->d_yf.ensure((size_t)g.nblk * CRT_NB);
+    e = e ? e : c->d_sys.ensure(1);
+    e = e ? e : c->d_status.ensure(2);
+    if (e) return e;
+    int2 s0{0, 0};
+    CRT_HIP(hipMemcpyAsync(c->d_sys.p, &s0, sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipMemsetAsync(c->d_status.p, 0, 2 * sizeof(int), c->stream));
+    c->tic(T_FACTOR);
+    if ((e = crt_launch_factor_band(c->stream, g, c->d_sys.p, c->d_tmp.p, c->d_panels.p, c->d_yf.p,
+                                    c->d_status.p + 1)))
+        return e;
+    c->toc(T_FACTOR);
+    if ((e = crt_launch_panel_band(c->stream, sanz, mb, g.nblk, g.D, 1, c->d_panels.p, c->d_tmp.p))) return e;
+    CRT_HIP(hipMemcpyAsync(a, c->d_tmp.p, nband * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    int st[2] = {0, 0};
+    CRT_HIP(hipMemcpyAsync(st, c->d_status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_FACTOR);
+    c->have_kpot = false;  // the forward workspace was reused
+    return std::max(st[0], st[1]);
+}
+
+// vre(a, b, pot): pot = -A^{-1} b from the band factor (src/vre.f90:45-69).
+int crt_vre(crt_ctx *c, int sanz, int mb, const double *a, const double *b, double *pot) {
+    if (!c || !a || !b || !pot || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    DevGrid g = band_grid(sanz, mb);
+    size_t nband = (size_t)sanz * (mb + 1), npanel = (size_t)g.nblk * (g.D + 1) * CRT_NB * CRT_NB;
+    int e = c->d_tmp.upload(reinterpret_cast<const zd *>(a), nband, c->stream);
+    e = e ? e : c->d_panels.ensure(npanel);
+    e = e ? e : c->d_yscr.ensure((size_t)g.nblk * CRT_NB);
+    e = e ? e : c->d_rhs.ensure(1);
+    e = e ? e : c->d_res.upload(reinterpret_cast<const zd *>(b), sanz, c->stream);
+    e = e ? e : c->d_rm.ensure(sanz);
+    if (e) return e;
+    if ((e = crt_launch_panel_band(c->stream, sanz, mb, g.nblk, g.D, 0, c->d_panels.p, c->d_tmp.p))) return e;
+    int4 r0{0, 0, 0, 0};
+    CRT_HIP(hipMemcpyAsync(c->d_rhs.p, &r0, sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    c->tic(T_SOLVE);
+    if ((e = crt_launch_solve(c->stream, sanz, g.nblk, g.D, 1, 1, 1, c->d_rhs.p, nullptr, nullptr, c->d_panels.p,
+                              nullptr, c->d_yscr.p, nullptr, c->d_res.p, c->d_rm.p)))
+        return e;
+    c->toc(T_SOLVE);
+    CRT_HIP(hipMemcpyAsync(pot, c->d_rm.p, sanz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_SOLVE);
+    c->have_kpot = false;
+    return 0;
+}
+
+// cholz(a, p, n): dense complex-symmetric Cholesky, strict lower factor into
+// a(j,i) j>i and the diagonal into p (src/cholz.f90:43-76).  Runs as a band
+// factorisation with mb = n-1.
+int crt_cholz(crt_ctx *c, int n, double *a, double *p) {
+    if (!c || !a || !p || n <= 0) return CRT_ERR_ARG;
+    const int mb = std::max(n - 1, 0);
+    std::vector<zd> band((size_t)n * (mb + 1), zd{0.0, 0.0});
+    const zd *A = reinterpret_cast<const zd *>(a);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) band[(size_t)i * (mb + 1) + (j - i + mb)] = A[(size_t)i * n + j];  // a(j,i) = a(i,j)
+    int e = crt_chol(c, n, mb, reinterpret_cast<double *>(band.data()));
+    if (e) return e;
+    zd *Aw = reinterpret_cast<zd *>(a);
+    zd *P = reinterpret_cast<zd *>(p);
+    for (int i = 0; i < n; ++i) {
+        P[i] = band[(size_t)i * (mb + 1) + mb];
+        for (int j = i + 1; j < n; ++j) Aw[(size_t)i * n + j] = band[(size_t)j * (mb + 1) + (i - j + mb)];  // a(j,i) = L(j,i)
+    }
+    return 0;
+}
 
 }  // extern "C"

@@ -80,6 +80,30 @@ __global__ void __launch_bounds__(256) crt_diag_kernel(DevGrid g, int nsys, cons
     dga[o] = dum;
 }
 
+// ----------------------------------------------------------------- assemble
+// Scaled lower-triangle nonzeros of every system, one thread per (system, entry):
+// a_ii/|a_ii| on the diagonal, a_ij fak_i fak_j elsewhere (src/scalab.f90:52-76).
+__global__ void __launch_bounds__(256) crt_assemble_kernel(DevGrid g, int nsys, const int2 *__restrict__ sys,
+                                                           const double *__restrict__ fak,
+                                                           const double *__restrict__ dga, zd *__restrict__ vals) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nsys * g.nnz) return;
+    int s = (int)(t / g.nnz), q = (int)(t - (long)s * g.nnz);
+    int2 lk = sys[s];
+    int i = g.crow[q], j = g.col[q];
+    zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
+    const double *fs = fak + (size_t)s * g.nblk * NB;
+    zd v;
+    if (j == i) {
+        double dum = dga[(size_t)s * g.nblk * NB + i];
+        v = zd{a.re / dum, a.im / dum};
+    } else {
+        double f = fs[i] * fs[j];
+        v = zd{a.re * f, a.im * f};
+    }
+    vals[t] = v;
+}
+
 // --------------------------------------------------------------- factorise
 struct FactorArgs {
     DevGrid g;
@@ -87,9 +111,11 @@ struct FactorArgs {
     const int2 *sys;    // (l, k) per system, 0-based
     const double *fak;  // [sys][nblk*NB]
     const double *dga;
+    const zd *vals;     // [sys][nnz] scaled nonzeros (crt_assemble_kernel)
     zd *panels;         // [sys][nblk][D+1][NB][NB]
     zd *y;              // [sys][nblk*NB]
     int *status;
+    const zd *band;     // optional: factor a given scaled band matrix (reference layout) instead
 };
 
 __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
@@ -99,9 +125,9 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
     const int s = blockIdx.x;
     if (s >= A.nsys) return;
     const int l = A.sys[s].x, k = A.sys[s].y;
-    const int enode = g.enr[l];
-    const double *fak = A.fak + (size_t)s * nblk * NB;
-    const double *dga = A.dga + (size_t)s * nblk * NB;
+    const int enode = A.band ? -1 : g.enr[l];
+    const double *fak = A.band ? nullptr : A.fak + (size_t)s * nblk * NB;
+    const zd *vals = A.band ? nullptr : A.vals + (size_t)s * g.nnz;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * NB2;
     zd *yout = A.y + (size_t)s * nblk * NB;
 
@@ -131,36 +157,49 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
         return slots + (size_t)(base[d] + K % m) * NB2;
     };
 
-    // assemble row block I into its (zeroed) slots; CSR scatter, one thread per entry
-    auto scatter_rows = [&](int I, int t0, int nthr) {
-        for (int r = 0; r < NB; ++r) {
-            int i = I * NB + r;
-            if (i >= g.sanz) {
-                if (t0 == 0) slot_of(I, I)[r * NB + r] = zd{1.0, 0.0};
-                continue;
-            }
-            int q0 = g.row_ptr[i], q1 = g.row_ptr[i + 1];
-            double fi = fak[i];
-            for (int q = q0 + t0; q < q1; q += nthr) {
-                int j = g.col[q];
-                zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], l, k);
-                zd v;
-                if (j == i) {
-                    double dum = dga[i];
-                    v = zd{a.re / dum, a.im / dum};
-                } else {
-                    double f = fi * fak[j];
-                    v = zd{a.re * f, a.im * f};
-                }
-                slot_of(I, j / NB)[r * NB + (j % NB)] = v;
-            }
+    // Entries of row block I: the CSR entries [row_ptr[8I], row_ptr[8I+8]) are
+    // contiguous; a given band matrix contributes (mb+1) entries per row.
+    auto block_count = [&](int I) -> int {
+        if (A.band) return NB * (g.mb + 1);
+        return g.row_ptr[min(I * NB + NB, g.sanz)] - g.row_ptr[I * NB];
+    };
+    auto load_entry = [&](int I, int t, int &i, int &j, zd &v) -> bool {
+        if (A.band) {
+            int r = t / (g.mb + 1), o = t - r * (g.mb + 1);
+            i = I * NB + r;
+            j = i - g.mb + o;
+            if (i >= g.sanz || j < 0) return false;
+            v = A.band[(size_t)i * (g.mb + 1) + o];
+            return true;
         }
+        int q = g.row_ptr[I * NB] + t;
+        i = g.crow[q];
+        j = g.col[q];
+        v = vals[q];
+        return true;
+    };
+    auto put_entry = [&](int I, int i, int j, zd v) {
+        slot_of(I, j / NB)[(i - I * NB) * NB + (j % NB)] = v;
+    };
+    // unit diagonal of padding rows and the unit source of the RHS block
+    auto block_extras = [&](int I, int t0) {
         if (t0 < NB) {
             int i = I * NB + t0;
+            if (i >= g.sanz) slot_of(I, I)[t0 * NB + t0] = zd{1.0, 0.0};
             zd bv = zd{0.0, 0.0};
             if (i < g.sanz && i == enode) bv = zd{-fak[i], 0.0};
             bring[(I % (D + 2)) * NB + t0] = bv;
         }
+    };
+    // direct scatter of a whole block (set-up)
+    auto scatter_rows = [&](int I, int t0, int nthr) {
+        const int cnt = block_count(I);
+        for (int t = t0; t < cnt; t += nthr) {
+            int i, j;
+            zd v;
+            if (load_entry(I, t, i, j, v)) put_entry(I, i, j, v);
+        }
+        block_extras(I, t0);
     };
 
     // ---- initial window: row blocks 0..min(D, nblk-1)
@@ -170,6 +209,18 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
     __syncthreads();
 
     const int wave = tid >> 6, lane = tid & 63;
+    const int ntrsm = ((NB * D + 63) / 64) * 64;  // threads of the TRSM phase
+    const int nspare = (int)blockDim.x - ntrsm;    // threads that assemble the entering block
+    // one prefetched CSR entry per spare thread, loaded a step ahead
+    int pf_i = -1, pf_j = 0;
+    zd pf_v = zd{0.0, 0.0};
+    auto prefetch = [&](int I) {
+        pf_i = -1;
+        if (tid < ntrsm || I >= nblk) return;
+        int t = tid - ntrsm;
+        if (t < block_count(I) && !load_entry(I, t, pf_i, pf_j, pf_v)) pf_i = -1;
+    };
+    prefetch(D + 1);
     int err = 0;
     for (int J = 0; J < nblk; ++J) {
         const int Ie = J + D + 1;  // entering row block
@@ -217,7 +268,6 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
         // ---------------- phase 2: TRSM + RHS update (rows 8..8D+7) | assemble entering block
         {
             int nrow = NB * D;
-            int ntrsm = ((nrow + 63) / 64) * 64;
             if (tid < nrow) {
                 int a = 1 + tid / NB, r = tid % NB;
                 if (J + a < nblk) {
@@ -239,10 +289,19 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
                     bring[((J + a) % (D + 2)) * NB + r] = bacc;
                 }
             } else if (tid >= ntrsm && Ie < nblk) {
-                scatter_rows(Ie, tid - ntrsm, blockDim.x - ntrsm);
+                const int t0 = tid - ntrsm;
+                if (pf_i >= 0) put_entry(Ie, pf_i, pf_j, pf_v);
+                // entries beyond one per spare thread are loaded directly
+                const int cnt = block_count(Ie);
+                for (int t = nspare + t0; t < cnt; t += nspare) {
+                    int i, j;
+                    zd v;
+                    if (load_entry(Ie, t, i, j, v)) put_entry(Ie, i, j, v);
+                }
+                block_extras(Ie, t0);
+                prefetch(Ie + 1);
             }
-            if (ntrsm >= (int)blockDim.x && Ie < nblk) {
-                // not enough spare threads: assemble after the TRSM
+            if (nspare <= 0 && Ie < nblk) {
                 __syncthreads();
                 scatter_rows(Ie, tid, blockDim.x);
             }
@@ -305,6 +364,8 @@ struct SolveArgs {
     const zd *y;         // [fsys][nblk*NB]
     zd *yscr;            // [nrhs][nblk*NB] forward result of right-hand sides without have_y
     zd *kpotN;
+    const zd *bdense;    // optional dense right-hand side (drop-in vre); fak = 1
+    zd *pot;             // optional dense solution output (drop-in vre)
 };
 
 __global__ void __launch_bounds__(256) crt_solve_kernel(SolveArgs A) {
@@ -316,11 +377,11 @@ __global__ void __launch_bounds__(256) crt_solve_kernel(SolveArgs A) {
     const int D = A.D, nblk = A.nblk;
     const int4 rq = A.rhs[rid];
     const int l = rq.x, k = rq.y, fs = rq.z;
-    const double *fak = A.fak + (size_t)fs * nblk * NB;
+    const double *fak = A.fak ? A.fak + (size_t)fs * nblk * NB : nullptr;
     const zd *panel = A.panels + (size_t)fs * nblk * (D + 1) * NB2;
     zd *ring = reinterpret_cast<zd *>(smem_s) + (size_t)wave * (D + 1) * NB;
     const int r = lane >> 3, c = lane & 7;
-    const int enode = A.enr[l];
+    const int enode = A.bdense ? -1 : A.enr[l];
     zd *yscr = A.yscr + (size_t)rid * nblk * NB;
 
     if (!rq.w) {
@@ -337,7 +398,12 @@ __global__ void __launch_bounds__(256) crt_solve_kernel(SolveArgs A) {
             acc = zadd(acc, shfl_xor_z(acc, 2));
             acc = zadd(acc, shfl_xor_z(acc, 4));
             int i = J * NB + r;
-            zd bv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+            zd bv = zd{0.0, 0.0};
+            if (A.bdense) {
+                if (i < A.sanz) bv = A.bdense[i];
+            } else if (i < A.sanz && i == enode) {
+                bv = zd{-fak[i], 0.0};
+            }
             zd rhs = zsub(bv, acc);  // valid on lanes with c == 0 for row r
             zd Ld = panel[((size_t)J * (D + 1)) * NB2 + lane];
             zd yv = shfl_z(rhs, (lane & 7) * NB);  // lane holds rhs of row (lane & 7)
@@ -386,8 +452,12 @@ __global__ void __launch_bounds__(256) crt_solve_kernel(SolveArgs A) {
             ring[(J % (D + 1)) * NB + lane] = xv;
             int i = J * NB + lane;
             if (i < A.sanz) {
-                double f = fak[i];
-                A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
+                if (A.pot) {
+                    A.pot[i] = xv;
+                } else {
+                    double f = fak[i];
+                    A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -404,9 +474,17 @@ int crt_launch_diag(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys,
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
+int crt_launch_assemble(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,
+                        const double *dga, zd *vals) {
+    long n = (long)nsys * g.nnz;
+    hipLaunchKernelGGL(crt_assemble_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, sys, fak,
+                       dga, vals);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
 int crt_launch_factor(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,
-                      const double *dga, zd *panels, zd *y, int *status) {
-    FactorArgs a{g, nsys, sys, fak, dga, panels, y, status};
+                      const double *dga, const zd *vals, zd *panels, zd *y, int *status) {
+    FactorArgs a{g, nsys, sys, fak, dga, vals, panels, y, status, nullptr};
     size_t lds = crt_factor_lds_bytes(g.D);
     if (lds > 160 * 1024) return CRT_ERR_UNSUPPORTED;
     static bool attr_set = false;
@@ -419,10 +497,46 @@ int crt_launch_factor(hipStream_t st, const DevGrid &g, int nsys, const int2 *sy
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
+// panel[J][a][r][c] <-> reference band layout a[i*(mb+1) + j-i+mb] (lower factor)
+__global__ void __launch_bounds__(256) crt_panel_band_kernel(int sanz, int mb, int nblk, int D, int to_band,
+                                                             zd *__restrict__ panel, zd *__restrict__ band) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)sanz * (mb + 1)) return;
+    int i = (int)(t / (mb + 1)), o = (int)(t - (long)i * (mb + 1));
+    int j = i - mb + o;
+    if (j < 0) {
+        if (to_band) band[t] = zd{0.0, 0.0};
+        return;
+    }
+    int J = j / NB, a = i / NB - J;
+    size_t pi = (((size_t)J * (D + 1) + a) * NB + (i % NB)) * NB + (j % NB);
+    if (to_band) band[t] = panel[pi];
+    else panel[pi] = band[t];
+}
+
+int crt_launch_panel_band(hipStream_t st, int sanz, int mb, int nblk, int D, int to_band, zd *panel, zd *band) {
+    long n = (long)sanz * (mb + 1);
+    hipLaunchKernelGGL(crt_panel_band_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sanz, mb, nblk, D,
+                       to_band, panel, band);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+// factor one scaled band matrix given in the reference layout (drop-in chol)
+int crt_launch_factor_band(hipStream_t st, const DevGrid &g, const int2 *sys, const zd *band, zd *panels, zd *y,
+                           int *status) {
+    FactorArgs a{g, 1, sys, nullptr, nullptr, nullptr, panels, y, status, band};
+    size_t lds = crt_factor_lds_bytes(g.D);
+    if (lds > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    (void)hipFuncSetAttribute((const void *)crt_factor_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    hipLaunchKernelGGL(crt_factor_kernel, dim3(1), dim3(256), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
 int crt_launch_solve(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs,
                      const int4 *rhs, const int *enr, const double *fak, const zd *panels, const zd *y,
-                     zd *yscr, zd *kpotN) {
-    SolveArgs a{sanz, nblk, D, kwnanz, eanz, nrhs, rhs, enr, fak, panels, y, yscr, kpotN};
+                     zd *yscr, zd *kpotN, const zd *bdense, zd *pot) {
+    SolveArgs a{sanz, nblk, D, kwnanz, eanz, nrhs, rhs, enr, fak, panels, y, yscr, kpotN, bdense, pot};
     const int waves = 4;
     size_t lds = crt_solve_lds_bytes(D, waves);
     hipLaunchKernelGGL(crt_solve_kernel, dim3((nrhs + waves - 1) / waves), dim3(64 * waves), lds, st, a);

@@ -101,9 +101,11 @@ __host__ __device__ __forceinline__ zd zsqrt(zd z) {
 struct DevGrid {
     int sanz, mb, elanz, relanz, eanz, maxn, npk, kwnanz, nanz;
     int nblk, D;              // row blocks and sub-diagonal block count
+    int nnz;                  // lower-triangle nonzeros
     // CSR of the lower triangle of A in node order (0-based), diagonal last
     const int *row_ptr;       // sanz+1
     const int *col;           // nnz
+    const int *crow;          // nnz: row of each entry
     const int *cptr;          // nnz+1 contribution ranges
     const int *contrib;       // >=0: e*npk+imn (area); <0: -(rel*npk+imn)-1 (mixed boundary)
     const int *rnr;           // relanz (0-based area element)

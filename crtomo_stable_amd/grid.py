@@ -187,6 +187,36 @@ def read_elem(path: str) -> Grid:
                 nrel=nrel, x=x, y=y, rnr=rnr, enr=np.zeros(0, np.int32))
 
 
+def fix_rnr(grid: Grid) -> Grid:
+    """Border-to-element check of ``src/relem.f90:197-229``: every boundary element
+    is attached to the area element (of the first type group) holding both its
+    nodes."""
+    elanz = grid.elanz
+    s0 = int(grid.selanz[0])
+    owner = {}
+    for k in range(elanz):
+        nodes = grid.nrel[k, :s0]
+        for a in range(s0):
+            for b in range(a + 1, s0):
+                owner[(min(nodes[a], nodes[b]), max(nodes[a], nodes[b]))] = k + 1
+    rnr = grid.rnr.copy()
+    for i in range(grid.relanz):
+        n1, n2 = grid.nrel[elanz + i, 0], grid.nrel[elanz + i, 1]
+        k = owner.get((min(n1, n2), max(n1, n2)))
+        if k is not None and rnr[i] != k:
+            rnr[i] = k
+    return replace(grid, rnr=rnr)
+
+
+def read_rho(path: str) -> np.ndarray:
+    """``rho.dat``: elanz, then magnitude (ohm m) and phase (mrad) per element;
+    returns sigma (``src/rsigma.f90:168``)."""
+    tok = _tokens(path)
+    n = int(tok[0])
+    v = np.array(tok[1:1 + 2 * n], dtype=float).reshape(n, 2)
+    return sigma_from_rho(v[:, 0], v[:, 1])
+
+
 def read_elem_records(path: str):
     """Raw records of an ``elem.dat``: header, type table, (snr, sx, sy), element lists, rnr."""
     tok = _tokens(path)

@@ -217,6 +217,29 @@ class Context:
                                  _ptr(cgfac), _ptr(bvec)), "crt_update")
         return dict(par=par, dpar=dpar, cgres=cgres, ncg=ncg.value, cgfac=cgfac, bvec=bvec)
 
+    # ------------------------------------------------- single-system drop-ins
+    def chol(self, sanz, mb, a):
+        """In-place band Cholesky of a (reference band layout); returns errnr."""
+        a2 = np.ascontiguousarray(a, np.complex128)
+        err = self.lib.crt_chol(self._h, int(sanz), int(mb), _ptr(a2))
+        if a2 is not a:
+            a[...] = a2
+        return err
+
+    def vre(self, sanz, mb, a, b):
+        pot = np.zeros(sanz, np.complex128)
+        _chk(self.lib.crt_vre(self._h, int(sanz), int(mb), _ptr(np.ascontiguousarray(a, np.complex128)),
+                              _ptr(np.ascontiguousarray(b, np.complex128)), _ptr(pot)), "crt_vre")
+        return pot
+
+    def cholz(self, a_colmajor):
+        """a (n, n) Fortran-ordered complex; returns (errnr, a_out, p)."""
+        n = a_colmajor.shape[0]
+        a = np.array(a_colmajor, dtype=np.complex128, order="F")
+        p = np.zeros(n, np.complex128)
+        err = self.lib.crt_cholz(self._h, n, a.ctypes.data_as(_p), _ptr(p))
+        return err, a, p
+
     def times(self):
         ms = np.zeros(8)
         self.lib.crt_get_times(self._h, _ptr(ms), 8)

@@ -1,0 +1,91 @@
+"""Writes the FP64 golden vectors of tests/golden/*.npz from the reference's own
+routines (the amdflang build of /root/reference/src made by oracle/Makefile, run
+in this container through oracle/ref_harness.f90).  The committed files are data:
+inputs are regenerated by crtomo_stable_amd.configs / the fixture grids, outputs
+are what the reference computed.
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from crtomo_stable_amd import configs, grid as G  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def fixture_workload(name):
+    d = os.path.join(HERE, "reference_fixtures", name)
+    g = G.read_elem(os.path.join(d, "elem.dat"))
+    g = G.replace(g, enr=G.read_elec(os.path.join(d, "elec.dat")))
+    g = G.fix_rnr(g)
+    survey, _, _ = G.read_volt(os.path.join(d, "volt.dat"))
+    sigma = G.read_rho(os.path.join(d, "rho.dat"))
+    return configs.Workload(name, g, survey, sigma)
+
+
+def forward_golden(w, sens=True, bsens=True):
+    R = O.RefSession(w.grid, w.survey)
+    kwn, kwi, amin, amax = R.rwaven()
+    elbg, relbg, kg, lb = R.precal()
+    kpot = R.forward(w.sigma, nthreads=4)
+    hpot, volt, sa = R.voltages(True)
+    out = dict(kwn=kwn, kwnwi=kwi, amin=amin, amax=amax, lbeta=lb, volt=volt, sigmaa=sa,
+               hpot_elec=hpot[:, w.grid.enr - 1], kpot_src1=kpot[:, 0, :])
+    if sens:
+        out["sens"] = R.sens(True)
+    if sens and bsens:
+        _, volt0, sa0 = R.voltages(False)
+        out["volt_bvolt"] = volt0
+        out["sens_bsens"] = R.sens(False)
+    return out, R
+
+
+def cg_golden(w, R, out, ltri, lam, eps, ncgmax, seed):
+    rng = np.random.default_rng(seed)
+    sa = out["sigmaa"]
+    dat = sa + 0.02 * (rng.standard_normal(sa.size) + 0.1j * rng.standard_normal(sa.size))
+    wmatd = 1.0 / (0.03 + 0.01 * rng.random(sa.size)) ** 2
+    wdfak = np.ones(sa.size, np.int32)
+    par = np.log(w.sigma) + 0.05 * rng.standard_normal(w.sigma.size)
+    smatm, nachbar = R.bsmatm(ltri, w.grid.nx, w.grid.nz, 1.0, 1.0)
+    cp = O.CGProblem(out["sens"], wmatd, wdfak, dat, sa, par, smatm, ltri, w.grid.nx, lam, eps, ncgmax,
+                     nachbar=nachbar if ltri else None, maxn=w.grid.maxn)
+    cgfac, bvec, dpar, cgres, ncg = R.update(cp)
+    p = f"cg{ltri}_"
+    return {p + "dat": dat, p + "wmatd": wmatd, p + "par": par, p + "smatm": smatm, p + "nachbar": nachbar,
+            p + "cgfac": cgfac, p + "bvec": bvec, p + "dpar": dpar, p + "cgres": cgres, p + "ncg": ncg,
+            p + "lam": lam, p + "eps": eps, p + "ncgmax": ncgmax}
+
+
+def main():
+    O.build()
+    if not O.ref_available():
+        raise SystemExit("reference build missing: make -C oracle ref")
+    # fixture grids of the reference's tests (CRMod voltages)
+    for name in ("minimal", "mixed_surf"):
+        w = fixture_workload(name)
+        out, _ = forward_golden(w, sens=(name == "minimal"))
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **out)
+    # seeded synthetic grids: forward, sensitivities and one CG update
+    w = configs.small_quad(cutmck=O.cutmck)
+    out, R = forward_golden(w)
+    out.update(cg_golden(w, R, out, 0, 10.0, 1e-4, w.grid.elanz // 10, 1))
+    np.savez_compressed(os.path.join(HERE, "ref_small_quad.npz"), **out)
+    w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    out, R = forward_golden(w, bsens=False)
+    out.pop("kpot_src1")
+    out.update(cg_golden(w, R, out, 1, 3.0, 1e-12, 400, 2))
+    np.savez_compressed(os.path.join(HERE, "ref_small_tri.npz"), **out)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,260 @@
+"""CPU checks that pin the oracle (oracle/crt_oracle.c) before it is trusted as
+the checker of the HIP path:
+
+* against the reference's own routines: committed FP64 golden vectors written by
+  tests/golden/make_golden.py from the amdflang build of the reference, and the
+  live reference build when oracle/_ref is present;
+* against the reference's committed REAL*4 outputs (tests/*/mod/volt.dat) and
+  CutMcK fixture pairs (elem.dat.orig -> elem.dat);
+* against mathematics: textbook Gauss-Legendre/Laguerre rules, scipy Bessel
+  functions within the Abramowitz & Stegun 9.8 error bounds, element-matrix
+  invariants, dense direct solves of tiny systems.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from crtomo_stable_amd import configs, grid as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+FIX = os.path.join(GOLD, "reference_fixtures")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def fixture(name):
+    d = os.path.join(FIX, name)
+    g = G.read_elem(os.path.join(d, "elem.dat"))
+    g = G.replace(g, enr=G.read_elec(os.path.join(d, "elec.dat")))
+    g = G.fix_rnr(g)
+    survey, mag, pha = G.read_volt(os.path.join(d, "volt.dat"))
+    return configs.Workload(name, g, survey, G.read_rho(os.path.join(d, "rho.dat"))), mag, pha
+
+
+def oracle_forward(O, w, mode=1, sens=True):
+    kwn, kwi, amin, amax = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn)
+    kpot = O.forward(w.grid, w.sigma, elbg, relbg, kg, lb)
+    hpot = O.rtrafo(kpot, kwi)
+    if mode == 1:
+        volt, sa = O.bvolti(w.grid, w.survey, hpot)
+    else:
+        volt, sa = O.bvolt(w.grid, w.survey, hpot)
+    out = dict(kwn=kwn, kwnwi=kwi, amin=amin, amax=amax, lbeta=lb, kpot=kpot, hpot=hpot, volt=volt, sigmaa=sa,
+               elbg=elbg, kg=kg)
+    if sens:
+        if mode == 1:
+            out["sens"] = O.bsensi(w.grid, w.survey, kwi, kpot, elbg, w.sigma, volt)
+        else:
+            out["sens"] = O.bsens(w.grid, w.survey, kwi, kpot, elbg)
+    return out
+
+
+# ------------------------------------------------------------------ textbook
+@pytest.mark.parametrize("n", [2, 3, 6, 10, 17, 26])
+def test_gauleg_textbook(oracle_lib, n):
+    x, w = oracle_lib.gauleg(n, 0.0, 1.0)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, 0.5 * (xr + 1.0), rtol=0, atol=1e-14)
+    assert np.allclose(w, 0.5 * wr, rtol=1e-10, atol=0)  # Newton stop at |dz| <= 3e-14 (src/gauleg.f90:11)
+    assert abs(w.sum() - 1.0) < 5e-14
+
+
+def test_gaulag_textbook(oracle_lib):
+    x, w = oracle_lib.gaulag(4, 0.0)
+    xr, wr = np.polynomial.laguerre.laggauss(4)
+    assert np.allclose(x, xr, rtol=1e-13)
+    assert np.allclose(w, wr, rtol=1e-12)
+    assert abs(w.sum() - 1.0) < 1e-13  # integral of exp(-x)
+
+
+def test_bessel_abramowitz_stegun(oracle_lib):
+    from scipy import special
+    xs = np.concatenate([np.geomspace(1e-3, 2.0, 200), np.linspace(2.0, 60.0, 200)])
+    for name, f in (("bessk0", special.k0), ("bessk1", special.k1), ("bessi0", special.i0), ("bessi1", special.i1)):
+        got = np.array([oracle_lib.bessel(name, x) for x in xs])
+        ref = f(xs)
+        ok = ref > 1e-300
+        # A&S 9.8.1-9.8.8: |eps| < 2.2e-7 relative to the leading asymptotic form
+        assert np.max(np.abs(got[ok] - ref[ok]) / ref[ok]) < 6e-7, name
+    assert oracle_lib.bessel("bessk0", 600.0) == 0.0 and oracle_lib.bessel("bessk1", 600.0) == 0.0
+
+
+def test_element_invariants(oracle_lib):
+    """Stiffness parts annihilate constants; the k^2 mass parts sum to the area
+    (linear triangles: src/elem3.f90:82-94)."""
+    for w in (configs.small_quad(cutmck=oracle_lib.cutmck),
+              configs.c1(scale=0.15, nelec=8, nmeas=20, seed=3, cutmck=oracle_lib.cutmck)):
+        g = w.grid
+        kwn = np.array([0.0, 0.7])
+        elbg, _, _, _ = oracle_lib.precal(g, kwn)
+        nk = g.maxn
+        full = np.zeros((2, g.elanz, nk, nk))
+        imn = 0
+        for m in range(nk):
+            for n in range(m + 1):
+                full[:, :, m, n] = elbg[:, imn, :]
+                full[:, :, n, m] = elbg[:, imn, :]
+                imn += 1
+        assert np.max(np.abs(full[0].sum(axis=2))) < 1e-12 * np.max(np.abs(full[0]))
+        if nk == 3:
+            nd = g.nrel[: g.elanz, :3] - 1
+            x, y = g.x[nd], g.y[nd]
+            area = 0.5 * ((x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0]))
+            mass = (full[1] - full[0]).sum(axis=(1, 2)) / 0.7 ** 2
+            assert np.allclose(mass, area, rtol=1e-12)
+
+
+def test_dense_bruteforce(oracle_lib):
+    """Band Cholesky + substitution equal a dense complex-symmetric solve, and
+    L L^T reproduces A (no conjugation), on a tiny mixed-boundary grid."""
+    O = oracle_lib
+    w = configs.small_quad(nx=10, nz=3, nelec=4, cutmck=O.cutmck)
+    g = w.grid
+    kwn, kwi, _, _ = O.rwaven(g, w.survey)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    a, b = O.kompab(g, w.sigma, elbg, relbg, kg, 2, 3)
+    fak = O.scalab(g.sanz, g.mb, a, b)
+    n, mb = g.sanz, g.mb
+    A = np.zeros((n, n), complex)
+    for i in range(n):
+        for j in range(max(0, i - mb), i + 1):
+            A[i, j] = A[j, i] = a[i * (mb + 1) + (j - i + mb)]
+    L = a.copy()
+    assert O.chol(n, mb, L) == 0
+    Ld = np.zeros((n, n), complex)
+    for i in range(n):
+        for j in range(max(0, i - mb), i + 1):
+            Ld[i, j] = L[i * (mb + 1) + (j - i + mb)]
+    assert rel(Ld @ Ld.T, A) < 1e-13
+    pot = O.vre(n, mb, L, b)
+    assert rel(pot, np.linalg.solve(A, -b)) < 1e-12
+    err, az, p = O.cholz(A)
+    assert err == 0 and rel(p, np.diag(Ld)) < 1e-13
+
+
+# ------------------------------------------------------------ reference data
+@pytest.mark.parametrize("name", ["minimal", "mixed_surf"])
+def test_cutmck_fixture_pairs(oracle_lib, name):
+    """CutMcK reproduces the committed elem.dat / elec.dat from the .orig files
+    token for token (cutmckee/cutmck.f:126-355)."""
+    d = os.path.join(FIX, name)
+    g = G.read_elem(os.path.join(d, "elem.dat.orig"))
+    perm, mb, _ = oracle_lib.cutmck(g)
+    s0, t0, mb0, types0, rec0, rest0 = G.read_elem_records(os.path.join(d, "elem.dat.orig"))
+    s1, t1, mb1, types1, rec1, rest1 = G.read_elem_records(os.path.join(d, "elem.dat"))
+    assert mb == mb1
+    snr_old = rec0[:, 0].astype(int)
+    snr_new = np.empty_like(snr_old)
+    snr_new[perm - 1] = snr_old
+    assert np.array_equal(snr_new, rec1[:, 0].astype(int))
+    old = np.array(rest0, dtype=int)
+    new = np.array(rest1, dtype=int)
+    nel_tok = sum(n * s for _, n, s in types0)
+    assert np.array_equal(perm[old[:nel_tok] - 1], new[:nel_tok])
+    assert np.array_equal(old[nel_tok:], new[nel_tok:])  # rnr unchanged
+    e_old = G.read_elec(os.path.join(d, "elec.dat.orig"))
+    assert np.array_equal(perm[e_old - 1], G.read_elec(os.path.join(d, "elec.dat")))
+
+
+@pytest.mark.parametrize("name", ["minimal", "testmodinv"])
+def test_volt_dat_real4(oracle_lib, name):
+    """CRMod voltages against the reference's committed volt.dat (REAL*4 output,
+    src/wdatm.f90:58-76, polarity normalised)."""
+    w, mag, pha = fixture(name)
+    r = oracle_forward(oracle_lib, w, mode=0, sens=False)
+    v = r["volt"]
+    ph = np.arctan2(v.imag, v.real)
+    ph = ph - np.round(ph / np.pi) * np.pi
+    assert np.max(np.abs(np.abs(v) - mag) / mag) < 1e-6
+    assert np.max(np.abs(1e3 * ph - pha)) < 1e-3
+
+
+@pytest.mark.parametrize("name", ["minimal", "mixed_surf", "small_quad", "small_tri"])
+def test_oracle_vs_reference_golden(oracle_lib, name):
+    """Forward, voltages and sensitivities against FP64 vectors of the reference build."""
+    ref = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    O = oracle_lib
+    if name in ("minimal", "mixed_surf"):
+        w, _, _ = fixture(name)
+    elif name == "small_quad":
+        w = configs.small_quad(cutmck=O.cutmck)
+    else:
+        w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    r = oracle_forward(O, w, mode=1, sens="sens" in ref)
+    assert np.array_equal(r["kwn"], ref["kwn"]) and np.array_equal(r["kwnwi"], ref["kwnwi"])
+    assert bool(r["lbeta"]) == bool(ref["lbeta"])
+    assert float(np.max(np.abs(r["volt"] - ref["volt"]) / np.abs(ref["volt"]))) < 1e-12
+    assert rel(r["sigmaa"], ref["sigmaa"]) < 1e-12
+    assert rel(r["hpot"][:, w.grid.enr - 1], ref["hpot_elec"]) < 1e-12
+    if "kpot_src1" in ref:
+        assert rel(r["kpot"][:, 0, :], ref["kpot_src1"]) < 1e-12
+    if "sens" in ref:
+        assert rel(r["sens"], ref["sens"].reshape(r["sens"].shape)) < 1e-11
+    if "sens_bsens" in ref:
+        r0 = oracle_forward(O, w, mode=0, sens=True)
+        assert float(np.max(np.abs(r0["volt"] - ref["volt_bvolt"]) / np.abs(ref["volt_bvolt"]))) < 1e-12
+        assert rel(r0["sens"], ref["sens_bsens"]) < 1e-11
+
+
+@pytest.mark.parametrize("name,ltri", [("small_quad", 0), ("small_tri", 1)])
+def test_oracle_cg_vs_reference_golden(oracle_lib, name, ltri):
+    """update + cjg (src/update.f90, src/cg_mod.f90) against the reference build:
+    identical operation order, so identical results."""
+    O = oracle_lib
+    ref = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    w = configs.small_quad(cutmck=O.cutmck) if ltri == 0 else \
+        configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    p = f"cg{ltri}_"
+    g = w.grid
+    if ltri == 0:
+        smatm = O.bsmatmreg(g, g.nx, g.nz, 1.0, 1.0)
+        nachbar = None
+    else:
+        nachbar = O.bnachbar(g)
+        assert np.array_equal(nachbar, ref[p + "nachbar"])
+        smatm = O.bsmatmtri(g, nachbar, 1.0, 1.0)
+    assert np.array_equal(smatm, ref[p + "smatm"])
+    cp = O.CGProblem(ref["sens"], ref[p + "wmatd"], np.ones(ref["sigmaa"].size, np.int32), ref[p + "dat"],
+                     ref["sigmaa"], ref[p + "par"], smatm, ltri, g.nx, float(ref[p + "lam"]),
+                     float(ref[p + "eps"]), int(ref[p + "ncgmax"]), nachbar=nachbar, maxn=g.maxn)
+    cgfac, bvec = O.update_rhs(cp)
+    dpar, cgres, ncg = O.cjggra(cp, cgfac, bvec)
+    assert ncg == int(ref[p + "ncg"])
+    assert np.array_equal(cgfac, ref[p + "cgfac"])
+    assert rel(bvec, ref[p + "bvec"]) < 1e-14
+    assert rel(dpar * cgfac, ref[p + "dpar"]) < 1e-12
+    assert np.array_equal(cgres[: ncg + 1], ref[p + "cgres"][: ncg + 1])
+
+
+def test_rwaven_bookkeeping(oracle_lib):
+    """Wavenumber count from int(real(6 log10(amax/amin))) (src/rwaven.f90:105)."""
+    w = configs.c0(cutmck=oracle_lib.cutmck)
+    kwn, kwi, amin, amax = oracle_lib.rwaven(w.grid, w.survey)
+    assert (amin, amax, kwn.size) == (1.0, 10.0, 10)
+    # k' = (k/k0)^1/2 substitution: Legendre part integrates sqrt-weighted rules exactly
+    k0 = 1.0 / (2.0 * amin)
+    ganz = kwn.size - 4
+    assert abs(kwi[:ganz].sum() - k0) < 1e-14 * k0  # integral of 1 over [0, k0]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref",
+                                                    "libcrtomo_ref.so")), reason="reference build absent")
+def test_oracle_vs_reference_live_c0(oracle_lib):
+    """C0 (60x20 quads, 188 dipole-dipole) through the live reference build."""
+    O = oracle_lib
+    w = configs.c0(cutmck=O.cutmck)
+    r = oracle_forward(O, w, mode=1)
+    R = O.RefSession(w.grid, w.survey)
+    kwn, kwi, _, _ = R.rwaven()
+    elbg, relbg, kg, lb = R.precal()
+    assert np.array_equal(elbg, r["elbg"]) and np.array_equal(kg, r["kg"])
+    kpot = R.forward(w.sigma, nthreads=4)
+    assert rel(r["kpot"], kpot) < 1e-12
+    hpot, volt, sa = R.voltages(True)
+    assert float(np.max(np.abs(r["volt"] - volt) / np.abs(volt))) < 1e-12
+    assert rel(r["sens"], R.sens(True)) < 1e-11
