@@ -22,6 +22,11 @@ int crt_launch_solve(hipStream_t, int, int, int, int, int, int, const int4 *, co
                      const zd *, const zd *, zd *, zd *, const zd *, zd *);
 int crt_launch_panel_band(hipStream_t, int, int, int, int, int, zd *, zd *);
 int crt_launch_factor_band(hipStream_t, const DevGrid &, const int2 *, const zd *, zd *, zd *, int *);
+int crt_factor16_supported(int mb);
+int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zd *, zd *,
+                        int *);
+int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
+                       const zd *, const zd *, zd *, zd *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
@@ -128,6 +133,7 @@ struct crt_ctx {
         DevGrid g;
         g.sanz = sanz; g.mb = mb; g.elanz = elanz; g.relanz = relanz; g.eanz = eanz; g.maxn = maxn;
         g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D; g.nnz = nnz;
+        g.fstride = nblk * CRT_NB;
         g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.crow = d_crow.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
         g.rnr = d_rnr.p; g.enr = d_enr.p; g.elbg = d_elbg.p; g.relbg = d_relbg.p; g.kg = d_kg.p;
         g.sigma = d_sigma.p;
@@ -535,8 +541,12 @@ int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    if (crt_factor_lds_bytes(c->D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
-    const int K = c->kwnanz, E = c->eanz, nblk = c->nblk, D = c->D;
+    // 16x16 matrix-core tiles when the band fits (mb <= 160), else 8x8 LDS tiles
+    const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
+    const int NBt = use16 ? 16 : CRT_NB;
+    const int K = c->kwnanz, E = c->eanz;
+    const int nblk = (c->sanz + NBt - 1) / NBt, D = std::max(1, (c->mb + NBt - 1) / NBt);
+    if (!use16 && crt_factor_lds_bytes(D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
     // systems to factorise: every (l,k) with mixed boundaries, else l = 1 per k
     // (src/fem.f90:355: lbeta.or.l.eq.1)
     std::vector<int2> sys;
@@ -553,14 +563,15 @@ int crt_forward(crt_ctx *c) {
             for (int l = 0; l < E; ++l) rhs.push_back(int4{l, k, fs, l == 0 ? 1 : 0});
         }
     }
-    const size_t panel_elems = (size_t)nblk * (D + 1) * CRT_NB * CRT_NB;
-    size_t per_sys = panel_elems * sizeof(zd) + (size_t)nblk * CRT_NB * (sizeof(zd) + 2 * sizeof(double)) +
+    const size_t fstride = (size_t)nblk * NBt;
+    const size_t panel_elems = (size_t)nblk * (D + 1) * NBt * NBt;
+    size_t per_sys = panel_elems * sizeof(zd) + fstride * (sizeof(zd) + 2 * sizeof(double)) +
                      (size_t)c->nnz * sizeof(zd);
     size_t batch = std::max<size_t>(1, std::min(sys.size(), c->panel_budget / per_sys));
     int e = c->d_panels.ensure(batch * panel_elems);
-    e = e ? e : c->d_yf.ensure(batch * nblk * CRT_NB);
-    e = e ? e : c->d_fak.ensure(batch * nblk * CRT_NB);
-    e = e ? e : c->d_dga.ensure(batch * nblk * CRT_NB);
+    e = e ? e : c->d_yf.ensure(batch * fstride);
+    e = e ? e : c->d_fak.ensure(batch * fstride);
+    e = e ? e : c->d_dga.ensure(batch * fstride);
     e = e ? e : c->d_vals.ensure(batch * c->nnz);
     e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
     e = e ? e : c->d_sys.ensure(sys.size());
@@ -568,10 +579,13 @@ int crt_forward(crt_ctx *c) {
     e = e ? e : c->d_status.ensure(sys.size() + 1);
     if (e) return e;
     size_t nyscr = c->lbeta ? 1 : rhs.size();
-    if ((e = c->d_yscr.ensure(nyscr * nblk * CRT_NB))) return e;
+    if ((e = c->d_yscr.ensure(nyscr * fstride))) return e;
     CRT_HIP(hipMemcpyAsync(c->d_sys.p, sys.data(), sys.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     CRT_HIP(hipMemsetAsync(c->d_status.p, 0, (sys.size() + 1) * sizeof(int), c->stream));
     DevGrid g = c->devgrid();
+    g.nblk = nblk;
+    g.D = D;
+    g.fstride = (int)fstride;
     double tdiag = 0, tfac = 0, tsol = 0;
     for (size_t s0 = 0; s0 < sys.size(); s0 += batch) {
         int nb = (int)std::min(batch, sys.size() - s0);
@@ -587,15 +601,22 @@ int crt_forward(crt_ctx *c) {
             return e;
         c->toc(T_DIAG);
         c->tic(T_FACTOR);
-        if ((e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
-                                   c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0)))
-            return e;
+        if (use16)
+            e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
+                                    c->d_yf.p, c->d_status.p + 1 + s0);
+        else
+            e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
+                                  c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
+        if (e) return e;
         c->toc(T_FACTOR);
         c->tic(T_SOLVE);
-        if ((e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                  c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr,
-                                  nullptr)))
-            return e;
+        if (use16)
+            e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
+                                   c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
+        else
+            e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
+                                 c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr, nullptr);
+        if (e) return e;
         c->toc(T_SOLVE);
         CRT_HIP(hipStreamSynchronize(c->stream));
         c->collect(T_DIAG); tdiag += c->times[T_DIAG];
@@ -614,7 +635,7 @@ int crt_forward(crt_ctx *c) {
                         st[q]);
                 if (q - 1 < batch) {
                     std::vector<double> dg(c->sanz);
-                    (void)hipMemcpy(dg.data(), c->d_dga.p + (q - 1) * nblk * CRT_NB, c->sanz * sizeof(double),
+                    (void)hipMemcpy(dg.data(), c->d_dga.p + (q - 1) * fstride, c->sanz * sizeof(double),
                                     hipMemcpyDeviceToHost);
                     for (int i = 0; i < c->sanz; ++i)
                         if (!(dg[i] > 0.0) || dg[i] == 1.0) {
@@ -1011,6 +1032,7 @@ static DevGrid band_grid(int sanz, int mb) {
     g.mb = mb;
     g.nblk = (sanz + CRT_NB - 1) / CRT_NB;
     g.D = std::max(1, (mb + CRT_NB - 1) / CRT_NB);
+    g.fstride = g.nblk * CRT_NB;
     return g;
 }
 

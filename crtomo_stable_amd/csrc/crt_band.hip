@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) crt_diag_kernel(DevGrid g, int nsys, cons
     int q = g.row_ptr[i + 1] - 1;  // diagonal is the last CSR entry of the row
     zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
     double dum = zabs(a);
-    size_t o = (size_t)s * g.nblk * NB + i;
+    size_t o = (size_t)s * g.fstride + i;
     if (!(dum > 0.0)) {
         status[s] = CRT_ERR_ZERODIAG;  // plain store: any non-zero code fails the call
         dum = 1.0;
@@ -92,10 +92,10 @@ __global__ void __launch_bounds__(256) crt_assemble_kernel(DevGrid g, int nsys, 
     int2 lk = sys[s];
     int i = g.crow[q], j = g.col[q];
     zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
-    const double *fs = fak + (size_t)s * g.nblk * NB;
+    const double *fs = fak + (size_t)s * g.fstride;
     zd v;
     if (j == i) {
-        double dum = dga[(size_t)s * g.nblk * NB + i];
+        double dum = dga[(size_t)s * g.fstride + i];
         v = zd{a.re / dum, a.im / dum};
     } else {
         double f = fs[i] * fs[j];

@@ -102,6 +102,7 @@ struct DevGrid {
     int sanz, mb, elanz, relanz, eanz, maxn, npk, kwnanz, nanz;
     int nblk, D;              // row blocks and sub-diagonal block count
     int nnz;                  // lower-triangle nonzeros
+    int fstride;              // per-system stride of fak / dga / y arrays (rows padded to a block)
     // CSR of the lower triangle of A in node order (0-based), diagonal last
     const int *row_ptr;       // sanz+1
     const int *col;           // nnz
