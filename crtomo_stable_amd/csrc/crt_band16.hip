@@ -7,10 +7,17 @@
 //     (real and imaginary part, 8 VGPRs each);
 //   * tiles on diagonal d = a-b slide one position per step; wave 0 owns the
 //     main diagonal, wave g >= 1 owns diagonals g and D+1-g, so every wave
-//     holds D+1 tiles at most and the rank-16 updates are balanced;
-//   * the factored panel column J and the entering row block J+D+1 are staged
-//     in LDS (about 8 KB per tile column), so two workgroups share a CU and one
-//     system's pivot chain overlaps the other's matrix-core updates.
+//     holds D+1 tiles at most and the rank-16 updates are balanced.  The code
+//     of each wave is instantiated separately, so every slot index is a
+//     compile-time constant;
+//   * per block column J: wave 0 factors the diagonal tile and runs the
+//     forward substitution of the factor's right-hand side (one LDS exchange
+//     per column), waves 1.. solve the panel rows X = A L^-T column by column
+//     and stage the entering row block from the CSR values, then every wave
+//     applies its rank-16 updates on the matrix cores;
+//   * the panel column and the entering row block are staged in LDS (about
+//     60 KB at D = 6), so two workgroups share a CU and one system's pivot
+//     chain overlaps the other's matrix-core updates.
 // Complex products use four real MFMAs: Tr -= Ar Br - Ai Bi, Ti -= Ar Bi + Ai Br.
 // Operand/result lanes of __builtin_amdgcn_mfma_f64_16x16x4f64 (checked on the
 // device): A[i][k] at lane i+16k, B[k][j] at lane j+16k, D[i][j] at lane
@@ -28,18 +35,56 @@ __device__ __forceinline__ zd shfl_xor_z16(zd v, int m) {
     return zd{__shfl_xor(v.re, m, 64), __shfl_xor(v.im, m, 64)};
 }
 
-// complex pivot: L = sqrt(s), inv = 1/L = conj(L)/|s|
-__device__ __forceinline__ void pivot_inv(zd s, zd &L, zd &inv, double &mag) {
-    L = zsqrt(s);
-    mag = zabs(s);
-    double rm = 1.0 / mag;
+__device__ __forceinline__ double readlane_d(double x, int src) {
+    long long b = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readlane((int)b, src);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/sqrt(m) for finite m > 0: hardware estimate y0 and one third-order step
+// y = y0 (1 + e/2 + 3e^2/8), e = 1 - m y0^2, which cubes the estimate's error
+__device__ __forceinline__ double rsqrt_f64(double m) {
+    double y0 = __builtin_amdgcn_rsq(m);
+    double e = fma(-m * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
+// Pivot of column c: L = principal sqrt(s) (branch cut on the negative real
+// axis, as Fortran's CDSQRT) and 1/L = conj(L)/|s|, from two reciprocal square
+// roots.  bad is set for a zero (or non-finite) pivot: src/chol.f90 error 28.
+__device__ __forceinline__ void pivot_fast(zd s, zd &L, zd &inv, bool &bad) {
+    double m2 = fma(s.re, s.re, s.im * s.im);
+    bad = !(m2 > 0.0) || !(m2 < 1.0e300);
+    double rm = rsqrt_f64(m2);  // 1/|s|
+    double mag = m2 * rm;       // |s|
+    double h = 0.5 * (mag + fabs(s.re));
+    double rh = rsqrt_f64(h);   // 1/t, t = sqrt((|s| + |Re s|)/2)
+    double t = h * rh;
+    double q = 0.5 * fabs(s.im) * rh;  // |Im s|/(2t)
+    bool pos = s.re >= 0.0;
+    L = zd{pos ? t : q, copysign(pos ? q : t, s.im)};
     inv = zd{L.re * rm, -L.im * rm};
 }
+
+__device__ __forceinline__ zd readlane_z(zd v, int src) { return zd{readlane_d(v.re, src), readlane_d(v.im, src)}; }
+__device__ __forceinline__ zd zsel(bool c, zd a, zd b) { return zd{c ? a.re : b.re, c ? a.im : b.im}; }
 
 template <int D>
 struct Cfg {
     static constexpr int W = 1 + D / 2 + (D & 1);  // waves
     static constexpr int NT = 64 * W;               // threads
+};
+
+// diagonals held by wave WV: segment A (diagonal dA, LA positions) and, for
+// 1 <= WV <= D/2, segment B (diagonal dB = D+1-WV, LB = WV positions)
+template <int D, int WV>
+struct WaveCfg {
+    static constexpr bool pair = WV >= 1 && WV <= D / 2;
+    static constexpr int dA = WV == 0 ? 0 : (pair ? WV : (D + 1) / 2);
+    static constexpr int LA = WV == 0 ? D + 1 : (pair ? D + 1 - WV : (D + 1) / 2);
+    static constexpr int dB = pair ? D + 1 - WV : 0;
+    static constexpr int LB = pair ? WV : 0;
 };
 
 }  // namespace
@@ -55,147 +100,179 @@ struct Factor16Args {
     int *status;
 };
 
+namespace {
+
+// LDS of one workgroup; a static layout keeps every address an immediate
 template <int D>
-__global__ void __launch_bounds__(Cfg<D>::NT) crt_factor16_kernel(Factor16Args A) {
-    constexpr int W = Cfg<D>::W;
+struct F16Lds {
+    zd P[(D + 1) * T16 * PS];  // panel tiles of block column J, 16 x PS each
+    zd S[(D + 1) * 256];       // staging tiles of the entering row block, 16 x 16 each
+    zd bring[(D + 2) * T16];   // right-hand-side blocks J .. J+D+1
+    zd col[32];                // current column of the diagonal factor, [16] = y of the column
+    zd pinv[T16];              // reciprocal pivots of the diagonal tile
+    zd yj[T16];                // forward-substituted right-hand side of block J
+};
+
+// CSR rows of block Ie (contiguous, values scaled by crt_assemble_kernel) into
+// the staging tiles; padding rows past sanz get a unit diagonal
+template <int D>
+__device__ __forceinline__ void stage_block(const DevGrid &g, int Ie, int t0, int nthr, const zd *vals,
+                                            const double *fak, int enode, F16Lds<D> &m) {
+    const int qb = g.row_ptr[Ie * T16], qe = g.row_ptr[min(Ie * T16 + T16, g.sanz)];
+    for (int q = qb + t0; q < qe; q += nthr) {
+        int i = g.crow[q], j = g.col[q];
+        m.S[(Ie - j / T16) * 256 + (i - Ie * T16) * T16 + (j % T16)] = vals[q];
+    }
+    if (t0 < T16) {
+        int i = Ie * T16 + t0;
+        if (i >= g.sanz) m.S[t0 * T16 + t0] = zd{1.0, 0.0};
+        zd bv = zd{0.0, 0.0};
+        if (i < g.sanz && i == enode) bv = zd{-fak[i], 0.0};
+        m.bring[(Ie % (D + 2)) * T16 + t0] = bv;
+    }
+}
+
+__device__ __forceinline__ d4 tile_re(const zd *t, int jc, int rq) {
+    return d4{t[rq * T16 + jc].re, t[(4 + rq) * T16 + jc].re, t[(8 + rq) * T16 + jc].re, t[(12 + rq) * T16 + jc].re};
+}
+__device__ __forceinline__ d4 tile_im(const zd *t, int jc, int rq) {
+    return d4{t[rq * T16 + jc].im, t[(4 + rq) * T16 + jc].im, t[(8 + rq) * T16 + jc].im, t[(12 + rq) * T16 + jc].im};
+}
+
+// T -= Pa Pb^T for 16x16 complex tiles in LDS (row stride PS), T in result layout
+__device__ __forceinline__ void mfma_update(d4 &tr, d4 &ti, const zd *Pa, const zd *Pb, int jc, int rq) {
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        zd va = Pa[jc * PS + 4 * kc + rq];
+        zd vb = Pb[jc * PS + 4 * kc + rq];
+        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.re, tr, 0, 0, 0);
+        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(va.im, vb.im, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.im, ti, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.im, vb.re, ti, 0, 0, 0);
+    }
+}
+
+template <int D, int WV>
+__device__ __forceinline__ void factor16_wave(const Factor16Args &A, F16Lds<D> &m) {
+    using WC = WaveCfg<D, WV>;
     constexpr int NT = Cfg<D>::NT;
-    extern __shared__ double smem16[];
+    constexpr int LA = WC::LA, LB = WC::LB, dA = WC::dA, dB = WC::dB;
+    constexpr int LBX = LB > 0 ? LB : 1;
     const DevGrid &g = A.g;
-    const int nblk = g.nblk;  // blocks of 16 rows
+    const int nblk = g.nblk;
     const int s = blockIdx.x;
-    if (s >= A.nsys) return;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int l = A.sys[s].x;
-    const int enode = g.enr[l];
+    const int lane = threadIdx.x & 63, tid = WV * 64 + lane;
+    const int enode = g.enr[A.sys[s].x];
     const double *fak = A.fak + (size_t)s * nblk * T16;
     const zd *vals = A.vals + (size_t)s * g.nnz;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     zd *yout = A.y + (size_t)s * nblk * T16;
-
-    zd *P = reinterpret_cast<zd *>(smem16);        // (D+1) tiles, 16 x PS
-    zd *S = P + (D + 1) * T16 * PS;                // (D+1) staging tiles, 16 x 16
-    zd *bring = S + (D + 1) * 256;                 // (D+2) x 16
-    zd *col = bring + (D + 2) * T16;               // 16
-    zd *pinv = col + T16;                          // 16
-    zd *yj = pinv + T16;                           // 16
-
-    // diagonals of this wave: A segment (positions 0..LA-1 in acc[0..LA-1]),
-    // B segment reversed (position p in acc[D-p], p = 0..LB-1)
-    int dA, LA, dB, LB;
-    if (wave == 0) { dA = 0; LA = D + 1; dB = 0; LB = 0; }
-    else if (wave <= D / 2) { dA = wave; LA = D + 1 - wave; dB = D + 1 - wave; LB = wave; }
-    else { dA = (D + 1) / 2; LA = (D + 1) / 2; dB = 0; LB = 0; }
-
-    d4 ar[D + 1], ai[D + 1];
-#pragma unroll
-    for (int q = 0; q <= D; ++q) {
-        ar[q] = d4{0.0, 0.0, 0.0, 0.0};
-        ai[q] = d4{0.0, 0.0, 0.0, 0.0};
-    }
-    for (int t = tid; t < (D + 1) * 256; t += NT) S[t] = zd{0.0, 0.0};
-    __syncthreads();
-
-    // result-layout coordinates of this lane: column jc, rows 4v + rq
     const int jc = lane & 15, rq = lane >> 4;
+
+    d4 Ar[LA], Ai[LA], Br[LBX], Bi[LBX];
+#pragma unroll
+    for (int q = 0; q < LA; ++q) Ar[q] = Ai[q] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < LBX; ++q) Br[q] = Bi[q] = d4{0.0, 0.0, 0.0, 0.0};
     int err = 0;
 
     for (int J = -(D + 1); J < nblk; ++J) {
         const int Ie = J + D + 1;  // entering row block
         // ------------------------------------------------ A: panel tiles -> LDS
         if (J >= 0) {
-            zd *pa = P + dA * T16 * PS;
+            zd *pa = m.P + dA * T16 * PS;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zd{ar[0][v], ai[0][v]};
-            if (LB > 0) {
-                zd *pb = P + dB * T16 * PS;
+            for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zd{Ar[0][v], Ai[0][v]};
+            if constexpr (LB > 0) {
+                zd *pb = m.P + dB * T16 * PS;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zd{ar[D][v], ai[D][v]};
+                for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zd{Br[0][v], Bi[0][v]};
             }
         }
         __syncthreads();
-        // ------------------- B: wave 0 factors the diagonal tile | others stage Ie
-        if (wave == 0 && J >= 0) {
+        // ---------------- B: wave 0 factors the diagonal tile | others stage Ie
+        if (WV == 0 && J >= 0) {
+            // right-looking factorisation of the diagonal tile in the result
+            // layout; column c travels through LDS (col), the next pivot is formed
+            // from two lane reads so the pivot chain does not wait for the exchange
             zd a[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = P[(4 * v + rq) * PS + jc];
-            zd yv = bring[(J % (D + 2)) * T16 + (lane & 15)];
+            for (int v = 0; v < 4; ++v) a[v] = m.P[(4 * v + rq) * PS + jc];
+            // lanes of column group jc carry the right-hand side entry jc
+            zd yv = m.bring[(J % (D + 2)) * T16 + jc];
+            zd pnext = readlane_z(a[0], 0);
 #pragma unroll
             for (int c = 0; c < T16; ++c) {
-                zd sp = shfl_z16(a[c / 4], c + 16 * (c % 4));
                 zd Lc, iv;
-                double mag;
-                pivot_inv(sp, Lc, iv, mag);
-                if (!(mag > 0.0)) err = CRT_ERR_PIVOT;
+                bool bad;
+                pivot_fast(pnext, Lc, iv, bad);
+                if (bad) err = CRT_ERR_PIVOT;
+                zd cv[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cv[v] = zsel(4 * v + rq == c, Lc, zmul(a[v], iv));
+                const zd yc = zmul(yv, iv);
+                if (c + 1 < T16) {
+                    const int n = c + 1;
+                    zd dn = readlane_z(a[n / 4], n + 16 * (n % 4));  // a(n, n)
+                    zd ln = readlane_z(cv[n / 4], c + 16 * (n % 4)); // L(n, c)
+                    pnext = zfms(dn, ln, ln);
+                }
                 if (jc == c) {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        int i = 4 * v + rq;
-                        if (i == c) a[v] = Lc;
-                        else if (i > c) a[v] = zmul(a[v], iv);
-                        if (i >= c) col[i] = a[v];
+                    for (int v = 0; v < 4; ++v) m.col[4 * v + rq] = cv[v];
+                    if (rq == 0) {
+                        m.col[16] = yc;
+                        m.pinv[c] = iv;
                     }
                 }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) a[v] = zsel(jc == c, cv[v], a[v]);
+                yv = zsel(jc == c, yc, yv);
                 __builtin_amdgcn_wave_barrier();
-                zd ljc = col[jc];
+                const zd ljc = m.col[jc], ycol = m.col[16];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    int i = 4 * v + rq;
-                    if (jc > c && i >= jc) a[v] = zfms(a[v], col[i], ljc);
+                    const int i = 4 * v + rq;
+                    a[v] = zsel(jc > c && i >= jc, zfms(a[v], m.col[i], ljc), a[v]);
                 }
-                // forward substitution of the right-hand side (lanes 0..15 hold y(lane))
-                zd yc = zmul(shfl_z16(yv, c), iv);
-                if ((lane & 15) == c) yv = yc;
-                if ((lane & 15) > c) yv = zfms(yv, col[lane & 15], yc);
-                if (lane == 0) pinv[c] = iv;
+                yv = zsel(jc > c, zfms(yv, ljc, ycol), yv);
                 __builtin_amdgcn_wave_barrier();
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 int i = 4 * v + rq;
-                P[i * PS + jc] = (jc <= i) ? a[v] : zd{0.0, 0.0};
+                m.P[i * PS + jc] = (jc <= i) ? a[v] : zd{0.0, 0.0};
             }
-            if (lane < T16) {
-                yj[lane] = yv;
-                bring[(J % (D + 2)) * T16 + lane] = yv;
-                yout[(size_t)J * T16 + lane] = yv;
+            if (rq == 0) {
+                m.yj[jc] = yv;
+                m.bring[(J % (D + 2)) * T16 + jc] = yv;
+                yout[(size_t)J * T16 + jc] = yv;
             }
-        } else if (Ie < nblk && (wave != 0 || J < 0)) {
-            // CSR entries of row block Ie are contiguous; scaled values from crt_assemble_kernel
-            const int t0 = (J < 0) ? tid : tid - 64, nthr = (J < 0) ? NT : NT - 64;
-            const int qb = g.row_ptr[Ie * T16], qe = g.row_ptr[min(Ie * T16 + T16, g.sanz)];
-            for (int q = qb + t0; q < qe; q += nthr) {
-                int i = g.crow[q], j = g.col[q];
-                S[(Ie - j / T16) * 256 + (i - Ie * T16) * T16 + (j % T16)] = vals[q];
-            }
-            if (t0 < T16) {
-                int i = Ie * T16 + t0;
-                if (i >= g.sanz) S[t0 * T16 + t0] = zd{1.0, 0.0};  // padding rows: unit diagonal
-                zd bv = zd{0.0, 0.0};
-                if (i < g.sanz && i == enode) bv = zd{-fak[i], 0.0};
-                bring[(Ie % (D + 2)) * T16 + t0] = bv;
-            }
+        } else if (Ie < nblk) {
+            if (J < 0) stage_block<D>(g, Ie, tid, NT, vals, fak, enode, m);
+            else stage_block<D>(g, Ie, tid - 64, NT - 64, vals, fak, enode, m);
         }
         __syncthreads();
-        // ------------------------------- C: TRSM X = A L^-T of the panel, RHS update
-        if (J >= 0 && tid < T16 * D) {
-            const int a = 1 + tid / T16, r = tid % T16;
-            if (J + a < nblk) {
-                zd *row = P + (a * T16 + r) * PS;
+        // ------------------ C: panel rows X = A L^-T (waves 1..), RHS update
+        if (WV >= 1 && J >= 0) {
+            const int t = tid - 64;
+            const int a = 1 + t / T16, r = t % T16;
+            if (t < T16 * D && J + a < nblk) {
+                zd *row = m.P + (a * T16 + r) * PS;
                 zd x[T16];
 #pragma unroll
                 for (int c = 0; c < T16; ++c) x[c] = row[c];
-                zd bacc = bring[((J + a) % (D + 2)) * T16 + r];
+                zd bacc = m.bring[((J + a) % (D + 2)) * T16 + r];
 #pragma unroll
                 for (int c = 0; c < T16; ++c) {
-                    zd sacc = x[c];
+                    x[c] = zmul(x[c], m.pinv[c]);
+                    bacc = zfms(bacc, x[c], m.yj[c]);
 #pragma unroll
-                    for (int cp = 0; cp < c; ++cp) sacc = zfms(sacc, x[cp], P[c * PS + cp]);
-                    x[c] = zmul(sacc, pinv[c]);
-                    bacc = zfms(bacc, x[c], yj[c]);
+                    for (int i = c + 1; i < T16; ++i) x[i] = zfms(x[i], x[c], m.P[i * PS + c]);
                 }
 #pragma unroll
                 for (int c = 0; c < T16; ++c) row[c] = x[c];
-                bring[((J + a) % (D + 2)) * T16 + r] = bacc;
+                m.bring[((J + a) % (D + 2)) * T16 + r] = bacc;
             }
         }
         __syncthreads();
@@ -204,80 +281,72 @@ __global__ void __launch_bounds__(Cfg<D>::NT) crt_factor16_kernel(Factor16Args A
             zd *pj = panel + (size_t)J * (D + 1) * 256;
             for (int t = tid; t < (D + 1) * 256; t += NT) {
                 int a = t >> 8, rc = t & 255;
-                pj[t] = (J + a < nblk) ? P[(a * T16 + (rc >> 4)) * PS + (rc & 15)] : zd{0.0, 0.0};
+                pj[t] = (J + a < nblk) ? m.P[(a * T16 + (rc >> 4)) * PS + (rc & 15)] : zd{0.0, 0.0};
             }
 #pragma unroll
-            for (int q = 0; q <= D; ++q) {
-                int d, p;
-                bool live;
-                if (q < LA) { d = dA; p = q; live = true; }
-                else { d = dB; p = D - q; live = (LB > 0); }
-                live = live && p >= 1 && (J + p + d < nblk);
-                if (live) {
-                    const zd *Pa = P + (p + d) * T16 * PS;  // L(J+p+d, J)
-                    const zd *Pb = P + p * T16 * PS;        // L(J+p, J)
+            for (int p = 1; p < LA; ++p)
+                if (J + p + dA < nblk)
+                    mfma_update(Ar[p], Ai[p], m.P + (p + dA) * T16 * PS, m.P + p * T16 * PS, jc, rq);
+            if constexpr (LB > 0) {
 #pragma unroll
-                    for (int kc = 0; kc < 4; ++kc) {
-                        zd va = Pa[jc * PS + 4 * kc + rq];
-                        zd vb = Pb[jc * PS + 4 * kc + rq];
-                        ar[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.re, ar[q], 0, 0, 0);
-                        ar[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.im, vb.im, ar[q], 0, 0, 0);
-                        ai[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.im, ai[q], 0, 0, 0);
-                        ai[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.im, vb.re, ai[q], 0, 0, 0);
-                    }
-                }
+                for (int p = 1; p < LB; ++p)
+                    if (J + p + dB < nblk)
+                        mfma_update(Br[p], Bi[p], m.P + (p + dB) * T16 * PS, m.P + p * T16 * PS, jc, rq);
             }
         }
-        // slide: position p -> p-1 on both diagonals; the entering tiles take the last slots
+        // slide: position p -> p-1; the entering tiles take the last positions
+#pragma unroll
+        for (int p = 0; p + 1 < LA; ++p) { Ar[p] = Ar[p + 1]; Ai[p] = Ai[p + 1]; }
         {
-            zd na[4], nb[4];
-            const zd *sa = S + dA * 256;
-            const zd *sb = S + dB * 256;
+            zd *sa = m.S + dA * 256;
+            Ar[LA - 1] = tile_re(sa, jc, rq);
+            Ai[LA - 1] = tile_im(sa, jc, rq);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                na[v] = sa[(4 * v + rq) * T16 + jc];
-                nb[v] = (LB > 0) ? sb[(4 * v + rq) * T16 + jc] : zd{0.0, 0.0};
-            }
-            d4 orr[D + 1], oii[D + 1];
+            for (int v = 0; v < 4; ++v) sa[(4 * v + rq) * T16 + jc] = zd{0.0, 0.0};
+        }
+        if constexpr (LB > 0) {
 #pragma unroll
-            for (int q = 0; q <= D; ++q) { orr[q] = ar[q]; oii[q] = ai[q]; }
+            for (int p = 0; p + 1 < LB; ++p) { Br[p] = Br[p + 1]; Bi[p] = Bi[p + 1]; }
+            zd *sb = m.S + dB * 256;
+            Br[LB - 1] = tile_re(sb, jc, rq);
+            Bi[LB - 1] = tile_im(sb, jc, rq);
 #pragma unroll
-            for (int q = 0; q <= D; ++q) {
-                d4 nr, ni;
-                if (q < LA - 1) { nr = orr[q + 1 <= D ? q + 1 : D]; ni = oii[q + 1 <= D ? q + 1 : D]; }
-                else if (q == LA - 1) { nr = d4{na[0].re, na[1].re, na[2].re, na[3].re}; ni = d4{na[0].im, na[1].im, na[2].im, na[3].im}; }
-                else if (q == LA) { nr = d4{nb[0].re, nb[1].re, nb[2].re, nb[3].re}; ni = d4{nb[0].im, nb[1].im, nb[2].im, nb[3].im}; }
-                else { nr = orr[q >= 1 ? q - 1 : 0]; ni = oii[q >= 1 ? q - 1 : 0]; }
-                ar[q] = nr;
-                ai[q] = ni;
-            }
-            // the staging tiles of this wave are consumed: clear them for the next block
-            zd *wa = S + dA * 256;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) wa[(4 * v + rq) * T16 + jc] = zd{0.0, 0.0};
-            if (LB > 0) {
-                zd *wb = S + dB * 256;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) wb[(4 * v + rq) * T16 + jc] = zd{0.0, 0.0};
-            }
+            for (int v = 0; v < 4; ++v) sb[(4 * v + rq) * T16 + jc] = zd{0.0, 0.0};
         }
         __syncthreads();
     }
-    if (wave == 0 && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+    if (WV == 0 && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+
+template <int D, int WV>
+__device__ __forceinline__ void factor16_dispatch(int wave, const Factor16Args &A, F16Lds<D> &m) {
+    if constexpr (WV < Cfg<D>::W) {
+        if (wave == WV) factor16_wave<D, WV>(A, m);
+        else factor16_dispatch<D, WV + 1>(wave, A, m);
+    }
+}
+
+}  // namespace
+
+template <int D>
+__global__ void __launch_bounds__(Cfg<D>::NT, 2) crt_factor16_kernel(Factor16Args A) {
+    __shared__ F16Lds<D> m;
+    if ((int)blockIdx.x >= A.nsys) return;
+    for (int t = threadIdx.x; t < (D + 1) * 256; t += Cfg<D>::NT) m.S[t] = zd{0.0, 0.0};
+    __syncthreads();
+    factor16_dispatch<D, 0>(threadIdx.x >> 6, A, m);
 }
 
 size_t crt_factor16_lds_bytes(int D) {
-    return ((size_t)(D + 1) * T16 * PS + (size_t)(D + 1) * 256 + (size_t)(D + 2) * T16 + 3 * T16) * sizeof(zd);
+    return ((size_t)(D + 1) * T16 * PS + (size_t)(D + 1) * 256 + (size_t)(D + 2) * T16 + 64) * sizeof(zd);
 }
+static_assert(sizeof(F16Lds<6>) == 62208, "LDS layout of the 16x16 factorisation");
 
 #define CRT_F16_MAXD 10
 
 template <int D>
 static int launch16(hipStream_t st, const Factor16Args &a) {
-    size_t lds = crt_factor16_lds_bytes(D);
-    (void)hipFuncSetAttribute((const void *)crt_factor16_kernel<D>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    hipLaunchKernelGGL(crt_factor16_kernel<D>, dim3(a.nsys), dim3(Cfg<D>::NT), lds, st, a);
+    hipLaunchKernelGGL(crt_factor16_kernel<D>, dim3(a.nsys), dim3(Cfg<D>::NT), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
