@@ -42,10 +42,10 @@ def _cutmck(grid: G.Grid, cutmck: Optional[Callable] = None) -> G.Grid:
 
 
 def quad_workload(name: str, nx: int, nz: int, dx: float, dz: float, nelec: int, elec_every: int,
-                  elec_first: int, quads, sigma_fn, cutmck=None) -> Workload:
+                  elec_first: int, quads, sigma_fn, cutmck=None, mixed: bool = True) -> Workload:
     n1 = nx + 1
     elec_nodes = [elec_first + i * elec_every + 1 for i in range(nelec)]
-    g = G.quad_grid(nx, nz, dx, dz, elec_nodes=elec_nodes)
+    g = G.quad_grid(nx, nz, dx, dz, elec_nodes=elec_nodes, mixed=mixed)
     cx, cy = g.centroids()
     sigma = sigma_fn(cx, cy)
     g = _cutmck(g, cutmck)
@@ -110,12 +110,14 @@ def c3(cutmck=None, nx: int = 400, nz: int = 100, nelec: int = 96, nmeas: int = 
     return quad_workload("c3", nx, nz, dx, dz, nelec, spacing_nodes, first, quads, sig, cutmck)
 
 
-def small_quad(nx: int = 24, nz: int = 8, nelec: int = 10, cutmck=None) -> Workload:
-    """A small C0-like grid for fast parity tests (mixed boundaries, type-8 quads)."""
+def small_quad(nx: int = 24, nz: int = 8, nelec: int = 10, cutmck=None, mixed: bool = True) -> Workload:
+    """A small C0-like grid for fast parity tests (type-8 quads; mixed boundaries
+    unless ``mixed=False``, which leaves every boundary no-flow)."""
     def sig(cx, cy):
         rho = 30.0 + 70.0 * (cx / cx.max())
         blk = (cx > 0.3 * cx.max()) & (cx < 0.5 * cx.max()) & (cy < -1.0)
         rho[blk] = 5.0
         return G.sigma_from_rho(rho, -5.0 - 10.0 * (cx / cx.max()))
     quads = G.dipole_dipole(nelec, [1, 2], range(1, 5))
-    return quad_workload("small_quad", nx, nz, 0.5, 0.5, nelec, 2, 2, quads, sig, cutmck)
+    return quad_workload("small_quad" if mixed else "small_quad_noflow", nx, nz, 0.5, 0.5, nelec, 2, 2, quads,
+                         sig, cutmck, mixed)

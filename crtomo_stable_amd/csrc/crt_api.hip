@@ -24,9 +24,9 @@ int crt_launch_panel_band(hipStream_t, int, int, int, int, int, zd *, zd *);
 int crt_launch_factor_band(hipStream_t, const DevGrid &, const int2 *, const zd *, zd *, zd *, int *);
 int crt_factor16_supported(int mb);
 int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zd *, zd *,
-                        int *);
+                        zd *, int *);
 int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
-                       const zd *, const zd *, zd *, zd *);
+                       const zd *, const zd *, const zd *, zd *, zd *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
@@ -119,7 +119,7 @@ struct crt_ctx {
     DBuf<int4> d_meas, d_rhs;
     DBuf<int2> d_sys;
     DBuf<double> d_x, d_y, d_kwn, d_kwnwi, d_elbg, d_relbg, d_kg, d_fak, d_dga, d_kfak, d_strom;
-    DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp, d_vals;
+    DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_dinv, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp, d_vals;
     // normal-equation state
     int ltri = -1, rnx = 0, rncol = 0, sens_nanz = 0, sens_manz = 0;
     DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
@@ -565,11 +565,12 @@ int crt_forward(crt_ctx *c) {
     }
     const size_t fstride = (size_t)nblk * NBt;
     const size_t panel_elems = (size_t)nblk * (D + 1) * NBt * NBt;
-    size_t per_sys = panel_elems * sizeof(zd) + fstride * (sizeof(zd) + 2 * sizeof(double)) +
+    size_t per_sys = panel_elems * sizeof(zd) + fstride * (2 * sizeof(zd) + 2 * sizeof(double)) +
                      (size_t)c->nnz * sizeof(zd);
     size_t batch = std::max<size_t>(1, std::min(sys.size(), c->panel_budget / per_sys));
     int e = c->d_panels.ensure(batch * panel_elems);
     e = e ? e : c->d_yf.ensure(batch * fstride);
+    e = e ? e : c->d_dinv.ensure(batch * fstride);
     e = e ? e : c->d_fak.ensure(batch * fstride);
     e = e ? e : c->d_dga.ensure(batch * fstride);
     e = e ? e : c->d_vals.ensure(batch * c->nnz);
@@ -603,7 +604,7 @@ int crt_forward(crt_ctx *c) {
         c->tic(T_FACTOR);
         if (use16)
             e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
-                                    c->d_yf.p, c->d_status.p + 1 + s0);
+                                    c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0);
         else
             e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
                                   c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
@@ -612,7 +613,7 @@ int crt_forward(crt_ctx *c) {
         c->tic(T_SOLVE);
         if (use16)
             e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                   c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
+                                   c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
         else
             e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
                                  c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr, nullptr);

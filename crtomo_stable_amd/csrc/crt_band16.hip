@@ -97,6 +97,7 @@ struct Factor16Args {
     const zd *vals;     // [sys][nnz]
     zd *panels;         // [sys][nblk][D+1][16][16]
     zd *y;              // [sys][nblk*16]
+    zd *dinv;           // [sys][nblk*16] reciprocal pivots 1/L(i,i)
     int *status;
 };
 
@@ -248,6 +249,8 @@ __device__ __forceinline__ void factor16_wave(const Factor16Args &A, F16Lds<D> &
                 m.bring[(J % (D + 2)) * T16 + jc] = yv;
                 yout[(size_t)J * T16 + jc] = yv;
             }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < T16) A.dinv[(size_t)s * nblk * T16 + (size_t)J * T16 + lane] = m.pinv[lane];
         } else if (Ie < nblk) {
             if (J < 0) stage_block<D>(g, Ie, tid, NT, vals, fak, enode, m);
             else stage_block<D>(g, Ie, tid - 64, NT - 64, vals, fak, enode, m);
@@ -356,8 +359,8 @@ int crt_factor16_supported(int mb) {
 }
 
 int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2 *sys, const double *fak,
-                        const zd *vals, zd *panels, zd *y, int *status) {
-    Factor16Args a{g16, nsys, sys, fak, vals, panels, y, status};
+                        const zd *vals, zd *panels, zd *y, zd *dinv, int *status) {
+    Factor16Args a{g16, nsys, sys, fak, vals, panels, y, dinv, status};
     switch (g16.D) {
         case 1: return launch16<1>(st, a);
         case 2: return launch16<2>(st, a);
@@ -374,44 +377,168 @@ int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2
 }
 
 // ------------------------------------------------------------------- solve
-// One wavefront per right-hand side on 16x16 panels.  Lane l holds entries
-// (r, c) = (l/16 + 4v, l%16), v = 0..3, of a tile.
+// One workgroup (4 waves) per right-hand side.  Per block the off-diagonal
+// tiles L(J, J-a) (forward) or L(J+a, J)^T (backward), a = 1..D, are split over
+// waves 1..3, each prefetching its tiles for the next block while the current
+// one is reduced; wave 0 substitutes through the diagonal tile with the
+// reciprocal pivots of the factorisation (no divisions).  The forward pass
+// starts at the electrode's block (the right-hand side is zero above it) and is
+// skipped when the factorisation already produced y.  Lanes hold tile entries
+// (r, c) = (l/16 + 4v, l%16), v = 0..3.
 struct Solve16Args {
-    int sanz, nblk, D, kwnanz, eanz, nrhs;
+    int sanz, nblk, kwnanz, eanz, nrhs;
     const int4 *rhs;     // (l, k, factor system, have_y)
     const int *enr;
     const double *fak;
     const zd *panels;
+    const zd *dinv;
     const zd *y;
     zd *yscr;
     zd *kpotN;
 };
 
-__global__ void __launch_bounds__(256) crt_solve16_kernel(Solve16Args A) {
-    extern __shared__ double smem_s16[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rid = blockIdx.x * (blockDim.x >> 6) + wave;
+namespace {
+constexpr int SW = 4;  // waves per right-hand side
+
+template <int D>
+__device__ __forceinline__ void solve16_fetch_diag(const zd *panel, const zd *dinv, int nblk, int J, int c,
+                                                   zd (&drow)[T16], zd &pnext) {
+    const int Jc = J < 0 ? 0 : (J < nblk ? J : nblk - 1);
+    const zd *Ld = panel + (size_t)Jc * (D + 1) * 256 + c * T16;
+#pragma unroll
+    for (int cc = 0; cc < T16; ++cc) drow[cc] = zd{Ld[cc].re, Ld[cc].im};
+    const zd *pv = dinv + (size_t)Jc * T16 + c;
+    pnext = zd{pv->re, pv->im};
+}
+
+__device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd (&drow)[T16]) {
+    if (lane < T16)
+#pragma unroll
+        for (int cc = 0; cc < T16; ++cc) {
+            dtile[lane * PS + cc].re = drow[cc].re;
+            dtile[lane * PS + cc].im = drow[cc].im;
+        }
+}
+
+template <int D>
+__global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(Solve16Args A) {
+    constexpr int NA = (D + SW - 2) / (SW - 1);  // off-diagonal tiles per worker wave
+    __shared__ zd ring[(D + 1) * T16];
+    __shared__ zd part[SW][T16];
+    __shared__ zd dtile[T16 * PS];  // diagonal tile of the current block
+    const int rid = blockIdx.x;
     if (rid >= A.nrhs) return;
-    const int D = A.D, nblk = A.nblk;
-    const int4 rq4 = A.rhs[rid];
-    const int l = rq4.x, k = rq4.y, fs = rq4.z;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = lane & 15, rq = lane >> 4;
+    const int nblk = A.nblk;
+    const int4 q4 = A.rhs[rid];
+    const int l = q4.x, k = q4.y, fs = q4.z;
     const double *fak = A.fak + (size_t)fs * nblk * T16;
     const zd *panel = A.panels + (size_t)fs * nblk * (D + 1) * 256;
-    zd *ring = reinterpret_cast<zd *>(smem_s16) + (size_t)wave * ((D + 1) * T16 + 256);
-    zd *tile = ring + (D + 1) * T16;
-    const int c = lane & 15, rq = lane >> 4;
+    const zd *dinv = A.dinv + (size_t)fs * nblk * T16;
     const int enode = A.enr[l];
     zd *yscr = A.yscr + (size_t)rid * nblk * T16;
+    const int J0 = q4.w ? 0 : enode / T16;  // y is zero above the electrode's block
 
-    if (!rq4.w) {
-        for (int J = 0; J < nblk; ++J) {
+    // The two wave roles run separate loops with the same barrier sequence, so
+    // their registers are allocated independently.
+    if (wave == 0) {
+        // ---- wave 0: substitution through the diagonal tiles
+        zd pcur, drow[T16], pnext;
+        if (!q4.w) {
+            solve16_fetch_diag<D>(panel, dinv, nblk, J0, c, drow, pnext);
+            solve16_store_diag(dtile, lane, drow);
+            pcur = pnext;
+            for (int J = J0; J < nblk; ++J) {
+                solve16_fetch_diag<D>(panel, dinv, nblk, J + 1, c, drow, pnext);
+                const int r = c;  // lane r = lane & 15 carries row r of the block
+                const int i = J * T16 + r;
+                zd yv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+                __syncthreads();
+#pragma unroll
+                for (int w = 1; w < SW; ++w) yv = zsub(yv, part[w][r]);
+#pragma unroll
+                for (int cc = 0; cc < T16; ++cc) {
+                    const zd ycc = zmul(readlane_z(yv, cc), readlane_z(pcur, cc));
+                    if (r == cc) yv = ycc;
+                    if (r > cc) yv = zfms(yv, dtile[r * PS + cc], ycc);
+                }
+                if (lane < T16) {
+                    ring[(J % (D + 1)) * T16 + lane] = yv;
+                    yscr[(size_t)J * T16 + lane] = yv;
+                }
+                __builtin_amdgcn_wave_barrier();
+                solve16_store_diag(dtile, lane, drow);
+                pcur = pnext;
+                __syncthreads();
+            }
+        }
+        const zd *yg = q4.w ? A.y + (size_t)fs * nblk * T16 : yscr;
+        __syncthreads();  // the forward pass has finished with dtile
+        solve16_fetch_diag<D>(panel, dinv, nblk, nblk - 1, c, drow, pnext);
+        solve16_store_diag(dtile, lane, drow);
+        pcur = pnext;
+        zd ycur = nblk - 1 >= J0 ? yg[(size_t)(nblk - 1) * T16 + c] : zd{0.0, 0.0};
+        for (int J = nblk - 1; J >= 0; --J) {
+            solve16_fetch_diag<D>(panel, dinv, nblk, J - 1, c, drow, pnext);
+            const zd ynxt = J - 1 >= J0 ? yg[(size_t)(J - 1) * T16 + c] : zd{0.0, 0.0};
+            __syncthreads();
+            zd xv = zd{-ycur.re, -ycur.im};  // lane c carries column c of the block
+#pragma unroll
+            for (int w = 1; w < SW; ++w) xv = zsub(xv, part[w][c]);
+#pragma unroll
+            for (int cc = T16 - 1; cc >= 0; --cc) {
+                const zd xcc = zmul(readlane_z(xv, cc), readlane_z(pcur, cc));
+                if (c == cc) xv = xcc;
+                if (c < cc) xv = zfms(xv, dtile[cc * PS + c], xcc);
+            }
+            if (lane < T16) {
+                ring[(J % (D + 1)) * T16 + lane] = xv;
+                const int i = J * T16 + lane;
+                if (i < A.sanz) {
+                    const double f = fak[i];
+                    A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            solve16_store_diag(dtile, lane, drow);
+            pcur = pnext;
+            ycur = ynxt;
+            __syncthreads();
+        }
+        return;
+    }
+
+    // ---- waves 1..3: products with the off-diagonal tiles a = wave + 3u
+    zd cur[NA][4];
+    if (!q4.w) {
+        auto fetch_f = [&](int J, zd (&t)[NA][4]) {
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int a = wave + (SW - 1) * u;
+                const bool live = a <= D && J - a >= J0 && J < nblk;
+                const zd *Lt = panel + ((size_t)(live ? J - a : 0) * (D + 1) + (live ? a : 0)) * 256;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const zd *p = Lt + (rq + 4 * v) * T16 + c;
+                    t[u][v] = live ? zd{p->re, p->im} : zd{0.0, 0.0};
+                }
+            }
+        };
+        fetch_f(J0, cur);
+        for (int J = J0; J < nblk; ++J) {
+            zd nxt[NA][4];
+            fetch_f(J + 1, nxt);
             // acc(r) = sum_a sum_c L(J, J-a)(r, c) y_{J-a}(c)
             zd acc[4] = {zd{0, 0}, zd{0, 0}, zd{0, 0}, zd{0, 0}};
-            for (int a = 1; a <= D && a <= J; ++a) {
-                const zd *Lt = panel + ((size_t)(J - a) * (D + 1) + a) * 256;
-                zd yv = ring[((J - a) % (D + 1)) * T16 + c];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[v] = zfma(acc[v], Lt[(rq + 4 * v) * T16 + c], yv);
+            for (int u = 0; u < NA; ++u) {
+                const int a = wave + (SW - 1) * u;
+                if (a <= D && J - a >= J0) {
+                    const zd yv = ring[((J - a) % (D + 1)) * T16 + c];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[v] = zfma(acc[v], cur[u][v], yv);
+                }
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -419,79 +546,79 @@ __global__ void __launch_bounds__(256) crt_solve16_kernel(Solve16Args A) {
                 acc[v] = zadd(acc[v], shfl_xor_z16(acc[v], 2));
                 acc[v] = zadd(acc[v], shfl_xor_z16(acc[v], 4));
                 acc[v] = zadd(acc[v], shfl_xor_z16(acc[v], 8));
+                if (c == 0) part[wave][rq + 4 * v] = acc[v];
             }
-            // diagonal tile into LDS, rhs of row r into lane r
-            const zd *Ld = panel + (size_t)J * (D + 1) * 256;
+            __syncthreads();
+            __syncthreads();
 #pragma unroll
-            for (int v = 0; v < 4; ++v) tile[(rq + 4 * v) * T16 + c] = Ld[(rq + 4 * v) * T16 + c];
-            // row r = rq + 4v is held (after the reduction) by every lane with lane>>4 == rq
-            zd rv = zd{0.0, 0.0};
+            for (int u = 0; u < NA; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                zd t = shfl_z16(acc[v], 16 * ((lane & 15) % 4));
-                if ((lane & 15) / 4 == v) rv = t;
-            }
-            int i = J * T16 + (lane & 15);
-            zd bv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
-            zd yv = zsub(bv, rv);
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int cc = 0; cc < T16; ++cc) {
-                zd ycc = zdiv(shfl_z16(yv, cc), tile[cc * T16 + cc]);
-                if ((lane & 15) == cc) yv = ycc;
-                if ((lane & 15) > cc) yv = zfms(yv, tile[(lane & 15) * T16 + cc], ycc);
-            }
-            if (lane < T16) {
-                ring[(J % (D + 1)) * T16 + lane] = yv;
-                yscr[(size_t)J * T16 + lane] = yv;
-            }
-            __builtin_amdgcn_wave_barrier();
+                for (int v = 0; v < 4; ++v) cur[u][v] = nxt[u][v];
         }
     }
-    const zd *yg = rq4.w ? A.y + (size_t)fs * nblk * T16 : yscr;
+    auto fetch_b = [&](int J, zd (&t)[NA][4]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int a = wave + (SW - 1) * u;
+            const bool live = a <= D && J >= 0 && J + a < nblk;
+            const zd *Lt = panel + ((size_t)(live ? J : 0) * (D + 1) + (live ? a : 0)) * 256;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const zd *p = Lt + (rq + 4 * v) * T16 + c;
+                t[u][v] = live ? zd{p->re, p->im} : zd{0.0, 0.0};
+            }
+        }
+    };
+    fetch_b(nblk - 1, cur);
+    __syncthreads();
     for (int J = nblk - 1; J >= 0; --J) {
+        zd nxt[NA][4];
+        fetch_b(J - 1, nxt);
         // acc(c) = sum_a sum_r L(J+a, J)(r, c) x_{J+a}(r)
         zd acc = zd{0.0, 0.0};
-        for (int a = 1; a <= D && J + a < nblk; ++a) {
-            const zd *Lt = panel + ((size_t)J * (D + 1) + a) * 256;
-            const zd *xr = ring + ((J + a) % (D + 1)) * T16;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc = zfma(acc, Lt[(rq + 4 * v) * T16 + c], xr[rq + 4 * v]);
+        for (int u = 0; u < NA; ++u) {
+            const int a = wave + (SW - 1) * u;
+            if (a <= D && J + a < nblk) {
+                const zd *xr = ring + ((J + a) % (D + 1)) * T16;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc = zfma(acc, cur[u][v], xr[rq + 4 * v]);
+            }
         }
         acc = zadd(acc, shfl_xor_z16(acc, 16));
         acc = zadd(acc, shfl_xor_z16(acc, 32));
-        const zd *Ld = panel + (size_t)J * (D + 1) * 256;
+        if (rq == 0) part[wave][c] = acc;
+        __syncthreads();
+        __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 4; ++v) tile[(rq + 4 * v) * T16 + c] = Ld[(rq + 4 * v) * T16 + c];
-        zd yJ = yg[(size_t)J * T16 + c];
-        zd xv = zd{-yJ.re - acc.re, -yJ.im - acc.im};  // lanes hold the rhs of column lane&15
-        __builtin_amdgcn_wave_barrier();
+        for (int u = 0; u < NA; ++u)
 #pragma unroll
-        for (int cc = T16 - 1; cc >= 0; --cc) {
-            zd xcc = zdiv(shfl_z16(xv, cc), tile[cc * T16 + cc]);
-            if (c == cc) xv = xcc;
-            if (c < cc) xv = zfms(xv, tile[cc * T16 + c], xcc);
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (lane < T16) {
-            ring[(J % (D + 1)) * T16 + lane] = xv;
-            int i = J * T16 + lane;
-            if (i < A.sanz) {
-                double f = fak[i];
-                A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
+            for (int v = 0; v < 4; ++v) cur[u][v] = nxt[u][v];
     }
 }
 
-int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs, const int4 *rhs,
-                       const int *enr, const double *fak, const zd *panels, const zd *y, zd *yscr, zd *kpotN) {
-    Solve16Args a{sanz, nblk, D, kwnanz, eanz, nrhs, rhs, enr, fak, panels, y, yscr, kpotN};
-    const int waves = 4;
-    size_t lds = (size_t)waves * ((D + 1) * T16 + 256) * sizeof(zd);
-    (void)hipFuncSetAttribute((const void *)crt_solve16_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    hipLaunchKernelGGL(crt_solve16_kernel, dim3((nrhs + waves - 1) / waves), dim3(64 * waves), lds, st, a);
+template <int D>
+int launch_solve16(hipStream_t st, const Solve16Args &a) {
+    hipLaunchKernelGGL(crt_solve16_kernel<D>, dim3(a.nrhs), dim3(64 * SW), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+}  // namespace
+
+int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs, const int4 *rhs,
+                       const int *enr, const double *fak, const zd *panels, const zd *dinv, const zd *y, zd *yscr,
+                       zd *kpotN) {
+    Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN};
+    switch (D) {
+        case 1: return launch_solve16<1>(st, a);
+        case 2: return launch_solve16<2>(st, a);
+        case 3: return launch_solve16<3>(st, a);
+        case 4: return launch_solve16<4>(st, a);
+        case 5: return launch_solve16<5>(st, a);
+        case 6: return launch_solve16<6>(st, a);
+        case 7: return launch_solve16<7>(st, a);
+        case 8: return launch_solve16<8>(st, a);
+        case 9: return launch_solve16<9>(st, a);
+        case 10: return launch_solve16<10>(st, a);
+        default: return CRT_ERR_UNSUPPORTED;
+    }
 }

@@ -94,12 +94,66 @@ struct SensArgs {
     zd *sens;              // (nanz, manz) column-major
 };
 
+// Per (measurement, wavenumber) the 4 x nk potentials of the element's nodes
+// are read once: u = phiB - phiA, v = phiN - phiM, and the bilinear form
+// h = sum_m v_m sum_n elbg(m,n) u_n runs on registers (elbg from LDS broadcast).
+template <int NK>
+__device__ __forceinline__ void sens_element(const SensArgs &A, int e, const zd *pot, const double *eb) {
+    constexpr int NP = NK * (NK + 1) / 2;
+    const int K = A.K, E = A.eanz;
+    const double pi = acos(-1.0);
+    for (int i = threadIdx.x; i < A.nanz; i += blockDim.x) {
+        const int4 m = A.meas[i];  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
+        const int ia = max(m.x, 1) - 1, ib = max(m.y, 1) - 1, im = max(m.z, 1) - 1, in = max(m.w, 1) - 1;
+        zd dum = zd{0.0, 0.0};
+        for (int k = 0; k < K; ++k) {
+            zd u[NK], v[NK];
+#pragma unroll
+            for (int n = 0; n < NK; ++n) {
+                const zd *pk = pot + ((size_t)n * K + k) * E;
+                zd pa = pk[ia], pb = pk[ib], pm = pk[im], pn = pk[in];
+                if (m.x <= 0) pa = zd{0.0, 0.0};
+                if (m.y <= 0) pb = zd{0.0, 0.0};
+                if (m.z <= 0) pm = zd{0.0, 0.0};
+                if (m.w <= 0) pn = zd{0.0, 0.0};
+                u[n] = zsub(pb, pa);
+                v[n] = zsub(pn, pm);
+            }
+            const double *w = eb + k * NP;
+            zd h = zd{0.0, 0.0};
+#pragma unroll
+            for (int mi = 0; mi < NK; ++mi) {
+                zd t = zd{0.0, 0.0};
+#pragma unroll
+                for (int mj = 0; mj < NK; ++mj) {
+                    const int hi = mi > mj ? mi : mj, lo = mi > mj ? mj : mi;
+                    const double wm = w[hi * (hi + 1) / 2 + lo];
+                    t.re = fma(wm, u[mj].re, t.re);
+                    t.im = fma(wm, u[mj].im, t.im);
+                }
+                h = zfma(h, v[mi], t);
+            }
+            dum.re = fma(h.re, A.kwnwi[k], dum.re);
+            dum.im = fma(h.im, A.kwnwi[k], dum.im);
+        }
+        zd out;
+        if (A.mode == 1) {
+            zd t = zd{dum.re / pi, dum.im / pi};
+            out = zdiv(zmul(t, A.sigma[e]), A.volt[i]);
+        } else {
+            double f = A.strom ? A.strom[i] / pi : 1.0 / pi;
+            out = zd{-(dum.re * f), -(dum.im * f)};
+        }
+        A.sens[(size_t)e * A.nanz + i] = out;
+    }
+}
+
 __global__ void __launch_bounds__(256) crt_sens_kernel(SensArgs A) {
     extern __shared__ double smem_v[];
     const int e = blockIdx.x;
     if (e >= A.elanz) return;
     const int K = A.K, E = A.eanz, nk = A.nkel[e];
-    zd *pot = reinterpret_cast<zd *>(smem_v);                    // [m][k][l]
+    zd *pot = reinterpret_cast<zd *>(smem_v);                           // [m][k][l]
     double *eb = reinterpret_cast<double *>(pot + (size_t)nk * K * E);  // [k][imn]
     const int per_node = K * E;
     for (int q = threadIdx.x; q < nk * per_node; q += blockDim.x) {
@@ -113,42 +167,8 @@ __global__ void __launch_bounds__(256) crt_sens_kernel(SensArgs A) {
         eb[q] = A.elbg[((size_t)k * A.npk + imn) * A.elanz + e];
     }
     __syncthreads();
-    const double pi = acos(-1.0);
-    for (int i = threadIdx.x; i < A.nanz; i += blockDim.x) {
-        int4 m = A.meas[i];  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
-        zd dum = zd{0.0, 0.0};
-        for (int k = 0; k < K; ++k) {
-            zd h = zd{0.0, 0.0};
-            const zd *pk = pot + (size_t)k * E;
-            for (int mi = 0; mi < nk; ++mi) {
-                const zd *pmi = pk + (size_t)mi * per_node;
-                zd s3 = m.z > 0 ? pmi[m.z - 1] : zd{0.0, 0.0};
-                zd s4 = m.w > 0 ? pmi[m.w - 1] : zd{0.0, 0.0};
-                zd dm = zsub(s4, s3);
-                for (int mj = 0; mj < nk; ++mj) {
-                    const zd *pmj = pk + (size_t)mj * per_node;
-                    zd s1 = m.x > 0 ? pmj[m.x - 1] : zd{0.0, 0.0};
-                    zd s2 = m.y > 0 ? pmj[m.y - 1] : zd{0.0, 0.0};
-                    int ia = mi > mj ? mi : mj, ib = mi > mj ? mj : mi;
-                    double w = eb[k * np + ia * (ia + 1) / 2 + ib];
-                    zd d = zmul(zsub(s2, s1), dm);
-                    h.re = h.re + w * d.re;
-                    h.im = h.im + w * d.im;
-                }
-            }
-            dum.re = dum.re + h.re * A.kwnwi[k];
-            dum.im = dum.im + h.im * A.kwnwi[k];
-        }
-        zd out;
-        if (A.mode == 1) {
-            zd t = zd{dum.re / pi, dum.im / pi};
-            out = zdiv(zmul(t, A.sigma[e]), A.volt[i]);
-        } else {
-            double f = A.strom ? A.strom[i] / pi : 1.0 / pi;
-            out = zd{-(dum.re * f), -(dum.im * f)};
-        }
-        A.sens[(size_t)e * A.nanz + i] = out;
-    }
+    if (nk == 3) sens_element<3>(A, e, pot, eb);
+    else sens_element<4>(A, e, pot, eb);
 }
 
 // kpot in the reference layout kpot(sanz, eanz, kwnanz): out[(k*eanz + l)*sanz + j]

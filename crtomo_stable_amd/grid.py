@@ -255,12 +255,16 @@ def read_volt(path: str):
 
 # ----------------------------------------------------------- synthetic grids
 def quad_grid(nx: int, nz: int, dx: float, dz: float, x0: float = 0.0,
-              elec_nodes: Optional[Sequence[int]] = None) -> Grid:
+              elec_nodes: Optional[Sequence[int]] = None, mixed: bool = True) -> Grid:
     """Regular quad grid in the conventions of the reference fixtures
     (``tests/minimal/Reference/grid/elem.dat.orig``): nodes row-major from the
     surface, type-8 quads counter-clockwise from bottom-left, type-12 no-flow
     surface left to right, type-11 mixed boundary clockwise (right side down,
-    bottom right to left, left side up).  Not yet CutMcK-ordered (mb = 0)."""
+    bottom right to left, left side up).  With ``mixed=False`` the sides and
+    the bottom are no-flow (type 12) as well, so no system depends on the
+    source electrode (lbeta false, one factorisation per wavenumber).
+    Not yet CutMcK-ordered (mb = 0)."""
+    mixed_flag = mixed
     n1 = nx + 1
     ix, iz = np.meshgrid(np.arange(n1), np.arange(nz + 1))
     x = (x0 + ix * dx).ravel().astype(np.float64)
@@ -289,9 +293,12 @@ def quad_grid(nx: int, nz: int, dx: float, dz: float, x0: float = 0.0,
     nrel[len(cells): len(cells) + len(surf), :2] = np.array(surf)
     nrel[len(cells) + len(surf):, :2] = np.array(mixed)
     enr = np.array([] if elec_nodes is None else elec_nodes, np.int32)
-    return Grid(sanz=n1 * (nz + 1), mb=0, typ=np.array([8, 12, 11]),
-                nelanz=np.array([len(cells), len(surf), len(mixed)]),
-                selanz=np.array([4, 2, 2]), nrel=nrel, x=x, y=y,
+    if mixed_flag:
+        typ, nelanz, selanz = [8, 12, 11], [len(cells), len(surf), len(mixed)], [4, 2, 2]
+    else:
+        typ, nelanz, selanz = [8, 12], [len(cells), len(surf) + len(mixed)], [4, 2]
+    return Grid(sanz=n1 * (nz + 1), mb=0, typ=np.array(typ), nelanz=np.array(nelanz),
+                selanz=np.array(selanz), nrel=nrel, x=x, y=y,
                 rnr=np.array(rnr, np.int32), enr=enr, nx=nx, nz=nz)
 
 

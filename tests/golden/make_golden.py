@@ -4,7 +4,7 @@ in this container through oracle/ref_harness.f90).  The committed files are data
 inputs are regenerated by crtomo_stable_amd.configs / the fixture grids, outputs
 are what the reference computed.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [name ...]
 """
 import os
 import sys
@@ -63,29 +63,38 @@ def cg_golden(w, R, out, ltri, lam, eps, ncgmax, seed):
             p + "lam": lam, p + "eps": eps, p + "ncgmax": ncgmax}
 
 
-def main():
+def main(names):
     O.build()
     if not O.ref_available():
         raise SystemExit("reference build missing: make -C oracle ref")
+    want = lambda n: not names or n in names  # noqa: E731
     # fixture grids of the reference's tests (CRMod voltages)
     for name in ("minimal", "mixed_surf"):
-        w = fixture_workload(name)
-        out, _ = forward_golden(w, sens=(name == "minimal"))
-        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **out)
+        if want(name):
+            w = fixture_workload(name)
+            out, _ = forward_golden(w, sens=(name == "minimal"))
+            np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **out)
     # seeded synthetic grids: forward, sensitivities and one CG update
-    w = configs.small_quad(cutmck=O.cutmck)
-    out, R = forward_golden(w)
-    out.update(cg_golden(w, R, out, 0, 10.0, 1e-4, w.grid.elanz // 10, 1))
-    np.savez_compressed(os.path.join(HERE, "ref_small_quad.npz"), **out)
-    w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
-    out, R = forward_golden(w, bsens=False)
-    out.pop("kpot_src1")
-    out.update(cg_golden(w, R, out, 1, 3.0, 1e-12, 400, 2))
-    np.savez_compressed(os.path.join(HERE, "ref_small_tri.npz"), **out)
+    if want("small_quad"):
+        w = configs.small_quad(cutmck=O.cutmck)
+        out, R = forward_golden(w)
+        out.update(cg_golden(w, R, out, 0, 10.0, 1e-4, w.grid.elanz // 10, 1))
+        np.savez_compressed(os.path.join(HERE, "ref_small_quad.npz"), **out)
+    if want("small_quad_noflow"):
+        # no mixed boundary: one factorisation per wavenumber serves every source
+        w = configs.small_quad(mixed=False, cutmck=O.cutmck)
+        out, R = forward_golden(w)
+        np.savez_compressed(os.path.join(HERE, "ref_small_quad_noflow.npz"), **out)
+    if want("small_tri"):
+        w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+        out, R = forward_golden(w, bsens=False)
+        out.pop("kpot_src1")
+        out.update(cg_golden(w, R, out, 1, 3.0, 1e-12, 400, 2))
+        np.savez_compressed(os.path.join(HERE, "ref_small_tri.npz"), **out)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -174,7 +174,7 @@ def test_volt_dat_real4(oracle_lib, name):
     assert np.max(np.abs(1e3 * ph - pha)) < 1e-3
 
 
-@pytest.mark.parametrize("name", ["minimal", "mixed_surf", "small_quad", "small_tri"])
+@pytest.mark.parametrize("name", ["minimal", "mixed_surf", "small_quad", "small_quad_noflow", "small_tri"])
 def test_oracle_vs_reference_golden(oracle_lib, name):
     """Forward, voltages and sensitivities against FP64 vectors of the reference build."""
     ref = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
@@ -183,22 +183,37 @@ def test_oracle_vs_reference_golden(oracle_lib, name):
         w, _, _ = fixture(name)
     elif name == "small_quad":
         w = configs.small_quad(cutmck=O.cutmck)
+    elif name == "small_quad_noflow":
+        w = configs.small_quad(mixed=False, cutmck=O.cutmck)
     else:
         w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
     r = oracle_forward(O, w, mode=1, sens="sens" in ref)
     assert np.array_equal(r["kwn"], ref["kwn"]) and np.array_equal(r["kwnwi"], ref["kwnwi"])
     assert bool(r["lbeta"]) == bool(ref["lbeta"])
-    assert float(np.max(np.abs(r["volt"] - ref["volt"]) / np.abs(ref["volt"]))) < 1e-12
-    assert rel(r["sigmaa"], ref["sigmaa"]) < 1e-12
-    assert rel(r["hpot"][:, w.grid.enr - 1], ref["hpot_elec"]) < 1e-12
-    if "kpot_src1" in ref:
-        assert rel(r["kpot"][:, 0, :], ref["kpot_src1"]) < 1e-12
+    if name == "small_quad_noflow":
+        # no-flow boundaries leave a large near-constant potential at small k; the
+        # voltages are differences of it, so their error scales with the potential
+        assert float(np.max(np.abs(r["volt"] - ref["volt"]))) < 1e-13 * float(np.abs(ref["hpot_elec"]).max())
+        # potentials agree up to that constant (rounding times the condition number)
+        he, hr = r["hpot"][:, w.grid.enr - 1], ref["hpot_elec"]
+        assert rel(he, hr) < 1e-8
+        assert rel(he - he.mean(axis=1, keepdims=True), hr - hr.mean(axis=1, keepdims=True)) < 1e-11
+        kp, kr = r["kpot"][:, 0, :], ref["kpot_src1"]
+        assert rel(kp - kp.mean(axis=1, keepdims=True), kr - kr.mean(axis=1, keepdims=True)) < 1e-10
+    else:
+        assert float(np.max(np.abs(r["volt"] - ref["volt"]) / np.abs(ref["volt"]))) < 1e-12
+        assert rel(r["sigmaa"], ref["sigmaa"]) < 1e-12
+        assert rel(r["hpot"][:, w.grid.enr - 1], ref["hpot_elec"]) < 1e-12
+        if "kpot_src1" in ref:
+            assert rel(r["kpot"][:, 0, :], ref["kpot_src1"]) < 1e-12
+    stol = 1e-10 if name == "small_quad_noflow" else 1e-11  # divided by the cancelling voltages
     if "sens" in ref:
-        assert rel(r["sens"], ref["sens"].reshape(r["sens"].shape)) < 1e-11
+        assert rel(r["sens"], ref["sens"].reshape(r["sens"].shape)) < stol
     if "sens_bsens" in ref:
         r0 = oracle_forward(O, w, mode=0, sens=True)
-        assert float(np.max(np.abs(r0["volt"] - ref["volt_bvolt"]) / np.abs(ref["volt_bvolt"]))) < 1e-12
-        assert rel(r0["sens"], ref["sens_bsens"]) < 1e-11
+        if name != "small_quad_noflow":
+            assert float(np.max(np.abs(r0["volt"] - ref["volt_bvolt"]) / np.abs(ref["volt_bvolt"]))) < 1e-12
+        assert rel(r0["sens"], ref["sens_bsens"]) < stol
 
 
 @pytest.mark.parametrize("name,ltri", [("small_quad", 0), ("small_tri", 1)])
