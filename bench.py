@@ -27,6 +27,21 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/<workload>_pmc_traffic.md, a line
+    `traffic <kernel> <bytes>`), or None when no such measurement is committed."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"^traffic\s+(\S+)\s+(\d+)\s*$", line.strip())
+        if m and m.group(1) == kernel:
+            return int(m.group(2))
+    return None
+
+
 def device_sync():
     lib = ctypes.CDLL("libamdhip64.so")
     lib.hipDeviceSynchronize()
@@ -136,9 +151,12 @@ def main():
     flop_sys = 8.0 * g.sanz * g.mb * (g.mb + 1) / 2.0
     fac_ms = phase["factor"]
     achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
+    d16 = -(-g.mb // 16)
+    kernel = f"crt_factor16_kernel<{d16}>" if d16 <= 10 and not os.environ.get("CRT_FACTOR8") else "crt_factor_kernel"
     roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
-                    kernel="crt_factor_kernel", flop_per_system=flop_sys, systems_per_launch=nsys,
+                    frac=round(achieved / FP64_PEAK_TFLOPS, 4),
+                    traffic=measured_traffic(kernel, args.workload),
+                    kernel=kernel, flop_per_system=flop_sys, systems_per_launch=nsys,
                     launch_ms=round(fac_ms, 4))
 
     out = {

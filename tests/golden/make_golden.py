@@ -85,6 +85,16 @@ def main(names):
         w = configs.small_quad(mixed=False, cutmck=O.cutmck)
         out, R = forward_golden(w)
         np.savez_compressed(os.path.join(HERE, "ref_small_quad_noflow.npz"), **out)
+    if want("small_tri_sytop"):
+        # boundary factors with the topography-aware singularity position (lsytop)
+        w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+        R = O.RefSession(w.grid, w.survey)
+        kwn, kwi, _, _ = R.rwaven()
+        elbg, relbg, kg, lb = R.precal(lsytop=True)
+        R.forward(w.sigma, nthreads=4)
+        hpot, volt, sa = R.voltages(True)
+        np.savez_compressed(os.path.join(HERE, "ref_small_tri_sytop.npz"), kg=kg, volt=volt, sigmaa=sa,
+                            lbeta=lb)
     if want("small_tri"):
         w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
         out, R = forward_golden(w, bsens=False)

@@ -129,3 +129,26 @@ def test_forward_parity_noflow(ctx, oracle_lib, mode):
     sens = ctx.get_sens()
     assert _rel_rows(sens.T, r["sens"].T) <= 1e-9
     assert _rel_rows(centered(hpot), centered(r["hpot"])) <= 1e-10
+
+
+def test_forward_sytop(ctx, oracle_lib):
+    """Topography-aware boundary factors (lsytop, src/bsytop.f90) on the device:
+    kg and the resulting voltages against the oracle."""
+    O = oracle_lib
+    w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7)
+    print(f"[forward sytop] sanz={w.grid.sanz} mb={w.grid.mb}", flush=True)
+    ctx.set_grid(w.grid)
+    ctx.set_survey(w.survey)
+    kwn, kwi, _, _ = ctx.rwaven()
+    lb = ctx.precal(lsytop=True)
+    ctx.set_sigma(w.sigma)
+    ctx.forward()
+    volt, _ = ctx.voltages(1)
+    _, _, kg = ctx.get_precal()
+    okwn, okwi, _, _ = O.rwaven(w.grid, w.survey)
+    elbg, relbg, okg, olb = O.precal(w.grid, okwn, lsytop=True)
+    assert lb == olb
+    assert np.abs(kg - okg).max() <= 1e-12 * np.abs(okg).max()
+    kpot = O.forward(w.grid, w.sigma, elbg, relbg, okg, olb)
+    ovolt, _ = O.bvolti(w.grid, w.survey, O.rtrafo(kpot, okwi))
+    assert float(np.max(np.abs(volt - ovolt) / np.abs(ovolt))) <= TOL

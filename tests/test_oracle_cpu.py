@@ -273,3 +273,39 @@ def test_oracle_vs_reference_live_c0(oracle_lib):
     hpot, volt, sa = R.voltages(True)
     assert float(np.max(np.abs(r["volt"] - volt) / np.abs(volt))) < 1e-12
     assert rel(r["sens"], R.sens(True)) < 1e-11
+
+
+def test_oracle_sytop_vs_reference_golden(oracle_lib):
+    """Boundary factors with the topography-aware singularity (lsytop, src/bsytop.f90,
+    src/beta.f90) and the resulting voltages against the reference build."""
+    O = oracle_lib
+    ref = np.load(os.path.join(GOLD, "ref_small_tri_sytop.npz"))
+    w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    kwn, kwi, _, _ = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn, lsytop=True)
+    assert bool(lb) == bool(ref["lbeta"])
+    assert rel(kg, ref["kg"].reshape(kg.shape)) < 1e-12
+    kg0 = O.precal(w.grid, kwn)[2]
+    assert rel(kg0, kg) > 1e-6  # the option changes the factors
+    kpot = O.forward(w.grid, w.sigma, elbg, relbg, kg, lb)
+    volt, sa = O.bvolti(w.grid, w.survey, O.rtrafo(kpot, kwi))
+    assert float(np.max(np.abs(volt - ref["volt"]) / np.abs(ref["volt"]))) < 1e-12
+
+
+def test_cholz_reconstruction(oracle_lib):
+    """cholz (src/cholz.f90:43-76): strict lower factor in a(j,i), diagonal in p;
+    L L^T reproduces the complex-symmetric input (no conjugation), the diagonal is
+    the principal square root branch, and a singular leading minor is reported."""
+    rng = np.random.default_rng(5)
+    n = 12
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    err, a, p = oracle_lib.cholz(A)
+    assert err == 0
+    L = np.tril(a, -1) + np.diag(p)
+    assert np.abs(L @ L.T - A).max() < 1e-12 * np.abs(A).max()
+    assert np.all(p.real >= 0.0)  # principal branch
+    A2 = A.copy()
+    A2[0, 0] = 0.0
+    err2, _, _ = oracle_lib.cholz(A2)
+    assert err2 != 0
