@@ -115,6 +115,8 @@ struct crt_ctx {
     int lbeta = 0;
     // device state
     DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_crow, d_cptr, d_contrib, d_nkel, d_status;
+    DBuf<int> d_tmap;
+    int tmapD = 0;
     int nnz = 0;
     DBuf<int4> d_meas, d_rhs;
     DBuf<int2> d_sys;
@@ -137,6 +139,8 @@ struct crt_ctx {
         g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.crow = d_crow.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
         g.rnr = d_rnr.p; g.enr = d_enr.p; g.elbg = d_elbg.p; g.relbg = d_relbg.p; g.kg = d_kg.p;
         g.sigma = d_sigma.p;
+        g.tmap = d_tmap.p;
+        g.tmapD = tmapD;
         return g;
     }
     void tic(int i) { (void)hipEventRecord(ev0[i], stream); }
@@ -389,6 +393,22 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     e = e ? e : c->d_cptr.upload(cptr.data(), cptr.size(), s);
     e = e ? e : c->d_contrib.upload(contrib.data(), contrib.size(), s);
     if (e) return e;
+    // tile map of the 16x16 block band (used by the factorisation for band widths up to 160)
+    c->tmapD = 0;
+    {
+        const int D16 = std::max(1, (mb + 15) / 16), nb16 = (sanz + 15) / 16;
+        if (D16 <= 10) {
+            std::vector<int> tmap((size_t)nb16 * (D16 + 1) * 256, -1);
+            for (int i = 0; i < sanz; ++i)
+                for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+                    const int j = col[q], I = i / 16, d = I - j / 16;
+                    tmap[((size_t)(I * (D16 + 1) + d) * 16 + (i % 16)) * 16 + (j % 16)] = q;
+                }
+            for (int i = sanz; i < nb16 * 16; ++i) tmap[((size_t)((i / 16) * (D16 + 1)) * 16 + (i % 16)) * 16 + (i % 16)] = -2;
+            if ((e = c->d_tmap.upload(tmap.data(), tmap.size(), s))) return e;
+            c->tmapD = D16;
+        }
+    }
     CRT_HIP(hipStreamSynchronize(s));
     c->have_grid = true;
     return 0;

@@ -67,6 +67,19 @@ __device__ __forceinline__ void pivot_fast(zd s, zd &L, zd &inv, bool &bad) {
     inv = zd{L.re * rm, -L.im * rm};
 }
 
+// r = 1/p = conj(p)/|p|^2 with the hardware reciprocal and two Newton steps;
+// bad as in pivot_fast
+__device__ __forceinline__ zd crecip(zd p, bool &bad) {
+    const double m2 = fma(p.re, p.re, p.im * p.im);
+    bad = !(m2 > 0.0) || !(m2 < 1.0e300);
+    double y = __builtin_amdgcn_rcp(m2);
+    double e = fma(-m2, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-m2, y, 1.0);
+    y = fma(y, e, y);
+    return zd{p.re * y, -p.im * y};
+}
+
 __device__ __forceinline__ zd readlane_z(zd v, int src) { return zd{readlane_d(v.re, src), readlane_d(v.im, src)}; }
 __device__ __forceinline__ zd zsel(bool c, zd a, zd b) { return zd{c ? a.re : b.re, c ? a.im : b.im}; }
 
@@ -106,12 +119,15 @@ namespace {
 // LDS of one workgroup; a static layout keeps every address an immediate
 template <int D>
 struct F16Lds {
-    zd P[(D + 1) * T16 * PS];  // panel tiles of block column J, 16 x PS each
-    zd S[(D + 1) * 256];       // staging tiles of the entering row block, 16 x 16 each
-    zd bring[(D + 2) * T16];   // right-hand-side blocks J .. J+D+1
-    zd col[32];                // current column of the diagonal factor, [16] = y of the column
-    zd pinv[T16];              // reciprocal pivots of the diagonal tile
-    zd yj[T16];                // forward-substituted right-hand side of block J
+    zd P[(D + 1) * T16 * PS];         // panel tiles of block column J, 16 x PS each
+    zd U[T16 * PS];                   // unit lower factor u(i,c) = a(i,c)/p_c of the diagonal tile
+    zd S[(D + 1) * 256];              // staging tiles of the entering row block, 16 x 16 each
+    zd bring[(D + 2) * T16];          // right-hand-side blocks J .. J+D+1
+    zd xcol[Cfg<D>::W][2][T16];       // per-wave column exchange of the panel solve
+    zd col[32];                       // current column of the diagonal tile, [16] = its RHS entry
+    zd pinv[T16];                     // 1/L(c,c)
+    zd sdiag[T16];                    // L(c,c)
+    zd yj[T16];                       // forward-substituted right-hand side of block J
 };
 
 // CSR rows of block Ie (contiguous, values scaled by crt_assemble_kernel) into
@@ -140,17 +156,30 @@ __device__ __forceinline__ d4 tile_im(const zd *t, int jc, int rq) {
     return d4{t[rq * T16 + jc].im, t[(4 + rq) * T16 + jc].im, t[(8 + rq) * T16 + jc].im, t[(12 + rq) * T16 + jc].im};
 }
 
-// T -= Pa Pb^T for 16x16 complex tiles in LDS (row stride PS), T in result layout
-__device__ __forceinline__ void mfma_update(d4 &tr, d4 &ti, const zd *Pa, const zd *Pb, int jc, int rq) {
+// (dr, di) = T - Pa Pb^T for 16x16 complex tiles in LDS (row stride PS), T and
+// the result in the accumulator layout; the result may go to another slot, so
+// the window slides without register moves
+__device__ __forceinline__ void mfma_update(d4 &dr, d4 &di, const d4 &tr, const d4 &ti, const zd *Pa, const zd *Pb,
+                                            int jc, int rq) {
+    zd va[4], vb[4];
 #pragma unroll
     for (int kc = 0; kc < 4; ++kc) {
-        zd va = Pa[jc * PS + 4 * kc + rq];
-        zd vb = Pb[jc * PS + 4 * kc + rq];
-        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.re, tr, 0, 0, 0);
-        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(va.im, vb.im, tr, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.re, vb.im, ti, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.im, vb.re, ti, 0, 0, 0);
+        va[kc] = Pa[jc * PS + 4 * kc + rq];
+        vb[kc] = Pb[jc * PS + 4 * kc + rq];
     }
+    d4 xr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].re, vb[0].re, tr, 0, 0, 0);
+    d4 xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].re, vb[0].im, ti, 0, 0, 0);
+    xr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[0].im, vb[0].im, xr, 0, 0, 0);
+    xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].im, vb[0].re, xi, 0, 0, 0);
+#pragma unroll
+    for (int kc = 1; kc < 4; ++kc) {
+        xr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].re, vb[kc].re, xr, 0, 0, 0);
+        xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].re, vb[kc].im, xi, 0, 0, 0);
+        xr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kc].im, vb[kc].im, xr, 0, 0, 0);
+        xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].im, vb[kc].re, xi, 0, 0, 0);
+    }
+    dr = xr;
+    di = xi;
 }
 
 template <int D, int WV>
@@ -179,127 +208,167 @@ __device__ __forceinline__ void factor16_wave(const Factor16Args &A, F16Lds<D> &
 
     for (int J = -(D + 1); J < nblk; ++J) {
         const int Ie = J + D + 1;  // entering row block
-        // ------------------------------------------------ A: panel tiles -> LDS
-        if (J >= 0) {
-            zd *pa = m.P + dA * T16 * PS;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zd{Ar[0][v], Ai[0][v]};
-            if constexpr (LB > 0) {
-                zd *pb = m.P + dB * T16 * PS;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zd{Br[0][v], Bi[0][v]};
-            }
-        }
-        __syncthreads();
         // ---------------- B: wave 0 factors the diagonal tile | others stage Ie
         if (WV == 0 && J >= 0) {
-            // right-looking factorisation of the diagonal tile in the result
-            // layout; column c travels through LDS (col), the next pivot is formed
-            // from two lane reads so the pivot chain does not wait for the exchange
+            // Right-looking elimination of the 16x16 diagonal tile in the result
+            // layout with the square roots deferred: the update uses the pivots p_c
+            // as a(i,j) -= a(i,c) a(j,c)/p_c, so only a complex reciprocal sits on
+            // the column-to-column chain; the next pivot is formed from two lane
+            // reads before the column exchange.  L(i,c) = a(i,c)/sqrt(p_c) at the end.
             zd a[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = m.P[(4 * v + rq) * PS + jc];
-            // lanes of column group jc carry the right-hand side entry jc
-            zd yv = m.bring[(J % (D + 2)) * T16 + jc];
-            zd pnext = readlane_z(a[0], 0);
+            for (int v = 0; v < 4; ++v) a[v] = zd{Ar[0][v], Ai[0][v]};
+            zd yv = m.bring[(J % (D + 2)) * T16 + jc];  // lanes of column group jc carry b(jc)
+            zd p = readlane_z(a[0], 0);
+            bool bad;
+            zd r = crecip(p, bad);
+            if (bad) err = CRT_ERR_PIVOT;
+            zd pk = zsel(lane == 0, p, zd{0.0, 0.0});  // lane c keeps p_c
 #pragma unroll
             for (int c = 0; c < T16; ++c) {
-                zd Lc, iv;
-                bool bad;
-                pivot_fast(pnext, Lc, iv, bad);
-                if (bad) err = CRT_ERR_PIVOT;
-                zd cv[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) cv[v] = zsel(4 * v + rq == c, Lc, zmul(a[v], iv));
-                const zd yc = zmul(yv, iv);
-                if (c + 1 < T16) {
-                    const int n = c + 1;
-                    zd dn = readlane_z(a[n / 4], n + 16 * (n % 4));  // a(n, n)
-                    zd ln = readlane_z(cv[n / 4], c + 16 * (n % 4)); // L(n, c)
-                    pnext = zfms(dn, ln, ln);
-                }
                 if (jc == c) {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) m.col[4 * v + rq] = cv[v];
-                    if (rq == 0) {
-                        m.col[16] = yc;
-                        m.pinv[c] = iv;
-                    }
+                    for (int v = 0; v < 4; ++v) m.col[4 * v + rq] = a[v];
+                    if (rq == 0) m.col[16] = yv;
                 }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) a[v] = zsel(jc == c, cv[v], a[v]);
-                yv = zsel(jc == c, yc, yv);
+                zd dn, l2;
+                if (c + 1 < T16) {
+                    const int n = c + 1;
+                    dn = readlane_z(a[n / 4], n + 16 * (n % 4));       // a(n, n)
+                    const zd ln = readlane_z(a[n / 4], c + 16 * (n % 4));  // a(n, c)
+                    l2 = zmul(ln, ln);
+                }
                 __builtin_amdgcn_wave_barrier();
-                const zd ljc = m.col[jc], ycol = m.col[16];
+                zd ci[4];
+                const zd cj = m.col[jc], wc = m.col[16];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int i = 4 * v + rq;
-                    a[v] = zsel(jc > c && i >= jc, zfms(a[v], m.col[i], ljc), a[v]);
+                for (int v = 0; v < 4; ++v) ci[v] = m.col[4 * v + rq];
+                __builtin_amdgcn_sched_barrier(0);  // one exchange: issue every read before the first use
+                const zd t = zsel(jc > c, zmul(cj, r), zd{0.0, 0.0});  // u(jc, c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) a[v] = zfms(a[v], ci[v], t);
+                yv = zfms(yv, wc, t);
+                if (rq == 0) m.U[jc * PS + c] = t;
+                if (c + 1 < T16) {
+                    p = zfms(dn, l2, r);
+                    r = crecip(p, bad);
+                    if (bad) err = CRT_ERR_PIVOT;
+                    pk = zsel(lane == c + 1, p, pk);
                 }
-                yv = zsel(jc > c, zfms(yv, ljc, ycol), yv);
                 __builtin_amdgcn_wave_barrier();
             }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int i = 4 * v + rq;
-                m.P[i * PS + jc] = (jc <= i) ? a[v] : zd{0.0, 0.0};
-            }
-            if (rq == 0) {
-                m.yj[jc] = yv;
-                m.bring[(J % (D + 2)) * T16 + jc] = yv;
-                yout[(size_t)J * T16 + jc] = yv;
+            // square roots of the pivots (lane c), then scale the columns
+            zd sc, ivc;
+            bool badc;
+            pivot_fast(pk, sc, ivc, badc);
+            if (lane < T16) {
+                m.pinv[lane] = ivc;
+                m.sdiag[lane] = sc;
+                A.dinv[(size_t)s * nblk * T16 + (size_t)J * T16 + lane] = ivc;
             }
             __builtin_amdgcn_wave_barrier();
-            if (lane < T16) A.dinv[(size_t)s * nblk * T16 + (size_t)J * T16 + lane] = m.pinv[lane];
+            const zd ivj = m.pinv[jc], sj = m.sdiag[jc];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 4 * v + rq;
+                m.P[i * PS + jc] = i > jc ? zmul(a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0});
+            }
+            if (rq == 0) {
+                const zd y = zmul(yv, ivj);
+                m.yj[jc] = y;
+                yout[(size_t)J * T16 + jc] = y;
+            }
         } else if (Ie < nblk) {
             if (J < 0) stage_block<D>(g, Ie, tid, NT, vals, fak, enode, m);
             else stage_block<D>(g, Ie, tid - 64, NT - 64, vals, fak, enode, m);
         }
         __syncthreads();
-        // ------------------ C: panel rows X = A L^-T (waves 1..), RHS update
+        // ---- C: each wave solves its panel tiles X = A L^-T in registers, RHS update
         if (WV >= 1 && J >= 0) {
-            const int t = tid - 64;
-            const int a = 1 + t / T16, r = t % T16;
-            if (t < T16 * D && J + a < nblk) {
-                zd *row = m.P + (a * T16 + r) * PS;
-                zd x[T16];
+            // Y = A U^-T column by column (unit diagonal), X = Y diag(1/L(c,c))
+            zd xa[4], xb[4];
 #pragma unroll
-                for (int c = 0; c < T16; ++c) x[c] = row[c];
-                zd bacc = m.bring[((J + a) % (D + 2)) * T16 + r];
+            for (int v = 0; v < 4; ++v) {
+                xa[v] = zd{Ar[0][v], Ai[0][v]};
+                xb[v] = zd{Br[0][v], Bi[0][v]};
+            }
+            zd *xca = m.xcol[WV][0], *xcb = m.xcol[WV][1];
 #pragma unroll
-                for (int c = 0; c < T16; ++c) {
-                    x[c] = zmul(x[c], m.pinv[c]);
-                    bacc = zfms(bacc, x[c], m.yj[c]);
+            for (int c = 0; c < T16; ++c) {
+                if (jc == c) {
 #pragma unroll
-                    for (int i = c + 1; i < T16; ++i) x[i] = zfms(x[i], x[c], m.P[i * PS + c]);
+                    for (int v = 0; v < 4; ++v) {
+                        xca[4 * v + rq] = xa[v];
+                        if constexpr (LB > 0) xcb[4 * v + rq] = xb[v];
+                    }
                 }
+                __builtin_amdgcn_wave_barrier();
+                const zd u = m.U[jc * PS + c];
+                zd ya[4], yb[4];
 #pragma unroll
-                for (int c = 0; c < T16; ++c) row[c] = x[c];
-                m.bring[((J + a) % (D + 2)) * T16 + r] = bacc;
+                for (int v = 0; v < 4; ++v) {
+                    ya[v] = xca[4 * v + rq];
+                    if constexpr (LB > 0) yb[v] = xcb[4 * v + rq];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // one exchange: issue every read before the first use
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    xa[v] = zfms(xa[v], ya[v], u);
+                    if constexpr (LB > 0) xb[v] = zfms(xb[v], yb[v], u);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            const zd ivj = m.pinv[jc];
+            zd *pa = m.P + dA * T16 * PS;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zmul(xa[v], ivj);
+            if constexpr (LB > 0) {
+                zd *pb = m.P + dB * T16 * PS;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zmul(xb[v], ivj);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // b(J+d) -= X(J+d, J) y_J: lanes 0..15 for the A tile, 16..31 for the B tile
+            const int half = lane >> 4, rr = lane & 15;
+            const int d = half == 0 ? dA : dB;
+            if ((half == 0 || (LB > 0 && half == 1)) && J + d < nblk) {
+                const zd *row = m.P + (d * T16 + rr) * PS;
+                zd *bp = m.bring + ((J + d) % (D + 2)) * T16 + rr;
+                zd acc = *bp;
+#pragma unroll
+                for (int c = 0; c < T16; ++c) acc = zfms(acc, row[c], m.yj[c]);
+                *bp = acc;
             }
         }
         __syncthreads();
         // ------------------- D: store panel, rank-16 updates on the matrix cores, slide
         if (J >= 0) {
             zd *pj = panel + (size_t)J * (D + 1) * 256;
-            for (int t = tid; t < (D + 1) * 256; t += NT) {
-                int a = t >> 8, rc = t & 255;
-                pj[t] = (J + a < nblk) ? m.P[(a * T16 + (rc >> 4)) * PS + (rc & 15)] : zd{0.0, 0.0};
+            constexpr int NPS = ((D + 1) * 256 + NT - 1) / NT;
+            zd pv[NPS];
+#pragma unroll
+            for (int u = 0; u < NPS; ++u) {
+                const int t = tid + u * NT;
+                const int a = t >> 8, rc = t & 255;
+                pv[u] = (t < (D + 1) * 256 && J + a < nblk) ? m.P[(a * T16 + (rc >> 4)) * PS + (rc & 15)]
+                                                             : zd{0.0, 0.0};
             }
 #pragma unroll
-            for (int p = 1; p < LA; ++p)
-                if (J + p + dA < nblk)
-                    mfma_update(Ar[p], Ai[p], m.P + (p + dA) * T16 * PS, m.P + p * T16 * PS, jc, rq);
-            if constexpr (LB > 0) {
+            for (int u = 0; u < NPS; ++u)
+                if (tid + u * NT < (D + 1) * 256) pj[tid + u * NT] = pv[u];
+        }
+        // position p -> p-1: each rank-16 update writes its result into the slot
+        // of the next position; the entering tiles take the last positions
 #pragma unroll
-                for (int p = 1; p < LB; ++p)
-                    if (J + p + dB < nblk)
-                        mfma_update(Br[p], Bi[p], m.P + (p + dB) * T16 * PS, m.P + p * T16 * PS, jc, rq);
+        for (int p = 1; p < LA; ++p) {
+            if (J >= 0 && J + p + dA < nblk)
+                mfma_update(Ar[p - 1], Ai[p - 1], Ar[p], Ai[p], m.P + (p + dA) * T16 * PS, m.P + p * T16 * PS, jc,
+                            rq);
+            else {
+                Ar[p - 1] = Ar[p];
+                Ai[p - 1] = Ai[p];
             }
         }
-        // slide: position p -> p-1; the entering tiles take the last positions
-#pragma unroll
-        for (int p = 0; p + 1 < LA; ++p) { Ar[p] = Ar[p + 1]; Ai[p] = Ai[p + 1]; }
         {
             zd *sa = m.S + dA * 256;
             Ar[LA - 1] = tile_re(sa, jc, rq);
@@ -309,7 +378,15 @@ __device__ __forceinline__ void factor16_wave(const Factor16Args &A, F16Lds<D> &
         }
         if constexpr (LB > 0) {
 #pragma unroll
-            for (int p = 0; p + 1 < LB; ++p) { Br[p] = Br[p + 1]; Bi[p] = Bi[p + 1]; }
+            for (int p = 1; p < LB; ++p) {
+                if (J >= 0 && J + p + dB < nblk)
+                    mfma_update(Br[p - 1], Bi[p - 1], Br[p], Bi[p], m.P + (p + dB) * T16 * PS, m.P + p * T16 * PS,
+                                jc, rq);
+                else {
+                    Br[p - 1] = Br[p];
+                    Bi[p - 1] = Bi[p];
+                }
+            }
             zd *sb = m.S + dB * 256;
             Br[LB - 1] = tile_re(sb, jc, rq);
             Bi[LB - 1] = tile_im(sb, jc, rq);
@@ -340,15 +417,404 @@ __global__ void __launch_bounds__(Cfg<D>::NT, 2) crt_factor16_kernel(Factor16Arg
     factor16_dispatch<D, 0>(threadIdx.x >> 6, A, m);
 }
 
-size_t crt_factor16_lds_bytes(int D) {
-    return ((size_t)(D + 1) * T16 * PS + (size_t)(D + 1) * 256 + (size_t)(D + 2) * T16 + 64) * sizeof(zd);
+static_assert(sizeof(F16Lds<6>) <= 80 * 1024, "two workgroups per CU at D = 6");
+
+// ---------------------------------------------------------------------------
+// Look-ahead variant (D >= 2), 1 + D/2 + (D odd) wavefronts, two workgroups per CU.
+//   wave 0       pivot wave: takes the diagonal tile (J+1,J+1) handed over by
+//                wave 1, applies its last rank-16 update X(J+1,J) X(J+1,J)^T and
+//                factors it (deferred square roots; L^-1 from the same column
+//                operations applied to an identity block) while the other waves
+//                update the window for panel J;
+//   waves 1..    diagonals g and D+1-g (pairs) or the middle diagonal, all
+//                positions, plus a contiguous share of diagonal 0, positions
+//                2..D; the lowest diagonal-0 tile of each share moves to the next
+//                lower wave (wave 1: to the pivot wave) through an LDS hand-off.
+// Per block column J: S1 -- the owners of the panel tiles compute X = A (L^-1)^T
+// on the matrix cores and update the right-hand side, the receivers take their
+// hand-off tiles; S2 -- rank-16 updates and the slide on waves 1.., the
+// diagonal factorisation of block J+1 on wave 0.  Entering tiles are gathered
+// from the CSR values through the tile map, so the window needs no staging LDS.
+namespace {
+
+template <int D>
+struct La4 {
+    static constexpr int NPAIR = D / 2;
+    static constexpr int MID = D & 1;
+    static constexpr int NU = NPAIR + MID;  // update waves 1..NU
+    static constexpr int W = NU + 1;
+    static constexpr int NT = 64 * W;
+    static constexpr int NZ = D - 1;        // diagonal-0 positions 2..D
+    static constexpr int zcount(int u) { return NZ / NU + ((u - 1) < NZ % NU ? 1 : 0); }
+    static constexpr int zlo(int u) { return u == 1 ? 2 : zlo(u - 1) + zcount(u - 1); }
+};
+
+template <int D>
+struct La4Lds {
+    zd P[D * T16 * PS];       // X(J+d, J), d = 1..D
+    zd Li[T16 * PS];          // L(J,J)^-1, lower, row-major
+    zd HO[La4<D>::NU][256];   // hand-off tiles: HO[u-1] from wave u to wave u-1 (u = 1: to wave 0)
+    zd bring[(D + 2) * T16];
+    zd col[48];
+    zd pinv[T16], sdiag[T16];
+    zd yj[T16];
+};
+
+__device__ __forceinline__ void mfma_mul(d4 &xr, d4 &xi, const zd *Pa, const zd *Pb, int jc, int rq) {
+    zd va[4], vb[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        va[kc] = Pa[jc * PS + 4 * kc + rq];
+        vb[kc] = Pb[jc * PS + 4 * kc + rq];
+    }
+    const d4 z = d4{0.0, 0.0, 0.0, 0.0};
+    xr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[0].re, vb[0].re, z, 0, 0, 0);
+    xi = __builtin_amdgcn_mfma_f64_16x16x4f64(va[0].re, vb[0].im, z, 0, 0, 0);
+    xr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].im, vb[0].im, xr, 0, 0, 0);
+    xi = __builtin_amdgcn_mfma_f64_16x16x4f64(va[0].im, vb[0].re, xi, 0, 0, 0);
+#pragma unroll
+    for (int kc = 1; kc < 4; ++kc) {
+        xr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kc].re, vb[kc].re, xr, 0, 0, 0);
+        xi = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kc].re, vb[kc].im, xi, 0, 0, 0);
+        xr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].im, vb[kc].im, xr, 0, 0, 0);
+        xi = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kc].im, vb[kc].re, xi, 0, 0, 0);
+    }
 }
-static_assert(sizeof(F16Lds<6>) == 62208, "LDS layout of the 16x16 factorisation");
+
+template <int D>
+__device__ __forceinline__ zd *la4_tile(La4Lds<D> &m, int d) { return m.P + (d - 1) * T16 * PS; }
+
+// tile (I, I-d) of the scaled system in the accumulator layout, gathered through the map
+template <int D>
+__device__ __forceinline__ void la4_gather(const DevGrid &g, const zd *vals, int I, int d, int jc, int rq, d4 &tr,
+                                           d4 &ti) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        tr[v] = 0.0;
+        ti[v] = 0.0;
+    }
+    if (I >= g.nblk) return;
+    const int *mp = g.tmap + ((size_t)(I * (D + 1) + d) * T16) * T16 + jc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int q = mp[(4 * v + rq) * T16];
+        if (q >= 0) {
+            const zd e = vals[q];
+            tr[v] = e.re;
+            ti[v] = e.im;
+        } else if (q == -2) {
+            tr[v] = 1.0;
+        }
+    }
+}
+
+// diagonal factorisation of block Jn on the pivot wave
+template <int D>
+__device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D> &m, int s, int Jn, int lane,
+                                                const d4 &tr, const d4 &ti, int &err) {
+    const int nblk = A.g.nblk;
+    const int jc = lane & 15, rq = lane >> 4;
+    zd a[4], z[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        a[v] = zd{tr[v], ti[v]};
+        z[v] = zd{4 * v + rq == jc ? 1.0 : 0.0, 0.0};
+    }
+    zd yv = m.bring[(Jn % (D + 2)) * T16 + jc];
+    zd p = readlane_z(a[0], 0);
+    bool bad;
+    zd r = crecip(p, bad);
+    if (bad) err = CRT_ERR_PIVOT;
+    zd pk = zsel(lane == 0, p, zd{0.0, 0.0});
+#pragma unroll
+    for (int c = 0; c < T16; ++c) {
+        if (jc == c) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                m.col[4 * v + rq] = a[v];
+                m.col[16 + 4 * v + rq] = z[v];
+            }
+            if (rq == 0) m.col[32] = yv;
+        }
+        zd dn, l2;
+        if (c + 1 < T16) {
+            const int n = c + 1;
+            dn = readlane_z(a[n / 4], n + 16 * (n % 4));
+            const zd ln = readlane_z(a[n / 4], c + 16 * (n % 4));
+            l2 = zmul(ln, ln);
+        }
+        __builtin_amdgcn_wave_barrier();
+        zd ci[4], zi[4];
+        const zd cj = m.col[jc], wc = m.col[32];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ci[v] = m.col[4 * v + rq];
+            zi[v] = m.col[16 + 4 * v + rq];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const zd t = zsel(jc > c, zmul(cj, r), zd{0.0, 0.0});  // u(jc, c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            a[v] = zfms(a[v], ci[v], t);
+            z[v] = zfms(z[v], zi[v], t);
+        }
+        yv = zfms(yv, wc, t);
+        if (c + 1 < T16) {
+            p = zfms(dn, l2, r);
+            r = crecip(p, bad);
+            if (bad) err = CRT_ERR_PIVOT;
+            pk = zsel(lane == c + 1, p, pk);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    zd sc, ivc;
+    bool badc;
+    pivot_fast(pk, sc, ivc, badc);
+    if (lane < T16) {
+        m.pinv[lane] = ivc;
+        m.sdiag[lane] = sc;
+        A.dinv[(size_t)s * nblk * T16 + (size_t)Jn * T16 + lane] = ivc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const zd ivj = m.pinv[jc], sj = m.sdiag[jc];
+    zd *pg = A.panels + ((size_t)s * nblk + Jn) * (D + 1) * 256;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = 4 * v + rq;
+        pg[i * T16 + jc] = i > jc ? zmul(a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0});
+        m.Li[jc * PS + i] = zmul(z[v], ivj);  // L^-1(jc, i) = (L^-T)(i, jc)
+    }
+    if (rq == 0) {
+        const zd y = zmul(yv, ivj);
+        m.yj[jc] = y;
+        A.y[(size_t)s * nblk * T16 + (size_t)Jn * T16 + jc] = y;
+    }
+}
+
+template <int D, int WV>
+__device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
+    using C = La4<D>;
+    constexpr int NT = C::NT;
+    constexpr bool isP = WV == 0;
+    constexpr bool isPair = WV >= 1 && WV <= C::NPAIR;
+    constexpr bool isMid = C::MID && WV == C::NPAIR + 1;
+    constexpr int dA = isPair ? WV : (isMid ? (D + 1) / 2 : 1);
+    constexpr int LA = isPair ? D + 1 - WV : (isMid ? (D + 1) / 2 : 1);
+    constexpr int dB = isPair ? D + 1 - WV : 0;
+    constexpr int LB = isPair ? WV : 0;
+    constexpr int LBX = LB > 0 ? LB : 1;
+    constexpr int NZW = isP ? 0 : C::zcount(WV);     // diagonal-0 slots of this wave
+    constexpr int ZLO = isP ? 0 : C::zlo(WV);        // position of slot 0
+    constexpr int NZX = NZW > 0 ? NZW : 1;
+    constexpr bool topZ = !isP && WV == C::NU;       // holds position D of diagonal 0
+    const DevGrid &g = A.g;
+    const int nblk = g.nblk;
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x & 63, tid = WV * 64 + lane;
+    const int enode = g.enr[A.sys[s].x];
+    const double *fak = A.fak + (size_t)s * nblk * T16;
+    const zd *vals = A.vals + (size_t)s * g.nnz;
+    zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
+    const int jc = lane & 15, rq = lane >> 4;
+
+    d4 Ar[LA], Ai[LA], Br[LBX], Bi[LBX], Zr[NZX], Zi[NZX];
+#pragma unroll
+    for (int q = 0; q < LA; ++q) Ar[q] = Ai[q] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < LBX; ++q) Br[q] = Bi[q] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < NZX; ++q) Zr[q] = Zi[q] = d4{0.0, 0.0, 0.0, 0.0};
+    d4 Tr = d4{0.0, 0.0, 0.0, 0.0}, Ti = Tr;  // pivot wave: tile (J+1, J+1)
+    int err = 0;
+
+    for (int J = -(D + 1); J < nblk; ++J) {
+        const int Ie = J + D + 1;
+        // ================================================================ S1
+        if constexpr (isP) {
+            if (lane < T16) {
+                const int i = Ie * T16 + lane;
+                m.bring[(Ie % (D + 2)) * T16 + lane] =
+                    (i < g.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+            }
+            const zd *h = m.HO[0];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const zd e = h[(4 * v + rq) * T16 + jc];
+                Tr[v] = e.re;
+                Ti[v] = e.im;
+            }
+        } else {
+            if (J >= 0) {
+                // X = A (L^-1)^T for the panel tiles at position 0, then b(J+d) -= X y_J
+                const bool liveA = J + dA < nblk, liveB = LB > 0 && J + dB < nblk;
+                zd *pa = la4_tile<D>(m, dA);
+                zd *pb = la4_tile<D>(m, LB > 0 ? dB : dA);
+                if (liveA)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zd{Ar[0][v], Ai[0][v]};
+                if constexpr (LB > 0)
+                    if (liveB)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zd{Br[0][v], Bi[0][v]};
+                __builtin_amdgcn_wave_barrier();
+                d4 xar, xai, xbr, xbi;
+                if (liveA) mfma_mul(xar, xai, pa, m.Li, jc, rq);
+                if constexpr (LB > 0)
+                    if (liveB) mfma_mul(xbr, xbi, pb, m.Li, jc, rq);
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    pa[(4 * v + rq) * PS + jc] = liveA ? zd{xar[v], xai[v]} : zd{0.0, 0.0};
+                    if constexpr (LB > 0) pb[(4 * v + rq) * PS + jc] = liveB ? zd{xbr[v], xbi[v]} : zd{0.0, 0.0};
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int half = lane >> 4, rr = lane & 15;
+                const int d = half == 0 ? dA : dB;
+                if ((half == 0 && liveA) || (LB > 0 && half == 1 && liveB)) {
+                    const zd *row = la4_tile<D>(m, d) + rr * PS;
+                    zd *bp = m.bring + ((J + d) % (D + 2)) * T16 + rr;
+                    zd acc = *bp;
+#pragma unroll
+                    for (int c = 0; c < T16; ++c) acc = zfms(acc, row[c], m.yj[c]);
+                    *bp = acc;
+                }
+            }
+            if constexpr (!topZ) {
+                // the top diagonal-0 slot takes the tile handed down by wave WV+1
+                const zd *h = m.HO[WV];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const zd e = h[(4 * v + rq) * T16 + jc];
+                    Zr[NZX - 1][v] = e.re;
+                    Zi[NZX - 1][v] = e.im;
+                }
+            }
+        }
+        __syncthreads();
+        // ================================================================ S2
+        if constexpr (isP) {
+            const int Jn = J + 1;
+            if (Jn >= 0 && Jn < nblk) {
+                if (J >= 0) {
+                    const zd *x1 = la4_tile<D>(m, 1);
+                    mfma_update(Tr, Ti, Tr, Ti, x1, x1, jc, rq);
+                }
+                la4_factor_diag<D>(A, m, s, Jn, lane, Tr, Ti, err);
+            }
+        } else {
+            // entering tiles of block row Ie (positions D-d at the next step)
+            d4 eAr, eAi, eBr, eBi, eZr, eZi;
+            la4_gather<D>(g, vals, Ie, dA, jc, rq, eAr, eAi);
+            if constexpr (LB > 0) la4_gather<D>(g, vals, Ie, dB, jc, rq, eBr, eBi);
+            if constexpr (topZ) la4_gather<D>(g, vals, Ie, 0, jc, rq, eZr, eZi);
+            if (J >= 0) {
+                // store the panel X(J+d, J), d = 1..D (the diagonal tile is stored by wave 0)
+                zd *pj = panel + (size_t)J * (D + 1) * 256 + 256;
+                constexpr int NPS = (D * 256 + NT - 65) / (NT - 64);
+                zd pv[NPS];
+#pragma unroll
+                for (int u = 0; u < NPS; ++u) {
+                    const int t = (tid - 64) + u * (NT - 64);
+                    const int d = 1 + (t >> 8), rc = t & 255;
+                    pv[u] = (t < D * 256 && J + d < nblk) ? la4_tile<D>(m, d)[(rc >> 4) * PS + (rc & 15)]
+                                                           : zd{0.0, 0.0};
+                }
+#pragma unroll
+                for (int u = 0; u < NPS; ++u) {
+                    const int t = (tid - 64) + u * (NT - 64);
+                    if (t < D * 256) pj[t] = pv[u];
+                }
+            }
+            // diagonal 0: slot q holds position ZLO+q; position ZLO goes down to wave WV-1
+            {
+                zd *h = m.HO[WV - 1];
+                d4 hr = Zr[0], hi = Zi[0];
+                if (J >= 0 && J + ZLO < nblk) {
+                    const zd *xp = la4_tile<D>(m, ZLO);
+                    mfma_update(hr, hi, Zr[0], Zi[0], xp, xp, jc, rq);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) h[(4 * v + rq) * T16 + jc] = zd{hr[v], hi[v]};
+            }
+#pragma unroll
+            for (int q = 1; q < NZW; ++q) {
+                const int pp = ZLO + q;
+                if (J >= 0 && J + pp < nblk) {
+                    const zd *xp = la4_tile<D>(m, pp);
+                    mfma_update(Zr[q - 1], Zi[q - 1], Zr[q], Zi[q], xp, xp, jc, rq);
+                } else {
+                    Zr[q - 1] = Zr[q];
+                    Zi[q - 1] = Zi[q];
+                }
+            }
+            if constexpr (topZ) {
+                Zr[NZX - 1] = eZr;
+                Zi[NZX - 1] = eZi;
+            }
+            // own diagonals: position p -> p-1, the entering tile takes the last position
+#pragma unroll
+            for (int p = 1; p < LA; ++p) {
+                if (J >= 0 && J + p + dA < nblk)
+                    mfma_update(Ar[p - 1], Ai[p - 1], Ar[p], Ai[p], la4_tile<D>(m, p + dA), la4_tile<D>(m, p), jc,
+                                rq);
+                else {
+                    Ar[p - 1] = Ar[p];
+                    Ai[p - 1] = Ai[p];
+                }
+            }
+            Ar[LA - 1] = eAr;
+            Ai[LA - 1] = eAi;
+            if constexpr (LB > 0) {
+#pragma unroll
+                for (int p = 1; p < LB; ++p) {
+                    if (J >= 0 && J + p + dB < nblk)
+                        mfma_update(Br[p - 1], Bi[p - 1], Br[p], Bi[p], la4_tile<D>(m, p + dB), la4_tile<D>(m, p),
+                                    jc, rq);
+                    else {
+                        Br[p - 1] = Br[p];
+                        Bi[p - 1] = Bi[p];
+                    }
+                }
+                Br[LB - 1] = eBr;
+                Bi[LB - 1] = eBi;
+            }
+        }
+        __syncthreads();
+    }
+    if (isP && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+
+template <int D, int WV>
+__device__ __forceinline__ void la4_dispatch(int wave, const Factor16Args &A, La4Lds<D> &m) {
+    if constexpr (WV < La4<D>::W) {
+        if (wave == WV) la4_wave<D, WV>(A, m);
+        else la4_dispatch<D, WV + 1>(wave, A, m);
+    }
+}
+
+}  // namespace
+
+template <int D>
+__global__ void __launch_bounds__(La4<D>::NT, 2) crt_factor16la_kernel(Factor16Args A) {
+    __shared__ La4Lds<D> m;
+    if ((int)blockIdx.x >= A.nsys) return;
+    for (int t = threadIdx.x; t < La4<D>::NU * 256; t += La4<D>::NT) m.HO[t >> 8][t & 255] = zd{0.0, 0.0};
+    __syncthreads();
+    la4_dispatch<D, 0>(threadIdx.x >> 6, A, m);
+}
 
 #define CRT_F16_MAXD 10
 
 template <int D>
 static int launch16(hipStream_t st, const Factor16Args &a) {
+    if constexpr (D >= 2) {
+        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
+        if (!v2) {
+            if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL(crt_factor16la_kernel<D>, dim3(a.nsys), dim3(La4<D>::NT), 0, st, a);
+            return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+        }
+    }
     hipLaunchKernelGGL(crt_factor16_kernel<D>, dim3(a.nsys), dim3(Cfg<D>::NT), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

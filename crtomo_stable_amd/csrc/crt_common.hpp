@@ -115,4 +115,10 @@ struct DevGrid {
     const double *relbg;      // [imn][rel]
     const double *kg;         // [k][l][rel]
     const zd *sigma;          // elanz
+    // 16x16 tile map of the lower band: for block row I and block diagonal d
+    // (0..D16), tmap[((I*(D16+1) + d)*16 + r)*16 + c] is the CSR index of entry
+    // (16I + r, 16(I-d) + c), -1 if that entry is zero, -2 for the unit diagonal
+    // of padding rows past sanz
+    const int *tmap;
+    int tmapD;                // D16 the map was built for (0 if none)
 };
