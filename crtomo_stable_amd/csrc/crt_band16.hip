@@ -81,6 +81,13 @@ __device__ __forceinline__ zd crecip(zd p, bool &bad) {
 }
 
 __device__ __forceinline__ zd readlane_z(zd v, int src) { return zd{readlane_d(v.re, src), readlane_d(v.im, src)}; }
+// 16-byte loads and stores of a complex value (the address must be 16-byte aligned)
+__device__ __forceinline__ zd ldz(const zd *p) {
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    return zd{v.x, v.y};
+}
+__device__ __forceinline__ void stz(zd *p, zd z) { *reinterpret_cast<double2 *>(p) = make_double2(z.re, z.im); }
+
 __device__ __forceinline__ zd zsel(bool c, zd a, zd b) { return zd{c ? a.re : b.re, c ? a.im : b.im}; }
 
 template <int D>
@@ -118,7 +125,7 @@ namespace {
 
 // LDS of one workgroup; a static layout keeps every address an immediate
 template <int D>
-struct F16Lds {
+struct alignas(16) F16Lds {
     zd P[(D + 1) * T16 * PS];         // panel tiles of block column J, 16 x PS each
     zd U[T16 * PS];                   // unit lower factor u(i,c) = a(i,c)/p_c of the diagonal tile
     zd S[(D + 1) * 256];              // staging tiles of the entering row block, 16 x 16 each
@@ -164,8 +171,8 @@ __device__ __forceinline__ void mfma_update(d4 &dr, d4 &di, const d4 &tr, const 
     zd va[4], vb[4];
 #pragma unroll
     for (int kc = 0; kc < 4; ++kc) {
-        va[kc] = Pa[jc * PS + 4 * kc + rq];
-        vb[kc] = Pb[jc * PS + 4 * kc + rq];
+        va[kc] = ldz(Pa + jc * PS + 4 * kc + rq);
+        vb[kc] = ldz(Pb + jc * PS + 4 * kc + rq);
     }
     d4 xr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].re, vb[0].re, tr, 0, 0, 0);
     d4 xi = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[0].re, vb[0].im, ti, 0, 0, 0);
@@ -450,7 +457,7 @@ struct La4 {
 };
 
 template <int D>
-struct La4Lds {
+struct alignas(16) La4Lds {
     zd P[D * T16 * PS];       // X(J+d, J), d = 1..D
     zd Li[T16 * PS];          // L(J,J)^-1, lower, row-major
     zd HO[La4<D>::NU][256];   // hand-off tiles: HO[u-1] from wave u to wave u-1 (u = 1: to wave 0)
@@ -464,8 +471,8 @@ __device__ __forceinline__ void mfma_mul(d4 &xr, d4 &xi, const zd *Pa, const zd 
     zd va[4], vb[4];
 #pragma unroll
     for (int kc = 0; kc < 4; ++kc) {
-        va[kc] = Pa[jc * PS + 4 * kc + rq];
-        vb[kc] = Pb[jc * PS + 4 * kc + rq];
+        va[kc] = ldz(Pa + jc * PS + 4 * kc + rq);
+        vb[kc] = ldz(Pb + jc * PS + 4 * kc + rq);
     }
     const d4 z = d4{0.0, 0.0, 0.0, 0.0};
     xr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[0].re, vb[0].re, z, 0, 0, 0);
@@ -484,27 +491,22 @@ __device__ __forceinline__ void mfma_mul(d4 &xr, d4 &xi, const zd *Pa, const zd 
 template <int D>
 __device__ __forceinline__ zd *la4_tile(La4Lds<D> &m, int d) { return m.P + (d - 1) * T16 * PS; }
 
-// tile (I, I-d) of the scaled system in the accumulator layout, gathered through the map
+// map entries (CSR indices) of this lane's four elements of tile (I, I-d)
 template <int D>
-__device__ __forceinline__ void la4_gather(const DevGrid &g, const zd *vals, int I, int d, int jc, int rq, d4 &tr,
-                                           d4 &ti) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        tr[v] = 0.0;
-        ti[v] = 0.0;
-    }
-    if (I >= g.nblk) return;
+__device__ __forceinline__ void la4_map(const DevGrid &g, int I, int d, int jc, int rq, int (&mi)[4]) {
     const int *mp = g.tmap + ((size_t)(I * (D + 1) + d) * T16) * T16 + jc;
 #pragma unroll
+    for (int v = 0; v < 4; ++v) mi[v] = I < g.nblk ? mp[(4 * v + rq) * T16] : -1;
+}
+
+// tile values in the accumulator layout from the map entries
+__device__ __forceinline__ void la4_vals(const zd *vals, const int (&mi)[4], d4 &tr, d4 &ti) {
+#pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int q = mp[(4 * v + rq) * T16];
-        if (q >= 0) {
-            const zd e = vals[q];
-            tr[v] = e.re;
-            ti[v] = e.im;
-        } else if (q == -2) {
-            tr[v] = 1.0;
-        }
+        const int q = mi[v];
+        const zd e = q >= 0 ? ldz(vals + q) : zd{q == -2 ? 1.0 : 0.0, 0.0};
+        tr[v] = e.re;
+        ti[v] = e.im;
     }
 }
 
@@ -531,10 +533,10 @@ __device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D>
         if (jc == c) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                m.col[4 * v + rq] = a[v];
-                m.col[16 + 4 * v + rq] = z[v];
+                stz(m.col + 4 * v + rq, a[v]);
+                stz(m.col + 16 + 4 * v + rq, z[v]);
             }
-            if (rq == 0) m.col[32] = yv;
+            if (rq == 0) stz(m.col + 32, yv);
         }
         zd dn, l2;
         if (c + 1 < T16) {
@@ -545,11 +547,11 @@ __device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D>
         }
         __builtin_amdgcn_wave_barrier();
         zd ci[4], zi[4];
-        const zd cj = m.col[jc], wc = m.col[32];
+        const zd cj = ldz(m.col + jc), wc = ldz(m.col + 32);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            ci[v] = m.col[4 * v + rq];
-            zi[v] = m.col[16 + 4 * v + rq];
+            ci[v] = ldz(m.col + 4 * v + rq);
+            zi[v] = ldz(m.col + 16 + 4 * v + rq);
         }
         __builtin_amdgcn_sched_barrier(0);
         const zd t = zsel(jc > c, zmul(cj, r), zd{0.0, 0.0});  // u(jc, c)
@@ -581,8 +583,8 @@ __device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D>
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = 4 * v + rq;
-        pg[i * T16 + jc] = i > jc ? zmul(a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0});
-        m.Li[jc * PS + i] = zmul(z[v], ivj);  // L^-1(jc, i) = (L^-T)(i, jc)
+        stz(pg + i * T16 + jc, i > jc ? zmul(a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0}));
+        stz(m.Li + jc * PS + i, zmul(z[v], ivj));  // L^-1(jc, i) = (L^-T)(i, jc)
     }
     if (rq == 0) {
         const zd y = zmul(yv, ivj);
@@ -626,6 +628,13 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
     for (int q = 0; q < NZX; ++q) Zr[q] = Zi[q] = d4{0.0, 0.0, 0.0, 0.0};
     d4 Tr = d4{0.0, 0.0, 0.0, 0.0}, Ti = Tr;  // pivot wave: tile (J+1, J+1)
     int err = 0;
+    // map entries of the entering tiles, loaded one block row ahead
+    int mA[4], mB[4], mZ[4];
+    if constexpr (!isP) {
+        la4_map<D>(g, 0, dA, jc, rq, mA);
+        if constexpr (LB > 0) la4_map<D>(g, 0, dB, jc, rq, mB);
+        if constexpr (topZ) la4_map<D>(g, 0, 0, jc, rq, mZ);
+    }
 
     for (int J = -(D + 1); J < nblk; ++J) {
         const int Ie = J + D + 1;
@@ -639,7 +648,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
             const zd *h = m.HO[0];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const zd e = h[(4 * v + rq) * T16 + jc];
+                const zd e = ldz(h + (4 * v + rq) * T16 + jc);
                 Tr[v] = e.re;
                 Ti[v] = e.im;
             }
@@ -651,11 +660,11 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 zd *pb = la4_tile<D>(m, LB > 0 ? dB : dA);
                 if (liveA)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) pa[(4 * v + rq) * PS + jc] = zd{Ar[0][v], Ai[0][v]};
+                    for (int v = 0; v < 4; ++v) stz(pa + (4 * v + rq) * PS + jc, zd{Ar[0][v], Ai[0][v]});
                 if constexpr (LB > 0)
                     if (liveB)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) pb[(4 * v + rq) * PS + jc] = zd{Br[0][v], Bi[0][v]};
+                        for (int v = 0; v < 4; ++v) stz(pb + (4 * v + rq) * PS + jc, zd{Br[0][v], Bi[0][v]});
                 __builtin_amdgcn_wave_barrier();
                 d4 xar, xai, xbr, xbi;
                 if (liveA) mfma_mul(xar, xai, pa, m.Li, jc, rq);
@@ -664,8 +673,8 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    pa[(4 * v + rq) * PS + jc] = liveA ? zd{xar[v], xai[v]} : zd{0.0, 0.0};
-                    if constexpr (LB > 0) pb[(4 * v + rq) * PS + jc] = liveB ? zd{xbr[v], xbi[v]} : zd{0.0, 0.0};
+                    stz(pa + (4 * v + rq) * PS + jc, liveA ? zd{xar[v], xai[v]} : zd{0.0, 0.0});
+                    if constexpr (LB > 0) stz(pb + (4 * v + rq) * PS + jc, liveB ? zd{xbr[v], xbi[v]} : zd{0.0, 0.0});
                 }
                 __builtin_amdgcn_wave_barrier();
                 const int half = lane >> 4, rr = lane & 15;
@@ -673,10 +682,10 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 if ((half == 0 && liveA) || (LB > 0 && half == 1 && liveB)) {
                     const zd *row = la4_tile<D>(m, d) + rr * PS;
                     zd *bp = m.bring + ((J + d) % (D + 2)) * T16 + rr;
-                    zd acc = *bp;
+                    zd acc = ldz(bp);
 #pragma unroll
-                    for (int c = 0; c < T16; ++c) acc = zfms(acc, row[c], m.yj[c]);
-                    *bp = acc;
+                    for (int c = 0; c < T16; ++c) acc = zfms(acc, ldz(row + c), ldz(m.yj + c));
+                    stz(bp, acc);
                 }
             }
             if constexpr (!topZ) {
@@ -684,7 +693,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 const zd *h = m.HO[WV];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const zd e = h[(4 * v + rq) * T16 + jc];
+                    const zd e = ldz(h + (4 * v + rq) * T16 + jc);
                     Zr[NZX - 1][v] = e.re;
                     Zi[NZX - 1][v] = e.im;
                 }
@@ -704,9 +713,12 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
         } else {
             // entering tiles of block row Ie (positions D-d at the next step)
             d4 eAr, eAi, eBr, eBi, eZr, eZi;
-            la4_gather<D>(g, vals, Ie, dA, jc, rq, eAr, eAi);
-            if constexpr (LB > 0) la4_gather<D>(g, vals, Ie, dB, jc, rq, eBr, eBi);
-            if constexpr (topZ) la4_gather<D>(g, vals, Ie, 0, jc, rq, eZr, eZi);
+            la4_vals(vals, mA, eAr, eAi);
+            if constexpr (LB > 0) la4_vals(vals, mB, eBr, eBi);
+            if constexpr (topZ) la4_vals(vals, mZ, eZr, eZi);
+            la4_map<D>(g, Ie + 1, dA, jc, rq, mA);
+            if constexpr (LB > 0) la4_map<D>(g, Ie + 1, dB, jc, rq, mB);
+            if constexpr (topZ) la4_map<D>(g, Ie + 1, 0, jc, rq, mZ);
             if (J >= 0) {
                 // store the panel X(J+d, J), d = 1..D (the diagonal tile is stored by wave 0)
                 zd *pj = panel + (size_t)J * (D + 1) * 256 + 256;
@@ -716,13 +728,13 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 for (int u = 0; u < NPS; ++u) {
                     const int t = (tid - 64) + u * (NT - 64);
                     const int d = 1 + (t >> 8), rc = t & 255;
-                    pv[u] = (t < D * 256 && J + d < nblk) ? la4_tile<D>(m, d)[(rc >> 4) * PS + (rc & 15)]
+                    pv[u] = (t < D * 256 && J + d < nblk) ? ldz(la4_tile<D>(m, d) + (rc >> 4) * PS + (rc & 15))
                                                            : zd{0.0, 0.0};
                 }
 #pragma unroll
                 for (int u = 0; u < NPS; ++u) {
                     const int t = (tid - 64) + u * (NT - 64);
-                    if (t < D * 256) pj[t] = pv[u];
+                    if (t < D * 256) stz(pj + t, pv[u]);
                 }
             }
             // diagonal 0: slot q holds position ZLO+q; position ZLO goes down to wave WV-1
@@ -734,7 +746,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                     mfma_update(hr, hi, Zr[0], Zi[0], xp, xp, jc, rq);
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) h[(4 * v + rq) * T16 + jc] = zd{hr[v], hi[v]};
+                for (int v = 0; v < 4; ++v) stz(h + (4 * v + rq) * T16 + jc, zd{hr[v], hi[v]});
             }
 #pragma unroll
             for (int q = 1; q < NZW; ++q) {
