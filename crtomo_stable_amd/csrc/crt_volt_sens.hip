@@ -171,6 +171,102 @@ __global__ void __launch_bounds__(256) crt_sens_kernel(SensArgs A) {
     else sens_element<4>(A, e, pot, eb);
 }
 
+// Electrode-pair form of the same sum.  Per element e and source pair (p, q):
+//   G(p,q) = sum_k w_k sum_{a,b} elbg(e,ab,k) phi_{k,p}(n_a) phi_{k,q}(n_b)     (symmetric)
+// so that sum_k w_k sum_{a,b} elbg (phiN-phiM)(n_a) (phiB-phiA)(n_b)
+//        = G(N,B) - G(N,A) - G(M,B) + G(M,A)   (pole electrodes: G = 0).
+// G is one (E x K*nk) x (K*nk x E) product per element, computed in 4x4
+// register blocks from LDS (Phi and T = w S Phi), then every measurement needs
+// four lookups instead of 4*nk potentials per wavenumber.
+template <int NTH, int R>
+__global__ void __launch_bounds__(NTH) crt_sens_g_kernel(SensArgs A, int Ep) {
+    extern __shared__ double smem_g[];
+    const int e = blockIdx.x;
+    if (e >= A.elanz) return;
+    const int tid = threadIdx.x;
+    const int K = A.K, E = A.eanz, nk = A.nkel[e], KA = K * nk;
+    zd *Phi = reinterpret_cast<zd *>(smem_g);  // [KA][Ep]
+    zd *T = Phi + (size_t)KA * Ep;             // [KA][Ep]
+    zd *G = Phi;                               // [Ep][Ep], after Phi and T are consumed
+    for (int t = tid; t < KA * Ep; t += NTH) {
+        const int ka = t / Ep, p = t - ka * Ep;
+        const int k = ka / nk, a = ka - k * nk;
+        const int node = A.nrel[(size_t)e * A.maxn + a] - 1;
+        Phi[t] = p < E ? A.kpotN[((size_t)node * K + k) * E + p] : zd{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int t = tid; t < KA * Ep; t += NTH) {
+        const int ka = t / Ep, q = t - ka * Ep;
+        const int k = ka / nk, a = ka - k * nk;
+        const double w = A.kwnwi[k];
+        zd acc = zd{0.0, 0.0};
+        for (int b = 0; b < nk; ++b) {
+            const int hi = a > b ? a : b, lo = a > b ? b : a;
+            const double sab = A.elbg[((size_t)k * A.npk + hi * (hi + 1) / 2 + lo) * A.elanz + e];
+            const zd ph = Phi[(k * nk + b) * Ep + q];
+            acc.re = fma(sab, ph.re, acc.re);
+            acc.im = fma(sab, ph.im, acc.im);
+        }
+        T[t] = zd{w * acc.re, w * acc.im};
+    }
+    __syncthreads();
+    const int nb4 = Ep / 4;
+    zd acc[R][4][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[r][i][j] = zd{0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int blk = tid + r * NTH;
+        if (blk < nb4 * nb4) {
+            const int pb = blk / nb4, qb = blk - pb * nb4;
+            for (int ka = 0; ka < KA; ++ka) {
+                zd ph[4], tq[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    ph[i] = Phi[ka * Ep + 4 * pb + i];
+                    tq[i] = T[ka * Ep + 4 * qb + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[r][i][j] = zfma(acc[r][i][j], ph[i], tq[j]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int blk = tid + r * NTH;
+        if (blk < nb4 * nb4) {
+            const int pb = blk / nb4, qb = blk - pb * nb4;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) G[(4 * pb + i) * Ep + 4 * qb + j] = acc[r][i][j];
+        }
+    }
+    __syncthreads();
+    const double pi = acos(-1.0);
+    for (int i = tid; i < A.nanz; i += NTH) {
+        const int4 m = A.meas[i];  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
+        auto gv = [&](int p, int q) -> zd { return (p > 0 && q > 0) ? G[(p - 1) * Ep + (q - 1)] : zd{0.0, 0.0}; };
+        const zd dum = zadd(zsub(gv(m.w, m.y), gv(m.w, m.x)), zsub(gv(m.z, m.x), gv(m.z, m.y)));
+        zd out;
+        if (A.mode == 1) {
+            zd t = zd{dum.re / pi, dum.im / pi};
+            out = zdiv(zmul(t, A.sigma[e]), A.volt[i]);
+        } else {
+            double f = A.strom ? A.strom[i] / pi : 1.0 / pi;
+            out = zd{-(dum.re * f), -(dum.im * f)};
+        }
+        A.sens[(size_t)e * A.nanz + i] = out;
+    }
+}
+
 // kpot in the reference layout kpot(sanz, eanz, kwnanz): out[(k*eanz + l)*sanz + j]
 __global__ void __launch_bounds__(256) crt_kpot_ref_kernel(int sanz, int eanz, int K, const zd *__restrict__ kpotN,
                                                            zd *__restrict__ out) {
@@ -211,6 +307,24 @@ int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int max
                     const int *nrel, const int *nkel, const int4 *meas, const double *kwnwi, const zd *kpotN,
                     const double *elbg, const zd *sigma, const zd *volt, const double *strom, zd *sens) {
     SensArgs a{elanz, nanz, eanz, K, maxn, npk, mode, nrel, nkel, meas, kwnwi, kpotN, elbg, sigma, volt, strom, sens};
+    // electrode-pair form when its LDS fits
+    {
+        const int Ep = (eanz + 3) / 4 * 4, KA = K * maxn, nb = (Ep / 4) * (Ep / 4);
+        const size_t lg = std::max((size_t)2 * KA * Ep, (size_t)Ep * Ep) * sizeof(zd);
+        static const bool direct = getenv("CRT_SENS_DIRECT") != nullptr;
+        if (!direct && lg <= 160 * 1024 && nb <= 1024) {
+            if (nb <= 256) {
+                (void)hipFuncSetAttribute((const void *)crt_sens_g_kernel<256, 1>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                hipLaunchKernelGGL((crt_sens_g_kernel<256, 1>), dim3(elanz), dim3(256), lg, s, a, Ep);
+            } else {
+                (void)hipFuncSetAttribute((const void *)crt_sens_g_kernel<512, 2>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                hipLaunchKernelGGL((crt_sens_g_kernel<512, 2>), dim3(elanz), dim3(512), lg, s, a, Ep);
+            }
+            return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+        }
+    }
     size_t lds = crt_sens_lds_bytes(maxn, K, eanz);
     if (lds > 160 * 1024) return CRT_ERR_UNSUPPORTED;
     static bool attr = false;
