@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-sens-meas", type=int, default=96)
@@ -116,7 +116,7 @@ def main():
         from crtomo_stable_amd import build
         build.build()
 
-    w = configs.c1() if args.workload == "c1" else configs.c0()
+    w = {"c0": configs.c0, "c1": configs.c1, "c3": configs.c3}[args.workload]()
     g, s = w.grid, w.survey
     ctx = lib.Context(0)
     ctx.set_grid(g)
@@ -152,7 +152,12 @@ def main():
     fac_ms = phase["factor"]
     achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
     d16 = -(-g.mb // 16)
-    kernel = f"crt_factor16_kernel<{d16}>" if d16 <= 10 and not os.environ.get("CRT_FACTOR8") else "crt_factor_kernel"
+    if os.environ.get("CRT_FACTOR8") or d16 > 16:
+        kernel = "crt_factor_kernel"
+    elif d16 == 1 or os.environ.get("CRT_FACTOR_V2") and d16 <= 10:
+        kernel = f"crt_factor16_kernel<{d16}>"
+    else:
+        kernel = f"crt_factor16{'la' if d16 <= 10 else 'st'}_kernel<{d16}>"
     roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP64_PEAK_TFLOPS, 4),
                     traffic=measured_traffic(kernel, args.workload),
@@ -173,9 +178,11 @@ def main():
         "dtype": "c128",
         "data": "synthetic",
         "config": {
-            "workload": ("c1: seeded Delaunay mesh 100x30 m, 2 m sinusoidal topography, complex GRF model, "
-                         "complex forward + bsensi Jacobian" if args.workload == "c1" else
-                         "c0: 60x20 quads, 30 electrodes, 188 dipole-dipole, forward + bsensi"),
+            "workload": {"c1": "c1: seeded Delaunay mesh 100x30 m, 2 m sinusoidal topography, complex GRF model, "
+                               "complex forward + bsensi Jacobian",
+                         "c0": "c0: 60x20 quads, 30 electrodes, 188 dipole-dipole, forward + bsensi",
+                         "c3": "c3: 400x100 quads (40501 nodes, mb 201), 96 electrodes, 4500 dipole-dipole + "
+                               "Wenner, layered model with three blocks, forward + bsensi"}[args.workload],
             "elements": g.elanz, "nodes": g.sanz, "mb": g.mb, "electrodes": E, "measurements": s.nanz,
             "wavenumbers": K, "systems_per_step": nsys, "parallelism": "1 GPU",
         },

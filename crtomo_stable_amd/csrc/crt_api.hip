@@ -397,7 +397,7 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     c->tmapD = 0;
     {
         const int D16 = std::max(1, (mb + 15) / 16), nb16 = (sanz + 15) / 16;
-        if (D16 <= 10) {
+        if (D16 <= 16) {
             std::vector<int> tmap((size_t)nb16 * (D16 + 1) * 256, -1);
             for (int i = 0; i < sanz; ++i)
                 for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
@@ -561,7 +561,7 @@ int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    // 16x16 matrix-core tiles when the band fits (mb <= 160), else 8x8 LDS tiles
+    // 16x16 matrix-core tiles when the band fits (mb <= 256), else 8x8 LDS tiles
     const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
     const int NBt = use16 ? 16 : CRT_NB;
     const int K = c->kwnanz, E = c->eanz;

@@ -511,9 +511,9 @@ __device__ __forceinline__ void la4_vals(const zd *vals, const int (&mi)[4], d4 
 }
 
 // diagonal factorisation of block Jn on the pivot wave
-template <int D>
-__device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D> &m, int s, int Jn, int lane,
-                                                const d4 &tr, const d4 &ti, int &err) {
+template <int D, class LDS>
+__device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, LDS &m, int s, int Jn, int lane,
+                                                const d4 &tr, const d4 &ti, zd yv, int &err) {
     const int nblk = A.g.nblk;
     const int jc = lane & 15, rq = lane >> 4;
     zd a[4], z[4];
@@ -522,7 +522,6 @@ __device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, La4Lds<D>
         a[v] = zd{tr[v], ti[v]};
         z[v] = zd{4 * v + rq == jc ? 1.0 : 0.0, 0.0};
     }
-    zd yv = m.bring[(Jn % (D + 2)) * T16 + jc];
     zd p = readlane_z(a[0], 0);
     bool bad;
     zd r = crecip(p, bad);
@@ -708,7 +707,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                     const zd *x1 = la4_tile<D>(m, 1);
                     mfma_update(Tr, Ti, Tr, Ti, x1, x1, jc, rq);
                 }
-                la4_factor_diag<D>(A, m, s, Jn, lane, Tr, Ti, err);
+                la4_factor_diag<D>(A, m, s, Jn, lane, Tr, Ti, ldz(m.bring + (Jn % (D + 2)) * T16 + jc), err);
             }
         } else {
             // entering tiles of block row Ie (positions D-d at the next step)
@@ -815,11 +814,148 @@ __global__ void __launch_bounds__(La4<D>::NT, 2) crt_factor16la_kernel(Factor16A
     la4_dispatch<D, 0>(threadIdx.x >> 6, A, m);
 }
 
-#define CRT_F16_MAXD 10
+// ---------------------------------------------------------------------------
+// Streaming left-looking variant for wide bands (D = 11..16), where the window
+// no longer fits on one CU.  Per block column J:
+//   stage   L(J, J-m), m = 1..D (row block J of the previous panels) into LDS;
+//   update  C(a) = A(J+a, J) - sum_m L(J+a, J-m) L(J, J-m)^T on the matrix cores,
+//           wave 0 takes C(0) and the factor's right-hand side, waves 1..7 the
+//           off-diagonal tiles (operands L(J+a, J-m) streamed from the panels);
+//   factor  C(0) on wave 0 (as in the look-ahead variant: L, L^-1, y);
+//   solve   X(a) = C(a) (L^-1)^T on the matrix cores, stored as panel column J.
+namespace {
+constexpr int SWV = 8;  // wavefronts of the streaming variant
+
+template <int D>
+struct alignas(16) StLds {
+    zd Lrow[D * T16 * PS];  // L(J, J-m), m = 1..D; reused as per-wave staging in the solve phase
+    zd Li[T16 * PS];
+    zd col[48];
+    zd pinv[T16], sdiag[T16];
+    zd yj[T16];
+};
+static_assert(SWV * T16 * PS <= 11 * T16 * PS, "staging fits in Lrow for D >= 11");
+
+// T -= Pa Pb^T with Pa a row-major 16x16 tile in global memory (stride 16), Pb in LDS (stride PS)
+__device__ __forceinline__ void mfma_update_gl(d4 &tr, d4 &ti, const zd *Pa, const zd *Pb, int jc, int rq) {
+    zd va[4], vb[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        va[kc] = ldz(Pa + jc * T16 + 4 * kc + rq);
+        vb[kc] = ldz(Pb + jc * PS + 4 * kc + rq);
+    }
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].re, vb[kc].re, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].re, vb[kc].im, ti, 0, 0, 0);
+        tr = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kc].im, vb[kc].im, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].im, vb[kc].re, ti, 0, 0, 0);
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(SWV * 64, 2) crt_factor16st_kernel(Factor16Args A) {
+    __shared__ StLds<D> m;
+    const int s = blockIdx.x;
+    if (s >= A.nsys) return;
+    const DevGrid &g = A.g;
+    const int nblk = g.nblk;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const int enode = g.enr[A.sys[s].x];
+    const double *fak = A.fak + (size_t)s * nblk * T16;
+    const zd *vals = A.vals + (size_t)s * g.nnz;
+    zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
+    const zd *yout = A.y + (size_t)s * nblk * T16;
+    constexpr int NA = (D + SWV - 2) / (SWV - 1);  // off-diagonal tiles per update wave
+    int err = 0;
+    for (int J = 0; J < nblk; ++J) {
+        // ---- stage row block J of the previous panels
+        for (int t = tid; t < D * 256; t += SWV * 64) {
+            const int mm = 1 + (t >> 8), rc = t & 255;
+            const zd v = J - mm >= 0 ? ldz(panel + ((size_t)(J - mm) * (D + 1) + mm) * 256 + rc) : zd{0.0, 0.0};
+            stz(m.Lrow + (mm - 1) * T16 * PS + (rc >> 4) * PS + (rc & 15), v);
+        }
+        __syncthreads();
+        // ---- left-looking update of block column J
+        d4 cr[NA], ci[NA];
+        d4 dr, di;
+        zd yv = zd{0.0, 0.0};
+        if (wave == 0) {
+            int mi[4];
+            la4_map<D>(g, J, 0, jc, rq, mi);
+            la4_vals(vals, mi, dr, di);
+            for (int mm = 1; mm <= D && J - mm >= 0; ++mm) {
+                const zd *lr = m.Lrow + (mm - 1) * T16 * PS;
+                mfma_update(dr, di, dr, di, lr, lr, jc, rq);
+            }
+            // right-hand side of block J: b_J - sum_m L(J, J-m) y_{J-m}
+            const int i = J * T16 + jc;
+            yv = (i < g.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+            for (int mm = 1; mm <= D && J - mm >= 0; ++mm) {
+                const zd *lr = m.Lrow + (mm - 1) * T16 * PS + jc * PS;
+                const zd *yp = yout + (size_t)(J - mm) * T16;
+#pragma unroll
+                for (int c = 0; c < T16; ++c) yv = zfms(yv, ldz(lr + c), ldz(yp + c));
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int a = wave + (SWV - 1) * u;
+                if (a <= D && J + a < nblk) {
+                    int mi[4];
+                    la4_map<D>(g, J + a, a, jc, rq, mi);
+                    la4_vals(vals, mi, cr[u], ci[u]);
+                    for (int mm = 1; mm <= D - a && J - mm >= 0; ++mm)
+                        mfma_update_gl(cr[u], ci[u], panel + ((size_t)(J - mm) * (D + 1) + a + mm) * 256,
+                                       m.Lrow + (mm - 1) * T16 * PS, jc, rq);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- diagonal factorisation of block J (wave 0)
+        if (wave == 0) la4_factor_diag<D>(A, m, s, J, lane, dr, di, yv, err);
+        __syncthreads();
+        // ---- X(a) = C(a) (L^-1)^T, stored as panel column J
+        if (wave > 0) {
+            zd *stg = m.Lrow + (wave - 1) * T16 * PS;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int a = wave + (SWV - 1) * u;
+                if (a <= D) {
+                    zd *pg = panel + ((size_t)J * (D + 1) + a) * 256;
+                    if (J + a < nblk) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) stz(stg + (4 * v + rq) * PS + jc, zd{cr[u][v], ci[u][v]});
+                        __builtin_amdgcn_wave_barrier();
+                        d4 xr, xi;
+                        mfma_mul(xr, xi, stg, m.Li, jc, rq);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{xr[v], xi[v]});
+                        __builtin_amdgcn_wave_barrier();
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{0.0, 0.0});
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0 && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+}  // namespace
+
+#define CRT_F16_MAXD 16
 
 template <int D>
 static int launch16(hipStream_t st, const Factor16Args &a) {
-    if constexpr (D >= 2) {
+    if constexpr (D > 10) {
+        if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL(crt_factor16st_kernel<D>, dim3(a.nsys), dim3(SWV * 64), 0, st, a);
+        return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+    }
+    if constexpr (D >= 2 && D <= 10) {
         static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
         if (!v2) {
             if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
@@ -850,6 +986,12 @@ int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2
         case 8: return launch16<8>(st, a);
         case 9: return launch16<9>(st, a);
         case 10: return launch16<10>(st, a);
+        case 11: return launch16<11>(st, a);
+        case 12: return launch16<12>(st, a);
+        case 13: return launch16<13>(st, a);
+        case 14: return launch16<14>(st, a);
+        case 15: return launch16<15>(st, a);
+        case 16: return launch16<16>(st, a);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }
@@ -1097,6 +1239,12 @@ int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, in
         case 8: return launch_solve16<8>(st, a);
         case 9: return launch_solve16<9>(st, a);
         case 10: return launch_solve16<10>(st, a);
+        case 11: return launch_solve16<11>(st, a);
+        case 12: return launch_solve16<12>(st, a);
+        case 13: return launch_solve16<13>(st, a);
+        case 14: return launch_solve16<14>(st, a);
+        case 15: return launch_solve16<15>(st, a);
+        case 16: return launch_solve16<16>(st, a);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }

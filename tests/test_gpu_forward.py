@@ -152,3 +152,28 @@ def test_forward_sytop(ctx, oracle_lib):
     kpot = O.forward(w.grid, w.sigma, elbg, relbg, okg, olb)
     ovolt, _ = O.bvolti(w.grid, w.survey, O.rtrafo(kpot, okwi))
     assert float(np.max(np.abs(volt - ovolt) / np.abs(ovolt))) <= TOL
+
+
+@pytest.mark.parametrize("nx,nz,mixed", [(100, 100, True), (120, 100, True), (150, 130, True), (120, 100, False)])
+def test_wide_band_sampled(ctx, oracle_lib, nx, nz, mixed):
+    """Band widths of 161..241 nodes (11..16 tiles of 16 rows, the streaming
+    factorisation): kpot of sampled (electrode, wavenumber) systems against the
+    oracle.  The no-flow case compares mean-removed potentials (see
+    test_forward_parity_noflow)."""
+    O = oracle_lib
+    w = configs.small_quad(nx=nx, nz=nz, nelec=8, mixed=mixed)
+    D = -(-w.grid.mb // 16)
+    print(f"[wide band] nx={nx} nz={nz} mixed={mixed} sanz={w.grid.sanz} mb={w.grid.mb} D={D}", flush=True)
+    assert D > 10
+    _run_gpu(ctx, w, 1)
+    kpot = ctx.get_kpot()
+    kwn, kwi, _, _ = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn)
+    K, E = kwn.size, w.grid.eanz
+    picks = [(1, 1), (E // 2, K // 2), (E, K)] if mixed else [(1, 1), (E, K)]
+    ref = O.forward(w.grid, w.sigma, elbg, relbg, kg, lb, systems=picks)
+    for l, k in picks:
+        a = kpot[k - 1, l - 1]; b = ref[k - 1, l - 1]
+        if not mixed:
+            a = a - a.mean(); b = b - b.mean()
+        assert np.abs(a - b).max() / np.abs(b).max() <= TOL, (l, k)
