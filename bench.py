@@ -94,6 +94,62 @@ def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
                 forward_s=fwd_full, sens_s=sens_full)
 
 
+def gn_step(ctx, w, ltri: int):
+    """One complex Gauss-Newton step of the inversion loop (src/inv.f90: forward on
+    the current model, bvolti, bsensi, then update -> cjg) on the device, with the
+    regularisation (bsmatm) and the starting lambda (blam0) also built there.
+    Synthetic data: voltages of the workload's model with 3 % + 1e-4 V Gaussian
+    noise; start model homogeneous 100 ohm m, -5 mrad; CG to eps = 1e-12 on the
+    squared residual (relative residual 1e-6), ncgmax = manz / 10
+    (src/rall.f90:735-738).  Returns phase times in ms and the CG figures."""
+    import numpy as np
+    g = w.grid
+    ctx.set_sigma(w.sigma)
+    ctx.forward()
+    v_true, _ = ctx.voltages(1)
+    rng = np.random.default_rng(11)
+    err = 0.03 * np.abs(v_true) + 1e-4
+    v_obs = v_true + err * rng.standard_normal(v_true.size) * v_true / np.abs(v_true)
+    dat = -np.log(v_obs)
+    wmatd = 1.0 / (err / np.abs(v_true)) ** 2
+    wdfak = np.ones(v_true.size, np.int32)
+    sig0 = np.full(g.elanz, (1.0 / 100.0) * np.exp(-1j * 5e-3))
+    par = np.log(sig0)
+    manz = g.elanz
+    ncgmax = manz // 10
+    device_sync()
+    t = {}
+    t0 = time.perf_counter()
+    ctx.set_sigma(sig0)
+    ctx.forward()
+    ctx.voltages(1)
+    device_sync()
+    t["forward"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ctx.sens(1)
+    device_sync()
+    t["sens"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    ctx.bsmatm(ltri, 1.0, 1.0, nx=g.nx, nz=getattr(g, "nz", 0))
+    lam = ctx.blam0(wmatd, wdfak, 1.0, 1.0)
+    device_sync()
+    t["bsmatm_blam0"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    r = ctx.update(lam, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+    device_sync()
+    t["update"] = time.perf_counter() - t1
+    t["step"] = time.perf_counter() - t0
+    tm = ctx.times()
+    nanz = v_true.size
+    j_bytes = 16.0 * nanz * manz
+    it_ms = tm["cg_matvec"]
+    return {"phase_ms": {k: round(1e3 * v, 3) for k, v in t.items()}, "lambda0": lam, "ncg": r["ncg"],
+            "ncgmax": ncgmax, "cg_rel_residual": float(np.sqrt(r["cgres"][r["ncg"]])) if r["ncg"] > 0 else None,
+            "cg_ms_per_iteration": round(it_ms, 4),
+            "cg_matvec_GBps": round(2.0 * j_bytes / (it_ms * 1e-3) / 1e9, 1) if it_ms > 0 else None,
+            "J_bytes": j_bytes}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,10 +157,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 512 MiB three times (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--gn", action="store_true",
+                    help="also time one Gauss-Newton step (forward, bsensi, bsmatm, blam0, update/cjg)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-sens-meas", type=int, default=96)
     ap.add_argument("--cpu-fwd-electrodes", type=int, default=4)
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_copy": a.numel() * 8, "copies": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in "
               "DESIGN.md (shard the (electrode, wavenumber) systems, all-reduce the CG vectors)",
@@ -190,6 +259,8 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "roofline": roofline,
     }
+    if args.gn:
+        out["gn_step"] = gn_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
     if not args.no_cpu_baseline:
         try:
             cb = cpu_baseline(w, args.cpu_threads, args.cpu_sens_meas, args.cpu_fwd_electrodes)

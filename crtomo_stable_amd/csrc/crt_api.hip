@@ -64,6 +64,13 @@ int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
 int crt_launch_cg_init(hipStream_t, const CGArgs &, const zd *);
 int crt_launch_cg_iter(hipStream_t, const CGArgs &);
 int crt_launch_cg_final(hipStream_t, int, double, const double *, zd *, zd *);
+int crt_launch_bsmatmreg(hipStream_t, int, int, int, int, const int *, const double *, const double *, double, double,
+                         double *);
+int crt_launch_bsmatmtri(hipStream_t, int, int, const int *, const int *, const double *, const double *, const int *,
+                         double, double, double *);
+size_t crt_blam0_scratch(int, int, int *, int *);
+int crt_launch_blam0(hipStream_t, int, int, int, const zd *, const double *, zd *, int, int, zd *, double *, double,
+                     double *);
 
 namespace {
 
@@ -126,6 +133,7 @@ struct crt_ctx {
     int ltri = -1, rnx = 0, rncol = 0, sens_nanz = 0, sens_manz = 0;
     DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
     DBuf<int> d_nachbar;
+    DBuf<double> d_jtj;
     DBuf<zd> d_res, d_rm, d_bvec, d_r, d_p, d_q, d_bv, d_dpar, d_ap, d_part_ap, d_par;
     DBuf<CGScalars> d_sc;
     int cg_check_every = 16;
@@ -150,9 +158,15 @@ struct crt_ctx {
         if (hipEventSynchronize(ev1[i]) == hipSuccess && hipEventElapsedTime(&ms, ev0[i], ev1[i]) == hipSuccess)
             times[i] = ms;
     }
+    // device -> host copy ordered after the work queued on the context stream (which
+    // does not synchronise with the null stream)
+    hipError_t copy_out(void *dst, const void *src, size_t bytes) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+        return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+    }
     int status_check() {
         int st = 0;
-        if (hipMemcpy(&st, d_status.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CRT_ERR_HIP;
+        if (copy_out(&st, d_status.p, sizeof(int)) != hipSuccess) return CRT_ERR_HIP;
         return st;
     }
     int status_reset() {
@@ -537,11 +551,11 @@ int crt_get_precal(crt_ctx *c, double *elbg, double *relbg, double *kg) {
     if (!c->have_precal) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
     const int K = c->kwnanz;
-    if (elbg) CRT_HIP(hipMemcpy(elbg, c->d_elbg.p, (size_t)c->elanz * c->npk * K * sizeof(double), hipMemcpyDeviceToHost));
+    if (elbg) CRT_HIP(c->copy_out(elbg, c->d_elbg.p, (size_t)c->elanz * c->npk * K * sizeof(double)));
     if (relbg && c->relanz)
-        CRT_HIP(hipMemcpy(relbg, c->d_relbg.p, (size_t)c->relanz * c->npk * sizeof(double), hipMemcpyDeviceToHost));
+        CRT_HIP(c->copy_out(relbg, c->d_relbg.p, (size_t)c->relanz * c->npk * sizeof(double)));
     if (kg && c->relanz)
-        CRT_HIP(hipMemcpy(kg, c->d_kg.p, (size_t)c->relanz * c->eanz * K * sizeof(double), hipMemcpyDeviceToHost));
+        CRT_HIP(c->copy_out(kg, c->d_kg.p, (size_t)c->relanz * c->eanz * K * sizeof(double)));
     return 0;
 }
 
@@ -646,7 +660,7 @@ int crt_forward(crt_ctx *c) {
     }
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol;
     std::vector<int> st(sys.size() + 1);
-    CRT_HIP(hipMemcpy(st.data(), c->d_status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    CRT_HIP(c->copy_out(st.data(), c->d_status.p, st.size() * sizeof(int)));
     int worst = 0;
     for (int v : st) worst = std::max(worst, v);
     if (worst && getenv("CRT_DEBUG")) {
@@ -656,8 +670,7 @@ int crt_forward(crt_ctx *c) {
                         st[q]);
                 if (q - 1 < batch) {
                     std::vector<double> dg(c->sanz);
-                    (void)hipMemcpy(dg.data(), c->d_dga.p + (q - 1) * fstride, c->sanz * sizeof(double),
-                                    hipMemcpyDeviceToHost);
+                    (void)c->copy_out(dg.data(), c->d_dga.p + (q - 1) * fstride, c->sanz * sizeof(double));
                     for (int i = 0; i < c->sanz; ++i)
                         if (!(dg[i] > 0.0) || dg[i] == 1.0) {
                             fprintf(stderr, "  row %d |a_ii| = %g\n", i, dg[i]);
@@ -759,7 +772,8 @@ int crt_get_sens(crt_ctx *c, double *sens) {
     if (!c || !sens) return CRT_ERR_ARG;
     if (!c->have_sens) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    CRT_HIP(hipMemcpy(sens, c->d_sens.p, (size_t)c->nanz * c->elanz * sizeof(zd), hipMemcpyDeviceToHost));
+    const size_t n = (size_t)(c->sens_nanz ? c->sens_nanz : c->nanz) * (c->sens_manz ? c->sens_manz : c->elanz);
+    CRT_HIP(c->copy_out(sens, c->d_sens.p, n * sizeof(zd)));
     return 0;
 }
 
@@ -791,6 +805,50 @@ int crt_set_regularization(crt_ctx *c, int ltri, int nx, int manz, int maxn, con
     if (!e && ltri != 0) e = c->d_nachbar.upload(nachbar, (size_t)manz * (maxn + 1), c->stream);
     if (!e && ltri == 0) e = c->d_nachbar.ensure(1);
     if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->ltri = ltri;
+    c->rnx = nx;
+    c->rncol = ncol;
+    c->sens_manz = manz;
+    return 0;
+}
+
+int crt_bsmatm(crt_ctx *c, int ltri, int nx, int nz, double alfx, double alfz, double *smatm) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    if (ltri != 0 && ltri != 1 && ltri != 2) return CRT_ERR_UNSUPPORTED;
+    CRT_HIP(hipSetDevice(c->device));
+    const int manz = c->elanz, maxn = c->maxn;
+    int e = 0, ncol;
+    if (ltri == 0) {
+        // regular grid: quads numbered row by row from the surface
+        if (nx <= 0 || nz <= 0 || (long)nx * nz != manz || maxn < 4) return CRT_ERR_ARG;
+        for (int i = 0; i < manz; ++i)
+            if (c->nkel[i] != 4) return CRT_ERR_UNSUPPORTED;
+        ncol = 3;
+        e = c->d_smatm.ensure((size_t)manz * ncol);
+        e = e ? e : c->d_nachbar.ensure(1);
+        e = e ? e : crt_launch_bsmatmreg(c->stream, nx, nz, manz, maxn, c->d_nrel.p, c->d_x.p, c->d_y.p, alfx, alfz,
+                                         c->d_smatm.p);
+    } else {
+        // every edge of every element is a node pair of its own: one element type
+        for (int i = 0; i < manz; ++i)
+            if (c->nkel[i] != maxn) return CRT_ERR_UNSUPPORTED;
+        ncol = maxn + 1;
+        std::vector<int> nb((size_t)manz * (maxn + 1));
+        e = crt_bnachbar(manz, maxn, c->nrel.data(), nb.data());
+        e = e ? e : c->d_nachbar.upload(nb.data(), nb.size(), c->stream);
+        e = e ? e : c->d_smatm.ensure((size_t)manz * ncol);
+        e = e ? e : crt_launch_bsmatmtri(c->stream, manz, maxn, c->d_nrel.p, c->d_nkel.p, c->d_x.p, c->d_y.p,
+                                         c->d_nachbar.p, alfx, alfz, c->d_smatm.p);
+    }
+    e = e ? e : c->d_rdiag.ensure(manz);
+    if (e) return e;
+    CRT_HIP(hipMemcpyAsync(c->d_rdiag.p, c->d_smatm.p + (size_t)(ltri == 0 ? 0 : maxn) * manz, manz * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream));
+    if (smatm)
+        CRT_HIP(hipMemcpyAsync(smatm, c->d_smatm.p, (size_t)manz * ncol * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
     CRT_HIP(hipStreamSynchronize(c->stream));
     c->ltri = ltri;
     c->rnx = nx;
@@ -887,9 +945,30 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if ((e = cg_prepare(c, nanz, manz, ncgmax, lam, eps, a))) return e;
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
-    if (dpar) CRT_HIP(hipMemcpy(dpar, c->d_dpar.p, manz * sizeof(zd), hipMemcpyDeviceToHost));
-    if (cgres) CRT_HIP(hipMemcpy(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (dpar) CRT_HIP(c->copy_out(dpar, c->d_dpar.p, manz * sizeof(zd)));
+    if (cgres) CRT_HIP(c->copy_out(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double)));
     if (ncg) *ncg = n;
+    return 0;
+}
+
+int crt_blam0(crt_ctx *c, int fpi, const double *wmatd, const int *wdfak, double alfx, double alfz,
+              double *lammax) {
+    if (!c || !wmatd || !wdfak || !lammax || !(alfx + alfz != 0.0)) return CRT_ERR_ARG;
+    if (!c->have_sens) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int ncb, cb;
+    const size_t np = crt_blam0_scratch(nanz, manz, &ncb, &cb);
+    int e = cg_weights(c, nanz, wmatd, wdfak);
+    e = e ? e : c->d_part_ap.ensure(np);
+    e = e ? e : c->d_ap.ensure(nanz);
+    e = e ? e : c->d_jtj.ensure((size_t)manz + 1);
+    e = e ? e : crt_launch_blam0(c->stream, nanz, manz, fpi ? 1 : 0, c->d_sens.p, c->d_w.p, c->d_part_ap.p, ncb, cb,
+                                 c->d_ap.p, c->d_jtj.p, 2.0 / (alfx + alfz), c->d_jtj.p + manz);
+    if (e) return e;
+    // the context stream does not synchronise with the null stream: copy on it
+    CRT_HIP(hipMemcpyAsync(lammax, c->d_jtj.p + manz, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CRT_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -905,7 +984,7 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
     // residual d - f(m)
     std::vector<zd> res(nanz), sa(nanz);
     if (sigmaa) std::memcpy(sa.data(), sigmaa, nanz * sizeof(zd));
-    else CRT_HIP(hipMemcpy(sa.data(), c->d_sigmaa.p, nanz * sizeof(zd), hipMemcpyDeviceToHost));
+    else CRT_HIP(c->copy_out(sa.data(), c->d_sigmaa.p, nanz * sizeof(zd)));
     const zd *dd = reinterpret_cast<const zd *>(dat);
     for (int i = 0; i < nanz; ++i) res[i] = zd{dd[i].re - sa[i].re, dd[i].im - sa[i].im};
     e = e ? e : c->d_res.upload(res.data(), nanz, c->stream);

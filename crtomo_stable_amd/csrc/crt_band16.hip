@@ -816,34 +816,62 @@ __global__ void __launch_bounds__(La4<D>::NT, 2) crt_factor16la_kernel(Factor16A
 
 // ---------------------------------------------------------------------------
 // Streaming left-looking variant for wide bands (D = 11..16), where the window
-// no longer fits on one CU.  Per block column J:
+// of D(D+1)/2 tiles no longer fits on one CU.  Per block column J:
 //   stage   L(J, J-m), m = 1..D (row block J of the previous panels) into LDS;
-//   update  C(a) = A(J+a, J) - sum_m L(J+a, J-m) L(J, J-m)^T on the matrix cores,
-//           wave 0 takes C(0) and the factor's right-hand side, waves 1..7 the
-//           off-diagonal tiles (operands L(J+a, J-m) streamed from the panels);
-//   factor  C(0) on wave 0 (as in the look-ahead variant: L, L^-1, y);
+//   update  C(a) = A(J+a, J) - sum_m L(J+a, J-m) L(J, J-m)^T on the matrix cores:
+//           the pivot wave takes C(0) and the factor's right-hand side, three update
+//           waves the off-diagonal tiles, whose operands L(J+a, J-m) stream from the
+//           panels with the next operand's load in flight during each product;
+//   factor  C(0) on the pivot wave (as in the look-ahead variant: L, L^-1, y);
 //   solve   X(a) = C(a) (L^-1)^T on the matrix cores, stored as panel column J.
+// Four waves and at most 256 VGPRs per lane keep two systems on each CU, so one
+// system's serial diagonal factorisation overlaps the other's products.
 namespace {
-constexpr int SWV = 8;  // wavefronts of the streaming variant
+constexpr int SWV = 4;        // wavefronts of the streaming variant
+constexpr int SNU = SWV - 1;  // update waves
+
+template <int D>
+struct StCfg {
+    static constexpr int NA = (D + SNU - 1) / SNU;  // off-diagonal tiles per update wave
+    // tile a (1..D, larger = void) of update wave w (1..SNU) in round u, in snake order:
+    // tile a takes D-a products, so each wave gets long and short chains alike
+    static constexpr int tile(int w, int u) { return SNU * u + 1 + ((u & 1) ? (SNU - w) : (w - 1)); }
+};
 
 template <int D>
 struct alignas(16) StLds {
-    zd Lrow[D * T16 * PS];  // L(J, J-m), m = 1..D; reused as per-wave staging in the solve phase
+    static_assert(D >= SNU, "solve staging fits in Lrow");
+    zd Lrow[D * T16 * PS];  // L(J, J-m), m = 1..D; the first SNU tiles stage C(a) in the solve
     zd Li[T16 * PS];
     zd col[48];
     zd pinv[T16], sdiag[T16];
     zd yj[T16];
 };
-static_assert(SWV * T16 * PS <= 11 * T16 * PS, "staging fits in Lrow for D >= 11");
 
-// T -= Pa Pb^T with Pa a row-major 16x16 tile in global memory (stride 16), Pb in LDS (stride PS)
-__device__ __forceinline__ void mfma_update_gl(d4 &tr, d4 &ti, const zd *Pa, const zd *Pb, int jc, int rq) {
-    zd va[4], vb[4];
+// row block J of panel columns J-m into LDS (all SWV*64 threads, one element each per tile)
+template <int D>
+__device__ __forceinline__ void st_stage(const zd *panel, StLds<D> &m, int J, int tid) {
+    zd v[D];
 #pragma unroll
-    for (int kc = 0; kc < 4; ++kc) {
-        va[kc] = ldz(Pa + jc * T16 + 4 * kc + rq);
-        vb[kc] = ldz(Pb + jc * PS + 4 * kc + rq);
-    }
+    for (int mm = 1; mm <= D; ++mm)
+        v[mm - 1] = J - mm >= 0 ? ldz(panel + ((size_t)(J - mm) * (D + 1) + mm) * 256 + tid) : zd{0.0, 0.0};
+#pragma unroll
+    for (int mm = 1; mm <= D; ++mm) stz(m.Lrow + (mm - 1) * T16 * PS + (tid >> 4) * PS + (tid & 15), v[mm - 1]);
+}
+
+// this lane's A-operand entries of L(J+a, J-mm) (row-major 16x16 panel tile)
+template <int D>
+__device__ __forceinline__ void st_operand(const zd *panel, int J, int a, int mm, int jc, int rq, zd (&o)[4]) {
+    const zd *p = panel + ((size_t)(J - mm) * (D + 1) + a + mm) * 256 + jc * T16 + rq;
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) o[kc] = ldz(p + 4 * kc);
+}
+
+// T -= Pa Pb^T with Pa's A-operand entries in registers and Pb in LDS (stride PS)
+__device__ __forceinline__ void mfma_update_rv(d4 &tr, d4 &ti, const zd (&va)[4], const zd *Pb, int jc, int rq) {
+    zd vb[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) vb[kc] = ldz(Pb + jc * PS + 4 * kc + rq);
 #pragma unroll
     for (int kc = 0; kc < 4; ++kc) {
         tr = __builtin_amdgcn_mfma_f64_16x16x4f64(-va[kc].re, vb[kc].re, tr, 0, 0, 0);
@@ -854,106 +882,164 @@ __device__ __forceinline__ void mfma_update_gl(d4 &tr, d4 &ti, const zd *Pa, con
 }
 
 template <int D>
-__global__ void __launch_bounds__(SWV * 64, 2) crt_factor16st_kernel(Factor16Args A) {
-    __shared__ StLds<D> m;
-    const int s = blockIdx.x;
-    if (s >= A.nsys) return;
+__device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m) {
     const DevGrid &g = A.g;
-    const int nblk = g.nblk;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nblk = g.nblk, s = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int jc = lane & 15, rq = lane >> 4;
     const int enode = g.enr[A.sys[s].x];
     const double *fak = A.fak + (size_t)s * nblk * T16;
     const zd *vals = A.vals + (size_t)s * g.nnz;
-    zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
+    const zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     const zd *yout = A.y + (size_t)s * nblk * T16;
-    constexpr int NA = (D + SWV - 2) / (SWV - 1);  // off-diagonal tiles per update wave
     int err = 0;
+    int mi[4];
+    d4 dr, di;
+    la4_map<D>(g, 0, 0, jc, rq, mi);
+    la4_vals(vals, mi, dr, di);
+    la4_map<D>(g, 1, 0, jc, rq, mi);
     for (int J = 0; J < nblk; ++J) {
-        // ---- stage row block J of the previous panels
-        for (int t = tid; t < D * 256; t += SWV * 64) {
-            const int mm = 1 + (t >> 8), rc = t & 255;
-            const zd v = J - mm >= 0 ? ldz(panel + ((size_t)(J - mm) * (D + 1) + mm) * 256 + rc) : zd{0.0, 0.0};
-            stz(m.Lrow + (mm - 1) * T16 * PS + (rc >> 4) * PS + (rc & 15), v);
+        st_stage<D>(panel, m, J, tid);
+        __syncthreads();
+        const int mx = J < D ? J : D;
+        // right-hand side of block J: b_J - sum_m L(J, J-m) y_{J-m}
+        const int i = J * T16 + jc;
+        zd yv = (i < g.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+        for (int mm = 1; mm <= mx; ++mm) {
+            const zd *lr = m.Lrow + (mm - 1) * T16 * PS + jc * PS;
+            const zd *yp = yout + (size_t)(J - mm) * T16;
+#pragma unroll
+            for (int c = 0; c < T16; ++c) yv = zfms(yv, ldz(lr + c), ldz(yp + c));
+        }
+        for (int mm = 1; mm <= mx; ++mm) {
+            const zd *lr = m.Lrow + (mm - 1) * T16 * PS;
+            mfma_update(dr, di, dr, di, lr, lr, jc, rq);
         }
         __syncthreads();
-        // ---- left-looking update of block column J
-        d4 cr[NA], ci[NA];
-        d4 dr, di;
-        zd yv = zd{0.0, 0.0};
-        if (wave == 0) {
-            int mi[4];
-            la4_map<D>(g, J, 0, jc, rq, mi);
-            la4_vals(vals, mi, dr, di);
-            for (int mm = 1; mm <= D && J - mm >= 0; ++mm) {
-                const zd *lr = m.Lrow + (mm - 1) * T16 * PS;
-                mfma_update(dr, di, dr, di, lr, lr, jc, rq);
-            }
-            // right-hand side of block J: b_J - sum_m L(J, J-m) y_{J-m}
-            const int i = J * T16 + jc;
-            yv = (i < g.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
-            for (int mm = 1; mm <= D && J - mm >= 0; ++mm) {
-                const zd *lr = m.Lrow + (mm - 1) * T16 * PS + jc * PS;
-                const zd *yp = yout + (size_t)(J - mm) * T16;
+        la4_factor_diag<D>(A, m, s, J, lane, dr, di, yv, err);
+        __syncthreads();
+        // diagonal tile of column J+1 (map loaded one column ahead)
+        la4_vals(vals, mi, dr, di);
+        la4_map<D>(g, J + 2, 0, jc, rq, mi);
+        __syncthreads();
+    }
+    if (lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+
+template <int D, int W>
+__device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &m) {
+    using C = StCfg<D>;
+    constexpr int NA = C::NA;
+    const DevGrid &g = A.g;
+    const int nblk = g.nblk, s = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const zd *vals = A.vals + (size_t)s * g.nnz;
+    zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
+    d4 cr[NA], ci[NA];
+    int mi[NA][4];
 #pragma unroll
-                for (int c = 0; c < T16; ++c) yv = zfms(yv, ldz(lr + c), ldz(yp + c));
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < NA; ++u) {
-                const int a = wave + (SWV - 1) * u;
-                if (a <= D && J + a < nblk) {
-                    int mi[4];
-                    la4_map<D>(g, J + a, a, jc, rq, mi);
-                    la4_vals(vals, mi, cr[u], ci[u]);
-                    for (int mm = 1; mm <= D - a && J - mm >= 0; ++mm)
-                        mfma_update_gl(cr[u], ci[u], panel + ((size_t)(J - mm) * (D + 1) + a + mm) * 256,
-                                       m.Lrow + (mm - 1) * T16 * PS, jc, rq);
-                }
-            }
+    for (int u = 0; u < NA; ++u) {
+        const int a = C::tile(W, u);
+        if (a <= D) {
+            la4_map<D>(g, a, a, jc, rq, mi[u]);
+            la4_vals(vals, mi[u], cr[u], ci[u]);
+            la4_map<D>(g, 1 + a, a, jc, rq, mi[u]);
         }
+    }
+    for (int J = 0; J < nblk; ++J) {
+        st_stage<D>(panel, m, J, tid);
         __syncthreads();
-        // ---- diagonal factorisation of block J (wave 0)
-        if (wave == 0) la4_factor_diag<D>(A, m, s, J, lane, dr, di, yv, err);
-        __syncthreads();
-        // ---- X(a) = C(a) (L^-1)^T, stored as panel column J
-        if (wave > 0) {
-            zd *stg = m.Lrow + (wave - 1) * T16 * PS;
+        // C(a) -= sum_m L(J+a, J-m) L(J, J-m)^T, next operand in flight
+        zd nx[4];
+        bool have = false;  // nx holds the first operand of round u
 #pragma unroll
-            for (int u = 0; u < NA; ++u) {
-                const int a = wave + (SWV - 1) * u;
-                if (a <= D) {
-                    zd *pg = panel + ((size_t)J * (D + 1) + a) * 256;
-                    if (J + a < nblk) {
+        for (int u = 0; u < NA; ++u) {
+            const int a = C::tile(W, u);
+            if (a <= D) {
+                const int mx = J + a < nblk ? (J < D - a ? J : D - a) : 0;
+                if (mx >= 1 && !have) st_operand<D>(panel, J, a, 1, jc, rq, nx);
+                have = false;
+                for (int mm = 1; mm <= mx; ++mm) {
+                    zd cur[4];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) stz(stg + (4 * v + rq) * PS + jc, zd{cr[u][v], ci[u][v]});
-                        __builtin_amdgcn_wave_barrier();
-                        d4 xr, xi;
-                        mfma_mul(xr, xi, stg, m.Li, jc, rq);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{xr[v], xi[v]});
-                        __builtin_amdgcn_wave_barrier();
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{0.0, 0.0});
+                    for (int kc = 0; kc < 4; ++kc) {
+                        cur[kc].re = nx[kc].re;
+                        cur[kc].im = nx[kc].im;
                     }
+                    if (mm < mx) {
+                        st_operand<D>(panel, J, a, mm + 1, jc, rq, nx);
+                    } else if (u + 1 < NA) {
+                        const int a1 = C::tile(W, u + 1);
+                        const int mx1 = J + a1 < nblk ? (J < D - a1 ? J : D - a1) : 0;
+                        if (a1 <= D && mx1 >= 1) {
+                            st_operand<D>(panel, J, a1, 1, jc, rq, nx);
+                            have = true;
+                        }
+                    }
+                    mfma_update_rv(cr[u], ci[u], cur, m.Lrow + (mm - 1) * T16 * PS, jc, rq);
                 }
+            }
+        }
+        __syncthreads();
+        __syncthreads();
+        // X(a) = C(a) (L^-1)^T, stored as panel column J
+        zd *stg = m.Lrow + (W - 1) * T16 * PS;
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int a = C::tile(W, u);
+            if (a <= D) {
+                zd *pg = panel + ((size_t)J * (D + 1) + a) * 256;
+                if (J + a < nblk) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) stz(stg + (4 * v + rq) * PS + jc, zd{cr[u][v], ci[u][v]});
+                    __builtin_amdgcn_wave_barrier();
+                    d4 xr, xi;
+                    mfma_mul(xr, xi, stg, m.Li, jc, rq);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{xr[v], xi[v]});
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) stz(pg + (4 * v + rq) * T16 + jc, zd{0.0, 0.0});
+                }
+                // entering tile of column J+1 (map loaded one column ahead), map of column J+2
+                la4_vals(vals, mi[u], cr[u], ci[u]);
+                la4_map<D>(g, J + 2 + a, a, jc, rq, mi[u]);
             }
         }
         __syncthreads();
     }
-    if (wave == 0 && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
+}
+
+template <int D, int W>
+__device__ __forceinline__ void st_dispatch(int wave, const Factor16Args &A, StLds<D> &m) {
+    if constexpr (W == 0) {
+        if (wave == 0) st_pivot_wave<D>(A, m);
+        else st_dispatch<D, 1>(wave, A, m);
+    } else if constexpr (W < SWV) {
+        if (wave == W) st_update_wave<D, W>(A, m);
+        else st_dispatch<D, W + 1>(wave, A, m);
+    }
 }
 }  // namespace
+
+template <int D>
+__global__ void __launch_bounds__(SWV * 64, 2) crt_factor16st_kernel(Factor16Args A) {
+    __shared__ StLds<D> m;
+    if ((int)blockIdx.x >= A.nsys) return;
+    st_dispatch<D, 0>(threadIdx.x >> 6, A, m);
+}
 
 #define CRT_F16_MAXD 16
 
 template <int D>
 static int launch16(hipStream_t st, const Factor16Args &a) {
-    if constexpr (D > 10) {
-        if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL(crt_factor16st_kernel<D>, dim3(a.nsys), dim3(SWV * 64), 0, st, a);
-        return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+    if constexpr (D >= SNU) {
+        static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
+        if (D > 10 || stream) {
+            if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL(crt_factor16st_kernel<D>, dim3(a.nsys), dim3(SWV * 64), 0, st, a);
+            return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+        }
     }
     if constexpr (D >= 2 && D <= 10) {
         static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;

@@ -71,6 +71,16 @@ def cutmck(grid):
     return perm, mb.value
 
 
+def bnachbar(grid):
+    """Edge neighbours of the area elements (``src/bnachbar.f90:44-90``), host code:
+    (maxn+1, elanz) = reference nachbar(elanz, maxn+1); last row = neighbour count."""
+    lib = load()
+    nrel = np.ascontiguousarray(np.asarray(grid.nrel, np.int32).reshape(-1, grid.maxn)[:grid.elanz])
+    nb = np.zeros((grid.maxn + 1, grid.elanz), np.int32)
+    _chk(lib.crt_bnachbar(grid.elanz, grid.maxn, _ptr(nrel), _ptr(nb)), "crt_bnachbar")
+    return nb
+
+
 class Context:
     """One device context holding grid, survey, element matrices, kpot and J."""
 
@@ -194,6 +204,26 @@ class Context:
         _chk(self.lib.crt_set_regularization(self._h, int(ltri), int(nx), manz, maxn, _ptr(sm), _ptr(nb)),
              "crt_set_regularization")
         self.manz = manz
+
+    def bsmatm(self, ltri, alfx=1.0, alfz=1.0, nx=0, nz=0):
+        """Regularisation built on the device (bsmatmreg for ltri 0 on the regular
+        nx x nz grid, bnachbar + bsmatmtri for ltri 1/2) and installed for the CG;
+        returns smatm as (ncol, manz) = reference smatm(manz, ncol)."""
+        g = self.grid
+        ncol = 3 if ltri == 0 else g.maxn + 1
+        sm = np.zeros((ncol, g.elanz))
+        _chk(self.lib.crt_bsmatm(self._h, int(ltri), int(nx), int(nz), C.c_double(alfx), C.c_double(alfz), _ptr(sm)),
+             "crt_bsmatm")
+        self.manz = g.elanz
+        return sm
+
+    def blam0(self, wmatd, wdfak, alfx=1.0, alfz=1.0, fpi=False):
+        """Starting lambda (``src/blam0.f90:120-175``) from the resident Jacobian."""
+        w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)
+        lam = C.c_double()
+        _chk(self.lib.crt_blam0(self._h, int(bool(fpi)), _ptr(w), _ptr(wd), C.c_double(alfx), C.c_double(alfz),
+                                C.byref(lam)), "crt_blam0")
+        return lam.value
 
     def cjg(self, lam, eps, ncgmax, wmatd, wdfak, cgfac, bvec):
         manz = np.asarray(cgfac).size

@@ -96,6 +96,26 @@ int crt_update(crt_ctx *ctx, double lam, double eps, int ncgmax, double step, co
                const int *wdfak, const double *dat, const double *sigmaa, double *par, double *dpar,
                double *cgres, int *ncg, double *cgfac, double *bvec0);
 
+/* Smoothness regularisation built on the device from the grid of crt_set_grid and
+ * installed for crt_cjg / crt_update (replaces crt_set_regularization):
+ * ltri 0 -> bsmatmreg (src/bsmatm_mod.f90:190-362) on the regular nx x nz quad grid,
+ * smatm (manz, 3); ltri 1/2 -> bnachbar + bsmatmtri (src/bnachbar.f90:44-90,
+ * src/bsmatm_mod.f90:364-460), smatm (manz, maxn+1).  smatm (column-major, may be
+ * NULL) receives a copy. */
+int crt_bsmatm(crt_ctx *ctx, int ltri, int nx, int nz, double alfx, double alfz, double *smatm);
+
+/* Edge neighbours of the area elements (src/bnachbar.f90:44-90), host code:
+ * nrel (elanz, maxn) element-major 1-based; nachbar (elanz, maxn+1) column-major,
+ * last column = neighbour count. */
+int crt_bnachbar(int elanz, int maxn, const int *nrel, int *nachbar);
+
+/* Starting regularisation parameter (src/blam0.f90:120-175) from the Jacobian on
+ * the device: lammax = 5 * 2/(alfx+alfz) * mean_j |(J^H W J 1)_j|, W = wmatd*wdfak;
+ * fpi != 0 uses the real parts only (the lfpi branch).  The fixed-lambda branches
+ * (llamf, lamnull) stay with the caller. */
+int crt_blam0(crt_ctx *ctx, int fpi, const double *wmatd, const int *wdfak, double alfx, double alfz,
+              double *lammax);
+
 /* Cuthill-McKee renumbering (cutmckee/cutmck.f:126-337): perm[old-1] = new
  * number; returns the band width in *mb. */
 int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel,

@@ -819,6 +819,41 @@ static double rdiag(const orc_cg_problem *p, int j) {
 
 /* RHS and Jacobi scaling (src/update.f90:63-136,179-231,236-293), complex case,
  * no prior, no reference model, no difference inversion. */
+/* src/blam0.f90:101-168: jtj(j) = |sum_i sum_k conj(J(i,j)) J(i,k) w_i| (lfpi: real
+ * parts, DABS), lammax = sum(jtj)/manz * 2/(alfx+alfz) * 5.  Same loop order as the
+ * reference (j outer, i, k inner). */
+double orc_blam0(const orc_cg_problem *p, int fpi, double alfx, double alfz) {
+    const int nanz = p->nanz, manz = p->manz;
+    const zdbl *J = p->sens;
+    double *jtj = (double *)malloc(sizeof(double) * (size_t)manz);
+    for (int j = 0; j < manz; ++j) {
+        if (fpi) {
+            double dum = 0.0;
+            for (int i = 0; i < nanz; ++i) {
+                double wi = p->wmatd[i] * (double)p->wdfak[i];
+                double aij = creal(J[(size_t)j * nanz + i]);
+                for (int k = 0; k < manz; ++k) dum = dum + aij * creal(J[(size_t)k * nanz + i]) * wi;
+            }
+            jtj[j] = fabs(dum);
+        } else {
+            zdbl cdum = 0.0;
+            for (int i = 0; i < nanz; ++i) {
+                zdbl wi = p->wmatd[i] * (double)p->wdfak[i];
+                zdbl aij = conj(J[(size_t)j * nanz + i]);
+                for (int k = 0; k < manz; ++k) cdum = cdum + aij * J[(size_t)k * nanz + i] * wi;
+            }
+            jtj[j] = cabs(cdum);
+        }
+    }
+    double lammax = 0.0;
+    for (int j = 0; j < manz; ++j) lammax += jtj[j];
+    free(jtj);
+    lammax = lammax / (double)manz;
+    lammax = lammax * 2.0 / (alfx + alfz);
+    lammax = lammax * 5.0;
+    return lammax;
+}
+
 void orc_update_rhs(const orc_cg_problem *p, double *cgfac, zdbl *bvec) {
     const int nanz = p->nanz, manz = p->manz;
     rmatvec(p, NULL, p->par, bvec);

@@ -112,6 +112,9 @@ typedef struct {
     const int *nachbar;     /* manz x maxn column-major (ltri 1/2) */
 } orc_cg_problem;
 void orc_update_rhs(const orc_cg_problem *p, double *cgfac, zdbl *bvec);
+/* starting regularisation parameter (src/blam0.f90:66-168, computed branch):
+ * the reference's triple loop over (j, i, k); fpi: real parts only (lfpi). */
+double orc_blam0(const orc_cg_problem *p, int fpi, double alfx, double alfz);
 void orc_cjggra(const orc_cg_problem *p, const double *cgfac, const zdbl *bvec,
                 zdbl *dpar, double *cgres, int *ncg);
 /* one CG matvec  out = C (J^H W J + lam R) C p  (src/cg_mod.f90:520-545,547-577,699-726) */

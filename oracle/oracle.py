@@ -267,6 +267,12 @@ def update_rhs(p: CGProblem):
     return cgfac, bvec
 
 
+def blam0(p: CGProblem, fpi=False, alfx=1.0, alfz=1.0):
+    f = lib().orc_blam0
+    f.restype = C.c_double
+    return f(C.byref(p.s), int(bool(fpi)), _d(alfx), _d(alfz))
+
+
 def cjggra(p: CGProblem, cgfac, bvec):
     dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1)
     ncg = _i()
@@ -365,6 +371,12 @@ class RefSession:
         nb = np.zeros((g.maxn + 1, g.elanz), np.int32)
         ref().ref_bsmatm(ltri, nx, nz, _d(alfx), _d(alfz), ptr(sm), ptr(nb))
         return sm, nb
+
+    def blam0(self, fpi=False, alfx=1.0, alfz=1.0):
+        """lammax of src/blam0.f90 on the state of the last update() call."""
+        f = ref().ref_blam0
+        f.restype = C.c_double
+        return f(int(bool(fpi)), _d(alfx), _d(alfz))
 
     def update(self, p: CGProblem):
         a = p.arrs

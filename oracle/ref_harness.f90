@@ -382,4 +382,18 @@ contains
     ref_update = errnr
   end function ref_update
 
+  ! starting regularisation parameter through src/blam0.f90:12, on the state that
+  ! ref_update left (sens, wmatd, wdfak); computed, not preset (llamf = 0)
+  real(c_double) function ref_blam0(lfpi_, alfx_, alfz_) bind(C, name='ref_blam0')
+    integer(c_int), value :: lfpi_
+    real(c_double), value :: alfx_, alfz_
+    external blam0
+    llamf = 0; lamnull_cri = 0d0; lamnull_fpi = 0d0
+    lfpi = lfpi_ /= 0; ldc = .false.; lverb = .false.
+    alfx = alfx_; alfz = alfz_
+    call blam0()
+    lfpi = .false.
+    ref_blam0 = lammax
+  end function ref_blam0
+
 end module crt_ref_harness

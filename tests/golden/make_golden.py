@@ -57,10 +57,14 @@ def cg_golden(w, R, out, ltri, lam, eps, ncgmax, seed):
     cp = O.CGProblem(out["sens"], wmatd, wdfak, dat, sa, par, smatm, ltri, w.grid.nx, lam, eps, ncgmax,
                      nachbar=nachbar if ltri else None, maxn=w.grid.maxn)
     cgfac, bvec, dpar, cgres, ncg = R.update(cp)
+    cgres[ncg + 1:] = 0.0  # the reference leaves the tail unset
+    # starting lambda (src/blam0.f90) on the same Jacobian and weights
+    lam0 = R.blam0(fpi=False, alfx=1.0, alfz=1.0)
+    lam0_fpi = R.blam0(fpi=True, alfx=2.0, alfz=0.5)
     p = f"cg{ltri}_"
     return {p + "dat": dat, p + "wmatd": wmatd, p + "par": par, p + "smatm": smatm, p + "nachbar": nachbar,
             p + "cgfac": cgfac, p + "bvec": bvec, p + "dpar": dpar, p + "cgres": cgres, p + "ncg": ncg,
-            p + "lam": lam, p + "eps": eps, p + "ncgmax": ncgmax}
+            p + "lam": lam, p + "eps": eps, p + "ncgmax": ncgmax, p + "lam0": lam0, p + "lam0_fpi": lam0_fpi}
 
 
 def main(names):

@@ -125,3 +125,38 @@ def test_cjg_dropin(ctx, oracle_lib):
     dpar, cgres, ncg = ctx.cjg(lam, eps, ncgmax, P["wmatd"], P["wdfak"], cgfac, bvec)
     assert abs(ncg - ncg_o) <= max(3, 0.03 * ncg_o)
     assert np.max(np.abs(dpar - dpar_o)) <= 1e-10 * np.max(np.abs(dpar_o))
+
+
+@pytest.mark.parametrize("name,ltri", [("small_quad", 0), ("small_tri", 1)])
+def test_device_regularization_and_blam0(ctx, name, ltri):
+    """Regularisation built on the device (bsmatmreg; bnachbar + bsmatmtri) and the
+    starting lambda (blam0) from the resident Jacobian, against the values the
+    reference build wrote for the same grid and Jacobian; then one update through
+    the device-built R against the reference's update.  smatm: bit-exact on the
+    regular grid (same operation order, no fused multiply-adds), 1e-14 on the
+    triangular grid (atan2/cos/sin of the device library); lambda 1e-12."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", f"ref_{name}.npz"))
+    w = _workload(ltri)
+    g = w.grid
+    p = f"cg{ltri}_"
+    print(f"[device R + blam0] {name} ltri={ltri} manz={g.elanz}", flush=True)
+    ctx.set_grid(g)
+    sm = ctx.bsmatm(ltri, 1.0, 1.0, nx=g.nx, nz=g.nz)
+    if ltri == 0:
+        assert np.array_equal(sm, ref[p + "smatm"])
+    else:
+        assert np.max(np.abs(sm - ref[p + "smatm"])) <= 1e-14 * np.max(np.abs(ref[p + "smatm"]))
+    ctx.set_sens(ref["sens"])
+    wmatd = ref[p + "wmatd"]
+    wdfak = np.ones(wmatd.size, np.int32)
+    lam0 = ctx.blam0(wmatd, wdfak, 1.0, 1.0)
+    lam0f = ctx.blam0(wmatd, wdfak, 2.0, 0.5, fpi=True)
+    print(f"lam0 {lam0!r} ref {float(ref[p + 'lam0'])!r}; fpi {lam0f!r} ref {float(ref[p + 'lam0_fpi'])!r}", flush=True)
+    assert abs(lam0 - float(ref[p + "lam0"])) <= 1e-12 * float(ref[p + "lam0"])
+    assert abs(lam0f - float(ref[p + "lam0_fpi"])) <= 1e-12 * float(ref[p + "lam0_fpi"])
+    eps, ncgmax, lam = float(ref[p + "eps"]), int(ref[p + "ncgmax"]), float(ref[p + "lam"])
+    r = ctx.update(lam, eps, ncgmax, wmatd, wdfak, ref[p + "dat"], ref[p + "par"], sigmaa=ref["sigmaa"])
+    ncg_ref = int(ref[p + "ncg"])
+    assert abs(r["ncg"] - ncg_ref) <= max(3, 0.03 * ncg_ref)
+    assert _rel(r["dpar"], ref[p + "dpar"]) <= (1e-2 if eps > 1e-8 else 1e-4)

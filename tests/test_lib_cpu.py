@@ -55,3 +55,21 @@ def test_no_cpu_fallback_in_product():
             if f.endswith(".py"):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in re.sub(r"#.*", "", txt).replace("oracle restatement", ""), f
+
+
+@pytest.mark.parametrize("name", ["small_quad", "small_tri"])
+def test_bnachbar_vs_reference_golden(name):
+    """Edge-sort neighbour table (host code of the library) against the table the
+    reference's bnachbar (src/bnachbar.f90) wrote for the same grid, and against the
+    oracle's pairwise scan; bit-exact."""
+    import os
+    from crtomo_stable_amd import configs, lib
+    from oracle import oracle as O
+    O.build()
+    w = configs.small_quad(cutmck=O.cutmck) if name == "small_quad" else \
+        configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    nb = lib.bnachbar(w.grid)
+    assert np.array_equal(nb, O.bnachbar(w.grid))
+    if name == "small_tri":
+        ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_small_tri.npz"))
+        assert np.array_equal(nb, ref["cg1_nachbar"])

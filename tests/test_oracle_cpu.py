@@ -244,6 +244,29 @@ def test_oracle_cg_vs_reference_golden(oracle_lib, name, ltri):
     assert rel(bvec, ref[p + "bvec"]) < 1e-14
     assert rel(dpar * cgfac, ref[p + "dpar"]) < 1e-12
     assert np.array_equal(cgres[: ncg + 1], ref[p + "cgres"][: ncg + 1])
+    # starting lambda: the oracle runs the reference's own loop order, the sums agree
+    # to rounding of the final sum order only
+    assert abs(O.blam0(cp, False, 1.0, 1.0) - float(ref[p + "lam0"])) <= 1e-13 * float(ref[p + "lam0"])
+    assert abs(O.blam0(cp, True, 2.0, 0.5) - float(ref[p + "lam0_fpi"])) <= 1e-13 * float(ref[p + "lam0_fpi"])
+
+
+def test_blam0_factorised_form(oracle_lib):
+    """blam0's triple loop equals 5 * 2/(alfx+alfz) * mean |J^H W (J 1)| (the form the
+    device computes), on a random complex J; a dropped conjugate or weight fails."""
+    O = oracle_lib
+    rng = np.random.default_rng(5)
+    nanz, manz = 37, 53
+    J = rng.standard_normal((nanz, manz)) + 1j * rng.standard_normal((nanz, manz))
+    w = rng.random(nanz) + 0.5
+    wd = rng.integers(1, 3, nanz).astype(np.int32)
+    z = np.zeros(nanz, np.complex128)
+    cp = O.CGProblem(J.T, w, wd, z, z, np.zeros(manz, np.complex128), np.zeros((3, manz)), 0, manz, 1.0, 1e-4, 1)
+    W = w * wd
+    expect = 5.0 * 2.0 / 3.0 * np.mean(np.abs(J.conj().T @ (W * J.sum(axis=1))))
+    assert abs(O.blam0(cp, False, 1.0, 2.0) - expect) <= 1e-12 * expect
+    Jr = J.real
+    expect_fpi = 5.0 * 2.0 / 3.0 * np.mean(np.abs(Jr.T @ (W * Jr.sum(axis=1))))
+    assert abs(O.blam0(cp, True, 1.0, 2.0) - expect_fpi) <= 1e-12 * expect_fpi
 
 
 def test_rwaven_bookkeeping(oracle_lib):
