@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #define CG_BLK 256
+#define CG_JHW_COLS 4  // columns per wave of the J^H pass
 
 struct CGScalars {
     double dr, dr0, dr1, alpha, beta;
@@ -246,16 +247,30 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A) {
     __shared__ double red[8];
     if (A.scn->done) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int cols_per_wave = 16;
-    const int jbase = (blockIdx.x * (blockDim.x >> 6) + wv) * cols_per_wave;
+    const int jbase = (blockIdx.x * (blockDim.x >> 6) + wv) * CG_JHW_COLS;
     double dpart = 0.0;
-    for (int jj = 0; jj < cols_per_wave; ++jj) {
+    for (int jj = 0; jj < CG_JHW_COLS; ++jj) {
         int j = jbase + jj;
         if (j >= A.manz) break;
         const zd *col = A.J + (size_t)j * A.nanz;
+        // four loads of the column in flight per lane; one accumulation chain in
+        // the order i = lane, lane+64, ... (the sum does not depend on the unroll)
         zd s = zd{0.0, 0.0};
-        for (int i = lane; i < A.nanz; i += 64) {
-            zd v = col[i], a = A.ap[i];
+        int i = lane;
+        for (; i + 192 < A.nanz; i += 256) {
+            const zd v0 = col[i], v1 = col[i + 64], v2 = col[i + 128], v3 = col[i + 192];
+            const zd a0 = A.ap[i], a1 = A.ap[i + 64], a2 = A.ap[i + 128], a3 = A.ap[i + 192];
+            s.re += v0.re * a0.re + v0.im * a0.im;
+            s.im += v0.re * a0.im - v0.im * a0.re;
+            s.re += v1.re * a1.re + v1.im * a1.im;
+            s.im += v1.re * a1.im - v1.im * a1.re;
+            s.re += v2.re * a2.re + v2.im * a2.im;
+            s.im += v2.re * a2.im - v2.im * a2.re;
+            s.re += v3.re * a3.re + v3.im * a3.im;
+            s.im += v3.re * a3.im - v3.im * a3.re;
+        }
+        for (; i < A.nanz; i += 64) {
+            const zd v = col[i], a = A.ap[i];
             s.re += v.re * a.re + v.im * a.im;
             s.im += v.re * a.im - v.im * a.re;
         }
@@ -320,7 +335,7 @@ int crt_cg_geometry(int nanz, int manz, int *ncb, int *cb, int *n_dr1, int *n_rr
     int c = std::max(16, std::min(1024, (manz + want - 1) / want));
     *cb = c;
     *ncb = (manz + c - 1) / c;
-    *n_dr1 = (manz + 4 * 16 - 1) / (4 * 16);
+    *n_dr1 = (manz + 4 * CG_JHW_COLS - 1) / (4 * CG_JHW_COLS);
     *n_rr = (manz + CG_BLK - 1) / CG_BLK;
     return 0;
 }

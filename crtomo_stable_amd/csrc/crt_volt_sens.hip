@@ -11,6 +11,9 @@
 //    (bsensi, src/bsensi.f90:98-187) or -(...) strom_i (bsens, src/bsens.f90:103-191)
 #include "crt_common.hpp"
 
+#include <algorithm>
+#include <cstdlib>
+
 // hpotE[e][l]: back-transformed potential at electrode node e for source l
 __global__ void __launch_bounds__(256) crt_hpot_elec_kernel(int eanz, int K, const int *__restrict__ enr,
                                                             const double *__restrict__ kwnwi,
@@ -267,6 +270,129 @@ __global__ void __launch_bounds__(NTH) crt_sens_g_kernel(SensArgs A, int Ep) {
     }
 }
 
+// Electrode-pair form on the matrix cores.  G is symmetric (S_k is), so only its
+// upper 16x16 tiles are formed: G[P,Q] = Phi[:,P]^T T[:,Q] with v_mfma_f64_16x16x4f64
+// (four real products per complex one), accumulated in registers over chunks of KC
+// wavenumbers whose Phi and T = w S Phi rows are staged in LDS.  The tiles then go to
+// LDS (over the Phi/T staging) for the measurement lookups.
+typedef double sd4 __attribute__((ext_vector_type(4)));
+constexpr int SGW = 8;  // wavefronts per element
+
+__device__ __forceinline__ int sens_upper_tile(int P, int Q, int nt) { return P * nt - P * (P - 1) / 2 + (Q - P); }
+
+template <int NT>
+__global__ void __launch_bounds__(SGW * 64) crt_sens_gm_kernel(SensArgs A, int KC) {
+    constexpr int EP = 16 * NT;
+    constexpr int NUT = NT * (NT + 1) / 2;
+    constexpr int TPW = (NUT + SGW - 1) / SGW;
+    extern __shared__ double smem_gm[];
+    const int e = blockIdx.x;
+    if (e >= A.elanz) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const int K = A.K, E = A.eanz, nk = A.nkel[e];
+    const int KAc = (KC * nk + 3) & ~3;
+    zd *Phi = reinterpret_cast<zd *>(smem_gm);  // [KAc][EP]
+    zd *T = Phi + (size_t)KAc * EP;             // [KAc][EP]
+    zd *G = Phi;                                // [NUT][256], after the last chunk
+    sd4 gr[TPW], gi[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) gr[u] = gi[u] = sd4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const int kc = min(KC, K - k0);
+        for (int t = tid; t < KAc * EP; t += SGW * 64) {
+            const int ka = t / EP, p = t - ka * EP;
+            const int kl = ka / nk, a = ka - kl * nk;
+            double2 v = make_double2(0.0, 0.0);
+            if (kl < kc && p < E) {
+                const int node = A.nrel[(size_t)e * A.maxn + a] - 1;
+                v = *reinterpret_cast<const double2 *>(A.kpotN + ((size_t)node * K + k0 + kl) * E + p);
+            }
+            *reinterpret_cast<double2 *>(Phi + t) = v;
+        }
+        __syncthreads();
+        for (int t = tid; t < KAc * EP; t += SGW * 64) {
+            const int ka = t / EP, q = t - ka * EP;
+            const int kl = ka / nk, a = ka - kl * nk;
+            zd acc = zd{0.0, 0.0};
+            if (kl < kc) {
+                const int k = k0 + kl;
+                const double w = A.kwnwi[k];
+                for (int b = 0; b < nk; ++b) {
+                    const int hi = a > b ? a : b, lo = a > b ? b : a;
+                    const double sab = A.elbg[((size_t)k * A.npk + hi * (hi + 1) / 2 + lo) * A.elanz + e];
+                    const zd ph = Phi[(kl * nk + b) * EP + q];
+                    acc.re = fma(sab, ph.re, acc.re);
+                    acc.im = fma(sab, ph.im, acc.im);
+                }
+                acc = zd{w * acc.re, w * acc.im};
+            }
+            *reinterpret_cast<double2 *>(T + t) = make_double2(acc.re, acc.im);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            if (wave + SGW * u < NUT) {
+                // tile (P, Q), P <= Q, numbered as in sens_upper_tile
+                int t = wave + SGW * u, P = 0;
+                while (t >= NT - P) {
+                    t -= NT - P;
+                    ++P;
+                }
+                const zd *pa = Phi + rq * EP + 16 * P + jc;
+                const zd *pb = T + rq * EP + 16 * (P + t) + jc;
+                for (int ks = 0; ks < KAc; ks += 4) {
+                    const double2 a2 = *reinterpret_cast<const double2 *>(pa + ks * EP);
+                    const double2 b2 = *reinterpret_cast<const double2 *>(pb + ks * EP);
+                    const zd a = zd{a2.x, a2.y}, b = zd{b2.x, b2.y};
+                    gr[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, gr[u], 0, 0, 0);
+                    gi[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, gi[u], 0, 0, 0);
+                    gr[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, gr[u], 0, 0, 0);
+                    gi[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, gi[u], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        const int t = wave + SGW * u;
+        if (t < NUT)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) G[t * 256 + (4 * v + rq) * 16 + jc] = zd{gr[u][v], gi[u][v]};
+    }
+    __syncthreads();
+    const double pi = acos(-1.0);
+    auto gv = [&](int p, int q) -> zd {
+        if (p <= 0 || q <= 0) return zd{0.0, 0.0};
+        const int a = min(p, q) - 1, b = max(p, q) - 1;
+        const double2 v =
+            *reinterpret_cast<const double2 *>(G + sens_upper_tile(a >> 4, b >> 4, NT) * 256 + (a & 15) * 16 + (b & 15));
+        return zd{v.x, v.y};
+    };
+    for (int i = tid; i < A.nanz; i += SGW * 64) {
+        const int4 m = A.meas[i];  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
+        const zd dum = zadd(zsub(gv(m.w, m.y), gv(m.w, m.x)), zsub(gv(m.z, m.x), gv(m.z, m.y)));
+        zd out;
+        if (A.mode == 1) {
+            zd t = zd{dum.re / pi, dum.im / pi};
+            out = zdiv(zmul(t, A.sigma[e]), A.volt[i]);
+        } else {
+            double f = A.strom ? A.strom[i] / pi : 1.0 / pi;
+            out = zd{-(dum.re * f), -(dum.im * f)};
+        }
+        A.sens[(size_t)e * A.nanz + i] = out;
+    }
+}
+
+template <int NT>
+static int launch_sens_gm(hipStream_t s, const SensArgs &a, int KC, size_t lds) {
+    (void)hipFuncSetAttribute((const void *)crt_sens_gm_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    hipLaunchKernelGGL(crt_sens_gm_kernel<NT>, dim3(a.elanz), dim3(SGW * 64), lds, s, a, KC);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
 // kpot in the reference layout kpot(sanz, eanz, kwnanz): out[(k*eanz + l)*sanz + j]
 __global__ void __launch_bounds__(256) crt_kpot_ref_kernel(int sanz, int eanz, int K, const zd *__restrict__ kpotN,
                                                            zd *__restrict__ out) {
@@ -307,12 +433,35 @@ int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int max
                     const int *nrel, const int *nkel, const int4 *meas, const double *kwnwi, const zd *kpotN,
                     const double *elbg, const zd *sigma, const zd *volt, const double *strom, zd *sens) {
     SensArgs a{elanz, nanz, eanz, K, maxn, npk, mode, nrel, nkel, meas, kwnwi, kpotN, elbg, sigma, volt, strom, sens};
+    static const int form = getenv("CRT_SENS_FORM") ? atoi(getenv("CRT_SENS_FORM")) : 2;
+    // electrode-pair form on the matrix cores: up to 128 electrodes
+    if (form == 2 && eanz <= 128) {
+        const int nt = (eanz + 15) / 16, nut = nt * (nt + 1) / 2;
+        const size_t gbytes = (size_t)nut * 256 * sizeof(zd);
+        // wavenumber chunk: Phi and T rows within max(G, 72 KB) of LDS
+        const size_t budget = std::max(gbytes, (size_t)72 * 1024);
+        int KC = K;
+        while (KC > 1 && (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd) > budget) --KC;
+        const size_t lds = std::max(gbytes, (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd));
+        if (lds <= 160 * 1024) {
+            switch (nt) {
+                case 1: return launch_sens_gm<1>(s, a, KC, lds);
+                case 2: return launch_sens_gm<2>(s, a, KC, lds);
+                case 3: return launch_sens_gm<3>(s, a, KC, lds);
+                case 4: return launch_sens_gm<4>(s, a, KC, lds);
+                case 5: return launch_sens_gm<5>(s, a, KC, lds);
+                case 6: return launch_sens_gm<6>(s, a, KC, lds);
+                case 7: return launch_sens_gm<7>(s, a, KC, lds);
+                case 8: return launch_sens_gm<8>(s, a, KC, lds);
+                default: break;
+            }
+        }
+    }
     // electrode-pair form when its LDS fits
     {
         const int Ep = (eanz + 3) / 4 * 4, KA = K * maxn, nb = (Ep / 4) * (Ep / 4);
         const size_t lg = std::max((size_t)2 * KA * Ep, (size_t)Ep * Ep) * sizeof(zd);
-        static const bool direct = getenv("CRT_SENS_DIRECT") != nullptr;
-        if (!direct && lg <= 160 * 1024 && nb <= 1024) {
+        if (form >= 1 && lg <= 160 * 1024 && nb <= 1024) {
             if (nb <= 256) {
                 (void)hipFuncSetAttribute((const void *)crt_sens_g_kernel<256, 1>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

@@ -12,7 +12,8 @@ where it is well posed:
   * the converged solve (eps = 1e-30): dpar 1e-10 relative, also against a dense
     direct solve of the scaled normal equations;
   * the reference's default stop (eps = 1e-4, ncgmax = manz/10) and eps = 1e-12:
-    ncg within 3 % (at least 3 iterations), dpar within the accuracy of a stop at
+    ncg within 3 % (at least 3 iterations) of the range the oracle's own stop takes
+    when its right-hand side is perturbed at rounding level (1e-15), dpar within the accuracy of a stop at
     that residual (1e-2 and 1e-4 relative), cgres[0] = ncg, history within REAL*4
     rounding for the first iterations.
 """
@@ -102,9 +103,19 @@ def test_update_reference_stop(ctx, oracle_lib, ltri, eps, lam):
     w = _workload(ltri)
     ncgmax = w.grid.elanz // 10 if eps == 1e-4 else 400  # src/rall.f90:735-738
     P = _problem(O, w, ltri)
-    _, _, _, dpar_o, cgres_o, ncg_o = _oracle_update(O, w, P, ltri, lam, eps, ncgmax)
+    cp, cgfac_o, bvec_o, dpar_o, cgres_o, ncg_o = _oracle_update(O, w, P, ltri, lam, eps, ncgmax)
+    # how far rounding alone moves the oracle's own stop: the same CG on right-hand
+    # sides perturbed at 1e-15 relative
+    spread = []
+    for seed in range(4):
+        rng = np.random.default_rng(100 + seed)
+        b2 = bvec_o * (1.0 + 1e-15 * rng.standard_normal(bvec_o.size))
+        spread.append(O.cjggra(cp, cgfac_o, b2)[2])
+    lo, hi = min(spread + [ncg_o]), max(spread + [ncg_o])
     r = _gpu_update(ctx, w, P, ltri, lam, eps, ncgmax)
-    assert abs(r["ncg"] - ncg_o) <= max(3, 0.03 * ncg_o)
+    tol = max(3, 0.03 * ncg_o)
+    print(f"ncg gpu {r['ncg']} oracle {ncg_o} rounding spread [{lo}, {hi}]", flush=True)
+    assert lo - tol <= r["ncg"] <= hi + tol
     assert _rel(r["dpar"], dpar_o) <= (1e-2 if eps > 1e-8 else 1e-4)
     assert np.allclose(r["cgres"][1:4], cgres_o[1:4], rtol=2e-6, atol=0)
 
