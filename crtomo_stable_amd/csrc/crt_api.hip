@@ -35,29 +35,8 @@ int crt_launch_sens(hipStream_t, int, int, int, int, int, int, int, const int *,
 int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
 
 // CG (crt_cg.hip)
-struct CGScalars {
-    double dr, dr0, dr1, alpha, beta;
-    int k, done, ncg, pad;
-};
-struct CGArgs {
-    int nanz, manz, ltri, nx, maxn, ncgmax;
-    double lam, eps;
-    const zd *J;
-    const double *w;
-    const double *smatm;
-    const int *nachbar;
-    const double *cgfac;
-    zd *r, *p, *q, *bv, *dpar;
-    zd *ap;
-    zd *part_ap;
-    double *part_dr1;
-    double *part_rr;
-    int n_dr1, n_rr, ncb, cb;
-    const CGScalars *sc;
-    CGScalars *scn;
-    double *cgres;
-};
 int crt_cg_geometry(int, int, int *, int *, int *, int *);
+int crt_launch_cg_j32(hipStream_t, size_t, const zd *, float2 *);
 int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const zd *, const zd *,
                       const double *, double *, zd *);
 int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
@@ -134,6 +113,9 @@ struct crt_ctx {
     DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
     DBuf<int> d_nachbar;
     DBuf<double> d_jtj;
+    DBuf<float2> d_sens32;  // complex FP32 copy of J for the CG iterations
+    int cg_jbits = 64;      // storage precision of J in the CG iterations
+    bool sens32_ok = false;
     DBuf<zd> d_res, d_rm, d_bvec, d_r, d_p, d_q, d_bv, d_dpar, d_ap, d_part_ap, d_par;
     DBuf<CGScalars> d_sc;
     int cg_check_every = 16;
@@ -763,6 +745,7 @@ int crt_sens(crt_ctx *c, int mode, const double *strom) {
     CRT_HIP(hipStreamSynchronize(c->stream));
     c->collect(T_SENS);
     c->have_sens = true;
+    c->sens32_ok = false;
     c->sens_nanz = c->nanz;
     c->sens_manz = c->elanz;
     return 0;
@@ -787,6 +770,7 @@ int crt_set_sens(crt_ctx *c, int nanz, int manz, const double *sens) {
     c->sens_nanz = nanz;
     c->sens_manz = manz;
     c->have_sens = true;
+    c->sens32_ok = false;
     return 0;
 }
 
@@ -878,6 +862,16 @@ static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, do
     a.nanz = nanz; a.manz = manz; a.ltri = c->ltri; a.nx = c->rnx; a.maxn = c->rncol - 1; a.ncgmax = ncgmax;
     a.lam = lam; a.eps = eps;
     a.J = c->d_sens.p; a.w = c->d_w.p; a.smatm = c->d_smatm.p; a.nachbar = c->d_nachbar.p;
+    a.J32 = nullptr;
+    if (c->cg_jbits == 32) {
+        const size_t n = (size_t)nanz * manz;
+        if (!c->sens32_ok) {
+            if ((e = c->d_sens32.ensure(n))) return e;
+            if ((e = crt_launch_cg_j32(c->stream, n, c->d_sens.p, c->d_sens32.p))) return e;
+            c->sens32_ok = true;
+        }
+        a.J32 = c->d_sens32.p;
+    }
     a.cgfac = c->d_cgfac.p;
     a.r = c->d_r.p; a.p = c->d_p.p; a.q = c->d_q.p; a.bv = c->d_bv.p; a.dpar = c->d_dpar.p; a.ap = c->d_ap.p;
     a.part_ap = c->d_part_ap.p; a.part_dr1 = c->d_part_dr1.p; a.part_rr = c->d_part_rr.p;
@@ -929,6 +923,14 @@ static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfa
     std::vector<double> w(nanz);
     for (int i = 0; i < nanz; ++i) w[i] = wmatd[i] * (double)wdfak[i];
     return c->d_w.upload(w.data(), nanz, c->stream);
+}
+
+int crt_set_cg_precision(crt_ctx *c, int jbits) {
+    if (!c || (jbits != 64 && jbits != 32)) return CRT_ERR_ARG;
+    c->cg_jbits = jbits;
+    if (jbits == 64) c->d_sens32.release();
+    c->sens32_ok = false;
+    return 0;
 }
 
 int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,

@@ -925,6 +925,67 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
     if (lane == 0 && err && A.status[s] == 0) A.status[s] = err;
 }
 
+// The products of update wave W in one block step away from the band's ends
+// (every tile live, tile a taking D-a products), in issue order: tile by tile, m upwards.
+template <int D, int W>
+struct StOps {
+    struct Op {
+        int u, a, mm;
+    };
+    static constexpr int count() {
+        int n = 0;
+        for (int u = 0; u < StCfg<D>::NA; ++u) {
+            const int a = StCfg<D>::tile(W, u);
+            if (a <= D) n += D - a;
+        }
+        return n;
+    }
+    static constexpr int N = count();
+    static constexpr Op op(int i) {
+        for (int u = 0; u < StCfg<D>::NA; ++u) {
+            const int a = StCfg<D>::tile(W, u);
+            if (a > D) continue;
+            for (int mm = 1; mm <= D - a; ++mm)
+                if (i-- == 0) return Op{u, a, mm};
+        }
+        return Op{0, 0, 0};
+    }
+};
+constexpr int ST_PD = 3;  // streamed operands in flight per update wave
+
+// product I of the steady-state sequence: consume operand slot I % ST_PD, refill it
+// with operand I + ST_PD, then T -= L(J+a, J-m) L(J, J-m)^T on the matrix cores
+template <int D, int W, int I, int NA>
+__device__ __forceinline__ void st_pipe(const zd *panel, const StLds<D> &m, int J, int jc, int rq, zd (&buf)[ST_PD][4],
+                                        d4 (&cr)[NA], d4 (&ci)[NA]) {
+    if constexpr (I < StOps<D, W>::N) {
+        constexpr auto o = StOps<D, W>::op(I);
+        zd cur[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) {
+            cur[kc].re = buf[I % ST_PD][kc].re;
+            cur[kc].im = buf[I % ST_PD][kc].im;
+        }
+        if constexpr (I + ST_PD < StOps<D, W>::N) {
+            constexpr auto on = StOps<D, W>::op(I + ST_PD);
+            st_operand<D>(panel, J, on.a, on.mm, jc, rq, buf[I % ST_PD]);
+        }
+        mfma_update_rv(cr[o.u], ci[o.u], cur, m.Lrow + (o.mm - 1) * T16 * PS, jc, rq);
+        // keep each product's loads and MFMAs together (bounds the live registers)
+        __builtin_amdgcn_sched_barrier(0);
+        st_pipe<D, W, I + 1, NA>(panel, m, J, jc, rq, buf, cr, ci);
+    }
+}
+
+template <int D, int W, int I>
+__device__ __forceinline__ void st_prime(const zd *panel, int J, int jc, int rq, zd (&buf)[ST_PD][4]) {
+    if constexpr (I < ST_PD && I < StOps<D, W>::N) {
+        constexpr auto o = StOps<D, W>::op(I);
+        st_operand<D>(panel, J, o.a, o.mm, jc, rq, buf[I]);
+        st_prime<D, W, I + 1>(panel, J, jc, rq, buf);
+    }
+}
+
 template <int D, int W>
 __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &m) {
     using C = StCfg<D>;
@@ -948,7 +1009,14 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
     for (int J = 0; J < nblk; ++J) {
         st_stage<D>(panel, m, J, tid);
         __syncthreads();
-        // C(a) -= sum_m L(J+a, J-m) L(J, J-m)^T, next operand in flight
+        // C(a) -= sum_m L(J+a, J-m) L(J, J-m)^T
+        if (J >= D && J + D < nblk) {
+            // away from the ends: a fixed sequence of products, ST_PD operands in flight
+            zd buf[ST_PD][4];
+            st_prime<D, W, 0>(panel, J, jc, rq, buf);
+            st_pipe<D, W, 0, NA>(panel, m, J, jc, rq, buf, cr, ci);
+        } else {
+        // near the ends: the next operand in flight
         zd nx[4];
         bool have = false;  // nx holds the first operand of round u
 #pragma unroll
@@ -978,6 +1046,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
                     mfma_update_rv(cr[u], ci[u], cur, m.Lrow + (mm - 1) * T16 * PS, jc, rq);
                 }
             }
+        }
         }
         __syncthreads();
         __syncthreads();

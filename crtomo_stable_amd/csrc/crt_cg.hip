@@ -18,31 +18,14 @@
 #define CG_BLK 256
 #define CG_JHW_COLS 4  // columns per wave of the J^H pass
 
-struct CGScalars {
-    double dr, dr0, dr1, alpha, beta;
-    int k, done, ncg, pad;
-};
-
-struct CGArgs {
-    int nanz, manz, ltri, nx, maxn, ncgmax;
-    double lam, eps;
-    const zd *J;            // (nanz, manz) column-major
-    const double *w;        // nanz: wmatd * wdfak
-    const double *smatm;    // (manz, ncol) column-major
-    const int *nachbar;     // (manz, maxn+1) column-major, 1-based (ltri 1/2)
-    const double *cgfac;    // manz
-    zd *r, *p, *q, *bv, *dpar;  // manz
-    zd *ap;                 // nanz (W-weighted after reduction: wap)
-    zd *part_ap;            // [ncb][nanz]
-    double *part_dr1;       // per block of the J^H kernel
-    double *part_rr;        // per block of the update kernel
-    int n_dr1, n_rr, ncb, cb;
-    const CGScalars *sc;    // state after the previous iteration (read only)
-    CGScalars *scn;         // state of this iteration (written by block 0)
-    double *cgres;          // ncgmax+1
-};
-
 namespace {
+
+// Jacobian entry as complex FP64 from the FP64 or the FP32 copy
+__device__ __forceinline__ zd ldj(const zd *p) { return *p; }
+__device__ __forceinline__ zd ldj(const float2 *p) {
+    const float2 v = *p;
+    return zd{(double)v.x, (double)v.y};
+}
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
     // fixed-order tree reduction over a 256-thread block
@@ -209,7 +192,8 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
 }
 
 // K_d: partial ap over column chunks: part_ap[cb][i] = sum_{j in chunk} J(i,j) q_j
-__global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A) {
+template <class JT>
+__global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A, const JT *J) {
     __shared__ zd qs[1024];
     if (A.scn->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,15 +203,15 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A) {
     __syncthreads();
     if (i >= A.nanz) return;
     zd acc0 = zd{0.0, 0.0}, acc1 = zd{0.0, 0.0};
-    const zd *col = A.J + (size_t)j0 * A.nanz + i;
+    const JT *col = J + (size_t)j0 * A.nanz + i;
     int j = j0;
     for (; j + 1 < j1; j += 2) {
-        zd v0 = col[0], v1 = col[A.nanz];
+        zd v0 = ldj(col), v1 = ldj(col + A.nanz);
         acc0 = zfma(acc0, v0, qs[j - j0]);
         acc1 = zfma(acc1, v1, qs[j + 1 - j0]);
         col += 2 * (size_t)A.nanz;
     }
-    if (j < j1) acc0 = zfma(acc0, col[0], qs[j - j0]);
+    if (j < j1) acc0 = zfma(acc0, ldj(col), qs[j - j0]);
     A.part_ap[(size_t)cb * A.nanz + i] = zadd(acc0, acc1);
 }
 
@@ -243,7 +227,8 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_apred_kernel(CGArgs A) {
 }
 
 // K_f: Bp_j = C_j (J^H wap + lam R q)_j, one wave per column, dr1 partials
-__global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A) {
+template <class JT>
+__global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A, const JT *J) {
     __shared__ double red[8];
     if (A.scn->done) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -252,13 +237,13 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A) {
     for (int jj = 0; jj < CG_JHW_COLS; ++jj) {
         int j = jbase + jj;
         if (j >= A.manz) break;
-        const zd *col = A.J + (size_t)j * A.nanz;
+        const JT *col = J + (size_t)j * A.nanz;
         // four loads of the column in flight per lane; one accumulation chain in
         // the order i = lane, lane+64, ... (the sum does not depend on the unroll)
         zd s = zd{0.0, 0.0};
         int i = lane;
         for (; i + 192 < A.nanz; i += 256) {
-            const zd v0 = col[i], v1 = col[i + 64], v2 = col[i + 128], v3 = col[i + 192];
+            const zd v0 = ldj(col + i), v1 = ldj(col + i + 64), v2 = ldj(col + i + 128), v3 = ldj(col + i + 192);
             const zd a0 = A.ap[i], a1 = A.ap[i + 64], a2 = A.ap[i + 128], a3 = A.ap[i + 192];
             s.re += v0.re * a0.re + v0.im * a0.im;
             s.im += v0.re * a0.im - v0.im * a0.re;
@@ -270,7 +255,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A) {
             s.im += v3.re * a3.im - v3.im * a3.re;
         }
         for (; i < A.nanz; i += 64) {
-            const zd v = col[i], a = A.ap[i];
+            const zd v = ldj(col + i), a = A.ap[i];
             s.re += v.re * a.re + v.im * a.im;
             s.im += v.re * a.im - v.im * a.re;
         }
@@ -359,14 +344,34 @@ int crt_launch_cg_init(hipStream_t s, const CGArgs &a, const zd *bvec) {
 
 int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
     hipLaunchKernelGGL(crt_cg_dir_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
-    hipLaunchKernelGGL(crt_cg_jq_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK, a.ncb), dim3(CG_BLK), 0, s, a);
+    const dim3 gq((a.nanz + CG_BLK - 1) / CG_BLK, a.ncb);
+    if (a.J32)
+        hipLaunchKernelGGL(crt_cg_jq_kernel<float2>, gq, dim3(CG_BLK), 0, s, a, a.J32);
+    else
+        hipLaunchKernelGGL(crt_cg_jq_kernel<zd>, gq, dim3(CG_BLK), 0, s, a, a.J);
     hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK), dim3(CG_BLK), 0, s, a);
-    hipLaunchKernelGGL(crt_cg_jhw_kernel, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a);
+    if (a.J32)
+        hipLaunchKernelGGL(crt_cg_jhw_kernel<float2>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J32);
+    else
+        hipLaunchKernelGGL(crt_cg_jhw_kernel<zd>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J);
     hipLaunchKernelGGL(crt_cg_upd_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 int crt_launch_cg_final(hipStream_t s, int manz, double step, const double *cgfac, zd *dpar, zd *par) {
     hipLaunchKernelGGL(crt_cg_final_kernel, dim3((manz + 255) / 256), dim3(256), 0, s, manz, step, cgfac, dpar, par);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+// complex FP32 copy of J for the CG iterations (round to nearest)
+__global__ void __launch_bounds__(256) crt_cg_j32_kernel(size_t n, const zd *J, float2 *J32) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const zd v = J[t];
+    J32[t] = make_float2((float)v.re, (float)v.im);
+}
+
+int crt_launch_cg_j32(hipStream_t s, size_t n, const zd *J, float2 *J32) {
+    hipLaunchKernelGGL(crt_cg_j32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, J, J32);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

@@ -122,3 +122,29 @@ struct DevGrid {
     const int *tmap;
     int tmapD;                // D16 the map was built for (0 if none)
 };
+
+// Gauss-Newton CG state shared by crt_cg.hip and crt_api.hip
+struct CGScalars {
+    double dr, dr0, dr1, alpha, beta;
+    int k, done, ncg, pad;
+};
+
+struct CGArgs {
+    int nanz, manz, ltri, nx, maxn, ncgmax;
+    double lam, eps;
+    const zd *J;            // (nanz, manz) column-major
+    const float2 *J32;      // the same in complex FP32 for the iterations, or null (FP64)
+    const double *w;        // nanz: wmatd * wdfak
+    const double *smatm;    // (manz, ncol) column-major
+    const int *nachbar;     // (manz, maxn+1) column-major, 1-based (ltri 1/2)
+    const double *cgfac;    // manz
+    zd *r, *p, *q, *bv, *dpar;  // manz
+    zd *ap;                 // nanz (W-weighted after reduction: wap)
+    zd *part_ap;            // [ncb][nanz]
+    double *part_dr1;       // per block of the J^H kernel
+    double *part_rr;        // per block of the update kernel
+    int n_dr1, n_rr, ncb, cb;
+    const CGScalars *sc;    // state after the previous iteration (read only)
+    CGScalars *scn;         // state of this iteration (written by block 0)
+    double *cgres;          // ncgmax+1
+};

@@ -225,6 +225,10 @@ class Context:
                                 C.byref(lam)), "crt_blam0")
         return lam.value
 
+    def set_cg_precision(self, jbits: int):
+        """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""
+        _chk(self.lib.crt_set_cg_precision(self._h, int(jbits)), "crt_set_cg_precision")
+
     def cjg(self, lam, eps, ncgmax, wmatd, wdfak, cgfac, bvec):
         manz = np.asarray(cgfac).size
         w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)

@@ -96,6 +96,13 @@ int crt_update(crt_ctx *ctx, double lam, double eps, int ncgmax, double step, co
                const int *wdfak, const double *dat, const double *sigmaa, double *par, double *dpar,
                double *cgres, int *ncg, double *cgfac, double *bvec0);
 
+/* Storage precision of J in the CG iterations of crt_cjg / crt_update: 64 (default,
+ * the reference's arithmetic) or 32 (complex FP32 copy of J, made once per Jacobian;
+ * products accumulate and all inner products run in FP64; the RHS and cgfac pass
+ * keeps FP64 J).  No counterpart in the reference: the mixed-precision form halves
+ * the bytes of each iteration. */
+int crt_set_cg_precision(crt_ctx *ctx, int jbits);
+
 /* Smoothness regularisation built on the device from the grid of crt_set_grid and
  * installed for crt_cjg / crt_update (replaces crt_set_regularization):
  * ltri 0 -> bsmatmreg (src/bsmatm_mod.f90:190-362) on the regular nx x nz quad grid,
