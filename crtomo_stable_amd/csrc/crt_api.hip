@@ -37,6 +37,7 @@ int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
 // CG (crt_cg.hip)
 int crt_cg_geometry(int, int, int *, int *, int *, int *);
 int crt_launch_cg_j32(hipStream_t, size_t, const zd *, float2 *);
+int crt_launch_cg_fsai(hipStream_t, const CGArgs &, zd *);
 int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const zd *, const zd *,
                       const double *, double *, zd *);
 int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
@@ -116,6 +117,11 @@ struct crt_ctx {
     DBuf<float2> d_sens32;  // complex FP32 copy of J for the CG iterations
     int cg_jbits = 64;      // storage precision of J in the CG iterations
     bool sens32_ok = false;
+    int cg_prec = 0;                      // 0 Jacobi scaling only, 1/2 FSAI patterns
+    std::vector<int> h_nachbar;           // host copy of the neighbour table (ltri 1/2)
+    DBuf<int> d_fpat, d_ftp;
+    DBuf<zd> d_fg, d_t, d_z;
+    DBuf<double> d_part_rz;
     DBuf<zd> d_res, d_rm, d_bvec, d_r, d_p, d_q, d_bv, d_dpar, d_ap, d_part_ap, d_par;
     DBuf<CGScalars> d_sc;
     int cg_check_every = 16;
@@ -787,6 +793,7 @@ int crt_set_regularization(crt_ctx *c, int ltri, int nx, int manz, int maxn, con
     int e = c->d_smatm.upload(smatm, (size_t)manz * ncol, c->stream);
     e = e ? e : c->d_rdiag.upload(rd.data(), manz, c->stream);
     if (!e && ltri != 0) e = c->d_nachbar.upload(nachbar, (size_t)manz * (maxn + 1), c->stream);
+    if (ltri != 0) c->h_nachbar.assign(nachbar, nachbar + (size_t)manz * (maxn + 1));
     if (!e && ltri == 0) e = c->d_nachbar.ensure(1);
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(c->stream));
@@ -822,6 +829,7 @@ int crt_bsmatm(crt_ctx *c, int ltri, int nx, int nz, double alfx, double alfz, d
         std::vector<int> nb((size_t)manz * (maxn + 1));
         e = crt_bnachbar(manz, maxn, c->nrel.data(), nb.data());
         e = e ? e : c->d_nachbar.upload(nb.data(), nb.size(), c->stream);
+        c->h_nachbar = nb;
         e = e ? e : c->d_smatm.ensure((size_t)manz * ncol);
         e = e ? e : crt_launch_bsmatmtri(c->stream, manz, maxn, c->d_nrel.p, c->d_nkel.p, c->d_x.p, c->d_y.p,
                                          c->d_nachbar.p, alfx, alfz, c->d_smatm.p);
@@ -872,6 +880,7 @@ static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, do
         }
         a.J32 = c->d_sens32.p;
     }
+    a.fg = nullptr; a.fpat = nullptr; a.ftp = nullptr; a.t = nullptr; a.z = nullptr; a.part_rz = nullptr;
     a.cgfac = c->d_cgfac.p;
     a.r = c->d_r.p; a.p = c->d_p.p; a.q = c->d_q.p; a.bv = c->d_bv.p; a.dpar = c->d_dpar.p; a.ap = c->d_ap.p;
     a.part_ap = c->d_part_ap.p; a.part_dr1 = c->d_part_dr1.p; a.part_rr = c->d_part_rr.p;
@@ -882,6 +891,67 @@ static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, do
 
 // Fletcher-Reeves iterations on the device; the host looks at the stop flag
 // every cg_check_every iterations.
+// FSAI pattern (row j: lower-triangular columns, j itself last) and its transpose
+static void fsai_pattern(const crt_ctx *c, int manz, int kind, std::vector<int> &pat, std::vector<int> &tp) {
+    constexpr int NP = CRT_FSAI_NP;
+    pat.assign((size_t)manz * NP, -1);
+    tp.assign((size_t)manz * NP, -1);
+    std::vector<int> nin(manz, 0);
+    for (int j = 0; j < manz; ++j) {
+        std::vector<int> cols;
+        if (c->ltri == 0) {
+            const int nx = c->rnx, ix = j % nx, iz = j / nx;
+            auto add = [&](int dz, int dx) {
+                const int x = ix + dx, z = iz + dz;
+                if (x >= 0 && x < nx && z >= 0) cols.push_back(z * nx + x);
+            };
+            if (kind == 2) {
+                add(-2, 0); add(-1, -1); add(-1, 0); add(-1, 1); add(0, -2); add(0, -1);
+            } else {
+                add(-1, 0); add(0, -1);
+            }
+        } else {
+            const int mx = (int)(c->h_nachbar.size() / manz) - 1;
+            for (int ij = 0; ij < mx; ++ij) {
+                const int nb = c->h_nachbar[(size_t)ij * manz + j] - 1;
+                if (nb >= 0 && nb < j && std::find(cols.begin(), cols.end(), nb) == cols.end()) cols.push_back(nb);
+            }
+            std::sort(cols.begin(), cols.end());
+        }
+        cols.push_back(j);
+        const int off = NP - (int)cols.size();
+        for (size_t s = 0; s < cols.size(); ++s) pat[(size_t)j * NP + off + s] = cols[s];
+    }
+    for (int j = 0; j < manz; ++j)
+        for (int s = 0; s < NP; ++s) {
+            const int col = pat[(size_t)j * NP + s];
+            if (col >= 0 && nin[col] < NP) tp[(size_t)col * NP + nin[col]++] = j * NP + s;
+        }
+}
+
+// build G of M = G^H G for the current cgfac and lambda (after the RHS pass)
+static int cg_fsai(crt_ctx *c, CGArgs &a) {
+    if (c->cg_prec == 0) return 0;
+    const int manz = a.manz;
+    std::vector<int> pat, tp;
+    fsai_pattern(c, manz, c->cg_prec, pat, tp);
+    int e = c->d_fpat.upload(pat.data(), pat.size(), c->stream);
+    e = e ? e : c->d_ftp.upload(tp.data(), tp.size(), c->stream);
+    e = e ? e : c->d_fg.ensure((size_t)manz * CRT_FSAI_NP);
+    e = e ? e : c->d_t.ensure(manz);
+    e = e ? e : c->d_z.ensure(manz);
+    e = e ? e : c->d_part_rz.ensure(a.n_rr);
+    if (e) return e;
+    a.fpat = c->d_fpat.p;
+    a.ftp = c->d_ftp.p;
+    if ((e = crt_launch_cg_fsai(c->stream, a, c->d_fg.p))) return e;
+    a.fg = c->d_fg.p;
+    a.t = c->d_t.p;
+    a.z = c->d_z.p;
+    a.part_rz = c->d_part_rz.p;
+    return 0;
+}
+
 static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
     int e;
     CGArgs a0 = a, a1 = a;
@@ -925,6 +995,12 @@ static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfa
     return c->d_w.upload(w.data(), nanz, c->stream);
 }
 
+int crt_set_cg_preconditioner(crt_ctx *c, int kind) {
+    if (!c || kind < 0 || kind > 2) return CRT_ERR_ARG;
+    c->cg_prec = kind;
+    return 0;
+}
+
 int crt_set_cg_precision(crt_ctx *c, int jbits) {
     if (!c || (jbits != 64 && jbits != 32)) return CRT_ERR_ARG;
     c->cg_jbits = jbits;
@@ -945,6 +1021,7 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if (e) return e;
     CGArgs a;
     if ((e = cg_prepare(c, nanz, manz, ncgmax, lam, eps, a))) return e;
+    if ((e = cg_fsai(c, a))) return e;
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
     if (dpar) CRT_HIP(c->copy_out(dpar, c->d_dpar.p, manz * sizeof(zd)));
@@ -1003,6 +1080,7 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
         return e;
     if (cgfac) CRT_HIP(hipMemcpyAsync(cgfac, c->d_cgfac.p, manz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (bvec0) CRT_HIP(hipMemcpyAsync(bvec0, c->d_bvec.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+    if ((e = cg_fsai(c, a))) return e;
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
     if ((e = crt_launch_cg_final(c->stream, manz, step, c->d_cgfac.p, c->d_dpar.p, c->d_par.p))) return e;

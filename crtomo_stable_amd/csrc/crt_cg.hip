@@ -155,6 +155,8 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
     }
     const int k = cur.k + 1;  // 1-based iteration number
     double dr = reduce_partials(A.part_rr, A.n_rr, red);
+    // preconditioned CG: beta from r^H M r; the stop test stays on |r|^2
+    const double rz = A.fg ? reduce_partials(A.part_rz, A.n_rr, red) : dr;
     double dr0, beta;
     bool stop = false;
     if (k == 1) {
@@ -163,7 +165,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
     } else {
         dr0 = cur.dr0;
         stop = dr <= dr0;
-        beta = dr / cur.dr1;
+        beta = A.fg ? rz / cur.rz1 : dr / cur.dr1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         CGScalars nx = cur;
@@ -172,6 +174,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
             nx.ncg = k - 1;
         } else {
             nx.dr = dr;
+            nx.rz = rz;
             nx.dr0 = dr0;
             nx.beta = beta;
             nx.k = k;
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
     if (stop) return;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < A.manz) {
-        zd r = A.r[j], p = A.p[j];
+        zd r = A.fg ? A.z[j] : A.r[j], p = A.p[j];
         zd pn = zd{r.re + beta * p.re, r.im + beta * p.im};
         A.p[j] = pn;
         double c = A.cgfac[j];
@@ -282,7 +285,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_upd_kernel(CGArgs A) {
     if (A.scn->done) return;
     double pbp = reduce_partials(A.part_dr1, A.n_dr1, red);
     double dr = A.scn->dr;
-    double alpha = dr / pbp;
+    double alpha = (A.fg ? A.scn->rz : dr) / pbp;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
     if (j < A.manz) {
@@ -297,6 +300,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_upd_kernel(CGArgs A) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.scn->alpha = alpha;
         A.scn->dr1 = dr;
+        A.scn->rz1 = A.scn->rz;
     }
 }
 
@@ -310,6 +314,152 @@ __global__ void __launch_bounds__(256) crt_cg_final_kernel(int manz, double step
     d = zd{d.re * c, d.im * c};
     dpar[j] = d;
     if (par) par[j] = zd{par[j].re + step * d.re, par[j].im + step * d.im};
+}
+
+// ------------------------------------------------- FSAI preconditioner M = G^H G
+// t = G r
+__global__ void __launch_bounds__(CG_BLK) crt_cg_pg_kernel(CGArgs A) {
+    if (A.scn->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.manz) return;
+    zd t = zd{0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < CRT_FSAI_NP; ++s) {
+        const int c = A.fpat[(size_t)j * CRT_FSAI_NP + s];
+        if (c >= 0) t = zfma(t, A.fg[(size_t)j * CRT_FSAI_NP + s], A.r[c]);
+    }
+    A.t[j] = t;
+}
+
+// z = G^H t and the partial sums of Re(r^H z)
+__global__ void __launch_bounds__(CG_BLK) crt_cg_pgh_kernel(CGArgs A) {
+    __shared__ double red[8];
+    if (A.scn->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (j < A.manz) {
+        zd z = zd{0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < CRT_FSAI_NP; ++s) {
+            const int e = A.ftp[(size_t)j * CRT_FSAI_NP + s];
+            if (e >= 0) {
+                const zd g = A.fg[e], t = A.t[e / CRT_FSAI_NP];
+                // conj(g) * t
+                z.re += g.re * t.re + g.im * t.im;
+                z.im += g.re * t.im - g.im * t.re;
+            }
+        }
+        A.z[j] = z;
+        const zd r = A.r[j];
+        v = r.re * z.re + r.im * z.im;
+    }
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) A.part_rz[blockIdx.x] = s;
+}
+
+// entry R(a, b) of the regularisation stencil
+__device__ __forceinline__ double rentry(const CGArgs &A, int a, int b) {
+    const int manz = A.manz;
+    const double *s = A.smatm;
+    if (A.ltri == 0) {
+        if (a == b) return s[a];
+        if (b == a + 1) return s[manz + a];
+        if (a == b + 1) return s[manz + b];
+        if (b == a + A.nx) return s[2 * manz + a];
+        if (a == b + A.nx) return s[2 * manz + b];
+        return 0.0;
+    }
+    if (a == b) return s[(size_t)A.maxn * manz + a];
+    for (int ij = 0; ij < A.maxn; ++ij)
+        if (A.nachbar[(size_t)ij * manz + a] == b + 1) return s[(size_t)ij * manz + a];
+    return 0.0;
+}
+
+// Row j of G: the Hermitian block B(S,S) of B = C (J^H W J + lam R) C on the
+// pattern S (last entry j), y = B(S,S)^-1 e_last, g = y / sqrt(y_last).  One wave
+// per cell: lanes reduce the weighted column products, lane 0 solves.
+__global__ void __launch_bounds__(256) crt_cg_fsai_kernel(CGArgs A, zd *g) {
+    constexpr int NP = CRT_FSAI_NP;
+    constexpr int NG = NP * (NP + 1) / 2;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (blockDim.x >> 6) + wv;
+    if (j >= A.manz) return;
+    int pc[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) pc[s] = A.fpat[(size_t)j * NP + s];
+    double gr[NG], gi[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) gr[q] = gi[q] = 0.0;
+    for (int i = lane; i < A.nanz; i += 64) {
+        const double wi = A.w[i];
+        zd v[NP];
+#pragma unroll
+        for (int s = 0; s < NP; ++s) v[s] = pc[s] >= 0 ? A.J[(size_t)pc[s] * A.nanz + i] : zd{0.0, 0.0};
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+#pragma unroll
+            for (int b = a; b < NP; ++b, ++q) {
+                // w conj(J_a) J_b
+                gr[q] += wi * (v[a].re * v[b].re + v[a].im * v[b].im);
+                gi[q] += wi * (v[a].re * v[b].im - v[a].im * v[b].re);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < NG; ++q)
+        for (int o = 32; o >= 1; o >>= 1) {
+            gr[q] += __shfl_xor(gr[q], o, 64);
+            gi[q] += __shfl_xor(gi[q], o, 64);
+        }
+    if (lane != 0) return;
+    // Hermitian block (upper triangle stored row-wise), unused slots -> identity
+    double c[NP];
+#pragma unroll
+    for (int s = 0; s < NP; ++s) c[s] = pc[s] >= 0 ? A.cgfac[pc[s]] : 0.0;
+    zd m[NP][NP + 1];
+    int q = 0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a)
+#pragma unroll
+        for (int b = a; b < NP; ++b, ++q) {
+            zd e;
+            if (pc[a] < 0 || pc[b] < 0) {
+                e = zd{a == b ? 1.0 : 0.0, 0.0};
+            } else {
+                const double rr = A.lam * rentry(A, pc[a], pc[b]);
+                e = zd{c[a] * c[b] * (gr[q] + rr), c[a] * c[b] * gi[q]};
+            }
+            m[a][b] = e;
+            m[b][a] = zd{e.re, -e.im};
+        }
+#pragma unroll
+    for (int a = 0; a < NP; ++a) m[a][NP] = zd{a == NP - 1 ? 1.0 : 0.0, 0.0};
+    // Gaussian elimination (Hermitian positive definite: no pivoting)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const zd inv = zdiv(zd{1.0, 0.0}, m[k][k]);
+#pragma unroll
+        for (int b = k; b <= NP; ++b) m[k][b] = zmul(m[k][b], inv);
+#pragma unroll
+        for (int a = k + 1; a < NP; ++a) {
+            const zd f = m[a][k];
+#pragma unroll
+            for (int b = k; b <= NP; ++b) m[a][b] = zsub(m[a][b], zmul(f, m[k][b]));
+        }
+    }
+    zd y[NP];
+#pragma unroll
+    for (int a = NP - 1; a >= 0; --a) {
+        zd acc = m[a][NP];
+#pragma unroll
+        for (int b = a + 1; b < NP; ++b) acc = zsub(acc, zmul(m[a][b], y[b]));
+        y[a] = acc;
+    }
+    const double yl = y[NP - 1].re;
+    const double sc = yl > 0.0 ? 1.0 / sqrt(yl) : 0.0;
+#pragma unroll
+    for (int s = 0; s < NP; ++s)
+        g[(size_t)j * NP + s] = pc[s] >= 0 ? zd{y[s].re * sc, y[s].im * sc} : zd{0.0, 0.0};
 }
 
 // ---------------------------------------------------------------- launchers
@@ -339,6 +489,10 @@ int crt_launch_cg_rmul(hipStream_t s, const CGArgs &a, const zd *x, zd *out) {
 
 int crt_launch_cg_init(hipStream_t s, const CGArgs &a, const zd *bvec) {
     hipLaunchKernelGGL(crt_cg_init_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a, bvec);
+    if (a.fg) {
+        hipLaunchKernelGGL(crt_cg_pg_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+        hipLaunchKernelGGL(crt_cg_pgh_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -355,6 +509,10 @@ int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
     else
         hipLaunchKernelGGL(crt_cg_jhw_kernel<zd>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J);
     hipLaunchKernelGGL(crt_cg_upd_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+    if (a.fg) {
+        hipLaunchKernelGGL(crt_cg_pg_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+        hipLaunchKernelGGL(crt_cg_pgh_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -373,5 +531,10 @@ __global__ void __launch_bounds__(256) crt_cg_j32_kernel(size_t n, const zd *J, 
 
 int crt_launch_cg_j32(hipStream_t s, size_t n, const zd *J, float2 *J32) {
     hipLaunchKernelGGL(crt_cg_j32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, J, J32);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_cg_fsai(hipStream_t s, const CGArgs &a, zd *g) {
+    hipLaunchKernelGGL(crt_cg_fsai_kernel, dim3((a.manz + 3) / 4), dim3(256), 0, s, a, g);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

@@ -126,8 +126,14 @@ struct DevGrid {
 // Gauss-Newton CG state shared by crt_cg.hip and crt_api.hip
 struct CGScalars {
     double dr, dr0, dr1, alpha, beta;
+    double rz, rz1;  // r^H M r of this and the previous iteration (preconditioned CG)
     int k, done, ncg, pad;
 };
+
+// factorised sparse approximate inverse M = G^H G of the CG operator: row j of the
+// lower-triangular G has CRT_FSAI_NP entries on the columns fsai_pat[j*NP + s]
+// (-1 unused, the last slot is j itself)
+#define CRT_FSAI_NP 8
 
 struct CGArgs {
     int nanz, manz, ltri, nx, maxn, ncgmax;
@@ -144,6 +150,11 @@ struct CGArgs {
     double *part_dr1;       // per block of the J^H kernel
     double *part_rr;        // per block of the update kernel
     int n_dr1, n_rr, ncb, cb;
+    const zd *fg;           // FSAI: G entries [manz][CRT_FSAI_NP], or null (Jacobi only)
+    const int *fpat;        // FSAI: column of each G entry, -1 unused
+    const int *ftp;         // FSAI: transposed pattern, entry index i*NP+s of G(i, j), -1 unused
+    zd *t, *z;              // FSAI: G r and z = G^H G r
+    double *part_rz;        // FSAI: per block Re(r^H z)
     const CGScalars *sc;    // state after the previous iteration (read only)
     CGScalars *scn;         // state of this iteration (written by block 0)
     double *cgres;          // ncgmax+1

@@ -225,6 +225,10 @@ class Context:
                                 C.byref(lam)), "crt_blam0")
         return lam.value
 
+    def set_cg_preconditioner(self, kind: int):
+        """0: Jacobi scaling (the reference); 1/2: FSAI on the stencil / wider pattern."""
+        _chk(self.lib.crt_set_cg_preconditioner(self._h, int(kind)), "crt_set_cg_preconditioner")
+
     def set_cg_precision(self, jbits: int):
         """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""
         _chk(self.lib.crt_set_cg_precision(self._h, int(jbits)), "crt_set_cg_precision")

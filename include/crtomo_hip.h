@@ -103,6 +103,16 @@ int crt_update(crt_ctx *ctx, double lam, double eps, int ncgmax, double step, co
  * the bytes of each iteration. */
 int crt_set_cg_precision(crt_ctx *ctx, int jbits);
 
+/* Preconditioner of the CG iterations of crt_cjg / crt_update.  0 (default): the
+ * reference's Jacobi scaling C alone.  1: in addition a factorised sparse
+ * approximate inverse M = G^H G of B = C (J^H W J + lam R) C, G lower triangular
+ * on the regularisation stencil (row j: its lower neighbours and j), built on the
+ * device after cgfac from the Hermitian blocks B(S_j, S_j).  2: the same on a wider
+ * lower pattern (regular grids: two cells left, the three cells above, two above).
+ * The stop test stays on |r|^2 of the scaled system, as in the reference; only the
+ * iteration count changes.  No counterpart in the reference. */
+int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
+
 /* Smoothness regularisation built on the device from the grid of crt_set_grid and
  * installed for crt_cjg / crt_update (replaces crt_set_regularization):
  * ltri 0 -> bsmatmreg (src/bsmatm_mod.f90:190-362) on the regular nx x nz quad grid,
