@@ -38,6 +38,25 @@ int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
 int crt_cg_geometry(int, int, int *, int *, int *, int *);
 int crt_launch_cg_j32(hipStream_t, size_t, const zd *, float2 *);
 int crt_launch_cg_fsai(hipStream_t, const CGArgs &, zd *);
+// learned preconditioner (crt_learned_pc.hip)
+struct PcCell {
+    int manz, nx, nz, nanz;
+    double lam;
+    const double *cgfac, *smatm, *cy;
+    double ymin, ymax;
+    const zd *J;
+    const double *w;
+};
+int crt_pc_nparams();
+int crt_pc_blocks(int);
+size_t crt_pc_gpart_size();
+int crt_launch_pc_features(hipStream_t, const PcCell &, zd *, float *);
+int crt_launch_pc_targets(hipStream_t, int, int, const int *, const zd *, const float *, float *, float *);
+int crt_launch_pc_norm(hipStream_t, int, const float *, float *);
+int crt_launch_pc_fwdbwd(hipStream_t, int, const float *, const float *, const float *, const float *, const float *,
+                         float *, float *, double *, double *, float *);
+int crt_launch_pc_adam(hipStream_t, float *, float *, float *, const float *, float, int);
+int crt_launch_pc_infer(hipStream_t, int, int, const float *, const float *, const float *, const int *, zd *);
 int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const zd *, const zd *,
                       const double *, double *, zd *);
 int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
@@ -122,6 +141,18 @@ struct crt_ctx {
     DBuf<int> d_fpat, d_ftp;
     DBuf<zd> d_fg, d_t, d_z;
     DBuf<double> d_part_rz;
+    // learned preconditioner: parameters, Adam moments, feature standardisation
+    DBuf<float> d_pc_par, d_pc_m, d_pc_v, d_pc_norm, d_pc_feat, d_pc_tgt, d_pc_mask, d_pc_act, d_pc_delta, d_pc_grad;
+    DBuf<double> d_pc_loss, d_pc_gpart, d_cy;
+    DBuf<zd> d_pc_cpl;
+    DBuf<float> d_pc_tfeat, d_pc_ttgt, d_pc_tmask;  // training set: every system trained on so far
+    int pc_tn = 0;
+    int pc_steps = 0, pc_manz = 0, pc_fallbacks = 0;
+    bool pc_ready = false, pc_norm_set = false;
+    double cy_min = 0.0, cy_max = 0.0;
+    // the last CG system (for training on it)
+    double last_lam = 0.0;
+    bool have_cgsys = false;
     DBuf<zd> d_res, d_rm, d_bvec, d_r, d_p, d_q, d_bv, d_dpar, d_ap, d_part_ap, d_par;
     DBuf<CGScalars> d_sc;
     int cg_check_every = 16;
@@ -930,11 +961,17 @@ static void fsai_pattern(const crt_ctx *c, int manz, int kind, std::vector<int> 
 }
 
 // build G of M = G^H G for the current cgfac and lambda (after the RHS pass)
+static int pc_features(crt_ctx *c, const CGArgs &a);
+
 static int cg_fsai(crt_ctx *c, CGArgs &a) {
+    c->last_lam = a.lam;
+    c->have_cgsys = true;
     if (c->cg_prec == 0) return 0;
     const int manz = a.manz;
+    const bool learned = c->cg_prec == 3;
+    if (learned && (!c->pc_ready || c->ltri != 0)) return CRT_ERR_STATE;
     std::vector<int> pat, tp;
-    fsai_pattern(c, manz, c->cg_prec, pat, tp);
+    fsai_pattern(c, manz, learned ? 1 : c->cg_prec, pat, tp);
     int e = c->d_fpat.upload(pat.data(), pat.size(), c->stream);
     e = e ? e : c->d_ftp.upload(tp.data(), tp.size(), c->stream);
     e = e ? e : c->d_fg.ensure((size_t)manz * CRT_FSAI_NP);
@@ -944,12 +981,61 @@ static int cg_fsai(crt_ctx *c, CGArgs &a) {
     if (e) return e;
     a.fpat = c->d_fpat.p;
     a.ftp = c->d_ftp.p;
-    if ((e = crt_launch_cg_fsai(c->stream, a, c->d_fg.p))) return e;
+    if (learned) {
+        if ((e = pc_features(c, a))) return e;
+        e = crt_launch_pc_infer(c->stream, manz, c->rnx, c->d_pc_par.p, c->d_pc_norm.p, c->d_pc_feat.p, c->d_fpat.p,
+                                c->d_fg.p);
+    } else {
+        e = crt_launch_cg_fsai(c->stream, a, c->d_fg.p);
+    }
+    if (e) return e;
     a.fg = c->d_fg.p;
     a.t = c->d_t.p;
     a.z = c->d_z.p;
     a.part_rz = c->d_part_rz.p;
     return 0;
+}
+
+// features of the cells of the current system (cgfac, lambda, R, depth)
+static int pc_features(crt_ctx *c, const CGArgs &a) {
+    const int manz = a.manz;
+    int e = 0;
+    {
+        // cell centroid depth coordinate (mean of the element's node y)
+        std::vector<double> cy(manz);
+        for (int i = 0; i < manz; ++i) {
+            double sy = 0.0;
+            for (int l = 0; l < c->nkel[i]; ++l) sy += c->y[c->nrel[(size_t)i * c->maxn + l] - 1];
+            cy[i] = sy / c->nkel[i];
+        }
+        c->cy_min = *std::min_element(cy.begin(), cy.end());
+        c->cy_max = *std::max_element(cy.begin(), cy.end());
+        if ((e = c->d_cy.upload(cy.data(), manz, c->stream))) return e;
+    }
+    if ((e = c->d_pc_feat.ensure((size_t)manz * 8))) return e;
+    if ((e = c->d_pc_cpl.ensure((size_t)manz * 3))) return e;
+    PcCell pc{manz, c->rnx, manz / std::max(c->rnx, 1), a.nanz, a.lam, a.cgfac, a.smatm, c->d_cy.p, c->cy_min,
+              c->cy_max, a.J, a.w};
+    return crt_launch_pc_features(c->stream, pc, c->d_pc_cpl.p, c->d_pc_feat.p);
+}
+
+static void pc_init_params(std::vector<float> &par) {
+    // Xavier-uniform weights from a fixed linear congruential sequence, zero biases
+    const int NI = 8, NH = 64, NO = 5;
+    par.assign(crt_pc_nparams(), 0.f);
+    unsigned long long st = 0x2545F4914F6CDD1DULL;
+    auto uni = [&]() {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    size_t o = 0;
+    const int shp[4][2] = {{NH, NI}, {NH, NH}, {NH, NH}, {NO, NH}};
+    for (int l = 0; l < 4; ++l) {
+        const int no = shp[l][0], ni = shp[l][1];
+        const double a = std::sqrt(6.0 / (ni + no)) * (l == 3 ? 0.1 : 1.0);
+        for (int q = 0; q < no * ni; ++q) par[o + q] = (float)(a * uni());
+        o += (size_t)no * ni + no;
+    }
 }
 
 static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
@@ -989,6 +1075,16 @@ static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
     return 0;
 }
 
+// the learned preconditioner falls back to the reference's Jacobi-scaled CG when it
+// does not reach the stop test within ncgmax iterations
+static int cg_fallback(crt_ctx *c, const CGArgs &a, int ncgmax, int *n) {
+    if (c->cg_prec != 3 || !a.fg || *n < ncgmax) return 0;
+    CGArgs aj = a;
+    aj.fg = nullptr;
+    ++c->pc_fallbacks;
+    return cg_iterate(c, aj, c->d_bvec.p, n);
+}
+
 static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfak) {
     std::vector<double> w(nanz);
     for (int i = 0; i < nanz; ++i) w[i] = wmatd[i] * (double)wdfak[i];
@@ -996,8 +1092,177 @@ static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfa
 }
 
 int crt_set_cg_preconditioner(crt_ctx *c, int kind) {
-    if (!c || kind < 0 || kind > 2) return CRT_ERR_ARG;
+    if (!c || kind < 0 || kind > 3) return CRT_ERR_ARG;
     c->cg_prec = kind;
+    return 0;
+}
+
+int crt_pc_size(void) { return 16 + crt_pc_nparams(); }
+
+int crt_pc_weights(crt_ctx *c, float *get, const float *set) {
+    if (!c) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    const int np = crt_pc_nparams();
+    if (set) {
+        int e = c->d_pc_norm.upload(set, 16, c->stream);
+        e = e ? e : c->d_pc_par.upload(set + 16, np, c->stream);
+        std::vector<float> zero(np, 0.f);
+        e = e ? e : c->d_pc_m.upload(zero.data(), np, c->stream);
+        e = e ? e : c->d_pc_v.upload(zero.data(), np, c->stream);
+        if (e) return e;
+        CRT_HIP(hipStreamSynchronize(c->stream));
+        c->pc_steps = 0;
+        c->pc_ready = c->pc_norm_set = true;
+    }
+    if (get) {
+        if (!c->pc_ready) return CRT_ERR_STATE;
+        CRT_HIP(c->copy_out(get, c->d_pc_norm.p, 16 * sizeof(float)));
+        CRT_HIP(c->copy_out(get + 16, c->d_pc_par.p, np * sizeof(float)));
+    }
+    return 0;
+}
+
+int crt_pc_train(crt_ctx *c, int steps, double lr, double *loss_first, double *loss_last) {
+    if (!c || steps < 1 || !(lr > 0.0)) return CRT_ERR_ARG;
+    if (!c->have_cgsys || !c->have_sens || c->ltri != 0) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    const int np = crt_pc_nparams();
+    int e = 0;
+    if (!c->pc_ready) {
+        std::vector<float> par, zero(np, 0.f);
+        pc_init_params(par);
+        e = c->d_pc_par.upload(par.data(), np, c->stream);
+        e = e ? e : c->d_pc_m.upload(zero.data(), np, c->stream);
+        e = e ? e : c->d_pc_v.upload(zero.data(), np, c->stream);
+        e = e ? e : c->d_pc_norm.ensure(16);
+        if (e) return e;
+        c->pc_steps = 0;
+    }
+    // teacher: the FSAI rows on the stencil pattern of the last system
+    CGArgs a;
+    if ((e = cg_prepare(c, nanz, manz, 1, c->last_lam, 1e-30, a))) return e;
+    std::vector<int> pat, tp;
+    fsai_pattern(c, manz, 1, pat, tp);
+    e = c->d_fpat.upload(pat.data(), pat.size(), c->stream);
+    e = e ? e : c->d_ftp.upload(tp.data(), tp.size(), c->stream);
+    e = e ? e : c->d_fg.ensure((size_t)manz * CRT_FSAI_NP);
+    if (e) return e;
+    a.fpat = c->d_fpat.p;
+    if ((e = crt_launch_cg_fsai(c->stream, a, c->d_fg.p))) return e;
+    if ((e = pc_features(c, a))) return e;
+    e = c->d_pc_tgt.ensure((size_t)manz * 5);
+    e = e ? e : c->d_pc_mask.ensure((size_t)manz * 5);
+    e = e ? e : c->d_pc_act.ensure((size_t)manz * (8 + 3 * 64));
+    e = e ? e : c->d_pc_delta.ensure((size_t)manz * (3 * 64 + 5));
+    e = e ? e : c->d_pc_grad.ensure(np);
+    e = e ? e : c->d_pc_gpart.ensure(crt_pc_gpart_size());
+    e = e ? e : c->d_pc_loss.ensure(crt_pc_blocks(manz));
+    e = e ? e : crt_launch_pc_targets(c->stream, manz, c->rnx, c->d_fpat.p, c->d_fg.p, c->d_pc_feat.p, c->d_pc_tgt.p,
+                                      c->d_pc_mask.p);
+    if (e) return e;
+    // append this system to the training set (at most 2^22 cells, oldest dropped)
+    {
+        const int cap = 1 << 22;
+        int keep = std::min(c->pc_tn, cap - manz);
+        if (keep < 0) keep = 0;
+        const int drop = c->pc_tn - keep, tot = keep + manz;
+        DBuf<float> nf, nt, nm;
+        e = nf.ensure((size_t)tot * 8);
+        e = e ? e : nt.ensure((size_t)tot * 5);
+        e = e ? e : nm.ensure((size_t)tot * 5);
+        if (e) return e;
+        if (keep) {
+            CRT_HIP(hipMemcpyAsync(nf.p, c->d_pc_tfeat.p + (size_t)drop * 8, (size_t)keep * 8 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            CRT_HIP(hipMemcpyAsync(nt.p, c->d_pc_ttgt.p + (size_t)drop * 5, (size_t)keep * 5 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
+            CRT_HIP(hipMemcpyAsync(nm.p, c->d_pc_tmask.p + (size_t)drop * 5, (size_t)keep * 5 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+        CRT_HIP(hipMemcpyAsync(nf.p + (size_t)keep * 8, c->d_pc_feat.p, (size_t)manz * 8 * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+        CRT_HIP(hipMemcpyAsync(nt.p + (size_t)keep * 5, c->d_pc_tgt.p, (size_t)manz * 5 * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+        CRT_HIP(hipMemcpyAsync(nm.p + (size_t)keep * 5, c->d_pc_mask.p, (size_t)manz * 5 * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+        CRT_HIP(hipStreamSynchronize(c->stream));
+        std::swap(c->d_pc_tfeat.p, nf.p); std::swap(c->d_pc_tfeat.n, nf.n);
+        std::swap(c->d_pc_ttgt.p, nt.p); std::swap(c->d_pc_ttgt.n, nt.n);
+        std::swap(c->d_pc_tmask.p, nm.p); std::swap(c->d_pc_tmask.n, nm.n);
+        c->pc_tn = tot;
+    }
+    const int ntr = c->pc_tn;
+    e = c->d_pc_act.ensure((size_t)ntr * (8 + 3 * 64));
+    e = e ? e : c->d_pc_delta.ensure((size_t)ntr * (3 * 64 + 5));
+    e = e ? e : c->d_pc_loss.ensure(crt_pc_blocks(ntr));
+    if (e) return e;
+    c->pc_manz = ntr;
+    if (!c->pc_norm_set) {
+        if ((e = crt_launch_pc_norm(c->stream, manz, c->d_pc_feat.p, c->d_pc_norm.p))) return e;
+        c->pc_norm_set = true;
+    }
+    const int nb = crt_pc_blocks(ntr);
+    std::vector<double> pl(nb);
+    auto loss = [&](double *out) -> int {
+        if (!out) return 0;
+        if (c->copy_out(pl.data(), c->d_pc_loss.p, nb * sizeof(double)) != hipSuccess) return CRT_ERR_HIP;
+        double t = 0.0;
+        for (double v : pl) t += v;
+        *out = t;
+        return 0;
+    };
+    for (int it = 0; it < steps; ++it) {
+        e = crt_launch_pc_fwdbwd(c->stream, ntr, c->d_pc_par.p, c->d_pc_norm.p, c->d_pc_tfeat.p, c->d_pc_ttgt.p,
+                                 c->d_pc_tmask.p, c->d_pc_act.p, c->d_pc_delta.p, c->d_pc_loss.p, c->d_pc_gpart.p,
+                                 c->d_pc_grad.p);
+        if (e) return e;
+        if (it == 0 && (e = loss(loss_first))) return e;
+        if ((e = crt_launch_pc_adam(c->stream, c->d_pc_par.p, c->d_pc_m.p, c->d_pc_v.p, c->d_pc_grad.p, (float)lr,
+                                    ++c->pc_steps)))
+            return e;
+    }
+    // loss after the last update
+    e = crt_launch_pc_fwdbwd(c->stream, ntr, c->d_pc_par.p, c->d_pc_norm.p, c->d_pc_tfeat.p, c->d_pc_ttgt.p,
+                             c->d_pc_tmask.p, c->d_pc_act.p, c->d_pc_delta.p, c->d_pc_loss.p, c->d_pc_gpart.p,
+                             c->d_pc_grad.p);
+    if (e || (e = loss(loss_last))) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->pc_ready = true;
+    return 0;
+}
+
+int crt_pc_grad(crt_ctx *c, double *loss, float *grad) {
+    // loss and gradient of the current parameters on the training system of the last
+    // crt_pc_train call (no update)
+    if (!c || !grad) return CRT_ERR_ARG;
+    if (!c->pc_ready || c->pc_manz == 0) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int manz = c->pc_manz;
+    const int nb = crt_pc_blocks(manz);
+    int e = crt_launch_pc_fwdbwd(c->stream, manz, c->d_pc_par.p, c->d_pc_norm.p, c->d_pc_tfeat.p, c->d_pc_ttgt.p,
+                                 c->d_pc_tmask.p, c->d_pc_act.p, c->d_pc_delta.p, c->d_pc_loss.p, c->d_pc_gpart.p,
+                                 c->d_pc_grad.p);
+    if (e) return e;
+    std::vector<double> pl(nb);
+    CRT_HIP(c->copy_out(pl.data(), c->d_pc_loss.p, nb * sizeof(double)));
+    CRT_HIP(c->copy_out(grad, c->d_pc_grad.p, crt_pc_nparams() * sizeof(float)));
+    if (loss) {
+        double t = 0.0;
+        for (double v : pl) t += v;
+        *loss = t;
+    }
+    return 0;
+}
+
+int crt_pc_data(crt_ctx *c, float *feat, float *tgt, float *mask) {
+    // the standardised network inputs' raw features, targets and mask of the training system
+    if (!c) return CRT_ERR_ARG;
+    if (c->pc_manz == 0) return CRT_ERR_STATE;
+    const size_t manz = c->pc_manz;
+    if (feat) CRT_HIP(c->copy_out(feat, c->d_pc_tfeat.p, manz * 8 * sizeof(float)));
+    if (tgt) CRT_HIP(c->copy_out(tgt, c->d_pc_ttgt.p, manz * 5 * sizeof(float)));
+    if (mask) CRT_HIP(c->copy_out(mask, c->d_pc_tmask.p, manz * 5 * sizeof(float)));
     return 0;
 }
 
@@ -1024,6 +1289,7 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if ((e = cg_fsai(c, a))) return e;
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
+    if ((e = cg_fallback(c, a, ncgmax, &n))) return e;
     if (dpar) CRT_HIP(c->copy_out(dpar, c->d_dpar.p, manz * sizeof(zd)));
     if (cgres) CRT_HIP(c->copy_out(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double)));
     if (ncg) *ncg = n;
@@ -1083,6 +1349,7 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
     if ((e = cg_fsai(c, a))) return e;
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
+    if ((e = cg_fallback(c, a, ncgmax, &n))) return e;
     if ((e = crt_launch_cg_final(c->stream, manz, step, c->d_cgfac.p, c->d_dpar.p, c->d_par.p))) return e;
     if (dpar) CRT_HIP(hipMemcpyAsync(dpar, c->d_dpar.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
     CRT_HIP(hipMemcpyAsync(par, c->d_par.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));

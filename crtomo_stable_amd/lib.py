@@ -229,6 +229,36 @@ class Context:
         """0: Jacobi scaling (the reference); 1/2: FSAI on the stencil / wider pattern."""
         _chk(self.lib.crt_set_cg_preconditioner(self._h, int(kind)), "crt_set_cg_preconditioner")
 
+    def pc_train(self, steps: int, lr: float = 1e-3):
+        """Train the learned preconditioner on the last CG system; returns the loss
+        before the first and after the last Adam step."""
+        l0, l1 = C.c_double(), C.c_double()
+        _chk(self.lib.crt_pc_train(self._h, int(steps), C.c_double(lr), C.byref(l0), C.byref(l1)), "crt_pc_train")
+        return l0.value, l1.value
+
+    def pc_weights(self, set_to=None):
+        """Model vector (16 standardisation values, then the parameters); optionally set it first."""
+        n = self.lib.crt_pc_size()
+        if set_to is not None:
+            w = np.ascontiguousarray(set_to, np.float32)
+            assert w.size == n
+            _chk(self.lib.crt_pc_weights(self._h, None, _ptr(w)), "crt_pc_weights")
+        out = np.zeros(n, np.float32)
+        _chk(self.lib.crt_pc_weights(self._h, _ptr(out), None), "crt_pc_weights")
+        return out
+
+    def pc_grad(self):
+        n = self.lib.crt_pc_size() - 16
+        g = np.zeros(n, np.float32)
+        loss = C.c_double()
+        _chk(self.lib.crt_pc_grad(self._h, C.byref(loss), _ptr(g)), "crt_pc_grad")
+        return loss.value, g
+
+    def pc_data(self, manz: int):
+        f = np.zeros((manz, 8), np.float32); t = np.zeros((manz, 5), np.float32); m = np.zeros((manz, 5), np.float32)
+        _chk(self.lib.crt_pc_data(self._h, _ptr(f), _ptr(t), _ptr(m)), "crt_pc_data")
+        return f, t, m
+
     def set_cg_precision(self, jbits: int):
         """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""
         _chk(self.lib.crt_set_cg_precision(self._h, int(jbits)), "crt_set_cg_precision")

@@ -109,9 +109,34 @@ int crt_set_cg_precision(crt_ctx *ctx, int jbits);
  * on the regularisation stencil (row j: its lower neighbours and j), built on the
  * device after cgfac from the Hermitian blocks B(S_j, S_j).  2: the same on a wider
  * lower pattern (regular grids: two cells left, the three cells above, two above).
- * The stop test stays on |r|^2 of the scaled system, as in the reference; only the
- * iteration count changes.  No counterpart in the reference. */
+ * 3: the learned preconditioner below.  The stop test stays on |r|^2 of the scaled
+ * system, as in the reference; only the iteration count changes.  No counterpart
+ * in the reference. */
 int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
+
+/* Learned preconditioner (kind 3 of crt_set_cg_preconditioner, regular grids): a
+ * per-cell network 8 -> 64 -> 64 -> 64 -> 5 emits the row of G on the stencil
+ * pattern {N, W, j} (as a correction to the estimate that neglects the N-W
+ * coupling) from the cell's coverage, its couplings B(W,j), B(N,j), B(N,W) in the
+ * scaled normal matrix and lam R_jj / coverage; inference with BF16 weights and
+ * FP32 accumulation.  If it does not reach the stop test
+ * within ncgmax iterations, the CG reruns with the Jacobi scaling alone.  No
+ * counterpart in the reference.
+ *   crt_pc_train: adds the system of the last crt_update / crt_cjg call to the
+ *     training set (targets: that system's FSAI rows, kind 1) and runs `steps`
+ *     full-batch Adam steps (learning rate lr) on every system added so far.
+ *     loss_first / loss_last (may be NULL) receive the mean squared error before
+ *     the first and after the last step.
+ *   crt_pc_weights: get and/or set the model, crt_pc_size() floats (16 feature
+ *     standardisation values, then the parameters).
+ *   crt_pc_grad: loss and parameter gradient on the training set (no update).
+ *   crt_pc_data: the training set's raw features (n x 8), targets (the corrections
+ *     to the analytic estimate) and mask (n x 5); n = cells added so far. */
+int crt_pc_train(crt_ctx *ctx, int steps, double lr, double *loss_first, double *loss_last);
+int crt_pc_size(void);
+int crt_pc_weights(crt_ctx *ctx, float *get, const float *set);
+int crt_pc_grad(crt_ctx *ctx, double *loss, float *grad);
+int crt_pc_data(crt_ctx *ctx, float *feat, float *tgt, float *mask);
 
 /* Smoothness regularisation built on the device from the grid of crt_set_grid and
  * installed for crt_cjg / crt_update (replaces crt_set_regularization):

@@ -220,3 +220,50 @@ def test_update_fsai_preconditioner(ctx, oracle_lib, ltri, kind):
     assert _rel(conv["dpar"], dpar_o) <= 1e-10
     assert _rel(stop["dpar"], dpar_s) <= 1e-4
     assert stop["ncg"] < ncg_s
+
+
+def test_learned_preconditioner(ctx, oracle_lib):
+    """Learned preconditioner (no counterpart in the reference): (1) the device loss and
+    parameter gradient of the network equal the CPU reference's (tests/pc_reference.py,
+    FP32 kernels: 1e-5 / 1e-4 relative); (2) training on the FSAI rows of one system
+    lowers the loss; (3) on another system (other weights, data and lambda) the learned
+    preconditioner leaves the converged update equal to the oracle's (1e-10), keeps the
+    stop-test update within the stop's accuracy of the Jacobi one (1e-4), and takes
+    fewer iterations than Jacobi."""
+    import pc_reference as R
+    O = oracle_lib
+    w = _workload(0)
+    manz = w.grid.elanz
+    P = _problem(O, w, 0, seed=0)
+    _gpu_update(ctx, w, P, 0, 3.0, 1e-12, 400)
+    ctx.pc_train(1, 1e-3)
+    loss, g = ctx.pc_grad()
+    wv = ctx.pc_weights()
+    feat, tgt, mask = ctx.pc_data(manz)
+    lref, gref = R.loss_grad(wv[16:].astype(np.float64), wv[:16], feat, tgt, mask)
+    print(f"pc loss {loss:.6e} ref {lref:.6e}; grad max diff {np.max(np.abs(g - gref)):.3e} of {np.max(np.abs(gref)):.3e}",
+          flush=True)
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
+    l0, l1 = ctx.pc_train(300, 3e-3)
+    print(f"pc training loss {l0:.4e} -> {l1:.4e}", flush=True)
+    assert l1 < 0.5 * l0
+    # the inversion's lambda search produces systems at other lambdas: train on them too
+    for seed, lam in ((1, 10.0), (2, 0.3)):
+        _gpu_update(ctx, w, _problem(O, w, 0, seed=seed), 0, lam, 1e-12, 400)
+        la, lb = ctx.pc_train(300, 3e-3)
+        print(f"pc training (lambda {lam}) loss {la:.4e} -> {lb:.4e}", flush=True)
+    P2 = _problem(O, w, 0, seed=5)
+    lam2 = 1.0
+    jac = _gpu_update(ctx, w, P2, 0, lam2, 1e-12, 400)
+    _, _, _, dpar_o, _, _ = _oracle_update(O, w, P2, 0, lam2, 1e-30, 3000)
+    ctx.set_cg_preconditioner(3)
+    try:
+        conv = _gpu_update(ctx, w, P2, 0, lam2, 1e-30, 3000)
+        stop = _gpu_update(ctx, w, P2, 0, lam2, 1e-12, 400)
+    finally:
+        ctx.set_cg_preconditioner(0)
+    print(f"learned: ncg {stop['ncg']} vs Jacobi {jac['ncg']}", flush=True)
+    assert _rel(conv["dpar"], dpar_o) <= 1e-10
+    assert _rel(stop["dpar"], jac["dpar"]) <= 1e-4
+    assert stop["ncg"] < jac["ncg"]

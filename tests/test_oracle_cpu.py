@@ -332,3 +332,25 @@ def test_cholz_reconstruction(oracle_lib):
     A2[0, 0] = 0.0
     err2, _, _ = oracle_lib.cholz(A2)
     assert err2 != 0
+
+
+def test_pc_reference_gradient_finite_differences():
+    """The CPU reference of the learned preconditioner's network: analytic gradient
+    against central differences (FP64) on random data; a wrong activation derivative
+    or a transposed weight fails it."""
+    import pc_reference as R
+    rng = np.random.default_rng(3)
+    m = 17
+    params = 0.3 * rng.standard_normal(R.NPAR)
+    norm = np.concatenate([rng.standard_normal(8), 0.5 + rng.random(8)])
+    feat = rng.standard_normal((m, 8))
+    tgt = rng.standard_normal((m, 5))
+    mask = (rng.random((m, 5)) > 0.2).astype(float)
+    loss, g = R.loss_grad(params, norm, feat, tgt, mask)
+    idx = rng.choice(R.NPAR, 60, replace=False)
+    for i in idx:
+        h = 1e-6
+        p1 = params.copy(); p1[i] += h
+        p2 = params.copy(); p2[i] -= h
+        fd = (R.loss_grad(p1, norm, feat, tgt, mask)[0] - R.loss_grad(p2, norm, feat, tgt, mask)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
