@@ -156,6 +156,28 @@ def gn_step(ctx, w, ltri: int):
                        "cg_ms_per_iteration": round(tmv["cg_matvec"], 4),
                        "dpar_rel_diff_vs_fp64_jacobi": float(np.max(np.abs(rv["dpar"] - d64)) / np.max(np.abs(d64))),
                        "step_ms": round(1e3 * (t["step"] - t["update"] + tv), 3)}
+    # learned preconditioner: trained on the system of a neighbouring lambda trial
+    # (2 lam0), then used at lam0; the training time is reported apart from the step
+    ctx.set_cg_precision(64)
+    ctx.set_cg_preconditioner(0)
+    ctx.update(2.0 * lam, 1e-12, 1, wmatd, wdfak, dat, par)
+    t1 = time.perf_counter()
+    l0, l1 = ctx.pc_train(300, 3e-3)
+    device_sync()
+    t_train = time.perf_counter() - t1
+    for name, jbits in (("learned", 64), ("fp32_jacobian_learned", 32)):
+        ctx.set_cg_precision(jbits)
+        ctx.set_cg_preconditioner(3)
+        t1 = time.perf_counter()
+        rv = ctx.update(lam, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+        device_sync()
+        tv = time.perf_counter() - t1
+        tmv = ctx.times()
+        forms[name] = {"update_ms": round(1e3 * tv, 3), "ncg": rv["ncg"],
+                       "cg_ms_per_iteration": round(tmv["cg_matvec"], 4),
+                       "dpar_rel_diff_vs_fp64_jacobi": float(np.max(np.abs(rv["dpar"] - d64)) / np.max(np.abs(d64))),
+                       "step_ms": round(1e3 * (t["step"] - t["update"] + tv), 3),
+                       "pc_train_ms": round(1e3 * t_train, 1), "pc_loss": [l0, l1]}
     ctx.set_cg_precision(64)
     ctx.set_cg_preconditioner(0)
     nanz = v_true.size
