@@ -1195,6 +1195,8 @@ __device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd
         }
 }
 
+}  // namespace
+
 template <int D>
 __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(Solve16Args A) {
     constexpr int NA = (D + SW - 2) / (SW - 1);  // off-diagonal tiles per worker wave
@@ -1371,6 +1373,8 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             for (int v = 0; v < 4; ++v) cur[u][v] = nxt[u][v];
     }
 }
+
+namespace {
 
 template <int D>
 int launch_solve16(hipStream_t st, const Solve16Args &a) {

@@ -1,0 +1,276 @@
+// crt_io.hip -- the reference's on-disk formats on the host side of the library, so a
+// CRMod tomodir runs without the Fortran host.
+//
+//   crmod.cfg  the ***FILES*** block: elem, elec, rho, config paths, then the
+//              potentials / measurements / sensitivities switches with their output
+//              paths (the layout of the reference's tests/*/exe/crmod.cfg)
+//   elem.dat   src/relem.f90:60-151 (header, type table, node table, element node
+//              lists, boundary-element neighbours) with the border-to-element check
+//              of :197-229
+//   elec.dat   src/relectr.f90 (count, then node numbers)
+//   rho.dat    src/rsigma.f90:168 (count, then magnitude and phase in mrad per
+//              element; sigma = (cos p - i sin p) / |rho|)
+//   config.dat src/rdatm.f90 (count, then current and potential electrode codes)
+//   volt.dat   src/wdatm.f90:58-76 (count, then strnr, vnr, REAL*4 magnitude and
+//              1000 * phase; a phase near +-pi swaps the potential electrodes)
+#include "crt_common.hpp"
+#include "../../include/crtomo_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Tok {
+    std::vector<std::string> t;
+    size_t p = 0;
+    bool load(const std::string &path) {
+        std::ifstream f(path);
+        if (!f) return false;
+        std::string w;
+        while (f >> w) t.push_back(w);
+        return true;
+    }
+    bool more(size_t n = 1) const { return p + n <= t.size(); }
+    long long i() { return std::stoll(t[p++]); }
+    double d() {
+        std::string s = t[p++];
+        for (char &ch : s)
+            if (ch == 'D' || ch == 'd') ch = 'E';  // Fortran exponent letter
+        return std::stod(s);
+    }
+};
+
+struct ElemFile {
+    int sanz = 0, typanz = 0, mb = 0, maxn = 0, elanz = 0, relanz = 0;
+    std::vector<int> typ, nelanz, selanz, nrel, rnr;
+    std::vector<double> x, y;
+};
+
+// src/relem.f90:60-151 and the border check of :197-229
+int read_elem(const std::string &path, ElemFile &g) {
+    Tok k;
+    if (!k.load(path)) return 1;
+    try {
+        g.sanz = (int)k.i();
+        g.typanz = (int)k.i();
+        g.mb = (int)k.i();
+        if (g.sanz <= 0 || g.typanz <= 0) return CRT_ERR_ARG;
+        for (int t = 0; t < g.typanz; ++t) {
+            g.typ.push_back((int)k.i());
+            g.nelanz.push_back((int)k.i());
+            g.selanz.push_back((int)k.i());
+            g.maxn = std::max(g.maxn, g.selanz.back());
+            if (g.typ.back() > 10) g.relanz += g.nelanz.back();
+            else g.elanz += g.nelanz.back();
+        }
+        std::vector<int> snr(g.sanz);
+        std::vector<double> sx(g.sanz), sy(g.sanz);
+        for (int i = 0; i < g.sanz; ++i) {
+            snr[i] = (int)k.i();
+            sx[i] = k.d();
+            sy[i] = k.d();
+        }
+        g.x.resize(g.sanz);
+        g.y.resize(g.sanz);
+        for (int i = 0; i < g.sanz; ++i) {
+            if (snr[i] < 1 || snr[i] > g.sanz) return CRT_ERR_ARG;
+            g.x[i] = sx[snr[i] - 1];
+            g.y[i] = sy[snr[i] - 1];
+        }
+        const int neltot = g.elanz + g.relanz;
+        g.nrel.assign((size_t)neltot * g.maxn, 0);
+        int iel = 0;
+        for (int t = 0; t < g.typanz; ++t)
+            for (int e = 0; e < g.nelanz[t]; ++e, ++iel)
+                for (int q = 0; q < g.selanz[t]; ++q) g.nrel[(size_t)iel * g.maxn + q] = (int)k.i();
+        for (int r = 0; r < g.relanz; ++r) g.rnr.push_back((int)k.i());
+    } catch (...) {
+        return CRT_ERR_ARG;
+    }
+    // every boundary element belongs to the area element (first type group) that holds
+    // both of its nodes
+    const int s0 = g.selanz[0];
+    std::vector<std::pair<long long, int>> owner;
+    owner.reserve((size_t)g.elanz * s0 * (s0 - 1) / 2);
+    for (int e = 0; e < g.elanz; ++e)
+        for (int a = 0; a < s0; ++a)
+            for (int b = a + 1; b < s0; ++b) {
+                long long n1 = g.nrel[(size_t)e * g.maxn + a], n2 = g.nrel[(size_t)e * g.maxn + b];
+                if (n1 > n2) std::swap(n1, n2);
+                owner.push_back({(n1 << 32) | n2, e + 1});
+            }
+    // the last element holding a pair wins, as in a forward scan that overwrites
+    std::stable_sort(owner.begin(), owner.end(),
+                     [](const std::pair<long long, int> &p, const std::pair<long long, int> &q) { return p.first < q.first; });
+    for (int r = 0; r < g.relanz; ++r) {
+        long long n1 = g.nrel[(size_t)(g.elanz + r) * g.maxn], n2 = g.nrel[(size_t)(g.elanz + r) * g.maxn + 1];
+        if (n1 > n2) std::swap(n1, n2);
+        const long long key = (n1 << 32) | n2;
+        auto it = std::upper_bound(owner.begin(), owner.end(), key,
+                                   [](long long v, const std::pair<long long, int> &p) { return v < p.first; });
+        if (it != owner.begin() && (it - 1)->first == key) g.rnr[r] = (it - 1)->second;
+    }
+    return 0;
+}
+
+int read_counted_ints(const std::string &path, int per, std::vector<long long> &v, int &n) {
+    Tok k;
+    if (!k.load(path)) return 1;
+    try {
+        n = (int)k.i();
+        if (n < 0) return CRT_ERR_ARG;
+        v.resize((size_t)n * per);
+        for (auto &x : v) x = k.i();
+    } catch (...) {
+        return CRT_ERR_ARG;
+    }
+    return 0;
+}
+
+int read_rho(const std::string &path, std::vector<double> &sigma, int &n) {
+    Tok k;
+    if (!k.load(path)) return 1;
+    try {
+        n = (int)k.i();
+        sigma.resize(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const double bet = k.d(), pha = k.d() / 1e3;
+            if (bet < 1e-12) return 11;  // src/rsigma.f90: errnr 11
+            sigma[2 * i] = std::cos(pha) / bet;
+            sigma[2 * i + 1] = -std::sin(pha) / bet;
+        }
+    } catch (...) {
+        return CRT_ERR_ARG;
+    }
+    return 0;
+}
+
+struct CrmodCfg {
+    std::string elem, elec, rho, config, pot, volt, sens;
+    bool wpot = false, wvolt = false, wsens = false;
+};
+
+std::string join(const std::string &dir, const std::string &p) {
+    if (p.empty() || p[0] == '/' || dir.empty()) return p;
+    return dir + "/" + p;
+}
+
+int read_crmod_cfg(const std::string &path, CrmodCfg &c) {
+    std::ifstream f(path);
+    if (!f) return 1;
+    std::vector<std::string> lines;
+    std::string l;
+    while (std::getline(f, l)) lines.push_back(l);
+    auto first = [](const std::string &s) {
+        std::istringstream is(s);
+        std::string w;
+        is >> w;
+        return w;
+    };
+    if (lines.size() < 11) return CRT_ERR_ARG;
+    const std::string dir = path.find('/') == std::string::npos ? "." : path.substr(0, path.rfind('/'));
+    c.elem = join(dir, first(lines[1]));
+    c.elec = join(dir, first(lines[2]));
+    c.rho = join(dir, first(lines[3]));
+    c.config = join(dir, first(lines[4]));
+    auto flag = [&](const std::string &s) { const std::string w = first(s); return !w.empty() && (w[0] == 'T' || w[0] == 't'); };
+    c.wpot = flag(lines[5]);
+    c.pot = join(dir, first(lines[6]));
+    c.wvolt = flag(lines[7]);
+    c.volt = join(dir, first(lines[8]));
+    c.wsens = flag(lines[9]);
+    c.sens = join(dir, first(lines[10]));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crt_tomodir_sizes(const char *cfg_path, int *sanz, int *mb, int *elanz, int *eanz, int *nanz) {
+    if (!cfg_path) return CRT_ERR_ARG;
+    CrmodCfg cfg;
+    int e = read_crmod_cfg(cfg_path, cfg);
+    if (e) return e;
+    ElemFile g;
+    if ((e = read_elem(cfg.elem, g))) return e;
+    std::vector<long long> el, cf;
+    int ne = 0, nc = 0;
+    if ((e = read_counted_ints(cfg.elec, 1, el, ne))) return e;
+    if ((e = read_counted_ints(cfg.config, 2, cf, nc))) return e;
+    if (sanz) *sanz = g.sanz;
+    if (mb) *mb = g.mb;
+    if (elanz) *elanz = g.elanz;
+    if (eanz) *eanz = ne;
+    if (nanz) *nanz = nc;
+    return 0;
+}
+
+int crt_write_volt(const char *path, int nanz, const int *strnr, const int *vnr, const double *sigmaa) {
+    if (!path || nanz < 0 || (nanz && (!strnr || !vnr || !sigmaa))) return CRT_ERR_ARG;
+    FILE *f = fopen(path, "w");
+    if (!f) return 1;
+    const double pi = std::acos(-1.0);
+    fprintf(f, "%12d\n", nanz);
+    for (int i = 0; i < nanz; ++i) {
+        const double re = sigmaa[2 * i], im = sigmaa[2 * i + 1];
+        const double bet = std::hypot(re, im);
+        double pha = std::atan2(im, re);
+        int v = vnr[i];
+        const double npi = std::round(pha / pi) * pi;
+        if (std::fabs(npi) > 1e-12) {
+            pha -= npi;
+            const int e1 = v % 10000, e2 = (v - e1) / 10000;
+            v = e1 * 10000 + e2;
+        }
+        // REAL*4 values, printed with enough digits to read back exactly
+        fprintf(f, "%12d %12d %16.8E %16.8E\n", strnr[i], v, (double)(float)bet, (double)(float)(1e3 * pha));
+    }
+    return fclose(f) == 0 ? 0 : 4;
+}
+
+int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path) {
+    if (!ctx || !cfg_path) return CRT_ERR_ARG;
+    CrmodCfg cfg;
+    int e = read_crmod_cfg(cfg_path, cfg);
+    if (e) return e;
+    ElemFile g;
+    if ((e = read_elem(cfg.elem, g))) return e;
+    std::vector<long long> el, cf;
+    int ne = 0, nc = 0, nr = 0;
+    if ((e = read_counted_ints(cfg.elec, 1, el, ne))) return e;
+    if ((e = read_counted_ints(cfg.config, 2, cf, nc))) return e;
+    std::vector<double> sigma;
+    if ((e = read_rho(cfg.rho, sigma, nr))) return e;
+    if (nr != g.elanz) return CRT_ERR_ARG;
+    std::vector<int> enr(el.begin(), el.end()), strnr(nc), vnr(nc);
+    for (int i = 0; i < nc; ++i) {
+        strnr[i] = (int)cf[2 * (size_t)i];
+        vnr[i] = (int)cf[2 * (size_t)i + 1];
+    }
+    if ((e = crt_set_grid(ctx, g.sanz, g.mb, g.typanz, g.typ.data(), g.nelanz.data(), g.selanz.data(), g.maxn,
+                          g.nrel.data(), g.x.data(), g.y.data(), g.rnr.data(), ne, enr.data())))
+        return e;
+    if ((e = crt_set_survey(ctx, nc, strnr.data(), vnr.data()))) return e;
+    int kwnanz = 0;
+    double amin = 0.0, amax = 0.0;
+    std::vector<double> kwn(64), kwnwi(64);
+    if ((e = crt_rwaven(ctx, &kwnanz, kwn.data(), kwnwi.data(), &amin, &amax))) return e;
+    int lbeta = 0;
+    if ((e = crt_precal(ctx, 0, &lbeta))) return e;
+    if ((e = crt_set_sigma(ctx, sigma.data()))) return e;
+    if ((e = crt_forward(ctx))) return e;
+    std::vector<double> volt(2 * (size_t)nc), sigmaa(2 * (size_t)nc);
+    if ((e = crt_voltages(ctx, 0, nullptr, volt.data(), sigmaa.data()))) return e;
+    if (cfg.wvolt && (e = crt_write_volt(cfg.volt.c_str(), nc, strnr.data(), vnr.data(), sigmaa.data()))) return e;
+    return 0;
+}
+
+}  // extern "C"

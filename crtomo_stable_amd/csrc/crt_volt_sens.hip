@@ -438,8 +438,8 @@ int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int max
     if (form == 2 && eanz <= 128) {
         const int nt = (eanz + 15) / 16, nut = nt * (nt + 1) / 2;
         const size_t gbytes = (size_t)nut * 256 * sizeof(zd);
-        // wavenumber chunk: Phi and T rows within max(G, 72 KB) of LDS
-        const size_t budget = std::max(gbytes, (size_t)72 * 1024);
+        // wavenumber chunk: Phi and T rows within max(G, 80 KB) of LDS (two workgroups per CU)
+        const size_t budget = std::max(gbytes, (size_t)80 * 1024);
         int KC = K;
         while (KC > 1 && (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd) > budget) --KC;
         const size_t lds = std::max(gbytes, (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd));

@@ -81,6 +81,22 @@ def bnachbar(grid):
     return nb
 
 
+def tomodir_sizes(cfg_path: str):
+    """(sanz, mb, elanz, eanz, nanz) of a CRMod tomodir, read by the library's host code."""
+    lib = load()
+    v = [C.c_int() for _ in range(5)]
+    _chk(lib.crt_tomodir_sizes(cfg_path.encode(), *[C.byref(x) for x in v]), "crt_tomodir_sizes")
+    return tuple(x.value for x in v)
+
+
+def write_volt(path: str, strnr, vnr, sigmaa):
+    """volt.dat as ``src/wdatm.f90:58-76`` writes it (library host code)."""
+    lib = load()
+    st = np.ascontiguousarray(strnr, np.int32); vn = np.ascontiguousarray(vnr, np.int32)
+    sa = np.ascontiguousarray(sigmaa, np.complex128)
+    _chk(lib.crt_write_volt(path.encode(), int(st.size), _ptr(st), _ptr(vn), _ptr(sa)), "crt_write_volt")
+
+
 class Context:
     """One device context holding grid, survey, element matrices, kpot and J."""
 
@@ -258,6 +274,11 @@ class Context:
         f = np.zeros((manz, 8), np.float32); t = np.zeros((manz, 5), np.float32); m = np.zeros((manz, 5), np.float32)
         _chk(self.lib.crt_pc_data(self._h, _ptr(f), _ptr(t), _ptr(m)), "crt_pc_data")
         return f, t, m
+
+    def tomodir_crmod(self, cfg_path: str):
+        """Run a CRMod tomodir from its crmod.cfg (grid, electrodes, rho, config read by
+        the library) and write volt.dat when the measurements switch is set."""
+        _chk(self.lib.crt_tomodir_crmod(self._h, cfg_path.encode()), "crt_tomodir_crmod")
 
     def set_cg_precision(self, jbits: int):
         """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""

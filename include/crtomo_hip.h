@@ -158,6 +158,20 @@ int crt_bnachbar(int elanz, int maxn, const int *nrel, int *nachbar);
 int crt_blam0(crt_ctx *ctx, int fpi, const double *wmatd, const int *wdfak, double alfx, double alfz,
               double *lammax);
 
+/* CRMod tomodir without the Fortran host: crmod.cfg (the ***FILES*** block: elem,
+ * elec, rho and config paths, then the potentials / measurements / sensitivities
+ * switches and output paths; relative paths from the directory of crmod.cfg),
+ * elem.dat (src/relem.f90:60-151 with the border check of :197-229), elec.dat
+ * (src/relectr.f90), rho.dat (src/rsigma.f90:168), config.dat (src/rdatm.f90);
+ * runs the forward and writes volt.dat when the measurements switch is T. */
+int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path);
+/* Sizes of the tomodir's grid, electrodes and configurations (host only). */
+int crt_tomodir_sizes(const char *cfg_path, int *sanz, int *mb, int *elanz, int *eanz, int *nanz);
+/* volt.dat as src/wdatm.f90:58-76 writes it: nanz, then strnr, vnr, REAL*4 |sigmaa|
+ * and 1000 * phase; a phase near +-pi is shifted by pi and swaps the potential
+ * electrodes of vnr.  Host only. */
+int crt_write_volt(const char *path, int nanz, const int *strnr, const int *vnr, const double *sigmaa);
+
 /* Cuthill-McKee renumbering (cutmckee/cutmck.f:126-337): perm[old-1] = new
  * number; returns the band width in *mb. */
 int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel,

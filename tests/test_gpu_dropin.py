@@ -65,3 +65,23 @@ def test_cholz_dropin(ctx, oracle_lib):
     # L L^T reproduces A (no conjugation)
     L = np.where(low, ag, 0) + np.diag(pg)
     assert np.max(np.abs(L @ L.T - A)) <= 1e-10 * np.max(np.abs(A))
+
+
+@pytest.mark.parametrize("name", ["minimal", "mixed_surf"])
+def test_tomodir_crmod_volt_dat(ctx, tmp_path, name):
+    """A CRMod tomodir run by the library alone (crmod.cfg, elem.dat, elec.dat, rho.dat,
+    config.dat read by its host code; forward on the device; volt.dat written as
+    src/wdatm.f90 writes it) against the reference's committed volt.dat: the same
+    configurations and polarity, magnitudes and phases within the file's REAL*4
+    precision."""
+    import os
+    from crtomo_stable_amd import grid as G
+    from test_lib_cpu import _tomodir
+    cfg, src = _tomodir(str(tmp_path), name)
+    print(f"[tomodir] {name}", flush=True)
+    ctx.tomodir_crmod(cfg)
+    s_out, m_out, p_out = G.read_volt(os.path.join(str(tmp_path), "mod", "volt.dat"))
+    s_ref, m_ref, p_ref = G.read_volt(os.path.join(src, "volt.dat"))
+    assert np.array_equal(s_out.strnr, s_ref.strnr) and np.array_equal(s_out.vnr, s_ref.vnr)
+    assert np.max(np.abs(m_out - m_ref) / m_ref) < 1e-6
+    assert np.max(np.abs(p_out - p_ref)) < 1e-3
