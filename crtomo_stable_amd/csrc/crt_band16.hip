@@ -30,6 +30,16 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int T16 = 16;
 constexpr int PS = 17;  // padded row stride (complex) of LDS panel tiles: conflict-free column reads
 
+// Workgroup barrier that orders LDS only: global loads stay in flight across it and
+// global stores drain in the background.  For phases whose hand-offs between waves
+// go through LDS alone (global data written in the kernel is not read back by
+// another wave before a full __syncthreads()).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ zd shfl_z16(zd v, int src) { return zd{__shfl(v.re, src, 64), __shfl(v.im, src, 64)}; }
 __device__ __forceinline__ zd shfl_xor_z16(zd v, int m) {
     return zd{__shfl_xor(v.re, m, 64), __shfl_xor(v.im, m, 64)};
@@ -698,7 +708,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         // ================================================================ S2
         if constexpr (isP) {
             const int Jn = J + 1;
@@ -790,7 +800,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 Bi[LB - 1] = eBi;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     if (isP && lane == 0 && err && A.status[s] == 0) A.status[s] = err;
 }
@@ -822,7 +832,8 @@ __global__ void __launch_bounds__(La4<D>::NT, 2) crt_factor16la_kernel(Factor16A
 //           the pivot wave takes C(0) and the factor's right-hand side, three update
 //           waves the off-diagonal tiles, whose operands L(J+a, J-m) stream from the
 //           panels with the next operand's load in flight during each product;
-//   factor  C(0) on the pivot wave (as in the look-ahead variant: L, L^-1, y);
+//   factor  C(0) on the pivot wave (as in the look-ahead variant: L, L^-1, y), as
+//           soon as its own products are done, while the update waves still run;
 //   solve   X(a) = C(a) (L^-1)^T on the matrix cores, stored as panel column J.
 // Four waves and at most 256 VGPRs per lane keep two systems on each CU, so one
 // system's serial diagonal factorisation overlaps the other's products.
@@ -846,6 +857,7 @@ struct alignas(16) StLds {
     zd col[48];
     zd pinv[T16], sdiag[T16];
     zd yj[T16];
+    zd yr[(D + 1) * T16];   // y of the last D+1 block rows (ring), for the right-hand side
 };
 
 // row block J of panel columns J-m into LDS (all SWV*64 threads, one element each per tile)
@@ -890,7 +902,6 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
     const double *fak = A.fak + (size_t)s * nblk * T16;
     const zd *vals = A.vals + (size_t)s * g.nnz;
     const zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
-    const zd *yout = A.y + (size_t)s * nblk * T16;
     int err = 0;
     int mi[4];
     d4 dr, di;
@@ -899,14 +910,14 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
     la4_map<D>(g, 1, 0, jc, rq, mi);
     for (int J = 0; J < nblk; ++J) {
         st_stage<D>(panel, m, J, tid);
-        __syncthreads();
+        lds_barrier();
         const int mx = J < D ? J : D;
         // right-hand side of block J: b_J - sum_m L(J, J-m) y_{J-m}
         const int i = J * T16 + jc;
         zd yv = (i < g.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
         for (int mm = 1; mm <= mx; ++mm) {
             const zd *lr = m.Lrow + (mm - 1) * T16 * PS + jc * PS;
-            const zd *yp = yout + (size_t)(J - mm) * T16;
+            const zd *yp = m.yr + ((J - mm) % (D + 1)) * T16;
 #pragma unroll
             for (int c = 0; c < T16; ++c) yv = zfms(yv, ldz(lr + c), ldz(yp + c));
         }
@@ -914,13 +925,14 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
             const zd *lr = m.Lrow + (mm - 1) * T16 * PS;
             mfma_update(dr, di, dr, di, lr, lr, jc, rq);
         }
-        __syncthreads();
+        // the diagonal needs C(0) only: it overlaps the update waves' products
         la4_factor_diag<D>(A, m, s, J, lane, dr, di, yv, err);
-        __syncthreads();
+        if (lane < T16) stz(m.yr + (J % (D + 1)) * T16 + lane, ldz(m.yj + lane));
+        lds_barrier();
         // diagonal tile of column J+1 (map loaded one column ahead)
         la4_vals(vals, mi, dr, di);
         la4_map<D>(g, J + 2, 0, jc, rq, mi);
-        __syncthreads();
+        __syncthreads();  // panel column J is read by the next stage
     }
     if (lane == 0 && err && A.status[s] == 0) A.status[s] = err;
 }
@@ -1008,7 +1020,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
     }
     for (int J = 0; J < nblk; ++J) {
         st_stage<D>(panel, m, J, tid);
-        __syncthreads();
+        lds_barrier();
         // C(a) -= sum_m L(J+a, J-m) L(J, J-m)^T
         if (J >= D && J + D < nblk) {
             // away from the ends: a fixed sequence of products, ST_PD operands in flight
@@ -1048,8 +1060,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
             }
         }
         }
-        __syncthreads();
-        __syncthreads();
+        lds_barrier();  // L^-1 of block J is in LDS
         // X(a) = C(a) (L^-1)^T, stored as panel column J
         zd *stg = m.Lrow + (W - 1) * T16 * PS;
 #pragma unroll
@@ -1075,7 +1086,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
                 la4_map<D>(g, J + 2 + a, a, jc, rq, mi[u]);
             }
         }
-        __syncthreads();
+        __syncthreads();  // panel column J is read by the next stage
     }
 }
 
