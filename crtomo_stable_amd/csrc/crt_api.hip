@@ -161,7 +161,7 @@ struct crt_ctx {
     DevGrid devgrid() const {
         DevGrid g;
         g.sanz = sanz; g.mb = mb; g.elanz = elanz; g.relanz = relanz; g.eanz = eanz; g.maxn = maxn;
-        g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D; g.nnz = nnz;
+        g.npk = npk; g.kwnanz = kwnanz; g.nanz = nanz; g.nblk = nblk; g.D = D; g.nnz = nnz; g.vstride = nnz + 2;
         g.fstride = nblk * CRT_NB;
         g.row_ptr = d_row_ptr.p; g.col = d_col.p; g.crow = d_crow.p; g.cptr = d_cptr.p; g.contrib = d_contrib.p;
         g.rnr = d_rnr.p; g.enr = d_enr.p; g.elbg = d_elbg.p; g.relbg = d_relbg.p; g.kg = d_kg.p;
@@ -431,13 +431,15 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     {
         const int D16 = std::max(1, (mb + 15) / 16), nb16 = (sanz + 15) / 16;
         if (D16 <= 16) {
-            std::vector<int> tmap((size_t)nb16 * (D16 + 1) * 256, -1);
+            const int nnz = (int)col.size();
+            std::vector<int> tmap((size_t)nb16 * (D16 + 1) * 256, nnz);
             for (int i = 0; i < sanz; ++i)
                 for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
                     const int j = col[q], I = i / 16, d = I - j / 16;
                     tmap[((size_t)(I * (D16 + 1) + d) * 16 + (i % 16)) * 16 + (j % 16)] = q;
                 }
-            for (int i = sanz; i < nb16 * 16; ++i) tmap[((size_t)((i / 16) * (D16 + 1)) * 16 + (i % 16)) * 16 + (i % 16)] = -2;
+            for (int i = sanz; i < nb16 * 16; ++i)
+                tmap[((size_t)((i / 16) * (D16 + 1)) * 16 + (i % 16)) * 16 + (i % 16)] = nnz + 1;
             if ((e = c->d_tmap.upload(tmap.data(), tmap.size(), s))) return e;
             c->tmapD = D16;
         }
@@ -626,7 +628,7 @@ int crt_forward(crt_ctx *c) {
     e = e ? e : c->d_dinv.ensure(batch * fstride);
     e = e ? e : c->d_fak.ensure(batch * fstride);
     e = e ? e : c->d_dga.ensure(batch * fstride);
-    e = e ? e : c->d_vals.ensure(batch * c->nnz);
+    e = e ? e : c->d_vals.ensure(batch * ((size_t)c->nnz + 2));
     e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
     e = e ? e : c->d_sys.ensure(sys.size());
     e = e ? e : c->d_rhs.ensure(rhs.size());

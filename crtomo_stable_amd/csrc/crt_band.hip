@@ -87,8 +87,13 @@ __global__ void __launch_bounds__(256) crt_assemble_kernel(DevGrid g, int nsys, 
                                                            const double *__restrict__ fak,
                                                            const double *__restrict__ dga, zd *__restrict__ vals) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nsys * g.nnz) return;
-    int s = (int)(t / g.nnz), q = (int)(t - (long)s * g.nnz);
+    if (t >= (long)nsys * g.vstride) return;
+    int s = (int)(t / g.vstride), q = (int)(t - (long)s * g.vstride);
+    if (q >= g.nnz) {
+        // the two constants after the nonzeros (zero entries and padding diagonals)
+        vals[t] = zd{q == g.nnz ? 0.0 : 1.0, 0.0};
+        return;
+    }
     int2 lk = sys[s];
     int i = g.crow[q], j = g.col[q];
     zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
@@ -127,7 +132,7 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
     const int l = A.sys[s].x, k = A.sys[s].y;
     const int enode = A.band ? -1 : g.enr[l];
     const double *fak = A.band ? nullptr : A.fak + (size_t)s * nblk * NB;
-    const zd *vals = A.band ? nullptr : A.vals + (size_t)s * g.nnz;
+    const zd *vals = A.band ? nullptr : A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * NB2;
     zd *yout = A.y + (size_t)s * nblk * NB;
 
@@ -476,7 +481,7 @@ int crt_launch_diag(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys,
 
 int crt_launch_assemble(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,
                         const double *dga, zd *vals) {
-    long n = (long)nsys * g.nnz;
+    long n = (long)nsys * g.vstride;
     hipLaunchKernelGGL(crt_assemble_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, sys, fak,
                        dga, vals);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;

@@ -211,7 +211,7 @@ __device__ __forceinline__ void factor16_wave(const Factor16Args &A, F16Lds<D> &
     const int lane = threadIdx.x & 63, tid = WV * 64 + lane;
     const int enode = g.enr[A.sys[s].x];
     const double *fak = A.fak + (size_t)s * nblk * T16;
-    const zd *vals = A.vals + (size_t)s * g.nnz;
+    const zd *vals = A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     zd *yout = A.y + (size_t)s * nblk * T16;
     const int jc = lane & 15, rq = lane >> 4;
@@ -506,15 +506,15 @@ template <int D>
 __device__ __forceinline__ void la4_map(const DevGrid &g, int I, int d, int jc, int rq, int (&mi)[4]) {
     const int *mp = g.tmap + ((size_t)(I * (D + 1) + d) * T16) * T16 + jc;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) mi[v] = I < g.nblk ? mp[(4 * v + rq) * T16] : -1;
+    for (int v = 0; v < 4; ++v) mi[v] = I < g.nblk ? mp[(4 * v + rq) * T16] : g.nnz;
 }
 
 // tile values in the accumulator layout from the map entries
 __device__ __forceinline__ void la4_vals(const zd *vals, const int (&mi)[4], d4 &tr, d4 &ti) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int q = mi[v];
-        const zd e = q >= 0 ? ldz(vals + q) : zd{q == -2 ? 1.0 : 0.0, 0.0};
+        // plain loads (zero and unit entries are values too): nothing waits on them here
+        const zd e = ldz(vals + mi[v]);
         tr[v] = e.re;
         ti[v] = e.im;
     }
@@ -624,7 +624,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
     const int lane = threadIdx.x & 63, tid = WV * 64 + lane;
     const int enode = g.enr[A.sys[s].x];
     const double *fak = A.fak + (size_t)s * nblk * T16;
-    const zd *vals = A.vals + (size_t)s * g.nnz;
+    const zd *vals = A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     const int jc = lane & 15, rq = lane >> 4;
 
@@ -900,7 +900,7 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
     const int jc = lane & 15, rq = lane >> 4;
     const int enode = g.enr[A.sys[s].x];
     const double *fak = A.fak + (size_t)s * nblk * T16;
-    const zd *vals = A.vals + (size_t)s * g.nnz;
+    const zd *vals = A.vals + (size_t)s * g.vstride;
     const zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     int err = 0;
     int mi[4];
@@ -1005,7 +1005,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
     const DevGrid &g = A.g;
     const int nblk = g.nblk, s = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int jc = lane & 15, rq = lane >> 4;
-    const zd *vals = A.vals + (size_t)s * g.nnz;
+    const zd *vals = A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     d4 cr[NA], ci[NA];
     int mi[NA][4];

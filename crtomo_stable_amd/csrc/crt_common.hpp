@@ -103,6 +103,7 @@ struct DevGrid {
     int nblk, D;              // row blocks and sub-diagonal block count
     int nnz;                  // lower-triangle nonzeros
     int fstride;              // per-system stride of fak / dga / y arrays (rows padded to a block)
+    int vstride;              // per-system stride of the CSR values: nnz entries, then 0 and 1
     // CSR of the lower triangle of A in node order (0-based), diagonal last
     const int *row_ptr;       // sanz+1
     const int *col;           // nnz
@@ -116,9 +117,10 @@ struct DevGrid {
     const double *kg;         // [k][l][rel]
     const zd *sigma;          // elanz
     // 16x16 tile map of the lower band: for block row I and block diagonal d
-    // (0..D16), tmap[((I*(D16+1) + d)*16 + r)*16 + c] is the CSR index of entry
-    // (16I + r, 16(I-d) + c), -1 if that entry is zero, -2 for the unit diagonal
-    // of padding rows past sanz
+    // (0..D16), tmap[((I*(D16+1) + d)*16 + r)*16 + c] is the index into a system's
+    // values of entry (16I + r, 16(I-d) + c): its CSR index, nnz if the entry is zero,
+    // nnz+1 (the value 1) for the unit diagonal of padding rows past sanz, so every
+    // tile element is a plain load
     const int *tmap;
     int tmapD;                // D16 the map was built for (0 if none)
 };
