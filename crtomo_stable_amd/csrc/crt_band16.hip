@@ -469,6 +469,7 @@ struct La4 {
 template <int D>
 struct alignas(16) La4Lds {
     zd P[D * T16 * PS];       // X(J+d, J), d = 1..D
+    zd A1[T16 * PS];          // A(J+1, J) from wave 1, then X(J+1, J) on the pivot wave
     zd Li[T16 * PS];          // L(J,J)^-1, lower, row-major
     zd HO[La4<D>::NU][256];   // hand-off tiles: HO[u-1] from wave u to wave u-1 (u = 1: to wave 0)
     zd bring[(D + 2) * T16];
@@ -520,67 +521,84 @@ __device__ __forceinline__ void la4_vals(const zd *vals, const int (&mi)[4], d4 
     }
 }
 
-// diagonal factorisation of block Jn on the pivot wave
-template <int D, class LDS>
-__device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, LDS &m, int s, int Jn, int lane,
-                                                const d4 &tr, const d4 &ti, zd yv, int &err) {
-    const int nblk = A.g.nblk;
+// diagonal factorisation of block Jn on the pivot wave, in three parts so the
+// column loop can be split across a barrier: init, columns [C0, C1), finish
+struct DiagChain {
+    zd a[4], z[4], yv, p, r, pk;
+};
+
+__device__ __forceinline__ void diag_init(DiagChain &q, int lane, const d4 &tr, const d4 &ti, zd yv, int &err) {
     const int jc = lane & 15, rq = lane >> 4;
-    zd a[4], z[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        a[v] = zd{tr[v], ti[v]};
-        z[v] = zd{4 * v + rq == jc ? 1.0 : 0.0, 0.0};
+        q.a[v] = zd{tr[v], ti[v]};
+        q.z[v] = zd{4 * v + rq == jc ? 1.0 : 0.0, 0.0};
     }
-    zd p = readlane_z(a[0], 0);
+    q.yv = yv;
+    q.p = readlane_z(q.a[0], 0);
     bool bad;
-    zd r = crecip(p, bad);
+    q.r = crecip(q.p, bad);
     if (bad) err = CRT_ERR_PIVOT;
-    zd pk = zsel(lane == 0, p, zd{0.0, 0.0});
+    q.pk = zsel(lane == 0, q.p, zd{0.0, 0.0});
+}
+
+template <int C0, int C1, class LDS>
+__device__ __forceinline__ void diag_cols(DiagChain &q, LDS &m, int lane, int &err) {
+    const int jc = lane & 15, rq = lane >> 4;
 #pragma unroll
-    for (int c = 0; c < T16; ++c) {
+    for (int c = C0; c < C1; ++c) {
         if (jc == c) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                stz(m.col + 4 * v + rq, a[v]);
-                stz(m.col + 16 + 4 * v + rq, z[v]);
+                stz(m.col + 4 * v + rq, q.a[v]);
+                stz(m.col + 16 + 4 * v + rq, q.z[v]);
             }
-            if (rq == 0) stz(m.col + 32, yv);
+            if (rq == 0) stz(m.col + 32, q.yv);
         }
         zd dn, l2;
         if (c + 1 < T16) {
             const int n = c + 1;
-            dn = readlane_z(a[n / 4], n + 16 * (n % 4));
-            const zd ln = readlane_z(a[n / 4], c + 16 * (n % 4));
+            dn = readlane_z(q.a[n / 4], n + 16 * (n % 4));
+            const zd ln = readlane_z(q.a[n / 4], c + 16 * (n % 4));
             l2 = zmul(ln, ln);
         }
         __builtin_amdgcn_wave_barrier();
+        // Column c changes a(i, jc) and z(i, jc) for jc > c only. Rows i <= c of a are
+        // never read again (the factor keeps i > jc), and z(i, c) is zero for i > c, so
+        // element v of a is skipped once 4v+3 <= c and element v of z while 4v > c.
         zd ci[4], zi[4];
         const zd cj = ldz(m.col + jc), wc = ldz(m.col + 32);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            ci[v] = ldz(m.col + 4 * v + rq);
-            zi[v] = ldz(m.col + 16 + 4 * v + rq);
+            if (4 * v + 3 > c) ci[v] = ldz(m.col + 4 * v + rq);
+            if (4 * v <= c) zi[v] = ldz(m.col + 16 + 4 * v + rq);
         }
         __builtin_amdgcn_sched_barrier(0);
-        const zd t = zsel(jc > c, zmul(cj, r), zd{0.0, 0.0});  // u(jc, c)
+        const zd t = zsel(jc > c, zmul(cj, q.r), zd{0.0, 0.0});  // u(jc, c)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            a[v] = zfms(a[v], ci[v], t);
-            z[v] = zfms(z[v], zi[v], t);
+            if (4 * v + 3 > c) q.a[v] = zfms(q.a[v], ci[v], t);
+            if (4 * v <= c) q.z[v] = zfms(q.z[v], zi[v], t);
         }
-        yv = zfms(yv, wc, t);
+        q.yv = zfms(q.yv, wc, t);
         if (c + 1 < T16) {
-            p = zfms(dn, l2, r);
-            r = crecip(p, bad);
+            bool bad;
+            q.p = zfms(dn, l2, q.r);
+            q.r = crecip(q.p, bad);
             if (bad) err = CRT_ERR_PIVOT;
-            pk = zsel(lane == c + 1, p, pk);
+            q.pk = zsel(lane == c + 1, q.p, q.pk);
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+template <int D, class LDS>
+__device__ __forceinline__ void diag_finish(const Factor16Args &A, LDS &m, int s, int Jn, int lane, DiagChain &q) {
+    const int nblk = A.g.nblk;
+    const int jc = lane & 15, rq = lane >> 4;
     zd sc, ivc;
     bool badc;
-    pivot_fast(pk, sc, ivc, badc);
+    pivot_fast(q.pk, sc, ivc, badc);
     if (lane < T16) {
         m.pinv[lane] = ivc;
         m.sdiag[lane] = sc;
@@ -592,15 +610,28 @@ __device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, LDS &m, i
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = 4 * v + rq;
-        stz(pg + i * T16 + jc, i > jc ? zmul(a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0}));
-        stz(m.Li + jc * PS + i, zmul(z[v], ivj));  // L^-1(jc, i) = (L^-T)(i, jc)
+        stz(pg + i * T16 + jc, i > jc ? zmul(q.a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0}));
+        stz(m.Li + jc * PS + i, zmul(q.z[v], ivj));  // L^-1(jc, i) = (L^-T)(i, jc)
     }
     if (rq == 0) {
-        const zd y = zmul(yv, ivj);
+        const zd y = zmul(q.yv, ivj);
         m.yj[jc] = y;
         A.y[(size_t)s * nblk * T16 + (size_t)Jn * T16 + jc] = y;
     }
 }
+
+template <int D, class LDS>
+__device__ __forceinline__ void la4_factor_diag(const Factor16Args &A, LDS &m, int s, int Jn, int lane,
+                                                const d4 &tr, const d4 &ti, zd yv, int &err) {
+    DiagChain q;
+    diag_init(q, lane, tr, ti, yv, err);
+    diag_cols<0, T16>(q, m, lane, err);
+    diag_finish<D>(A, m, s, Jn, lane, q);
+}
+
+// columns of the next diagonal block the pivot wave factorises before the first
+// barrier of a step (the rest after it)
+constexpr int LA4_SPLIT = 4;
 
 template <int D, int WV>
 __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
@@ -636,6 +667,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
 #pragma unroll
     for (int q = 0; q < NZX; ++q) Zr[q] = Zi[q] = d4{0.0, 0.0, 0.0, 0.0};
     d4 Tr = d4{0.0, 0.0, 0.0, 0.0}, Ti = Tr;  // pivot wave: tile (J+1, J+1)
+    DiagChain chain;                           // pivot wave: its factorisation across S1 and S2
     int err = 0;
     // map entries of the entering tiles, loaded one block row ahead
     int mA[4], mB[4], mZ[4];
@@ -661,6 +693,32 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 Tr[v] = e.re;
                 Ti[v] = e.im;
             }
+            const int Jn = J + 1;
+            if (Jn >= 0 && Jn < nblk) {
+                zd *brow = m.bring + (Jn % (D + 2)) * T16;
+                if (J >= 0) {
+                    // X(J+1, J) = A(J+1, J) (L^-1)^T from the tile wave 1 left in A1 (the same
+                    // product wave 1 forms for the panel), then b(J+1) -= X y_J and the
+                    // rank-16 update of the diagonal tile
+                    d4 xr, xi;
+                    mfma_mul(xr, xi, m.A1, m.Li, jc, rq);
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) stz(m.A1 + (4 * v + rq) * PS + jc, zd{xr[v], xi[v]});
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < T16) {
+                        const zd *row = m.A1 + lane * PS;
+                        zd acc = ldz(brow + lane);
+#pragma unroll
+                        for (int c = 0; c < T16; ++c) acc = zfms(acc, ldz(row + c), ldz(m.yj + c));
+                        stz(brow + lane, acc);
+                    }
+                    mfma_update(Tr, Ti, Tr, Ti, m.A1, m.A1, jc, rq);
+                    __builtin_amdgcn_wave_barrier();
+                }
+                diag_init(chain, lane, Tr, Ti, ldz(brow + jc), err);
+                diag_cols<0, LA4_SPLIT>(chain, m, lane, err);
+            }
         } else {
             if (J >= 0) {
                 // X = A (L^-1)^T for the panel tiles at position 0, then b(J+d) -= X y_J
@@ -680,15 +738,25 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 if constexpr (LB > 0)
                     if (liveB) mfma_mul(xbr, xbi, pb, m.Li, jc, rq);
                 __builtin_amdgcn_wave_barrier();
+                // panel tiles X(J+d, J) to LDS for the updates and straight to the panel in
+                // global memory (rows 4v..4v+3 of a tile are 64 consecutive entries)
+                zd *pj = panel + (size_t)J * (D + 1) * 256 + rq * T16 + jc;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    stz(pa + (4 * v + rq) * PS + jc, liveA ? zd{xar[v], xai[v]} : zd{0.0, 0.0});
-                    if constexpr (LB > 0) stz(pb + (4 * v + rq) * PS + jc, liveB ? zd{xbr[v], xbi[v]} : zd{0.0, 0.0});
+                    const zd xa = liveA ? zd{xar[v], xai[v]} : zd{0.0, 0.0};
+                    stz(pa + (4 * v + rq) * PS + jc, xa);
+                    stz(pj + dA * 256 + 64 * v, xa);
+                    if constexpr (LB > 0) {
+                        const zd xb = liveB ? zd{xbr[v], xbi[v]} : zd{0.0, 0.0};
+                        stz(pb + (4 * v + rq) * PS + jc, xb);
+                        stz(pj + dB * 256 + 64 * v, xb);
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
                 const int half = lane >> 4, rr = lane & 15;
                 const int d = half == 0 ? dA : dB;
-                if ((half == 0 && liveA) || (LB > 0 && half == 1 && liveB)) {
+                // b(J+1) is left to the pivot wave, which needs it first
+                if ((half == 0 && liveA && dA != 1) || (LB > 0 && half == 1 && liveB)) {
                     const zd *row = la4_tile<D>(m, d) + rr * PS;
                     zd *bp = m.bring + ((J + d) % (D + 2)) * T16 + rr;
                     zd acc = ldz(bp);
@@ -713,11 +781,8 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
         if constexpr (isP) {
             const int Jn = J + 1;
             if (Jn >= 0 && Jn < nblk) {
-                if (J >= 0) {
-                    const zd *x1 = la4_tile<D>(m, 1);
-                    mfma_update(Tr, Ti, Tr, Ti, x1, x1, jc, rq);
-                }
-                la4_factor_diag<D>(A, m, s, Jn, lane, Tr, Ti, ldz(m.bring + (Jn % (D + 2)) * T16 + jc), err);
+                diag_cols<LA4_SPLIT, T16>(chain, m, lane, err);
+                diag_finish<D>(A, m, s, Jn, lane, chain);
             }
         } else {
             // entering tiles of block row Ie (positions D-d at the next step)
@@ -728,24 +793,6 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
             la4_map<D>(g, Ie + 1, dA, jc, rq, mA);
             if constexpr (LB > 0) la4_map<D>(g, Ie + 1, dB, jc, rq, mB);
             if constexpr (topZ) la4_map<D>(g, Ie + 1, 0, jc, rq, mZ);
-            if (J >= 0) {
-                // store the panel X(J+d, J), d = 1..D (the diagonal tile is stored by wave 0)
-                zd *pj = panel + (size_t)J * (D + 1) * 256 + 256;
-                constexpr int NPS = (D * 256 + NT - 65) / (NT - 64);
-                zd pv[NPS];
-#pragma unroll
-                for (int u = 0; u < NPS; ++u) {
-                    const int t = (tid - 64) + u * (NT - 64);
-                    const int d = 1 + (t >> 8), rc = t & 255;
-                    pv[u] = (t < D * 256 && J + d < nblk) ? ldz(la4_tile<D>(m, d) + (rc >> 4) * PS + (rc & 15))
-                                                           : zd{0.0, 0.0};
-                }
-#pragma unroll
-                for (int u = 0; u < NPS; ++u) {
-                    const int t = (tid - 64) + u * (NT - 64);
-                    if (t < D * 256) stz(pj + t, pv[u]);
-                }
-            }
             // diagonal 0: slot q holds position ZLO+q; position ZLO goes down to wave WV-1
             {
                 zd *h = m.HO[WV - 1];
@@ -798,6 +845,11 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
                 }
                 Br[LB - 1] = eBr;
                 Bi[LB - 1] = eBi;
+            }
+            if constexpr (dA == 1) {
+                // A(J+2, J+1) is final now: hand it to the pivot wave for the next step
+#pragma unroll
+                for (int v = 0; v < 4; ++v) stz(m.A1 + (4 * v + rq) * PS + jc, zd{Ar[0][v], Ai[0][v]});
             }
         }
         lds_barrier();
