@@ -190,12 +190,107 @@ def gn_step(ctx, w, ltri: int):
             "J_bytes": j_bytes, "cg_forms": forms}
 
 
+def bench_c2(args):
+    """Batched forward (the C2 workload): ``args.models`` seeded models of 5k..40k
+    cells with 48..96 electrodes each, every (electrode, wavenumber) system of all
+    models assembled, factorised and solved by one graph launch per step."""
+    import numpy as np
+    from crtomo_stable_amd import configs, lib
+
+    t0 = time.perf_counter()
+    ctxs, works, flop, nsys = [], [], 0.0, 0
+    for i in range(args.models):
+        w = configs.c2_model(i)
+        c = lib.Context(0)
+        c.set_grid(w.grid)
+        c.set_survey(w.survey)
+        kwn, _, _, _ = c.rwaven()
+        lbeta = c.precal()
+        c.set_sigma(w.sigma)
+        n = kwn.size * (w.grid.eanz if lbeta else 1)
+        flop += n * 8.0 * w.grid.sanz * w.grid.mb * (w.grid.mb + 1) / 2.0
+        nsys += n
+        ctxs.append(c)
+        works.append(w)
+        if (i + 1) % 32 == 0:
+            print(f"[c2] {i + 1} models set up ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    b = lib.Batch(ctxs, autotune=True)
+    for _ in range(args.warmup):
+        b.forward()
+    device_sync()
+    dev_ms = 0.0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        b.forward()
+        dev_ms += b.info()[0]
+    device_sync()
+    elapsed = time.perf_counter() - t1
+    ms_step = 1e3 * elapsed / args.steps
+    dev_ms /= args.steps
+    _, n_launch, variant, tune = b.info()
+    assert n_launch == nsys
+    mbs = np.array([w.grid.mb for w in works])
+    cells = np.array([w.grid.elanz for w in works])
+    achieved = flop / (dev_ms * 1e-3) / 1e12
+    out = {
+        "metric": "forward solves/s",
+        "value": round(nsys * 1e3 / ms_step, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"c2: {args.models} seeded quad models, {cells.min()}..{cells.max()} cells, mb "
+                        f"{mbs.min()}..{mbs.max()}, every (electrode, wavenumber) system of all models in one "
+                        f"graph launch (FP64 band Cholesky)",
+            "models": args.models, "systems_per_step": nsys, "parallelism": "1 GPU",
+        },
+        "graph_ms": round(dev_ms, 3),
+        "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                         frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
+                         kernel="whole graph (band-Cholesky FLOP over the graph's device time, so a lower "
+                                "bound of the factor kernels' rate)",
+                         flop_per_step=flop),
+        "factor_variant_per_D": {int(D): int(variant[D]) for D in range(17) if variant[D]},
+        "tune_ms": {int(D): [round(float(x), 3) for x in tune[D]] for D in range(17) if tune[D].any()},
+        "setup_s": round(t1 - t0, 1),
+    }
+    if not args.no_cpu_baseline:
+        try:
+            w0 = works[0]
+            cb = cpu_baseline(w0, args.cpu_threads, 1, args.cpu_fwd_electrodes)
+            # per-system time of model 0, scaled to every system by sanz mb^2
+            n0 = ctxs[0].kwnanz * w0.grid.eanz
+            t_sys = cb["forward_s"] / n0
+            work0 = w0.grid.sanz * w0.grid.mb ** 2
+            t_all = sum(t_sys * (c.kwnanz * w.grid.eanz) * (w.grid.sanz * w.grid.mb ** 2) / work0
+                        for c, w in zip(ctxs, works))
+            out["cpu_baseline"] = dict(value=round(nsys / t_all, 4), unit="solves/s", cores=cb["cores"],
+                                       kind=cb["kind"],
+                                       sample=cb["sample"] + f"; model 0 only (sanz {w0.grid.sanz}, mb "
+                                              f"{w0.grid.mb}), its forward time per system scaled to all "
+                                              f"{nsys} systems by sanz*mb^2")
+        except Exception as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c3"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3"])
+    ap.add_argument("--models", type=int, default=256, help="models of the c2 batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB three times (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -225,6 +320,8 @@ def main():
         from crtomo_stable_amd import build
         build.build()
 
+    if args.workload == "c2":
+        return bench_c2(args)
     w = {"c0": configs.c0, "c1": configs.c1, "c3": configs.c3}[args.workload]()
     g, s = w.grid, w.survey
     ctx = lib.Context(0)

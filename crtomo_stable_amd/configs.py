@@ -7,6 +7,9 @@
   triangles over 100x30 m with 2 m sinusoidal topography, 64 electrodes at
   1.5 m, 2000 dipole-dipole + Wenner quadrupoles, Gaussian-random-field
   magnitude 10..1000 ohm-m (5 m correlation) and phase -1..-30 mrad.
+* ``c2_model(i)``: model i of the batched set, a seeded quad mesh of 5k..40k
+  cells at aspect 4:1 with 48..96 surface electrodes and a Gaussian-random-field
+  model (2..20 m correlation, 10..1000 ohm-m, -1..-30 mrad).
 * ``c3``: 400x100 quads, 96 electrodes at 2 m, 4500 dipole-dipole/Wenner
   quadrupoles, three blocks in a layered 50/200 ohm-m background.
 * ``small``: reduced versions of the same generators for parity tests.
@@ -83,6 +86,31 @@ def c1(cutmck=None, scale: float = 1.0, nmeas: int = 2000, nelec: int = 64, seed
     g = _cutmck(g, cutmck)
     return Workload("c1", g, G.encode(quads), sigma,
                     notes=f"{g.elanz} triangles, {g.sanz} nodes, mb={g.mb}")
+
+
+def c2_model(i: int, cutmck=None, seed: int = 2, min_cells: int = 5000, max_cells: int = 40000) -> Workload:
+    """Model ``i`` of the batched forward set: cell count uniform in
+    [min_cells, max_cells] at nx = 4 nz (0.5 m cells), 48..96 electrodes spread over
+    the surface, dipole-dipole a = 1..2, n = 1..8, magnitude 10..1000 ohm-m and
+    phase -1..-30 mrad as Gaussian random fields of one correlation length in
+    2..20 m.  Everything follows from (seed, i)."""
+    rng = np.random.default_rng([seed, i])
+    cells = rng.uniform(min_cells, max_cells)
+    nz = max(4, int(round(np.sqrt(cells / 4.0))))
+    nx = 4 * nz
+    nelec = int(min(rng.integers(48, 97), nx // 2))
+    spacing_nodes = max(1, (nx - 4) // (nelec - 1))
+    first = (nx - spacing_nodes * (nelec - 1)) // 2
+    corr = float(rng.uniform(2.0, 20.0))
+    fseed = int(rng.integers(1 << 30))
+
+    def sig(cx, cy):
+        f = G.uniformize(G.gaussian_random_field(cx, cy, corr, seed=fseed))
+        fp = G.uniformize(G.gaussian_random_field(cx, cy, corr, seed=fseed + 1))
+        return G.sigma_from_rho(10.0 ** (1.0 + 2.0 * f), -1.0 - 29.0 * fp)
+
+    quads = G.dipole_dipole(nelec, [1, 2], range(1, 9))
+    return quad_workload(f"c2[{i}]", nx, nz, 0.5, 0.5, nelec, spacing_nodes, first, quads, sig, cutmck)
 
 
 def c3(cutmck=None, nx: int = 400, nz: int = 100, nelec: int = 96, nmeas: int = 4500, seed: int = 3) -> Workload:

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -24,7 +25,7 @@ int crt_launch_panel_band(hipStream_t, int, int, int, int, int, zd *, zd *);
 int crt_launch_factor_band(hipStream_t, const DevGrid &, const int2 *, const zd *, zd *, zd *, int *);
 int crt_factor16_supported(int mb);
 int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zd *, zd *,
-                        zd *, int *);
+                        zd *, int *, int);
 int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                        const zd *, const zd *, const zd *, zd *, zd *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
@@ -273,6 +274,175 @@ double bsytop_h(const crt_ctx *c) {
         }
     }
     return sytop;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batches
+// Forward of many models (contexts) as one graph: every model's systems are
+// assembled, factorised and solved in chunks through one shared workspace, all
+// launches captured once into a hipGraph and replayed by crt_batch_forward.
+struct crt_batch {
+    struct Chunk {
+        size_t s0;  // first system
+        int nb;     // systems
+        size_t r0;  // first right-hand side in the concatenated per-chunk lists
+        int nr;     // right-hand sides
+    };
+    struct Model {
+        crt_ctx *c = nullptr;
+        bool use16 = true;
+        int nblk = 0, D = 0, NBt = 16, variant = 0;
+        size_t fstride = 0, panel_elems = 0, nsys = 0, nrhs = 0, batch = 0, nyscr = 0;
+        std::vector<Chunk> chunks;
+        DBuf<int2> sys;
+        DBuf<int4> rhs;
+        DBuf<int> status;  // 1 + nsys
+    };
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<std::unique_ptr<Model>> m;
+    DBuf<zd> panels, yf, dinv, vals, yscr;
+    DBuf<double> fak, dga;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float ms = 0.f;
+    int variant[17] = {0};  // factor kernel variant per band width in tiles (0: default)
+    float tune_ms[17][4] = {};
+    size_t nsys_total = 0;
+    ~crt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// systems and right-hand sides of one model as crt_forward orders them, split into
+// chunks of at most `cap` systems
+int batch_plan(crt_batch::Model &M, size_t ws_bytes) {
+    crt_ctx *c = M.c;
+    M.use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
+    M.NBt = M.use16 ? 16 : CRT_NB;
+    const int K = c->kwnanz, E = c->eanz;
+    M.nblk = (c->sanz + M.NBt - 1) / M.NBt;
+    M.D = std::max(1, (c->mb + M.NBt - 1) / M.NBt);
+    if (!M.use16 && crt_factor_lds_bytes(M.D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    std::vector<int2> sys;
+    std::vector<int4> rhs;
+    for (int k = 0; k < K; ++k) {
+        if (c->lbeta) {
+            for (int l = 0; l < E; ++l) {
+                rhs.push_back(int4{l, k, (int)sys.size(), 1});
+                sys.push_back(int2{l, k});
+            }
+        } else {
+            int fs = (int)sys.size();
+            sys.push_back(int2{0, k});
+            for (int l = 0; l < E; ++l) rhs.push_back(int4{l, k, fs, l == 0 ? 1 : 0});
+        }
+    }
+    M.nsys = sys.size();
+    M.nrhs = rhs.size();
+    M.fstride = (size_t)M.nblk * M.NBt;
+    M.panel_elems = (size_t)M.nblk * (M.D + 1) * M.NBt * M.NBt;
+    const size_t per_sys = M.panel_elems * sizeof(zd) + M.fstride * (2 * sizeof(zd) + 2 * sizeof(double)) +
+                           ((size_t)c->nnz + 2) * sizeof(zd);
+    M.batch = std::max<size_t>(1, std::min(M.nsys, ws_bytes / per_sys));
+    M.nyscr = c->lbeta ? 1 : M.nrhs;
+    std::vector<int4> rall;
+    M.chunks.clear();
+    for (size_t s0 = 0; s0 < M.nsys; s0 += M.batch) {
+        const int nb = (int)std::min(M.batch, M.nsys - s0);
+        crt_batch::Chunk ch{s0, nb, rall.size(), 0};
+        for (auto q : rhs)
+            if ((size_t)q.z >= s0 && (size_t)q.z < s0 + nb) rall.push_back(int4{q.x, q.y, q.z - (int)s0, q.w});
+        ch.nr = (int)(rall.size() - ch.r0);
+        M.chunks.push_back(ch);
+    }
+    int e = M.sys.upload(sys.data(), sys.size(), c->stream);
+    e = e ? e : M.rhs.upload(rall.data(), rall.size(), c->stream);
+    e = e ? e : M.status.ensure(M.nsys + 1);
+    e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
+    if (e) return e;
+    return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+DevGrid batch_grid(const crt_batch::Model &M) {
+    DevGrid g = M.c->devgrid();
+    g.nblk = M.nblk;
+    g.D = M.D;
+    g.fstride = (int)M.fstride;
+    return g;
+}
+
+// diagonal scaling and assembly of one chunk into the shared workspace
+int batch_assemble(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
+    const DevGrid g = batch_grid(M);
+    int e = crt_launch_diag(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, M.status.p + 1 + ch.s0);
+    return e ? e : crt_launch_assemble(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, b->vals.p);
+}
+
+int batch_factor(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch, int variant) {
+    const DevGrid g = batch_grid(M);
+    if (M.use16)
+        return crt_launch_factor16(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->vals.p, b->panels.p, b->yf.p,
+                                   b->dinv.p, M.status.p + 1 + ch.s0, variant);
+    return crt_launch_factor(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, b->vals.p, b->panels.p,
+                             b->yf.p, M.status.p + 1 + ch.s0);
+}
+
+int batch_solve(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
+    crt_ctx *c = M.c;
+    const int K = c->kwnanz, E = c->eanz;
+    if (M.use16)
+        return crt_launch_solve16(b->stream, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, b->fak.p,
+                                  b->panels.p, b->dinv.p, b->yf.p, b->yscr.p, c->d_kpotN.p);
+    return crt_launch_solve(b->stream, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, b->fak.p,
+                            b->panels.p, b->yf.p, b->yscr.p, c->d_kpotN.p, nullptr, nullptr);
+}
+
+// time each factor kernel variant that exists for band width D on the first chunk
+// of the first model of that width; keep the fastest
+int batch_tune(crt_batch *b) {
+    for (int D = 1; D <= 16; ++D) {
+        crt_batch::Model *M = nullptr;
+        for (auto &q : b->m)
+            if (q->use16 && q->D == D) { M = q.get(); break; }
+        if (!M) continue;
+        int cand[3], nc = 0;
+        if (D >= 2 && D <= 10) cand[nc++] = 1;
+        if (D >= 3) cand[nc++] = 2;
+        if (D <= 10) cand[nc++] = 3;
+        if (nc < 2) continue;
+        const crt_batch::Chunk &ch = M->chunks[0];
+        float best = 0.f;
+        for (int i = 0; i < nc; ++i) {
+            float t = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                int e = batch_assemble(b, *M, ch);
+                if (e) return e;
+                CRT_HIP(hipEventRecord(b->e0, b->stream));
+                e = batch_factor(b, *M, ch, cand[i]);
+                if (e) return e;
+                CRT_HIP(hipEventRecord(b->e1, b->stream));
+                CRT_HIP(hipEventSynchronize(b->e1));
+                float ms = 0.f;
+                CRT_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+                t = rep == 0 ? ms : std::min(t, ms);
+            }
+            b->tune_ms[D][cand[i]] = t;
+            if (i == 0 || t < best) {
+                best = t;
+                b->variant[D] = cand[i];
+            }
+        }
+    }
+    return 0;
 }
 
 }  // namespace
@@ -659,7 +829,7 @@ int crt_forward(crt_ctx *c) {
         c->tic(T_FACTOR);
         if (use16)
             e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
-                                    c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0);
+                                    c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0, 0);
         else
             e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
                                   c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
@@ -1563,6 +1733,123 @@ int crt_cholz(crt_ctx *c, int n, double *a, double *p) {
         P[i] = band[(size_t)i * (mb + 1) + mb];
         for (int j = i + 1; j < n; ++j) Aw[(size_t)i * n + j] = band[(size_t)j * (mb + 1) + (i - j + mb)];  // a(j,i) = L(j,i)
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------ batches
+int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int flags, crt_batch **out) {
+    if (!ctxs || n <= 0 || !out || (flags & ~1)) return CRT_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return CRT_ERR_ARG;
+        if (!ctxs[i]->have_precal || !ctxs[i]->have_sigma) return CRT_ERR_STATE;
+        if (ctxs[i]->device != ctxs[0]->device) return CRT_ERR_ARG;
+    }
+    std::unique_ptr<crt_batch> b(new crt_batch());
+    b->device = ctxs[0]->device;
+    CRT_HIP(hipSetDevice(b->device));
+    CRT_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    CRT_HIP(hipEventCreate(&b->e0));
+    CRT_HIP(hipEventCreate(&b->e1));
+    if (workspace_bytes == 0) workspace_bytes = (size_t)64 << 30;
+    size_t np = 0, nf = 0, nv = 0, ny = 0;
+    for (int i = 0; i < n; ++i) {
+        auto M = std::make_unique<crt_batch::Model>();
+        M->c = ctxs[i];
+        int e = batch_plan(*M, workspace_bytes);
+        if (e) return e;
+        np = std::max(np, M->batch * M->panel_elems);
+        nf = std::max(nf, M->batch * M->fstride);
+        nv = std::max(nv, M->batch * ((size_t)M->c->nnz + 2));
+        ny = std::max(ny, M->nyscr * M->fstride);
+        b->nsys_total += M->nsys;
+        b->m.push_back(std::move(M));
+    }
+    int e = b->panels.ensure(np);
+    e = e ? e : b->yf.ensure(nf);
+    e = e ? e : b->dinv.ensure(nf);
+    e = e ? e : b->fak.ensure(nf);
+    e = e ? e : b->dga.ensure(nf);
+    e = e ? e : b->vals.ensure(nv);
+    e = e ? e : b->yscr.ensure(ny);
+    if (e) return e;
+    if (flags & 1) {
+        for (auto &M : b->m) CRT_HIP(hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), b->stream));
+        if ((e = batch_tune(b.get()))) return e;
+    }
+    for (auto &M : b->m) M->variant = M->use16 ? b->variant[M->D] : 0;
+    *out = b.release();
+    return 0;
+}
+
+int crt_batch_forward(crt_batch *b) {
+    if (!b) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(b->device));
+    for (auto &M : b->m) {
+        if (!M->c->have_sigma) return CRT_ERR_STATE;
+        // cleared outside the graph: a memset node is not replayed reliably
+        CRT_HIP(hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), b->stream));
+    }
+    if (!b->exec) {
+        CRT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int e = 0;
+        for (auto &M : b->m) {
+            for (const auto &ch : M->chunks) {
+                e = batch_assemble(b, *M, ch);
+                e = e ? e : batch_factor(b, *M, ch, M->variant);
+                e = e ? e : batch_solve(b, *M, ch);
+                if (e) break;
+            }
+            if (e) break;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        if (e || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return e ? e : CRT_ERR_HIP;
+        }
+        b->graph = g;
+        if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            b->exec = nullptr;
+            return CRT_ERR_HIP;
+        }
+    }
+    CRT_HIP(hipEventRecord(b->e0, b->stream));
+    CRT_HIP(hipGraphLaunch(b->exec, b->stream));
+    CRT_HIP(hipEventRecord(b->e1, b->stream));
+    CRT_HIP(hipEventSynchronize(b->e1));
+    CRT_HIP(hipEventElapsedTime(&b->ms, b->e0, b->e1));
+    int worst = 0;
+    for (auto &M : b->m) {
+        std::vector<int> st(M->nsys + 1);
+        CRT_HIP(hipMemcpyAsync(st.data(), M->status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        CRT_HIP(hipStreamSynchronize(b->stream));
+        int w = 0;
+        for (int v : st) w = std::max(w, v);
+        worst = std::max(worst, w);
+        M->c->have_kpot = w == 0;
+        M->c->have_volt = M->c->have_sens = false;
+    }
+    return worst;
+}
+
+int crt_batch_info(crt_batch *b, double *ms, long long *nsys, int *variant, float *tune_ms) {
+    if (!b) return CRT_ERR_ARG;
+    if (ms) *ms = b->ms;
+    if (nsys) *nsys = (long long)b->nsys_total;
+    if (variant)
+        for (int D = 0; D <= 16; ++D) variant[D] = b->variant[D];
+    if (tune_ms)
+        for (int D = 0; D <= 16; ++D)
+            for (int v = 0; v < 4; ++v) tune_ms[D * 4 + v] = b->tune_ms[D][v];
+    return 0;
+}
+
+int crt_batch_destroy(crt_batch *b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
     return 0;
 }
 

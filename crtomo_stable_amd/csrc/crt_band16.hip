@@ -1163,26 +1163,38 @@ __global__ void __launch_bounds__(SWV * 64, 2) crt_factor16st_kernel(Factor16Arg
 
 #define CRT_F16_MAXD 16
 
+// kernel variants: 1 look-ahead (D = 2..10), 2 streaming (D >= 3), 3 plain
+// right-looking (D <= 10); 0 picks 1 up to D = 10 (3 at D = 1) and 2 above
+// (CRT_FACTOR_ST=1 and CRT_FACTOR_V2=1 change that default)
 template <int D>
-static int launch16(hipStream_t st, const Factor16Args &a) {
-    if constexpr (D >= SNU) {
+static int launch16(hipStream_t st, const Factor16Args &a, int variant) {
+    if (variant == 0) {
         static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
-        if (D > 10 || stream) {
+        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
+        variant = (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
+    }
+    if (variant == 2) {
+        if constexpr (D >= SNU) {
             if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
             hipLaunchKernelGGL(crt_factor16st_kernel<D>, dim3(a.nsys), dim3(SWV * 64), 0, st, a);
             return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
         }
+        return CRT_ERR_UNSUPPORTED;
     }
-    if constexpr (D >= 2 && D <= 10) {
-        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
-        if (!v2) {
+    if (variant == 1) {
+        if constexpr (D >= 2 && D <= 10) {
             if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
             hipLaunchKernelGGL(crt_factor16la_kernel<D>, dim3(a.nsys), dim3(La4<D>::NT), 0, st, a);
             return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
         }
+        return CRT_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(crt_factor16_kernel<D>, dim3(a.nsys), dim3(Cfg<D>::NT), 0, st, a);
-    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+    if (variant != 3) return CRT_ERR_ARG;
+    if constexpr (D <= 10) {
+        hipLaunchKernelGGL(crt_factor16_kernel<D>, dim3(a.nsys), dim3(Cfg<D>::NT), 0, st, a);
+        return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+    }
+    return CRT_ERR_UNSUPPORTED;
 }
 
 int crt_factor16_supported(int mb) {
@@ -1191,25 +1203,25 @@ int crt_factor16_supported(int mb) {
 }
 
 int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2 *sys, const double *fak,
-                        const zd *vals, zd *panels, zd *y, zd *dinv, int *status) {
+                        const zd *vals, zd *panels, zd *y, zd *dinv, int *status, int variant) {
     Factor16Args a{g16, nsys, sys, fak, vals, panels, y, dinv, status};
     switch (g16.D) {
-        case 1: return launch16<1>(st, a);
-        case 2: return launch16<2>(st, a);
-        case 3: return launch16<3>(st, a);
-        case 4: return launch16<4>(st, a);
-        case 5: return launch16<5>(st, a);
-        case 6: return launch16<6>(st, a);
-        case 7: return launch16<7>(st, a);
-        case 8: return launch16<8>(st, a);
-        case 9: return launch16<9>(st, a);
-        case 10: return launch16<10>(st, a);
-        case 11: return launch16<11>(st, a);
-        case 12: return launch16<12>(st, a);
-        case 13: return launch16<13>(st, a);
-        case 14: return launch16<14>(st, a);
-        case 15: return launch16<15>(st, a);
-        case 16: return launch16<16>(st, a);
+        case 1: return launch16<1>(st, a, variant);
+        case 2: return launch16<2>(st, a, variant);
+        case 3: return launch16<3>(st, a, variant);
+        case 4: return launch16<4>(st, a, variant);
+        case 5: return launch16<5>(st, a, variant);
+        case 6: return launch16<6>(st, a, variant);
+        case 7: return launch16<7>(st, a, variant);
+        case 8: return launch16<8>(st, a, variant);
+        case 9: return launch16<9>(st, a, variant);
+        case 10: return launch16<10>(st, a, variant);
+        case 11: return launch16<11>(st, a, variant);
+        case 12: return launch16<12>(st, a, variant);
+        case 13: return launch16<13>(st, a, variant);
+        case 14: return launch16<14>(st, a, variant);
+        case 15: return launch16<15>(st, a, variant);
+        case 16: return launch16<16>(st, a, variant);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }

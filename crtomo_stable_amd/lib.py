@@ -333,3 +333,53 @@ class Context:
         ms = np.zeros(8)
         self.lib.crt_get_times(self._h, _ptr(ms), 8)
         return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec"), ms))
+
+
+class Batch:
+    """Forward of many contexts as one graph launch (``crt_batch_*``).
+
+    Every context needs its grid, survey, wavenumbers, precal and sigma; after
+    ``forward()`` each holds kpot as after ``Context.forward()``.  ``autotune``
+    times the factor kernel variants per band width on the first chunk and keeps
+    the fastest."""
+
+    def __init__(self, contexts, workspace_bytes: int = 0, autotune: bool = False):
+        lib = load()
+        lib.crt_batch_create.argtypes = [C.POINTER(_p), C.c_int, C.c_size_t, C.c_int, C.POINTER(_p)]
+        arr = (_p * len(contexts))(*[c._h.value for c in contexts])
+        h = _p()
+        _chk(lib.crt_batch_create(arr, len(contexts), workspace_bytes, 1 if autotune else 0, C.byref(h)),
+             "crt_batch_create")
+        self._h = h
+        self.lib = lib
+        self.contexts = list(contexts)
+
+    def forward(self):
+        _chk(self.lib.crt_batch_forward(self._h), "crt_batch_forward")
+
+    def info(self):
+        """(device ms of the last graph launch, systems per launch, variant per band
+        width in tiles, tuning times [D, variant] in ms)."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        var = np.zeros(17, np.int32)
+        tune = np.zeros(68, np.float32)
+        _chk(self.lib.crt_batch_info(self._h, C.byref(ms), C.byref(n), _ptr(var), _ptr(tune)), "crt_batch_info")
+        return ms.value, n.value, var, tune.reshape(17, 4)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.crt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

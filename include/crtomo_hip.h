@@ -24,6 +24,8 @@
 #ifndef CRTOMO_HIP_H
 #define CRTOMO_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -189,6 +191,28 @@ int crt_cholz(crt_ctx *ctx, int n, double *a, double *p);
  * [0] diag, [1] factor, [2] solve, [3] voltages, [4] sensitivities,
  * [5] precal, [6] cg total, [7] cg matvec (mean per iteration). */
 int crt_get_times(crt_ctx *ctx, double *ms, int n);
+
+/* Batched forward of many models of uneven size (crt_forward of every context,
+ * as one graph launch).  No counterpart in the reference, whose host runs one
+ * model per process (src/fem.f90:328-397); a caller with many surveys or models
+ * (model ensembles, training data for the learned preconditioner) replaces N
+ * crt_forward calls by one crt_batch_forward.
+ *   crt_batch_create: plans each context's systems (every context needs precal
+ *     and sigma; all on one device) in chunks through one shared workspace of
+ *     at most `workspace_bytes` (0: 64 GiB); flags bit 0 times each factor
+ *     kernel variant per band width on the first chunk and keeps the fastest.
+ *   crt_batch_forward: first call captures every launch into a hipGraph, later
+ *     calls replay it; the contexts' sigma may change between calls, their grid,
+ *     survey and wavenumbers may not.  Leaves kpot on each context, as
+ *     crt_forward does (crt_voltages, crt_sens follow per context).
+ *   crt_batch_info: device time of the last graph launch (ms), systems per
+ *     launch, variant[17] per band width (0 default), tune_ms[17*4] timings.
+ * A batch owns its stream; calls on a batch and on its contexts must not overlap. */
+typedef struct crt_batch crt_batch;
+int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int flags, crt_batch **out);
+int crt_batch_forward(crt_batch *b);
+int crt_batch_info(crt_batch *b, double *ms, long long *nsys, int *variant, float *tune_ms);
+int crt_batch_destroy(crt_batch *b);
 
 #ifdef __cplusplus
 }
