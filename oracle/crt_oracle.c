@@ -930,6 +930,159 @@ void orc_cjggra(const orc_cg_problem *p, const double *cgfac, const zdbl *bvec0,
 /* -------------------------------------------------------------- CutMcK */
 /* Cuthill-McKee with start nodes of minimal degree, best band width kept
  * (cutmckee/cutmck.f:126-300).  Returns mb, or -1 on a disconnected graph. */
+/* ---------------------------------------------------------------- buncert */
+
+/* column-major 1-based element (i, j) of an n x n matrix */
+#define M2(a, n, i, j) (a)[(size_t)((j) - 1) * (size_t)(n) + (size_t)((i) - 1)]
+
+/* src/bmcm_mod.f90:226-244: upper triangle (k, j), j >= k, mirrored */
+static void orc_bata(const orc_cg_problem *p, double *ata) {
+    const int n = p->manz, nanz = p->nanz;
+    for (int k = 1; k <= n; ++k)
+        for (int j = k; j <= n; ++j) {
+            zdbl cdum = 0.0;
+            for (int i = 1; i <= nanz; ++i) {
+                const zdbl a = p->sens[(size_t)(k - 1) * nanz + (i - 1)];
+                const zdbl b = p->sens[(size_t)(j - 1) * nanz + (i - 1)];
+                cdum = cdum + conj(a) * b * p->wmatd[i - 1] * (double)p->wdfak[i - 1];
+            }
+            M2(ata, n, k, j) = creal(cdum);
+            M2(ata, n, j, k) = creal(cdum);
+        }
+}
+
+/* src/bmcm_mod.f90:317-383, loops and write order as there */
+static int orc_bata_reg(const orc_cg_problem *p, const double *ata, double *reg) {
+    const int n = p->manz, nx = p->nx, maxn = p->maxn;
+    const double lam = p->lam;
+    const double *sm = p->smatm;
+    memcpy(reg, ata, sizeof(double) * (size_t)n * n);
+    if (p->ltri == 0) {
+#define S0(i, c) sm[(size_t)((c) - 1) * n + (size_t)((i) - 1)]
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= n; ++i) {
+                M2(reg, n, j, j) = M2(ata, n, j, j) + lam * S0(i, 1);
+                if (i + 1 < n) {
+                    M2(reg, n, i + 1, j) = M2(ata, n, i + 1, j) + lam * S0(i + 1, 2);
+                    M2(reg, n, j, i + 1) = M2(reg, n, i + 1, j);
+                }
+                if (i + nx < n) {
+                    M2(reg, n, i + nx, j) = M2(ata, n, i + nx, j) + lam * S0(i + nx, 3);
+                    M2(reg, n, j, i + nx) = M2(reg, n, i + nx, j);
+                }
+            }
+#undef S0
+    } else if (p->ltri == 3 || p->ltri == 4) {
+        for (int i = 1; i <= n; ++i) M2(reg, n, i, i) = M2(ata, n, i, i) + lam * sm[i - 1];
+    } else if (p->ltri == 1 || p->ltri == 2 || (p->ltri > 4 && p->ltri < 15)) {
+#define SN(j, k) sm[(size_t)((k) - 1) * n + (size_t)((j) - 1)]
+#define NB(j, k) p->nachbar[(size_t)((k) - 1) * n + (size_t)((j) - 1)]
+        for (int j = 1; j <= n; ++j) {
+            M2(reg, n, j, j) = M2(ata, n, j, j) + lam * SN(j, maxn + 1);
+            for (int k = 1; k <= NB(j, maxn + 1); ++k) {
+                const int ik = NB(j, k);
+                if (ik == 0) continue;
+                M2(reg, n, ik, j) = M2(ata, n, ik, j) + lam * SN(j, k);
+                M2(reg, n, j, ik) = M2(reg, n, ik, j);
+            }
+        }
+#undef SN
+#undef NB
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+/* src/chold.f90 (stops on a non-positive pivot: here 108) */
+static int orc_chold(double *a, double *pd, int n) {
+    for (int j = 1; j <= n; ++j) {
+        for (int k = 1; k <= j - 1; ++k)
+            for (int i = j; i <= n; ++i) M2(a, n, i, j) = M2(a, n, i, j) - M2(a, n, i, k) * M2(a, n, j, k);
+        if (M2(a, n, j, j) <= 0) return 108;
+        pd[j - 1] = sqrt(M2(a, n, j, j));
+        M2(a, n, j, j) = pd[j - 1];
+        for (int i = j + 1; i <= n; ++i) M2(a, n, i, j) = M2(a, n, i, j) / pd[j - 1];
+    }
+    return 0;
+}
+
+/* src/linvd.f90: L^-1 in the lower triangle, then (L^-1)^T L^-1 in the upper */
+static void orc_linvd(double *a, const double *pd, int n) {
+    for (int i = 1; i <= n; ++i) {
+        M2(a, n, i, i) = 1. / pd[i - 1];
+        for (int j = i + 1; j <= n; ++j) {
+            double s = 0.;
+            for (int k = i; k <= j - 1; ++k) s = s - M2(a, n, j, k) * M2(a, n, k, i);
+            M2(a, n, j, i) = s / pd[j - 1];
+            M2(a, n, i, j) = 0.;
+        }
+    }
+    for (int i = 1; i <= n; ++i) {
+        M2(a, n, i, i) = M2(a, n, i, i) * M2(a, n, i, i);
+        for (int k = i + 1; k <= n; ++k) M2(a, n, i, i) = M2(a, n, i, i) + M2(a, n, k, i) * M2(a, n, k, i);
+        for (int j = i + 1; j <= n; ++j)
+            for (int k = j; k <= n; ++k) M2(a, n, i, j) = M2(a, n, i, j) + M2(a, n, k, i) * M2(a, n, k, j);
+    }
+}
+
+/* src/gauss_dble.f90: Gauss-Jordan in place, no pivoting */
+static int orc_gauss_dble(double *a, int n) {
+    double *temp = (double *)malloc(sizeof(double) * (size_t)n);
+    const double eps = 2.220446049250313e-16;
+    for (int i = 1; i <= n; ++i) {
+        if (fabs(M2(a, n, i, i)) < eps) { free(temp); return 108; }
+        const double d = 1.0 / M2(a, n, i, i);
+        for (int r = 1; r <= n; ++r) temp[r - 1] = M2(a, n, r, i);
+        for (int j = 1; j <= n; ++j) {
+            const double c = M2(a, n, i, j) * d;
+            for (int r = 1; r <= n; ++r) M2(a, n, r, j) = M2(a, n, r, j) - temp[r - 1] * c;
+            M2(a, n, i, j) = c;
+        }
+        for (int r = 1; r <= n; ++r) M2(a, n, r, i) = temp[r - 1] * (-d);
+        M2(a, n, i, i) = d;
+    }
+    free(temp);
+    return 0;
+}
+
+/* MATMUL(a, b), k ascending */
+static void orc_matmul(const double *a, const double *b, double *c, int n) {
+    for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= n; ++i) {
+            double s = 0.0;
+            for (int k = 1; k <= n; ++k) s += M2(a, n, i, k) * M2(b, n, k, j);
+            M2(c, n, i, j) = s;
+        }
+}
+
+int orc_buncert(const orc_cg_problem *p, int lres, int lcov2, int gauss, double *ata, double *ata_reg,
+                double *cov, double *res, double *cov2) {
+    const int n = p->manz;
+    orc_bata(p, ata);
+    if (orc_bata_reg(p, ata, ata_reg)) return -1;
+    if (!cov) return 0;
+    memcpy(cov, ata_reg, sizeof(double) * (size_t)n * n);
+    if (!gauss) {
+        double *pd = (double *)malloc(sizeof(double) * (size_t)n);
+        int e = orc_chold(cov, pd, n);
+        if (!e) orc_linvd(cov, pd, n);
+        free(pd);
+        if (e) return e;
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= i - 1; ++j) M2(cov, n, i, j) = M2(cov, n, j, i);
+    } else {
+        int e = orc_gauss_dble(cov, n);
+        if (e) return e;
+    }
+    if (lres && res) {
+        orc_matmul(cov, ata, res, n);
+        if (lcov2 && cov2) orc_matmul(res, cov, cov2, n);
+    }
+    return 0;
+}
+#undef M2
+
 int orc_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn,
                const int *nrel, int elanz, int *perm, int *start_used) {
     enum { GRMAX = 500, SPMAX = 100 };

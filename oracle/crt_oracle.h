@@ -120,6 +120,16 @@ void orc_cjggra(const orc_cg_problem *p, const double *cgfac, const zdbl *bvec,
 /* one CG matvec  out = C (J^H W J + lam R) C p  (src/cg_mod.f90:520-545,547-577,699-726) */
 void orc_cg_matvec(const orc_cg_problem *p, const double *cgfac, const zdbl *pvec, zdbl *ap, zdbl *out);
 
+/* Linearised model uncertainty (src/bmcm_mod.f90:49-175 buncert, complex case, lam = p->lam):
+ * bata (:176-289), bata_reg (:291-413, the reference's own write order per ltri 0, 1/2/5..14,
+ * 3/4), bmcm (:415-615: chold + linvd + lower fill, or Gauss_dble when gauss), bres
+ * (:617-688), bmcm2 (:690-765).  Matrices are manz x manz column-major; ata and ata_reg
+ * are required, cov/res/cov2 may be NULL (res needs lres, cov2 needs lres and lcov2).
+ * Returns 0, or 108 where the reference stops on a non-positive pivot (chold) or sets
+ * errnr 108 after Gauss_dble. */
+int orc_buncert(const orc_cg_problem *p, int lres, int lcov2, int gauss, double *ata, double *ata_reg,
+                double *cov, double *res, double *cov2);
+
 /* Cuthill-McKee renumbering (cutmckee/cutmck.f:126-337). Writes perm (sanz, 1-based new
  * number of each old node) and returns the band width; start_used gets the chosen start. */
 int orc_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn,

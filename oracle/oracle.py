@@ -273,6 +273,22 @@ def blam0(p: CGProblem, fpi=False, alfx=1.0, alfz=1.0):
     return f(C.byref(p.s), int(bool(fpi)), _d(alfx), _d(alfz))
 
 
+def buncert(p: CGProblem, lres=True, lcov2=True, gauss=False):
+    """(err, dict of manz x manz matrices ata, ata_reg, cov, res, cov2) of
+    src/bmcm_mod.f90:49-175 with lam = p.s.lam (res/cov2 None when not asked)."""
+    n = p.manz
+    m = [np.zeros((n, n)) for _ in range(5)]
+    err = lib().orc_buncert(C.byref(p.s), int(bool(lres)), int(bool(lcov2)), int(bool(gauss)),
+                            *[ptr(x) for x in m])
+    names = ["ata", "ata_reg", "cov", "res", "cov2"]
+    out = {k: x.T for k, x in zip(names, m)}  # column-major storage
+    if not lres:
+        out["res"] = None
+    if not (lres and lcov2):
+        out["cov2"] = None
+    return err, out
+
+
 def cjggra(p: CGProblem, cgfac, bvec):
     dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1)
     ncg = _i()
@@ -377,6 +393,37 @@ class RefSession:
         f = ref().ref_blam0
         f.restype = C.c_double
         return f(int(bool(fpi)), _d(alfx), _d(alfz))
+
+    def buncert(self, lam, sigma, lres=True, lcov2=True, gauss=False):
+        """src/bmcm_mod.f90:49 buncert on the state of the last update() call (J, W)
+        and bsmatm() call (R): the main diagonals the reference writes to ata.diag,
+        ata_reg.diag, cov1_m.diag, res_m.diag, cov2_m.diag, parsed back (first data
+        column of each, cov1_m/cov2_m: all three).  The system must be positive
+        definite: the reference's chold stops the process otherwise."""
+        import shutil
+        import tempfile
+        d = tempfile.mkdtemp(prefix="crt_buncert_")
+        try:
+            s = np.ascontiguousarray(sigma, np.complex128)
+            db = d.encode()
+            err = ref().ref_buncert(_d(lam), int(bool(lres)), int(bool(lcov2)), int(bool(gauss)), ptr(s),
+                                   C.c_char_p(db), len(db))
+            _chk(err, "ref_buncert")
+            out = {}
+            for key, fname, ncol in (("ata", "ata.diag", 1), ("ata_reg", "ata_reg.diag", 1),
+                                     ("cov1", "cov1_m.diag", 3), ("res", "res_m.diag", 1),
+                                     ("cov2", "cov2_m.diag", 3)):
+                path = os.path.join(d, fname)
+                if not os.path.exists(path):
+                    continue
+                lines = open(path).read().splitlines()
+                n = int(float(lines[0].split()[0]))
+                rows = [[float(t.replace("D", "E")) for t in ln.replace(",", " ").split()] for ln in lines[1:n + 1]]
+                arr = np.array(rows)
+                out[key] = arr[:, 0] if ncol == 1 else arr[:, :3]
+            return out
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
 
     def update(self, p: CGProblem):
         a = p.arrs

@@ -396,4 +396,50 @@ contains
     ref_blam0 = lammax
   end function ref_blam0
 
+  ! linearised model uncertainty through the reference driver buncert
+  ! (src/bmcm_mod.f90:49) with lamalt = lam_, on the state of the last ref_update
+  ! (sens, wmatd, wdfak) and ref_bsmatm (ltri, nx, smatm, nachbar) calls; sigma_
+  ! (manz) feeds bpar and the cov1_m/cov2_m columns.  The reference writes its
+  ! .diag files (ata, ata_reg, cov1_m, res_m, cov2_m) into the directory dir_.
+  ! Its chold stops the process on a non-positive pivot: callers pass positive
+  ! definite systems only.
+  integer(c_int) function ref_buncert(lam_, lres_, lcov2_, lgauss_, sigma_, dir_, ndir_) &
+       bind(C, name='ref_buncert')
+    use bmcm_mod, only: buncert
+    use pathmod, only: ramd, lnramd, slash
+    real(c_double), value :: lam_
+    integer(c_int), value :: lres_, lcov2_, lgauss_, ndir_
+    complex(c_double_complex), intent(in) :: sigma_(*)
+    character(kind=c_char), intent(in) :: dir_(*)
+    integer :: j
+    ramd = ' '
+    do j = 1, ndir_
+       ramd(j:j) = dir_(j)
+    end do
+    lnramd = ndir_; slash = '/'
+    lres = lres_ /= 0; lcov2 = lcov2_ /= 0; lgauss = lgauss_ /= 0
+    lfpi = .false.; lelerr = .false.; ldc = .false.; lverb = .false.; lverb_dat = .false.
+    mswitch = 0
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(manz))
+    sigma = sigma_(1:manz)
+    if (allocated(mnr)) deallocate(mnr)
+    allocate(mnr(elanz))
+    do j = 1, elanz
+       mnr(j) = j
+    end do
+    if (allocated(par)) deallocate(par)
+    allocate(par(manz))
+    ! bata reads wmatdp (the FPI weights) for its dig_fpi column whatever lfpi is
+    if (allocated(wmatdp)) deallocate(wmatdp)
+    allocate(wmatdp(nanz))
+    wmatdp = wmatd
+    fprun = 93
+    open(fprun, file=ramd(1:lnramd)//'/run.ctr', status='replace')
+    errnr = 0
+    call buncert(94, lam_)
+    close(fprun)
+    ref_buncert = errnr
+  end function ref_buncert
+
 end module crt_ref_harness

@@ -105,6 +105,27 @@ def main(names):
         out.pop("kpot_src1")
         out.update(cg_golden(w, R, out, 1, 3.0, 1e-12, 400, 2))
         np.savez_compressed(os.path.join(HERE, "ref_small_tri.npz"), **out)
+    if want("buncert"):
+        # model uncertainty (src/bmcm_mod.f90 buncert) on the CG problems of the two
+        # grids above: ltri 1 through chold/linvd, ltri 0 through Gauss_dble (its
+        # regularisation fill leaves ata_reg indefinite, where chold stops)
+        out = {}
+        for ltri, name, gauss in ((0, "small_quad", True), (1, "small_tri", False)):
+            w = configs.small_quad(cutmck=O.cutmck) if ltri == 0 else \
+                configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+            ref = np.load(os.path.join(HERE, f"ref_{name}.npz"))
+            p = f"cg{ltri}_"
+            R = O.RefSession(w.grid, w.survey)
+            smatm, nachbar = R.bsmatm(ltri, w.grid.nx, w.grid.nz, 1.0, 1.0)
+            cp = O.CGProblem(ref["sens"], ref[p + "wmatd"], np.ones(ref["sigmaa"].size, np.int32), ref[p + "dat"],
+                             ref["sigmaa"], ref[p + "par"], smatm, ltri, w.grid.nx, float(ref[p + "lam"]),
+                             float(ref[p + "eps"]), 2, nachbar=nachbar if ltri else None, maxn=w.grid.maxn)
+            R.update(cp)
+            bu = R.buncert(float(ref[p + "lam"]), w.sigma, lres=True, lcov2=True, gauss=gauss)
+            for k, v in bu.items():
+                out[f"q{ltri}_{k}"] = v
+            out[f"q{ltri}_gauss"] = gauss
+        np.savez_compressed(os.path.join(HERE, "ref_buncert.npz"), **out)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

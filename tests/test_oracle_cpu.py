@@ -354,3 +354,114 @@ def test_pc_reference_gradient_finite_differences():
         p2 = params.copy(); p2[i] -= h
         fd = (R.loss_grad(p1, norm, feat, tgt, mask)[0] - R.loss_grad(p2, norm, feat, tgt, mask)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def _bu_problem(O, ltri, ncgmax=2):
+    """The CG problems of the golden grids (ltri 0: small_quad, 1: small_tri)."""
+    name = "small_quad" if ltri == 0 else "small_tri"
+    w = configs.small_quad(cutmck=O.cutmck) if ltri == 0 else \
+        configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    ref = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    p = f"cg{ltri}_"
+    g = w.grid
+    if ltri == 0:
+        smatm, nachbar = O.bsmatmreg(g, g.nx, g.nz, 1.0, 1.0), None
+    else:
+        nachbar = O.bnachbar(g)
+        smatm = O.bsmatmtri(g, nachbar, 1.0, 1.0)
+    cp = O.CGProblem(ref["sens"], ref[p + "wmatd"], np.ones(ref["sigmaa"].size, np.int32), ref[p + "dat"],
+                     ref["sigmaa"], ref[p + "par"], smatm, ltri, g.nx, float(ref[p + "lam"]),
+                     float(ref[p + "eps"]), ncgmax, nachbar=nachbar, maxn=g.maxn)
+    return cp, w
+
+
+def _close(a, b, tol=1e-15):
+    a = np.asarray(a); b = np.asarray(b)
+    if np.array_equal(a, b, equal_nan=True):
+        return True
+    ok = np.isfinite(b)
+    return np.array_equal(np.isnan(a), np.isnan(b)) and np.abs(a[ok] - b[ok]).max() <= tol * np.abs(b[ok]).max()
+
+
+@pytest.mark.parametrize("ltri", [0, 1])
+def test_oracle_buncert_vs_reference_golden(oracle_lib, ltri):
+    """buncert (src/bmcm_mod.f90:49-175): the main diagonals the reference wrote to
+    ata.diag, ata_reg.diag, cov1_m.diag, res_m.diag and cov2_m.diag.  ltri 1 runs
+    chold + linvd; ltri 0 runs Gauss_dble, since the reference's ltri-0 fill of
+    ata_reg leaves it indefinite (chold stops there; the oracle returns 108)."""
+    O = oracle_lib
+    gold = np.load(os.path.join(GOLD, "ref_buncert.npz"))
+    cp, w = _bu_problem(O, ltri)
+    q = f"q{ltri}_"
+    gauss = bool(gold[q + "gauss"])
+    err, m = O.buncert(cp, gauss=gauss)
+    assert err == 0
+    assert _close(m["ata"].diagonal(), gold[q + "ata"])
+    assert _close(m["ata_reg"].diagonal(), gold[q + "ata_reg"])
+    with np.errstate(invalid="ignore"):
+        sd = np.sqrt(m["cov"].diagonal())
+        sd2 = np.sqrt(m["cov2"].diagonal())
+    assert _close(sd * 1e2, gold[q + "cov1"][:, 0])
+    assert _close(w.sigma.real * sd, gold[q + "cov1"][:, 1])
+    assert _close(w.sigma.imag * sd, gold[q + "cov1"][:, 2])
+    assert _close(np.abs(m["res"].diagonal()), gold[q + "res"])
+    assert _close(sd2 * 1e2, gold[q + "cov2"][:, 0])
+    assert _close((w.sigma * sd2).real, gold[q + "cov2"][:, 1])
+    if ltri == 0:
+        assert O.buncert(cp, gauss=False)[0] == 108
+
+
+def test_oracle_buncert_algebra(oracle_lib):
+    """buncert against numpy on a random complex J: ata = Re(J^H W J), the ltri-0 fill
+    in closed form (the reference's loop leaves lam*smatm(manz,1) on the diagonal and
+    lam*smatm(lo,2) on every off-diagonal entry, lo = min(i,j), or smatm(hi,2) when
+    lo = 1 < hi < manz), the neighbour fill of ltri 1, cov ata_reg = I for chold+linvd
+    and Gauss_dble, res = cov ata, cov2 = res cov."""
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    nanz, manz, nx = 45, 36, 6
+    J = rng.standard_normal((nanz, manz)) + 1j * rng.standard_normal((nanz, manz))
+    w = rng.random(nanz) + 0.5
+    wd = rng.integers(1, 3, nanz).astype(np.int32)
+    z = np.zeros(nanz, np.complex128)
+    ata_np = (J.conj().T @ ((w * wd)[:, None] * J)).real
+    sm0 = np.stack([rng.random(manz) + 2.0, -0.01 * rng.random(manz), -0.01 * rng.random(manz)])
+    cp0 = O.CGProblem(J.T, w, wd, z, z, np.zeros(manz, np.complex128), sm0, 0, nx, 0.7, 1e-4, 1)
+    err, m = O.buncert(cp0, gauss=True)
+    assert err == 0
+    assert np.abs(m["ata"] - ata_np).max() <= 1e-13 * np.abs(ata_np).max()
+    i = np.arange(1, manz + 1)
+    lo = np.minimum.outer(i, i); hi = np.maximum.outer(i, i)
+    s2 = sm0[1]
+    add = np.where(lo >= 2, s2[lo - 1], np.where(hi <= manz - 1, s2[hi - 1], 0.0))
+    reg0 = m["ata"] + 0.7 * add
+    reg0[np.diag_indices(manz)] = m["ata"].diagonal() + 0.7 * sm0[0][manz - 1]
+    assert np.array_equal(m["ata_reg"], reg0)
+    assert np.abs(m["cov"] @ m["ata_reg"] - np.eye(manz)).max() <= 1e-9
+    # ltri 1: a chain of neighbours j-1, j+1 with symmetric couplings
+    maxn = 4
+    nb = np.zeros((maxn + 1, manz), np.int32)
+    sm1 = np.zeros((maxn + 1, manz))
+    cpl = -0.3 * (rng.random(manz) + 0.5)
+    for j in range(manz):
+        ks = [x for x in (j - 1, j + 1) if 0 <= x < manz]
+        for k, x in enumerate(ks):
+            nb[k, j] = x + 1
+            sm1[k, j] = cpl[min(j, x)]
+        nb[maxn, j] = len(ks)
+        sm1[maxn, j] = 1.0 + 0.6 * len(ks)
+    cp1 = O.CGProblem(J.T, w, wd, z, z, np.zeros(manz, np.complex128), sm1, 1, nx, 0.7, 1e-4, 1,
+                      nachbar=nb, maxn=maxn)
+    err, m1 = O.buncert(cp1)
+    assert err == 0
+    R = np.diag(sm1[maxn])
+    for j in range(manz - 1):
+        R[j, j + 1] = R[j + 1, j] = cpl[j]
+    assert np.abs(m1["ata_reg"] - (ata_np + 0.7 * R)).max() <= 1e-13 * np.abs(ata_np).max()
+    reg = m1["ata_reg"]
+    assert np.abs(m1["cov"] @ reg - np.eye(manz)).max() <= 1e-10
+    assert np.array_equal(m1["cov"], m1["cov"].T)
+    assert np.abs(m1["res"] - m1["cov"] @ m1["ata"]).max() <= 1e-12 * np.abs(m1["res"]).max()
+    assert np.abs(m1["cov2"] - m1["res"] @ m1["cov"]).max() <= 1e-12 * np.abs(m1["cov2"]).max()
+    err, mg = O.buncert(cp1, gauss=True)
+    assert err == 0 and np.abs(mg["cov"] - m1["cov"]).max() <= 1e-10 * np.abs(m1["cov"]).max()
