@@ -28,6 +28,14 @@ int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const d
                         zd *, int *, int);
 int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                        const zd *, const zd *, const zd *, zd *, zd *);
+// mixed precision (crt_band32.hip, FP32-panel solve modes in crt_band16.hip)
+int crt_factor32_supported(int mb);
+int crt_launch_factor32(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zf *, zf *, zf *,
+                        int *);
+int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const int *, const int *, const int2 *,
+                            const double *, const zd *, const zd *, zd *);
+int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
+                           const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
@@ -100,7 +108,7 @@ struct DBuf {
     }
 };
 
-enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_N };
+enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_N };
 
 }  // namespace
 
@@ -124,6 +132,21 @@ struct crt_ctx {
     DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_crow, d_cptr, d_contrib, d_nkel, d_status;
     DBuf<int> d_tmap;
     int tmapD = 0;
+    // full rows of the symmetric matrix over the lower-triangle CSR: row pointers and
+    // (column, CSR index) per entry, for the refinement residual
+    DBuf<int> d_frow;
+    DBuf<int2> d_fent;
+    // forward precision: 64 (FP64 factorisation) or 32 (FP32 factorisation + FP64
+    // iterative refinement until max|fak d| <= refine_tol max|fak x| per system)
+    int fwd_bits = 64;
+    double refine_tol = 1e-14;     // stop once the correction is this small
+    double refine_accept = 1e-12;  // ... or once it stalls at most this large
+    int refine_max = 12;
+    int refine_sweeps = 0;      // most sweeps any batch of the last forward needed
+    double refine_ratio = 0.0;  // largest final correction ratio of the last forward
+    int refine_fallbacks = 0;   // batches of the last forward redone in FP64
+    DBuf<zd> d_xacc;
+    DBuf<double> d_rstat;
     int nnz = 0;
     DBuf<int4> d_meas, d_rhs;
     DBuf<int2> d_sys;
@@ -614,6 +637,26 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
             c->tmapD = D16;
         }
     }
+    {
+        std::vector<int> frow(sanz + 1, 0);
+        for (int i = 0; i < sanz; ++i)
+            for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+                frow[i + 1]++;
+                if (col[q] != i) frow[col[q] + 1]++;
+            }
+        for (int i = 0; i < sanz; ++i) frow[i + 1] += frow[i];
+        std::vector<int2> fent(std::max(frow[sanz], 1));
+        std::vector<int> fill(frow.begin(), frow.end() - 1);
+        for (int i = 0; i < sanz; ++i)
+            for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+                const int j = col[q];
+                fent[fill[i]++] = int2{j, q};
+                if (j != i) fent[fill[j]++] = int2{i, q};
+            }
+        e = c->d_frow.upload(frow.data(), frow.size(), s);
+        e = e ? e : c->d_fent.upload(fent.data(), fent.size(), s);
+        if (e) return e;
+    }
     CRT_HIP(hipStreamSynchronize(s));
     c->have_grid = true;
     return 0;
@@ -762,6 +805,68 @@ int crt_set_sigma(crt_ctx *c, const double *sigma) {
     return 0;
 }
 
+// FP32 factorisation of one batch of systems, first solve and FP64 refinement
+// sweeps (crt_band32.hip).  Returns 0 when every right-hand side met the
+// tolerance, -1 when the batch has to be redone in FP64 (no convergence within
+// refine_max sweeps, a growing correction, or an FP32 pivot failure), else an
+// error code.  Device times (ms) of the factorisation, the first solve and the
+// sweeps go to tf, ts, tr.
+static int forward_batch_f32(crt_ctx *c, const DevGrid &g, int nb, size_t s0, int nrhs, int D, int &sweeps,
+                             double &ratio, double &tf, double &ts, double &tr) {
+    const int K = c->kwnanz, E = c->eanz, nblk = g.nblk, fstride = g.fstride;
+    zf *panels = reinterpret_cast<zf *>(c->d_panels.p);
+    zf *y32 = reinterpret_cast<zf *>(c->d_yf.p);
+    zf *dinv32 = reinterpret_cast<zf *>(c->d_dinv.p);
+    c->tic(T_FACTOR);
+    int e = crt_launch_factor32(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, panels, y32, dinv32,
+                                c->d_status.p + 1 + s0);
+    if (e) return e;
+    c->toc(T_FACTOR);
+    c->tic(T_SOLVE);
+    e = crt_launch_solve16_f32(c->stream, c->sanz, nblk, D, K, E, nrhs, c->d_rhs.p, c->d_enr.p, c->d_fak.p, panels,
+                               dinv32, y32, c->d_yscr.p, c->d_kpotN.p, c->d_xacc.p, c->d_rstat.p, 1);
+    if (e) return e;
+    c->toc(T_SOLVE);
+    CRT_HIP(hipStreamSynchronize(c->stream));
+    c->collect(T_FACTOR); tf = c->times[T_FACTOR];
+    c->collect(T_SOLVE); ts = c->times[T_SOLVE];
+    std::vector<int> st(nb);
+    CRT_HIP(c->copy_out(st.data(), c->d_status.p + 1 + s0, nb * sizeof(int)));
+    for (int v : st)
+        if (v) return -1;
+    std::vector<double> h(2 * (size_t)nrhs);
+    double prev = HUGE_VAL;
+    tr = 0.0;
+    sweeps = 0;
+    for (int sw = 1; sw <= c->refine_max; ++sw) {
+        c->tic(T_REFINE);
+        e = crt_launch_refine_resid(c->stream, c->sanz, fstride, g.vstride, nrhs, c->d_rhs.p, c->d_enr.p,
+                                    c->d_frow.p, c->d_fent.p, c->d_fak.p, c->d_vals.p, c->d_xacc.p, c->d_yscr.p);
+        e = e ? e : crt_launch_solve16_f32(c->stream, c->sanz, nblk, D, K, E, nrhs, c->d_rhs.p, c->d_enr.p,
+                                           c->d_fak.p, panels, dinv32, y32, c->d_yscr.p, c->d_kpotN.p, c->d_xacc.p,
+                                           c->d_rstat.p, 2);
+        if (e) return e;
+        c->toc(T_REFINE);
+        CRT_HIP(c->copy_out(h.data(), c->d_rstat.p, h.size() * sizeof(double)));
+        c->collect(T_REFINE);
+        tr += c->times[T_REFINE];
+        sweeps = sw;
+        ratio = 0.0;
+        for (int q = 0; q < nrhs; ++q) {
+            const double d = h[2 * q], x = h[2 * q + 1];
+            const double rq = x > 0.0 ? d / x : (d > 0.0 ? HUGE_VAL : 0.0);
+            if (!(rq <= ratio)) ratio = rq;  // NaN propagates as "not converged"
+        }
+        if (getenv("CRT_DEBUG")) fprintf(stderr, "crt_forward fp32: batch at %zu sweep %d ratio %.3e\n", s0, sw, ratio);
+        if (ratio <= c->refine_tol) return 0;
+        // no longer halving: at the FP64 rounding floor of the residual (accept when
+        // that floor is below refine_accept) or not converging (redo in FP64)
+        if (!(ratio < 0.5 * prev)) return ratio <= c->refine_accept ? 0 : -1;
+        prev = ratio;
+    }
+    return -1;
+}
+
 int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
@@ -772,6 +877,8 @@ int crt_forward(crt_ctx *c) {
     const int K = c->kwnanz, E = c->eanz;
     const int nblk = (c->sanz + NBt - 1) / NBt, D = std::max(1, (c->mb + NBt - 1) / NBt);
     if (!use16 && crt_factor_lds_bytes(D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
+    const bool fp32 = c->fwd_bits == 32;
+    if (fp32 && !(use16 && crt_factor32_supported(c->mb) && c->tmapD == D)) return CRT_ERR_UNSUPPORTED;
     // systems to factorise: every (l,k) with mixed boundaries, else l = 1 per k
     // (src/fem.f90:355: lbeta.or.l.eq.1)
     std::vector<int2> sys;
@@ -805,6 +912,13 @@ int crt_forward(crt_ctx *c) {
     e = e ? e : c->d_status.ensure(sys.size() + 1);
     if (e) return e;
     size_t nyscr = c->lbeta ? 1 : rhs.size();
+    // refinement: per right-hand side of a batch the potentials x and the residual
+    const size_t rhs_batch = c->lbeta ? batch : std::min(rhs.size(), batch * (size_t)E);
+    if (fp32) {
+        nyscr = std::max(nyscr, rhs_batch);
+        if ((e = c->d_xacc.ensure(rhs_batch * fstride))) return e;
+        if ((e = c->d_rstat.ensure(2 * rhs_batch))) return e;
+    }
     if ((e = c->d_yscr.ensure(nyscr * fstride))) return e;
     CRT_HIP(hipMemcpyAsync(c->d_sys.p, sys.data(), sys.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     CRT_HIP(hipMemsetAsync(c->d_status.p, 0, (sys.size() + 1) * sizeof(int), c->stream));
@@ -812,7 +926,10 @@ int crt_forward(crt_ctx *c) {
     g.nblk = nblk;
     g.D = D;
     g.fstride = (int)fstride;
-    double tdiag = 0, tfac = 0, tsol = 0;
+    double tdiag = 0, tfac = 0, tsol = 0, tref = 0;
+    c->refine_sweeps = 0;
+    c->refine_ratio = 0.0;
+    c->refine_fallbacks = 0;
     for (size_t s0 = 0; s0 < sys.size(); s0 += batch) {
         int nb = (int)std::min(batch, sys.size() - s0);
         // right-hand sides of this batch, factor index relative to the batch
@@ -826,29 +943,51 @@ int crt_forward(crt_ctx *c) {
         if ((e = crt_launch_assemble(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p)))
             return e;
         c->toc(T_DIAG);
-        c->tic(T_FACTOR);
-        if (use16)
-            e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
-                                    c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0, 0);
-        else
-            e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
-                                  c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
-        if (e) return e;
-        c->toc(T_FACTOR);
-        c->tic(T_SOLVE);
-        if (use16)
-            e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                   c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
-        else
-            e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                 c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr, nullptr);
-        if (e) return e;
-        c->toc(T_SOLVE);
+        bool done = false;
+        if (fp32) {
+            int sweeps = 0;
+            double ratio = 0.0, tf = 0.0, ts = 0.0, tr = 0.0;
+            e = forward_batch_f32(c, g, nb, s0, (int)r.size(), D, sweeps, ratio, tf, ts, tr);
+            if (e > 0) return e;
+            tfac += tf; tsol += ts; tref += tr;
+            c->refine_sweeps = std::max(c->refine_sweeps, sweeps);
+            if (e == 0) {
+                c->refine_ratio = std::max(c->refine_ratio, ratio);
+                done = true;
+            } else {
+                // no convergence (or an FP32 pivot failure): this batch again in FP64
+                c->refine_fallbacks++;
+                CRT_HIP(hipMemsetAsync(c->d_status.p + 1 + s0, 0, nb * sizeof(int), c->stream));
+            }
+        }
+        if (!done) {
+            c->tic(T_FACTOR);
+            if (use16)
+                e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
+                                        c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0, 0);
+            else
+                e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
+                                      c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
+            if (e) return e;
+            c->toc(T_FACTOR);
+            c->tic(T_SOLVE);
+            if (use16)
+                e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
+                                       c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
+            else
+                e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
+                                     c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr, nullptr);
+            if (e) return e;
+            c->toc(T_SOLVE);
+        }
         CRT_HIP(hipStreamSynchronize(c->stream));
         c->collect(T_DIAG); tdiag += c->times[T_DIAG];
-        c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
-        c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
+        if (!done) {
+            c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
+            c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
+        }
     }
+    c->times[T_REFINE] = tref;
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol;
     std::vector<int> st(sys.size() + 1);
     CRT_HIP(c->copy_out(st.data(), c->d_status.p, st.size() * sizeof(int)));
@@ -1443,6 +1582,25 @@ int crt_set_cg_precision(crt_ctx *c, int jbits) {
     c->cg_jbits = jbits;
     if (jbits == 64) c->d_sens32.release();
     c->sens32_ok = false;
+    return 0;
+}
+
+int crt_set_forward_precision(crt_ctx *c, int bits, double tol, int max_sweeps) {
+    if (!c || (bits != 64 && bits != 32) || !(tol > 0.0) || max_sweeps < 1) return CRT_ERR_ARG;
+    if (bits == 32 && c->have_grid && !(crt_factor32_supported(c->mb) && c->tmapD == (c->mb + 15) / 16))
+        return CRT_ERR_UNSUPPORTED;
+    c->fwd_bits = bits;
+    c->refine_tol = tol;
+    c->refine_accept = std::max(tol, 1e-12);
+    c->refine_max = max_sweeps;
+    return 0;
+}
+
+int crt_get_refine_info(crt_ctx *c, int *sweeps, double *ratio, int *fallbacks) {
+    if (!c) return CRT_ERR_ARG;
+    if (sweeps) *sweeps = c->refine_sweeps;
+    if (ratio) *ratio = c->refine_ratio;
+    if (fallbacks) *fallbacks = c->refine_fallbacks;
     return 0;
 }
 

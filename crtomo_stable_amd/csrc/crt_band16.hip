@@ -24,6 +24,8 @@
 // j+16(i%4), element i/4.
 #include "crt_common.hpp"
 
+#include <type_traits>
+
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -1235,31 +1237,43 @@ int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2
 // starts at the electrode's block (the right-hand side is zero above it) and is
 // skipped when the factorisation already produced y.  Lanes hold tile entries
 // (r, c) = (l/16 + 4v, l%16), v = 0..3.
-struct Solve16Args {
+// PT is the panel element type: zd, or zf for the FP32 factorisation (read into
+// FP64 arithmetic).  MODE 0: x scaled by fak into kpotN.  MODE 1 (first solve of
+// the refinement): also x into xacc.  MODE 2 (refinement sweep): the right-hand
+// side is the negated residual in yscr (overwritten by y), x is added to xacc,
+// kpotN gets fak * xacc, and stats[2 rid + 0/1] the max-norms of fak * x and of
+// fak * xacc.
+template <class PT>
+struct SolveArgs {
     int sanz, nblk, kwnanz, eanz, nrhs;
     const int4 *rhs;     // (l, k, factor system, have_y)
     const int *enr;
     const double *fak;
-    const zd *panels;
-    const zd *dinv;
-    const zd *y;
+    const PT *panels;
+    const PT *dinv;
+    const PT *y;
     zd *yscr;
     zd *kpotN;
+    zd *xacc;            // MODE 1/2: [rhs][nblk*16]
+    double *stats;       // MODE 2: [rhs][2]
 };
+using Solve16Args = SolveArgs<zd>;
 
 namespace {
 constexpr int SW = 4;  // waves per right-hand side
 
-template <int D>
-__device__ __forceinline__ void solve16_fetch_diag(const zd *panel, const zd *dinv, int nblk, int J, int c,
+template <int D, class PT>
+__device__ __forceinline__ void solve16_fetch_diag(const PT *panel, const PT *dinv, int nblk, int J, int c,
                                                    zd (&drow)[T16], zd &pnext) {
     const int Jc = J < 0 ? 0 : (J < nblk ? J : nblk - 1);
-    const zd *Ld = panel + (size_t)Jc * (D + 1) * 256 + c * T16;
+    const PT *Ld = panel + (size_t)Jc * (D + 1) * 256 + c * T16;
 #pragma unroll
     for (int cc = 0; cc < T16; ++cc) drow[cc] = zd{Ld[cc].re, Ld[cc].im};
-    const zd *pv = dinv + (size_t)Jc * T16 + c;
+    const PT *pv = dinv + (size_t)Jc * T16 + c;
     pnext = zd{pv->re, pv->im};
 }
+
+__device__ __forceinline__ double cmax_abs(zd z) { return fmax(fabs(z.re), fabs(z.im)); }
 
 __device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd (&drow)[T16]) {
     if (lane < T16)
@@ -1272,8 +1286,8 @@ __device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd
 
 }  // namespace
 
-template <int D>
-__global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(Solve16Args A) {
+template <int D, class PT = zd, int MODE = 0>
+__global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(SolveArgs<PT> A) {
     constexpr int NA = (D + SW - 2) / (SW - 1);  // off-diagonal tiles per worker wave
     __shared__ zd ring[(D + 1) * T16];
     __shared__ zd part[SW][T16];
@@ -1286,18 +1300,21 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
     const int4 q4 = A.rhs[rid];
     const int l = q4.x, k = q4.y, fs = q4.z;
     const double *fak = A.fak + (size_t)fs * nblk * T16;
-    const zd *panel = A.panels + (size_t)fs * nblk * (D + 1) * 256;
-    const zd *dinv = A.dinv + (size_t)fs * nblk * T16;
+    const PT *panel = A.panels + (size_t)fs * nblk * (D + 1) * 256;
+    const PT *dinv = A.dinv + (size_t)fs * nblk * T16;
     const int enode = A.enr[l];
     zd *yscr = A.yscr + (size_t)rid * nblk * T16;
-    const int J0 = q4.w ? 0 : enode / T16;  // y is zero above the electrode's block
+    // a refinement sweep runs the forward pass over every block (its right-hand
+    // side is a full residual); otherwise y is zero above the electrode's block
+    const bool have_y = MODE == 2 ? false : q4.w != 0;
+    const int J0 = (MODE == 2 || have_y) ? 0 : enode / T16;
 
     // The two wave roles run separate loops with the same barrier sequence, so
     // their registers are allocated independently.
     if (wave == 0) {
         // ---- wave 0: substitution through the diagonal tiles
         zd pcur, drow[T16], pnext;
-        if (!q4.w) {
+        if (!have_y) {
             solve16_fetch_diag<D>(panel, dinv, nblk, J0, c, drow, pnext);
             solve16_store_diag(dtile, lane, drow);
             pcur = pnext;
@@ -1305,7 +1322,9 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 solve16_fetch_diag<D>(panel, dinv, nblk, J + 1, c, drow, pnext);
                 const int r = c;  // lane r = lane & 15 carries row r of the block
                 const int i = J * T16 + r;
-                zd yv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
+                zd yv;
+                if constexpr (MODE == 2) yv = yscr[(size_t)J * T16 + r];
+                else yv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
                 __syncthreads();
 #pragma unroll
                 for (int w = 1; w < SW; ++w) yv = zsub(yv, part[w][r]);
@@ -1325,15 +1344,30 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 __syncthreads();
             }
         }
-        const zd *yg = q4.w ? A.y + (size_t)fs * nblk * T16 : yscr;
+        const PT *yf = A.y + (size_t)fs * nblk * T16;
+        const zd *yg = nullptr;
+        if constexpr (std::is_same<PT, zd>::value) yg = have_y ? yf : yscr;
+        // y of row i: the factor's own (have_y) or this right-hand side's forward pass
+        auto yat = [&](int i) -> zd {
+            if constexpr (std::is_same<PT, zd>::value) {
+                return yg[i];
+            } else {
+                if (have_y) {
+                    const PT e = yf[i];
+                    return zd{e.re, e.im};
+                }
+                return yscr[i];
+            }
+        };
         __syncthreads();  // the forward pass has finished with dtile
         solve16_fetch_diag<D>(panel, dinv, nblk, nblk - 1, c, drow, pnext);
         solve16_store_diag(dtile, lane, drow);
         pcur = pnext;
-        zd ycur = nblk - 1 >= J0 ? yg[(size_t)(nblk - 1) * T16 + c] : zd{0.0, 0.0};
+        zd ycur = nblk - 1 >= J0 ? yat((nblk - 1) * T16 + c) : zd{0.0, 0.0};
+        double dmax = 0.0, xmax = 0.0;
         for (int J = nblk - 1; J >= 0; --J) {
             solve16_fetch_diag<D>(panel, dinv, nblk, J - 1, c, drow, pnext);
-            const zd ynxt = J - 1 >= J0 ? yg[(size_t)(J - 1) * T16 + c] : zd{0.0, 0.0};
+            const zd ynxt = J - 1 >= J0 ? yat((J - 1) * T16 + c) : zd{0.0, 0.0};
             __syncthreads();
             zd xv = zd{-ycur.re, -ycur.im};  // lane c carries column c of the block
 #pragma unroll
@@ -1349,7 +1383,15 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 const int i = J * T16 + lane;
                 if (i < A.sanz) {
                     const double f = fak[i];
-                    A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xv.re * f, xv.im * f};
+                    zd xo = xv;
+                    if constexpr (MODE >= 1) {
+                        zd *xa = A.xacc + (size_t)rid * nblk * T16 + i;
+                        if constexpr (MODE == 2) xo = zadd(*xa, xv);
+                        *xa = xo;
+                        dmax = fmax(dmax, f * cmax_abs(xv));
+                        xmax = fmax(xmax, f * cmax_abs(xo));
+                    }
+                    A.kpotN[((size_t)i * A.kwnanz + k) * A.eanz + l] = zd{xo.re * f, xo.im * f};
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -1358,21 +1400,32 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             ycur = ynxt;
             __syncthreads();
         }
+        if constexpr (MODE >= 1) {
+#pragma unroll
+            for (int m = 1; m < T16; m <<= 1) {
+                dmax = fmax(dmax, __shfl_xor(dmax, m, 64));
+                xmax = fmax(xmax, __shfl_xor(xmax, m, 64));
+            }
+            if (lane == 0) {
+                A.stats[2 * (size_t)rid] = dmax;
+                A.stats[2 * (size_t)rid + 1] = xmax;
+            }
+        }
         return;
     }
 
     // ---- waves 1..3: products with the off-diagonal tiles a = wave + 3u
     zd cur[NA][4];
-    if (!q4.w) {
+    if (!have_y) {
         auto fetch_f = [&](int J, zd (&t)[NA][4]) {
 #pragma unroll
             for (int u = 0; u < NA; ++u) {
                 const int a = wave + (SW - 1) * u;
                 const bool live = a <= D && J - a >= J0 && J < nblk;
-                const zd *Lt = panel + ((size_t)(live ? J - a : 0) * (D + 1) + (live ? a : 0)) * 256;
+                const PT *Lt = panel + ((size_t)(live ? J - a : 0) * (D + 1) + (live ? a : 0)) * 256;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const zd *p = Lt + (rq + 4 * v) * T16 + c;
+                    const PT *p = Lt + (rq + 4 * v) * T16 + c;
                     t[u][v] = live ? zd{p->re, p->im} : zd{0.0, 0.0};
                 }
             }
@@ -1413,10 +1466,10 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
         for (int u = 0; u < NA; ++u) {
             const int a = wave + (SW - 1) * u;
             const bool live = a <= D && J >= 0 && J + a < nblk;
-            const zd *Lt = panel + ((size_t)(live ? J : 0) * (D + 1) + (live ? a : 0)) * 256;
+            const PT *Lt = panel + ((size_t)(live ? J : 0) * (D + 1) + (live ? a : 0)) * 256;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const zd *p = Lt + (rq + 4 * v) * T16 + c;
+                const PT *p = Lt + (rq + 4 * v) * T16 + c;
                 t[u][v] = live ? zd{p->re, p->im} : zd{0.0, 0.0};
             }
         }
@@ -1451,34 +1504,51 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
 
 namespace {
 
-template <int D>
-int launch_solve16(hipStream_t st, const Solve16Args &a) {
-    hipLaunchKernelGGL(crt_solve16_kernel<D>, dim3(a.nrhs), dim3(64 * SW), 0, st, a);
+template <int D, class PT, int MODE>
+int launch_solve16(hipStream_t st, const SolveArgs<PT> &a) {
+    hipLaunchKernelGGL((crt_solve16_kernel<D, PT, MODE>), dim3(a.nrhs), dim3(64 * SW), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+template <class PT, int MODE>
+int solve16_dispatch(hipStream_t st, int D, const SolveArgs<PT> &a) {
+    switch (D) {
+        case 1: return launch_solve16<1, PT, MODE>(st, a);
+        case 2: return launch_solve16<2, PT, MODE>(st, a);
+        case 3: return launch_solve16<3, PT, MODE>(st, a);
+        case 4: return launch_solve16<4, PT, MODE>(st, a);
+        case 5: return launch_solve16<5, PT, MODE>(st, a);
+        case 6: return launch_solve16<6, PT, MODE>(st, a);
+        case 7: return launch_solve16<7, PT, MODE>(st, a);
+        case 8: return launch_solve16<8, PT, MODE>(st, a);
+        case 9: return launch_solve16<9, PT, MODE>(st, a);
+        case 10: return launch_solve16<10, PT, MODE>(st, a);
+        case 11: return launch_solve16<11, PT, MODE>(st, a);
+        case 12: return launch_solve16<12, PT, MODE>(st, a);
+        case 13: return launch_solve16<13, PT, MODE>(st, a);
+        case 14: return launch_solve16<14, PT, MODE>(st, a);
+        case 15: return launch_solve16<15, PT, MODE>(st, a);
+        case 16: return launch_solve16<16, PT, MODE>(st, a);
+        default: return CRT_ERR_UNSUPPORTED;
+    }
 }
 }  // namespace
 
 int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs, const int4 *rhs,
                        const int *enr, const double *fak, const zd *panels, const zd *dinv, const zd *y, zd *yscr,
                        zd *kpotN) {
-    Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN};
-    switch (D) {
-        case 1: return launch_solve16<1>(st, a);
-        case 2: return launch_solve16<2>(st, a);
-        case 3: return launch_solve16<3>(st, a);
-        case 4: return launch_solve16<4>(st, a);
-        case 5: return launch_solve16<5>(st, a);
-        case 6: return launch_solve16<6>(st, a);
-        case 7: return launch_solve16<7>(st, a);
-        case 8: return launch_solve16<8>(st, a);
-        case 9: return launch_solve16<9>(st, a);
-        case 10: return launch_solve16<10>(st, a);
-        case 11: return launch_solve16<11>(st, a);
-        case 12: return launch_solve16<12>(st, a);
-        case 13: return launch_solve16<13>(st, a);
-        case 14: return launch_solve16<14>(st, a);
-        case 15: return launch_solve16<15>(st, a);
-        case 16: return launch_solve16<16>(st, a);
-        default: return CRT_ERR_UNSUPPORTED;
-    }
+    Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, nullptr, nullptr};
+    return solve16_dispatch<zd, 0>(st, D, a);
+}
+
+// substitution through the FP32 panels for the refinement: mode 1 is the first
+// solve (x into xacc), mode 2 a sweep (negated residual in yscr, x added to xacc,
+// the max-norms of the correction and of the sum into stats)
+int crt_launch_solve16_f32(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs,
+                           const int4 *rhs, const int *enr, const double *fak, const zf *panels, const zf *dinv,
+                           const zf *y, zd *yscr, zd *kpotN, zd *xacc, double *stats, int mode) {
+    SolveArgs<zf> a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, xacc, stats};
+    if (mode == 1) return solve16_dispatch<zf, 1>(st, D, a);
+    if (mode == 2) return solve16_dispatch<zf, 2>(st, D, a);
+    return CRT_ERR_ARG;
 }

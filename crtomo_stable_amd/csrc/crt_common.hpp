@@ -89,6 +89,11 @@ __host__ __device__ __forceinline__ zd zsqrt(zd z) {
     return zd{fabs(z.im) / (2.0 * t), copysign(t, z.im)};
 }
 
+// complex FP32 (panels of the mixed-precision factorisation)
+struct zf {
+    float re, im;
+};
+
 // ---------------------------------------------------------------- geometry
 // Band tile used by the FP64 factorisation: rows are grouped into blocks of
 // CRT_NB rows; the factor is stored as block-column panels

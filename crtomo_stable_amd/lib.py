@@ -280,6 +280,19 @@ class Context:
         the library) and write volt.dat when the measurements switch is set."""
         _chk(self.lib.crt_tomodir_crmod(self._h, cfg_path.encode()), "crt_tomodir_crmod")
 
+    def set_forward_precision(self, bits: int, tol: float = 1e-14, max_sweeps: int = 12):
+        """64: FP64 band factorisation (default); 32: FP32 factorisation on the matrix
+        cores with FP64 iterative refinement of every system to ``tol``."""
+        _chk(self.lib.crt_set_forward_precision(self._h, int(bits), C.c_double(tol), int(max_sweeps)),
+             "crt_set_forward_precision")
+
+    def refine_info(self):
+        """(sweeps, final correction ratio, batches redone in FP64) of the last forward."""
+        sw, fb = C.c_int(), C.c_int()
+        ratio = C.c_double()
+        _chk(self.lib.crt_get_refine_info(self._h, C.byref(sw), C.byref(ratio), C.byref(fb)), "crt_get_refine_info")
+        return sw.value, ratio.value, fb.value
+
     def set_cg_precision(self, jbits: int):
         """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""
         _chk(self.lib.crt_set_cg_precision(self._h, int(jbits)), "crt_set_cg_precision")
@@ -330,9 +343,9 @@ class Context:
         return err, a, p
 
     def times(self):
-        ms = np.zeros(8)
-        self.lib.crt_get_times(self._h, _ptr(ms), 8)
-        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec"), ms))
+        ms = np.zeros(9)
+        self.lib.crt_get_times(self._h, _ptr(ms), 9)
+        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine"), ms))
 
 
 class Batch:

@@ -187,9 +187,23 @@ int crt_chol(crt_ctx *ctx, int sanz, int mb, double *a);
 int crt_vre(crt_ctx *ctx, int sanz, int mb, const double *a, const double *b, double *pot);
 int crt_cholz(crt_ctx *ctx, int n, double *a, double *p);
 
+/* Forward precision (no counterpart in the reference, whose chol/vre run in
+ * COMPLEX*16, src/chol.f90:7, src/vre.f90:5).  bits 64 (default): FP64 band
+ * factorisation.  bits 32: the factorisation runs in complex FP32 on the matrix
+ * cores and every (electrode, wavenumber) system is refined in FP64 (residual
+ * A x - g from the FP64 matrix, correction through the FP32 factor) until the
+ * correction's max-norm is below tol times the potentials' (or stalls below
+ * max(tol, 1e-12)), at most max_sweeps sweeps; a batch that does not converge
+ * is redone in FP64.  Needs a band of 33..256 nodes (3..16 tiles of 16): 903 otherwise.
+ * crt_get_refine_info: most sweeps of any batch, largest final correction
+ * ratio, batches redone in FP64, all of the last crt_forward. */
+int crt_set_forward_precision(crt_ctx *ctx, int bits, double tol, int max_sweeps);
+int crt_get_refine_info(crt_ctx *ctx, int *sweeps, double *ratio, int *fallbacks);
+
 /* Device time of the last call of each phase, milliseconds:
  * [0] diag, [1] factor, [2] solve, [3] voltages, [4] sensitivities,
- * [5] precal, [6] cg total, [7] cg matvec (mean per iteration). */
+ * [5] precal, [6] cg total, [7] cg matvec (mean per iteration),
+ * [8] refinement sweeps of the FP32 forward (total). */
 int crt_get_times(crt_ctx *ctx, double *ms, int n);
 
 /* Batched forward of many models of uneven size (crt_forward of every context,
