@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
@@ -103,6 +104,7 @@ def gn_step(ctx, w, ltri: int):
     squared residual (relative residual 1e-6), ncgmax = manz / 10
     (src/rall.f90:735-738).  Returns phase times in ms and the CG figures."""
     import numpy as np
+    from crtomo_stable_amd import lib
     g = w.grid
     ctx.set_sigma(w.sigma)
     ctx.forward()
@@ -122,13 +124,15 @@ def gn_step(ctx, w, ltri: int):
     t0 = time.perf_counter()
     ctx.set_sigma(sig0)
     ctx.forward()
-    ctx.voltages(1)
+    v64, _ = ctx.voltages(1)
     device_sync()
     t["forward"] = time.perf_counter() - t0
+    tmf = ctx.times()
     t1 = time.perf_counter()
     ctx.sens(1)
     device_sync()
     t["sens"] = time.perf_counter() - t1
+    s64 = ctx.get_sens()
     t1 = time.perf_counter()
     ctx.bsmatm(ltri, 1.0, 1.0, nx=g.nx, nz=getattr(g, "nz", 0))
     lam = ctx.blam0(wmatd, wdfak, 1.0, 1.0)
@@ -180,6 +184,44 @@ def gn_step(ctx, w, ltri: int):
                        "pc_train_ms": round(1e3 * t_train, 1), "pc_loss": [l0, l1]}
     ctx.set_cg_precision(64)
     ctx.set_cg_preconditioner(0)
+    # the same step with the mixed-precision forward (FP32 factorisation + FP64
+    # refinement), FP64 Jacobi CG; errors against the FP64 step per voltage,
+    # sensitivity row and model update
+    mixed = None
+    try:
+        ctx.set_forward_precision(32)
+    except lib.CRTError as e:
+        mixed = {"unsupported": str(e)}
+    if mixed is None:
+        try:
+            t1 = time.perf_counter()
+            ctx.set_sigma(sig0)
+            ctx.forward()
+            v32, _ = ctx.voltages(1)
+            device_sync()
+            tf = time.perf_counter() - t1
+            sweeps, ratio, fallbacks = ctx.refine_info()
+            tm32 = ctx.times()
+            t1 = time.perf_counter()
+            ctx.sens(1)
+            device_sync()
+            tsn = time.perf_counter() - t1
+            s32 = ctx.get_sens()
+        finally:
+            ctx.set_forward_precision(64)
+        t1 = time.perf_counter()
+        r32 = ctx.update(lam, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+        device_sync()
+        tu = time.perf_counter() - t1
+        srow = np.abs(s32 - s64).max(axis=1) / np.abs(s64).max(axis=1)
+        mixed = {"forward_ms": round(1e3 * tf, 3), "factor_ms": round(tm32["factor"], 3),
+                 "first_solve_ms": round(tm32["solve"], 3), "refine_ms": round(tm32["refine"], 3),
+                 "sweeps": sweeps, "final_correction_ratio": ratio, "fp64_fallback_batches": fallbacks,
+                 "sens_ms": round(1e3 * tsn, 3), "update_ms": round(1e3 * tu, 3), "ncg": r32["ncg"],
+                 "volt_rel_err_vs_fp64": float(np.max(np.abs(v32 - v64) / np.abs(v64))),
+                 "sens_row_rel_err_vs_fp64": float(srow.max()),
+                 "dpar_rel_diff_vs_fp64": float(np.max(np.abs(r32["dpar"] - d64)) / np.max(np.abs(d64))),
+                 "step_ms": round(1e3 * (tf + tsn + t["bsmatm_blam0"] + tu), 3)}
     nanz = v_true.size
     j_bytes = 16.0 * nanz * manz
     it_ms = tm["cg_matvec"]
@@ -187,7 +229,9 @@ def gn_step(ctx, w, ltri: int):
             "ncgmax": ncgmax, "cg_rel_residual": float(np.sqrt(r["cgres"][r["ncg"]])) if r["ncg"] > 0 else None,
             "cg_ms_per_iteration": round(it_ms, 4),
             "cg_matvec_GBps": round(2.0 * j_bytes / (it_ms * 1e-3) / 1e9, 1) if it_ms > 0 else None,
-            "J_bytes": j_bytes, "cg_forms": forms}
+            "J_bytes": j_bytes, "cg_forms": forms, "forward_device_ms": {k: round(tmf[k], 3) for k in
+                                                                       ("diag", "factor", "solve", "volt")},
+            "mixed_precision_forward": mixed}
 
 
 def bench_c2(args):
@@ -296,6 +340,8 @@ def main():
                     help="only copy 512 MiB three times (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--gn", action="store_true",
                     help="also time one Gauss-Newton step (forward, bsensi, bsmatm, blam0, update/cjg)")
+    ap.add_argument("--forward-precision", type=int, default=64, choices=[64, 32],
+                    help="32: FP32 band factorisation with FP64 iterative refinement in the timed step")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-sens-meas", type=int, default=96)
     ap.add_argument("--cpu-fwd-electrodes", type=int, default=4)
@@ -332,6 +378,8 @@ def main():
     ctx.set_sigma(w.sigma)
     K, E = kwn.size, g.eanz
     nsys = K * E if lbeta else K
+    if args.forward_precision == 32:
+        ctx.set_forward_precision(32)
 
     def step():
         ctx.forward()
@@ -341,7 +389,7 @@ def main():
     for _ in range(args.warmup):
         step()
     device_sync()
-    acc = dict(diag=0.0, factor=0.0, solve=0.0, volt=0.0, sens=0.0)
+    acc = dict(diag=0.0, factor=0.0, solve=0.0, refine=0.0, volt=0.0, sens=0.0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -358,14 +406,18 @@ def main():
     fac_ms = phase["factor"]
     achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
     d16 = -(-g.mb // 16)
-    if os.environ.get("CRT_FACTOR8") or d16 > 16:
+    peak = FP64_PEAK_TFLOPS
+    if args.forward_precision == 32:
+        kernel = f"crt_factor32st_kernel<{d16}>"
+        peak = FP32_PEAK_TFLOPS
+    elif os.environ.get("CRT_FACTOR8") or d16 > 16:
         kernel = "crt_factor_kernel"
     elif d16 == 1 or os.environ.get("CRT_FACTOR_V2") and d16 <= 10:
         kernel = f"crt_factor16_kernel<{d16}>"
     else:
         kernel = f"crt_factor16{'la' if d16 <= 10 else 'st'}_kernel<{d16}>"
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP64_PEAK_TFLOPS, 4),
+    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4),
                     traffic=measured_traffic(kernel, args.workload),
                     kernel=kernel, flop_per_system=flop_sys, systems_per_launch=nsys,
                     launch_ms=round(fac_ms, 4))
@@ -396,6 +448,11 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "roofline": roofline,
     }
+    if args.forward_precision == 32:
+        sweeps, ratio, fallbacks = ctx.refine_info()
+        out["dtype"] = "c64 factorisation, c128 refinement"
+        out["refinement"] = {"sweeps": sweeps, "final_correction_ratio": ratio, "fp64_fallback_batches": fallbacks}
+        ctx.set_forward_precision(64)
     if args.gn:
         out["gn_step"] = gn_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
     if not args.no_cpu_baseline:

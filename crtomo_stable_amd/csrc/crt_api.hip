@@ -36,6 +36,9 @@ int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const
                             const double *, const zd *, const zd *, zd *);
 int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                            const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
+// model uncertainty (crt_bmcm.hip)
+int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, const double *, const int *, double,
+                 int, double *, double *, double *, double *, double *, double *, int *, int *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
@@ -1624,6 +1627,48 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if (cgres) CRT_HIP(c->copy_out(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double)));
     if (ncg) *ncg = n;
     return 0;
+}
+
+int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, int lres, int lcov2, int gauss,
+                double *ata, double *ata_reg, double *cov, double *res, double *cov2, double *diag) {
+    if (!c || !wmatd || !wdfak) return CRT_ERR_ARG;
+    if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int e = cg_weights(c, nanz, wmatd, wdfak);
+    if (e) return e;
+    const size_t n2 = (size_t)manz * manz;
+    const bool want_res = lres != 0, want_cov2 = lres != 0 && lcov2 != 0;
+    // dense work buffers live for this call only (five manz^2 matrices at most)
+    DBuf<double> d_ata, d_reg, d_cov, d_res, d_cov2, d_z;
+    DBuf<int> d_info, d_ipiv;
+    e = d_ata.ensure(n2);
+    e = e ? e : d_reg.ensure(n2);
+    e = e ? e : d_cov.ensure(n2);
+    e = e ? e : (want_res ? d_res.ensure(n2) : 0);
+    e = e ? e : (want_cov2 ? d_cov2.ensure(n2) : 0);
+    e = e ? e : d_z.ensure(2 * (size_t)nanz * manz);
+    e = e ? e : d_info.ensure(1);
+    e = e ? e : d_ipiv.ensure(manz);
+    if (e) return e;
+    e = crt_bmcm_run(c->stream, nanz, manz, c->d_sens.p, c->d_w.p, c->ltri, c->rnx, c->rncol - 1, c->d_smatm.p,
+                     c->d_nachbar.p, lam, gauss ? 1 : 0, d_ata.p, d_reg.p, d_cov.p, want_res ? d_res.p : nullptr,
+                     want_cov2 ? d_cov2.p : nullptr, d_z.p, d_info.p, d_ipiv.p);
+    const bool have_cov = e == 0;
+    if (e && e != 108) return e;
+    const double *mats[5] = {d_ata.p, d_reg.p, have_cov ? d_cov.p : nullptr, have_cov && want_res ? d_res.p : nullptr,
+                             have_cov && want_cov2 ? d_cov2.p : nullptr};
+    double *outs[5] = {ata, ata_reg, cov, res, cov2};
+    for (int q = 0; q < 5; ++q) {
+        if (!mats[q]) continue;
+        if (outs[q]) CRT_HIP(c->copy_out(outs[q], mats[q], n2 * sizeof(double)));
+        if (diag) {
+            CRT_HIP(hipMemcpy2DAsync(diag + (size_t)q * manz, sizeof(double), mats[q], (manz + 1) * sizeof(double),
+                                     sizeof(double), manz, hipMemcpyDeviceToHost, c->stream));
+            CRT_HIP(hipStreamSynchronize(c->stream));
+        }
+    }
+    return e;
 }
 
 int crt_blam0(crt_ctx *c, int fpi, const double *wmatd, const int *wdfak, double alfx, double alfz,

@@ -90,7 +90,7 @@ __host__ __device__ __forceinline__ zd zsqrt(zd z) {
 }
 
 // complex FP32 (panels of the mixed-precision factorisation)
-struct zf {
+struct alignas(8) zf {
     float re, im;
 };
 

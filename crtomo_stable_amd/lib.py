@@ -221,6 +221,26 @@ class Context:
              "crt_set_regularization")
         self.manz = manz
 
+    def buncert(self, lam, wmatd, wdfak, lres=True, lcov2=True, gauss=False, full=True):
+        """Model uncertainty (src/bmcm_mod.f90:49-175) on the device: (errnr, dict of
+        ata, ata_reg, cov, res, cov2 as manz x manz arrays when ``full``, and their main
+        diagonals under "diag_<name>").  errnr 108: the factorisation rejected ata_reg."""
+        manz = self.manz
+        w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)
+        names = ["ata", "ata_reg", "cov", "res", "cov2"]
+        want = [True, True, True, bool(lres), bool(lres and lcov2)]
+        mats = [np.zeros((manz, manz)) if (full and wq) else None for wq in want]
+        diag = np.full((5, manz), np.nan)
+        err = self.lib.crt_buncert(self._h, C.c_double(lam), _ptr(w), _ptr(wd), int(bool(lres)), int(bool(lcov2)),
+                                   int(bool(gauss)), *[_ptr(m) for m in mats], _ptr(diag))
+        if err not in (0, 108):
+            raise CRTError(err, "crt_buncert")
+        out = {}
+        for q, (k, m) in enumerate(zip(names, mats)):
+            out[k] = None if m is None else m.T  # column-major storage
+            out["diag_" + k] = diag[q] if want[q] else None
+        return err, out
+
     def bsmatm(self, ltri, alfx=1.0, alfz=1.0, nx=0, nz=0):
         """Regularisation built on the device (bsmatmreg for ltri 0 on the regular
         nx x nz grid, bnachbar + bsmatmtri for ltri 1/2) and installed for the CG;

@@ -179,6 +179,21 @@ int crt_write_volt(const char *path, int nanz, const int *strnr, const int *vnr,
 int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel,
                int elanz, int *perm, int *mb);
 
+/* Linearised model uncertainty of the complex inversion, buncert
+ * (src/bmcm_mod.f90:49-175): ata = Re(J^H W J) (bata), ata_reg = ata + lam R in
+ * the reference's own fill order (bata_reg; ltri 0, 1, 2), cov = ata_reg^-1 by
+ * Cholesky (chold + linvd) or, with gauss, by LU (the Gauss_dble branch), res =
+ * cov ata (bres, when lres) and cov2 = res cov (bmcm2, when lres and lcov2).
+ * Uses J on the device (crt_sens / crt_set_sens) and the regularisation of
+ * crt_set_regularization / crt_bsmatm; W = wmatd * wdfak.  Each output is a
+ * manz x manz column-major host array or NULL; diag (NULL or 5*manz) gets the
+ * five main diagonals in that order (what the reference writes to ata.diag,
+ * ata_reg.diag, cov1_m.diag, res_m.diag, cov2_m.diag).  Returns 108 when the
+ * factorisation rejects ata_reg (the reference's errnr; ata and ata_reg are
+ * still returned). */
+int crt_buncert(crt_ctx *ctx, double lam, const double *wmatd, const int *wdfak, int lres, int lcov2, int gauss,
+                double *ata, double *ata_reg, double *cov, double *res, double *cov2, double *diag);
+
 /* Single-system drop-ins for the named Fortran routines, host arrays in/out:
  * chol(a) (src/chol.f90:7), vre(a,b,pot) (src/vre.f90:5), cholz(a,p,n,...)
  * (src/cholz.f90:1).  Band layout: entry (i,j), j<=i, 1-based, at
