@@ -132,7 +132,6 @@ def gn_step(ctx, w, ltri: int):
     ctx.sens(1)
     device_sync()
     t["sens"] = time.perf_counter() - t1
-    s64 = ctx.get_sens()
     t1 = time.perf_counter()
     ctx.bsmatm(ltri, 1.0, 1.0, nx=g.nx, nz=getattr(g, "nz", 0))
     lam = ctx.blam0(wmatd, wdfak, 1.0, 1.0)
@@ -144,6 +143,7 @@ def gn_step(ctx, w, ltri: int):
     t["update"] = time.perf_counter() - t1
     t["step"] = time.perf_counter() - t0
     tm = ctx.times()
+    s64 = ctx.get_sens()  # host copy for the mixed-precision comparison, outside the step
     # the same update in the other CG forms: complex FP32 copy of J in the iterations,
     # and the FSAI preconditioner (pattern 2) on top of the Jacobi scaling
     d64 = r["dpar"]
@@ -232,6 +232,29 @@ def gn_step(ctx, w, ltri: int):
             "J_bytes": j_bytes, "cg_forms": forms, "forward_device_ms": {k: round(tmf[k], 3) for k in
                                                                        ("diag", "factor", "solve", "volt")},
             "mixed_precision_forward": mixed}
+
+
+def buncert_step(ctx, w, ltri: int):
+    """Model uncertainty (src/bmcm_mod.f90:49-175) on the step's Jacobian with unit
+    data weights and the device regularisation: Re(J^H W J) (dsyrk), the reference's
+    regularisation fill, the inverse by Cholesky, res = cov ata and cov2 = res cov;
+    only the five diagonals come back to the host."""
+    import numpy as np
+    g = w.grid
+    nanz, manz = w.survey.nanz, g.elanz
+    ctx.bsmatm(ltri, 1.0, 1.0, nx=getattr(g, "nx", 0), nz=getattr(g, "nz", 0))
+    wmatd = np.ones(nanz); wdfak = np.ones(nanz, np.int32)
+    lam = 1.0
+    device_sync()
+    t0 = time.perf_counter()
+    err, d = ctx.buncert(lam, wmatd, wdfak, gauss=False, full=False)
+    device_sync()
+    t = time.perf_counter() - t0
+    flop = 2.0 * nanz * manz * manz + manz ** 3 / 3.0 + 2.0 * manz ** 3 / 3.0 + 2 * 2.0 * manz ** 3
+    return {"ms": round(1e3 * t, 1), "errnr": err, "manz": manz, "nanz": nanz, "ltri": ltri,
+            "flop": flop, "TFLOPs": round(flop / t / 1e12, 2),
+            "note": "dsyrk 2 nanz manz^2 (as a full product count) + potrf manz^3/3 + potri 2 manz^3/3 + "
+                    "two dgemm 2 manz^3, FP64; timed with host transfers of the diagonals only"}
 
 
 def bench_c2(args):
@@ -342,6 +365,9 @@ def main():
                     help="also time one Gauss-Newton step (forward, bsensi, bsmatm, blam0, update/cjg)")
     ap.add_argument("--forward-precision", type=int, default=64, choices=[64, 32],
                     help="32: FP32 band factorisation with FP64 iterative refinement in the timed step")
+    ap.add_argument("--buncert", action="store_true",
+                    help="also time the model uncertainty (crt_buncert: Re(J^H W J), +lam R, inverse, "
+                         "resolution, cov2) on the step's Jacobian")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--cpu-sens-meas", type=int, default=96)
     ap.add_argument("--cpu-fwd-electrodes", type=int, default=4)
@@ -455,6 +481,8 @@ def main():
         ctx.set_forward_precision(64)
     if args.gn:
         out["gn_step"] = gn_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
+    if args.buncert:
+        out["buncert"] = buncert_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
     if not args.no_cpu_baseline:
         try:
             cb = cpu_baseline(w, args.cpu_threads, args.cpu_sens_meas, args.cpu_fwd_electrodes)
