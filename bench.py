@@ -95,6 +95,38 @@ def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
                 forward_s=fwd_full, sens_s=sens_full)
 
 
+def other_rooflines(g, s, K, E, nsys, phase, bits):
+    """Algorithmic HBM bytes over the measured phase time (HIP events on the library's
+    stream) for the HBM-bound kernels of one step: substitution, assembly (diagonal
+    and value passes) and the Jacobian.  Bytes per unit as in the design notes:
+      substitution: the factor's band read once, 16 (mb+1) sanz per right-hand side (8
+        with FP32 panels), and the kpot write 16 sanz;
+      assembly: elbg slice 8 elanz npk, sigma 16 elanz, the CSR values 16 nnz (nnz: the
+        lower-triangle entries incl. the diagonal), per system;
+      Jacobian: J write 16 nanz elanz, kpot 16 sanz E K, elbg 8 elanz npk K."""
+    nrhs = E * K
+    pb = 8 if bits == 32 else 16
+    solve_b = nrhs * (pb * (g.mb + 1) * g.sanz + 16 * g.sanz)
+    import numpy as np
+    el = np.asarray(g.nrel[:g.elanz], np.int64)
+    pairs = [np.maximum(el[:, a], el[:, c]) * (g.sanz + 1) + np.minimum(el[:, a], el[:, c])
+             for a in range(el.shape[1]) for c in range(a + 1)]
+    pk = np.concatenate(pairs)
+    nnz = np.unique(pk[(pk % (g.sanz + 1)) > 0]).size
+    asm_b = nsys * (8 * g.elanz * g.npk + 16 * g.elanz + 16 * nnz)
+    sens_b = 16.0 * s.nanz * g.elanz + 16.0 * g.sanz * E * K + 8.0 * g.elanz * g.npk * K
+    out = {}
+    for name, key, b in (("substitution (crt_solve16_kernel, first solve)", "solve", solve_b),
+                         ("assembly (crt_diag_kernel + crt_assemble_kernel)", "diag", asm_b),
+                         ("Jacobian (crt_sens_gm_kernel)", "sens", sens_b)):
+        ms = phase.get(key, 0.0)
+        if ms > 0:
+            gbs = b / (ms * 1e-3) / 1e9
+            out[name] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=round(gbs / HBM_PEAK_GBS, 4), bytes=b, ms=round(ms, 4))
+    return out
+
+
 def gn_step(ctx, w, ltri: int):
     """One complex Gauss-Newton step of the inversion loop (src/inv.f90: forward on
     the current model, bvolti, bsensi, then update -> cjg) on the device, with the
@@ -473,6 +505,7 @@ def main():
         "forward_systems_per_s": round(nsys * 1e3 / ms_step, 1),
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "roofline": roofline,
+        "roofline_other_kernels": other_rooflines(g, s, K, E, nsys, phase, args.forward_precision),
     }
     if args.forward_precision == 32:
         sweeps, ratio, fallbacks = ctx.refine_info()

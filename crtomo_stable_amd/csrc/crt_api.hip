@@ -24,6 +24,7 @@ int crt_launch_solve(hipStream_t, int, int, int, int, int, int, const int4 *, co
 int crt_launch_panel_band(hipStream_t, int, int, int, int, int, zd *, zd *);
 int crt_launch_factor_band(hipStream_t, const DevGrid &, const int2 *, const zd *, zd *, zd *, int *);
 int crt_factor16_supported(int mb);
+int crt_factor16_resident(int D, int variant);
 int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zd *, zd *,
                         zd *, int *, int);
 int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
@@ -903,6 +904,13 @@ int crt_forward(crt_ctx *c) {
     size_t per_sys = panel_elems * sizeof(zd) + fstride * (2 * sizeof(zd) + 2 * sizeof(double)) +
                      (size_t)c->nnz * sizeof(zd);
     size_t batch = std::max<size_t>(1, std::min(sys.size(), c->panel_budget / per_sys));
+    if (use16 && !fp32 && batch < sys.size()) {
+        // one workgroup per system: a launch takes ceil(nb / resident) rounds, so batches
+        // that are whole multiples of the resident count leave no partly filled round
+        // but the last (C3: 1344 systems as 512 + 512 + 320 instead of 672 + 672)
+        const size_t res = (size_t)crt_factor16_resident(D, 0);
+        if (res > 0 && batch >= res) batch = batch / res * res;
+    }
     int e = c->d_panels.ensure(batch * panel_elems);
     e = e ? e : c->d_yf.ensure(batch * fstride);
     e = e ? e : c->d_dinv.ensure(batch * fstride);

@@ -1199,6 +1199,55 @@ static int launch16(hipStream_t st, const Factor16Args &a, int variant) {
     return CRT_ERR_UNSUPPORTED;
 }
 
+// workgroups of the factor kernel that launch16 picks for (D, variant) resident on the
+// whole device at once, 0 when unknown
+template <int D>
+static int resident16(int variant) {
+    if (variant == 0) {
+        static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
+        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
+        variant = (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
+    }
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (variant == 2) {
+        if constexpr (D >= SNU)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, crt_factor16st_kernel<D>, SWV * 64, 0);
+    } else if (variant == 1) {
+        if constexpr (D >= 2 && D <= 10)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, crt_factor16la_kernel<D>, La4<D>::NT, 0);
+    } else if (variant == 3) {
+        if constexpr (D <= 10)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, crt_factor16_kernel<D>, Cfg<D>::NT, 0);
+    }
+    if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+
+int crt_factor16_resident(int D, int variant) {
+    switch (D) {
+        case 1: return resident16<1>(variant);
+        case 2: return resident16<2>(variant);
+        case 3: return resident16<3>(variant);
+        case 4: return resident16<4>(variant);
+        case 5: return resident16<5>(variant);
+        case 6: return resident16<6>(variant);
+        case 7: return resident16<7>(variant);
+        case 8: return resident16<8>(variant);
+        case 9: return resident16<9>(variant);
+        case 10: return resident16<10>(variant);
+        case 11: return resident16<11>(variant);
+        case 12: return resident16<12>(variant);
+        case 13: return resident16<13>(variant);
+        case 14: return resident16<14>(variant);
+        case 15: return resident16<15>(variant);
+        case 16: return resident16<16>(variant);
+        default: return 0;
+    }
+}
+
 int crt_factor16_supported(int mb) {
     int D = (mb + 15) / 16;
     return D >= 1 && D <= CRT_F16_MAXD;
