@@ -325,11 +325,22 @@ struct crt_batch {
         DBuf<int2> sys;
         DBuf<int4> rhs;
         DBuf<int> status;  // 1 + nsys
+        int slot = 0;      // workspace slot and stream
+    };
+    // One workspace slot per stream: model i runs on slot i % nslot, so the partly
+    // filled last round of one model's launches overlaps the next model's on another
+    // stream.  Slot 0 uses the batch's own stream.
+    struct Slot {
+        hipStream_t st = nullptr;
+        hipEvent_t join = nullptr;
+        zd *panels = nullptr, *yf = nullptr, *dinv = nullptr, *vals = nullptr, *yscr = nullptr;
+        double *fak = nullptr, *dga = nullptr;
     };
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr;
     std::vector<std::unique_ptr<Model>> m;
+    std::vector<Slot> slots;
     DBuf<zd> panels, yf, dinv, vals, yscr;
     DBuf<double> fak, dga;
     hipGraph_t graph = nullptr;
@@ -343,6 +354,11 @@ struct crt_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (fork) (void)hipEventDestroy(fork);
+        for (size_t i = 0; i < slots.size(); ++i) {
+            if (slots[i].join) (void)hipEventDestroy(slots[i].join);
+            if (i > 0 && slots[i].st) (void)hipStreamDestroy(slots[i].st);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -380,6 +396,11 @@ int batch_plan(crt_batch::Model &M, size_t ws_bytes) {
     const size_t per_sys = M.panel_elems * sizeof(zd) + M.fstride * (2 * sizeof(zd) + 2 * sizeof(double)) +
                            ((size_t)c->nnz + 2) * sizeof(zd);
     M.batch = std::max<size_t>(1, std::min(M.nsys, ws_bytes / per_sys));
+    if (M.use16 && M.batch < M.nsys) {
+        // whole multiples of the resident workgroup count (see crt_forward)
+        const size_t res = (size_t)crt_factor16_resident(M.D, 0);
+        if (res > 0 && M.batch >= res) M.batch = M.batch / res * res;
+    }
     M.nyscr = c->lbeta ? 1 : M.nrhs;
     std::vector<int4> rall;
     M.chunks.clear();
@@ -407,30 +428,33 @@ DevGrid batch_grid(const crt_batch::Model &M) {
     return g;
 }
 
-// diagonal scaling and assembly of one chunk into the shared workspace
+// diagonal scaling and assembly of one chunk into the model's workspace slot
 int batch_assemble(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
     const DevGrid g = batch_grid(M);
-    int e = crt_launch_diag(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, M.status.p + 1 + ch.s0);
-    return e ? e : crt_launch_assemble(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, b->vals.p);
+    const crt_batch::Slot &w = b->slots[M.slot];
+    int e = crt_launch_diag(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.dga, M.status.p + 1 + ch.s0);
+    return e ? e : crt_launch_assemble(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.dga, w.vals);
 }
 
 int batch_factor(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch, int variant) {
     const DevGrid g = batch_grid(M);
+    const crt_batch::Slot &w = b->slots[M.slot];
     if (M.use16)
-        return crt_launch_factor16(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->vals.p, b->panels.p, b->yf.p,
-                                   b->dinv.p, M.status.p + 1 + ch.s0, variant);
-    return crt_launch_factor(b->stream, g, ch.nb, M.sys.p + ch.s0, b->fak.p, b->dga.p, b->vals.p, b->panels.p,
-                             b->yf.p, M.status.p + 1 + ch.s0);
+        return crt_launch_factor16(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.vals, w.panels, w.yf, w.dinv,
+                                   M.status.p + 1 + ch.s0, variant);
+    return crt_launch_factor(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.dga, w.vals, w.panels, w.yf,
+                             M.status.p + 1 + ch.s0);
 }
 
 int batch_solve(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
     crt_ctx *c = M.c;
     const int K = c->kwnanz, E = c->eanz;
+    const crt_batch::Slot &w = b->slots[M.slot];
     if (M.use16)
-        return crt_launch_solve16(b->stream, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, b->fak.p,
-                                  b->panels.p, b->dinv.p, b->yf.p, b->yscr.p, c->d_kpotN.p);
-    return crt_launch_solve(b->stream, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, b->fak.p,
-                            b->panels.p, b->yf.p, b->yscr.p, c->d_kpotN.p, nullptr, nullptr);
+        return crt_launch_solve16(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak,
+                                  w.panels, w.dinv, w.yf, w.yscr, c->d_kpotN.p);
+    return crt_launch_solve(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak, w.panels,
+                            w.yf, w.yscr, c->d_kpotN.p, nullptr, nullptr);
 }
 
 // time each factor kernel variant that exists for band width D on the first chunk
@@ -453,10 +477,10 @@ int batch_tune(crt_batch *b) {
             for (int rep = 0; rep < 2; ++rep) {
                 int e = batch_assemble(b, *M, ch);
                 if (e) return e;
-                CRT_HIP(hipEventRecord(b->e0, b->stream));
+                CRT_HIP(hipEventRecord(b->e0, b->slots[M->slot].st));
                 e = batch_factor(b, *M, ch, cand[i]);
                 if (e) return e;
-                CRT_HIP(hipEventRecord(b->e1, b->stream));
+                CRT_HIP(hipEventRecord(b->e1, b->slots[M->slot].st));
                 CRT_HIP(hipEventSynchronize(b->e1));
                 float ms = 0.f;
                 CRT_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
@@ -1962,12 +1986,23 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
     CRT_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     CRT_HIP(hipEventCreate(&b->e0));
     CRT_HIP(hipEventCreate(&b->e1));
+    CRT_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     if (workspace_bytes == 0) workspace_bytes = (size_t)64 << 30;
+    int nslot = 2;
+    if (const char *v = getenv("CRT_BATCH_STREAMS")) nslot = std::max(1, std::min(8, atoi(v)));
+    nslot = std::min(nslot, n);
+    b->slots.resize(nslot);
+    for (int q = 0; q < nslot; ++q) {
+        if (q == 0) b->slots[q].st = b->stream;
+        else CRT_HIP(hipStreamCreateWithFlags(&b->slots[q].st, hipStreamNonBlocking));
+        CRT_HIP(hipEventCreateWithFlags(&b->slots[q].join, hipEventDisableTiming));
+    }
     size_t np = 0, nf = 0, nv = 0, ny = 0;
     for (int i = 0; i < n; ++i) {
         auto M = std::make_unique<crt_batch::Model>();
         M->c = ctxs[i];
-        int e = batch_plan(*M, workspace_bytes);
+        M->slot = i % nslot;
+        int e = batch_plan(*M, workspace_bytes / nslot);
         if (e) return e;
         np = std::max(np, M->batch * M->panel_elems);
         nf = std::max(nf, M->batch * M->fstride);
@@ -1976,16 +2011,27 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
         b->nsys_total += M->nsys;
         b->m.push_back(std::move(M));
     }
-    int e = b->panels.ensure(np);
-    e = e ? e : b->yf.ensure(nf);
-    e = e ? e : b->dinv.ensure(nf);
-    e = e ? e : b->fak.ensure(nf);
-    e = e ? e : b->dga.ensure(nf);
-    e = e ? e : b->vals.ensure(nv);
-    e = e ? e : b->yscr.ensure(ny);
+    int e = b->panels.ensure(np * nslot);
+    e = e ? e : b->yf.ensure(nf * nslot);
+    e = e ? e : b->dinv.ensure(nf * nslot);
+    e = e ? e : b->fak.ensure(nf * nslot);
+    e = e ? e : b->dga.ensure(nf * nslot);
+    e = e ? e : b->vals.ensure(nv * nslot);
+    e = e ? e : b->yscr.ensure(ny * nslot);
     if (e) return e;
+    for (int q = 0; q < nslot; ++q) {
+        crt_batch::Slot &w = b->slots[q];
+        w.panels = b->panels.p + q * np;
+        w.yf = b->yf.p + q * nf;
+        w.dinv = b->dinv.p + q * nf;
+        w.fak = b->fak.p + q * nf;
+        w.dga = b->dga.p + q * nf;
+        w.vals = b->vals.p + q * nv;
+        w.yscr = b->yscr.p + q * ny;
+    }
     if (flags & 1) {
         for (auto &M : b->m) CRT_HIP(hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), b->stream));
+        CRT_HIP(hipStreamSynchronize(b->stream));  // the tuning runs on the slot streams
         if ((e = batch_tune(b.get()))) return e;
     }
     for (auto &M : b->m) M->variant = M->use16 ? b->variant[M->D] : 0;
@@ -2003,6 +2049,9 @@ int crt_batch_forward(crt_batch *b) {
     }
     if (!b->exec) {
         CRT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        // the other slot streams join the capture through an event of the origin stream
+        CRT_HIP(hipEventRecord(b->fork, b->stream));
+        for (size_t q = 1; q < b->slots.size(); ++q) CRT_HIP(hipStreamWaitEvent(b->slots[q].st, b->fork, 0));
         int e = 0;
         for (auto &M : b->m) {
             for (const auto &ch : M->chunks) {
@@ -2012,6 +2061,11 @@ int crt_batch_forward(crt_batch *b) {
                 if (e) break;
             }
             if (e) break;
+        }
+        for (size_t q = 1; q < b->slots.size(); ++q) {
+            if (hipEventRecord(b->slots[q].join, b->slots[q].st) != hipSuccess ||
+                hipStreamWaitEvent(b->stream, b->slots[q].join, 0) != hipSuccess)
+                e = e ? e : CRT_ERR_HIP;
         }
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(b->stream, &g);
@@ -2059,6 +2113,7 @@ int crt_batch_info(crt_batch *b, double *ms, long long *nsys, int *variant, floa
 int crt_batch_destroy(crt_batch *b) {
     if (!b) return 0;
     (void)hipSetDevice(b->device);
+    for (auto &w : b->slots) (void)hipStreamSynchronize(w.st);
     (void)hipStreamSynchronize(b->stream);
     delete b;
     return 0;
