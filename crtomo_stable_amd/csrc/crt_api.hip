@@ -28,7 +28,8 @@ int crt_factor16_resident(int D, int variant);
 int crt_launch_factor16(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zd *, zd *,
                         zd *, int *, int);
 int crt_launch_solve16(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
-                       const zd *, const zd *, const zd *, zd *, zd *);
+                       const zd *, const zd *, const zd *, zd *, zd *, int);
+int crt_factor16_resolve(int D, int variant);
 // mixed precision (crt_band32.hip, FP32-panel solve modes in crt_band16.hip)
 int crt_factor32_supported(int mb);
 int crt_launch_factor32(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zf *, zf *, zf *,
@@ -452,7 +453,8 @@ int batch_solve(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
     const crt_batch::Slot &w = b->slots[M.slot];
     if (M.use16)
         return crt_launch_solve16(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak,
-                                  w.panels, w.dinv, w.yf, w.yscr, c->d_kpotN.p);
+                                  w.panels, w.dinv, w.yf, w.yscr, c->d_kpotN.p,
+                                  crt_factor16_resolve(M.D, M.variant) != 3);
     return crt_launch_solve(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak, w.panels,
                             w.yf, w.yscr, c->d_kpotN.p, nullptr, nullptr);
 }
@@ -1008,7 +1010,8 @@ int crt_forward(crt_ctx *c) {
             c->tic(T_SOLVE);
             if (use16)
                 e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
-                                       c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p);
+                                       c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p,
+                                       crt_factor16_resolve(D, 0) != 3);
             else
                 e = crt_launch_solve(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
                                      c->d_fak.p, c->d_panels.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p, nullptr, nullptr);

@@ -609,11 +609,15 @@ __device__ __forceinline__ void diag_finish(const Factor16Args &A, LDS &m, int s
     __builtin_amdgcn_wave_barrier();
     const zd ivj = m.pinv[jc], sj = m.sdiag[jc];
     zd *pg = A.panels + ((size_t)s * nblk + Jn) * (D + 1) * 256;
+    // the diagonal tile keeps L on and below the diagonal and, above it, the strict
+    // lower triangle of L^-1 transposed (tile(i, jc) = L^-1(jc, i), i < jc): the
+    // substitution applies L^-T as a product instead of a serial chain
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = 4 * v + rq;
-        stz(pg + i * T16 + jc, i > jc ? zmul(q.a[v], ivj) : (i == jc ? sj : zd{0.0, 0.0}));
-        stz(m.Li + jc * PS + i, zmul(q.z[v], ivj));  // L^-1(jc, i) = (L^-T)(i, jc)
+        const zd li = zmul(q.z[v], ivj);  // L^-1(jc, i) = (L^-T)(i, jc)
+        stz(pg + i * T16 + jc, i > jc ? zmul(q.a[v], ivj) : (i == jc ? sj : li));
+        stz(m.Li + jc * PS + i, li);
     }
     if (rq == 0) {
         const zd y = zmul(q.yv, ivj);
@@ -1248,6 +1252,14 @@ int crt_factor16_resident(int D, int variant) {
     }
 }
 
+// the variant launch16 runs for (D, variant)
+int crt_factor16_resolve(int D, int variant) {
+    if (variant != 0) return variant;
+    static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
+    static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
+    return (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
+}
+
 int crt_factor16_supported(int mb) {
     int D = (mb + 15) / 16;
     return D >= 1 && D <= CRT_F16_MAXD;
@@ -1324,6 +1336,15 @@ __device__ __forceinline__ void solve16_fetch_diag(const PT *panel, const PT *di
 
 __device__ __forceinline__ double cmax_abs(zd z) { return fmax(fabs(z.re), fabs(z.im)); }
 
+// What the substitution wave needs of block J in the backward pass, per lane (row
+// c = lane % 16, column quarter rq = lane / 16): four entries L(J,J)(c, 4rq..4rq+3)
+// of the diagonal tile, 1/L(c,c), y(c) and fak(c).  Fetched three blocks ahead, so
+// the loads of a block have two block steps to arrive.
+struct BackStage {
+    zd d[4], p, y;
+    double f;
+};
+
 __device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd (&drow)[T16]) {
     if (lane < T16)
 #pragma unroll
@@ -1335,7 +1356,9 @@ __device__ __forceinline__ void solve16_store_diag(zd *dtile, int lane, const zd
 
 }  // namespace
 
-template <int D, class PT = zd, int MODE = 0>
+// INV: the diagonal tiles hold L^-1 above the diagonal (the look-ahead, streaming and
+// FP32 factorisations write it), so the backward pass applies L(J,J)^-T as a product
+template <int D, class PT = zd, int MODE = 0, bool INV = false>
 __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(SolveArgs<PT> A) {
     constexpr int NA = (D + SW - 2) / (SW - 1);  // off-diagonal tiles per worker wave
     __shared__ zd ring[(D + 1) * T16];
@@ -1408,30 +1431,76 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 return yscr[i];
             }
         };
+        auto fetch = [&](int J, BackStage &b) {
+            const int Jc = J < 0 ? 0 : (J < nblk ? J : nblk - 1);
+            const PT *Ld = panel + (size_t)Jc * (D + 1) * 256 + c * T16 + 4 * rq;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b.d[j] = zd{Ld[j].re, Ld[j].im};
+            const PT pv = dinv[(size_t)Jc * T16 + c];
+            b.p = zd{pv.re, pv.im};
+            const int i = J * T16 + c;
+            b.y = J >= J0 && J >= 0 ? yat(i) : zd{0.0, 0.0};
+            b.f = J >= 0 && i < A.sanz ? fak[i] : 0.0;
+        };
+        auto store = [&](const BackStage &b) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) dtile[c * PS + 4 * rq + j] = b.d[j];
+        };
+        auto move = [&](BackStage &to, const BackStage &from) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) to.d[j] = zd{from.d[j].re, from.d[j].im};
+            to.p = zd{from.p.re, from.p.im};
+            to.y = zd{from.y.re, from.y.im};
+            to.f = from.f;
+        };
         __syncthreads();  // the forward pass has finished with dtile
-        solve16_fetch_diag<D>(panel, dinv, nblk, nblk - 1, c, drow, pnext);
-        solve16_store_diag(dtile, lane, drow);
-        pcur = pnext;
-        zd ycur = nblk - 1 >= J0 ? yat((nblk - 1) * T16 + c) : zd{0.0, 0.0};
+        BackStage s1, s2;
+        {
+            BackStage s0;
+            fetch(nblk - 1, s0);
+            fetch(nblk - 2, s1);
+            fetch(nblk - 3, s2);
+            store(s0);
+            pcur = s0.p;
+            drow[0] = s0.y;   // y of the current block
+            drow[1].re = s0.f;  // fak of the current block's row c
+        }
         double dmax = 0.0, xmax = 0.0;
         for (int J = nblk - 1; J >= 0; --J) {
-            solve16_fetch_diag<D>(panel, dinv, nblk, J - 1, c, drow, pnext);
-            const zd ynxt = J - 1 >= J0 ? yat((J - 1) * T16 + c) : zd{0.0, 0.0};
+            BackStage sn;
+            fetch(J - 3, sn);
+            const zd ycur = drow[0];
+            const double fcur = drow[1].re;
             __syncthreads();
             zd xv = zd{-ycur.re, -ycur.im};  // lane c carries column c of the block
 #pragma unroll
             for (int w = 1; w < SW; ++w) xv = zsub(xv, part[w][c]);
+            if constexpr (INV) {
+                // x(c) = rv(c) / L(c,c) + sum_{r > c} L^-1(r, c) rv(r); lane (c, rq) takes
+                // r = 4rq..4rq+3, L^-1(r, c) sits at tile(c, r)
+                zd sacc = zd{0.0, 0.0};
 #pragma unroll
-            for (int cc = T16 - 1; cc >= 0; --cc) {
-                const zd xcc = zmul(readlane_z(xv, cc), readlane_z(pcur, cc));
-                if (c == cc) xv = xcc;
-                if (c < cc) xv = zfms(xv, dtile[cc * PS + c], xcc);
+                for (int j = 0; j < 4; ++j) {
+                    const int r = 4 * rq + j;
+                    const zd rv = shfl_z16(xv, r);
+                    if (r > c) sacc = zfma(sacc, dtile[c * PS + r], rv);
+                }
+                sacc = zadd(sacc, shfl_xor_z16(sacc, 16));
+                sacc = zadd(sacc, shfl_xor_z16(sacc, 32));
+                xv = zadd(zmul(xv, pcur), sacc);
+            } else {
+#pragma unroll
+                for (int cc = T16 - 1; cc >= 0; --cc) {
+                    const zd xcc = zmul(readlane_z(xv, cc), readlane_z(pcur, cc));
+                    if (c == cc) xv = xcc;
+                    if (c < cc) xv = zfms(xv, dtile[cc * PS + c], xcc);
+                }
             }
             if (lane < T16) {
                 ring[(J % (D + 1)) * T16 + lane] = xv;
                 const int i = J * T16 + lane;
                 if (i < A.sanz) {
-                    const double f = fak[i];
+                    const double f = fcur;
                     zd xo = xv;
                     if constexpr (MODE >= 1) {
                         zd *xa = A.xacc + (size_t)rid * nblk * T16 + i;
@@ -1444,9 +1513,12 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 }
             }
             __builtin_amdgcn_wave_barrier();
-            solve16_store_diag(dtile, lane, drow);
-            pcur = pnext;
-            ycur = ynxt;
+            store(s1);
+            pcur = s1.p;
+            drow[0] = s1.y;
+            drow[1].re = s1.f;
+            move(s1, s2);
+            move(s2, sn);
             __syncthreads();
         }
         if constexpr (MODE >= 1) {
@@ -1463,8 +1535,24 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
         return;
     }
 
-    // ---- waves 1..3: products with the off-diagonal tiles a = wave + 3u
-    zd cur[NA][4];
+    // ---- waves 1..3: products with the off-diagonal tiles a = wave + 3u; the tiles of
+    // PD blocks beyond the current one are in flight (the loop waits on HBM latency,
+    // not bandwidth: with FP32 panels, half the bytes, it takes as long)
+    constexpr int PD = (NA <= 3 && D > 6) ? 2 : 1;  // D <= 6: four workgroups per CU, 128 VGPRs
+    zd cur[NA][4], nx1[NA][4];
+    auto rotate = [&](zd (&nxt)[NA][4]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if constexpr (PD == 2) {
+                    cur[u][v] = nx1[u][v];
+                    nx1[u][v] = nxt[u][v];
+                } else {
+                    cur[u][v] = nxt[u][v];
+                }
+            }
+    };
     if (!have_y) {
         auto fetch_f = [&](int J, zd (&t)[NA][4]) {
 #pragma unroll
@@ -1480,9 +1568,10 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             }
         };
         fetch_f(J0, cur);
+        if constexpr (PD == 2) fetch_f(J0 + 1, nx1);
         for (int J = J0; J < nblk; ++J) {
             zd nxt[NA][4];
-            fetch_f(J + 1, nxt);
+            fetch_f(J + PD, nxt);
             // acc(r) = sum_a sum_c L(J, J-a)(r, c) y_{J-a}(c)
             zd acc[4] = {zd{0, 0}, zd{0, 0}, zd{0, 0}, zd{0, 0}};
 #pragma unroll
@@ -1504,10 +1593,7 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             }
             __syncthreads();
             __syncthreads();
-#pragma unroll
-            for (int u = 0; u < NA; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) cur[u][v] = nxt[u][v];
+            rotate(nxt);
         }
     }
     auto fetch_b = [&](int J, zd (&t)[NA][4]) {
@@ -1524,10 +1610,11 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
         }
     };
     fetch_b(nblk - 1, cur);
+    if constexpr (PD == 2) fetch_b(nblk - 2, nx1);
     __syncthreads();
     for (int J = nblk - 1; J >= 0; --J) {
         zd nxt[NA][4];
-        fetch_b(J - 1, nxt);
+        fetch_b(J - PD, nxt);
         // acc(c) = sum_a sum_r L(J+a, J)(r, c) x_{J+a}(r)
         zd acc = zd{0.0, 0.0};
 #pragma unroll
@@ -1544,50 +1631,48 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
         if (rq == 0) part[wave][c] = acc;
         __syncthreads();
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NA; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) cur[u][v] = nxt[u][v];
+        rotate(nxt);
     }
 }
 
 namespace {
 
-template <int D, class PT, int MODE>
+template <int D, class PT, int MODE, bool INV>
 int launch_solve16(hipStream_t st, const SolveArgs<PT> &a) {
-    hipLaunchKernelGGL((crt_solve16_kernel<D, PT, MODE>), dim3(a.nrhs), dim3(64 * SW), 0, st, a);
+    hipLaunchKernelGGL((crt_solve16_kernel<D, PT, MODE, INV>), dim3(a.nrhs), dim3(64 * SW), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-template <class PT, int MODE>
+template <class PT, int MODE, bool INV>
 int solve16_dispatch(hipStream_t st, int D, const SolveArgs<PT> &a) {
     switch (D) {
-        case 1: return launch_solve16<1, PT, MODE>(st, a);
-        case 2: return launch_solve16<2, PT, MODE>(st, a);
-        case 3: return launch_solve16<3, PT, MODE>(st, a);
-        case 4: return launch_solve16<4, PT, MODE>(st, a);
-        case 5: return launch_solve16<5, PT, MODE>(st, a);
-        case 6: return launch_solve16<6, PT, MODE>(st, a);
-        case 7: return launch_solve16<7, PT, MODE>(st, a);
-        case 8: return launch_solve16<8, PT, MODE>(st, a);
-        case 9: return launch_solve16<9, PT, MODE>(st, a);
-        case 10: return launch_solve16<10, PT, MODE>(st, a);
-        case 11: return launch_solve16<11, PT, MODE>(st, a);
-        case 12: return launch_solve16<12, PT, MODE>(st, a);
-        case 13: return launch_solve16<13, PT, MODE>(st, a);
-        case 14: return launch_solve16<14, PT, MODE>(st, a);
-        case 15: return launch_solve16<15, PT, MODE>(st, a);
-        case 16: return launch_solve16<16, PT, MODE>(st, a);
+        case 1: return launch_solve16<1, PT, MODE, INV>(st, a);
+        case 2: return launch_solve16<2, PT, MODE, INV>(st, a);
+        case 3: return launch_solve16<3, PT, MODE, INV>(st, a);
+        case 4: return launch_solve16<4, PT, MODE, INV>(st, a);
+        case 5: return launch_solve16<5, PT, MODE, INV>(st, a);
+        case 6: return launch_solve16<6, PT, MODE, INV>(st, a);
+        case 7: return launch_solve16<7, PT, MODE, INV>(st, a);
+        case 8: return launch_solve16<8, PT, MODE, INV>(st, a);
+        case 9: return launch_solve16<9, PT, MODE, INV>(st, a);
+        case 10: return launch_solve16<10, PT, MODE, INV>(st, a);
+        case 11: return launch_solve16<11, PT, MODE, INV>(st, a);
+        case 12: return launch_solve16<12, PT, MODE, INV>(st, a);
+        case 13: return launch_solve16<13, PT, MODE, INV>(st, a);
+        case 14: return launch_solve16<14, PT, MODE, INV>(st, a);
+        case 15: return launch_solve16<15, PT, MODE, INV>(st, a);
+        case 16: return launch_solve16<16, PT, MODE, INV>(st, a);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }
 }  // namespace
 
+// inv: the factor variant stored L^-1 above the diagonal of its diagonal tiles
 int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs, const int4 *rhs,
                        const int *enr, const double *fak, const zd *panels, const zd *dinv, const zd *y, zd *yscr,
-                       zd *kpotN) {
+                       zd *kpotN, int inv) {
     Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, nullptr, nullptr};
-    return solve16_dispatch<zd, 0>(st, D, a);
+    return inv ? solve16_dispatch<zd, 0, true>(st, D, a) : solve16_dispatch<zd, 0, false>(st, D, a);
 }
 
 // substitution through the FP32 panels for the refinement: mode 1 is the first
@@ -1597,7 +1682,7 @@ int crt_launch_solve16_f32(hipStream_t st, int sanz, int nblk, int D, int kwnanz
                            const int4 *rhs, const int *enr, const double *fak, const zf *panels, const zf *dinv,
                            const zf *y, zd *yscr, zd *kpotN, zd *xacc, double *stats, int mode) {
     SolveArgs<zf> a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, xacc, stats};
-    if (mode == 1) return solve16_dispatch<zf, 1>(st, D, a);
-    if (mode == 2) return solve16_dispatch<zf, 2>(st, D, a);
+    if (mode == 1) return solve16_dispatch<zf, 1, true>(st, D, a);
+    if (mode == 2) return solve16_dispatch<zf, 2, true>(st, D, a);
     return CRT_ERR_ARG;
 }

@@ -272,11 +272,14 @@ __device__ __forceinline__ void chain32(const Factor32Args &A, St32Lds<D> &m, in
     __builtin_amdgcn_wave_barrier();
     const zf ivj = m.pinv[jc], sj = m.sdiag[jc];
     zf *pg = A.panels + ((size_t)s * nblk + Jn) * (D + 1) * 256;
+    // L^-1 above the diagonal of the diagonal tile (tile(i, jc) = L^-1(jc, i), i < jc),
+    // as the FP64 kernels store it for the substitution
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = 4 * v + rq;
-        stc(pg + i * T16 + jc, i > jc ? cmul(q.a[v], ivj) : (i == jc ? sj : zf{0.0f, 0.0f}));
-        stc(m.Li + jc * PSF + i, cmul(q.z[v], ivj));  // L^-1(jc, i)
+        const zf li = cmul(q.z[v], ivj);  // L^-1(jc, i)
+        stc(pg + i * T16 + jc, i > jc ? cmul(q.a[v], ivj) : (i == jc ? sj : li));
+        stc(m.Li + jc * PSF + i, li);
     }
     if (rq == 0) {
         const zf y = cmul(q.yv, ivj);
