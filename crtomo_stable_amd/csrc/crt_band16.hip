@@ -1172,13 +1172,16 @@ __global__ void __launch_bounds__(SWV * 64, 2) crt_factor16st_kernel(Factor16Arg
 // kernel variants: 1 look-ahead (D = 2..10), 2 streaming (D >= 3), 3 plain
 // right-looking (D <= 10); 0 picks 1 up to D = 10 (3 at D = 1) and 2 above
 // (CRT_FACTOR_ST=1 and CRT_FACTOR_V2=1 change that default)
+int crt_factor16_resolve(int D, int variant) {
+    if (variant != 0) return variant;
+    static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
+    static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
+    return (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
+}
+
 template <int D>
 static int launch16(hipStream_t st, const Factor16Args &a, int variant) {
-    if (variant == 0) {
-        static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
-        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
-        variant = (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
-    }
+    variant = crt_factor16_resolve(D, variant);
     if (variant == 2) {
         if constexpr (D >= SNU) {
             if (a.g.tmapD != D) return CRT_ERR_UNSUPPORTED;
@@ -1207,11 +1210,7 @@ static int launch16(hipStream_t st, const Factor16Args &a, int variant) {
 // whole device at once, 0 when unknown
 template <int D>
 static int resident16(int variant) {
-    if (variant == 0) {
-        static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
-        static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
-        variant = (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
-    }
+    variant = crt_factor16_resolve(D, variant);
     int per_cu = 0, dev = 0, cus = 0;
     hipError_t e = hipErrorInvalidValue;
     if (variant == 2) {
@@ -1250,14 +1249,6 @@ int crt_factor16_resident(int D, int variant) {
         case 16: return resident16<16>(variant);
         default: return 0;
     }
-}
-
-// the variant launch16 runs for (D, variant)
-int crt_factor16_resolve(int D, int variant) {
-    if (variant != 0) return variant;
-    static const bool stream = getenv("CRT_FACTOR_ST") != nullptr;
-    static const bool v2 = getenv("CRT_FACTOR_V2") != nullptr;
-    return (D >= SNU && (D > 10 || stream)) ? 2 : ((D >= 2 && D <= 10 && !v2) ? 1 : 3);
 }
 
 int crt_factor16_supported(int mb) {
