@@ -1670,6 +1670,10 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
     const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    // J and the regularisation must both cover manz cells
+    if ((size_t)nanz * manz > c->d_sens.n || (size_t)manz * c->rncol > c->d_smatm.n ||
+        (c->ltri != 0 && (size_t)manz * c->rncol > c->d_nachbar.n))
+        return CRT_ERR_STATE;
     int e = cg_weights(c, nanz, wmatd, wdfak);
     if (e) return e;
     const size_t n2 = (size_t)manz * manz;

@@ -81,12 +81,13 @@ __global__ void crt_bmcm_regn_kernel(int n, int maxn, double lam, const double *
         long long best = -1;
         double s = 0.0;
         if (a == b) { best = (long long)a * (maxn + 2); s = SM(a, maxn + 1); }
-        for (int kk = 1; kk <= NB(b, maxn + 1); ++kk)
+        const int nbb = min(NB(b, maxn + 1), maxn), nba = min(NB(a, maxn + 1), maxn);
+        for (int kk = 1; kk <= nbb; ++kk)
             if (NB(b, kk) == a) {
                 const long long key = (long long)b * (maxn + 2) + kk;
                 if (key > best) { best = key; s = SM(b, kk); }
             }
-        for (int kk = 1; kk <= NB(a, maxn + 1); ++kk)
+        for (int kk = 1; kk <= nba; ++kk)
             if (NB(a, kk) == b) {
                 const long long key = (long long)a * (maxn + 2) + kk;
                 if (key > best) { best = key; s = SM(a, kk); }
@@ -99,9 +100,9 @@ __global__ void crt_bmcm_regn_kernel(int n, int maxn, double lam, const double *
         reg[(size_t)(j - 1) * n + j - 1] = v;
         return;
     }
-    if (k > NB(j, maxn + 1)) return;
+    if (k > min(NB(j, maxn + 1), maxn)) return;
     const int ik = NB(j, k);
-    if (ik == 0) return;
+    if (ik <= 0 || ik > n) return;
     double v;
     final_value(ik, j, v);
     reg[(size_t)(j - 1) * n + ik - 1] = v;
