@@ -1929,6 +1929,40 @@ int crt_chol(crt_ctx *c, int sanz, int mb, double *a) {
     return std::max(st[0], st[1]);
 }
 
+// choldc(a) (src/choldc.f90:7): the real band LL^T of the DC path, run by the complex
+// kernel on the same numbers with zero imaginary parts (which stay exactly zero while
+// every pivot is positive).  Where the reference stops on s <= 0 (errnr 28) the complex
+// factorisation would take an imaginary square root instead: a diagonal entry of L
+// that is not real and positive returns 28, and a is then left as it was.
+int crt_choldc(crt_ctx *c, int sanz, int mb, double *a) {
+    if (!c || !a || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    const size_t n = (size_t)sanz * (mb + 1);
+    std::vector<double> z(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        z[2 * i] = a[i];
+        z[2 * i + 1] = 0.0;
+    }
+    const int e = crt_chol(c, sanz, mb, z.data());
+    if (e) return e;
+    for (int i = 0; i < sanz; ++i) {
+        const size_t d = (size_t)i * (mb + 1) + mb;
+        if (z[2 * d + 1] != 0.0 || !(z[2 * d] > 0.0)) return CRT_ERR_PIVOT;
+    }
+    for (size_t i = 0; i < n; ++i) a[i] = z[2 * i];
+    return 0;
+}
+
+// vredc(a, b, pot) (src/vredc.f90:1): pot = -A^{-1} b for the real factor and real
+// right-hand side of the DC path, complex pot out as the reference's dcmplx(potdc)
+int crt_vredc(crt_ctx *c, int sanz, int mb, const double *a, const double *b, double *pot) {
+    if (!c || !a || !b || !pot || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    const size_t n = (size_t)sanz * (mb + 1);
+    std::vector<double> za(2 * n, 0.0), zb(2 * (size_t)sanz, 0.0);
+    for (size_t i = 0; i < n; ++i) za[2 * i] = a[i];
+    for (int i = 0; i < sanz; ++i) zb[2 * (size_t)i] = b[i];
+    return crt_vre(c, sanz, mb, za.data(), zb.data(), pot);
+}
+
 // vre(a, b, pot): pot = -A^{-1} b from the band factor (src/vre.f90:45-69).
 int crt_vre(crt_ctx *c, int sanz, int mb, const double *a, const double *b, double *pot) {
     if (!c || !a || !b || !pot || sanz <= 0 || mb < 0) return CRT_ERR_ARG;

@@ -354,6 +354,20 @@ class Context:
                               _ptr(np.ascontiguousarray(b, np.complex128)), _ptr(pot)), "crt_vre")
         return pot
 
+    def choldc(self, sanz, mb, a):
+        """In-place real band Cholesky (src/choldc.f90); returns errnr."""
+        a2 = np.ascontiguousarray(a, np.float64)
+        err = self.lib.crt_choldc(self._h, int(sanz), int(mb), _ptr(a2))
+        if a2 is not a:
+            a[...] = a2
+        return err
+
+    def vredc(self, sanz, mb, a, b):
+        pot = np.zeros(sanz, np.complex128)
+        _chk(self.lib.crt_vredc(self._h, int(sanz), int(mb), _ptr(np.ascontiguousarray(a, np.float64)),
+                                _ptr(np.ascontiguousarray(b, np.float64)), _ptr(pot)), "crt_vredc")
+        return pot
+
     def cholz(self, a_colmajor):
         """a (n, n) Fortran-ordered complex; returns (errnr, a_out, p)."""
         n = a_colmajor.shape[0]

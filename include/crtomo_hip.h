@@ -201,6 +201,11 @@ int crt_buncert(crt_ctx *ctx, double lam, const double *wmatd, const int *wdfak,
 int crt_chol(crt_ctx *ctx, int sanz, int mb, double *a);
 int crt_vre(crt_ctx *ctx, int sanz, int mb, const double *a, const double *b, double *pot);
 int crt_cholz(crt_ctx *ctx, int n, double *a, double *p);
+/* DC twins: choldc(a) (src/choldc.f90:7), real band in/out, errnr 28 where the
+ * reference finds s <= 0; vredc(a,b,pot) (src/vredc.f90:1), real band factor and
+ * right-hand side, complex pot (sanz COMPLEX*16) out. */
+int crt_choldc(crt_ctx *ctx, int sanz, int mb, double *a);
+int crt_vredc(crt_ctx *ctx, int sanz, int mb, const double *a, const double *b, double *pot);
 
 /* Forward precision (no counterpart in the reference, whose chol/vre run in
  * COMPLEX*16, src/chol.f90:7, src/vre.f90:5).  bits 64 (default): FP64 band

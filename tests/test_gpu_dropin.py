@@ -1,5 +1,6 @@
 """GPU parity of the single-system drop-ins chol(a), vre(a,b,pot) and cholz(a,p,n)
-against the oracle restatements of src/chol.f90, src/vre.f90 and src/cholz.f90.
+against the oracle restatements of src/chol.f90, src/vre.f90 and src/cholz.f90, and of
+the DC twins choldc(a), vredc(a,b,pot) (src/choldc.f90, src/vredc.f90).
 Tolerance 1e-10 relative (FP64); error code 28 on a vanishing pivot."""
 import numpy as np
 import pytest
@@ -48,6 +49,50 @@ def test_chol_pivot_error(ctx):
             a[i * (mb + 1) + mb - 1] = -1.0
     a[17 * (mb + 1): 18 * (mb + 1)] = 0.0
     assert ctx.chol(sanz, mb, a) == 28
+
+
+def test_choldc_vredc_dropin(ctx, oracle_lib):
+    """DC twins: the real band of the DC path (kompab with a real conductivity, scaled),
+    choldc against the complex oracle chol on the same numbers and against L L^T = A,
+    vredc against the oracle vre; a negative pivot gives errnr 28 (src/choldc.f90)."""
+    O = oracle_lib
+    w = configs.small_quad()
+    kwn, kwi, _, _ = O.rwaven(w.grid, w.survey)
+    elbg, relbg, kg, lb = O.precal(w.grid, kwn)
+    sig = w.sigma.real.astype(np.complex128)
+    a, b = O.kompab(w.grid, sig, elbg, relbg, kg, 3, 2)
+    O.scalab(w.grid.sanz, w.grid.mb, a, b)
+    sanz, mb = w.grid.sanz, w.grid.mb
+    assert np.abs(a.imag).max() == 0.0
+    ar = a.real.copy()
+    assert ctx.choldc(sanz, mb, ar) == 0
+    ao = a.copy()
+    assert O.chol(sanz, mb, ao) == 0
+    mask = np.zeros((sanz, mb + 1), bool)
+    for i in range(sanz):
+        mask[i, max(0, mb - i):] = True
+    m = mask.ravel()
+    assert np.max(np.abs(ar[m] - ao.real[m])) <= 1e-10 * np.max(np.abs(ao[m]))
+    # L L^T reproduces the scaled matrix
+    L = np.zeros((sanz, sanz)); A = np.zeros((sanz, sanz))
+    for i in range(sanz):
+        for j in range(max(0, i - mb), i + 1):
+            L[i, j] = ar[i * (mb + 1) + mb - (i - j)]
+            A[i, j] = A[j, i] = a.real[i * (mb + 1) + mb - (i - j)]
+    assert np.max(np.abs(L @ L.T - A)) <= 1e-12 * np.max(np.abs(A))
+    rng = np.random.default_rng(6)
+    bb = rng.standard_normal(sanz)
+    pg = ctx.vredc(sanz, mb, ar, bb)
+    po = O.vre(sanz, mb, ao, bb.astype(np.complex128))
+    assert np.max(np.abs(pg - po)) <= 1e-10 * np.max(np.abs(po))
+    assert np.max(np.abs(pg - (-np.linalg.solve(A, bb)))) <= 1e-10 * np.max(np.abs(pg))
+    # indefinite: one negative diagonal
+    neg = np.zeros(40 * 6)
+    for i in range(40):
+        neg[i * 6 + 5] = 4.0 if i != 17 else -4.0
+        if i > 0:
+            neg[i * 6 + 4] = -1.0
+    assert ctx.choldc(40, 5, neg) == 28
 
 
 def test_cholz_dropin(ctx, oracle_lib):
