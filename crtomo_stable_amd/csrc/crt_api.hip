@@ -16,6 +16,7 @@ int crt_launch_precal(hipStream_t, int, int, int, int, int, int, const int *, co
                       double *, int *);
 int crt_launch_diag(hipStream_t, const DevGrid &, int, const int2 *, double *, double *, int *);
 int crt_launch_assemble(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *);
+int crt_launch_area(hipStream_t, const DevGrid &, int, zd *);
 int crt_launch_factor(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, const zd *,
                       zd *, zd *, int *);
 size_t crt_factor_lds_bytes(int D);
@@ -141,6 +142,11 @@ struct crt_ctx {
     // (column, CSR index) per entry, for the refinement residual
     DBuf<int> d_frow;
     DBuf<int2> d_fent;
+    // area contributions before boundary ones in every entry's list: the area part is
+    // summed once per wavenumber (d_area) and shared by every source
+    bool asplit = false;
+    DBuf<int> d_csplit;
+    DBuf<zd> d_area;
     // forward precision: 64 (FP64 factorisation) or 32 (FP32 factorisation + FP64
     // iterative refinement until max|fak d| <= refine_tol max|fak x| per system)
     int fwd_bits = 64;
@@ -197,6 +203,8 @@ struct crt_ctx {
         g.sigma = d_sigma.p;
         g.tmap = d_tmap.p;
         g.tmapD = tmapD;
+        g.csplit = asplit ? d_csplit.p : nullptr;
+        g.area = nullptr;  // set by the callers that computed the area sums
         return g;
     }
     void tic(int i) { (void)hipEventRecord(ev0[i], stream); }
@@ -417,6 +425,7 @@ int batch_plan(crt_batch::Model &M, size_t ws_bytes) {
     e = e ? e : M.rhs.upload(rall.data(), rall.size(), c->stream);
     e = e ? e : M.status.ensure(M.nsys + 1);
     e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
+    e = e ? e : (c->asplit ? c->d_area.ensure((size_t)K * c->nnz) : 0);
     if (e) return e;
     return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : CRT_ERR_HIP;
 }
@@ -426,7 +435,16 @@ DevGrid batch_grid(const crt_batch::Model &M) {
     g.nblk = M.nblk;
     g.D = M.D;
     g.fstride = (int)M.fstride;
+    if (M.c->asplit) g.area = M.c->d_area.p;
     return g;
+}
+
+// the model's per-wavenumber area sums (before its chunks: sigma may change between calls)
+int batch_area(crt_batch *b, crt_batch::Model &M) {
+    if (!M.c->asplit) return 0;
+    DevGrid g = batch_grid(M);
+    g.area = nullptr;
+    return crt_launch_area(b->slots[M.slot].st, g, M.c->kwnanz, M.c->d_area.p);
 }
 
 // diagonal scaling and assembly of one chunk into the model's workspace slot
@@ -626,6 +644,17 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     }
     cptr.push_back((int)contrib.size());
     for (int i = 0; i < sanz; ++i) row_ptr[i + 1] += row_ptr[i];
+    // split point of each list: area contributions first, then boundary ones
+    std::vector<int> csplit(col.size());
+    c->asplit = true;
+    for (size_t q = 0; q < col.size(); ++q) {
+        int p = cptr[q];
+        while (p < cptr[q + 1] && contrib[p] >= 0) ++p;
+        csplit[q] = p;
+        for (int r = p; r < cptr[q + 1]; ++r)
+            if (contrib[r] >= 0) c->asplit = false;
+    }
+    if (csplit.empty()) csplit.push_back(0);
     if (contrib.empty()) contrib.push_back(0);
 
     c->nblk = (sanz + CRT_NB - 1) / CRT_NB;
@@ -648,6 +677,7 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     c->nnz = (int)col.size();
     e = e ? e : c->d_cptr.upload(cptr.data(), cptr.size(), s);
     e = e ? e : c->d_contrib.upload(contrib.data(), contrib.size(), s);
+    e = e ? e : c->d_csplit.upload(csplit.data(), csplit.size(), s);
     if (e) return e;
     // tile map of the 16x16 block band (used by the factorisation for band widths up to 160)
     c->tmapD = 0;
@@ -963,6 +993,11 @@ int crt_forward(crt_ctx *c) {
     g.nblk = nblk;
     g.D = D;
     g.fstride = (int)fstride;
+    if (c->asplit) {
+        if ((e = c->d_area.ensure((size_t)K * c->nnz))) return e;
+        if ((e = crt_launch_area(c->stream, g, K, c->d_area.p))) return e;
+        g.area = c->d_area.p;
+    }
     double tdiag = 0, tfac = 0, tsol = 0, tref = 0;
     c->refine_sweeps = 0;
     c->refine_ratio = 0.0;
@@ -2072,7 +2107,9 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
     }
     if (flags & 1) {
         for (auto &M : b->m) CRT_HIP(hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), b->stream));
-        CRT_HIP(hipStreamSynchronize(b->stream));  // the tuning runs on the slot streams
+        for (auto &M : b->m)
+            if ((e = batch_area(b.get(), *M))) return e;
+        for (auto &w : b->slots) CRT_HIP(hipStreamSynchronize(w.st));  // the tuning runs on the slot streams
         if ((e = batch_tune(b.get()))) return e;
     }
     for (auto &M : b->m) M->variant = M->use16 ? b->variant[M->D] : 0;
@@ -2095,6 +2132,7 @@ int crt_batch_forward(crt_batch *b) {
         for (size_t q = 1; q < b->slots.size(); ++q) CRT_HIP(hipStreamWaitEvent(b->slots[q].st, b->fork, 0));
         int e = 0;
         for (auto &M : b->m) {
+            if ((e = batch_area(b, *M))) break;
             for (const auto &ch : M->chunks) {
                 e = batch_assemble(b, *M, ch);
                 e = e ? e : batch_factor(b, *M, ch, M->variant);

@@ -57,7 +57,42 @@ __device__ __forceinline__ zd assemble_entry(const DevGrid &g, int e0, int e1, i
     return s;
 }
 
+// A(i,j) of system (l, k) for CSR entry q: the wavenumber's area sum (when the grid has
+// one) continued with the entry's boundary contributions, the same operations in the
+// same order as assemble_entry, so the value is identical
+__device__ __forceinline__ zd assemble_q(const DevGrid &g, int q, int l, int k) {
+#pragma clang fp contract(off)
+    if (!g.area) return assemble_entry(g, g.cptr[q], g.cptr[q + 1], l, k);
+    zd s = g.area[(size_t)k * g.nnz + q];
+    const int e1 = g.cptr[q + 1];
+    for (int p = g.csplit[q]; p < e1; ++p) {
+        const int cc = -g.contrib[p] - 1;
+        const int rel = cc / g.npk, imn = cc - rel * g.npk;
+        const double dum = g.relbg[(size_t)imn * g.relanz + rel] * g.kg[((size_t)k * g.eanz + l) * g.relanz + rel];
+        const zd sg = g.sigma[g.rnr[rel]];
+        s.re = s.re + dum * sg.re;
+        s.im = s.im + dum * sg.im;
+    }
+    return s;
+}
+
 }  // namespace
+
+// -------------------------------------------------------------- area sums
+// area[k*nnz + q]: the area-element contributions of entry q at wavenumber k
+// (src/kompab.f90:113-121 for the elements of types 3, 5, 8), one thread per (k, q)
+__global__ void __launch_bounds__(256) crt_area_kernel(DevGrid g, int K, zd *__restrict__ area) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)K * g.nnz) return;
+    const int k = (int)(t / g.nnz), q = (int)(t - (long)k * g.nnz);
+    area[t] = assemble_entry(g, g.cptr[q], g.csplit[q], 0, k);
+}
+
+int crt_launch_area(hipStream_t st, const DevGrid &g, int K, zd *area) {
+    const long n = (long)K * g.nnz;
+    hipLaunchKernelGGL(crt_area_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, K, area);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
 
 // ------------------------------------------------------------------ diagonal
 // fak_i = 1/sqrt|a_ii|, dga_i = |a_ii| per system (src/scalab.f90:54-66)
@@ -69,7 +104,7 @@ __global__ void __launch_bounds__(256) crt_diag_kernel(DevGrid g, int nsys, cons
     int s = (int)(t / g.sanz), i = (int)(t - (long)s * g.sanz);
     int2 lk = sys[s];
     int q = g.row_ptr[i + 1] - 1;  // diagonal is the last CSR entry of the row
-    zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
+    zd a = assemble_q(g, q, lk.x, lk.y);
     double dum = zabs(a);
     size_t o = (size_t)s * g.fstride + i;
     if (!(dum > 0.0)) {
@@ -96,7 +131,7 @@ __global__ void __launch_bounds__(256) crt_assemble_kernel(DevGrid g, int nsys, 
     }
     int2 lk = sys[s];
     int i = g.crow[q], j = g.col[q];
-    zd a = assemble_entry(g, g.cptr[q], g.cptr[q + 1], lk.x, lk.y);
+    zd a = assemble_q(g, q, lk.x, lk.y);
     const double *fs = fak + (size_t)s * g.fstride;
     zd v;
     if (j == i) {

@@ -128,6 +128,12 @@ struct DevGrid {
     // tile element is a plain load
     const int *tmap;
     int tmapD;                // D16 the map was built for (0 if none)
+    // Every contribution list holds its area elements before its boundary elements
+    // (checked on the host), so the area part of an entry is the same sum for every
+    // source: area[k*nnz + q] holds it per wavenumber, csplit[q] is where the boundary
+    // contributions of entry q start.  area null: every entry is summed in full.
+    const int *csplit;
+    const zd *area;
 };
 
 // Gauss-Newton CG state shared by crt_cg.hip and crt_api.hip
