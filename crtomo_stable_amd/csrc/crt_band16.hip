@@ -1446,6 +1446,8 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
         };
         __syncthreads();  // the forward pass has finished with dtile
         BackStage s1, s2;
+        zd ycur;      // y(c) of the current block
+        double fcur;  // fak of the current block's row c
         {
             BackStage s0;
             fetch(nblk - 1, s0);
@@ -1453,15 +1455,13 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             fetch(nblk - 3, s2);
             store(s0);
             pcur = s0.p;
-            drow[0] = s0.y;   // y of the current block
-            drow[1].re = s0.f;  // fak of the current block's row c
+            ycur = s0.y;
+            fcur = s0.f;
         }
         double dmax = 0.0, xmax = 0.0;
         for (int J = nblk - 1; J >= 0; --J) {
             BackStage sn;
             fetch(J - 3, sn);
-            const zd ycur = drow[0];
-            const double fcur = drow[1].re;
             __syncthreads();
             zd xv = zd{-ycur.re, -ycur.im};  // lane c carries column c of the block
 #pragma unroll
@@ -1506,8 +1506,8 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             __builtin_amdgcn_wave_barrier();
             store(s1);
             pcur = s1.p;
-            drow[0] = s1.y;
-            drow[1].re = s1.f;
+            ycur = s1.y;
+            fcur = s1.f;
             move(s1, s2);
             move(s2, sn);
             __syncthreads();
