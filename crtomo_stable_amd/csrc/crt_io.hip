@@ -13,6 +13,12 @@
 //   config.dat src/rdatm.f90 (count, then current and potential electrode codes)
 //   volt.dat   src/wdatm.f90:58-76 (count, then strnr, vnr, REAL*4 magnitude and
 //              1000 * phase; a phase near +-pi swaps the potential electrodes)
+//   pot*.dat   src/bpot.f90 + src/wpot.f90 (one file per measurement: node x, y and the
+//              complex potential of its current dipole)
+//   sens*.dat  src/bsens.f90 + src/wsens.f90 (one file per measurement: |sum J|, its
+//              phase in mrad, then element centroid x, y and dV/dsigma per element)
+// Text files hold REAL*4 values; the reference writes them list-directed, whose layout
+// is the compiler's, so the values (not the spacing) are what matches.
 #include "crt_common.hpp"
 #include "../../include/crtomo_hip.h"
 
@@ -21,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -190,6 +197,34 @@ int read_crmod_cfg(const std::string &path, CrmodCfg &c) {
     return 0;
 }
 
+// Name of file n of a numbered series (src/wpot.f90:34-44, src/wsens.f90:46-60 with
+// filpat.f90 and intcha.f90): the directory length as filpat reports it (at least 1),
+// the name up to its first '.', then n in lnanz = int(log10(real(nanz))) + 1 digits,
+// zero-filled (asterisks when n has more digits, as Fortran's I format), then the four
+// characters from the '.' on.  base_len gets the name's length before the number.
+std::string numbered_name(const std::string &datei, int n, int nanz, int &lnanz, int &base_len) {
+    lnanz = (int)std::log10((float)nanz) + 1;
+    const size_t sl = datei.rfind('/');
+    const int id2 = sl == std::string::npos ? 0 : (int)sl + 1;  // 1-based last slash
+    const int idum2 = id2 >= 2 ? id2 - 1 : 1;
+    const size_t dot = (size_t)idum2 <= datei.size() ? datei.find('.', idum2) : std::string::npos;
+    const int idx = dot == std::string::npos ? 0 : (int)dot - idum2 + 1;
+    const int idum = idum2 + idx - 1;
+    base_len = idum - idum2 - 1;
+    std::string digits = std::to_string(n);
+    if ((int)digits.size() > lnanz) digits = std::string(lnanz, '*');
+    else digits = std::string(lnanz - digits.size(), '0') + digits;
+    std::string ext = idum >= 0 && (size_t)idum < datei.size() ? datei.substr(idum, 4) : std::string();
+    while (!ext.empty() && ext.back() == ' ') ext.pop_back();
+    return datei.substr(0, std::max(idum, 0)) + digits + ext;
+}
+
+// one REAL*4 value per column, enough digits to read back exactly
+void put_row(FILE *f, std::initializer_list<double> v) {
+    for (double x : v) fprintf(f, " %16.8E", (double)(float)x);
+    fputc('\n', f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -270,6 +305,65 @@ int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path) {
     std::vector<double> volt(2 * (size_t)nc), sigmaa(2 * (size_t)nc);
     if ((e = crt_voltages(ctx, 0, nullptr, volt.data(), sigmaa.data()))) return e;
     if (cfg.wvolt && (e = crt_write_volt(cfg.volt.c_str(), nc, strnr.data(), vnr.data(), sigmaa.data()))) return e;
+    if (cfg.wpot) {
+        // bpot (src/bpot.f90:25-40): per measurement the potential of its current
+        // dipole, (hpot(:,B) - hpot(:,A)) * strom with strom = 1, written by wpot
+        // (src/wpot.f90:45-50): node coordinates and the complex potential per node
+        std::vector<double> hpot(2 * (size_t)g.sanz * ne);
+        if ((e = crt_get_hpot(ctx, hpot.data()))) return e;
+        for (int i = 0; i < nc; ++i) {
+            const int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+            int lnanz = 0, blen = 0;
+            const std::string name = numbered_name(cfg.pot, i + 1, nc, lnanz, blen);
+            if (blen > 8 - lnanz) return 15;  // src/wpot.f90:36-40
+            FILE *f = fopen(name.c_str(), "w");
+            if (!f) return 1;
+            for (int j = 0; j < g.sanz; ++j) {
+                double re = 0.0, im = 0.0;
+                if (e2 > 0) { re += hpot[2 * ((size_t)(e2 - 1) * g.sanz + j)]; im += hpot[2 * ((size_t)(e2 - 1) * g.sanz + j) + 1]; }
+                if (e1 > 0) { re -= hpot[2 * ((size_t)(e1 - 1) * g.sanz + j)]; im -= hpot[2 * ((size_t)(e1 - 1) * g.sanz + j) + 1]; }
+                put_row(f, {g.x[j], g.y[j], re, im});
+            }
+            if (fclose(f) != 0) return 4;
+        }
+    }
+    if (cfg.wsens) {
+        // bsens (dV/dsigma, strom = 1) and wsens (src/wsens.f90:62-80): per measurement
+        // |sum_j J(i,j)| and 1000 atan2 of that sum, then element centroids (the mean of
+        // the element's node coordinates, src/relem.f90:128-140) and J per element
+        if ((e = crt_sens(ctx, 0, nullptr))) return e;
+        std::vector<double> J(2 * (size_t)nc * g.elanz);
+        if ((e = crt_get_sens(ctx, J.data()))) return e;
+        std::vector<double> cx(g.elanz, 0.0), cy(g.elanz, 0.0);
+        for (int t = 0, iel = 0, ia = 0; t < g.typanz; ++t)
+            for (int q = 0; q < g.nelanz[t]; ++q, ++iel) {
+                if (g.typ[t] > 10) continue;
+                for (int k = 0; k < g.selanz[t]; ++k) {
+                    const int nd = g.nrel[(size_t)iel * g.maxn + k] - 1;
+                    cx[ia] += g.x[nd];
+                    cy[ia] += g.y[nd];
+                }
+                cx[ia] /= g.selanz[t];
+                cy[ia] /= g.selanz[t];
+                ++ia;
+            }
+        for (int i = 0; i < nc; ++i) {
+            int lnanz = 0, blen = 0;
+            const std::string name = numbered_name(cfg.sens, i + 1, nc, lnanz, blen);
+            FILE *f = fopen(name.c_str(), "w");
+            if (!f) return 1;
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < g.elanz; ++j) {
+                sr += J[2 * ((size_t)j * nc + i)];
+                si += J[2 * ((size_t)j * nc + i) + 1];
+            }
+            if (std::hypot(sr, si) > 0.0) put_row(f, {std::hypot(sr, si), 1e3 * std::atan2(si, sr)});
+            else put_row(f, {0.0, 0.0});
+            for (int j = 0; j < g.elanz; ++j)
+                put_row(f, {cx[j], cy[j], J[2 * ((size_t)j * nc + i)], J[2 * ((size_t)j * nc + i) + 1]});
+            if (fclose(f) != 0) return 4;
+        }
+    }
     return 0;
 }
 

@@ -177,6 +177,8 @@ class Context:
     def get_kpot(self):
         """kpot as (kwnanz, eanz, sanz) = reference kpot(sanz, eanz, kwnanz)."""
         g = self.grid
+        if g is None or g.eanz == 0:
+            raise ValueError("get_kpot needs the grid with its electrodes (set_grid)")
         out = np.zeros((self.kwnanz, g.eanz, g.sanz), np.complex128)
         _chk(self.lib.crt_get_kpot(self._h, _ptr(out)), "crt_get_kpot")
         return out
@@ -190,6 +192,8 @@ class Context:
 
     def get_hpot(self):
         g = self.grid
+        if g is None or g.eanz == 0:
+            raise ValueError("get_hpot needs the grid with its electrodes (set_grid)")
         out = np.zeros((g.eanz, g.sanz), np.complex128)
         _chk(self.lib.crt_get_hpot(self._h, _ptr(out)), "crt_get_hpot")
         return out

@@ -165,7 +165,11 @@ int crt_blam0(crt_ctx *ctx, int fpi, const double *wmatd, const int *wdfak, doub
  * switches and output paths; relative paths from the directory of crmod.cfg),
  * elem.dat (src/relem.f90:60-151 with the border check of :197-229), elec.dat
  * (src/relectr.f90), rho.dat (src/rsigma.f90:168), config.dat (src/rdatm.f90);
- * runs the forward and writes volt.dat when the measurements switch is T. */
+ * runs the forward and writes volt.dat when the measurements switch is T, the
+ * potential files of src/bpot.f90 + src/wpot.f90 when the potentials switch is T
+ * (errnr 15 for a base name too long for the numbering) and the sensitivity files of
+ * src/bsens.f90 + src/wsens.f90 when the sensitivities switch is T (file n of a series:
+ * the name with n zero-filled to int(log10(nanz))+1 digits before its extension). */
 int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path);
 /* Sizes of the tomodir's grid, electrodes and configurations (host only). */
 int crt_tomodir_sizes(const char *cfg_path, int *sanz, int *mb, int *elanz, int *eanz, int *nanz);

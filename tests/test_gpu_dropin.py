@@ -130,3 +130,61 @@ def test_tomodir_crmod_volt_dat(ctx, tmp_path, name):
     assert np.array_equal(s_out.strnr, s_ref.strnr) and np.array_equal(s_out.vnr, s_ref.vnr)
     assert np.max(np.abs(m_out - m_ref) / m_ref) < 1e-6
     assert np.max(np.abs(p_out - p_ref)) < 1e-3
+
+
+@pytest.mark.parametrize("name", ["minimal", "mixed_surf"])
+def test_tomodir_crmod_pot_sens_files(ctx, tmp_path, name):
+    """The CRMod outputs of the potentials and sensitivities switches (src/bpot.f90 +
+    src/wpot.f90, src/bsens.f90 + src/wsens.f90): one file per measurement, named with
+    its number zero-filled to int(log10(nanz)) + 1 digits; the potential of each
+    measurement's current dipole at every node, and per element the centroid and
+    dV/dsigma, against the library's own hpot and J within the files' REAL*4."""
+    import os
+    from crtomo_stable_amd import grid as G, lib
+    from test_lib_cpu import _tomodir
+    cfg, src = _tomodir(str(tmp_path), name)
+    txt = open(cfg).read().replace("F ! potentials ?", "T ! potentials ?").replace(
+        "F ! sensitivities ?", "T ! sensitivities ?")
+    open(cfg, "w").write(txt)
+    for d in ("pot", "sens"):
+        os.makedirs(os.path.join(str(tmp_path), "mod", d), exist_ok=True)
+    print(f"[tomodir pot/sens] {name}", flush=True)
+    ctx.tomodir_crmod(cfg)
+    g = G.read_elem(os.path.join(src, "elem.dat"))
+    # the electrodes too: the buffers of get_hpot are sized by them
+    g.enr = np.ascontiguousarray(G.read_elec(os.path.join(src, "elec.dat")), np.int32)
+    assert g.eanz > 0
+    survey, _, _ = G.read_volt(os.path.join(src, "volt.dat"))
+    nanz = survey.nanz
+    nd = int(np.log10(np.float32(nanz))) + 1
+    pots = sorted(os.listdir(os.path.join(str(tmp_path), "mod", "pot")))
+    sens = sorted(os.listdir(os.path.join(str(tmp_path), "mod", "sens")))
+    assert pots == [f"pot{i:0{nd}d}.dat" for i in range(1, nanz + 1)]
+    assert sens == [f"sens{i:0{nd}d}.dat" for i in range(1, nanz + 1)]
+    # the library's own arrays after the run (its context holds grid, survey, kpot, J)
+    ctx.grid, ctx.survey = g, survey
+    hpot = ctx.get_hpot()          # (eanz, sanz)
+    J = ctx.get_sens()             # (elanz, nanz): dV/dsigma of the sensitivities switch
+    cx, cy = g.centroids()
+    for i in sorted({0, nanz // 2, nanz - 1}):
+        e1, e2 = survey.strnr[i] % 10000, survey.strnr[i] // 10000
+        p = np.zeros(g.sanz, np.complex128)
+        if e2 > 0:
+            p += hpot[e2 - 1]
+        if e1 > 0:
+            p -= hpot[e1 - 1]
+        a = np.loadtxt(os.path.join(str(tmp_path), "mod", "pot", pots[i]))
+        assert a.shape == (g.sanz, 4)
+        assert np.allclose(a[:, 0], g.x.astype(np.float32), rtol=0, atol=1e-6 * np.abs(g.x).max())
+        assert np.allclose(a[:, 1], g.y.astype(np.float32), rtol=0, atol=1e-6 * max(np.abs(g.y).max(), 1.0))
+        assert np.max(np.abs(a[:, 2] + 1j * a[:, 3] - p)) <= 1e-6 * np.max(np.abs(p))
+        b = np.loadtxt(os.path.join(str(tmp_path), "mod", "sens", sens[i]), skiprows=1)
+        head = np.loadtxt(os.path.join(str(tmp_path), "mod", "sens", sens[i]), max_rows=1)
+        assert b.shape == (g.elanz, 4)
+        assert np.allclose(b[:, 0], cx, rtol=1e-6, atol=1e-6 * np.abs(cx).max())
+        assert np.allclose(b[:, 1], cy, rtol=1e-6, atol=1e-6 * max(np.abs(cy).max(), 1.0))
+        jr = J[:, i]
+        assert np.max(np.abs(b[:, 2] + 1j * b[:, 3] - jr)) <= 1e-6 * np.max(np.abs(jr))
+        s = jr.sum()
+        assert abs(head[0] - abs(s)) <= 1e-6 * abs(s)
+        assert abs(head[1] - 1e3 * np.angle(s)) <= 1e-3 * max(1.0, abs(1e3 * np.angle(s)))
