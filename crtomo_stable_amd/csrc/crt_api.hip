@@ -72,12 +72,13 @@ int crt_launch_pc_fwdbwd(hipStream_t, int, const float *, const float *, const f
                          float *, float *, double *, double *, float *);
 int crt_launch_pc_adam(hipStream_t, float *, float *, float *, const float *, float, int);
 int crt_launch_pc_infer(hipStream_t, int, int, const float *, const float *, const float *, const int *, zd *);
-int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const zd *, const zd *,
-                      const double *, double *, zd *);
+int crt_launch_real_part(hipStream_t, size_t, const zd *, double *);
+int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const double *, const zd *,
+                      const zd *, const double *, double *, zd *);
 int crt_launch_cg_rmul(hipStream_t, const CGArgs &, const zd *, zd *);
 int crt_launch_cg_init(hipStream_t, const CGArgs &, const zd *);
 int crt_launch_cg_iter(hipStream_t, const CGArgs &);
-int crt_launch_cg_final(hipStream_t, int, double, const double *, zd *, zd *);
+int crt_launch_cg_final(hipStream_t, int, double, const double *, zd *, zd *, int);
 int crt_launch_bsmatmreg(hipStream_t, int, int, int, int, const int *, const double *, const double *, double, double,
                          double *);
 int crt_launch_bsmatmtri(hipStream_t, int, int, const int *, const int *, const double *, const double *, const int *,
@@ -171,6 +172,11 @@ struct crt_ctx {
     DBuf<float2> d_sens32;  // complex FP32 copy of J for the CG iterations
     int cg_jbits = 64;      // storage precision of J in the CG iterations
     bool sens32_ok = false;
+    // 0: complex inversion; 1: DC inversion (ldc); 2: final phase improvement (lfpi)
+    int inv_kind = 0;
+    DBuf<double> d_sensr;   // real J of the DC / FPI CG (sensdc, or DBLE(sens))
+    bool sensr_ok = false;
+    DBuf<double> d_re;      // real parts for the DC getters
     int cg_prec = 0;                      // 0 Jacobi scaling only, 1/2 FSAI patterns
     std::vector<int> h_nachbar;           // host copy of the neighbour table (ltri 1/2)
     DBuf<int> d_fpat, d_ftp;
@@ -219,6 +225,15 @@ struct crt_ctx {
     hipError_t copy_out(void *dst, const void *src, size_t bytes) {
         hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
         return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+    }
+    // n complex values to the host; the DC inversion hands out their real parts
+    // (kpotdc, hpotdc, sensdc are REAL*8 in src/alloci.f90:24-28)
+    int copy_out_kind(double *dst, const zd *src, size_t n) {
+        if (inv_kind != 1) return copy_out(dst, src, n * sizeof(zd)) == hipSuccess ? 0 : CRT_ERR_HIP;
+        int e = d_re.ensure(n);
+        e = e ? e : crt_launch_real_part(stream, n, src, d_re.p);
+        if (e) return e;
+        return copy_out(dst, d_re.p, n * sizeof(double)) == hipSuccess ? 0 : CRT_ERR_HIP;
     }
     int status_check() {
         int st = 0;
@@ -857,11 +872,31 @@ int crt_set_sigma(crt_ctx *c, const double *sigma) {
     if (!c || !sigma) return CRT_ERR_ARG;
     if (!c->have_grid) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    int e = c->d_sigma.upload(reinterpret_cast<const zd *>(sigma), c->elanz, c->stream);
+    const zd *zs = reinterpret_cast<const zd *>(sigma);
+    std::vector<zd> sre;
+    if (c->inv_kind == 1) {
+        // the DC forward assembles DBLE(sigma) (src/kompadc.f90:112-118) and bsendc
+        // takes DBLE(sigma/volt) with a real volt (src/bsendc.f90:166-167)
+        sre.resize(c->elanz);
+        for (int i = 0; i < c->elanz; ++i) sre[i] = zd{zs[i].re, 0.0};
+        zs = sre.data();
+    }
+    int e = c->d_sigma.upload(zs, c->elanz, c->stream);
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(c->stream));
     c->have_sigma = true;
     c->have_kpot = c->have_volt = c->have_sens = false;
+    return 0;
+}
+
+int crt_set_inversion_kind(crt_ctx *c, int kind) {
+    if (!c || kind < 0 || kind > 2) return CRT_ERR_ARG;
+    if (kind != c->inv_kind) {
+        // the conductivity and everything derived from it belong to the previous kind
+        c->have_sigma = c->have_kpot = c->have_volt = c->have_sens = false;
+        c->sens32_ok = c->sensr_ok = false;
+    }
+    c->inv_kind = kind;
     return 0;
 }
 
@@ -1097,14 +1132,13 @@ int crt_get_kpot(crt_ctx *c, double *kpot) {
     int e = c->d_tmp.ensure(n);
     if (e) return e;
     if ((e = crt_launch_kpot_ref(c->stream, c->sanz, c->eanz, c->kwnanz, c->d_kpotN.p, c->d_tmp.p))) return e;
-    CRT_HIP(hipMemcpyAsync(kpot, c->d_tmp.p, n * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
-    CRT_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    return c->copy_out_kind(kpot, c->d_tmp.p, n);
 }
 
 int crt_voltages(crt_ctx *c, int mode, const double *kfak, double *volt, double *sigmaa) {
     if (!c || (mode != 0 && mode != 1)) return CRT_ERR_ARG;
     if (!c->have_kpot || !c->have_survey) return CRT_ERR_STATE;
+    if (c->inv_kind == 1 && mode == 0) return CRT_ERR_UNSUPPORTED;  // bvolt has no DC branch
     CRT_HIP(hipSetDevice(c->device));
     int e = c->d_hpotE.ensure((size_t)c->eanz * c->eanz);
     e = e ? e : c->d_volt.ensure(c->nanz);
@@ -1118,7 +1152,9 @@ int crt_voltages(crt_ctx *c, int mode, const double *kfak, double *volt, double 
     if ((e = c->status_reset())) return e;
     c->tic(T_VOLT);
     e = crt_launch_hpot_elec(c->stream, c->eanz, c->kwnanz, c->d_enr.p, c->d_kwnwi.p, c->d_kpotN.p, c->d_hpotE.p);
-    e = e ? e : crt_launch_volt(c->stream, c->nanz, c->eanz, mode, c->d_meas.p, c->d_hpotE.p, dk, c->d_volt.p,
+    // mode 2 of the kernel: bvolti's DC branch, volt = dcmplx(real superposition)
+    e = e ? e : crt_launch_volt(c->stream, c->nanz, c->eanz, c->inv_kind == 1 ? 2 : mode, c->d_meas.p, c->d_hpotE.p,
+                                dk, c->d_volt.p,
                                 c->d_sigmaa.p, c->d_status.p);
     c->toc(T_VOLT);
     if (e) return e;
@@ -1140,15 +1176,14 @@ int crt_get_hpot(crt_ctx *c, double *hpot) {
     if (e) return e;
     if ((e = crt_launch_hpot_full(c->stream, c->sanz, c->eanz, c->kwnanz, c->d_kwnwi.p, c->d_kpotN.p, c->d_tmp.p)))
         return e;
-    CRT_HIP(hipMemcpyAsync(hpot, c->d_tmp.p, n * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
-    CRT_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    return c->copy_out_kind(hpot, c->d_tmp.p, n);
 }
 
 int crt_sens(crt_ctx *c, int mode, const double *strom) {
     if (!c || (mode != 0 && mode != 1)) return CRT_ERR_ARG;
     if (!c->have_kpot || !c->have_survey) return CRT_ERR_STATE;
     if (mode == 1 && !c->have_volt) return CRT_ERR_STATE;
+    if (c->inv_kind == 1 && mode == 0) return CRT_ERR_UNSUPPORTED;  // the DC twin is bsendc (of bsensi)
     CRT_HIP(hipSetDevice(c->device));
     int e = c->d_sens.ensure((size_t)c->nanz * c->elanz);
     if (e) return e;
@@ -1167,6 +1202,7 @@ int crt_sens(crt_ctx *c, int mode, const double *strom) {
     c->collect(T_SENS);
     c->have_sens = true;
     c->sens32_ok = false;
+    c->sensr_ok = false;
     c->sens_nanz = c->nanz;
     c->sens_manz = c->elanz;
     return 0;
@@ -1177,21 +1213,30 @@ int crt_get_sens(crt_ctx *c, double *sens) {
     if (!c->have_sens) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
     const size_t n = (size_t)(c->sens_nanz ? c->sens_nanz : c->nanz) * (c->sens_manz ? c->sens_manz : c->elanz);
-    CRT_HIP(c->copy_out(sens, c->d_sens.p, n * sizeof(zd)));
-    return 0;
+    return c->copy_out_kind(sens, c->d_sens.p, n);
 }
 
 // ------------------------------------------------------------- normal equations
 int crt_set_sens(crt_ctx *c, int nanz, int manz, const double *sens) {
     if (!c || !sens || nanz <= 0 || manz <= 0) return CRT_ERR_ARG;
     CRT_HIP(hipSetDevice(c->device));
-    int e = c->d_sens.upload(reinterpret_cast<const zd *>(sens), (size_t)nanz * manz, c->stream);
+    const size_t n = (size_t)nanz * manz;
+    const zd *zs = reinterpret_cast<const zd *>(sens);
+    std::vector<zd> ex;
+    if (c->inv_kind == 1) {
+        // DC: a real sensdc(nanz, manz)
+        ex.resize(n);
+        for (size_t t = 0; t < n; ++t) ex[t] = zd{sens[t], 0.0};
+        zs = ex.data();
+    }
+    int e = c->d_sens.upload(zs, n, c->stream);
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(c->stream));
     c->sens_nanz = nanz;
     c->sens_manz = manz;
     c->have_sens = true;
     c->sens32_ok = false;
+    c->sensr_ok = false;
     return 0;
 }
 
@@ -1286,7 +1331,19 @@ static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, do
     a.lam = lam; a.eps = eps;
     a.J = c->d_sens.p; a.w = c->d_w.p; a.smatm = c->d_smatm.p; a.nachbar = c->d_nachbar.p;
     a.J32 = nullptr;
-    if (c->cg_jbits == 32) {
+    a.Jr = nullptr;
+    a.part = c->inv_kind;
+    if (c->inv_kind != 0) {
+        // DC / FPI: the reference's Jacobi-scaled CG on a real J (src/cg_mod.f90:106-196)
+        if (c->cg_jbits != 64 || c->cg_prec != 0) return CRT_ERR_UNSUPPORTED;
+        const size_t n = (size_t)nanz * manz;
+        if (!c->sensr_ok) {
+            if ((e = c->d_sensr.ensure(n))) return e;
+            if ((e = crt_launch_real_part(c->stream, n, c->d_sens.p, c->d_sensr.p))) return e;
+            c->sensr_ok = true;
+        }
+        a.Jr = c->d_sensr.p;
+    } else if (c->cg_jbits == 32) {
         const size_t n = (size_t)nanz * manz;
         if (!c->sens32_ok) {
             if ((e = c->d_sens32.ensure(n))) return e;
@@ -1655,6 +1712,7 @@ int crt_set_cg_precision(crt_ctx *c, int jbits) {
     c->cg_jbits = jbits;
     if (jbits == 64) c->d_sens32.release();
     c->sens32_ok = false;
+    c->sensr_ok = false;
     return 0;
 }
 
@@ -1703,6 +1761,7 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
                 double *ata, double *ata_reg, double *cov, double *res, double *cov2, double *diag) {
     if (!c || !wmatd || !wdfak) return CRT_ERR_ARG;
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
+    if (c->inv_kind != 0) return CRT_ERR_UNSUPPORTED;  // bata_dc / the FPI branch of bmcm_mod
     CRT_HIP(hipSetDevice(c->device));
     const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
     // J and the regularisation must both cover manz cells
@@ -1757,7 +1816,9 @@ int crt_blam0(crt_ctx *c, int fpi, const double *wmatd, const int *wdfak, double
     e = e ? e : c->d_part_ap.ensure(np);
     e = e ? e : c->d_ap.ensure(nanz);
     e = e ? e : c->d_jtj.ensure((size_t)manz + 1);
-    e = e ? e : crt_launch_blam0(c->stream, nanz, manz, fpi ? 1 : 0, c->d_sens.p, c->d_w.p, c->d_part_ap.p, ncb, cb,
+    // the DC branch over sensdc is the FPI form on a J with zero imaginary parts
+    const int rpart = (fpi || c->inv_kind == 1) ? 1 : 0;
+    e = e ? e : crt_launch_blam0(c->stream, nanz, manz, rpart, c->d_sens.p, c->d_w.p, c->d_part_ap.p, ncb, cb,
                                  c->d_ap.p, c->d_jtj.p, 2.0 / (alfx + alfz), c->d_jtj.p + manz);
     if (e) return e;
     // the context stream does not synchronise with the null stream: copy on it
@@ -1790,7 +1851,7 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
     CGArgs a;
     if ((e = cg_prepare(c, nanz, manz, ncgmax, lam, eps, a))) return e;
     if ((e = crt_launch_cg_rmul(c->stream, a, c->d_par.p, c->d_rm.p))) return e;
-    if ((e = crt_launch_cg_rhs(c->stream, nanz, manz, lam, c->d_sens.p, c->d_w.p, c->d_res.p, c->d_rm.p,
+    if ((e = crt_launch_cg_rhs(c->stream, nanz, manz, lam, c->d_sens.p, a.Jr, c->d_w.p, c->d_res.p, c->d_rm.p,
                                c->d_rdiag.p, c->d_cgfac.p, c->d_bvec.p)))
         return e;
     if (cgfac) CRT_HIP(hipMemcpyAsync(cgfac, c->d_cgfac.p, manz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1799,7 +1860,7 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
     int n = 0;
     if ((e = cg_iterate(c, a, c->d_bvec.p, &n))) return e;
     if ((e = cg_fallback(c, a, ncgmax, &n))) return e;
-    if ((e = crt_launch_cg_final(c->stream, manz, step, c->d_cgfac.p, c->d_dpar.p, c->d_par.p))) return e;
+    if ((e = crt_launch_cg_final(c->stream, manz, step, c->d_cgfac.p, c->d_dpar.p, c->d_par.p, a.part))) return e;
     if (dpar) CRT_HIP(hipMemcpyAsync(dpar, c->d_dpar.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
     CRT_HIP(hipMemcpyAsync(par, c->d_par.p, manz * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
     if (cgres) CRT_HIP(hipMemcpyAsync(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double),

@@ -11,6 +11,9 @@
 // dr <= eps * dr_0 on the squared residual.  Scalars live on the device; every
 // reduction has a fixed order, so the iteration is deterministic.  J is
 // column-major (nanz, manz) complex FP64, as written by the sensitivity kernel.
+// The DC inversion and the final phase improvement (cjggdc, src/cg_mod.f90:106-196)
+// run the same kernels on a real J (imaginary parts zero throughout) from the real
+// or the imaginary part of bvec; the real copy halves the bytes per iteration.
 #include "crt_common.hpp"
 
 #include <algorithm>
@@ -20,8 +23,9 @@
 
 namespace {
 
-// Jacobian entry as complex FP64 from the FP64 or the FP32 copy
+// Jacobian entry as complex FP64 from the FP64, the FP32 or the real copy
 __device__ __forceinline__ zd ldj(const zd *p) { return *p; }
+__device__ __forceinline__ zd ldj(const double *p) { return zd{*p, 0.0}; }
 __device__ __forceinline__ zd ldj(const float2 *p) {
     const float2 v = *p;
     return zd{(double)v.x, (double)v.y};
@@ -72,10 +76,11 @@ __device__ __forceinline__ zd rstencil(const CGArgs &A, const zd *x, int i) {
 
 // ----------------------------------------------------------- update: RHS pass
 // One wave per column j: cgfac_j and bvec_j in a single pass over J.
+template <class JT>
 struct RhsArgs {
     int nanz, manz;
     double lam;
-    const zd *J;
+    const JT *J;
     const double *w;
     const zd *res;       // d - f (nanz)
     const zd *rm;        // R m (manz)
@@ -84,15 +89,16 @@ struct RhsArgs {
     zd *bvec;
 };
 
-__global__ void __launch_bounds__(256) crt_cg_rhs_kernel(RhsArgs A) {
+template <class JT>
+__global__ void __launch_bounds__(256) crt_cg_rhs_kernel(RhsArgs<JT> A) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = blockIdx.x * (blockDim.x >> 6) + wv;
     if (j >= A.manz) return;
-    const zd *col = A.J + (size_t)j * A.nanz;
+    const JT *col = A.J + (size_t)j * A.nanz;
     double nrm = 0.0;
     zd s = zd{0.0, 0.0};
     for (int i = lane; i < A.nanz; i += 64) {
-        zd v = col[i];
+        zd v = ldj(col + i);
         double wi = A.w[i];
         nrm += (v.re * v.re + v.im * v.im) * wi;
         zd d = A.res[i];
@@ -128,6 +134,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_init_kernel(CGArgs A, const zd 
     double v = 0.0;
     if (j < A.manz) {
         zd b = bvec[j];
+        if (A.part) b = zd{A.part == 1 ? b.re : b.im, 0.0};  // bvecdc (src/cg_mod.f90:123-127)
         A.r[j] = b;
         A.p[j] = zd{0.0, 0.0};
         A.dpar[j] = zd{0.0, 0.0};
@@ -305,11 +312,13 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_upd_kernel(CGArgs A) {
 }
 
 // final scaling dpar <- C dpar (src/update.f90:302-307) and par += step dpar
+// (FPI: dpar = i * dpar first, src/update.f90:302-304)
 __global__ void __launch_bounds__(256) crt_cg_final_kernel(int manz, double step, const double *cgfac, zd *dpar,
-                                                           zd *par) {
+                                                           zd *par, int part) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= manz) return;
     zd d = dpar[j];
+    if (part == 2) d = zd{0.0, d.re};
     double c = cgfac[j];
     d = zd{d.re * c, d.im * c};
     dpar[j] = d;
@@ -475,10 +484,15 @@ int crt_cg_geometry(int nanz, int manz, int *ncb, int *cb, int *n_dr1, int *n_rr
     return 0;
 }
 
-int crt_launch_cg_rhs(hipStream_t s, int nanz, int manz, double lam, const zd *J, const double *w, const zd *res,
-                      const zd *rm, const double *rdiag, double *cgfac, zd *bvec) {
-    RhsArgs a{nanz, manz, lam, J, w, res, rm, rdiag, cgfac, bvec};
-    hipLaunchKernelGGL(crt_cg_rhs_kernel, dim3((manz + 3) / 4), dim3(256), 0, s, a);
+int crt_launch_cg_rhs(hipStream_t s, int nanz, int manz, double lam, const zd *J, const double *Jr,
+                      const double *w, const zd *res, const zd *rm, const double *rdiag, double *cgfac, zd *bvec) {
+    if (Jr) {
+        RhsArgs<double> a{nanz, manz, lam, Jr, w, res, rm, rdiag, cgfac, bvec};
+        hipLaunchKernelGGL(crt_cg_rhs_kernel<double>, dim3((manz + 3) / 4), dim3(256), 0, s, a);
+    } else {
+        RhsArgs<zd> a{nanz, manz, lam, J, w, res, rm, rdiag, cgfac, bvec};
+        hipLaunchKernelGGL(crt_cg_rhs_kernel<zd>, dim3((manz + 3) / 4), dim3(256), 0, s, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -499,12 +513,16 @@ int crt_launch_cg_init(hipStream_t s, const CGArgs &a, const zd *bvec) {
 int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
     hipLaunchKernelGGL(crt_cg_dir_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
     const dim3 gq((a.nanz + CG_BLK - 1) / CG_BLK, a.ncb);
-    if (a.J32)
+    if (a.Jr)
+        hipLaunchKernelGGL(crt_cg_jq_kernel<double>, gq, dim3(CG_BLK), 0, s, a, a.Jr);
+    else if (a.J32)
         hipLaunchKernelGGL(crt_cg_jq_kernel<float2>, gq, dim3(CG_BLK), 0, s, a, a.J32);
     else
         hipLaunchKernelGGL(crt_cg_jq_kernel<zd>, gq, dim3(CG_BLK), 0, s, a, a.J);
     hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK), dim3(CG_BLK), 0, s, a);
-    if (a.J32)
+    if (a.Jr)
+        hipLaunchKernelGGL(crt_cg_jhw_kernel<double>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.Jr);
+    else if (a.J32)
         hipLaunchKernelGGL(crt_cg_jhw_kernel<float2>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J32);
     else
         hipLaunchKernelGGL(crt_cg_jhw_kernel<zd>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J);
@@ -516,8 +534,21 @@ int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-int crt_launch_cg_final(hipStream_t s, int manz, double step, const double *cgfac, zd *dpar, zd *par) {
-    hipLaunchKernelGGL(crt_cg_final_kernel, dim3((manz + 255) / 256), dim3(256), 0, s, manz, step, cgfac, dpar, par);
+int crt_launch_cg_final(hipStream_t s, int manz, double step, const double *cgfac, zd *dpar, zd *par, int part) {
+    hipLaunchKernelGGL(crt_cg_final_kernel, dim3((manz + 255) / 256), dim3(256), 0, s, manz, step, cgfac, dpar, par,
+                       part);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+// real parts of a complex array (the real Jacobian of the DC / FPI branches, and
+// the DC getters)
+__global__ void __launch_bounds__(256) crt_real_part_kernel(size_t n, const zd *x, double *out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = x[t].re;
+}
+
+int crt_launch_real_part(hipStream_t s, size_t n, const zd *x, double *out) {
+    hipLaunchKernelGGL(crt_real_part_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, out);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 

@@ -66,8 +66,11 @@ __global__ void __launch_bounds__(256) crt_volt_kernel(int nanz, int eanz, int m
     };
     zd d1 = hp(m.w, m.y), d2 = hp(m.w, m.x), d3 = hp(m.z, m.y), d4 = hp(m.z, m.x);
     zd v = zsub(zsub(d1, d2), zsub(d3, d4));
+    // mode 2, the DC branch of bvolti: volt = dcmplx(V) with a +0 imaginary part, so
+    // that -log(V) of a negative V has the imaginary part -pi (src/bvolti.f90:65,89)
+    if (mode == 2) v.im = 0.0;
     volt[i] = v;
-    if (mode == 1) {
+    if (mode >= 1) {
         double a = zabs(v);
         if (a == 0.0) {
             *status = CRT_ERR_ZEROVOLT;

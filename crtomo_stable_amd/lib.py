@@ -165,6 +165,16 @@ class Context:
         _chk(self.lib.crt_get_precal(self._h, _ptr(elbg), _ptr(relbg), _ptr(kg)), "crt_get_precal")
         return elbg, relbg, kg
 
+    def set_inversion_kind(self, kind: int):
+        """0 complex inversion, 1 DC inversion (ldc: REAL*8 kpot/hpot/sens, bsendc,
+        cjggdc), 2 final phase improvement (lfpi: cjggdc on DBLE(sens)).  Discards
+        sigma and its results: set_sigma again afterwards."""
+        _chk(self.lib.crt_set_inversion_kind(self._h, int(kind)), "crt_set_inversion_kind")
+        self.inv_kind = int(kind)
+
+    def _pot_dtype(self):
+        return np.float64 if getattr(self, "inv_kind", 0) == 1 else np.complex128
+
     def set_sigma(self, sigma):
         s = np.ascontiguousarray(sigma, np.complex128)
         assert s.size == self.grid.elanz
@@ -179,7 +189,7 @@ class Context:
         g = self.grid
         if g is None or g.eanz == 0:
             raise ValueError("get_kpot needs the grid with its electrodes (set_grid)")
-        out = np.zeros((self.kwnanz, g.eanz, g.sanz), np.complex128)
+        out = np.zeros((self.kwnanz, g.eanz, g.sanz), self._pot_dtype())
         _chk(self.lib.crt_get_kpot(self._h, _ptr(out)), "crt_get_kpot")
         return out
 
@@ -194,7 +204,7 @@ class Context:
         g = self.grid
         if g is None or g.eanz == 0:
             raise ValueError("get_hpot needs the grid with its electrodes (set_grid)")
-        out = np.zeros((g.eanz, g.sanz), np.complex128)
+        out = np.zeros((g.eanz, g.sanz), self._pot_dtype())
         _chk(self.lib.crt_get_hpot(self._h, _ptr(out)), "crt_get_hpot")
         return out
 
@@ -204,14 +214,14 @@ class Context:
 
     def get_sens(self):
         """J as (manz, nanz) = reference sens(nanz, manz)."""
-        out = np.zeros((self.grid.elanz, self.survey.nanz), np.complex128)
+        out = np.zeros((self.grid.elanz, self.survey.nanz), self._pot_dtype())
         _chk(self.lib.crt_get_sens(self._h, _ptr(out)), "crt_get_sens")
         return out
 
     # ------------------------------------------------------- normal equations
     def set_sens(self, sens):
         """sens as (manz, nanz) = reference sens(nanz, manz)."""
-        s = np.ascontiguousarray(sens, np.complex128)
+        s = np.ascontiguousarray(sens, self._pot_dtype())
         manz, nanz = s.shape
         _chk(self.lib.crt_set_sens(self._h, nanz, manz, _ptr(s)), "crt_set_sens")
 

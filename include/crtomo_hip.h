@@ -58,6 +58,20 @@ int crt_get_precal(crt_ctx *ctx, double *elbg, double *relbg, double *kg);
 /* Complex conductivity per area element (sigmamod sigma). */
 int crt_set_sigma(crt_ctx *ctx, const double *sigma);
 
+/* Which of CRTomo's inversions the context serves (femmod ldc, invmod lfpi):
+ * 0 (default) the complex inversion; 1 the DC inversion (ldc): the forward assembles
+ * DBLE(sigma) (kompadc/scaldc/choldc/vredc, src/inv.f90:355-398), crt_voltages mode 1
+ * runs bvolti's DC branch (src/bvolti.f90:55-66), crt_sens mode 1 runs bsendc
+ * (src/bsendc.f90:1), crt_get_kpot/crt_get_hpot/crt_get_sens hand out REAL*8 kpotdc,
+ * hpotdc, sensdc and crt_set_sens takes a REAL*8 sensdc(nanz, manz); 2 the final phase
+ * improvement (lfpi).  For 1 and 2, crt_update/crt_cjg run cjggdc (src/cg_mod.f90:106)
+ * on the real Jacobian from DBLE(bvec) (DC) or DIMAG(bvec) (FPI), and crt_update stores
+ * dpar = i*dpar for FPI (src/update.f90:302-304); crt_blam0 takes the sensdc branch for 1.
+ * Not for 1/2: crt_buncert, bvolt/bsens (mode 0, DC), the FP32 and FSAI/learned CG
+ * forms (CRT_ERR_UNSUPPORTED).  A change of kind discards sigma and its results:
+ * call crt_set_sigma after it. */
+int crt_set_inversion_kind(crt_ctx *ctx, int kind);
+
 /* The forward loop of src/fem.f90:328-397 / src/inv.f90:415-484 for every
  * (electrode, wavenumber): kompab + scalab + chol + vre (or kompb + vre).
  * kpot stays on the device. */

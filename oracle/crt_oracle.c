@@ -927,6 +927,304 @@ void orc_cjggra(const orc_cg_problem *p, const double *cgfac, const zdbl *bvec0,
     free(r); free(pv); free(bv); free(ap);
 }
 
+/* ------------------------------------------------------------- DC twins */
+/* The real-valued inversion (ldc) and the final phase improvement (lfpi):
+ * kompadc/scaldc/choldc/kompbdc/vredc in the forward loop of src/inv.f90:355-398,
+ * rtrafo's DC branch, bvolti's DC branch, bsendc, and update/cjggdc. */
+
+/* src/kompadc.f90:70-125,128 (lsr off, no sink): real band with DBLE(sigma) */
+int orc_kompadc(const orc_grid *g, const zdbl *sigma, const double *elbg, const double *relbg,
+                const double *kg, int l, int k, double *a, double *b) {
+    const int maxn = g->maxn, npk = NPK(maxn), mb = g->mb, sanz = g->sanz;
+    memset(a, 0, sizeof(double) * (size_t)(mb + 1) * sanz);
+    memset(b, 0, sizeof(double) * (size_t)sanz);
+    int iel = 0;
+    for (int t = 0; t < g->typanz; ++t) {
+        int ntyp = g->typ[t], nkel = g->selanz[t];
+        for (int j = 0; j < g->nelanz[t]; ++j, ++iel) {
+            if (ntyp > 11) continue;
+            int ikl = 0;
+            for (int kk = 0; kk < nkel; ++kk) {
+                int nzp = g->nrel[iel * maxn + kk];
+                for (int ll = 0; ll <= kk; ++ll) {
+                    int nnp = g->nrel[iel * maxn + ll];
+                    if (abs(nzp - nnp) > mb) return 19;
+                    int im = imax_(nzp, nnp) * mb + imin_(nzp, nnp);
+                    double dum, dum2;
+                    if (ntyp == 11) {
+                        int rel = iel - g->elanz;
+                        dum = relbg[(size_t)ikl * g->relanz + rel] *
+                              kg[((size_t)(k - 1) * g->eanz + (l - 1)) * g->relanz + rel];
+                        dum2 = creal(sigma[g->rnr[rel] - 1]);
+                    } else {
+                        dum = elbg[((size_t)(k - 1) * npk + ikl) * g->elanz + iel];
+                        dum2 = creal(sigma[iel]);
+                    }
+                    a[im - 1] = a[im - 1] + dum * dum2;
+                    ++ikl;
+                }
+            }
+        }
+    }
+    b[g->enr[l - 1] - 1] = -1.0;
+    return 0;
+}
+
+/* src/scaldc.f90:49-75: unit diagonal, errnr 27 on a non-positive diagonal */
+int orc_scaldc(int sanz, int mb, double *a, double *b, double *fak) {
+    for (int i = 1; i <= sanz; ++i) {
+        int idi = i * (mb + 1);
+        double dum = a[idi - 1];
+        if (dum <= 0.0) return 27;
+        a[idi - 1] = 1.0;
+        fak[i - 1] = 1.0 / sqrt(dum);
+        b[i - 1] = b[i - 1] * fak[i - 1];
+        if (i == 1) continue;
+        int i0 = i * mb, ja = imax_(1, i - mb);
+        for (int j = ja; j <= i - 1; ++j) a[i0 + j - 1] = a[i0 + j - 1] * fak[i - 1] * fak[j - 1];
+    }
+    return 0;
+}
+
+/* real band Cholesky, row oriented, errnr 28 on s <= 0 (src/choldc.f90:44-76) */
+int orc_choldc(int sanz, int mb, double *a) {
+    const int m1 = mb + 1;
+    for (int i = 1; i <= sanz; ++i) {
+        int idi = i * m1, fi = imax_(1, i - mb), i0 = idi - i;
+        for (int j = fi; j <= i; ++j) {
+            int ij = i0 + j, j0 = j * mb;
+            double s = a[ij - 1];
+            for (int k = fi; k <= j - 1; ++k) s = s - a[i0 + k - 1] * a[j0 + k - 1];
+            if (j < i) {
+                a[ij - 1] = s / a[j * m1 - 1];
+            } else {
+                if (s <= 0.0) return 28;
+                a[idi - 1] = sqrt(s);
+            }
+        }
+    }
+    return 0;
+}
+
+/* pot = -A^{-1} b in real arithmetic (src/vredc.f90:45-76) */
+void orc_vredc(int sanz, int mb, const double *a, const double *b, double *pot) {
+    const int m1 = mb + 1;
+    int idi = 0;
+    for (int i = 1; i <= sanz; ++i) {
+        idi = i * m1;
+        double s = b[i - 1];
+        int i0 = idi - i, jlow = imax_(1, i - mb);
+        for (int j = jlow; j <= i - 1; ++j) s = s - a[i0 + j - 1] * pot[j - 1];
+        pot[i - 1] = s / a[idi - 1];
+    }
+    for (int i = sanz; i >= 1; --i) {
+        pot[i - 1] = -pot[i - 1] / a[idi - 1];
+        int jlow = imax_(1, i - mb), i0 = idi - i;
+        for (int j = jlow; j <= i - 1; ++j) pot[j - 1] = pot[j - 1] + a[i0 + j - 1] * pot[i - 1];
+        idi -= m1;
+    }
+}
+
+/* DC forward loop (src/inv.f90:355-398; kompbdc src/kompbdc.f90:38-43):
+ * kpotdc(j,l,k) = DBLE(pot(j)) * fak(j), layout [k][l][j] as orc_forward */
+int orc_forward_dc(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
+                   const double *relbg, const double *kg, int lbeta, double *kpotdc) {
+    const int sanz = g->sanz, mb = g->mb, eanz = g->eanz;
+    double *a = (double *)malloc(sizeof(double) * (size_t)(mb + 1) * sanz);
+    double *b = (double *)malloc(sizeof(double) * (size_t)sanz);
+    double *pot = (double *)malloc(sizeof(double) * (size_t)sanz);
+    double *fak = (double *)malloc(sizeof(double) * (size_t)sanz);
+    int err = 0;
+    for (int k = 1; k <= kwnanz && !err; ++k)
+        for (int l = 1; l <= eanz && !err; ++l) {
+            if (lbeta || l == 1) {
+                err = orc_kompadc(g, sigma, elbg, relbg, kg, l, k, a, b);
+                if (!err) err = orc_scaldc(sanz, mb, a, b, fak);
+                if (!err) err = orc_choldc(sanz, mb, a);
+            } else {
+                for (int i = 0; i < sanz; ++i) b[i] = 0.0;
+                b[g->enr[l - 1] - 1] = -fak[g->enr[l - 1] - 1];
+            }
+            if (err) break;
+            orc_vredc(sanz, mb, a, b, pot);
+            double *kp = kpotdc + ((size_t)(k - 1) * eanz + (l - 1)) * sanz;
+            for (int j = 0; j < sanz; ++j) kp[j] = pot[j] * fak[j];
+        }
+    free(a); free(b); free(pot); free(fak);
+    return err;
+}
+
+/* src/rtrafo.f90:48-66 (ldc) */
+void orc_rtrafo_dc(int sanz, int eanz, int kwnanz, const double *kwnwi, const double *kpotdc, double *hpotdc) {
+    const double pi = acos(-1.0);
+    for (int l = 0; l < eanz; ++l)
+        for (int j = 0; j < sanz; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < kwnanz; ++k) s = s + kpotdc[((size_t)k * eanz + l) * sanz + j] * kwnwi[k];
+            hpotdc[(size_t)l * sanz + j] = s / pi;
+        }
+}
+
+/* src/bvolti.f90:55-66,80-89 (ldc): real superposition, volt = dcmplx(V), sigmaa = -log(volt) */
+int orc_bvolti_dc(int nanz, const int *strnr, const int *vnr, const int *enr, int sanz,
+                  const double *hpotdc, zdbl *volt, zdbl *sigmaa) {
+    for (int i = 0; i < nanz; ++i) {
+        int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+        int e3 = vnr[i] % 10000, e4 = (vnr[i] - e3) / 10000;
+#define HD(e, c) ((double)(imin_(e, 1) * imin_(c, 1)) * \
+                  hpotdc[(size_t)(imax_(c, 1) - 1) * sanz + enr[imax_(e, 1) - 1] - 1])
+        double d1 = HD(e4, e2), d2 = HD(e4, e1), d3 = HD(e3, e2), d4 = HD(e3, e1);
+#undef HD
+        volt[i] = CMPLX((d1 - d2) - (d3 - d4), 0.0);
+        if (cabs(volt[i]) == 0.0) return 82;
+        sigmaa[i] = -clog(volt[i]);
+    }
+    return 0;
+}
+
+/* src/bsendc.f90:93-175 (swrtr = 1, mnr = identity): sensdc nanz x elanz column-major */
+void orc_bsendc(const orc_grid *g, int nanz, const int *strnr, const int *vnr, int kwnanz,
+                const double *kwnwi, const double *kpotdc, const double *elbg, const zdbl *sigma,
+                const zdbl *volt, double *sensdc) {
+    const double pi = acos(-1.0);
+    const int sanz = g->sanz, eanz = g->eanz, npk = NPK(g->maxn);
+    memset(sensdc, 0, sizeof(double) * (size_t)nanz * g->elanz);
+    for (int i = 0; i < nanz; ++i) {
+        int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+        int e3 = vnr[i] % 10000, e4 = (vnr[i] - e3) / 10000;
+        double sup[4] = {0.0, 0.0, 0.0, 0.0};
+        int iel = 0;
+        for (int t = 0; t < g->typanz; ++t) {
+            const int nkel = g->selanz[t];
+            if (g->typ[t] > 10) continue;
+            for (int j = 0; j < g->nelanz[t]; ++j, ++iel) {
+                double dum = 0.0;
+                for (int k = 0; k < kwnanz; ++k) {
+                    const double *kp = kpotdc + (size_t)k * eanz * sanz;
+                    double h = 0.0;
+                    for (int mi = 0; mi < nkel; ++mi) {
+                        int nzp = g->nrel[iel * g->maxn + mi];
+                        for (int mj = 0; mj < nkel; ++mj) {
+                            int nnp = g->nrel[iel * g->maxn + mj];
+                            int ia = imax_(mi, mj) + 1, ib = imin_(mi, mj) + 1;
+                            int imn = ia * (ia - 1) / 2 + ib;
+                            if (e1 > 0) sup[0] = kp[(size_t)(e1 - 1) * sanz + nnp - 1];
+                            if (e2 > 0) sup[1] = kp[(size_t)(e2 - 1) * sanz + nnp - 1];
+                            if (e3 > 0) sup[2] = kp[(size_t)(e3 - 1) * sanz + nzp - 1];
+                            if (e4 > 0) sup[3] = kp[(size_t)(e4 - 1) * sanz + nzp - 1];
+                            double d = (sup[1] - sup[0]) * (sup[3] - sup[2]);
+                            h = h + elbg[((size_t)k * npk + imn - 1) * g->elanz + iel] * d;
+                        }
+                    }
+                    dum = dum + h * kwnwi[k];
+                }
+                dum = dum / pi;
+                sensdc[(size_t)iel * nanz + i] = sensdc[(size_t)iel * nanz + i] +
+                                                 dum * creal(sigma[iel] / volt[i]);
+            }
+        }
+    }
+}
+
+/* real Jacobian entry of the DC (kind 1: sensdc) or FPI (kind 2: DBLE(sens)) branches */
+static double jre(const orc_cg_problem *p, int kind, const double *sensdc, int i, int j) {
+    const size_t o = (size_t)j * p->nanz + i;
+    return kind == 1 ? sensdc[o] : creal(p->sens[o]);
+}
+
+/* update RHS and scaling, DC/FPI branches (src/update.f90:182-191,243-251,261-270,276-290) */
+void orc_update_rhs_dc(const orc_cg_problem *p, int kind, const double *sensdc, double *cgfac, zdbl *bvec) {
+    const int nanz = p->nanz, manz = p->manz;
+    rmatvec(p, NULL, p->par, bvec);
+    for (int j = 0; j < manz; ++j) {
+        double dum = 0.0;
+        for (int i = 0; i < nanz; ++i) {
+            double a = jre(p, kind, sensdc, i, j);
+            dum = dum + a * a * p->wmatd[i] * (double)p->wdfak[i];
+        }
+        dum = dum + p->lam * rdiag(p, j);
+        cgfac[j] = 1.0 / sqrt(dum);
+    }
+    for (int j = 0; j < manz; ++j) {
+        zdbl cd = 0.0;
+        for (int i = 0; i < nanz; ++i)
+            cd = cd + (jre(p, kind, sensdc, i, j) * p->wmatd[i] * (double)p->wdfak[i]) * (p->dat[i] - p->sigmaa[i]);
+        bvec[j] = cd - p->lam * bvec[j];
+        bvec[j] = bvec[j] * cgfac[j];
+    }
+}
+
+/* R p of bpdc (src/cg_mod.f90:247-264) / bpdctri (:288-296), real, scaled by cgfac */
+static void rmatvec_dc(const orc_cg_problem *p, const double *c, const double *v, double *out) {
+    const int manz = p->manz, nx = p->nx;
+    const double *s = p->smatm;
+    for (int i = 1; i <= manz; ++i) {
+        double dum = 0.0;
+        if (p->ltri == 0) {
+            if (i > 1) dum = v[i - 2] * s[manz + i - 2] * c[i - 2];
+            if (i < manz) dum = dum + v[i] * s[manz + i - 1] * c[i];
+            if (i > nx) dum = dum + v[i - nx - 1] * s[2 * manz + i - nx - 1] * c[i - nx - 1];
+            if (i < manz - nx + 1) dum = dum + v[i + nx - 1] * s[2 * manz + i - 1] * c[i + nx - 1];
+            out[i - 1] = dum + v[i - 1] * s[i - 1] * c[i - 1];
+        } else {
+            for (int ij = 0; ij < p->maxn; ++ij) {
+                int in = p->nachbar[(size_t)ij * manz + i - 1];
+                if (in != 0) dum = dum + v[in - 1] * s[(size_t)ij * manz + i - 1] * c[in - 1];
+            }
+            out[i - 1] = dum + v[i - 1] * s[(size_t)p->maxn * manz + i - 1] * c[i - 1];
+        }
+    }
+}
+
+/* cjggdc (src/cg_mod.f90:106-196) with bapdc (:198-226), bpdc/bpdctri, bbdc (:385-419).
+ * bvec0 is the complex scaled RHS; the CG runs on its real (kind 1) or imaginary
+ * (kind 2) part.  dpar = dcmplx(real CG solution), unscaled. */
+void orc_cjggdc(const orc_cg_problem *p, int kind, const double *sensdc, const double *cgfac,
+                const zdbl *bvec0, zdbl *dpar, double *cgres, int *ncg_out) {
+    const int manz = p->manz, nanz = p->nanz;
+    double *r = (double *)malloc(sizeof(double) * manz), *pv = (double *)calloc(manz, sizeof(double));
+    double *bv = (double *)malloc(sizeof(double) * manz), *ap = (double *)malloc(sizeof(double) * nanz);
+    double *dp = (double *)calloc(manz, sizeof(double));
+    double dr = 0, dr0 = 0, dr1 = 0, beta, alpha;
+    int ncg = p->ncgmax;
+    for (int j = 0; j < manz; ++j) r[j] = kind == 2 ? cimag(bvec0[j]) : creal(bvec0[j]);
+    for (int k = 1; k <= p->ncgmax; ++k) {
+        dr = 0.0;
+        for (int j = 0; j < manz; ++j) dr = dr + r[j] * r[j];
+        if (k == 1) {
+            dr0 = dr * p->eps;
+            beta = 0.0;
+        } else {
+            beta = dr / dr1;
+        }
+        if (dr <= dr0) { ncg = k - 1; break; }
+        for (int j = 0; j < manz; ++j) pv[j] = r[j] + beta * pv[j];
+        for (int i = 0; i < nanz; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < manz; ++j) s = s + pv[j] * jre(p, kind, sensdc, i, j) * cgfac[j];
+            ap[i] = s;
+        }
+        rmatvec_dc(p, cgfac, pv, bv);
+        for (int j = 0; j < manz; ++j) {
+            double dum = 0.0;
+            for (int i = 0; i < nanz; ++i)
+                dum = dum + jre(p, kind, sensdc, i, j) * p->wmatd[i] * (double)p->wdfak[i] * ap[i];
+            bv[j] = dum + p->lam * bv[j];
+            bv[j] = bv[j] * cgfac[j];
+        }
+        dr1 = 0.0;
+        for (int j = 0; j < manz; ++j) dr1 = dr1 + pv[j] * bv[j];
+        alpha = dr / dr1;
+        for (int j = 0; j < manz; ++j) { dp[j] = dp[j] + alpha * pv[j]; r[j] = r[j] - alpha * bv[j]; }
+        dr1 = dr;
+        cgres[k] = (double)(float)(p->eps * dr / dr0);
+    }
+    cgres[0] = (double)(float)ncg;
+    *ncg_out = ncg;
+    for (int j = 0; j < manz; ++j) dpar[j] = CMPLX(dp[j], 0.0);
+    free(r); free(pv); free(bv); free(ap); free(dp);
+}
+
 /* -------------------------------------------------------------- CutMcK */
 /* Cuthill-McKee with start nodes of minimal degree, best band width kept
  * (cutmckee/cutmck.f:126-300).  Returns mb, or -1 on a disconnected graph. */

@@ -120,6 +120,27 @@ void orc_cjggra(const orc_cg_problem *p, const double *cgfac, const zdbl *bvec,
 /* one CG matvec  out = C (J^H W J + lam R) C p  (src/cg_mod.f90:520-545,547-577,699-726) */
 void orc_cg_matvec(const orc_cg_problem *p, const double *cgfac, const zdbl *pvec, zdbl *ap, zdbl *out);
 
+/* DC twins (real-valued inversion ldc, final phase improvement lfpi):
+ * src/kompadc.f90, scaldc.f90, choldc.f90, vredc.f90, kompbdc.f90 and the DC forward loop
+ * src/inv.f90:355-398 (kpotdc [k][l][j]); rtrafo's and bvolti's DC branches; bsendc;
+ * update's DC/FPI branches (kind 1: sensdc, kind 2: DBLE(sens)) and cjggdc. */
+int orc_kompadc(const orc_grid *g, const zdbl *sigma, const double *elbg, const double *relbg,
+                const double *kg, int l, int k, double *a, double *b);
+int orc_scaldc(int sanz, int mb, double *a, double *b, double *fak);
+int orc_choldc(int sanz, int mb, double *a);
+void orc_vredc(int sanz, int mb, const double *a, const double *b, double *pot);
+int orc_forward_dc(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
+                   const double *relbg, const double *kg, int lbeta, double *kpotdc);
+void orc_rtrafo_dc(int sanz, int eanz, int kwnanz, const double *kwnwi, const double *kpotdc, double *hpotdc);
+int orc_bvolti_dc(int nanz, const int *strnr, const int *vnr, const int *enr, int sanz,
+                  const double *hpotdc, zdbl *volt, zdbl *sigmaa);
+void orc_bsendc(const orc_grid *g, int nanz, const int *strnr, const int *vnr, int kwnanz,
+                const double *kwnwi, const double *kpotdc, const double *elbg, const zdbl *sigma,
+                const zdbl *volt, double *sensdc);
+void orc_update_rhs_dc(const orc_cg_problem *p, int kind, const double *sensdc, double *cgfac, zdbl *bvec);
+void orc_cjggdc(const orc_cg_problem *p, int kind, const double *sensdc, const double *cgfac,
+                const zdbl *bvec0, zdbl *dpar, double *cgres, int *ncg);
+
 /* Linearised model uncertainty (src/bmcm_mod.f90:49-175 buncert, complex case, lam = p->lam):
  * bata (:176-289), bata_reg (:291-413, the reference's own write order per ltri 0, 1/2/5..14,
  * 3/4), bmcm (:415-615: chold + linvd + lower fill, or Gauss_dble when gauss), bres

@@ -297,6 +297,59 @@ def cjggra(p: CGProblem, cgfac, bvec):
     return dpar, cgres, ncg.value
 
 
+def forward_dc(grid, sigma, elbg, relbg, kg, lbeta):
+    """kpotdc (kwnanz, eanz, sanz) of the DC forward loop (src/inv.f90:355-398)."""
+    og = OracleGrid(grid)
+    sigma = np.ascontiguousarray(sigma, np.complex128)
+    K = elbg.shape[0]
+    kpot = np.zeros((K, grid.eanz, grid.sanz))
+    err = lib().orc_forward_dc(og.ref, ptr(sigma), K, ptr(elbg), ptr(relbg), ptr(kg), int(lbeta), ptr(kpot))
+    _chk(err, "forward_dc")
+    return kpot
+
+
+def rtrafo_dc(kpotdc, kwnwi):
+    K, E, S = kpotdc.shape
+    hpot = np.zeros((E, S))
+    lib().orc_rtrafo_dc(S, E, K, ptr(np.ascontiguousarray(kwnwi, np.float64)), ptr(np.ascontiguousarray(kpotdc)),
+                        ptr(hpot))
+    return hpot
+
+
+def bvolti_dc(grid, survey, hpotdc):
+    volt = np.zeros(survey.nanz, np.complex128); sa = np.zeros_like(volt)
+    err = lib().orc_bvolti_dc(survey.nanz, ptr(survey.strnr), ptr(survey.vnr), ptr(grid.enr), grid.sanz,
+                              ptr(np.ascontiguousarray(hpotdc)), ptr(volt), ptr(sa))
+    _chk(err, "bvolti_dc")
+    return volt, sa
+
+
+def bsendc(grid, survey, kwnwi, kpotdc, elbg, sigma, volt):
+    """sensdc (manz, nanz) = the reference's sensdc(nanz, manz)."""
+    og = OracleGrid(grid)
+    sens = np.zeros((grid.elanz, survey.nanz))
+    lib().orc_bsendc(og.ref, survey.nanz, ptr(survey.strnr), ptr(survey.vnr), kpotdc.shape[0],
+                     ptr(np.ascontiguousarray(kwnwi, np.float64)), ptr(np.ascontiguousarray(kpotdc)),
+                     ptr(elbg), ptr(np.ascontiguousarray(sigma, np.complex128)),
+                     ptr(np.ascontiguousarray(volt, np.complex128)), ptr(sens))
+    return sens
+
+
+def update_dc(p: CGProblem, kind, sensdc=None):
+    """The DC (kind 1, sensdc (manz, nanz) real) or FPI (kind 2, DBLE of p's sens)
+    branches of update + cjggdc: (cgfac, bvec, dpar, cgres, ncg) with dpar scaled
+    as src/update.f90:302-307 (FPI: dpar = i * CG solution)."""
+    sd = None if sensdc is None else np.ascontiguousarray(sensdc, np.float64)
+    cgfac = np.zeros(p.manz); bvec = np.zeros(p.manz, np.complex128)
+    lib().orc_update_rhs_dc(C.byref(p.s), int(kind), ptr(sd), ptr(cgfac), ptr(bvec))
+    dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1); ncg = _i()
+    lib().orc_cjggdc(C.byref(p.s), int(kind), ptr(sd), ptr(cgfac), ptr(bvec), ptr(dpar), ptr(cgres),
+                     C.byref(ncg))
+    if kind == 2:
+        dpar = 1j * dpar.real
+    return cgfac, bvec, dpar * cgfac, cgres, ncg.value
+
+
 def cg_matvec(p: CGProblem, cgfac, pvec):
     ap = np.zeros(p.nanz, np.complex128); out = np.zeros(p.manz, np.complex128)
     lib().orc_cg_matvec(C.byref(p.s), ptr(cgfac), ptr(np.ascontiguousarray(pvec, np.complex128)), ptr(ap),
@@ -388,11 +441,31 @@ class RefSession:
         ref().ref_bsmatm(ltri, nx, nz, _d(alfx), _d(alfz), ptr(sm), ptr(nb))
         return sm, nb
 
-    def blam0(self, fpi=False, alfx=1.0, alfz=1.0):
-        """lammax of src/blam0.f90 on the state of the last update() call."""
+    def blam0(self, fpi=False, alfx=1.0, alfz=1.0, dc=False):
+        """lammax of src/blam0.f90 on the state of the last update() call (dc: the
+        ldc branch over sensdc)."""
         f = ref().ref_blam0
         f.restype = C.c_double
-        return f(int(bool(fpi)), _d(alfx), _d(alfz))
+        return f(int(bool(fpi)), _d(alfx), _d(alfz), int(bool(dc)))
+
+    def forward_dc(self, sigma):
+        g = self.grid
+        kpot = np.zeros((self.K, g.eanz, g.sanz))
+        s = np.ascontiguousarray(sigma, np.complex128)
+        _chk(ref().ref_forward_dc(ptr(s), ptr(kpot)), "ref_forward_dc")
+        return kpot
+
+    def voltages_dc(self):
+        g = self.grid
+        hpot = np.zeros((g.eanz, g.sanz))
+        volt = np.zeros(self.survey.nanz, np.complex128); sa = np.zeros_like(volt)
+        _chk(ref().ref_voltages_dc(ptr(hpot), ptr(volt), ptr(sa)), "ref_voltages_dc")
+        return hpot, volt, sa
+
+    def sens_dc(self):
+        sens = np.zeros((self.grid.elanz, self.survey.nanz))
+        ref().ref_sens_dc(ptr(sens))
+        return sens
 
     def buncert(self, lam, sigma, lres=True, lcov2=True, gauss=False):
         """src/bmcm_mod.f90:49 buncert on the state of the last update() call (J, W)
@@ -425,13 +498,16 @@ class RefSession:
         finally:
             shutil.rmtree(d, ignore_errors=True)
 
-    def update(self, p: CGProblem):
+    def update(self, p: CGProblem, kind=0, sensdc=None):
+        """src/update.f90 -> cjg; kind 1: DC (ldc, sensdc (manz, nanz)), kind 2: FPI (lfpi)."""
         a = p.arrs
+        sd = np.zeros(1) if sensdc is None else np.ascontiguousarray(sensdc, np.float64)
         cgfac = np.zeros(p.manz); bvec = np.zeros(p.manz, np.complex128)
         dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1); ncg = _i()
         _chk(ref().ref_update(p.s.ncgmax, _d(p.s.eps), _d(p.s.lam), ptr(a["sens"]), ptr(a["wmatd"]),
                               ptr(a["wdfak"]), ptr(a["dat"]), ptr(a["sigmaa"]), ptr(a["par"]), ptr(cgfac),
-                              ptr(bvec), ptr(dpar), ptr(cgres), C.byref(ncg)), "ref_update")
+                              ptr(bvec), ptr(dpar), ptr(cgres), C.byref(ncg), int(kind), ptr(sd)),
+             "ref_update")
         return cgfac, bvec, dpar, cgres, ncg.value
 
 

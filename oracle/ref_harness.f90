@@ -294,6 +294,76 @@ contains
     ref_sens = 0
   end function ref_sens
 
+  ! DC forward loop exactly as src/inv.f90:355-398 (lsr off, no Dirichlet), serial
+  integer(c_int) function ref_forward_dc(sigma_, kpotdc_) bind(C, name='ref_forward_dc')
+    complex(c_double_complex), intent(in) :: sigma_(*)
+    real(c_double), intent(out) :: kpotdc_(*)
+    real(kind(0d0)), allocatable :: la(:), lb(:), lfak(:)
+    complex(kind(0d0)), allocatable :: lpot(:)
+    integer :: j, l, k, n
+    external kompadc, scaldc, choldc, kompbdc, vredc
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = sigma_(1:elanz)
+    if (allocated(kpotdc)) deallocate(kpotdc)
+    allocate(kpotdc(sanz, eanz, kwnanz))
+    allocate(la((mb + 1) * sanz), lb(sanz), lpot(sanz), lfak(sanz))
+    errnr = 0
+    do k = 1, kwnanz
+       do l = 1, eanz
+          if (lbeta .or. l == 1) then
+             call kompadc(l, k, la, lb)
+             call scaldc(la, lb, lfak)
+             call choldc(la)
+          else
+             call kompbdc(l, lb, lfak)
+          end if
+          call vredc(la, lb, lpot)
+          do j = 1, sanz
+             kpotdc(j, l, k) = dble(lpot(j)) * lfak(j)
+          end do
+       end do
+    end do
+    n = size(kpotdc)
+    kpotdc_(1:n) = reshape(kpotdc, (/n/))
+    ref_forward_dc = errnr
+  end function ref_forward_dc
+
+  ! rtrafo + bvolti with ldc set (src/rtrafo.f90:48-66, src/bvolti.f90:55-66)
+  integer(c_int) function ref_voltages_dc(hpotdc_, volt_, sigmaa_) bind(C, name='ref_voltages_dc')
+    real(c_double), intent(out) :: hpotdc_(*)
+    complex(c_double_complex), intent(out) :: volt_(*), sigmaa_(*)
+    integer :: n
+    external rtrafo, bvolti
+    if (allocated(hpotdc)) deallocate(hpotdc)
+    allocate(hpotdc(sanz, eanz))
+    ldc = .true.
+    errnr = 0
+    call rtrafo()
+    call bvolti()
+    ldc = .false.
+    n = size(hpotdc); hpotdc_(1:n) = reshape(hpotdc, (/n/))
+    volt_(1:nanz) = volt; sigmaa_(1:nanz) = sigmaa
+    ref_voltages_dc = errnr
+  end function ref_voltages_dc
+
+  ! bsendc (src/bsendc.f90:1) on the state of ref_forward_dc / ref_voltages_dc
+  integer(c_int) function ref_sens_dc(sensdc_) bind(C, name='ref_sens_dc')
+    real(c_double), intent(out) :: sensdc_(*)
+    integer :: j, n
+    external bsendc
+    if (allocated(mnr)) deallocate(mnr)
+    allocate(mnr(elanz))
+    do j = 1, elanz
+       mnr(j) = j
+    end do
+    if (allocated(sensdc)) deallocate(sensdc)
+    allocate(sensdc(nanz, manz))
+    call bsendc(.false.)
+    n = size(sensdc); sensdc_(1:n) = reshape(sensdc, (/n/))
+    ref_sens_dc = 0
+  end function ref_sens_dc
+
   integer(c_int) function ref_chol(sanz_, mb_, a_) bind(C, name='ref_chol')
     integer(c_int), value :: sanz_, mb_
     complex(c_double_complex), intent(inout) :: a_(*)
@@ -339,8 +409,9 @@ contains
 
   ! one model update through src/update.f90:27 (complex, ltri 0/1/2, no prior)
   integer(c_int) function ref_update(ncgmax_, eps_, lam_, sens_, wmatd_, wdfak_, dat_, sigmaa_, &
-       par_, cgfac_, bvec_, dpar_, cgres_, ncg_) bind(C, name='ref_update')
-    integer(c_int), value :: ncgmax_
+       par_, cgfac_, bvec_, dpar_, cgres_, ncg_, kind_, sensdc_) bind(C, name='ref_update')
+    integer(c_int), value :: ncgmax_, kind_
+    real(c_double), intent(in) :: sensdc_(*)
     real(c_double), value :: eps_, lam_
     complex(c_double_complex), intent(in) :: sens_(*), dat_(*), sigmaa_(*), par_(*)
     real(c_double), intent(in) :: wmatd_(*)
@@ -353,6 +424,13 @@ contains
     if (allocated(sens)) deallocate(sens)
     allocate(sens(nanz, manz))
     sens = reshape(sens_(1:nanz * manz), (/nanz, manz/))
+    if (allocated(sensdc)) deallocate(sensdc)
+    allocate(sensdc(nanz, manz))
+    if (kind_ == 1) then
+       sensdc = reshape(sensdc_(1:nanz * manz), (/nanz, manz/))
+    else
+       sensdc = 0d0
+    end if
     if (allocated(wmatd)) deallocate(wmatd, wdfak, dat, par, dpar, dpar2)
     allocate(wmatd(nanz), wdfak(nanz), dat(nanz), par(manz), dpar(manz), dpar2(manz))
     wmatd = wmatd_(1:nanz); wdfak = wdfak_(1:nanz); dat = dat_(1:nanz)
@@ -364,6 +442,8 @@ contains
     if (allocated(cgres)) deallocate(cgres, cgres2, bvec, cgfac)
     llam = .false.; lprior = .false.; lw_ref = .false.; ldiff = .false.; lfpi = .false.
     ldc = .false.; lphi0 = .false.; lverb = .false.
+    ! kind 1: the DC inversion (ldc, sensdc); kind 2: final phase improvement (lfpi)
+    ldc = kind_ == 1; lfpi = kind_ == 2
     lam = lam_; eps = eps_; step = 1d0
     ! pass 1: ncgmax = 0 leaves the scaled RHS in bvec
     ncgmax = 0
@@ -379,20 +459,21 @@ contains
     dpar_(1:manz) = dpar
     cgres_(1:ncgmax + 1) = cgres(1:ncgmax + 1)
     ncg_ = ncg
+    ldc = .false.; lfpi = .false.
     ref_update = errnr
   end function ref_update
 
   ! starting regularisation parameter through src/blam0.f90:12, on the state that
   ! ref_update left (sens, wmatd, wdfak); computed, not preset (llamf = 0)
-  real(c_double) function ref_blam0(lfpi_, alfx_, alfz_) bind(C, name='ref_blam0')
-    integer(c_int), value :: lfpi_
+  real(c_double) function ref_blam0(lfpi_, alfx_, alfz_, ldc_) bind(C, name='ref_blam0')
+    integer(c_int), value :: lfpi_, ldc_
     real(c_double), value :: alfx_, alfz_
     external blam0
     llamf = 0; lamnull_cri = 0d0; lamnull_fpi = 0d0
-    lfpi = lfpi_ /= 0; ldc = .false.; lverb = .false.
+    lfpi = lfpi_ /= 0; ldc = ldc_ /= 0; lverb = .false.
     alfx = alfx_; alfz = alfz_
     call blam0()
-    lfpi = .false.
+    lfpi = .false.; ldc = .false.
     ref_blam0 = lammax
   end function ref_blam0
 

@@ -126,6 +126,52 @@ def main(names):
                 out[f"q{ltri}_{k}"] = v
             out[f"q{ltri}_gauss"] = gauss
         np.savez_compressed(os.path.join(HERE, "ref_buncert.npz"), **out)
+    if want("dc"):
+        # DC inversion (ldc) and final phase improvement (lfpi): kompadc..vredc forward,
+        # rtrafo/bvolti DC branches, bsendc, update + cjggdc, blam0's ldc branch; every
+        # second measurement with swapped potential electrodes (negative voltages); the
+        # FPI update runs on the complex Jacobian of ref_small_quad / ref_small_tri
+        out = {}
+        for ltri in (0, 1):
+            w = configs.small_quad(cutmck=O.cutmck) if ltri == 0 else \
+                configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+            s = w.survey
+            vnr = s.vnr.copy(); e3, e4 = vnr % 10000, vnr // 10000
+            sw = (np.arange(vnr.size) % 2 == 0) & (e3 > 0)
+            vnr[sw] = e3[sw] * 10000 + e4[sw]
+            s = G.Survey(strnr=s.strnr.copy(), vnr=vnr)
+            R = O.RefSession(w.grid, s)
+            R.rwaven(); R.precal()
+            kp = R.forward_dc(w.sigma)
+            hp, volt, sa = R.voltages_dc()
+            sd = R.sens_dc()
+            p = f"dc{ltri}_"
+            out.update({p + "vnr": vnr, p + "kpot_src1": kp[:, 0, :], p + "hpot_elec": hp[:, w.grid.enr - 1],
+                        p + "volt": volt, p + "sigmaa": sa, p + "sens": sd})
+            Rc = O.RefSession(w.grid, w.survey)
+            Rc.rwaven(); Rc.precal()
+            Rc.forward(w.sigma, nthreads=4)
+            _, vc, sac = Rc.voltages(True)
+            sc = Rc.sens(True)
+            smatm, nachbar = Rc.bsmatm(ltri, w.grid.nx, w.grid.nz, 1.0, 1.0)
+            rng = np.random.default_rng(20 + ltri)
+            wmatd = 1.0 / (0.03 + 0.01 * rng.random(sac.size)) ** 2
+            par = np.log(w.sigma) + 0.05 * (rng.standard_normal(w.sigma.size) + 1j * rng.standard_normal(w.sigma.size))
+            out[p + "wmatd"], out[p + "par"] = wmatd, par
+            for kind in (1, 2):
+                sa_k = sa if kind == 1 else sac
+                dat = sa_k + 0.02 * (rng.standard_normal(sa_k.size) + 0.1j * rng.standard_normal(sa_k.size))
+                cp = O.CGProblem(np.zeros_like(sc) if kind == 1 else sc, wmatd, np.ones(sa_k.size, np.int32), dat,
+                                 sa_k, par, smatm, ltri, w.grid.nx, 3.0, 1e-12, 400,
+                                 nachbar=nachbar if ltri else None, maxn=w.grid.maxn)
+                cgfac, bvec, dpar, cgres, ncg = Rc.update(cp, kind=kind, sensdc=sd if kind == 1 else None)
+                cgres[ncg + 1:] = 0.0
+                q = f"{p}k{kind}_"
+                out.update({q + "dat": dat, q + "cgfac": cgfac, q + "bvec": bvec, q + "dpar": dpar,
+                            q + "cgres": cgres, q + "ncg": ncg})
+                if kind == 1:
+                    out[q + "lam0"] = Rc.blam0(dc=True)
+        np.savez_compressed(os.path.join(HERE, "ref_dc.npz"), **out)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -465,3 +465,107 @@ def test_oracle_buncert_algebra(oracle_lib):
     assert np.abs(m1["cov2"] - m1["res"] @ m1["cov"]).max() <= 1e-12 * np.abs(m1["cov2"]).max()
     err, mg = O.buncert(cp1, gauss=True)
     assert err == 0 and np.abs(mg["cov"] - m1["cov"]).max() <= 1e-10 * np.abs(m1["cov"]).max()
+
+
+def _dc_workload(O, ltri):
+    w = configs.small_quad(cutmck=O.cutmck) if ltri == 0 else \
+        configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
+    gold = np.load(os.path.join(GOLD, "ref_dc.npz"))
+    s = G.Survey(strnr=w.survey.strnr.copy(), vnr=gold[f"dc{ltri}_vnr"].astype(np.int32))
+    return w, s, gold
+
+
+@pytest.mark.parametrize("ltri", [0, 1])
+def test_oracle_dc_vs_reference_golden(oracle_lib, ltri):
+    """DC twins (kompadc/scaldc/choldc/kompbdc/vredc, rtrafo and bvolti DC branches,
+    bsendc, update + cjggdc for ldc and lfpi, blam0's ldc branch) against FP64 vectors
+    of the reference build: the same operation order, so equal results (bsendc divides
+    sigma by a complex volt, the oracle's complex division rounds differently)."""
+    O = oracle_lib
+    w, s, gold = _dc_workload(O, ltri)
+    g, p = w.grid, f"dc{ltri}_"
+    kwn, kwi, _, _ = O.rwaven(g, s)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    kp = O.forward_dc(g, w.sigma, elbg, relbg, kg, lb)
+    assert np.array_equal(kp[:, 0, :], gold[p + "kpot_src1"])
+    hp = O.rtrafo_dc(kp, kwi)
+    assert np.array_equal(hp[:, g.enr - 1], gold[p + "hpot_elec"])
+    volt, sa = O.bvolti_dc(g, s, hp)
+    assert np.array_equal(volt, gold[p + "volt"]) and np.array_equal(sa, gold[p + "sigmaa"])
+    assert (volt.real < 0).any() and np.all(sa.imag[volt.real < 0] == -np.pi)
+    sd = O.bsendc(g, s, kwi, kp, elbg, w.sigma, volt)
+    assert rel(sd, gold[p + "sens"]) < 1e-14
+    # update + cjggdc on the reference's sensdc (DC) and complex sens (FPI)
+    ref = np.load(os.path.join(GOLD, "ref_small_quad.npz" if ltri == 0 else "ref_small_tri.npz"))
+    if ltri == 0:
+        smatm, nachbar = O.bsmatmreg(g, g.nx, g.nz, 1.0, 1.0), None
+    else:
+        nachbar = O.bnachbar(g)
+        smatm = O.bsmatmtri(g, nachbar, 1.0, 1.0)
+    for kind in (1, 2):
+        q = f"{p}k{kind}_"
+        sa_k = gold[p + "sigmaa"] if kind == 1 else ref["sigmaa"]
+        sens = np.zeros(ref["sens"].shape, np.complex128) if kind == 1 else ref["sens"]
+        cp = O.CGProblem(sens, gold[p + "wmatd"], np.ones(sa_k.size, np.int32), gold[q + "dat"], sa_k,
+                         gold[p + "par"], smatm, ltri, g.nx, 3.0, 1e-12, 400, nachbar=nachbar, maxn=g.maxn)
+        sdk = gold[p + "sens"] if kind == 1 else None
+        cgfac, bvec, dpar, cgres, ncg = O.update_dc(cp, kind, sdk)
+        assert ncg == int(gold[q + "ncg"])
+        assert np.array_equal(cgfac, gold[q + "cgfac"]) and np.array_equal(bvec, gold[q + "bvec"])
+        assert np.array_equal(dpar, gold[q + "dpar"])
+        assert np.array_equal(cgres[: ncg + 1], gold[q + "cgres"][: ncg + 1])
+        if kind == 1:
+            cpd = O.CGProblem(gold[p + "sens"].astype(np.complex128), gold[p + "wmatd"], np.ones(sa_k.size, np.int32),
+                              gold[q + "dat"], sa_k, gold[p + "par"], smatm, ltri, g.nx, 3.0, 1e-12, 400,
+                              nachbar=nachbar, maxn=g.maxn)
+            assert abs(O.blam0(cpd, True, 1.0, 1.0) - float(gold[q + "lam0"])) <= 1e-13 * float(gold[q + "lam0"])
+
+
+def test_oracle_dc_against_mathematics(oracle_lib):
+    """What the DC twins must be, independent of the reference build: the DC forward
+    equals the complex forward of Re(sigma) (kompadc assembles DBLE(sigma)); bsendc is
+    Re(bsensi) for a real model; the converged cjggdc solves the real scaled normal
+    equations C (J^T W J + lam R) C y = Re(bvec) (a transposed J or a dropped weight
+    fails), and FPI does the same on DBLE(J) from Im(bvec)."""
+    O = oracle_lib
+    w = configs.small_quad(cutmck=O.cutmck)
+    g, s = w.grid, w.survey
+    kwn, kwi, _, _ = O.rwaven(g, s)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    sr = w.sigma.real.astype(np.complex128)
+    kp = O.forward_dc(g, w.sigma, elbg, relbg, kg, lb)
+    kc = O.forward(g, sr, elbg, relbg, kg, lb)
+    assert np.all(kc.imag == 0.0) and rel(kp, kc.real) < 1e-12
+    hp = O.rtrafo_dc(kp, kwi)
+    volt, sa = O.bvolti_dc(g, s, hp)
+    vc, _ = O.bvolti(g, s, O.rtrafo(kc, kwi))
+    assert float(np.max(np.abs(volt - vc) / np.abs(vc))) < 1e-12
+    sd = O.bsendc(g, s, kwi, kp, elbg, sr, volt)
+    sc = O.bsensi(g, s, kwi, kc, elbg, sr, vc)
+    assert rel(sd, sc.real) < 1e-12
+    rng = np.random.default_rng(4)
+    nanz, manz = s.nanz, g.elanz
+    wm = rng.random(nanz) + 0.5
+    wd = rng.integers(1, 3, nanz).astype(np.int32)
+    dat = sa + 0.05 * (rng.standard_normal(nanz) + 1j * rng.standard_normal(nanz))
+    par = np.log(w.sigma) + 0.1 * (rng.standard_normal(manz) + 1j * rng.standard_normal(manz))
+    smatm = O.bsmatmreg(g, g.nx, g.nz, 1.0, 1.0)
+    Jc = sc.T + 1j * 0.3 * rng.standard_normal(sc.T.shape) * np.abs(sc.T).max()
+    for kind, J in ((1, sd.T), (2, Jc.real)):
+        cp = O.CGProblem(np.zeros((manz, nanz), np.complex128) if kind == 1 else Jc.T, wm, wd, dat,
+                         sa, par, smatm, 0, g.nx, 2.0, 1e-30, 5000)
+        cgfac, bvec, dpar, _, ncg = O.update_dc(cp, kind, sd if kind == 1 else None)
+        W = wm * wd
+        R = np.zeros((manz, manz))
+        for j in range(manz):
+            e = np.zeros(manz, np.complex128); e[j] = 1.0
+            R[:, j] = O.cg_matvec(O.CGProblem(np.zeros((manz, nanz), np.complex128), wm, wd, dat, sa, par, smatm,
+                                              0, g.nx, 1.0, 1e-30, 1), np.ones(manz), e)[1].real
+        assert np.allclose(cgfac, 1.0 / np.sqrt((J ** 2 * W[:, None]).sum(0) + 2.0 * np.diag(R)), rtol=1e-14)
+        rhs = J.T @ (W * (dat - sa)) - 2.0 * (R @ par)
+        assert rel(bvec, cgfac * rhs) < 1e-12
+        Cm = np.diag(cgfac)
+        B = Cm @ (J.T @ (W[:, None] * J) + 2.0 * R) @ Cm
+        y = np.linalg.solve(B, (bvec.real if kind == 1 else bvec.imag))
+        expect = cgfac * y * (1.0 if kind == 1 else 1j)
+        assert ncg < 5000 and rel(dpar, expect) < 1e-9
