@@ -254,6 +254,53 @@ def gn_step(ctx, w, ltri: int):
                  "sens_row_rel_err_vs_fp64": float(srow.max()),
                  "dpar_rel_diff_vs_fp64": float(np.max(np.abs(r32["dpar"] - d64)) / np.max(np.abs(d64))),
                  "step_ms": round(1e3 * (tf + tsn + t["bsmatm_blam0"] + tu), 3)}
+    # the DC inversion's step (ldc: kompadc..vredc forward on DBLE(sigma), bvolti's DC
+    # branch, bsendc, update + cjggdc on the real Jacobian) and the final phase
+    # improvement's update (lfpi: cjggdc on DBLE(sens) from DIMAG(bvec)), same data
+    dc = {}
+    try:
+        ctx.set_inversion_kind(1)
+        td = {}
+        device_sync()
+        t0 = time.perf_counter()
+        ctx.set_sigma(sig0)
+        ctx.forward()
+        ctx.voltages(1)
+        device_sync()
+        td["forward"] = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        ctx.sens(1)
+        device_sync()
+        td["sens"] = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        ctx.bsmatm(ltri, 1.0, 1.0, nx=g.nx, nz=getattr(g, "nz", 0))
+        lam_dc = ctx.blam0(wmatd, wdfak, 1.0, 1.0)
+        device_sync()
+        td["bsmatm_blam0"] = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        rd = ctx.update(lam_dc, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+        device_sync()
+        td["update"] = time.perf_counter() - t1
+        td["step"] = time.perf_counter() - t0
+        itd = ctx.times()["cg_matvec"]
+        dc["dc_step"] = {"phase_ms": {k: round(1e3 * v, 3) for k, v in td.items()}, "lambda0": lam_dc,
+                         "ncg": rd["ncg"], "cg_ms_per_iteration": round(itd, 4),
+                         "cg_matvec_GBps": round(2.0 * 8.0 * v_true.size * manz / (itd * 1e-3) / 1e9, 1)
+                         if itd > 0 else None}
+        ctx.set_inversion_kind(2)
+        ctx.set_sigma(sig0)
+        ctx.forward()
+        ctx.voltages(1)
+        ctx.sens(1)
+        device_sync()
+        t1 = time.perf_counter()
+        rf = ctx.update(lam, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+        device_sync()
+        itf = ctx.times()["cg_matvec"]
+        dc["fpi_update"] = {"update_ms": round(1e3 * (time.perf_counter() - t1), 3), "ncg": rf["ncg"],
+                            "cg_ms_per_iteration": round(itf, 4)}
+    finally:
+        ctx.set_inversion_kind(0)
     nanz = v_true.size
     j_bytes = 16.0 * nanz * manz
     it_ms = tm["cg_matvec"]
@@ -263,7 +310,7 @@ def gn_step(ctx, w, ltri: int):
             "cg_matvec_GBps": round(2.0 * j_bytes / (it_ms * 1e-3) / 1e9, 1) if it_ms > 0 else None,
             "J_bytes": j_bytes, "cg_forms": forms, "forward_device_ms": {k: round(tmf[k], 3) for k in
                                                                        ("diag", "factor", "solve", "volt")},
-            "mixed_precision_forward": mixed}
+            "mixed_precision_forward": mixed, "dc_and_fpi": dc}
 
 
 def buncert_step(ctx, w, ltri: int):

@@ -68,7 +68,7 @@ int crt_set_sigma(crt_ctx *ctx, const double *sigma);
  * on the real Jacobian from DBLE(bvec) (DC) or DIMAG(bvec) (FPI), and crt_update stores
  * dpar = i*dpar for FPI (src/update.f90:302-304); crt_blam0 takes the sensdc branch for 1.
  * Not for 1/2: crt_buncert, bvolt/bsens (mode 0, DC), the FP32 and FSAI/learned CG
- * forms (CRT_ERR_UNSUPPORTED).  A change of kind discards sigma and its results:
+ * forms (903).  A change of kind discards sigma and its results:
  * call crt_set_sigma after it. */
 int crt_set_inversion_kind(crt_ctx *ctx, int kind);
 
