@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_apred_kernel(CGArgs A) {
     A.ap[i] = zd{s.re * wi, s.im * wi};
 }
 
-// K_f: Bp_j = C_j (J^H wap + lam R q)_j, one wave per column, dr1 partials
+// K_f: Bp_j = C_j (J^H wap + lam R q)_j, CG_JHW_COLS columns per wave, dr1 partials
 template <class JT>
 __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A, const JT *J) {
     __shared__ double red[8];
@@ -244,42 +244,60 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A, const JT *
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jbase = (blockIdx.x * (blockDim.x >> 6) + wv) * CG_JHW_COLS;
     double dpart = 0.0;
-    for (int jj = 0; jj < CG_JHW_COLS; ++jj) {
-        int j = jbase + jj;
-        if (j >= A.manz) break;
-        const JT *col = J + (size_t)j * A.nanz;
-        // four loads of the column in flight per lane; one accumulation chain in
-        // the order i = lane, lane+64, ... (the sum does not depend on the unroll)
-        zd s = zd{0.0, 0.0};
+    if (jbase < A.manz) {
+        const int nc = min(CG_JHW_COLS, A.manz - jbase);
+        // the wave's columns share every load of wap (columns past manz re-read the
+        // last one and are dropped); per column one accumulation chain in the order
+        // i = lane, lane+64, ... (the sums do not depend on the unroll)
+        const JT *col[CG_JHW_COLS];
+#pragma unroll
+        for (int c = 0; c < CG_JHW_COLS; ++c) col[c] = J + (size_t)(jbase + min(c, nc - 1)) * A.nanz;
+        zd s[CG_JHW_COLS];
+#pragma unroll
+        for (int c = 0; c < CG_JHW_COLS; ++c) s[c] = zd{0.0, 0.0};
+        constexpr int U = sizeof(JT) >= 16 ? 2 : 4;  // rows per lane in flight
         int i = lane;
-        for (; i + 192 < A.nanz; i += 256) {
-            const zd v0 = ldj(col + i), v1 = ldj(col + i + 64), v2 = ldj(col + i + 128), v3 = ldj(col + i + 192);
-            const zd a0 = A.ap[i], a1 = A.ap[i + 64], a2 = A.ap[i + 128], a3 = A.ap[i + 192];
-            s.re += v0.re * a0.re + v0.im * a0.im;
-            s.im += v0.re * a0.im - v0.im * a0.re;
-            s.re += v1.re * a1.re + v1.im * a1.im;
-            s.im += v1.re * a1.im - v1.im * a1.re;
-            s.re += v2.re * a2.re + v2.im * a2.im;
-            s.im += v2.re * a2.im - v2.im * a2.re;
-            s.re += v3.re * a3.re + v3.im * a3.im;
-            s.im += v3.re * a3.im - v3.im * a3.re;
+        for (; i + 64 * (U - 1) < A.nanz; i += 64 * U) {
+            zd a[U], v[U][CG_JHW_COLS];
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[u] = A.ap[i + 64 * u];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int c = 0; c < CG_JHW_COLS; ++c) v[u][c] = ldj(col[c] + i + 64 * u);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int c = 0; c < CG_JHW_COLS; ++c) {
+                    s[c].re += v[u][c].re * a[u].re + v[u][c].im * a[u].im;
+                    s[c].im += v[u][c].re * a[u].im - v[u][c].im * a[u].re;
+                }
         }
         for (; i < A.nanz; i += 64) {
-            const zd v = ldj(col + i), a = A.ap[i];
-            s.re += v.re * a.re + v.im * a.im;
-            s.im += v.re * a.im - v.im * a.re;
+            const zd a = A.ap[i];
+#pragma unroll
+            for (int c = 0; c < CG_JHW_COLS; ++c) {
+                const zd v = ldj(col[c] + i);
+                s[c].re += v.re * a.re + v.im * a.im;
+                s[c].im += v.re * a.im - v.im * a.re;
+            }
         }
-        for (int o = 32; o >= 1; o >>= 1) {
-            s.re += __shfl_xor(s.re, o, 64);
-            s.im += __shfl_xor(s.im, o, 64);
-        }
+#pragma unroll
+        for (int c = 0; c < CG_JHW_COLS; ++c)
+            for (int o = 32; o >= 1; o >>= 1) {
+                s[c].re += __shfl_xor(s[c].re, o, 64);
+                s[c].im += __shfl_xor(s[c].im, o, 64);
+            }
         if (lane == 0) {
-            zd t = rstencil(A, A.q, j);
-            double c = A.cgfac[j];
-            zd b = zd{(s.re + A.lam * t.re) * c, (s.im + A.lam * t.im) * c};
-            A.bv[j] = b;
-            zd p = A.p[j];
-            dpart += p.re * b.re + p.im * b.im;
+            for (int c = 0; c < nc; ++c) {
+                const int j = jbase + c;
+                zd t = rstencil(A, A.q, j);
+                double cf = A.cgfac[j];
+                zd b = zd{(s[c].re + A.lam * t.re) * cf, (s[c].im + A.lam * t.im) * cf};
+                A.bv[j] = b;
+                zd p = A.p[j];
+                dpart += p.re * b.re + p.im * b.im;
+            }
         }
     }
     double s = block_sum(dpart, red);
