@@ -215,6 +215,20 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A, const JT *J
     zd acc0 = zd{0.0, 0.0}, acc1 = zd{0.0, 0.0};
     const JT *col = J + (size_t)j0 * A.nanz + i;
     int j = j0;
+    // 16-byte entries: four column loads in flight; even columns into acc0, odd ones
+    // into acc1, in column order (the same sums as two at a time).  The 8-byte copies
+    // measured slower this way (C3: 0.229 -> 0.25 ms per pass) and keep two.
+    if constexpr (sizeof(JT) >= 16) {
+        for (; j + 3 < j1; j += 4) {
+            const size_t n = A.nanz;
+            zd v0 = ldj(col), v1 = ldj(col + n), v2 = ldj(col + 2 * n), v3 = ldj(col + 3 * n);
+            acc0 = zfma(acc0, v0, qs[j - j0]);
+            acc1 = zfma(acc1, v1, qs[j + 1 - j0]);
+            acc0 = zfma(acc0, v2, qs[j + 2 - j0]);
+            acc1 = zfma(acc1, v3, qs[j + 3 - j0]);
+            col += 4 * n;
+        }
+    }
     for (; j + 1 < j1; j += 2) {
         zd v0 = ldj(col), v1 = ldj(col + A.nanz);
         acc0 = zfma(acc0, v0, qs[j - j0]);
