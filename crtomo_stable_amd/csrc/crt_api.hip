@@ -15,6 +15,8 @@ int crt_launch_precal(hipStream_t, int, int, int, int, int, int, const int *, co
                       const double *, const double *, const int *, const double *, double, double *, double *,
                       double *, int *);
 int crt_launch_diag(hipStream_t, const DevGrid &, int, const int2 *, double *, double *, int *);
+int crt_launch_dc_pivot(hipStream_t, int, int, int, const zd *, int *);
+int crt_launch_dc_pivot32(hipStream_t, int, int, int, const zf *, int *);
 int crt_launch_assemble(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *);
 int crt_launch_area(hipStream_t, const DevGrid &, int, zd *);
 int crt_launch_factor(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, const zd *,
@@ -166,6 +168,7 @@ struct crt_ctx {
     DBuf<zd> d_sigma, d_kpotN, d_panels, d_yf, d_dinv, d_yscr, d_hpotE, d_volt, d_sigmaa, d_sens, d_tmp, d_vals;
     // normal-equation state
     int ltri = -1, rnx = 0, rncol = 0, sens_nanz = 0, sens_manz = 0;
+    int reg_manz = 0;  // cells of the installed regularisation (smatm, rdiag, nachbar rows)
     DBuf<double> d_smatm, d_rdiag, d_w, d_cgfac, d_part_dr1, d_part_rr, d_cgres;
     DBuf<int> d_nachbar;
     DBuf<double> d_jtj;
@@ -211,6 +214,7 @@ struct crt_ctx {
         g.tmapD = tmapD;
         g.csplit = asplit ? d_csplit.p : nullptr;
         g.area = nullptr;  // set by the callers that computed the area sums
+        g.dc = inv_kind == 1 ? 1 : 0;
         return g;
     }
     void tic(int i) { (void)hipEventRecord(ev0[i], stream); }
@@ -473,9 +477,12 @@ int batch_assemble(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch
 int batch_factor(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch, int variant) {
     const DevGrid g = batch_grid(M);
     const crt_batch::Slot &w = b->slots[M.slot];
-    if (M.use16)
-        return crt_launch_factor16(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.vals, w.panels, w.yf, w.dinv,
-                                   M.status.p + 1 + ch.s0, variant);
+    if (M.use16) {
+        int e = crt_launch_factor16(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.vals, w.panels, w.yf, w.dinv,
+                                    M.status.p + 1 + ch.s0, variant);
+        if (!e && g.dc) e = crt_launch_dc_pivot(w.st, ch.nb, M.c->sanz, g.fstride, w.dinv, M.status.p + 1 + ch.s0);
+        return e;
+    }
     return crt_launch_factor(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.dga, w.vals, w.panels, w.yf,
                              M.status.p + 1 + ch.s0);
 }
@@ -916,6 +923,8 @@ static int forward_batch_f32(crt_ctx *c, const DevGrid &g, int nb, size_t s0, in
     int e = crt_launch_factor32(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, panels, y32, dinv32,
                                 c->d_status.p + 1 + s0);
     if (e) return e;
+    // DC: a non-positive pivot sends the batch to the FP64 kernels, which report it
+    if (g.dc && (e = crt_launch_dc_pivot32(c->stream, nb, c->sanz, fstride, dinv32, c->d_status.p + 1 + s0))) return e;
     c->toc(T_FACTOR);
     c->tic(T_SOLVE);
     e = crt_launch_solve16_f32(c->stream, c->sanz, nblk, D, K, E, nrhs, c->d_rhs.p, c->d_enr.p, c->d_fak.p, panels,
@@ -1069,9 +1078,12 @@ int crt_forward(crt_ctx *c) {
         }
         if (!done) {
             c->tic(T_FACTOR);
-            if (use16)
+            if (use16) {
                 e = crt_launch_factor16(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_vals.p, c->d_panels.p,
                                         c->d_yf.p, c->d_dinv.p, c->d_status.p + 1 + s0, 0);
+                if (!e && g.dc)
+                    e = crt_launch_dc_pivot(c->stream, nb, c->sanz, (int)fstride, c->d_dinv.p, c->d_status.p + 1 + s0);
+            }
             else
                 e = crt_launch_factor(c->stream, g, nb, c->d_sys.p + s0, c->d_fak.p, c->d_dga.p, c->d_vals.p,
                                       c->d_panels.p, c->d_yf.p, c->d_status.p + 1 + s0);
@@ -1099,8 +1111,10 @@ int crt_forward(crt_ctx *c) {
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol;
     std::vector<int> st(sys.size() + 1);
     CRT_HIP(c->copy_out(st.data(), c->d_status.p, st.size() * sizeof(int)));
-    int worst = 0;
-    for (int v : st) worst = std::max(worst, v);
+    // the first failing system in the reference's loop order (wavenumber, then
+    // electrode: src/fem.f90:328-333) names the error
+    int worst = st[0];
+    for (size_t q = 1; q < st.size() && !worst; ++q) worst = st[q];
     if (worst && getenv("CRT_DEBUG")) {
         for (size_t q = 1; q < st.size(); ++q)
             if (st[q]) {
@@ -1260,7 +1274,7 @@ int crt_set_regularization(crt_ctx *c, int ltri, int nx, int manz, int maxn, con
     c->ltri = ltri;
     c->rnx = nx;
     c->rncol = ncol;
-    c->sens_manz = manz;
+    c->reg_manz = manz;
     return 0;
 }
 
@@ -1305,7 +1319,24 @@ int crt_bsmatm(crt_ctx *c, int ltri, int nx, int nz, double alfx, double alfz, d
     c->ltri = ltri;
     c->rnx = nx;
     c->rncol = ncol;
-    c->sens_manz = manz;
+    c->reg_manz = manz;
+    return 0;
+}
+
+// Sizes of the resident Jacobian, and the check that every buffer the normal-equation
+// kernels index by (measurement, cell) covers them: J holds nanz*manz entries and, when
+// the regularisation is used, smatm / rdiag / nachbar were installed for the same manz
+// cells.  A mismatch (J and R set for different grids) is refused before any launch.
+static int jsizes(const crt_ctx *c, bool need_reg, int *nanz, int *manz) {
+    const int n = c->sens_nanz ? c->sens_nanz : c->nanz, m = c->sens_manz ? c->sens_manz : c->elanz;
+    if (n <= 0 || m <= 0 || (size_t)n * m > c->d_sens.n) return CRT_ERR_STATE;
+    if (need_reg) {
+        if (c->ltri < 0 || c->reg_manz != m || (size_t)m * c->rncol > c->d_smatm.n || (size_t)m > c->d_rdiag.n)
+            return CRT_ERR_STATE;
+        if (c->ltri != 0 && (size_t)m * c->rncol > c->d_nachbar.n) return CRT_ERR_STATE;
+    }
+    *nanz = n;
+    *manz = m;
     return 0;
 }
 
@@ -1567,7 +1598,8 @@ int crt_pc_train(crt_ctx *c, int steps, double lr, double *loss_first, double *l
     if (!c || steps < 1 || !(lr > 0.0)) return CRT_ERR_ARG;
     if (!c->have_cgsys || !c->have_sens || c->ltri != 0) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
     const int np = crt_pc_nparams();
     int e = 0;
     if (!c->pc_ready) {
@@ -1740,7 +1772,8 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if (!c || !wmatd || !wdfak || !cgfac || !bvec || ncgmax < 0) return CRT_ERR_ARG;
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
     int e = cg_weights(c, nanz, wmatd, wdfak);
     e = e ? e : c->d_cgfac.upload(cgfac, manz, c->stream);
     e = e ? e : c->d_bvec.upload(reinterpret_cast<const zd *>(bvec), manz, c->stream);
@@ -1763,11 +1796,8 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
     if (c->inv_kind != 0) return CRT_ERR_UNSUPPORTED;  // bata_dc / the FPI branch of bmcm_mod
     CRT_HIP(hipSetDevice(c->device));
-    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
-    // J and the regularisation must both cover manz cells
-    if ((size_t)nanz * manz > c->d_sens.n || (size_t)manz * c->rncol > c->d_smatm.n ||
-        (c->ltri != 0 && (size_t)manz * c->rncol > c->d_nachbar.n))
-        return CRT_ERR_STATE;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
     int e = cg_weights(c, nanz, wmatd, wdfak);
     if (e) return e;
     const size_t n2 = (size_t)manz * manz;
@@ -1809,7 +1839,8 @@ int crt_blam0(crt_ctx *c, int fpi, const double *wmatd, const int *wdfak, double
     if (!c || !wmatd || !wdfak || !lammax || !(alfx + alfz != 0.0)) return CRT_ERR_ARG;
     if (!c->have_sens) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, false, &nanz, &manz)) return e0;
     int ncb, cb;
     const size_t np = crt_blam0_scratch(nanz, manz, &ncb, &cb);
     int e = cg_weights(c, nanz, wmatd, wdfak);
@@ -1834,7 +1865,8 @@ int crt_update(crt_ctx *c, double lam, double eps, int ncgmax, double step, cons
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
     if (!sigmaa && !c->have_volt) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
-    const int nanz = c->sens_nanz ? c->sens_nanz : c->nanz, manz = c->sens_manz ? c->sens_manz : c->elanz;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
     int e = cg_weights(c, nanz, wmatd, wdfak);
     // residual d - f(m)
     std::vector<zd> res(nanz), sa(nanz);
@@ -2229,9 +2261,11 @@ int crt_batch_forward(crt_batch *b) {
         std::vector<int> st(M->nsys + 1);
         CRT_HIP(hipMemcpyAsync(st.data(), M->status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
         CRT_HIP(hipStreamSynchronize(b->stream));
-        int w = 0;
-        for (int v : st) w = std::max(w, v);
-        worst = std::max(worst, w);
+        // as crt_forward: the first failing system names the model's error, the first
+        // failing model the batch's
+        int w = st[0];
+        for (size_t q = 1; q < st.size() && !w; ++q) w = st[q];
+        if (!worst) worst = w;
         M->c->have_kpot = w == 0;
         M->c->have_volt = M->c->have_sens = false;
     }

@@ -107,12 +107,28 @@ __global__ void __launch_bounds__(256) crt_diag_kernel(DevGrid g, int nsys, cons
     zd a = assemble_q(g, q, lk.x, lk.y);
     double dum = zabs(a);
     size_t o = (size_t)s * g.fstride + i;
-    if (!(dum > 0.0)) {
+    if (!(dum > 0.0) || (g.dc && !(a.re > 0.0))) {
         status[s] = CRT_ERR_ZERODIAG;  // plain store: any non-zero code fails the call
         dum = 1.0;
     }
     fak[o] = 1.0 / sqrt(dum);
     dga[o] = dum;
+}
+
+// choldc stops on a pivot s <= 0 (src/choldc.f90:66-70).  The DC forward runs the
+// complex factorisation on real matrices: while every s > 0 the pivots and their
+// reciprocals stay real and positive, and the first s < 0 makes its square root, and
+// every reciprocal pivot after it, non-real (s = 0 is caught by the factor kernels).
+// One thread per (system, row) over the reciprocal pivots dinv.
+template <class Z>
+__global__ void __launch_bounds__(256) crt_dc_pivot_kernel(int nsys, int sanz, int fstride, const Z *__restrict__ dinv,
+                                                           int *__restrict__ status) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nsys * sanz) return;
+    const int s = (int)(t / sanz), i = (int)(t - (long)s * sanz);
+    const Z r = dinv[(size_t)s * fstride + i];
+    // an earlier error of this system (scaldc's 27) stays: the routines report in order
+    if ((r.im != 0 || !(r.re > 0)) && status[s] == 0) status[s] = CRT_ERR_PIVOT;
 }
 
 // ----------------------------------------------------------------- assemble
@@ -274,7 +290,7 @@ __global__ void __launch_bounds__(256) crt_factor_kernel(FactorArgs A) {
             for (int cc = 0; cc < NB; ++cc) {
                 zd pv = shfl_z(v, cc * NB + cc);
                 zd piv = zsqrt(pv);
-                if (zabs(pv) <= 0.0) err = CRT_ERR_PIVOT;
+                if (zabs(pv) <= 0.0 || (A.g.dc && !(pv.re > 0.0 && pv.im == 0.0))) err = CRT_ERR_PIVOT;
                 zd pinv = zinv(piv);
                 if (r == cc && c == cc) v = piv;
                 if (r > cc && c == cc) v = zmul(v, pinv);
@@ -512,6 +528,20 @@ int crt_launch_diag(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys,
     hipLaunchKernelGGL(crt_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, sys,
                        fak, dga, status);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+template <class Z>
+static int launch_dc_pivot(hipStream_t st, int nsys, int sanz, int fstride, const Z *dinv, int *status) {
+    const long n = (long)nsys * sanz;
+    hipLaunchKernelGGL(crt_dc_pivot_kernel<Z>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nsys, sanz,
+                       fstride, dinv, status);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+int crt_launch_dc_pivot(hipStream_t st, int nsys, int sanz, int fstride, const zd *dinv, int *status) {
+    return launch_dc_pivot(st, nsys, sanz, fstride, dinv, status);
+}
+int crt_launch_dc_pivot32(hipStream_t st, int nsys, int sanz, int fstride, const zf *dinv, int *status) {
+    return launch_dc_pivot(st, nsys, sanz, fstride, dinv, status);
 }
 
 int crt_launch_assemble(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,

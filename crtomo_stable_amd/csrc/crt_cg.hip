@@ -169,6 +169,10 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_dir_kernel(CGArgs A) {
     if (k == 1) {
         dr0 = dr * A.eps;
         beta = 0.0;
+        // cjggdc tests the stop before its first step as well (src/cg_mod.f90:143-150):
+        // a zero right-hand side ends the DC / FPI solve with ncg = 0 and dpar = 0;
+        // cjggra tests from the second iteration on (src/cg_mod.f90:455-461)
+        stop = A.part != 0 && dr <= dr0;
     } else {
         dr0 = cur.dr0;
         stop = dr <= dr0;

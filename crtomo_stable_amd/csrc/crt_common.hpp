@@ -134,6 +134,10 @@ struct DevGrid {
     // contributions of entry q start.  area null: every entry is summed in full.
     const int *csplit;
     const zd *area;
+    // 1 for the DC inversion (kompadc / scaldc / choldc on DBLE(sigma)): a diagonal entry
+    // or pivot that is not real and positive is an error there (src/scaldc.f90:54-58,
+    // src/choldc.f90:66-70), where the complex routines only stop on zero
+    int dc;
 };
 
 // Gauss-Newton CG state shared by crt_cg.hip and crt_api.hip

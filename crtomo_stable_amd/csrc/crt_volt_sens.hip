@@ -445,6 +445,12 @@ int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int max
         const size_t budget = std::max(gbytes, (size_t)80 * 1024);
         int KC = K;
         while (KC > 1 && (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd) > budget) --KC;
+        // CRT_SENS_KC: a smaller chunk (more chunks per element) for tests of the
+        // cross-chunk accumulation; never more than fits
+        if (const char *ev = getenv("CRT_SENS_KC")) {
+            const int f = atoi(ev);
+            if (f >= 1 && f < KC) KC = f;
+        }
         const size_t lds = std::max(gbytes, (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd));
         if (lds <= 160 * 1024) {
             switch (nt) {

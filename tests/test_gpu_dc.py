@@ -150,3 +150,71 @@ def test_dc_fpi_update(ctx, oracle_lib, kind, ltri, ncgmax, eps):
                 ctx.buncert(lam, P["wmatd"], P["wdfak"])
     finally:
         ctx.set_inversion_kind(0)
+
+
+@pytest.mark.parametrize("ltri", [0, 1])
+def test_fpi_zero_rhs(ctx, oracle_lib, ltri):
+    """cjggdc tests its stop before the first step as well (src/cg_mod.f90:143-150):
+    a zero right-hand side (the FPI case with Im(bvec) = 0: data equal to the forward
+    values and a real model) ends the solve with ncg = 0 and dpar = 0, as the oracle."""
+    O = oracle_lib
+    w = _workload(ltri)
+    P = _cg_problem(O, w, ltri, 2)
+    dat = P["sa"].copy()
+    par = P["par"].real.astype(np.complex128)
+    lam, eps, ncgmax = 3.0, 1e-4, 200
+    cp = O.CGProblem(P["sens"], P["wmatd"], P["wdfak"], dat, P["sa"], par, P["smatm"], ltri, w.grid.nx,
+                     lam, eps, ncgmax, nachbar=P["nachbar"], maxn=w.grid.maxn)
+    cgfac_o, bvec_o, dpar_o, cgres_o, ncg_o = O.update_dc(cp, 2)
+    assert np.all(bvec_o.imag == 0.0)
+    assert ncg_o == 0 and np.all(dpar_o == 0.0)
+    try:
+        ctx.set_inversion_kind(2)
+        ctx.set_grid(w.grid)
+        ctx.set_sens(P["sens"])
+        ctx.set_regularization(ltri, P["smatm"], nx=w.grid.nx, nachbar=P["nachbar"])
+        r = ctx.update(lam, eps, ncgmax, P["wmatd"], P["wdfak"], dat, par, sigmaa=P["sa"])
+        print(f"[fpi zero rhs] ltri={ltri} ncg gpu {r['ncg']} oracle {ncg_o}", flush=True)
+        assert r["ncg"] == 0 and r["cgres"][0] == 0.0
+        assert np.all(r["dpar"] == 0.0)
+        assert np.array_equal(r["par"], par)
+    finally:
+        ctx.set_inversion_kind(0)
+
+
+@pytest.mark.parametrize("fac,code", [(-1.0, 28), (-4.0, 27)])
+def test_dc_error_codes(ctx, oracle_lib, fac, code):
+    """The DC twins stop where the complex routines do not: scaldc on a non-positive
+    diagonal entry (errnr 27, src/scaldc.f90:54-58) and choldc on a non-positive pivot
+    (errnr 28, src/choldc.f90:66-70).  One cell's conductivity is made negative: at
+    -1 times its value every diagonal entry stays positive but the matrix is
+    indefinite (28), at -4 times the diagonal of its corner nodes turns negative (27).
+    The oracle's DC forward returns the same codes in the same (wavenumber, electrode)
+    order; the complex forward of the same model does not stop."""
+    O = oracle_lib
+    w = configs.small_quad()
+    g = w.grid
+    cx, cy = g.centroids()
+    sig = w.sigma.real.astype(np.complex128)
+    e = int(np.argmin((cx - cx.mean()) ** 2 + (cy + 2.0) ** 2))
+    sig[e] *= fac
+    kwn, kwi, _, _ = O.rwaven(g, w.survey)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    with pytest.raises(RuntimeError, match=f"errnr {code}"):
+        O.forward_dc(g, sig, elbg, relbg, kg, lb)
+    try:
+        ctx.set_inversion_kind(1)
+        ctx.set_grid(g)
+        ctx.set_survey(w.survey)
+        ctx.rwaven()
+        ctx.precal()
+        ctx.set_sigma(sig)
+        with pytest.raises(CRTError) as ei:
+            ctx.forward()
+        print(f"[dc error] fac={fac} code {ei.value.code}", flush=True)
+        assert ei.value.code == code
+    finally:
+        ctx.set_inversion_kind(0)
+    # the complex inversion factorises the same matrices without a stop
+    ctx.set_sigma(sig)
+    ctx.forward()
