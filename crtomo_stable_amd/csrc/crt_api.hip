@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -88,6 +89,42 @@ int crt_launch_bsmatmtri(hipStream_t, int, int, const int *, const int *, const 
 size_t crt_blam0_scratch(int, int, int *, int *);
 int crt_launch_blam0(hipStream_t, int, int, int, const zd *, const double *, zd *, int, int, zd *, double *, double,
                      double *);
+
+// ------------------------------------------------------- static CG scratch
+// crt_cjg_noalloc keeps every buffer of its solve (weights, cgfac, right-hand side, the
+// CG vectors, partial sums, scalar states, residual history) in this array instead of
+// context buffers, so that the call allocates nothing.  Each GPU holds its own copy,
+// from module load until the process ends; calls on one GPU share it.
+constexpr size_t CRT_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int CRT_SCRATCH_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[CRT_SCRATCH_BYTES];
+static std::mutex scratch_mu;
+static unsigned char *scratch_dev[CRT_SCRATCH_DEVICES];  // device address of `scratch` per GPU
+
+// Called with `device` current (crt_create): where that GPU holds `scratch`.  The
+// array's name is a host-side stand-in, never a device address: kernels and copies
+// take the pointer looked up here.
+static int scratch_lookup(int device) {
+    if (device < 0 || device >= CRT_SCRATCH_DEVICES) return CRT_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_dev[device]) return 0;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CRT_ERR_HIP;
+    scratch_dev[device] = static_cast<unsigned char *>(p);
+    return 0;
+}
+
+static unsigned char *scratch_base(int device) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    return device >= 0 && device < CRT_SCRATCH_DEVICES ? scratch_dev[device] : nullptr;
+}
+
+// w = wmatd * wdfak on the device (the weights of the normal equations)
+__global__ void __launch_bounds__(256) crt_noalloc_weights_kernel(int n, const int *__restrict__ wdfak,
+                                                                  double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = w[i] * (double)wdfak[i];
+}
 
 namespace {
 
@@ -560,6 +597,7 @@ int crt_create(crt_ctx **out, int device) {
     }
     if (const char *b = getenv("CRT_PANEL_BUDGET_GB")) c->panel_budget = (size_t)(atof(b) * (double)(1ull << 30));
     if (c->d_status.ensure(1)) { crt_destroy(c); return CRT_ERR_ALLOC; }
+    if (int e = scratch_lookup(device)) { crt_destroy(c); return e; }
     *out = c;
     return 0;
 }
@@ -1510,13 +1548,16 @@ static void pc_init_params(std::vector<float> &par) {
     }
 }
 
+// The two scalar states alternate between a.scn[0] and a.scn[1]; the residual history
+// goes to a.cgres (both set by the caller: context buffers or the static scratch).
 static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
     int e;
+    CGScalars *S = a.scn;
     CGArgs a0 = a, a1 = a;
-    a0.sc = c->d_sc.p; a0.scn = c->d_sc.p + 1;
-    a1.sc = c->d_sc.p + 1; a1.scn = c->d_sc.p;
+    a0.sc = S; a0.scn = S + 1;
+    a1.sc = S + 1; a1.scn = S;
     CGArgs ai = a;
-    ai.scn = c->d_sc.p;  // init writes state 0
+    ai.scn = S;  // init writes state 0
     c->tic(T_CG);
     if ((e = crt_launch_cg_init(c->stream, ai, bvec))) return e;
     CGScalars sc{};
@@ -1528,7 +1569,7 @@ static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
         for (int q = 0; q < n; ++q, ++it)
             if ((e = crt_launch_cg_iter(c->stream, (it & 1) ? a1 : a0))) return e;
         c->toc(T_CGMV);
-        CRT_HIP(hipMemcpyAsync(&sc, c->d_sc.p + (it & 1), sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
+        CRT_HIP(hipMemcpyAsync(&sc, S + (it & 1), sizeof(CGScalars), hipMemcpyDeviceToHost, c->stream));
         CRT_HIP(hipStreamSynchronize(c->stream));
         c->collect(T_CGMV);
         tmv += c->times[T_CGMV];
@@ -1542,7 +1583,7 @@ static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
     // per-iteration device time of the batches (the last batch may include no-op launches)
     c->times[T_CGMV] = it > 0 ? tmv / it : 0.0;
     double ncgd = (double)(float)(*ncg);
-    CRT_HIP(hipMemcpyAsync(c->d_cgres.p, &ncgd, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipMemcpyAsync(a.cgres, &ncgd, sizeof(double), hipMemcpyHostToDevice, c->stream));
     CRT_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1786,6 +1827,71 @@ int crt_cjg(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd,
     if ((e = cg_fallback(c, a, ncgmax, &n))) return e;
     if (dpar) CRT_HIP(c->copy_out(dpar, c->d_dpar.p, manz * sizeof(zd)));
     if (cgres) CRT_HIP(c->copy_out(cgres, c->d_cgres.p, ((size_t)ncgmax + 1) * sizeof(double)));
+    if (ncg) *ncg = n;
+    return 0;
+}
+
+// crt_cjg without an allocation: the same kernels and order as crt_cjg (so the same
+// results), every buffer of the solve taken from the device's static scratch array.
+// Complex inversion, FP64 J, Jacobi scaling only; a solve whose buffers exceed the
+// array returns CRT_ERR_SCRATCH.  Calls that overlap on one GPU race on the array.
+int crt_cjg_noalloc(crt_ctx *c, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,
+                    const double *cgfac, const double *bvec, double *dpar, double *cgres, int *ncg) {
+    if (!c || !wmatd || !wdfak || !cgfac || !bvec || ncgmax < 0) return CRT_ERR_ARG;
+    if (!c->have_sens) return CRT_ERR_STATE;
+    if (c->inv_kind != 0 || c->cg_jbits != 64 || c->cg_prec != 0) return CRT_ERR_UNSUPPORTED;
+    int nanz = 0, manz = 0;
+    if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
+    CRT_HIP(hipSetDevice(c->device));
+    unsigned char *base = scratch_base(c->device);
+    if (!base) return CRT_ERR_STATE;
+    int ncb, cb, n_dr1, n_rr;
+    crt_cg_geometry(nanz, manz, &ncb, &cb, &n_dr1, &n_rr);
+    // layout: 256-byte aligned pieces, in this order
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return base + at;
+    };
+    double *w = reinterpret_cast<double *>(take(nanz * sizeof(double)));
+    int *wd = reinterpret_cast<int *>(take(nanz * sizeof(int)));
+    double *cf = reinterpret_cast<double *>(take(manz * sizeof(double)));
+    zd *bv0 = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *r = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *p = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *q = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *bv = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *dp = reinterpret_cast<zd *>(take(manz * sizeof(zd)));
+    zd *ap = reinterpret_cast<zd *>(take(nanz * sizeof(zd)));
+    zd *part_ap = reinterpret_cast<zd *>(take((size_t)ncb * nanz * sizeof(zd)));
+    double *part_dr1 = reinterpret_cast<double *>(take(n_dr1 * sizeof(double)));
+    double *part_rr = reinterpret_cast<double *>(take(n_rr * sizeof(double)));
+    double *res = reinterpret_cast<double *>(take(((size_t)ncgmax + 1) * sizeof(double)));
+    CGScalars *sc = reinterpret_cast<CGScalars *>(take(2 * sizeof(CGScalars)));
+    if (off > CRT_SCRATCH_BYTES) return CRT_ERR_SCRATCH;
+    CRT_HIP(hipMemcpyAsync(w, wmatd, nanz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipMemcpyAsync(wd, wdfak, nanz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(crt_noalloc_weights_kernel, dim3((nanz + 255) / 256), dim3(256), 0, c->stream, nanz, wd, w);
+    CRT_HIP(hipGetLastError());
+    CRT_HIP(hipMemcpyAsync(cf, cgfac, manz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CRT_HIP(hipMemcpyAsync(bv0, bvec, manz * sizeof(zd), hipMemcpyHostToDevice, c->stream));
+    // the residual history starts at zero (the array keeps the previous call's values)
+    CRT_HIP(hipMemsetAsync(res, 0, ((size_t)ncgmax + 1) * sizeof(double), c->stream));
+    CGArgs a{};
+    a.nanz = nanz; a.manz = manz; a.ltri = c->ltri; a.nx = c->rnx; a.maxn = c->rncol - 1; a.ncgmax = ncgmax;
+    a.lam = lam; a.eps = eps;
+    a.J = c->d_sens.p; a.J32 = nullptr; a.Jr = nullptr; a.part = 0;
+    a.w = w; a.smatm = c->d_smatm.p; a.nachbar = c->d_nachbar.p; a.cgfac = cf;
+    a.r = r; a.p = p; a.q = q; a.bv = bv; a.dpar = dp; a.ap = ap;
+    a.part_ap = part_ap; a.part_dr1 = part_dr1; a.part_rr = part_rr;
+    a.n_dr1 = n_dr1; a.n_rr = n_rr; a.ncb = ncb; a.cb = cb;
+    a.fg = nullptr; a.fpat = nullptr; a.ftp = nullptr; a.t = nullptr; a.z = nullptr; a.part_rz = nullptr;
+    a.sc = sc; a.scn = sc; a.cgres = res;
+    int n = 0;
+    if (int e = cg_iterate(c, a, bv0, &n)) return e;
+    if (dpar) CRT_HIP(c->copy_out(dpar, dp, manz * sizeof(zd)));
+    if (cgres) CRT_HIP(c->copy_out(cgres, res, ((size_t)ncgmax + 1) * sizeof(double)));
     if (ncg) *ncg = n;
     return 0;
 }

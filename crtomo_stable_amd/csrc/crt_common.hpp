@@ -23,6 +23,7 @@ enum {
     CRT_ERR_STATE = 901,
     CRT_ERR_ARG = 902,
     CRT_ERR_UNSUPPORTED = 903,
+    CRT_ERR_SCRATCH = 904,
 };
 
 #define CRT_HIP(call)                                                                      \

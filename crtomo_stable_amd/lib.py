@@ -331,13 +331,17 @@ class Context:
         """64: FP64 J in the CG iterations (default); 32: complex FP32 copy of J."""
         _chk(self.lib.crt_set_cg_precision(self._h, int(jbits)), "crt_set_cg_precision")
 
-    def cjg(self, lam, eps, ncgmax, wmatd, wdfak, cgfac, bvec):
+    def cjg(self, lam, eps, ncgmax, wmatd, wdfak, cgfac, bvec, noalloc=False):
+        """cjg (src/cg_mod.f90:84) on the resident J and regularisation: (raw dpar,
+        cgres, ncg).  ``noalloc`` calls crt_cjg_noalloc, which keeps its buffers in the
+        library's static device scratch instead of allocating."""
         manz = np.asarray(cgfac).size
         w = np.ascontiguousarray(wmatd, np.float64); wd = np.ascontiguousarray(wdfak, np.int32)
         cf = np.ascontiguousarray(cgfac, np.float64); bv = np.ascontiguousarray(bvec, np.complex128)
         dpar = np.zeros(manz, np.complex128); cgres = np.zeros(ncgmax + 1); ncg = C.c_int()
-        _chk(self.lib.crt_cjg(self._h, C.c_double(lam), C.c_double(eps), int(ncgmax), _ptr(w), _ptr(wd), _ptr(cf),
-                              _ptr(bv), _ptr(dpar), _ptr(cgres), C.byref(ncg)), "crt_cjg")
+        fn = self.lib.crt_cjg_noalloc if noalloc else self.lib.crt_cjg
+        _chk(fn(self._h, C.c_double(lam), C.c_double(eps), int(ncgmax), _ptr(w), _ptr(wd), _ptr(cf), _ptr(bv),
+                _ptr(dpar), _ptr(cgres), C.byref(ncg)), "crt_cjg_noalloc" if noalloc else "crt_cjg")
         return dpar, cgres, ncg.value
 
     def update(self, lam, eps, ncgmax, wmatd, wdfak, dat, par, sigmaa=None, step=1.0):

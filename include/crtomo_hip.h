@@ -17,7 +17,8 @@
  *   19 band width exceeded, 26 element geometry, 27 zero diagonal,
  *   28 non-positive pivot, 35 boundary geometry, 73 duplicate electrodes,
  *   82 zero voltage, 97 allocation; 900+ are library codes (900 HIP error,
- *   901 call order, 902 argument, 903 unsupported configuration).
+ *   901 call order, 902 argument, 903 unsupported configuration, 904 a solve larger
+ *   than the static scratch of crt_cjg_noalloc).
  * A context is bound to one device and one HIP stream; calls on one context
  * must not overlap.
  */
@@ -103,6 +104,20 @@ int crt_set_regularization(crt_ctx *ctx, int ltri, int nx, int manz, int maxn, c
  * history cgres(ncgmax+1) (cgres(1) = ncg) and ncg. */
 int crt_cjg(crt_ctx *ctx, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,
             const double *cgfac, const double *bvec, double *dpar, double *cgres, int *ncg);
+
+/* crt_cjg without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as crt_cjg;
+ * every buffer of the solve (weights, cgfac, right-hand side, CG vectors, partial
+ * sums, scalar states, residual history) lives in one static 64 MiB device array per
+ * GPU, held from library load until the process ends (about 12 MB for 40000 cells,
+ * 4500 data and ncgmax 4000).  It needs the Jacobian and regularisation already on
+ * the context (crt_sens / crt_set_sens, crt_set_regularization / crt_bsmatm), the
+ * complex inversion, FP64 J and the Jacobi scaling (903 otherwise); a solve whose
+ * buffers exceed the array returns 904.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (from any contexts or threads) race. */
+int crt_cjg_noalloc(crt_ctx *ctx, double lam, double eps, int ncgmax, const double *wmatd, const int *wdfak,
+                    const double *cgfac, const double *bvec, double *dpar, double *cgres, int *ncg);
 
 /* update (src/update.f90:27, complex case, no prior / reference model): RHS and
  * cgfac from J on the device, cjg, dpar <- cgfac*dpar, par += step*dpar.
