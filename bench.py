@@ -28,18 +28,35 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 HBM_PEAK_GBS = 8000.0
 
 
+KERNEL_SOURCES = {  # the source file each benched kernel is compiled from
+    "crt_factor16la_kernel": "crt_band16.hip", "crt_factor16st_kernel": "crt_band16.hip",
+    "crt_factor16_kernel": "crt_band16.hip", "crt_solve16_kernel": "crt_band16.hip",
+    "crt_factor32st_kernel": "crt_band32.hip", "crt_factor_kernel": "crt_band.hip",
+    "crt_sens_gm_kernel": "crt_volt_sens.hip", "crt_cg_jq_kernel": "crt_cg.hip", "crt_cg_jhw_kernel": "crt_cg.hip",
+}
+
+
+def source_hash(kernel: str) -> str:
+    """First 12 hex digits of the SHA-1 of the source file of ``kernel``."""
+    import hashlib
+    name = KERNEL_SOURCES[kernel.split("<")[0]]
+    with open(os.path.join(ROOT, "crtomo_stable_amd", "csrc", name), "rb") as f:
+        return hashlib.sha1(f.read()).hexdigest()[:12]
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (profiles/<workload>_pmc_traffic.md, a line
-    `traffic <kernel> <bytes>`), or None when no such measurement is committed."""
+    `traffic <kernel> <bytes> <source sha1[:12]>`).  None when no measurement of this
+    kernel is committed, or when its source changed since it was measured."""
     import re
     path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(r"^traffic\s+(\S+)\s+(\d+)\s*$", line.strip())
+        m = re.match(r"^traffic\s+(\S+)\s+(\d+)\s+([0-9a-f]{12})\s*$", line.strip())
         if m and m.group(1) == kernel:
-            return int(m.group(2))
+            return int(m.group(2)) if m.group(3) == source_hash(kernel) else None
     return None
 
 
@@ -48,51 +65,83 @@ def device_sync():
     lib.hipDeviceSynchronize()
 
 
-def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
-    """Reference OpenMP Fortran build (oracle/_ref) on a bounded sample of the same
-    workload; falls back to the C restatement when the reference build is absent."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may run on (its CPU affinity set)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _ref_sample(w, threads: int, sens_meas: int, fwd_electrodes: int):
+    """One timing of the reference build (oracle/_ref) on `threads` OpenMP threads:
+    the forward of min(K, threads) wavenumbers x `fwd_electrodes` electrodes and bsensi
+    of `sens_meas` measurements, each scaled to the full step."""
     import numpy as np
     from oracle import oracle as O
 
     g, s = w.grid, w.survey
+    R = O.RefSession(g, s)
+    kwn, _, _, _ = R.rwaven()
+    R.precal()
+    r = O.ref()
+    r.ref_forward_sample.restype = ctypes.c_double
+    r.ref_sens_sample.restype = ctypes.c_double
+    K, E = kwn.size, g.eanz
+    nk = min(K, threads)
+    sig = np.ascontiguousarray(w.sigma, np.complex128)
+    t_f = r.ref_forward_sample(O.ptr(sig), threads, nk, fwd_electrodes)
+    # the reference spreads the forward over wavenumbers only (src/fem.f90:257-277)
+    waves = -(-K // nk)
+    fwd_full = waves * E * t_f / fwd_electrodes
+    t_s = r.ref_sens_sample(threads, sens_meas)
+    sens_full = t_s * s.nanz / sens_meas
+    return dict(threads=threads, forward_s=fwd_full, sens_s=sens_full, sample_forward_s=t_f, sample_sens_s=t_s,
+                nk=nk, fwd_electrodes=fwd_electrodes, sens_meas=sens_meas)
+
+
+def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
+    """Reference OpenMP Fortran build (oracle/_ref, amdflang -O2 -fopenmp) on a bounded
+    sample of the same workload, on every core this process may use and on one
+    thread; the C restatement (one thread, forward only) when the reference build is
+    absent."""
+    from oracle import oracle as O
+
+    g, s = w.grid, w.survey
     if O.ref_available():
-        R = O.RefSession(g, s)
-        kwn, kwi, _, _ = R.rwaven()
-        R.precal()
-        r = O.ref()
-        r.ref_forward_sample.restype = ctypes.c_double
-        r.ref_sens_sample.restype = ctypes.c_double
-        K, E = kwn.size, g.eanz
-        nk = min(K, threads)
-        sig = np.ascontiguousarray(w.sigma, np.complex128)
-        t_f = r.ref_forward_sample(O.ptr(sig), threads, nk, fwd_electrodes)
-        # the reference spreads the forward over wavenumbers only (src/fem.f90:257-277)
-        waves = -(-K // min(K, threads))
-        fwd_full = waves * E * t_f / fwd_electrodes
-        t_s = r.ref_sens_sample(threads, sens_meas)
-        sens_full = t_s * s.nanz / sens_meas
-        kind = "reference"
-        sample = (f"reference OpenMP Fortran (amdflang -O2 -fopenmp): forward of {nk} wavenumbers x "
-                  f"{fwd_electrodes} electrodes ({t_f:.2f} s), bsensi of {sens_meas} of {s.nanz} "
-                  f"measurements ({t_s:.2f} s), scaled to the full step")
-    else:
-        kwn, kwi, _, _ = O.rwaven(g, s)
-        elbg, relbg, kg, lb = O.precal(g, kwn)
-        K, E = kwn.size, g.eanz
-        systems = [(l, 1) for l in range(1, fwd_electrodes + 1)]
-        t0 = time.perf_counter()
-        O.forward(g, w.sigma, elbg, relbg, kg, lb, systems=systems)
-        t_f = time.perf_counter() - t0
-        fwd_full = t_f * K * E / fwd_electrodes
-        fwd_full_threads = fwd_full
-        threads = 1
-        sens_full = float("nan")
-        kind = "port"
-        sample = f"C restatement, 1 thread: forward of {fwd_electrodes} systems scaled to {K * E}"
-        return dict(value=1.0 / fwd_full_threads, unit="solves/s", cores=1, kind=kind, sample=sample,
-                    forward_s=fwd_full, sens_s=None)
-    return dict(value=1.0 / (fwd_full + sens_full), unit="solves/s", cores=threads, kind=kind, sample=sample,
-                forward_s=fwd_full, sens_s=sens_full)
+        a = _ref_sample(w, threads, sens_meas, fwd_electrodes)
+        one = _ref_sample(w, 1, max(4, sens_meas // 12), fwd_electrodes)
+        sample = (f"reference OpenMP Fortran (amdflang -O2 -fopenmp) on {threads} threads: forward of {a['nk']} "
+                  f"wavenumbers x {fwd_electrodes} electrodes ({a['sample_forward_s']:.2f} s), bsensi of "
+                  f"{sens_meas} of {s.nanz} measurements ({a['sample_sens_s']:.2f} s), scaled to the full step; "
+                  f"one thread: {one['nk']} wavenumber x {fwd_electrodes} electrodes "
+                  f"({one['sample_forward_s']:.2f} s) and {one['sens_meas']} measurements "
+                  f"({one['sample_sens_s']:.2f} s)")
+        return dict(value=1.0 / (a["forward_s"] + a["sens_s"]), unit="solves/s", cores=threads, kind="reference",
+                    sample=sample, forward_s=a["forward_s"], sens_s=a["sens_s"], cpu_model=cpu_model(),
+                    one_thread=dict(value=1.0 / (one["forward_s"] + one["sens_s"]), forward_s=one["forward_s"],
+                                    sens_s=one["sens_s"]))
+    kwn, kwi, _, _ = O.rwaven(g, s)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    K, E = kwn.size, g.eanz
+    systems = [(l, 1) for l in range(1, fwd_electrodes + 1)]
+    t0 = time.perf_counter()
+    O.forward(g, w.sigma, elbg, relbg, kg, lb, systems=systems)
+    t_f = time.perf_counter() - t0
+    fwd_full = t_f * K * E / fwd_electrodes
+    sample = f"C restatement, 1 thread: forward of {fwd_electrodes} systems scaled to {K * E}"
+    return dict(value=1.0 / fwd_full, unit="solves/s", cores=1, kind="port", sample=sample,
+                forward_s=fwd_full, sens_s=None, cpu_model=cpu_model())
 
 
 def other_rooflines(g, s, K, E, nsys, phase, bits):
@@ -313,6 +362,107 @@ def gn_step(ctx, w, ltri: int):
             "mixed_precision_forward": mixed, "dc_and_fpi": dc}
 
 
+def gn_inversion(nit: int, train_models: int, pc_steps: int, eps: float = 1e-4):
+    """The metric's second half: Gauss-Newton time at equal misfit on the 400 x 100 quad
+    workload (96 electrodes, 4500 measurements, 3 % + 1e-4 V noise, start model
+    homogeneous 100 ohm-m / -5 mrad), the loop of crtomo_stable_amd.inversion (lam0 from
+    blam0, halved per step, a step that does not lower the misfit retried at half
+    length), with each CG form:
+      fp64_jacobi    the reference's method on the device: FP64 J, Jacobi scaling;
+      fp32j_fsai     complex FP32 copy of J in the iterations, FSAI preconditioner;
+      fp32j_learned  FP32 J copy, learned preconditioner trained beforehand on the CG
+                     systems of `train_models` inversions of other models (seeded
+                     Gaussian random fields, 10..1000 ohm-m, correlation 2..20 m) on
+                     the same grid and survey, none of them the scored model.
+    fp64_jacobi runs `nit` steps; every other form runs until its misfit is within 1 %
+    of that final misfit (at most nit + 2 steps), and its total Gauss-Newton time is
+    compared.  The CG stops at the reference's default eps (1e-4 on the squared
+    residual, src/rall.f90:210) with ncgmax = manz/10: at eps 1e-12 (relative residual
+    1e-6) every form reaches ncgmax from the second step on, so those runs compare
+    per-iteration costs only; the first step at 1e-12, where all forms converge, is
+    reported beside it (first_step_eps_1e-12)."""
+    import numpy as np
+    from crtomo_stable_amd import configs, inversion as INV, lib
+
+    w = configs.c3()
+    g = w.grid
+    manz = g.elanz
+    ctx = lib.Context(0)
+    ctx.set_grid(g)
+    ctx.set_survey(w.survey)
+    ctx.rwaven()
+    ctx.precal()
+    dat, wmatd = INV.synthetic_data(ctx, w.sigma)
+    wdfak = np.ones(dat.size, np.int32)
+    par0 = np.full(manz, np.log((1.0 / 100.0) * np.exp(-1j * 5e-3)))
+    out = {"workload": "c3: 400x100 quads (40000 cells), 96 electrodes, 4500 dipole-dipole + Wenner, 3 % + 1e-4 V "
+                       "noise; complex Gauss-Newton from 100 ohm-m / -5 mrad, lambda lam0 (blam0) halved per step "
+                       "with half-step control; ncgmax manz/10",
+           "cg_eps": eps, "reference_steps": nit, "forms": {}, "first_step_eps_1e-12": {}}
+    forms = (("fp64_jacobi", 64, 0), ("fp32j_fsai", 32, 2), ("fp32j_learned", 32, 3))
+    lamfix, target = None, None
+    for name, jbits, prec in forms:
+        t_train, losses = 0.0, []
+        if prec == 3:
+            # training inversions of other models (their CG in the FP32-J FSAI form), each
+            # system added to the training set and trained on right after its update
+            ctx.set_cg_precision(32)
+            ctx.set_cg_preconditioner(2)
+            t0 = time.perf_counter()
+
+            def hook(c, it):
+                losses.append(c.pc_train(pc_steps, 3e-3))
+            for m in range(train_models):
+                rng = np.random.default_rng(100 + m)
+                sig_m = configs.grf_sigma(g, float(rng.uniform(2.0, 20.0)), seed=1000 + m)
+                dat_m, w_m = INV.synthetic_data(ctx, sig_m, seed=200 + m)
+                INV.gauss_newton(ctx, g, dat_m, w_m, par0, nit, ltri=0, eps=eps, lam_factor=0.5, on_update=hook)
+            device_sync()
+            t_train = time.perf_counter() - t0
+        ctx.set_cg_precision(jbits)
+        ctx.set_cg_preconditioner(prec)
+        steps = nit if target is None else nit + 2
+        r = INV.gauss_newton(ctx, g, dat, wmatd, par0, steps, lamfix=lamfix, ltri=0, eps=eps, lam_factor=0.5,
+                             rmsmin=1.0 if target is None else 1.01 * target)
+        if target is None:
+            lamfix, target = r.lamfix, r.final_rms
+        st = r.step_ms()
+        its = [i for i in r.iterations if i.ncg]
+        f = {"rms": [round(i.rms, 4) for i in r.iterations], "final_rms": r.final_rms, "steps": len(st),
+             "ncg": [i.ncg for i in its], "lambda": [i.lam for i in its],
+             "step_length": [i.ms.get("step") for i in its],
+             "step_ms": [round(x, 2) for x in st], "total_ms": round(float(np.sum(st)), 2),
+             "phase_ms_total": {k: round(float(np.sum([i.ms.get(k, 0.0) for i in its])), 2)
+                                for k in ("sens", "update", "forward", "forward_half_step")}}
+        if prec == 3:
+            f["training"] = {"models": train_models, "adam_steps_per_system": pc_steps,
+                             "seconds": round(t_train, 1), "systems": len(losses),
+                             "loss_first_last_of_each_call": losses[:1] + losses[-1:]}
+        out["forms"][name] = f
+        # the first step at eps 1e-12 (relative residual 1e-6) from the start model
+        ctx.set_sigma(np.exp(par0))
+        ctx.forward()
+        ctx.voltages(1)
+        ctx.sens(1)
+        device_sync()
+        t0 = time.perf_counter()
+        u = ctx.update(lamfix, 1e-12, manz // 10, wmatd, wdfak, dat, par0)
+        device_sync()
+        out["first_step_eps_1e-12"][name] = {"update_ms": round(1e3 * (time.perf_counter() - t0), 2),
+                                             "ncg": u["ncg"]}
+    ctx.set_cg_precision(64)
+    ctx.set_cg_preconditioner(0)
+    ctx.close()
+    base = out["forms"]["fp64_jacobi"]
+    for name, f in out["forms"].items():
+        f["speedup_vs_fp64_jacobi"] = round(base["total_ms"] / f["total_ms"], 3)
+        f["final_rms_rel_diff_vs_fp64_jacobi"] = round(abs(f["final_rms"] - base["final_rms"]) / base["final_rms"], 5)
+    b1 = out["first_step_eps_1e-12"]["fp64_jacobi"]["update_ms"]
+    for name, f in out["first_step_eps_1e-12"].items():
+        f["speedup_vs_fp64_jacobi"] = round(b1 / f["update_ms"], 3)
+    return out
+
+
 def buncert_step(ctx, w, ltri: int):
     """Model uncertainty (src/bmcm_mod.f90:49-175) on the step's Jacobian with unit
     data weights and the device regularisation: Re(J^H W J) (dsyrk), the reference's
@@ -438,8 +588,10 @@ def main():
     ap.add_argument("--workload", default="c1", choices=["c1", "c0", "c2", "c3"])
     ap.add_argument("--models", type=int, default=256, help="models of the c2 batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="only copy 512 MiB three times (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--no-gn-inversion", action="store_true",
+                    help="skip the Gauss-Newton comparison of CG forms on the 400x100 workload")
+    ap.add_argument("--gn-iterations", type=int, default=4)
+    ap.add_argument("--gn-train-models", type=int, default=2)
     ap.add_argument("--gn", action="store_true",
                     help="also time one Gauss-Newton step (forward, bsensi, bsmatm, blam0, update/cjg)")
     ap.add_argument("--forward-precision", type=int, default=64, choices=[64, 32],
@@ -447,19 +599,15 @@ def main():
     ap.add_argument("--buncert", action="store_true",
                     help="also time the model uncertainty (crt_buncert: Re(J^H W J), +lam R, inverse, "
                          "resolution, cov2) on the step's Jacobian")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--cpu-sens-meas", type=int, default=96)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline (0: every "
+                    "core this process may use)")
+    ap.add_argument("--cpu-sens-meas", type=int, default=0, help="0: max(96, 3 x threads)")
     ap.add_argument("--cpu-fwd-electrodes", type=int, default=4)
     args = ap.parse_args()
-    if args.copy_probe:
-        import torch
-        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_per_copy": a.numel() * 8, "copies": 3}), flush=True)
-        return
+    if args.cpu_threads <= 0:
+        args.cpu_threads = host_cores()
+    if args.cpu_sens_meas <= 0:
+        args.cpu_sens_meas = max(96, 3 * args.cpu_threads)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in "
               "DESIGN.md (shard the (electrode, wavenumber) systems, all-reduce the CG vectors)",
@@ -563,6 +711,10 @@ def main():
         out["gn_step"] = gn_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
     if args.buncert:
         out["buncert"] = buncert_step(ctx, w, 0 if getattr(g, "nx", 0) else 1)
+    if not args.no_gn_inversion and args.workload == "c1":
+        ctx.close()
+        ctx = None
+        out["gn_inversion"] = gn_inversion(args.gn_iterations, args.gn_train_models, 100)
     if not args.no_cpu_baseline:
         try:
             cb = cpu_baseline(w, args.cpu_threads, args.cpu_sens_meas, args.cpu_fwd_electrodes)
@@ -570,7 +722,8 @@ def main():
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
-    ctx.close()
+    if ctx is not None:
+        ctx.close()
 
 
 if __name__ == "__main__":

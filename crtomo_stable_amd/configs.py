@@ -113,6 +113,15 @@ def c2_model(i: int, cutmck=None, seed: int = 2, min_cells: int = 5000, max_cell
     return quad_workload(f"c2[{i}]", nx, nz, 0.5, 0.5, nelec, spacing_nodes, first, quads, sig, cutmck)
 
 
+def grf_sigma(grid: G.Grid, corr: float, seed: int) -> np.ndarray:
+    """A complex conductivity on ``grid``: magnitude log-uniform 10..1000 ohm-m and phase
+    -1..-30 mrad, each a Gaussian random field of correlation length ``corr`` (m)."""
+    cx, cy = grid.centroids()
+    f = G.uniformize(G.gaussian_random_field(cx, cy, corr, seed=seed))
+    fp = G.uniformize(G.gaussian_random_field(cx, cy, corr, seed=seed + 1))
+    return G.sigma_from_rho(10.0 ** (1.0 + 2.0 * f), -1.0 - 29.0 * fp)
+
+
 def c3(cutmck=None, nx: int = 400, nz: int = 100, nelec: int = 96, nmeas: int = 4500, seed: int = 3) -> Workload:
     dx = dz = 1.0 * (400 / nx) * 0.5 if nx != 400 else 0.5
     spacing_nodes = max(1, (nx - 4) // (nelec - 1))
