@@ -160,6 +160,11 @@ enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_
 
 struct crt_ctx {
     int device = 0;
+    // the single-system drop-ins (crt_chol, crt_vre and the twins built on them) share
+    // the workspace below and the stream; the reference calls chol/vre from its OpenMP
+    // loop over wavenumbers (src/fem.f90:328-380), so these calls take this lock and
+    // may come from several host threads at once
+    std::recursive_mutex dropin_mu;
     hipStream_t stream = nullptr;
     hipEvent_t ev0[T_N], ev1[T_N];
     double times[T_N] = {0};
@@ -2135,6 +2140,7 @@ static DevGrid band_grid(int sanz, int mb) {
 // the reference layout (src/chol.f90:46-78); errnr 28 on a vanishing pivot.
 int crt_chol(crt_ctx *c, int sanz, int mb, double *a) {
     if (!c || !a || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->dropin_mu);
     CRT_HIP(hipSetDevice(c->device));
     DevGrid g = band_grid(sanz, mb);
     if (crt_factor_lds_bytes(g.D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
@@ -2200,6 +2206,7 @@ int crt_vredc(crt_ctx *c, int sanz, int mb, const double *a, const double *b, do
 // vre(a, b, pot): pot = -A^{-1} b from the band factor (src/vre.f90:45-69).
 int crt_vre(crt_ctx *c, int sanz, int mb, const double *a, const double *b, double *pot) {
     if (!c || !a || !b || !pot || sanz <= 0 || mb < 0) return CRT_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->dropin_mu);
     CRT_HIP(hipSetDevice(c->device));
     DevGrid g = band_grid(sanz, mb);
     size_t nband = (size_t)sanz * (mb + 1), npanel = (size_t)g.nblk * (g.D + 1) * CRT_NB * CRT_NB;

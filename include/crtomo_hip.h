@@ -20,7 +20,10 @@
  *   901 call order, 902 argument, 903 unsupported configuration, 904 a solve larger
  *   than the static scratch of crt_cjg_noalloc).
  * A context is bound to one device and one HIP stream; calls on one context
- * must not overlap.
+ * must not overlap, except the single-system drop-ins crt_chol, crt_vre,
+ * crt_choldc, crt_vredc and crt_cholz, which serialise on the context and may be
+ * called from several host threads at once (the reference calls chol/vre inside
+ * its OpenMP loop over wavenumbers).
  */
 #ifndef CRTOMO_HIP_H
 #define CRTOMO_HIP_H

@@ -16,6 +16,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libcrtomo_ref.so")
+# the reference routines and harness with chol, vre, choldc and vredc replaced by the
+# library's Fortran shims (crtomo_stable_amd/fortran, make -C oracle dropin)
+DROPIN_PATH = os.path.join(HERE, "_ref", "libcrtomo_dropin.so")
 
 _i = C.c_int
 _d = C.c_double
@@ -387,9 +390,11 @@ def ref():
 class RefSession:
     """Fills the reference modules and drives its routines (oracle/ref_harness.f90)."""
 
-    def __init__(self, grid, survey):
+    def __init__(self, grid, survey, lib=None):
+        """``lib``: another build of the harness (the drop-in library, whose chol / vre /
+        choldc / vredc are the GPU shims); the reference build by default."""
         self.grid, self.survey = grid, survey
-        r = ref()
+        r = self.r = ref() if lib is None else lib
         self._keep = [grid.typ, grid.nelanz, grid.selanz, grid.nrel, grid.x, grid.y, grid.rnr, grid.enr]
         rnr = grid.rnr if grid.rnr.size else np.zeros(1, np.int32)
         self._keep.append(rnr)
@@ -400,7 +405,7 @@ class RefSession:
 
     def rwaven(self):
         kwn = np.zeros(30); kwi = np.zeros(30); n = _i(); amin = _d(); amax = _d()
-        _chk(ref().ref_rwaven(C.byref(n), ptr(kwn), ptr(kwi), C.byref(amin), C.byref(amax)), "ref_rwaven")
+        _chk(self.r.ref_rwaven(C.byref(n), ptr(kwn), ptr(kwi), C.byref(amin), C.byref(amax)), "ref_rwaven")
         self.K = n.value
         return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
 
@@ -410,27 +415,27 @@ class RefSession:
         relbg = np.zeros((g.npk, max(g.relanz, 1)))
         kg = np.zeros((self.K, g.eanz, max(g.relanz, 1)))
         lb = _i()
-        _chk(ref().ref_precal(int(lsytop), ptr(elbg), ptr(relbg), ptr(kg), C.byref(lb)), "ref_precal")
+        _chk(self.r.ref_precal(int(lsytop), ptr(elbg), ptr(relbg), ptr(kg), C.byref(lb)), "ref_precal")
         return elbg, relbg, kg, bool(lb.value)
 
     def forward(self, sigma, nthreads=1):
         g = self.grid
         kpot = np.zeros((self.K, g.eanz, g.sanz), np.complex128)
         s = np.ascontiguousarray(sigma, np.complex128)
-        _chk(ref().ref_forward(ptr(s), int(nthreads), ptr(kpot)), "ref_forward")
+        _chk(self.r.ref_forward(ptr(s), int(nthreads), ptr(kpot)), "ref_forward")
         return kpot
 
     def voltages(self, complex_inv=True):
         g = self.grid
         hpot = np.zeros((g.eanz, g.sanz), np.complex128)
         volt = np.zeros(self.survey.nanz, np.complex128); sa = np.zeros_like(volt)
-        err = ref().ref_voltages(int(complex_inv), ptr(hpot), ptr(volt), ptr(sa))
+        err = self.r.ref_voltages(int(complex_inv), ptr(hpot), ptr(volt), ptr(sa))
         _chk(err, "ref_voltages")
         return hpot, volt, sa
 
     def sens(self, inversion=True):
         sens = np.zeros((self.grid.elanz, self.survey.nanz), np.complex128)
-        ref().ref_sens(int(inversion), ptr(sens))
+        self.r.ref_sens(int(inversion), ptr(sens))
         return sens
 
     def bsmatm(self, ltri, nx=0, nz=0, alfx=1.0, alfz=1.0):
@@ -438,13 +443,13 @@ class RefSession:
         ncol = 3 if ltri == 0 else g.maxn + 1
         sm = np.zeros((ncol, g.elanz))
         nb = np.zeros((g.maxn + 1, g.elanz), np.int32)
-        ref().ref_bsmatm(ltri, nx, nz, _d(alfx), _d(alfz), ptr(sm), ptr(nb))
+        self.r.ref_bsmatm(ltri, nx, nz, _d(alfx), _d(alfz), ptr(sm), ptr(nb))
         return sm, nb
 
     def blam0(self, fpi=False, alfx=1.0, alfz=1.0, dc=False):
         """lammax of src/blam0.f90 on the state of the last update() call (dc: the
         ldc branch over sensdc)."""
-        f = ref().ref_blam0
+        f = self.r.ref_blam0
         f.restype = C.c_double
         return f(int(bool(fpi)), _d(alfx), _d(alfz), int(bool(dc)))
 
@@ -452,19 +457,19 @@ class RefSession:
         g = self.grid
         kpot = np.zeros((self.K, g.eanz, g.sanz))
         s = np.ascontiguousarray(sigma, np.complex128)
-        _chk(ref().ref_forward_dc(ptr(s), ptr(kpot)), "ref_forward_dc")
+        _chk(self.r.ref_forward_dc(ptr(s), ptr(kpot)), "ref_forward_dc")
         return kpot
 
     def voltages_dc(self):
         g = self.grid
         hpot = np.zeros((g.eanz, g.sanz))
         volt = np.zeros(self.survey.nanz, np.complex128); sa = np.zeros_like(volt)
-        _chk(ref().ref_voltages_dc(ptr(hpot), ptr(volt), ptr(sa)), "ref_voltages_dc")
+        _chk(self.r.ref_voltages_dc(ptr(hpot), ptr(volt), ptr(sa)), "ref_voltages_dc")
         return hpot, volt, sa
 
     def sens_dc(self):
         sens = np.zeros((self.grid.elanz, self.survey.nanz))
-        ref().ref_sens_dc(ptr(sens))
+        self.r.ref_sens_dc(ptr(sens))
         return sens
 
     def buncert(self, lam, sigma, lres=True, lcov2=True, gauss=False):
@@ -479,7 +484,7 @@ class RefSession:
         try:
             s = np.ascontiguousarray(sigma, np.complex128)
             db = d.encode()
-            err = ref().ref_buncert(_d(lam), int(bool(lres)), int(bool(lcov2)), int(bool(gauss)), ptr(s),
+            err = self.r.ref_buncert(_d(lam), int(bool(lres)), int(bool(lcov2)), int(bool(gauss)), ptr(s),
                                    C.c_char_p(db), len(db))
             _chk(err, "ref_buncert")
             out = {}
@@ -504,7 +509,7 @@ class RefSession:
         sd = np.zeros(1) if sensdc is None else np.ascontiguousarray(sensdc, np.float64)
         cgfac = np.zeros(p.manz); bvec = np.zeros(p.manz, np.complex128)
         dpar = np.zeros(p.manz, np.complex128); cgres = np.zeros(p.s.ncgmax + 1); ncg = _i()
-        _chk(ref().ref_update(p.s.ncgmax, _d(p.s.eps), _d(p.s.lam), ptr(a["sens"]), ptr(a["wmatd"]),
+        _chk(self.r.ref_update(p.s.ncgmax, _d(p.s.eps), _d(p.s.lam), ptr(a["sens"]), ptr(a["wmatd"]),
                               ptr(a["wdfak"]), ptr(a["dat"]), ptr(a["sigmaa"]), ptr(a["par"]), ptr(cgfac),
                               ptr(bvec), ptr(dpar), ptr(cgres), C.byref(ncg), int(kind), ptr(sd)),
              "ref_update")
