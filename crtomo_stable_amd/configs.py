@@ -45,10 +45,11 @@ def _cutmck(grid: G.Grid, cutmck: Optional[Callable] = None) -> G.Grid:
 
 
 def quad_workload(name: str, nx: int, nz: int, dx: float, dz: float, nelec: int, elec_every: int,
-                  elec_first: int, quads, sigma_fn, cutmck=None, mixed: bool = True) -> Workload:
+                  elec_first: int, quads, sigma_fn, cutmck=None, mixed: bool = True,
+                  dirichlet: bool = False) -> Workload:
     n1 = nx + 1
     elec_nodes = [elec_first + i * elec_every + 1 for i in range(nelec)]
-    g = G.quad_grid(nx, nz, dx, dz, elec_nodes=elec_nodes, mixed=mixed)
+    g = G.quad_grid(nx, nz, dx, dz, elec_nodes=elec_nodes, mixed=mixed, dirichlet=dirichlet)
     cx, cy = g.centroids()
     sigma = sigma_fn(cx, cy)
     g = _cutmck(g, cutmck)
@@ -147,14 +148,16 @@ def c3(cutmck=None, nx: int = 400, nz: int = 100, nelec: int = 96, nmeas: int = 
     return quad_workload("c3", nx, nz, dx, dz, nelec, spacing_nodes, first, quads, sig, cutmck)
 
 
-def small_quad(nx: int = 24, nz: int = 8, nelec: int = 10, cutmck=None, mixed: bool = True) -> Workload:
+def small_quad(nx: int = 24, nz: int = 8, nelec: int = 10, cutmck=None, mixed: bool = True,
+               dirichlet: bool = False) -> Workload:
     """A small C0-like grid for fast parity tests (type-8 quads; mixed boundaries
-    unless ``mixed=False``, which leaves every boundary no-flow)."""
+    unless ``mixed=False``, which leaves every boundary no-flow, or ``dirichlet=True``,
+    which makes the sides and the bottom Dirichlet boundaries, type 13)."""
     def sig(cx, cy):
         rho = 30.0 + 70.0 * (cx / cx.max())
         blk = (cx > 0.3 * cx.max()) & (cx < 0.5 * cx.max()) & (cy < -1.0)
         rho[blk] = 5.0
         return G.sigma_from_rho(rho, -5.0 - 10.0 * (cx / cx.max()))
     quads = G.dipole_dipole(nelec, [1, 2], range(1, 5))
-    return quad_workload("small_quad" if mixed else "small_quad_noflow", nx, nz, 0.5, 0.5, nelec, 2, 2, quads,
-                         sig, cutmck, mixed)
+    name = "small_quad_dirichlet" if dirichlet else ("small_quad" if mixed else "small_quad_noflow")
+    return quad_workload(name, nx, nz, 0.5, 0.5, nelec, 2, 2, quads, sig, cutmck, mixed, dirichlet)

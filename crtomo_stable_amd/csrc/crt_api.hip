@@ -17,6 +17,7 @@ int crt_launch_precal(hipStream_t, int, int, int, int, int, int, const int *, co
                       double *, int *);
 int crt_launch_diag(hipStream_t, const DevGrid &, int, const int2 *, double *, double *, int *);
 int crt_launch_dc_pivot(hipStream_t, int, int, int, const zd *, int *);
+int crt_launch_zero_sources(hipStream_t, int, int, int, int, const int2 *, zd *);
 int crt_launch_dc_pivot32(hipStream_t, int, int, int, const zf *, int *);
 int crt_launch_assemble(hipStream_t, const DevGrid &, int, const int2 *, const double *, const double *, zd *);
 int crt_launch_area(hipStream_t, const DevGrid &, int, zd *);
@@ -179,6 +180,17 @@ struct crt_ctx {
     int kwnanz = 0;
     std::vector<double> kwn, kwnwi;
     int lbeta = 0;
+    // Dirichlet boundaries: nodes of type-13 elements (lrandb, src/randb.f90) and of the
+    // boundary values of boundary.dat (lrandb2, src/randb2.f90; the device runs zero
+    // values only); dmask holds the per-entry identity rows and columns they make
+    bool has13 = false;
+    std::vector<unsigned char> dnode13;
+    std::vector<int> bnd_nodes;
+    bool bnd_zero = true;
+    std::vector<int> h_col, h_crow;  // CSR column and row of every entry (host copy)
+    DBuf<unsigned char> d_dmask;
+    bool dmask_ok = false, dmask_any = false;
+    std::vector<unsigned char> dnode;  // every Dirichlet node of the last dmask build
     // device state
     DBuf<int> d_nrel, d_etyp, d_btyp, d_rnr, d_enr, d_row_ptr, d_col, d_crow, d_cptr, d_contrib, d_nkel, d_status;
     DBuf<int> d_tmap;
@@ -257,6 +269,7 @@ struct crt_ctx {
         g.csplit = asplit ? d_csplit.p : nullptr;
         g.area = nullptr;  // set by the callers that computed the area sums
         g.dc = inv_kind == 1 ? 1 : 0;
+        g.dmask = dmask_any ? d_dmask.p : nullptr;
         return g;
     }
     void tic(int i) { (void)hipEventRecord(ev0[i], stream); }
@@ -633,7 +646,6 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     c->elanz = c->relanz = 0;
     bool boundary_seen = false;
     for (int t = 0; t < typanz; ++t) {
-        if (typ[t] == 13) return CRT_ERR_UNSUPPORTED;  // Dirichlet boundaries are not on this path
         if (typ[t] > 10) { c->relanz += nelanz[t]; boundary_seen = true; }
         else {
             if (boundary_seen) return CRT_ERR_ARG;  // area groups precede boundary groups
@@ -664,6 +676,26 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
                 else c->btyp[iel - c->elanz] = typ[t];
             }
     }
+    // nodes of the type-13 (Dirichlet) elements
+    c->has13 = false;
+    c->dnode13.assign(sanz, 0);
+    {
+        int iel = 0;
+        for (int t = 0; t < typanz; ++t)
+            for (int j = 0; j < nelanz[t]; ++j, ++iel) {
+                if (typ[t] != 13) continue;
+                c->has13 = true;
+                for (int m = 0; m < selanz[t]; ++m) {
+                    const int nd = nrel[(size_t)iel * maxn + m];
+                    if (nd < 1 || nd > sanz) return CRT_ERR_ARG;
+                    c->dnode13[nd - 1] = 1;
+                }
+            }
+    }
+    c->bnd_nodes.clear();
+    c->bnd_zero = true;
+    c->dmask_ok = false;
+    c->dmask_any = false;
     // assembly map: contributions to each lower-triangle entry in the element order
     // of src/kompab.f90:80-138 (area elements, then type-11 boundary elements)
     struct Ent { int i, j, code; };
@@ -721,6 +753,8 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     }
     if (csplit.empty()) csplit.push_back(0);
     if (contrib.empty()) contrib.push_back(0);
+    c->h_col = col;
+    c->h_crow = crow;
 
     c->nblk = (sanz + CRT_NB - 1) / CRT_NB;
     c->D = std::max(1, (mb + CRT_NB - 1) / CRT_NB);
@@ -873,6 +907,19 @@ int crt_set_wavenumbers(crt_ctx *c, int kwnanz, const double *kwn, const double 
     return 0;
 }
 
+int crt_set_boundary_values(crt_ctx *c, int n, const int *nodes, const double *values) {
+    if (!c || n < 0 || (n > 0 && (!nodes || !values))) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    for (int i = 0; i < n; ++i)
+        if (nodes[i] < 1 || nodes[i] > c->sanz) return CRT_ERR_BOUNDARY_NODE;  // src/rrandb.f90:62-66
+    c->bnd_nodes.assign(nodes, nodes + n);
+    c->bnd_zero = true;
+    for (int i = 0; i < 2 * n; ++i) c->bnd_zero = c->bnd_zero && values[i] == 0.0;
+    c->dmask_ok = false;
+    c->have_kpot = c->have_volt = c->have_sens = false;
+    return 0;
+}
+
 int crt_precal(crt_ctx *c, int lsytop, int *lbeta) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_grid || !c->have_kwn) return CRT_ERR_STATE;
@@ -900,6 +947,8 @@ int crt_precal(crt_ctx *c, int lsytop, int *lbeta) {
     c->lbeta = 0;
     for (int t = 0; t < c->typanz; ++t)
         if (c->typ[t] == 11) c->lbeta = 1;
+    // mixed and Dirichlet boundaries exclude each other (src/precal.f90:102-106,130-135)
+    if (c->lbeta && c->has13) return CRT_ERR_BOUNDARY_MIX;
     if (lbeta) *lbeta = c->lbeta;
     c->have_precal = true;
     return 0;
@@ -947,6 +996,55 @@ int crt_set_inversion_kind(crt_ctx *c, int kind) {
         c->sens32_ok = c->sensr_ok = false;
     }
     c->inv_kind = kind;
+    return 0;
+}
+
+// Identity rows and columns of the Dirichlet nodes (randb for type-13 elements,
+// randb2 for zero boundary values): per CSR entry 1 when it touches such a node, 2 on
+// its diagonal.  Refuses prescribed non-zero values, whose lifting changes the
+// right-hand side of every system (903); with `no_dirichlet_sources` also a source
+// electrode on a Dirichlet node (the batched path does not clear such sources).
+static int build_dmask(crt_ctx *c, bool no_dirichlet_sources = false) {
+    if (!c->bnd_zero) return CRT_ERR_UNSUPPORTED;
+    if (!c->dmask_ok) {
+        c->dnode = c->dnode13;
+        for (int nd : c->bnd_nodes) c->dnode[nd - 1] = 1;
+        c->dmask_any = false;
+        for (unsigned char v : c->dnode) c->dmask_any = c->dmask_any || v;
+        if (c->dmask_any) {
+            std::vector<unsigned char> m(c->h_col.size());
+            for (size_t q = 0; q < m.size(); ++q) {
+                const int i = c->h_crow[q], j = c->h_col[q];
+                m[q] = (c->dnode[i] || c->dnode[j]) ? (i == j ? 2 : 1) : 0;
+            }
+            if (int e = c->d_dmask.upload(m.data(), m.size(), c->stream)) return e;
+        }
+        c->dmask_ok = true;
+    }
+    if (c->dmask_any && no_dirichlet_sources)
+        for (int nd : c->enr0)
+            if (c->dnode[nd]) return CRT_ERR_UNSUPPORTED;
+    return 0;
+}
+
+// The systems whose right-hand side randb / randb2 clear: the source electrode sits on a
+// Dirichlet node and the system is assembled by kompab (every (l, k) with mixed
+// boundaries, l = 1 per k otherwise; the other sources of a shared factor get kompb's
+// fresh right-hand side, src/kompb.f90:55-60, which keeps the source).  Their
+// potentials are zero, as the reference's solve of a zero right-hand side gives.
+static int zero_cleared_sources(crt_ctx *c) {
+    if (!c->dmask_any) return 0;
+    std::vector<int2> pairs;
+    for (int l = 0; l < c->eanz; ++l)
+        if (c->dnode[c->enr0[l]] && (c->lbeta || l == 0))
+            for (int k = 0; k < c->kwnanz; ++k) pairs.push_back(int2{l, k});
+    if (pairs.empty()) return 0;
+    DBuf<int2> d_pairs;
+    int e = d_pairs.upload(pairs.data(), pairs.size(), c->stream);
+    e = e ? e : crt_launch_zero_sources(c->stream, c->sanz, c->kwnanz, c->eanz, (int)pairs.size(), d_pairs.p,
+                                        c->d_kpotN.p);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1018,6 +1116,7 @@ int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
+    if (int e0 = build_dmask(c)) return e0;
     // 16x16 matrix-core tiles when the band fits (mb <= 256), else 8x8 LDS tiles
     const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
     const int NBt = use16 ? 16 : CRT_NB;
@@ -1176,6 +1275,7 @@ int crt_forward(crt_ctx *c) {
             }
     }
     if (worst) return worst;
+    if ((e = zero_cleared_sources(c))) return e;
     c->have_kpot = true;
     c->have_volt = c->have_sens = false;
     return 0;
@@ -2261,6 +2361,8 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
         if (!ctxs[i]) return CRT_ERR_ARG;
         if (!ctxs[i]->have_precal || !ctxs[i]->have_sigma) return CRT_ERR_STATE;
         if (ctxs[i]->device != ctxs[0]->device) return CRT_ERR_ARG;
+        CRT_HIP(hipSetDevice(ctxs[i]->device));
+        if (int e0 = build_dmask(ctxs[i], true)) return e0;
     }
     std::unique_ptr<crt_batch> b(new crt_batch());
     b->device = ctxs[0]->device;

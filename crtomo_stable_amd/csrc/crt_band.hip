@@ -62,6 +62,11 @@ __device__ __forceinline__ zd assemble_entry(const DevGrid &g, int e0, int e1, i
 // same order as assemble_entry, so the value is identical
 __device__ __forceinline__ zd assemble_q(const DevGrid &g, int q, int l, int k) {
 #pragma clang fp contract(off)
+    // randb / randb2 with zero values (src/randb.f90:40-84): identity row and column
+    if (g.dmask) {
+        const unsigned char m = g.dmask[q];
+        if (m) return zd{m == 2 ? 1.0 : 0.0, 0.0};
+    }
     if (!g.area) return assemble_entry(g, g.cptr[q], g.cptr[q + 1], l, k);
     zd s = g.area[(size_t)k * g.nnz + q];
     const int e1 = g.cptr[q + 1];
@@ -542,6 +547,23 @@ int crt_launch_dc_pivot(hipStream_t st, int nsys, int sanz, int fstride, const z
 }
 int crt_launch_dc_pivot32(hipStream_t st, int nsys, int sanz, int fstride, const zf *dinv, int *status) {
     return launch_dc_pivot(st, nsys, sanz, fstride, dinv, status);
+}
+
+// kpotN[(node K + k) E + l] = 0 for every node of the listed (l, k) systems
+__global__ void __launch_bounds__(256) crt_zero_sources_kernel(int sanz, int K, int E, int np,
+                                                               const int2 *__restrict__ pairs, zd *__restrict__ kpotN) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)np * sanz) return;
+    const int p = (int)(t / sanz), j = (int)(t - (long)p * sanz);
+    const int2 lk = pairs[p];
+    kpotN[((size_t)j * K + lk.y) * E + lk.x] = zd{0.0, 0.0};
+}
+
+int crt_launch_zero_sources(hipStream_t st, int sanz, int K, int E, int np, const int2 *pairs, zd *kpotN) {
+    const long n = (long)np * sanz;
+    hipLaunchKernelGGL(crt_zero_sources_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sanz, K, E, np,
+                       pairs, kpotN);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 int crt_launch_assemble(hipStream_t st, const DevGrid &g, int nsys, const int2 *sys, const double *fak,

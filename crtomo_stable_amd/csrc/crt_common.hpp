@@ -17,6 +17,8 @@ enum {
     CRT_ERR_BETA = 35,
     CRT_ERR_ELEC = 73,
     CRT_ERR_ZEROVOLT = 82,
+    CRT_ERR_BOUNDARY_NODE = 30,
+    CRT_ERR_BOUNDARY_MIX = 101,
     CRT_ERR_ALLOC = 97,
     CRT_ERR_TYPE = 18,
     CRT_ERR_HIP = 900,
@@ -139,6 +141,10 @@ struct DevGrid {
     // or pivot that is not real and positive is an error there (src/scaldc.f90:54-58,
     // src/choldc.f90:66-70), where the complex routines only stop on zero
     int dc;
+    // Dirichlet nodes (type-13 elements, src/randb.f90; homogeneous boundary values,
+    // src/randb2.f90): per CSR entry 0 as assembled, 1 forced to zero (a row or column
+    // of such a node), 2 forced to one (its diagonal); null when the grid has none
+    const unsigned char *dmask;
 };
 
 // Gauss-Newton CG state shared by crt_cg.hip and crt_api.hip

@@ -160,8 +160,12 @@ int read_rho(const std::string &path, std::vector<double> &sigma, int &n) {
 }
 
 struct CrmodCfg {
-    std::string elem, elec, rho, config, pot, volt, sens;
+    std::string elem, elec, rho, config, pot, volt, sens, randb;
     bool wpot = false, wvolt = false, wsens = false;
+    // src/fem.f90:177-193: another dataset, 2D (0) or 2.5D (1), fictitious sink and its
+    // node, boundary values and their file, the mswitch bits (lana 0, wkfak 1, lsr 2)
+    bool lagain = false, lsink = false, lrandb2 = false;
+    int swrtr = 1, nsink = 0, mswitch = 0;
 };
 
 std::string join(const std::string &dir, const std::string &p) {
@@ -194,6 +198,50 @@ int read_crmod_cfg(const std::string &path, CrmodCfg &c) {
     c.volt = join(dir, first(lines[8]));
     c.wsens = flag(lines[9]);
     c.sens = join(dir, first(lines[10]));
+    // the switches after the files (src/fem.f90:177-193); the reference stops with errnr 3
+    // when one of them is missing, mswitch alone is optional
+    if (lines.size() < 17) return 3;
+    try {
+        c.lagain = flag(lines[11]);
+        c.swrtr = std::stoi(first(lines[12]));
+        c.lsink = flag(lines[13]);
+        c.nsink = std::stoi(first(lines[14]));
+        c.lrandb2 = flag(lines[15]);
+        c.randb = join(dir, first(lines[16]));
+        c.mswitch = lines.size() > 17 && !first(lines[17]).empty() ? std::stoi(first(lines[17])) : 0;
+    } catch (...) {
+        return 3;
+    }
+    return 0;
+}
+
+// boundary.dat (src/rrandb.f90:85-120, complex case): rwdanz, then node and value per
+// line; a value is "(re,im)" as Fortran writes a complex, or a real number
+int read_boundary(const std::string &path, std::vector<int> &nodes, std::vector<double> &vals) {
+    std::ifstream f(path);
+    if (!f) return 1;
+    int n = 0;
+    if (!(f >> n) || n < 0) return 3;
+    nodes.resize(n);
+    vals.assign(2 * (size_t)n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        std::string v;
+        if (!(f >> nodes[i] >> v)) return 3;
+        if (v[0] == '(') {
+            std::string rest = v;
+            while (rest.find(')') == std::string::npos) {
+                std::string more;
+                if (!(f >> more)) return 3;
+                rest += more;
+            }
+            for (char &ch : rest)
+                if (ch == '(' || ch == ')' || ch == ',') ch = ' ';
+            std::istringstream is(rest);
+            if (!(is >> vals[2 * i] >> vals[2 * i + 1])) return 3;
+        } else {
+            try { vals[2 * i] = std::stod(v); } catch (...) { return 3; }
+        }
+    }
     return 0;
 }
 
@@ -294,12 +342,38 @@ int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path) {
                           g.nrel.data(), g.x.data(), g.y.data(), g.rnr.data(), ne, enr.data())))
         return e;
     if ((e = crt_set_survey(ctx, nc, strnr.data(), vnr.data()))) return e;
+    // switches this forward does not run are refused instead of ignored
+    if (cfg.lagain || cfg.lsink || (cfg.mswitch & 1) || (cfg.mswitch & 4)) {
+        fprintf(stderr, "crt_tomodir_crmod: %s is not supported\n",
+                cfg.lagain ? "another dataset (lagain)" : cfg.lsink ? "the fictitious sink (lsink)" :
+                (cfg.mswitch & 1) ? "the analytic solution (mswitch lana)" : "singularity removal (mswitch lsr)");
+        return CRT_ERR_UNSUPPORTED;
+    }
+    if (cfg.swrtr != 0 && cfg.swrtr != 1) return 3;
+    if (cfg.lrandb2) {
+        std::vector<int> bn;
+        std::vector<double> bv;
+        if ((e = read_boundary(cfg.randb, bn, bv))) return e;
+        if ((e = crt_set_boundary_values(ctx, (int)bn.size(), bn.data(), bv.data()))) return e;
+    }
     int kwnanz = 0;
     double amin = 0.0, amax = 0.0;
     std::vector<double> kwn(64), kwnwi(64);
-    if ((e = crt_rwaven(ctx, &kwnanz, kwn.data(), kwnwi.data(), &amin, &amax))) return e;
+    if (cfg.swrtr == 0) {
+        // 2D (src/fem.f90:241-250): one wavenumber k = 0; rtrafo and bsens take kpot
+        // itself (src/fem.f90:393), which the back-transform gives with the weight pi
+        kwnanz = 1;
+        kwn[0] = 0.0;
+        kwnwi[0] = std::acos(-1.0);
+        if ((e = crt_set_wavenumbers(ctx, 1, kwn.data(), kwnwi.data()))) return e;
+    } else if ((e = crt_rwaven(ctx, &kwnanz, kwn.data(), kwnwi.data(), &amin, &amax))) return e;
     int lbeta = 0;
     if ((e = crt_precal(ctx, 0, &lbeta))) return e;
+    if ((cfg.mswitch & 2) && lbeta) {
+        // wkfak with mixed boundaries asks for the numerical k-factors of bkfak
+        fprintf(stderr, "crt_tomodir_crmod: k-factors (mswitch wkfak) are not supported\n");
+        return CRT_ERR_UNSUPPORTED;
+    }
     if ((e = crt_set_sigma(ctx, sigma.data()))) return e;
     if ((e = crt_forward(ctx))) return e;
     std::vector<double> volt(2 * (size_t)nc), sigmaa(2 * (size_t)nc);

@@ -93,8 +93,6 @@ class Grid:
 
     def validate(self) -> None:
         """Band and ordering assumptions of the path (``src/kompab.f90:98-102``)."""
-        if np.any(self.typ == 13):
-            raise ValueError("Dirichlet (type 13) boundaries are not supported on this path")
         groups = list(self.typ)
         area_seen_after_boundary = False
         seen_boundary = False
@@ -255,15 +253,16 @@ def read_volt(path: str):
 
 # ----------------------------------------------------------- synthetic grids
 def quad_grid(nx: int, nz: int, dx: float, dz: float, x0: float = 0.0,
-              elec_nodes: Optional[Sequence[int]] = None, mixed: bool = True) -> Grid:
+              elec_nodes: Optional[Sequence[int]] = None, mixed: bool = True, dirichlet: bool = False) -> Grid:
     """Regular quad grid in the conventions of the reference fixtures
     (``tests/minimal/Reference/grid/elem.dat.orig``): nodes row-major from the
     surface, type-8 quads counter-clockwise from bottom-left, type-12 no-flow
     surface left to right, type-11 mixed boundary clockwise (right side down,
     bottom right to left, left side up).  With ``mixed=False`` the sides and
     the bottom are no-flow (type 12) as well, so no system depends on the
-    source electrode (lbeta false, one factorisation per wavenumber).
-    Not yet CutMcK-ordered (mb = 0)."""
+    source electrode (lbeta false, one factorisation per wavenumber).  With
+    ``dirichlet=True`` the sides and the bottom are type-13 elements (zero potential,
+    src/randb.f90) instead.  Not yet CutMcK-ordered (mb = 0)."""
     mixed_flag = mixed
     n1 = nx + 1
     ix, iz = np.meshgrid(np.arange(n1), np.arange(nz + 1))
@@ -293,7 +292,9 @@ def quad_grid(nx: int, nz: int, dx: float, dz: float, x0: float = 0.0,
     nrel[len(cells): len(cells) + len(surf), :2] = np.array(surf)
     nrel[len(cells) + len(surf):, :2] = np.array(mixed)
     enr = np.array([] if elec_nodes is None else elec_nodes, np.int32)
-    if mixed_flag:
+    if dirichlet:
+        typ, nelanz, selanz = [8, 12, 13], [len(cells), len(surf), len(mixed)], [4, 2, 2]
+    elif mixed_flag:
         typ, nelanz, selanz = [8, 12, 11], [len(cells), len(surf), len(mixed)], [4, 2, 2]
     else:
         typ, nelanz, selanz = [8, 12], [len(cells), len(surf) + len(mixed)], [4, 2]

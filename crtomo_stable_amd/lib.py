@@ -147,6 +147,13 @@ class Context:
         self.kwnanz = n.value
         return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
 
+    def set_boundary_values(self, nodes, values):
+        """boundary.dat (src/rrandb.f90): 1-based nodes and complex values; zero values
+        make them Dirichlet nodes, non-zero values are refused by forward() (903)."""
+        nd = np.ascontiguousarray(nodes, np.int32)
+        vals = np.ascontiguousarray(values, np.complex128)
+        _chk(self.lib.crt_set_boundary_values(self._h, nd.size, _ptr(nd), _ptr(vals)), "crt_set_boundary_values")
+
     def set_wavenumbers(self, kwn, kwnwi):
         kwn = np.ascontiguousarray(kwn, np.float64); kwi = np.ascontiguousarray(kwnwi, np.float64)
         _chk(self.lib.crt_set_wavenumbers(self._h, kwn.size, _ptr(kwn), _ptr(kwi)), "crt_set_wavenumbers")

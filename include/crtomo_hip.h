@@ -14,7 +14,8 @@
  *     reference modules: elbg(elanz,npk,kwnanz), kg(relanz,eanz,kwnanz),
  *     kpot(sanz,eanz,kwnanz), hpot(sanz,eanz), sens(nanz,manz).
  * Every function returns 0 or the reference's errnr code (src/error.txt):
- *   19 band width exceeded, 26 element geometry, 27 zero diagonal,
+ *   19 band width exceeded, 26 element geometry, 27 zero diagonal, 30 boundary
+ *   node beyond sanz, 101 mixed and Dirichlet boundaries together,
  *   28 non-positive pivot, 35 boundary geometry, 73 duplicate electrodes,
  *   82 zero voltage, 97 allocation; 900+ are library codes (900 HIP error,
  *   901 call order, 902 argument, 903 unsupported configuration, 904 a solve larger
@@ -55,7 +56,20 @@ int crt_rwaven(crt_ctx *ctx, int *kwnanz, double *kwn, double *kwnwi, double *am
 /* Or set them explicitly (2D: kwnanz = 1, kwn = 0, swrtr = 0 in the reference). */
 int crt_set_wavenumbers(crt_ctx *ctx, int kwnanz, const double *kwn, const double *kwnwi);
 
-/* Element matrices and mixed-boundary factors on the device (src/precal.f90:13). */
+/* Prescribed boundary values (boundary.dat, src/rrandb.f90; randb2 in the forward,
+ * src/randb2.f90:29-79): n nodes (1-based; errnr 30 beyond sanz) with complex values
+ * (interleaved; the DC inversion takes the real parts as randbdc2 does).  Zero values
+ * make the nodes Dirichlet nodes as the type-13 elements of the grid do (randb,
+ * src/randb.f90:40-84, which crt_set_grid reads from the element types): identity rows
+ * and columns, zero potential.  A source electrode on a Dirichlet node has the zero
+ * potential of its cleared right-hand side where kompab assembles its system and
+ * kompb's unit source elsewhere, as in the reference.  crt_forward returns 903 for
+ * non-zero values (crt_batch_create also for a source on a Dirichlet node).  n = 0
+ * removes them. */
+int crt_set_boundary_values(crt_ctx *ctx, int n, const int *nodes, const double *values);
+
+/* Element matrices and mixed-boundary factors on the device (src/precal.f90:13);
+ * errnr 101 for a grid with both mixed (type 11) and Dirichlet (type 13) boundaries. */
 int crt_precal(crt_ctx *ctx, int lsytop, int *lbeta);
 int crt_get_precal(crt_ctx *ctx, double *elbg, double *relbg, double *kg);
 

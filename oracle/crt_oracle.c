@@ -397,8 +397,7 @@ int orc_precal(const orc_grid *g, int kwnanz, const double *kwn, int lsytop,
             }
         }
     }
-    *lbeta = lb;
-    if (lrandb) return 13; /* Dirichlet type-13 boundaries are not part of this path */
+    *lbeta = lb;  /* lrandb: the forward loops apply randb / randdc (has_type13) */
     return 0;
 }
 
@@ -522,6 +521,89 @@ int orc_cholz(zdbl *a, zdbl *p, int n) {
 }
 
 /* forward loop over wavenumbers and electrodes (src/fem.f90:336-397) */
+/* ------------------------------------------------------- Dirichlet boundaries */
+/* Prescribed boundary values of boundary.dat (src/rrandb.f90: rwdanz, rwdnr, rwd; the
+ * DC twins use the real parts, rwddc), applied by randb2 / randbdc2 when n > 0. */
+static int g_rwdanz = 0;
+static const int *g_rwdnr = NULL;
+static const zdbl *g_rwd = NULL;
+
+void orc_set_boundary(int n, const int *nodes, const zdbl *values) {
+    g_rwdanz = n > 0 ? n : 0;
+    g_rwdnr = nodes;
+    g_rwd = values;
+}
+
+static int has_type13(const orc_grid *g) {
+    for (int t = 0; t < g->typanz; ++t)
+        if (g->typ[t] == 13) return 1;
+    return 0;
+}
+
+/* src/randb.f90:40-84: every node k of a type-13 element gets b(k) = 0, a(k,k) = 1 and
+ * the rest of its row and column zero.  T is zdbl (randb) or double (randdc). */
+#define RANDB_BODY(T)                                                                      \
+    const int sanz = g->sanz, mb = g->mb, m1 = mb + 1;                                     \
+    int iel = 0;                                                                           \
+    for (int t = 0; t < g->typanz; ++t) {                                                  \
+        const int ntyp = g->typ[t], nkel = g->selanz[t];                                   \
+        for (int j2 = 0; j2 < g->nelanz[t]; ++j2, ++iel) {                                 \
+            if (ntyp != 13) continue;                                                      \
+            for (int ir = 0; ir < nkel; ++ir) {                                            \
+                const int k = g->nrel[(size_t)iel * g->maxn + ir];                         \
+                b[k - 1] = (T)0.0;                                                         \
+                const int idk = k * m1;                                                    \
+                a[idk - 1] = (T)1.0;                                                       \
+                if (k != 1) {                                                              \
+                    const int ia = imax_(1, mb + 2 - k);                                   \
+                    for (int i = ia; i <= mb; ++i) a[idk + i - m1 - 1] = (T)0.0;           \
+                }                                                                          \
+                if (k == sanz) continue;                                                   \
+                const int ia = imax_(1, k - sanz + m1);                                    \
+                for (int i = ia; i <= mb; ++i) {                                           \
+                    const int jj = k - i + m1;                                             \
+                    a[(jj - 1) * m1 + i - 1] = (T)0.0;                                     \
+                }                                                                          \
+            }                                                                              \
+        }                                                                                  \
+    }
+
+void orc_randb(const orc_grid *g, zdbl *a, zdbl *b) { RANDB_BODY(zdbl) }
+void orc_randdc(const orc_grid *g, double *a, double *b) { RANDB_BODY(double) }
+
+/* src/randb2.f90:29-79 (randbdc2.f90 with the real values): node k of the list takes
+ * the value w: b(k) = -w, a(k,k) = 1, and every other entry of row and column k moves
+ * w * a(j,k) into b(j) before it is zeroed. */
+#define RANDB2_BODY(T, VAL)                                                                \
+    const int sanz = g->sanz, mb = g->mb, m1 = mb + 1;                                     \
+    for (int ir = 0; ir < g_rwdanz; ++ir) {                                                \
+        const int k = g_rwdnr[ir];                                                         \
+        const T w = VAL(g_rwd[ir]);                                                        \
+        b[k - 1] = -w;                                                                     \
+        const int idk = k * m1;                                                            \
+        a[idk - 1] = (T)1.0;                                                               \
+        if (k != 1) {                                                                      \
+            const int ia = imax_(1, mb + 2 - k);                                           \
+            for (int i = ia; i <= mb; ++i) {                                               \
+                const int jj = k + i - m1, ki = idk + i - m1;                              \
+                b[jj - 1] = b[jj - 1] + w * a[ki - 1];                                     \
+                a[ki - 1] = (T)0.0;                                                        \
+            }                                                                              \
+        }                                                                                  \
+        if (k == sanz) continue;                                                           \
+        const int ia = imax_(1, k - sanz + m1);                                            \
+        for (int i = ia; i <= mb; ++i) {                                                   \
+            const int jj = k - i + m1, ji = (jj - 1) * m1 + i;                             \
+            b[jj - 1] = b[jj - 1] + w * a[ji - 1];                                         \
+            a[ji - 1] = (T)0.0;                                                            \
+        }                                                                                  \
+    }
+
+#define ZVAL(z) (z)
+#define RVAL(z) creal(z)
+void orc_randb2(const orc_grid *g, zdbl *a, zdbl *b) { RANDB2_BODY(zdbl, ZVAL) }
+void orc_randbdc2(const orc_grid *g, double *a, double *b) { RANDB2_BODY(double, RVAL) }
+
 int orc_forward(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
                 const double *relbg, const double *kg, int lbeta, zdbl *kpot,
                 int nsys, const int *sysl, const int *sysk) {
@@ -539,6 +621,9 @@ int orc_forward(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *
         else { k = s / eanz + 1; l = s % eanz + 1; }
         if (lbeta || l == 1 || k != lastk) {
             err = orc_kompab(g, sigma, elbg, relbg, kg, kwnanz, l, k, a, b);
+            /* src/fem.f90:362-363 */
+            if (!err && has_type13(g)) orc_randb(g, a, b);
+            if (!err && g_rwdanz) orc_randb2(g, a, b);
             if (!err) err = orc_scalab(sanz, mb, a, b, fak);
             if (!err) err = orc_chol(sanz, mb, a);
             if (!err && !lbeta && l != 1) orc_kompb(sanz, g->enr[l - 1], b, fak);
@@ -1039,6 +1124,9 @@ int orc_forward_dc(const orc_grid *g, const zdbl *sigma, int kwnanz, const doubl
         for (int l = 1; l <= eanz && !err; ++l) {
             if (lbeta || l == 1) {
                 err = orc_kompadc(g, sigma, elbg, relbg, kg, l, k, a, b);
+                /* src/inv.f90:379-380 */
+                if (!err && has_type13(g)) orc_randdc(g, a, b);
+                if (!err && g_rwdanz) orc_randbdc2(g, a, b);
                 if (!err) err = orc_scaldc(sanz, mb, a, b, fak);
                 if (!err) err = orc_choldc(sanz, mb, a);
             } else {

@@ -73,6 +73,15 @@ int orc_cholz(zdbl *a, zdbl *p, int n);
 /* Whole forward loop over (k, l) (src/fem.f90:328-397, src/inv.f90:415-484).
  * Systems with index list: if nsys>0 only (l,k) pairs sysl[i], sysk[i] (1-based) are
  * solved (used to time a bounded sample); kpot columns of other systems untouched. */
+/* Dirichlet boundaries: type-13 elements (src/randb.f90, src/randdc.f90) and the
+ * prescribed values of boundary.dat (src/randb2.f90, src/randbdc2.f90); the forward
+ * loops apply them after kompab / kompadc as src/fem.f90:362-363 and
+ * src/inv.f90:379-380 do.  orc_set_boundary keeps the caller's arrays (n = 0: none). */
+void orc_set_boundary(int n, const int *nodes, const zdbl *values);
+void orc_randb(const orc_grid *g, zdbl *a, zdbl *b);
+void orc_randdc(const orc_grid *g, double *a, double *b);
+void orc_randb2(const orc_grid *g, zdbl *a, zdbl *b);
+void orc_randbdc2(const orc_grid *g, double *a, double *b);
 int orc_forward(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
                 const double *relbg, const double *kg, int lbeta, zdbl *kpot,
                 int nsys, const int *sysl, const int *sysk);

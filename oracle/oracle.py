@@ -144,6 +144,23 @@ def forward(grid, sigma, elbg, relbg, kg, lbeta, systems=None):
     return kpot
 
 
+_boundary_keep = None
+
+
+def set_boundary(nodes=None, values=None):
+    """Prescribed boundary values for the oracle's forward loops (randb2 / randbdc2);
+    None or empty removes them."""
+    global _boundary_keep
+    if nodes is None or len(nodes) == 0:
+        _boundary_keep = None
+        lib().orc_set_boundary(0, None, None)
+        return
+    nd = np.ascontiguousarray(nodes, np.int32)
+    vals = np.ascontiguousarray(values, np.complex128)
+    _boundary_keep = (nd, vals)
+    lib().orc_set_boundary(nd.size, ptr(nd), ptr(vals))
+
+
 def rtrafo(kpot, kwnwi):
     K, E, S = kpot.shape
     hpot = np.zeros((E, S), np.complex128)
@@ -402,6 +419,14 @@ class RefSession:
                             ptr(grid.selanz), grid.maxn, ptr(grid.nrel), ptr(grid.x), ptr(grid.y),
                             ptr(rnr), grid.eanz, ptr(grid.enr)), "ref_set_grid")
         _chk(r.ref_set_data(survey.nanz, ptr(survey.strnr), ptr(survey.vnr)), "ref_set_data")
+
+    def set_boundary(self, nodes=None, values=None):
+        """boundary.dat values (randb2 / randbdc2 in the reference's forward loops)."""
+        nd = np.ascontiguousarray([] if nodes is None else nodes, np.int32)
+        vals = np.ascontiguousarray([] if values is None else values, np.complex128)
+        self._bnd = (nd, vals)
+        _chk(self.r.ref_set_boundary(nd.size, ptr(nd if nd.size else np.zeros(1, np.int32)),
+                                     ptr(vals if vals.size else np.zeros(1, np.complex128))), "ref_set_boundary")
 
     def rwaven(self):
         kwn = np.zeros(30); kwi = np.zeros(30); n = _i(); amin = _d(); amax = _d()

@@ -25,6 +25,7 @@ module crt_ref_harness
   use invmod
   use cjgmod
   use bsmatm_mod
+  use randbmod
   implicit none
 
 contains
@@ -108,6 +109,26 @@ contains
     ref_set_data = 0
   end function ref_set_data
 
+  ! boundary.dat values (src/rrandb.f90: rwdanz, rwdnr, rwd and the DC rwddc = DBLE(rwd));
+  ! n = 0 switches the prescribed values off (lrandb2)
+  integer(c_int) function ref_set_boundary(n, nodes, vals) bind(C, name='ref_set_boundary')
+    integer(c_int), value :: n
+    integer(c_int), intent(in) :: nodes(*)
+    complex(c_double_complex), intent(in) :: vals(*)
+    if (allocated(rwdnr)) deallocate(rwdnr)
+    if (allocated(rwd)) deallocate(rwd)
+    if (allocated(rwddc)) deallocate(rwddc)
+    rwdanz = n
+    lrandb2 = n > 0
+    allocate(rwdnr(max(n, 1)), rwd(max(n, 1)), rwddc(max(n, 1)))
+    if (n > 0) then
+       rwdnr(1:n) = nodes(1:n)
+       rwd(1:n) = vals(1:n)
+       rwddc(1:n) = dble(vals(1:n))
+    end if
+    ref_set_boundary = 0
+  end function ref_set_boundary
+
   integer(c_int) function ref_rwaven(kwnanz_, kwn_, kwnwi_, amin_, amax_) bind(C, name='ref_rwaven')
     integer(c_int), intent(out) :: kwnanz_
     real(c_double), intent(out) :: kwn_(30), kwnwi_(30), amin_, amax_
@@ -151,7 +172,7 @@ contains
     complex(kind(0d0)), allocatable :: la(:), lb(:), lpot(:)
     real(kind(0d0)), allocatable :: lfak(:)
     integer :: j, l, k, n
-    external kompab, scalab, chol, kompb, vre
+    external kompab, randb, randb2, scalab, chol, kompb, vre
     if (allocated(sigma)) deallocate(sigma)
     allocate(sigma(elanz))
     sigma = sigma_(1:elanz)
@@ -162,12 +183,14 @@ contains
     !$OMP PARALLEL DEFAULT (none) &
     !$OMP FIRSTPRIVATE (lfak,lpot,la,lb) &
     !$OMP PRIVATE (j,l,k) &
-    !$OMP SHARED (kwnanz,eanz,lbeta,sanz,kpot)
+    !$OMP SHARED (kwnanz,eanz,lbeta,lrandb,lrandb2,sanz,kpot)
     !$OMP DO
     do k = 1, kwnanz
        do l = 1, eanz
           if (lbeta .or. l == 1) then
              call kompab(l, k, la, lb)
+             if (lrandb) call randb(la, lb)
+             if (lrandb2) call randb2(la, lb)
              call scalab(la, lb, lfak)
              call chol(la)
           else
@@ -301,7 +324,7 @@ contains
     real(kind(0d0)), allocatable :: la(:), lb(:), lfak(:)
     complex(kind(0d0)), allocatable :: lpot(:)
     integer :: j, l, k, n
-    external kompadc, scaldc, choldc, kompbdc, vredc
+    external kompadc, randdc, randbdc2, scaldc, choldc, kompbdc, vredc
     if (allocated(sigma)) deallocate(sigma)
     allocate(sigma(elanz))
     sigma = sigma_(1:elanz)
@@ -313,6 +336,8 @@ contains
        do l = 1, eanz
           if (lbeta .or. l == 1) then
              call kompadc(l, k, la, lb)
+             if (lrandb) call randdc(la, lb)
+             if (lrandb2) call randbdc2(la, lb)
              call scaldc(la, lb, lfak)
              call choldc(la)
           else

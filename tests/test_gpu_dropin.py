@@ -5,7 +5,7 @@ Tolerance 1e-10 relative (FP64); error code 28 on a vanishing pivot."""
 import numpy as np
 import pytest
 
-from crtomo_stable_amd import configs
+from crtomo_stable_amd import configs, grid as G
 
 pytestmark = pytest.mark.gpu
 
@@ -188,3 +188,67 @@ def test_tomodir_crmod_pot_sens_files(ctx, tmp_path, name):
         s = jr.sum()
         assert abs(head[0] - abs(s)) <= 1e-6 * abs(s)
         assert abs(head[1] - 1e3 * np.angle(s)) <= 1e-3 * max(1.0, abs(1e3 * np.angle(s)))
+
+
+def _dirichlet_rb_tomodir(tmp):
+    """The reference's examples/DirichletRB/Modeling tomodir (3 x 1 quads, type-13
+    sides and bottom, 2D, boundary.dat of zero values), measurements switched on."""
+    import os
+    import shutil
+    src = os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures", "dirichlet_rb")
+    for d, files in (("grid", ("elem.dat", "elec.dat")), ("rho", ("rho.dat",)), ("config", ("config.dat",)),
+                     ("exe", ("boundary.dat", "crmod.cfg")), ("mod", ())):
+        os.makedirs(os.path.join(tmp, d), exist_ok=True)
+        for f in files:
+            shutil.copyfile(os.path.join(src, f), os.path.join(tmp, d, f))
+    cfg = os.path.join(tmp, "exe", "crmod.cfg")
+    lines = open(cfg).read().split("\n")
+    lines[7] = "T                ! measurements ?"
+    open(cfg, "w").write("\n".join(lines))
+    return cfg, src
+
+
+def test_tomodir_crmod_2d_dirichlet(ctx, oracle_lib, tmp_path):
+    """crmod.cfg lines 12-17 (src/fem.f90:177-193): 2D (swrtr = 0: one wavenumber k = 0,
+    kpot is the potential, src/fem.f90:241-250,393) with type-13 Dirichlet boundaries and
+    boundary.dat values on the reference's DirichletRB example; volt.dat against the
+    oracle's bvolt of the same 2D forward (REAL*4 of the file)."""
+    import os
+    O = oracle_lib
+    cfg, src = _dirichlet_rb_tomodir(str(tmp_path))
+    ctx.tomodir_crmod(cfg)
+    survey, mag, pha = G.read_volt(os.path.join(str(tmp_path), "mod", "volt.dat"))
+    g = G.read_elem(os.path.join(src, "elem.dat"))
+    g = G.fix_rnr(G.replace(g, enr=G.read_elec(os.path.join(src, "elec.dat"))))
+    sig = G.read_rho(os.path.join(src, "rho.dat"))
+    kwn, kwi = np.zeros(1), np.full(1, np.pi)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    bn = np.array([5, 6, 7, 8], np.int32)
+    try:
+        O.set_boundary(bn, np.zeros(4, np.complex128))
+        kp = O.forward(g, sig, elbg, relbg, kg, lb)
+    finally:
+        O.set_boundary()
+    hp = O.rtrafo(kp, kwi)
+    assert np.array_equal(hp, kp[0])  # the weight pi makes the back-transform kpot itself
+    _, sa = O.bvolt(g, survey, hp)
+    assert np.allclose(mag, np.abs(sa), rtol=2e-7, atol=0)
+    print(f"[tomodir 2D dirichlet] |V| {np.abs(sa)}", flush=True)
+
+
+@pytest.mark.parametrize("line,text", [(13, "T ! fictitious sink ?"), (11, "T ! another dataset ?"),
+                                       (17, "   4")])
+def test_tomodir_crmod_refuses_unsupported_switches(ctx, tmp_path, line, text):
+    """The switches this forward does not run (lsink, lagain, mswitch lsr) return 903
+    instead of being ignored."""
+    from crtomo_stable_amd.lib import CRTError
+    from test_lib_cpu import _tomodir
+    cfg, _ = _tomodir(str(tmp_path), "minimal")
+    lines = open(cfg).read().rstrip("\n").split("\n")
+    while len(lines) <= line:
+        lines.append("")
+    lines[line] = text
+    open(cfg, "w").write("\n".join(lines) + "\n")
+    with pytest.raises(CRTError) as ei:
+        ctx.tomodir_crmod(cfg)
+    assert ei.value.code == 903

@@ -569,3 +569,76 @@ def test_oracle_dc_against_mathematics(oracle_lib):
         y = np.linalg.solve(B, (bvec.real if kind == 1 else bvec.imag))
         expect = cgfac * y * (1.0 if kind == 1 else 1j)
         assert ncg < 5000 and rel(dpar, expect) < 1e-9
+
+
+# ------------------------------------------------------- Dirichlet boundaries
+def _bnd_case(name):
+    if name == "type13":
+        return configs.small_quad(dirichlet=True), None, None
+    w = configs.small_quad(mixed=(name == "mixed_values"))
+    rng = np.random.default_rng(3)
+    nodes = np.arange(w.grid.sanz - 20, w.grid.sanz, dtype=np.int32) + 1
+    if name == "sources_on_dirichlet":
+        nodes = np.concatenate([w.grid.enr[:2], nodes]).astype(np.int32)
+        return w, nodes, np.zeros(nodes.size, np.complex128)
+    vals = rng.standard_normal(nodes.size) + 1j * rng.standard_normal(nodes.size)
+    return w, nodes, vals
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref",
+                                                    "libcrtomo_ref.so")), reason="reference build absent")
+@pytest.mark.parametrize("name", ["type13", "noflow_values", "mixed_values", "sources_on_dirichlet"])
+def test_oracle_dirichlet_vs_reference(oracle_lib, name):
+    """randb / randdc (type-13 elements) and randb2 / randbdc2 (boundary.dat values) in
+    the forward loops (src/fem.f90:362-363, src/inv.f90:379-380) against the reference's
+    own routines: complex 1e-12, DC bit for bit.  Mathematics: a Dirichlet node carries
+    exactly zero potential (randb) or, in every system that kompab assembles, exactly its
+    prescribed value (randb2; the other sources of a no-flow grid reuse the factor with
+    kompb's fresh right-hand side, which drops the values, as the reference does)."""
+    O = oracle_lib
+    w, nodes, vals = _bnd_case(name)
+    g = w.grid
+    kwn, kwi, _, _ = O.rwaven(g, w.survey)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    R = O.RefSession(g, w.survey)
+    R.rwaven()
+    R.precal()
+    try:
+        O.set_boundary(nodes, vals)
+        R.set_boundary(nodes, vals)
+        kp = O.forward(g, w.sigma, elbg, relbg, kg, lb)
+        kd = O.forward_dc(g, w.sigma, elbg, relbg, kg, lb)
+        assert rel(kp, R.forward(w.sigma, nthreads=2)) <= 1e-12
+        assert np.array_equal(kd, R.forward_dc(w.sigma))
+    finally:
+        O.set_boundary()
+        R.set_boundary()
+    if nodes is None:
+        d = np.zeros(g.sanz, bool)
+        iel = 0
+        for t, n, k in zip(g.typ, g.nelanz, g.selanz):
+            if t == 13:
+                d[g.nrel[iel:iel + n, :k].ravel() - 1] = True
+            iel += n
+        assert d.any() and np.all(kp[:, :, d] == 0.0) and np.all(kd[:, :, d] == 0.0)
+    else:
+        assembled = kp if lb else kp[:, :1]
+        assert np.allclose(assembled[:, :, nodes - 1], vals, rtol=0, atol=1e-14 * np.abs(vals).max())
+        assert np.allclose(kd[:, :1 if not lb else None, nodes - 1], vals.real, rtol=0, atol=1e-14)
+
+
+def test_oracle_mixed_and_dirichlet_refused(oracle_lib):
+    """precal stops with errnr 101 when a grid has mixed (11) and Dirichlet (13)
+    boundary elements (src/precal.f90:102-106, 130-135)."""
+    O = oracle_lib
+    g = _mixed_and_dirichlet()
+    kwn, _, _, _ = O.rwaven(g, configs.small_quad().survey)
+    with pytest.raises(RuntimeError, match="errnr 101"):
+        O.precal(g, kwn)
+
+
+def _mixed_and_dirichlet():
+    g = configs.small_quad().grid
+    n11 = int(g.nelanz[-1])
+    return G.replace(g, typ=np.array(list(g.typ) + [13]), nelanz=np.array(list(g.nelanz[:-1]) + [n11 // 2, n11 - n11 // 2]),
+                     selanz=np.array(list(g.selanz) + [2]))
