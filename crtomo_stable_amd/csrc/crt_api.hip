@@ -88,6 +88,7 @@ int crt_launch_bsmatmreg(hipStream_t, int, int, int, int, const int *, const dou
 int crt_launch_bsmatmtri(hipStream_t, int, int, const int *, const int *, const double *, const double *, const int *,
                          double, double, double *);
 size_t crt_blam0_scratch(int, int, int *, int *);
+int crt_launch_besp_pairs(hipStream_t, int, const double *, const double *, double *, int *);
 int crt_launch_blam0(hipStream_t, int, int, int, const zd *, const double *, zd *, int, int, zd *, double *, double,
                      double *);
 
@@ -2043,6 +2044,87 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
         }
     }
     return e;
+}
+
+// besp_elem (src/besp_elem.f90:1-97): grid statistics of the area-element centroids
+// (espx, espy as src/relem.f90:128-143 forms them: the sum of the node coordinates in
+// node order over the node count).  The O(elanz^2) pair scan runs on the device; the
+// neighbour distances abst (to the last edge neighbour of src/bnachbar.f90, as the
+// reference's loop leaves it), their extremes, mean, mean absolute deviation and median
+// (MDIAN1: sorted, mean of the middle pair for an even count) on the host.
+int crt_besp_elem(crt_ctx *c, double *stats, const char *gstat_path) {
+    if (!c || !stats) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    const int n = c->elanz, maxn = c->maxn;
+    if (n < 2) return CRT_ERR_ARG;
+    CRT_HIP(hipSetDevice(c->device));
+    std::vector<double> cx(n, 0.0), cy(n, 0.0);
+    for (int e = 0; e < n; ++e) {
+        const int nk = c->nkel[e];
+        for (int m = 0; m < nk; ++m) {
+            const int nd = c->nrel[(size_t)e * maxn + m] - 1;
+            cx[e] = cx[e] + c->x[nd];
+            cy[e] = cy[e] + c->y[nd];
+        }
+        cx[e] = cx[e] / nk;
+        cy[e] = cy[e] / nk;
+    }
+    DBuf<double> d_cx, d_cy, d_part;
+    int nb = 0;
+    int e = d_cx.upload(cx.data(), n, c->stream);
+    e = e ? e : d_cy.upload(cy.data(), n, c->stream);
+    e = e ? e : d_part.ensure((size_t)6 * ((n + 255) / 256));
+    e = e ? e : crt_launch_besp_pairs(c->stream, n, d_cx.p, d_cy.p, d_part.p, &nb);
+    if (e) return e;
+    std::vector<double> part((size_t)6 * nb);
+    CRT_HIP(c->copy_out(part.data(), d_part.p, part.size() * sizeof(double)));
+    double v[6] = {HUGE_VAL, 0.0, HUGE_VAL, 0.0, HUGE_VAL, 0.0};
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 6; ++q) v[q] = (q & 1) ? std::max(v[q], part[6 * b + q]) : std::min(v[q], part[6 * b + q]);
+    // the reference's running values start at 1e5 (minima) and 0 (maxima)
+    const double grid_min = std::min(1e5, std::sqrt(v[0])), grid_max = std::sqrt(v[1]);
+    // grid_minx = MAX(MIN(grid_minx, ax), grid_min) per pair ends at the larger of the
+    // smallest |dx| and the final grid_min (grid_min never grows); the same for y
+    const double grid_minx = std::max(std::min(1e5, v[2]), grid_min), grid_maxx = v[3];
+    const double grid_miny = std::max(std::min(1e5, v[4]), grid_min), grid_maxy = v[5];
+    // neighbour distances
+    std::vector<int> nb_((size_t)n * (maxn + 1));
+    if ((e = crt_bnachbar(n, maxn, c->nrel.data(), nb_.data()))) return e;
+    std::vector<double> abst(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < maxn; ++j) {
+            const int k = nb_[(size_t)j * n + i];
+            if (k == 0) continue;
+            const double dx = cx[i] - cx[k - 1], dy = cy[i] - cy[k - 1];
+            abst[i] = std::sqrt(dx * dx + dy * dy);
+        }
+    double esp_max = abst[0], esp_min = abst[0], sum = 0.0;
+    for (double a : abst) { esp_max = std::max(esp_max, a); esp_min = std::min(esp_min, a); sum += a; }
+    const double esp_mit = sum / n;
+    double sd = 0.0;
+    for (double a : abst) sd += std::sqrt((a - esp_mit) * (a - esp_mit));
+    const double esp_std = sd / std::max(1, n - 1);
+    std::vector<double> srt(abst);
+    std::sort(srt.begin(), srt.end());
+    const int n2 = n / 2;
+    const double esp_med = (2 * n2 == n) ? 0.5 * (srt[n2 - 1] + srt[n2]) : srt[n2];
+    const double out[11] = {esp_min, esp_max, esp_mit, esp_med, esp_std, grid_min, grid_max,
+                            grid_minx, grid_maxx, grid_miny, grid_maxy};
+    std::memcpy(stats, out, sizeof(out));
+    if (gstat_path) {
+        // inv.gstat as src/besp_elem.f90:74-93 writes it
+        FILE *f = fopen(gstat_path, "w");
+        if (!f) return 1;
+        fprintf(f, "Grid statistics:\n\n");
+        fprintf(f, "%20sGridcells:\t%10d\n", "", n);
+        fprintf(f, "%20sESP Min/Max:\t%10.4f%10.4f\n", "", esp_min, esp_max);
+        fprintf(f, "%20sGRID Min/Max:\t%10.4f%10.4f\n", "", grid_min, grid_max);
+        fprintf(f, "%20sGRID-x Min/Max:\t%10.4f%10.4f\n", "", grid_minx, grid_maxx);
+        fprintf(f, "%20sGRID-y Min/Max:\t%10.4f%10.4f\n", "", grid_miny, grid_maxy);
+        fprintf(f, "%20sMean/Median/Var:\t%10.4f%10.4f%10.4f\n", "", esp_mit, esp_med, esp_std);
+        if (fclose(f) != 0) return 4;
+    }
+    return 0;
 }
 
 int crt_blam0(crt_ctx *c, int fpi, const double *wmatd, const int *wdfak, double alfx, double alfz,

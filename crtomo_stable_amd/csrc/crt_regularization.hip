@@ -240,3 +240,54 @@ int crt_launch_blam0(hipStream_t s, int nanz, int manz, int fpi, const zd *J, co
     hipLaunchKernelGGL(crt_blam0_sum_kernel, dim3(1), dim3(256), 0, s, manz, jtj, a2, out);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
+
+// ------------------------------------------------------------- grid statistics
+// besp_elem (src/besp_elem.f90:34-63): over every ordered pair of area-element
+// centroids the smallest and largest distance and the extremes of |dx| and |dy|.  The
+// square root is monotone and correctly rounded, so the extremes of the squared distance
+// fl(fl(dx dx) + fl(dy dy)) give those of the distance; every reduction is a min or a max
+// and therefore exact in any order.  Per block: partial [min d2, max d2, min |dx|,
+// max |dx|, min |dy|, max |dy|].
+__global__ void __launch_bounds__(256) crt_besp_pairs_kernel(int n, const double *__restrict__ cx,
+                                                             const double *__restrict__ cy, double *__restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double sx[256], sy[256];
+    __shared__ double red[6][256];
+    double v[6] = {HUGE_VAL, 0.0, HUGE_VAL, 0.0, HUGE_VAL, 0.0};
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double xi = i < n ? cx[i] : 0.0, yi = i < n ? cy[i] : 0.0;
+    for (int j0 = 0; j0 < n; j0 += 256) {
+        const int j = j0 + threadIdx.x;
+        sx[threadIdx.x] = j < n ? cx[j] : 0.0;
+        sy[threadIdx.x] = j < n ? cy[j] : 0.0;
+        __syncthreads();
+        if (i < n) {
+            const int jn = min(256, n - j0);
+            for (int t = 0; t < jn; ++t) {
+                if (j0 + t == i) continue;
+                const double ax = fabs(xi - sx[t]), ay = fabs(yi - sy[t]);
+                const double d2 = ax * ax + ay * ay;
+                v[0] = fmin(v[0], d2); v[1] = fmax(v[1], d2);
+                v[2] = fmin(v[2], ax); v[3] = fmax(v[3], ax);
+                v[4] = fmin(v[4], ay); v[5] = fmax(v[5], ay);
+            }
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 6; ++q)
+                red[q][threadIdx.x] = (q & 1) ? fmax(red[q][threadIdx.x], red[q][threadIdx.x + w])
+                                              : fmin(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+int crt_launch_besp_pairs(hipStream_t s, int n, const double *cx, const double *cy, double *part, int *nblocks) {
+    *nblocks = (n + 255) / 256;
+    hipLaunchKernelGGL(crt_besp_pairs_kernel, dim3(*nblocks), dim3(256), 0, s, n, cx, cy, part);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

@@ -97,6 +97,10 @@ def write_volt(path: str, strnr, vnr, sigmaa):
     _chk(lib.crt_write_volt(path.encode(), int(st.size), _ptr(st), _ptr(vn), _ptr(sa)), "crt_write_volt")
 
 
+BESP_NAMES = ("esp_min", "esp_max", "esp_mit", "esp_med", "esp_std", "grid_min", "grid_max", "grid_minx",
+              "grid_maxx", "grid_miny", "grid_maxy")
+
+
 class Context:
     """One device context holding grid, survey, element matrices, kpot and J."""
 
@@ -146,6 +150,15 @@ class Context:
              "crt_rwaven")
         self.kwnanz = n.value
         return kwn[: n.value].copy(), kwi[: n.value].copy(), amin.value, amax.value
+
+    def besp_elem(self, gstat_path=None):
+        """Grid statistics of besp_elem (src/besp_elem.f90): dict of esp_min, esp_max,
+        esp_mit, esp_med, esp_std, grid_min, grid_max, grid_minx, grid_maxx, grid_miny,
+        grid_maxy; ``gstat_path`` also writes inv.gstat."""
+        st = np.zeros(11)
+        p = None if gstat_path is None else gstat_path.encode()
+        _chk(self.lib.crt_besp_elem(self._h, _ptr(st), p), "crt_besp_elem")
+        return dict(zip(BESP_NAMES, st))
 
     def set_boundary_values(self, nodes, values):
         """boundary.dat (src/rrandb.f90): 1-based nodes and complex values; zero values

@@ -199,6 +199,13 @@ int crt_bsmatm(crt_ctx *ctx, int ltri, int nx, int nz, double alfx, double alfz,
  * last column = neighbour count. */
 int crt_bnachbar(int elanz, int maxn, const int *nrel, int *nachbar);
 
+/* Grid statistics of besp_elem (src/besp_elem.f90:1-97) for the grid of crt_set_grid:
+ * stats[11] = esp_min, esp_max, esp_mit, esp_med, esp_std (distances of each cell's
+ * centroid to its last edge neighbour), grid_min, grid_max, grid_minx, grid_maxx,
+ * grid_miny, grid_maxy (over all centroid pairs, the O(elanz^2) scan on the device).
+ * gstat_path (may be NULL) receives inv.gstat in the reference's format. */
+int crt_besp_elem(crt_ctx *ctx, double *stats, const char *gstat_path);
+
 /* Starting regularisation parameter (src/blam0.f90:120-175) from the Jacobian on
  * the device: lammax = 5 * 2/(alfx+alfz) * mean_j |(J^H W J 1)_j|, W = wmatd*wdfak;
  * fpi != 0 uses the real parts only (the lfpi branch).  The fixed-lambda branches

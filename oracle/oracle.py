@@ -377,6 +377,51 @@ def cg_matvec(p: CGProblem, cgfac, pvec):
     return ap, out
 
 
+def besp(grid):
+    """Restatement of besp_elem (src/besp_elem.f90:21-72) in numpy: centroids as
+    src/relem.f90:122-143 (sequential sums over the nodes), the distance to the last
+    edge neighbour per cell, and the pair extremes with the reference's running-value
+    rules (the minima start at 1e5; grid_minx / grid_miny never fall below the running
+    grid_min).  Pairs in blocks of rows."""
+    g = grid
+    n, maxn = g.elanz, g.maxn
+    cx = np.zeros(n); cy = np.zeros(n)
+    iel = 0
+    for t, ne, sel in zip(g.typ, g.nelanz, g.selanz):
+        if t < 10:
+            for m in range(sel):
+                idx = g.nrel[iel:iel + ne, m] - 1
+                cx[iel:iel + ne] = cx[iel:iel + ne] + g.x[idx]
+                cy[iel:iel + ne] = cy[iel:iel + ne] + g.y[idx]
+            cx[iel:iel + ne] /= sel
+            cy[iel:iel + ne] /= sel
+        iel += ne
+    cx, cy = cx[:n], cy[:n]
+    nb = bnachbar(g)  # (maxn + 1, elanz), last row the count
+    abst = np.zeros(n)
+    for j in range(maxn):
+        k = nb[j]
+        has = k > 0
+        d = np.sqrt((cx[has] - cx[k[has] - 1]) ** 2 + (cy[has] - cy[k[has] - 1]) ** 2)
+        abst[has] = d
+    gmin, gmax, mnx, mxx, mny, mxy = 1e5, 0.0, 1e5, 0.0, 1e5, 0.0
+    for i0 in range(0, n, 512):
+        ax = np.abs(cx[i0:i0 + 512, None] - cx[None, :])
+        ay = np.abs(cy[i0:i0 + 512, None] - cy[None, :])
+        ar = np.sqrt(ax ** 2 + ay ** 2)
+        off = np.ones_like(ar, bool)
+        rows = np.arange(i0, min(i0 + 512, n))
+        off[rows - i0, rows] = False
+        gmin = min(gmin, ar[off].min()); gmax = max(gmax, ar[off].max())
+        mnx = min(mnx, ax[off].min()); mxx = max(mxx, ax[off].max())
+        mny = min(mny, ay[off].min()); mxy = max(mxy, ay[off].max())
+    mit = abst.sum() / n
+    std = np.sum(np.sqrt((abst - mit) ** 2)) / max(1, n - 1)
+    srt = np.sort(abst)
+    med = 0.5 * (srt[n // 2 - 1] + srt[n // 2]) if n % 2 == 0 else srt[n // 2]
+    return np.array([abst.min(), abst.max(), mit, med, std, gmin, gmax, max(mnx, gmin), mxx, max(mny, gmin), mxy])
+
+
 def cutmck(grid):
     perm = np.zeros(grid.sanz, np.int32); st = _i()
     mb = lib().orc_cutmck(grid.sanz, grid.typanz, ptr(grid.nelanz), ptr(grid.selanz), grid.maxn,
@@ -427,6 +472,14 @@ class RefSession:
         self._bnd = (nd, vals)
         _chk(self.r.ref_set_boundary(nd.size, ptr(nd if nd.size else np.zeros(1, np.int32)),
                                      ptr(vals if vals.size else np.zeros(1, np.complex128))), "ref_set_boundary")
+
+    def besp(self):
+        """besp_elem's statistics (src/besp_elem.f90) in the order esp_min, esp_max,
+        esp_mit, esp_med, esp_std, grid_min, grid_max, grid_minx, grid_maxx, grid_miny,
+        grid_maxy (writes inv.gstat into the working directory)."""
+        st = np.zeros(11)
+        _chk(self.r.ref_besp(ptr(st)), "ref_besp")
+        return st
 
     def rwaven(self):
         kwn = np.zeros(30); kwi = np.zeros(30); n = _i(); amin = _d(); amax = _d()

@@ -409,6 +409,40 @@ contains
     ref_vre = 0
   end function ref_vre
 
+  ! besp_elem (src/besp_elem.f90) on the grid of ref_set_grid: centroids as
+  ! src/relem.f90:122-143 forms them, bnachbar, then the statistics (inv.gstat goes to
+  ! the working directory)
+  integer(c_int) function ref_besp(stats_) bind(C, name='ref_besp')
+    real(c_double), intent(out) :: stats_(11)
+    integer :: i, j, k, ifln, idum
+    external bnachbar, besp_elem
+    if (allocated(espx)) deallocate(espx, espy)
+    allocate(espx(elanz), espy(elanz))
+    espx = 0.; espy = 0.
+    idum = 0; ifln = 0
+    do i = 1, typanz
+       do j = 1, nelanz(i)
+          if (typ(i) < 10) then
+             ifln = ifln + 1
+             do k = 1, selanz(i)
+                espx(ifln) = espx(ifln) + sx(snr(nrel(idum + j, k)))
+                espy(ifln) = espy(ifln) + sy(snr(nrel(idum + j, k)))
+             end do
+             espx(ifln) = espx(ifln) / selanz(i)
+             espy(ifln) = espy(ifln) / selanz(i)
+          end if
+       end do
+       idum = idum + nelanz(i)
+    end do
+    if (allocated(nachbar)) deallocate(nachbar)
+    call bnachbar()
+    errnr = 0
+    call besp_elem()
+    stats_ = (/esp_min, esp_max, esp_mit, esp_med, esp_std, grid_min, grid_max, grid_minx, grid_maxx, &
+         grid_miny, grid_maxy/)
+    ref_besp = errnr
+  end function ref_besp
+
   ! regularisation matrix through the reference dispatcher bsmatm (src/bsmatm_mod.f90:62)
   integer(c_int) function ref_bsmatm(ltri_, nx_, nz_, alfx_, alfz_, smatm_, nachbar_) &
        bind(C, name='ref_bsmatm')

@@ -642,3 +642,22 @@ def _mixed_and_dirichlet():
     n11 = int(g.nelanz[-1])
     return G.replace(g, typ=np.array(list(g.typ) + [13]), nelanz=np.array(list(g.nelanz[:-1]) + [n11 // 2, n11 - n11 // 2]),
                      selanz=np.array(list(g.selanz) + [2]))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref",
+                                                    "libcrtomo_ref.so")), reason="reference build absent")
+@pytest.mark.parametrize("name", ["small_quad", "small_tri", "c0"])
+def test_oracle_besp_vs_reference(oracle_lib, name, tmp_path, monkeypatch):
+    """besp_elem's grid statistics: the numpy restatement against the reference's own
+    routine; the extremes and the median exactly, the mean and the mean deviation (sums
+    whose order the compilers choose) within 1e-14."""
+    O = oracle_lib
+    w = {"small_quad": configs.small_quad, "c0": configs.c0}.get(name, lambda: configs.c1(
+        scale=0.2, nelec=12, nmeas=60, seed=7))()
+    monkeypatch.chdir(tmp_path)
+    r = O.RefSession(w.grid, w.survey).besp()
+    o = O.besp(w.grid)
+    exact = [0, 1, 3, 5, 6, 7, 8, 9, 10]
+    assert np.array_equal(o[exact], r[exact])
+    assert np.all(np.abs(o[[2, 4]] - r[[2, 4]]) <= 1e-14 * np.maximum(np.abs(r[[2, 4]]), 1e-300))
+    assert os.path.exists(tmp_path / "inv.gstat")
