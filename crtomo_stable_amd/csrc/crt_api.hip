@@ -44,7 +44,7 @@ int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const
 int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                            const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
 // model uncertainty (crt_bmcm.hip)
-int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, const double *, const int *, double,
+int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, int, const double *, const int *, double,
                  int, double *, double *, double *, double *, double *, double *, int *, int *);
 int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, const zd *, zd *);
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
@@ -2006,7 +2006,9 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
                 double *ata, double *ata_reg, double *cov, double *res, double *cov2, double *diag) {
     if (!c || !wmatd || !wdfak) return CRT_ERR_ARG;
     if (!c->have_sens || c->ltri < 0) return CRT_ERR_STATE;
-    if (c->inv_kind != 0) return CRT_ERR_UNSUPPORTED;  // bata_dc / the FPI branch of bmcm_mod
+    // the DC inversion's bata reads sensdc (src/bmcm_mod.f90:197-206); the final phase
+    // improvement runs the complex branches with the caller's lam_cri and wmatdr
+    // (src/bmcm_mod.f90:56-83)
     CRT_HIP(hipSetDevice(c->device));
     int nanz = 0, manz = 0;
     if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
@@ -2026,7 +2028,8 @@ int crt_buncert(crt_ctx *c, double lam, const double *wmatd, const int *wdfak, i
     e = e ? e : d_info.ensure(1);
     e = e ? e : d_ipiv.ensure(manz);
     if (e) return e;
-    e = crt_bmcm_run(c->stream, nanz, manz, c->d_sens.p, c->d_w.p, c->ltri, c->rnx, c->rncol - 1, c->d_smatm.p,
+    e = crt_bmcm_run(c->stream, nanz, manz, c->d_sens.p, c->d_w.p, c->inv_kind == 1, c->ltri, c->rnx, c->rncol - 1,
+                     c->d_smatm.p,
                      c->d_nachbar.p, lam, gauss ? 1 : 0, d_ata.p, d_reg.p, d_cov.p, want_res ? d_res.p : nullptr,
                      want_cov2 ? d_cov2.p : nullptr, d_z.p, d_info.p, d_ipiv.p);
     const bool have_cov = e == 0;

@@ -1,6 +1,6 @@
-// crt_bmcm.hip -- linearised model uncertainty of the complex inversion on the
-// device (src/bmcm_mod.f90:49-175, buncert): dense manz x manz FP64 matrices
-//   ata     = Re(J^H W J)                                  (bata, :176-289)
+// crt_bmcm.hip -- linearised model uncertainty on the device (src/bmcm_mod.f90:49-175,
+// buncert): dense manz x manz FP64 matrices
+//   ata     = Re(J^H W J), J^T W J for the DC inversion    (bata, :176-289)
 //   ata_reg = ata + lam R, in the reference's write order  (bata_reg, :291-413)
 //   cov     = ata_reg^-1  (chold + linvd, :415-615, or the Gauss_dble branch)
 //   res     = cov ata                                      (bres, :617-688)
@@ -17,12 +17,14 @@
 namespace {
 
 // Z = sqrt(w) J in the real view of J: column k holds (re, im) of every
-// measurement, so Re(J^H W J) = Z^T Z
-__global__ void crt_bmcm_scale_kernel(size_t n, int nanz, const double *J, const double *w, double *Z) {
+// measurement, so Re(J^H W J) = Z^T Z.  real_only (the DC inversion's bata, which
+// reads the real sensdc, src/bmcm_mod.f90:197-206): the imaginary parts count as zero.
+__global__ void crt_bmcm_scale_kernel(size_t n, int nanz, const double *J, const double *w, int real_only,
+                                      double *Z) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
     const int i = (int)((idx % (2 * (size_t)nanz)) / 2);
-    Z[idx] = J[idx] * sqrt(w[i]);
+    Z[idx] = (real_only && (idx & 1)) ? 0.0 : J[idx] * sqrt(w[i]);
 }
 
 // a(r, c) = a(c, r) for r < c: the upper triangle from the lower one
@@ -125,8 +127,8 @@ __global__ void crt_bmcm_regn_kernel(int n, int maxn, double lam, const double *
 // The whole buncert on device buffers (each manz*manz, z 2*nanz*manz, info 1 int,
 // ipiv manz ints).  res/cov2 may be null when not asked.  Returns 0, 108 (the
 // reference's errnr) for a matrix the factorisation rejects, or a library code.
-int crt_bmcm_run(hipStream_t st, int nanz, int manz, const zd *J, const double *w, int ltri, int nx, int maxn,
-                 const double *smatm, const int *nachbar, double lam, int gauss, double *ata, double *reg,
+int crt_bmcm_run(hipStream_t st, int nanz, int manz, const zd *J, const double *w, int real_only, int ltri, int nx,
+                 int maxn, const double *smatm, const int *nachbar, double lam, int gauss, double *ata, double *reg,
                  double *cov, double *res, double *cov2, double *z, int *info, int *ipiv) {
     if (nanz <= 0 || manz <= 0) return CRT_ERR_ARG;
     if ((size_t)manz * manz > (size_t)INT32_MAX || 2 * (size_t)nanz * manz > (size_t)INT32_MAX)
@@ -134,7 +136,7 @@ int crt_bmcm_run(hipStream_t st, int nanz, int manz, const zd *J, const double *
     const int n = manz;
     const size_t nz = 2 * (size_t)nanz * manz;
     hipLaunchKernelGGL(crt_bmcm_scale_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, nz, nanz,
-                       reinterpret_cast<const double *>(J), w, z);
+                       reinterpret_cast<const double *>(J), w, real_only, z);
     CRT_HIP(hipGetLastError());
     rocblas_handle h;
     if (rocblas_create_handle(&h) != rocblas_status_success) return CRT_ERR_HIP;

@@ -85,7 +85,9 @@ int crt_set_sigma(crt_ctx *ctx, const double *sigma);
  * improvement (lfpi).  For 1 and 2, crt_update/crt_cjg run cjggdc (src/cg_mod.f90:106)
  * on the real Jacobian from DBLE(bvec) (DC) or DIMAG(bvec) (FPI), and crt_update stores
  * dpar = i*dpar for FPI (src/update.f90:302-304); crt_blam0 takes the sensdc branch for 1.
- * Not for 1/2: crt_buncert, bvolt/bsens (mode 0, DC), the FP32 and FSAI/learned CG
+ * crt_buncert runs bata on the real Jacobian for 1 (src/bmcm_mod.f90:197-206) and the
+ * complex branches for 2 (the caller passes lam_cri and wmatdr, :56-83).
+ * Not for 1/2: bvolt/bsens (mode 0, DC), the FP32 and FSAI/learned CG
  * forms (903).  A change of kind discards sigma and its results:
  * call crt_set_sigma after it. */
 int crt_set_inversion_kind(crt_ctx *ctx, int kind);

@@ -9,7 +9,8 @@ against the oracle's DC twins, which match the reference build bit for bit
   DBLE(sens) from DIMAG(bvec) (FPI): cgfac and bvec 1e-12, dpar 1e-10 after 3
   iterations and for the converged solve (eps 1e-30), where CG rounding does not
   decide the result (the complex CG tests explain the bound);
-* blam0's ldc branch: 1e-12.
+* blam0's ldc branch: 1e-12; buncert's DC branch (bata on sensdc) as the complex
+  buncert tests hold it.
 """
 import numpy as np
 import pytest
@@ -146,8 +147,17 @@ def test_dc_fpi_update(ctx, oracle_lib, kind, ltri, ncgmax, eps):
                                            P["sa"], P["par"], P["smatm"], ltri, w.grid.nx, lam, eps, ncgmax,
                                            nachbar=P["nachbar"], maxn=w.grid.maxn), True, 1.0, 1.0)) <= \
                 1e-12 * ctx.blam0(P["wmatd"], P["wdfak"])
-            with pytest.raises(CRTError):
-                ctx.buncert(lam, P["wmatd"], P["wdfak"])
+            # buncert of the DC inversion: bata on the real sensdc (src/bmcm_mod.f90:197-206),
+            # the rest as the complex branches; the oracle's bata on sensdc with zero
+            # imaginary parts forms the same products in the same order
+            from test_gpu_bmcm import _check
+            cpd = O.CGProblem(P["sensdc"].astype(np.complex128), P["wmatd"], P["wdfak"], P["dat"], P["sa"],
+                              P["par"], P["smatm"], ltri, w.grid.nx, lam, eps, ncgmax, nachbar=P["nachbar"],
+                              maxn=w.grid.maxn)
+            err_o, m = O.buncert(cpd, gauss=True)
+            err_d, d = ctx.buncert(lam, P["wmatd"], P["wdfak"], gauss=True)
+            assert err_o == 0 and err_d == 0
+            _check(d, m, True)
     finally:
         ctx.set_inversion_kind(0)
 
