@@ -2204,109 +2204,98 @@ int crt_get_times(crt_ctx *c, double *ms, int n) {
     return 0;
 }
 
-// Cuthill-McKee renumbering with minimal-degree start nodes; the start giving
-// the smallest band width wins (cutmckee/cutmck.f:126-300).
+// Cuthill-McKee renumbering (cutmckee/cutmck.f:126-300), restated:
+//   * node graph of every element's node pairs, neighbours kept in the order the pairs
+//     first appear (at most 500 per node, as the reference's graph array);
+//   * start candidates: the nodes of the smallest degree in node order, the degree
+//     raised until 100 candidates are found or it passes elanz/10; the last candidate
+//     found is not tried (the reference counts one past its list);
+//   * from each start a breadth-first numbering in which every node hands out its
+//     unnumbered neighbours by rising degree, equal degrees latest-first;
+//   * the first start with the smallest band width max |new(i) - new(j)| wins.
+// perm[old - 1] = new number, 1-based.
+namespace {
+
+struct NodeGraph {
+    std::vector<std::vector<int>> nb;  // 1-based neighbours in first-appearance order
+    int add_pair(int a, int b) {
+        auto &la = nb[a - 1];
+        if (std::find(la.begin(), la.end(), b) != la.end()) return 0;
+        if (la.size() >= 500 || nb[b - 1].size() >= 500) return CRT_ERR_UNSUPPORTED;
+        la.push_back(b);
+        nb[b - 1].push_back(a);
+        return 0;
+    }
+    int degree(int v) const { return (int)nb[v - 1].size(); }
+};
+
+// breadth-first numbering from `start`; false when the graph is not connected
+bool cm_number(const NodeGraph &g, int start, std::vector<int> &order, std::vector<int> &newnum) {
+    const int n = (int)g.nb.size();
+    order.clear();
+    std::fill(newnum.begin(), newnum.end(), 0);
+    order.push_back(start);
+    newnum[start - 1] = 1;
+    std::vector<std::pair<int, int>> cand;  // (degree, -position)
+    for (size_t head = 0; head < order.size(); ++head) {
+        const auto &lst = g.nb[order[head] - 1];
+        cand.clear();
+        for (int pos = 0; pos < (int)lst.size(); ++pos)
+            if (!newnum[lst[pos] - 1]) cand.emplace_back(g.degree(lst[pos]), -pos);
+        std::sort(cand.begin(), cand.end());
+        for (const auto &cp : cand) {
+            const int v = lst[-cp.second];
+            order.push_back(v);
+            newnum[v - 1] = (int)order.size();
+        }
+    }
+    return (int)order.size() == n;
+}
+
+}  // namespace
+
 int crt_cutmck(int sanz, int typanz, const int *nelanz, const int *selanz, int maxn, const int *nrel, int elanz,
                int *perm, int *mb_out) {
     if (sanz <= 0 || !perm) return CRT_ERR_ARG;
-    const int GRMAX = 500, SPMAX = 100;
-    // adjacency lists in insertion order (the order decides ties)
-    std::vector<int> graph((size_t)GRMAX * sanz, 0), grad(sanz, 0);
-    auto G = [&](int k, int n) -> int & { return graph[(size_t)(n - 1) * GRMAX + (k - 1)]; };
-    int idum = 0;
-    for (int l = 0; l < typanz; ++l) {
-        for (int m = 1; m <= nelanz[l]; ++m) {
-            const int *el = nrel + (size_t)(idum + m - 1) * maxn;
-            for (int i = 1; i <= selanz[l] - 1; ++i) {
-                int nzp = el[i - 1];
-                for (int j = i + 1; j <= selanz[l]; ++j) {
-                    int nnp = el[j - 1];
-                    if (nzp < 1 || nzp > sanz || nnp < 1 || nnp > sanz) return CRT_ERR_ARG;
-                    bool dup = false;
-                    for (int k = 1; k <= GRMAX; ++k) {
-                        if (G(k, nzp) == nnp) { dup = true; break; }
-                        if (G(k, nzp) > 0) continue;
-                        G(k, nzp) = nnp;
-                        grad[nzp - 1]++;
-                        break;
-                    }
-                    if (dup) continue;
-                    grad[nnp - 1]++;
-                    if (grad[nnp - 1] <= GRMAX) G(grad[nnp - 1], nnp) = nzp;
+    NodeGraph g;
+    g.nb.resize(sanz);
+    size_t first = 0;  // first element of the current type group
+    for (int t = 0; t < typanz; ++t) {
+        for (int m = 0; m < nelanz[t]; ++m) {
+            const int *el = nrel + (first + m) * maxn;
+            for (int i = 0; i < selanz[t]; ++i)
+                for (int j = i + 1; j < selanz[t]; ++j) {
+                    if (el[i] < 1 || el[i] > sanz || el[j] < 1 || el[j] > sanz) return CRT_ERR_ARG;
+                    if (int e = g.add_pair(el[i], el[j])) return e;
                 }
-            }
         }
-        idum += nelanz[l];
+        first += nelanz[t];
     }
-    for (int i = 0; i < sanz; ++i)
-        if (grad[i] > GRMAX) return CRT_ERR_UNSUPPORTED;
-    int mingd = *std::min_element(grad.begin(), grad.end());
-    if (mingd == 0) return CRT_ERR_ARG;
-    int spanz = std::min(std::max(elanz / 10, 1000), SPMAX);
-    std::vector<int> start(sanz + 1, 0);
-    int k = 0;
-    for (;;) {
-        bool done = false;
-        for (int i = 1; i <= sanz; ++i)
-            if (grad[i - 1] == mingd) {
-                start[++k] = i;
-                if (k >= spanz) { done = true; break; }
-            }
-        if (done || mingd > elanz / 10) break;
-        ++mingd;
+    int dmin = sanz;
+    for (int v = 1; v <= sanz; ++v) dmin = std::min(dmin, g.degree(v));
+    if (dmin == 0) return CRT_ERR_ARG;
+    const int want = 100;
+    std::vector<int> starts;
+    for (int d = dmin;; ++d) {
+        for (int v = 1; v <= sanz && (int)starts.size() < want; ++v)
+            if (g.degree(v) == d) starts.push_back(v);
+        if ((int)starts.size() >= want || d > elanz / 10) break;
     }
-    spanz = k - 1;
-    std::vector<int> neu(sanz), neuin(sanz), level(sanz + 2);
-    std::vector<char> num(sanz);
-    int mmin = sanz;
-    bool found = false;
-    for (int is = 1; is <= spanz; ++is) {
-        int nstart = start[is];
-        neu[0] = nstart;
-        neuin[nstart - 1] = 1;
-        std::fill(num.begin(), num.end(), 0);
-        num[nstart - 1] = 1;
-        level[1] = 1;
-        int levs = 1, leve = 1, nlev = 1, l = 1;
-        for (;;) {
-            for (int j = levs; j <= leve; ++j) {
-                int nzp = neu[j - 1], gradzp = grad[nzp - 1];
-                for (;;) {
-                    int mingr = GRMAX, kk = 0;
-                    for (int i = 1; i <= gradzp; ++i) {
-                        int nnp = G(i, nzp);
-                        if (num[nnp - 1] || grad[nnp - 1] > mingr) continue;
-                        mingr = grad[nnp - 1];
-                        kk = nnp;
-                    }
-                    if (!kk) break;
-                    ++l;
-                    neu[l - 1] = kk;
-                    neuin[kk - 1] = l;
-                    num[kk - 1] = 1;
-                }
-            }
-            levs += level[nlev];
-            ++nlev;
-            if (nlev > sanz) return CRT_ERR_ARG;
-            level[nlev] = l - levs + 1;
-            leve += level[nlev];
-            if (level[nlev] == 0 && leve < sanz) return CRT_ERR_ARG;  // disconnected grid
-            if (leve >= sanz) break;
-        }
-        int mb = 0;
-        for (int i = 1; i <= sanz; ++i) {
-            int nzp = neuin[i - 1];
-            for (int j = 1; j <= grad[i - 1]; ++j) mb = std::max(mb, std::abs(neuin[G(j, i) - 1] - nzp));
-        }
-        if (mb < mmin) {
-            mmin = mb;
-            std::copy(neuin.begin(), neuin.end(), perm);
-            found = true;
+    if (!starts.empty()) starts.pop_back();
+    std::vector<int> order, newnum(sanz);
+    int best = sanz + 1;
+    for (int st : starts) {
+        if (!cm_number(g, st, order, newnum)) return CRT_ERR_ARG;  // disconnected grid
+        int bw = 0;
+        for (int v = 1; v <= sanz; ++v)
+            for (int w : g.nb[v - 1]) bw = std::max(bw, std::abs(newnum[v - 1] - newnum[w - 1]));
+        if (bw < best) {
+            best = bw;
+            std::copy(newnum.begin(), newnum.end(), perm);
         }
     }
-    if (!found) return CRT_ERR_ARG;
-    if (mb_out) *mb_out = mmin;
+    if (best > sanz) return CRT_ERR_ARG;
+    if (mb_out) *mb_out = best;
     return 0;
 }
 
