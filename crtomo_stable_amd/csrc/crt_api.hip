@@ -43,6 +43,21 @@ int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const
                             const double *, const zd *, const zd *, zd *);
 int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                            const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
+// forward by static condensation onto the mixed-boundary nodes (crt_condense.hip)
+int crt_launch_cond_gather(hipStream_t, int, int, int, int, const int *, const zd *, zd *);
+int crt_launch_cond_fg(hipStream_t, int, int, int, int, int, const int *, const zd *, double *);
+int crt_launch_cond_rhs(hipStream_t, int, int, int, int, int, int, const int *, const int *, const int *, const zd *,
+                        const double *, const double *, zd *);
+int crt_launch_cond_trsm(hipStream_t, int, int, int, int, int, int, const zd *, const zd *, zd *, const int *,
+                         const int *, const int *, const zd *, int, const int *);
+int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, int, const int *, const int *,
+                          const int *, const int *, const int *, const zd *, const double *, const double *,
+                          const zd *, zd *, zd *);
+int crt_cond_dense_max_tiles();
+int crt_launch_cond_dense(hipStream_t, const DevGrid &, int, int, int, int, int, int, const zd *, const zd *,
+                          const int *, const int *, const int *, const double *, zd *, zd *, zd *, zd *, int *);
+int crt_launch_cond_gemm(hipStream_t, int, int, int, int, int, int, int, int, int, const zd *, const zd *,
+                         const double *, const int *, zd *);
 // model uncertainty (crt_bmcm.hip)
 int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, int, const double *, const int *, double,
                  int, double *, double *, double *, double *, double *, double *, int *, int *);
@@ -156,7 +171,7 @@ struct DBuf {
     }
 };
 
-enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_N };
+enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_SCHUR, T_N };
 
 }  // namespace
 
@@ -256,6 +271,25 @@ struct crt_ctx {
     DBuf<CGScalars> d_sc;
     int cg_check_every = 16;
     size_t panel_budget = (size_t)96 << 30;  // bytes of factor panels kept in flight
+    // forward method: -1 condensed where it applies, else the band factorisation of
+    // every system; 0 always the band factorisation (the reference's algorithm);
+    // 1 condensed (903 where it does not apply)
+    int fwd_method = -1;
+    int last_method = 0;
+    // static condensation plan of the grid (boundary nodes G, interior nodes I)
+    struct Cond {
+        bool built = false, ok = false;
+        int nI = 0, nG = 0, nGp = 0, nt = 0, mbI = 0, DI = 0, nblkI = 0, nnzI = 0, E = 0, Ep = 0, ncol = 0;
+        int ntile = 0, nent = 0, order = 0;
+        std::vector<int> nodeI, nodeG;
+        DBuf<int> d_row_ptr, d_col, d_crow, d_zero, d_tmap, d_qmap, d_enr0;
+        DBuf<int> d_tptr, d_erow, d_ecol, d_ecolg, d_eq, d_j0;
+        DBuf<int> d_gi_ptr, d_gi_row, d_gi_q, d_gmap, d_qdiag, d_nodeI, d_nodeG, d_eG;
+        DBuf<zd> areaI, vals, panels, yI, dinv, X, S0, Gr, XG, Lt, Li, rv;
+        DBuf<double> fak, dga, fG;
+        DBuf<int2> sysI;
+        DBuf<int> stI, stD;
+    } cond;
 
     DevGrid devgrid() const {
         DevGrid g;
@@ -639,6 +673,7 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     if (!c || sanz <= 0 || mb < 0 || typanz <= 0 || maxn <= 0 || maxn > 4) return CRT_ERR_ARG;
     CRT_HIP(hipSetDevice(c->device));
     c->have_grid = c->have_precal = c->have_kpot = c->have_volt = c->have_sens = false;
+    c->cond.built = c->cond.ok = false;
     c->sanz = sanz; c->mb = mb; c->typanz = typanz; c->maxn = maxn; c->eanz = eanz;
     c->npk = maxn * (maxn + 1) / 2;
     c->typ.assign(typ, typ + typanz);
@@ -1113,11 +1148,352 @@ static int forward_batch_f32(crt_ctx *c, const DevGrid &g, int nb, size_t s0, in
     return -1;
 }
 
+// ------------------------------------------------ condensed forward: plan
+// The boundary nodes G are the nodes of the type-11 elements (their terms hold the
+// source through kg, src/kompab.f90:111-116), the interior nodes I the rest.  The
+// interior band is numbered in the order of smallest band width among the reference's
+// (CutMcK) numbering, the nodes sorted by x and the nodes sorted by y; the boundary
+// nodes by their first interior neighbour, so the right-hand sides of a 16-column tile
+// of [A_IG | b_I] start close together.  Not applicable (ok false): no boundary
+// nodes, an interior band of one tile or more than 16, more than 640 boundary nodes.
+static int cond_plan(crt_ctx *c) {
+    auto &P = c->cond;
+    if (P.built) return 0;
+    P.built = true;
+    P.ok = false;
+    const int sanz = c->sanz, nnz = c->nnz;
+    const std::vector<int> &col = c->h_col, &crow = c->h_crow;
+    std::vector<char> inG(sanz, 0);
+    {
+        int iel = 0;
+        for (int t = 0; t < c->typanz; ++t)
+            for (int j = 0; j < c->nelanz[t]; ++j, ++iel) {
+                if (c->typ[t] != 11) continue;
+                for (int m = 0; m < c->selanz[t]; ++m) inG[c->nrel[(size_t)iel * c->maxn + m] - 1] = 1;
+            }
+    }
+    P.nodeI.clear();
+    P.nodeG.clear();
+    for (int i = 0; i < sanz; ++i) (inG[i] ? P.nodeG : P.nodeI).push_back(i);
+    P.nI = (int)P.nodeI.size();
+    P.nG = (int)P.nodeG.size();
+    if (P.nG == 0 || P.nI == 0) return 0;
+    P.nGp = (P.nG + 15) / 16 * 16;
+    P.nt = P.nGp / 16;
+    if (P.nt > crt_cond_dense_max_tiles()) return 0;
+    // interior numbering of smallest band width
+    std::vector<int> best, pos(sanz, -1);
+    int bestw = INT32_MAX;
+    for (int order = 0; order < 3; ++order) {
+        std::vector<int> ord = P.nodeI;
+        if (order == 1)
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+                return c->x[a] != c->x[b] ? c->x[a] < c->x[b] : c->y[a] < c->y[b];
+            });
+        if (order == 2)
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+                return c->y[a] != c->y[b] ? c->y[a] < c->y[b] : c->x[a] < c->x[b];
+            });
+        std::vector<int> ps(sanz, -1);
+        for (int i = 0; i < P.nI; ++i) ps[ord[i]] = i;
+        int w = 0;
+        for (int q = 0; q < nnz; ++q) {
+            const int a = ps[crow[q]], b = ps[col[q]];
+            if (a >= 0 && b >= 0) w = std::max(w, std::abs(a - b));
+        }
+        if (w < bestw) {
+            bestw = w;
+            best = ord;
+            pos = ps;
+            P.order = order;
+        }
+    }
+    P.nodeI = best;
+    P.mbI = bestw;
+    P.DI = std::max(1, (bestw + 15) / 16);
+    if (P.DI < 2 || P.DI > 16) return 0;
+    P.nblkI = (P.nI + 15) / 16;
+    // interior CSR (lower triangle, diagonal last) and its entries in the full CSR
+    struct E3 { int r, cc, q; };
+    std::vector<E3> ie;
+    std::vector<int> gidx(sanz, -1);
+    for (int q = 0; q < nnz; ++q) {
+        const int a = pos[crow[q]], b = pos[col[q]];
+        if (a >= 0 && b >= 0) ie.push_back({std::max(a, b), std::min(a, b), q});
+    }
+    std::sort(ie.begin(), ie.end(), [](const E3 &u, const E3 &v) { return u.r != v.r ? u.r < v.r : u.cc < v.cc; });
+    P.nnzI = (int)ie.size();
+    std::vector<int> rp(P.nI + 1, 0), ci(P.nnzI), cr(P.nnzI), qm(P.nnzI);
+    for (int e = 0; e < P.nnzI; ++e) {
+        rp[ie[e].r + 1]++;
+        ci[e] = ie[e].cc;
+        cr[e] = ie[e].r;
+        qm[e] = ie[e].q;
+    }
+    for (int i = 0; i < P.nI; ++i) rp[i + 1] += rp[i];
+    for (int i = 0; i < P.nI; ++i)
+        if (rp[i + 1] == rp[i] || ci[rp[i + 1] - 1] != i) return CRT_ERR_STATE;  // every row ends on its diagonal
+    const int D = P.DI;
+    std::vector<int> tmap((size_t)P.nblkI * (D + 1) * 256, P.nnzI);
+    for (int i = 0; i < P.nI; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            const int j = ci[q], I = i / 16, d = I - j / 16;
+            tmap[((size_t)(I * (D + 1) + d) * 16 + (i % 16)) * 16 + (j % 16)] = q;
+        }
+    for (int i = P.nI; i < P.nblkI * 16; ++i)
+        tmap[((size_t)((i / 16) * (D + 1)) * 16 + (i % 16)) * 16 + (i % 16)] = P.nnzI + 1;
+    // boundary numbering: by the first interior neighbour
+    std::vector<int> key(sanz, INT32_MAX);
+    for (int q = 0; q < nnz; ++q) {
+        const int i = crow[q], j = col[q];
+        if (inG[i] && pos[j] >= 0) key[i] = std::min(key[i], pos[j]);
+        if (inG[j] && pos[i] >= 0) key[j] = std::min(key[j], pos[i]);
+    }
+    std::stable_sort(P.nodeG.begin(), P.nodeG.end(), [&](int a, int b) { return key[a] < key[b]; });
+    for (int g = 0; g < P.nG; ++g) gidx[P.nodeG[g]] = g;
+    const int nGp = P.nGp;
+    std::vector<int> gmap((size_t)nGp * nGp, -1), qdiag(P.nG, -1);
+    std::vector<std::vector<std::pair<int, int>>> gi(P.nG);
+    struct RE { int row, colx, q; };
+    std::vector<RE> re;
+    for (int q = 0; q < nnz; ++q) {
+        const int i = crow[q], j = col[q];
+        const int gi_ = gidx[i], gj = gidx[j];
+        if (gi_ >= 0 && gj >= 0) {
+            gmap[(size_t)gj * nGp + gi_] = q;
+            gmap[(size_t)gi_ * nGp + gj] = q;
+            if (i == j) qdiag[gi_] = q;
+        } else if (gi_ >= 0 || gj >= 0) {
+            const int g = gi_ >= 0 ? gi_ : gj, ii = gi_ >= 0 ? pos[j] : pos[i];
+            gi[g].push_back({ii, q});
+            re.push_back({ii, g, q});
+        }
+    }
+    for (int g = 0; g < P.nG; ++g)
+        if (qdiag[g] < 0) return CRT_ERR_STATE;
+    // sources: U column l of an interior electrode node, b_G entry of a boundary one
+    P.E = c->eanz;
+    P.Ep = (P.E + 15) / 16 * 16;
+    std::vector<int> eG(std::max(P.E, 1), -1);
+    for (int l = 0; l < P.E; ++l) {
+        const int nd = c->enr0[l];
+        if (pos[nd] >= 0) re.push_back({pos[nd], nGp + l, -1});
+        else eG[l] = gidx[nd];
+    }
+    P.ncol = nGp + P.Ep;
+    P.ntile = P.ncol / 16;
+    std::sort(re.begin(), re.end(), [](const RE &u, const RE &v) {
+        if (u.colx / 16 != v.colx / 16) return u.colx / 16 < v.colx / 16;
+        return u.row != v.row ? u.row < v.row : u.colx < v.colx;
+    });
+    P.nent = (int)re.size();
+    std::vector<int> tptr(P.ntile + 1, 0), erow(std::max(P.nent, 1)), ecol(std::max(P.nent, 1)),
+        ecolg(std::max(P.nent, 1)), eq(std::max(P.nent, 1)), j0(P.ntile, P.nblkI);
+    for (int e = 0; e < P.nent; ++e) {
+        tptr[re[e].colx / 16 + 1]++;
+        erow[e] = re[e].row;
+        ecol[e] = re[e].colx % 16;
+        ecolg[e] = re[e].colx;
+        eq[e] = re[e].q;
+    }
+    for (int t = 0; t < P.ntile; ++t) tptr[t + 1] += tptr[t];
+    for (int t = 0; t < P.ntile; ++t)
+        if (tptr[t] < tptr[t + 1]) j0[t] = erow[tptr[t]] / 16;
+    std::vector<int> gptr(P.nG + 1, 0), grow_, gq;
+    for (int g = 0; g < P.nG; ++g) {
+        std::sort(gi[g].begin(), gi[g].end());
+        for (auto &pr : gi[g]) {
+            grow_.push_back(pr.first);
+            gq.push_back(pr.second);
+        }
+        gptr[g + 1] = (int)grow_.size();
+    }
+    if (grow_.empty()) { grow_.push_back(0); gq.push_back(0); }
+    std::vector<int> zero(P.nnzI + 1, 0), enr0(std::max(c->eanz, 1), 0);
+    hipStream_t st = c->stream;
+    int e = 0;
+    e = e ? e : P.d_row_ptr.upload(rp.data(), rp.size(), st);
+    e = e ? e : P.d_col.upload(ci.data(), ci.size(), st);
+    e = e ? e : P.d_crow.upload(cr.data(), cr.size(), st);
+    e = e ? e : P.d_qmap.upload(qm.data(), qm.size(), st);
+    e = e ? e : P.d_zero.upload(zero.data(), zero.size(), st);
+    e = e ? e : P.d_enr0.upload(enr0.data(), enr0.size(), st);
+    e = e ? e : P.d_tmap.upload(tmap.data(), tmap.size(), st);
+    e = e ? e : P.d_tptr.upload(tptr.data(), tptr.size(), st);
+    e = e ? e : P.d_erow.upload(erow.data(), erow.size(), st);
+    e = e ? e : P.d_ecol.upload(ecol.data(), ecol.size(), st);
+    e = e ? e : P.d_ecolg.upload(ecolg.data(), ecolg.size(), st);
+    e = e ? e : P.d_eq.upload(eq.data(), eq.size(), st);
+    e = e ? e : P.d_j0.upload(j0.data(), j0.size(), st);
+    e = e ? e : P.d_gi_ptr.upload(gptr.data(), gptr.size(), st);
+    e = e ? e : P.d_gi_row.upload(grow_.data(), grow_.size(), st);
+    e = e ? e : P.d_gi_q.upload(gq.data(), gq.size(), st);
+    e = e ? e : P.d_gmap.upload(gmap.data(), gmap.size(), st);
+    e = e ? e : P.d_qdiag.upload(qdiag.data(), qdiag.size(), st);
+    e = e ? e : P.d_nodeI.upload(P.nodeI.data(), P.nodeI.size(), st);
+    e = e ? e : P.d_nodeG.upload(P.nodeG.data(), P.nodeG.size(), st);
+    e = e ? e : P.d_eG.upload(eG.data(), eG.size(), st);
+    if (e) return e;
+    CRT_HIP(hipStreamSynchronize(st));
+    P.ok = true;
+    return 0;
+}
+
+// ------------------------------------------------ condensed forward: solve
+// Every wavenumber chunk: the interior area sums, scaling, assembly and band
+// factorisation (the band kernels on the interior grid), the substitutions for
+// [Z | U], the Schur complements and right-hand sides, the dense factorisation and
+// solve of every (l, k), and the interior potentials.  Errors: a zero interior
+// diagonal or pivot names the wavenumber's first source (the interior rows are the
+// same for every source), a boundary one its own source; the first failing system
+// in the reference's loop order (wavenumber, then electrode) is returned.
+static int forward_condensed(crt_ctx *c) {
+    auto &P = c->cond;
+    const int K = c->kwnanz, E = c->eanz;
+    hipStream_t st = c->stream;
+    DevGrid g = c->devgrid();
+    int e = c->d_area.ensure((size_t)K * c->nnz);
+    e = e ? e : crt_launch_area(st, g, K, c->d_area.p);
+    if (e) return e;
+    g.area = c->d_area.p;
+    const int nGp = P.nGp, nt = P.nt, ldx = P.nblkI * 16, D = P.DI;
+    const size_t fstrideI = (size_t)P.nblkI * 16, ntri = (size_t)nt * (nt + 1) / 2;
+    const size_t per_k = sizeof(zd) * ((size_t)P.ncol * ldx + (size_t)P.nblkI * (D + 1) * 256 + 2 * fstrideI +
+                                       (size_t)P.nnzI * 2 + 2 + (size_t)nGp * nGp + 2 * (size_t)E * nGp +
+                                       (size_t)E * (ntri + nt) * 256 + P.nent) +
+                         sizeof(double) * (2 * fstrideI + nGp);
+    const int kc = (int)std::max<size_t>(1, std::min<size_t>(K, c->panel_budget / per_k));
+    e = P.areaI.ensure((size_t)kc * P.nnzI);
+    e = e ? e : P.vals.ensure((size_t)kc * (P.nnzI + 2));
+    e = e ? e : P.panels.ensure((size_t)kc * P.nblkI * (D + 1) * 256);
+    e = e ? e : P.yI.ensure((size_t)kc * fstrideI);
+    e = e ? e : P.dinv.ensure((size_t)kc * fstrideI);
+    e = e ? e : P.fak.ensure((size_t)kc * fstrideI);
+    e = e ? e : P.dga.ensure((size_t)kc * fstrideI);
+    e = e ? e : P.fG.ensure((size_t)kc * nGp);
+    e = e ? e : P.rv.ensure((size_t)kc * std::max(P.nent, 1));
+    e = e ? e : P.X.ensure((size_t)kc * P.ncol * ldx);
+    e = e ? e : P.S0.ensure((size_t)kc * nGp * nGp);
+    e = e ? e : P.Gr.ensure((size_t)kc * E * nGp);
+    e = e ? e : P.XG.ensure((size_t)kc * E * nGp);
+    e = e ? e : P.Lt.ensure((size_t)kc * E * ntri * 256);
+    e = e ? e : P.Li.ensure((size_t)kc * E * nt * 256);
+    e = e ? e : P.sysI.ensure(kc);
+    e = e ? e : P.stI.ensure(K);
+    e = e ? e : P.stD.ensure((size_t)K * E);
+    e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
+    if (e) return e;
+    std::vector<int2> sysI(kc);
+    for (int kk = 0; kk < kc; ++kk) sysI[kk] = int2{0, kk};
+    CRT_HIP(hipMemcpyAsync(P.sysI.p, sysI.data(), kc * sizeof(int2), hipMemcpyHostToDevice, st));
+    CRT_HIP(hipMemsetAsync(P.stI.p, 0, K * sizeof(int), st));
+    CRT_HIP(hipMemsetAsync(P.stD.p, 0, (size_t)K * E * sizeof(int), st));
+    // the interior grid: the band kernels' view of A_II (area sums only)
+    DevGrid gI = g;
+    gI.sanz = P.nI; gI.mb = P.mbI; gI.nblk = P.nblkI; gI.D = D; gI.nnz = P.nnzI;
+    gI.fstride = (int)fstrideI; gI.vstride = P.nnzI + 2;
+    gI.row_ptr = P.d_row_ptr.p; gI.col = P.d_col.p; gI.crow = P.d_crow.p;
+    gI.cptr = P.d_zero.p; gI.csplit = P.d_zero.p; gI.contrib = P.d_zero.p;
+    gI.enr = P.d_enr0.p;
+    gI.tmap = P.d_tmap.p; gI.tmapD = D;
+    gI.area = P.areaI.p;
+    gI.dmask = nullptr;
+    const int variant = D <= 10 ? 1 : 2;
+    double tdiag = 0, tfac = 0, tsol = 0, tsch = 0;
+    for (int k0 = 0; k0 < K; k0 += kc) {
+        const int n = std::min(kc, K - k0);
+        c->tic(T_DIAG);
+        e = crt_launch_cond_gather(st, n, k0, c->nnz, P.nnzI, P.d_qmap.p, c->d_area.p, P.areaI.p);
+        e = e ? e : crt_launch_diag(st, gI, n, P.sysI.p, P.fak.p, P.dga.p, P.stI.p + k0);
+        e = e ? e : crt_launch_assemble(st, gI, n, P.sysI.p, P.fak.p, P.dga.p, P.vals.p);
+        e = e ? e : crt_launch_cond_fg(st, n, k0, P.nG, nGp, c->nnz, P.d_qdiag.p, c->d_area.p, P.fG.p);
+        e = e ? e : crt_launch_cond_rhs(st, n, k0, P.nent, c->nnz, nGp, (int)fstrideI, P.d_erow.p, P.d_ecolg.p,
+                                        P.d_eq.p, c->d_area.p, P.fak.p, P.fG.p, P.rv.p);
+        if (e) return e;
+        c->toc(T_DIAG);
+        c->tic(T_FACTOR);
+        e = crt_launch_factor16(st, gI, n, P.sysI.p, P.fak.p, P.vals.p, P.panels.p, P.yI.p, P.dinv.p, P.stI.p + k0,
+                                variant);
+        if (!e && g.dc) e = crt_launch_dc_pivot(st, n, P.nI, (int)fstrideI, P.dinv.p, P.stI.p + k0);
+        if (e) return e;
+        c->toc(T_FACTOR);
+        c->tic(T_SOLVE);
+        e = crt_launch_cond_trsm(st, D, 0, P.nblkI, ldx, P.ntile, n, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
+                                 P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
+        e = e ? e : crt_launch_cond_trsm(st, D, 1, P.nblkI, ldx, P.ntile, n, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
+                                         P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
+        e = e ? e : crt_launch_cond_schur(st, n, k0, P.nG, nGp, E, ldx, P.ntile, c->nnz, (int)fstrideI, P.d_gi_ptr.p,
+                                          P.d_gi_row.p, P.d_gi_q.p, P.d_gmap.p, P.d_eG.p, c->d_area.p, P.fak.p,
+                                          P.fG.p, P.X.p, P.S0.p, P.Gr.p);
+        if (e) return e;
+        c->toc(T_SOLVE);
+        c->tic(T_SCHUR);
+        e = crt_launch_cond_dense(st, g, P.nG, nGp, E, K, k0, n * E, P.S0.p, P.Gr.p, P.d_gmap.p, P.d_qdiag.p,
+                                  P.d_nodeG.p, P.fG.p, P.Lt.p, P.Li.p, P.XG.p, c->d_kpotN.p, P.stD.p + (size_t)k0 * E);
+        if (e) return e;
+        c->toc(T_SCHUR);
+        c->tic(T_REFINE);
+        e = crt_launch_cond_gemm(st, n, k0, P.nI, ldx, P.ntile, nGp, E, K, (int)fstrideI, P.X.p, P.XG.p, P.fak.p,
+                                 P.d_nodeI.p, c->d_kpotN.p);
+        if (e) return e;
+        c->toc(T_REFINE);
+        CRT_HIP(hipStreamSynchronize(st));
+        c->collect(T_DIAG); tdiag += c->times[T_DIAG];
+        c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
+        c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
+        c->collect(T_SCHUR); tsch += c->times[T_SCHUR];
+        c->collect(T_REFINE); tsol += c->times[T_REFINE];
+    }
+    c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol; c->times[T_SCHUR] = tsch;
+    c->times[T_REFINE] = 0.0;
+    std::vector<int> sI(K), sD((size_t)K * E);
+    CRT_HIP(c->copy_out(sI.data(), P.stI.p, K * sizeof(int)));
+    CRT_HIP(c->copy_out(sD.data(), P.stD.p, (size_t)K * E * sizeof(int)));
+    for (int k = 0; k < K; ++k) {
+        if (sI[k]) return sI[k];
+        for (int l = 0; l < E; ++l)
+            if (sD[(size_t)k * E + l]) return sD[(size_t)k * E + l];
+    }
+    return 0;
+}
+
+int crt_set_forward_method(crt_ctx *c, int method) {
+    if (!c || method < -1 || method > 1) return CRT_ERR_ARG;
+    c->fwd_method = method;
+    return 0;
+}
+
+int crt_get_forward_method(crt_ctx *c, int *method) {
+    if (!c || !method) return CRT_ERR_ARG;
+    *method = c->last_method;
+    return 0;
+}
+
 int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
     if (int e0 = build_dmask(c)) return e0;
+    // the condensed solve where the grid has mixed boundaries and no Dirichlet rows
+    if (c->fwd_method != 0 && c->lbeta && !c->dmask_any && c->asplit && c->fwd_bits == 64) {
+        if (int e0 = cond_plan(c)) return e0;
+    }
+    const bool cond = c->fwd_method != 0 && c->lbeta && !c->dmask_any && c->asplit && c->fwd_bits == 64 &&
+                      c->cond.ok;
+    if (c->fwd_method == 1 && !cond) return CRT_ERR_UNSUPPORTED;
+    if (cond) {
+        c->refine_sweeps = 0;
+        c->refine_ratio = 0.0;
+        c->refine_fallbacks = 0;
+        c->last_method = 1;
+        c->have_kpot = false;
+        if (int e0 = forward_condensed(c)) return e0;
+        c->have_kpot = true;
+        c->have_volt = c->have_sens = false;
+        return 0;
+    }
+    c->last_method = 0;
+    c->times[T_SCHUR] = 0.0;
     // 16x16 matrix-core tiles when the band fits (mb <= 256), else 8x8 LDS tiles
     const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
     const int NBt = use16 ? 16 : CRT_NB;

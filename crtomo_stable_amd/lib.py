@@ -340,6 +340,18 @@ class Context:
         _chk(self.lib.crt_set_forward_precision(self._h, int(bits), C.c_double(tol), int(max_sweeps)),
              "crt_set_forward_precision")
 
+    def set_forward_method(self, method: int):
+        """-1 (default): the condensed solve where the grid allows it, else the band
+        factorisation of every system; 0: always the band factorisation (the
+        reference's algorithm); 1: always condensed (903 where it does not apply)."""
+        _chk(self.lib.crt_set_forward_method(self._h, int(method)), "crt_set_forward_method")
+
+    def forward_method(self) -> int:
+        """Method the last forward used: 0 band per system, 1 condensed."""
+        m = C.c_int()
+        _chk(self.lib.crt_get_forward_method(self._h, C.byref(m)), "crt_get_forward_method")
+        return m.value
+
     def refine_info(self):
         """(sweeps, final correction ratio, batches redone in FP64) of the last forward."""
         sw, fb = C.c_int(), C.c_int()
@@ -415,9 +427,10 @@ class Context:
         return err, a, p
 
     def times(self):
-        ms = np.zeros(9)
-        self.lib.crt_get_times(self._h, _ptr(ms), 9)
-        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine"), ms))
+        ms = np.zeros(10)
+        self.lib.crt_get_times(self._h, _ptr(ms), 10)
+        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine", "schur"),
+                        ms))
 
 
 class Batch:

@@ -279,10 +279,27 @@ int crt_vredc(crt_ctx *ctx, int sanz, int mb, const double *a, const double *b, 
 int crt_set_forward_precision(crt_ctx *ctx, int bits, double tol, int max_sweeps);
 int crt_get_refine_info(crt_ctx *ctx, int *sweeps, double *ratio, int *fallbacks);
 
+/* How crt_forward solves the systems of a grid with mixed boundaries (no counterpart
+ * in the reference, which factorises every (electrode, wavenumber) system,
+ * src/fem.f90:328-397; the potentials agree to rounding):
+ *   0  the band factorisation of every system (kompab + scalab + chol + vre);
+ *   1  static condensation onto the mixed-boundary nodes: the interior block, the same
+ *      for every source, factorised once per wavenumber, a dense complex-symmetric
+ *      factorisation of the boundary Schur complement per system; 903 where it does
+ *      not apply (no type-11 elements, Dirichlet rows, the FP32 forward, an interior
+ *      band of one or more than 16 tiles of 16, more than 640 boundary nodes);
+ *  -1  (default) 1 where it applies, else 0.
+ * crt_get_forward_method: the method of the last crt_forward. */
+int crt_set_forward_method(crt_ctx *ctx, int method);
+int crt_get_forward_method(crt_ctx *ctx, int *method);
+
 /* Device time of the last call of each phase, milliseconds:
  * [0] diag, [1] factor, [2] solve, [3] voltages, [4] sensitivities,
  * [5] precal, [6] cg total, [7] cg matvec (mean per iteration),
- * [8] refinement sweeps of the FP32 forward (total). */
+ * [8] refinement sweeps of the FP32 forward (total), [9] the dense boundary
+ * factorisations of the condensed forward (total).  The condensed forward counts its
+ * interior band factorisation as [1] and its substitutions, Schur complements and
+ * interior potentials as [2]. */
 int crt_get_times(crt_ctx *ctx, double *ms, int n);
 
 /* Batched forward of many models of uneven size (crt_forward of every context,

@@ -36,6 +36,7 @@ def _ctx(w):
     c.rwaven()
     c.lbeta = c.precal()
     c.set_sigma(w.sigma)
+    c.set_forward_method(0)  # the batch runs the band factorisation of every system
     return c
 
 
