@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Throughput of the CRTomo forward + Jacobian path on one MI355X.
 
-One step = the complex forward solve for every (electrode, wavenumber) system
-(on-the-fly assembly, scaling, band Cholesky, substitution), the inverse
-Fourier back-transform with the voltages (bvolti) and the full Jacobian
-(bsensi), on the seeded C1 workload: ~20k-triangle Delaunay mesh over
+One step = the complex forward solve for every (electrode, wavenumber) system,
+the inverse Fourier back-transform with the voltages (bvolti) and the full
+Jacobian (bsensi), on the seeded C1 workload: ~20k-triangle Delaunay mesh over
 100 x 30 m with 2 m sinusoidal topography, 64 electrodes at 1.5 m, 2000
-dipole-dipole + Wenner quadrupoles, complex GRF model.  Grid, element matrices
-and conductivities are resident in HBM before the timed region.
+dipole-dipole + Wenner quadrupoles, complex GRF model.  The forward runs the
+library's default method: static condensation onto the mixed-boundary nodes
+(interior band factorised once per wavenumber, dense boundary Schur complement
+per system); the band factorisation of every system (the reference's
+algorithm) is timed beside it (band_per_system).  Grid, element matrices and
+conductivities are resident in HBM before the timed region.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -29,6 +32,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 KERNEL_SOURCES = {  # the source file each benched kernel is compiled from
+    "crt_cond_trsm_kernel": "crt_condense.hip", "crt_cond_dense_kernel": "crt_condense.hip",
+    "crt_cond_gemm_kernel": "crt_condense.hip",
     "crt_factor16la_kernel": "crt_band16.hip", "crt_factor16st_kernel": "crt_band16.hip",
     "crt_factor16_kernel": "crt_band16.hip", "crt_solve16_kernel": "crt_band16.hip",
     "crt_factor32st_kernel": "crt_band32.hip", "crt_factor_kernel": "crt_band.hip",
@@ -368,9 +373,14 @@ def gn_inversion(nit: int, train_models: int, pc_steps: int, eps: float = 1e-4):
     homogeneous 100 ohm-m / -5 mrad), the loop of crtomo_stable_amd.inversion (lam0 from
     blam0, halved per step, a step that does not lower the misfit retried at half
     length), with each CG form:
-      fp64_jacobi    the reference's method on the device: FP64 J, Jacobi scaling;
-      fp32j_fsai     complex FP32 copy of J in the iterations, FSAI preconditioner;
-      fp32j_learned  FP32 J copy, learned preconditioner trained beforehand on the CG
+      fp64_jacobi    the plain FP64 GPU path, the reference's algorithms on the device:
+                     band factorisation of every (electrode, wavenumber) system, FP64 J,
+                     Jacobi-scaled CG;
+      fp64_jacobi_condensed  the same with the condensed forward;
+      fp32j_fsai     condensed forward, complex FP32 copy of J in the iterations, FSAI
+                     preconditioner;
+      fp32j_learned  condensed forward, FP32 J copy, learned preconditioner trained
+                     beforehand on the CG
                      systems of `train_models` inversions of other models (seeded
                      Gaussian random fields, 10..1000 ohm-m, correlation 2..20 m) on
                      the same grid and survey, none of them the scored model.
@@ -399,9 +409,11 @@ def gn_inversion(nit: int, train_models: int, pc_steps: int, eps: float = 1e-4):
                        "noise; complex Gauss-Newton from 100 ohm-m / -5 mrad, lambda lam0 (blam0) halved per step "
                        "with half-step control; ncgmax manz/10",
            "cg_eps": eps, "reference_steps": nit, "forms": {}, "first_step_eps_1e-12": {}}
-    forms = (("fp64_jacobi", 64, 0), ("fp32j_fsai", 32, 2), ("fp32j_learned", 32, 3))
+    forms = (("fp64_jacobi", 64, 0, 0), ("fp64_jacobi_condensed", 64, 0, 1), ("fp32j_fsai", 32, 2, 1),
+             ("fp32j_learned", 32, 3, 1))
     lamfix, target = None, None
-    for name, jbits, prec in forms:
+    for name, jbits, prec, method in forms:
+        ctx.set_forward_method(method)
         t_train, losses = 0.0, []
         if prec == 3:
             # training inversions of other models (their CG in the FP32-J FSAI form), each
@@ -428,7 +440,8 @@ def gn_inversion(nit: int, train_models: int, pc_steps: int, eps: float = 1e-4):
             lamfix, target = r.lamfix, r.final_rms
         st = r.step_ms()
         its = [i for i in r.iterations if i.ncg]
-        f = {"rms": [round(i.rms, 4) for i in r.iterations], "final_rms": r.final_rms, "steps": len(st),
+        f = {"forward_method": "band per system" if method == 0 else "condensed",
+             "rms": [round(i.rms, 4) for i in r.iterations], "final_rms": r.final_rms, "steps": len(st),
              "ncg": [i.ncg for i in its], "lambda": [i.lam for i in its],
              "step_length": [i.ms.get("step") for i in its],
              "step_ms": [round(x, 2) for x in st], "total_ms": round(float(np.sum(st)), 2),
@@ -452,6 +465,7 @@ def gn_inversion(nit: int, train_models: int, pc_steps: int, eps: float = 1e-4):
                                              "ncg": u["ncg"]}
     ctx.set_cg_precision(64)
     ctx.set_cg_preconditioner(0)
+    ctx.set_forward_method(-1)
     ctx.close()
     base = out["forms"]["fp64_jacobi"]
     for name, f in out["forms"].items():
@@ -594,6 +608,9 @@ def main():
     ap.add_argument("--gn-train-models", type=int, default=2)
     ap.add_argument("--gn", action="store_true",
                     help="also time one Gauss-Newton step (forward, bsensi, bsmatm, blam0, update/cjg)")
+    ap.add_argument("--forward-method", type=int, default=-1, choices=[-1, 0, 1],
+                    help="-1: the library default (condensed where the grid allows), 0: band factorisation of "
+                         "every system, 1: condensed")
     ap.add_argument("--forward-precision", type=int, default=64, choices=[64, 32],
                     help="32: FP32 band factorisation with FP64 iterative refinement in the timed step")
     ap.add_argument("--buncert", action="store_true",
@@ -639,41 +656,87 @@ def main():
         ctx.voltages(1)
         ctx.sens(1)
 
-    for _ in range(args.warmup):
-        step()
-    device_sync()
-    acc = dict(diag=0.0, factor=0.0, solve=0.0, refine=0.0, volt=0.0, sens=0.0)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        t = ctx.times()
-        for key in acc:
-            acc[key] += t[key]
-    device_sync()
-    elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / args.steps
-    phase = {k: v / args.steps for k, v in acc.items()}
+    keys = ("diag", "factor", "solve", "schur", "product", "refine", "volt", "sens")
 
-    # band Cholesky: 8 sanz mb(mb+1)/2 real FLOP per (electrode, wavenumber) system
-    flop_sys = 8.0 * g.sanz * g.mb * (g.mb + 1) / 2.0
-    fac_ms = phase["factor"]
-    achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
-    d16 = -(-g.mb // 16)
-    peak = FP64_PEAK_TFLOPS
-    if args.forward_precision == 32:
-        kernel = f"crt_factor32st_kernel<{d16}>"
-        peak = FP32_PEAK_TFLOPS
-    elif os.environ.get("CRT_FACTOR8") or d16 > 16:
-        kernel = "crt_factor_kernel"
-    elif d16 == 1 or os.environ.get("CRT_FACTOR_V2") and d16 <= 10:
-        kernel = f"crt_factor16_kernel<{d16}>"
+    def timed(nsteps, nwarm):
+        for _ in range(nwarm):
+            step()
+        device_sync()
+        acc = {k: 0.0 for k in keys}
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            step()
+            t = ctx.times()
+            for key in acc:
+                acc[key] += t[key]
+        device_sync()
+        return 1e3 * (time.perf_counter() - t0) / nsteps, {k: v / nsteps for k, v in acc.items()}
+
+    def band_roofline(phase):
+        # band Cholesky: 8 sanz mb(mb+1)/2 real FLOP per (electrode, wavenumber) system
+        flop_sys = 8.0 * g.sanz * g.mb * (g.mb + 1) / 2.0
+        fac_ms = phase["factor"]
+        achieved = flop_sys * nsys / (fac_ms * 1e-3) / 1e12
+        d16 = -(-g.mb // 16)
+        peak = FP64_PEAK_TFLOPS
+        if args.forward_precision == 32:
+            kernel = f"crt_factor32st_kernel<{d16}>"
+            peak = FP32_PEAK_TFLOPS
+        elif os.environ.get("CRT_FACTOR8") or d16 > 16:
+            kernel = "crt_factor_kernel"
+        elif d16 == 1 or os.environ.get("CRT_FACTOR_V2") and d16 <= 10:
+            kernel = f"crt_factor16_kernel<{d16}>"
+        else:
+            kernel = f"crt_factor16{'la' if d16 <= 10 else 'st'}_kernel<{d16}>"
+        return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4), traffic=measured_traffic(kernel, args.workload),
+                    kernel=kernel, flop_per_system=flop_sys, systems_per_launch=nsys, launch_ms=round(fac_ms, 4))
+
+    def condensed_rooflines(phase):
+        """Algorithmic FLOP of each kernel of the condensed forward over its measured
+        phase time (HIP events on the library's stream):
+          interior band factorisation: 8 nI mbI(mbI+1)/2 per wavenumber;
+          substitution (forward and back, both passes): 2 x 8 nI mbI per right-hand side,
+            nG + E right-hand sides per wavenumber;
+          dense boundary factorisation: (4/3) nG^3 + 16 nG^2 per (electrode, wavenumber);
+          interior potentials: 8 nI nG per (electrode, wavenumber)."""
+        info = ctx.condense_info()
+        nI, nG, mbI, DI = info["interior"], info["boundary"], info["band"], info["tiles"]
+        NI = -(-(-(-nG // 16)) // 8)
+        ent = {
+            "factor": (f"crt_factor16{'la' if DI <= 10 else 'st'}_kernel<{DI}>",
+                       K * 8.0 * nI * mbI * (mbI + 1) / 2.0, K),
+            "solve": (f"crt_cond_trsm_kernel<{DI}, false> + crt_cond_trsm_kernel<{DI}, true>",
+                      K * (nG + E) * 2.0 * 8.0 * nI * mbI, K * (nG + E)),
+            "schur": (f"crt_cond_dense_kernel<{NI}>", K * E * (4.0 / 3.0 * nG ** 3 + 16.0 * nG ** 2), K * E),
+            "product": ("crt_cond_gemm_kernel", K * E * 8.0 * nI * nG, K * E),
+        }
+        out = {}
+        for key, (kernel, flop, units) in ent.items():
+            ms = phase[key]
+            if ms <= 0:
+                continue
+            a = flop / (ms * 1e-3) / 1e12
+            out[key] = dict(bound="mfma", achieved=round(a, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                            frac=round(a / FP64_PEAK_TFLOPS, 4), traffic=measured_traffic(kernel.split(" + ")[0],
+                                                                                          args.workload),
+                            kernel=kernel, flop_per_launch=flop, units_per_launch=units, launch_ms=round(ms, 4))
+        return info, out
+
+    ctx.set_forward_method(args.forward_method)
+    ms_step, phase = timed(args.steps, args.warmup)
+    method = ctx.forward_method()
+    if method == 1:
+        info, rl = condensed_rooflines(phase)
+        top = max(rl, key=lambda k: rl[k]["launch_ms"])
+        roofline = rl[top]
+        others = {k: v for k, v in rl.items() if k != top}
+        others.update({k: v for k, v in other_rooflines(g, s, K, E, nsys, phase, args.forward_precision).items()
+                       if k.startswith("Jacobian")})
     else:
-        kernel = f"crt_factor16{'la' if d16 <= 10 else 'st'}_kernel<{d16}>"
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
-                    frac=round(achieved / peak, 4),
-                    traffic=measured_traffic(kernel, args.workload),
-                    kernel=kernel, flop_per_system=flop_sys, systems_per_launch=nsys,
-                    launch_ms=round(fac_ms, 4))
+        info = None
+        roofline = band_roofline(phase)
+        others = other_rooflines(g, s, K, E, nsys, phase, args.forward_precision)
 
     out = {
         "metric": "forward+Jacobian solves/s",
@@ -696,12 +759,26 @@ def main():
                                "Wenner, layered model with three blocks, forward + bsensi"}[args.workload],
             "elements": g.elanz, "nodes": g.sanz, "mb": g.mb, "electrodes": E, "measurements": s.nanz,
             "wavenumbers": K, "systems_per_step": nsys, "parallelism": "1 GPU",
+            "forward_method": "condensed" if method == 1 else "band per system",
         },
         "forward_systems_per_s": round(nsys * 1e3 / ms_step, 1),
         "phase_ms": {k: round(v, 4) for k, v in phase.items()},
         "roofline": roofline,
-        "roofline_other_kernels": other_rooflines(g, s, K, E, nsys, phase, args.forward_precision),
+        "roofline_other_kernels": others,
     }
+    if info is not None:
+        out["condensation"] = info
+    if method == 1 and args.forward_precision == 64:
+        # the reference's algorithm on the same workload: every system factorised
+        ctx.set_forward_method(0)
+        try:
+            ms_b, phase_b = timed(max(2, args.steps // 2), 1)
+            out["band_per_system"] = {"ms_per_step": round(ms_b, 3), "value": round(1e3 / ms_b, 4),
+                                      "phase_ms": {k: round(v, 4) for k, v in phase_b.items()},
+                                      "roofline": band_roofline(phase_b),
+                                      "speedup_of_condensed": round(ms_b / ms_step, 3)}
+        finally:
+            ctx.set_forward_method(args.forward_method)
     if args.forward_precision == 32:
         sweeps, ratio, fallbacks = ctx.refine_info()
         out["dtype"] = "c64 factorisation, c128 refinement"

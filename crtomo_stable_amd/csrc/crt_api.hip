@@ -55,7 +55,10 @@ int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, i
                           const zd *, zd *, zd *);
 int crt_cond_dense_max_tiles();
 int crt_launch_cond_dense(hipStream_t, const DevGrid &, int, int, int, int, int, int, const zd *, const zd *,
-                          const int *, const int *, const int *, const double *, zd *, zd *, zd *, zd *, int *);
+                          const int *, const int *, const zd *, int, const int *, const int *, const double *, zd *,
+                          zd *, zd *, zd *, int *);
+int crt_launch_cond_bval(hipStream_t, const DevGrid &, int, int, int, int, int, const int *, const int2 *,
+                         const double *, zd *);
 int crt_launch_cond_gemm(hipStream_t, int, int, int, int, int, int, int, int, int, const zd *, const zd *,
                          const double *, const int *, zd *);
 // model uncertainty (crt_bmcm.hip)
@@ -171,7 +174,7 @@ struct DBuf {
     }
 };
 
-enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_SCHUR, T_N };
+enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_SCHUR, T_PRODUCT, T_N };
 
 }  // namespace
 
@@ -284,8 +287,10 @@ struct crt_ctx {
         std::vector<int> nodeI, nodeG;
         DBuf<int> d_row_ptr, d_col, d_crow, d_zero, d_tmap, d_qmap, d_enr0;
         DBuf<int> d_tptr, d_erow, d_ecol, d_ecolg, d_eq, d_j0;
-        DBuf<int> d_gi_ptr, d_gi_row, d_gi_q, d_gmap, d_qdiag, d_nodeI, d_nodeG, d_eG;
-        DBuf<zd> areaI, vals, panels, yI, dinv, X, S0, Gr, XG, Lt, Li, rv;
+        DBuf<int> d_gi_ptr, d_gi_row, d_gi_q, d_gmap, d_qdiag, d_nodeI, d_nodeG, d_eG, d_bptr, d_bent, d_bq;
+        DBuf<int2> d_bg;
+        int nb = 0;
+        DBuf<zd> areaI, vals, panels, yI, dinv, X, S0, Gr, XG, Lt, Li, rv, bval;
         DBuf<double> fak, dga, fG;
         DBuf<int2> sysI;
         DBuf<int> stI, stD;
@@ -1271,6 +1276,44 @@ static int cond_plan(crt_ctx *c) {
     }
     for (int g = 0; g < P.nG; ++g)
         if (qdiag[g] < 0) return CRT_ERR_STATE;
+    // entries that hold boundary terms (the node pairs of the type-11 elements), per
+    // lower tile of the dense boundary matrix: (element c*16 + r of the tile, entry)
+    const size_t ntri = (size_t)P.nt * (P.nt + 1) / 2;
+    struct BE { int e, q, g1, g2; };
+    std::vector<std::vector<BE>> bt(ntri);
+    {
+        std::vector<char> seen((size_t)nGp * nGp, 0);
+        int iel = 0;
+        for (int t = 0; t < c->typanz; ++t)
+            for (int j = 0; j < c->nelanz[t]; ++j, ++iel) {
+                if (c->typ[t] != 11) continue;
+                const int ns = c->selanz[t];
+                for (int a = 0; a < ns; ++a)
+                    for (int b = 0; b <= a; ++b) {
+                        int g1 = gidx[c->nrel[(size_t)iel * c->maxn + a] - 1];
+                        int g2 = gidx[c->nrel[(size_t)iel * c->maxn + b] - 1];
+                        if (g1 < g2) std::swap(g1, g2);
+                        if (seen[(size_t)g2 * nGp + g1]) continue;
+                        seen[(size_t)g2 * nGp + g1] = 1;
+                        const int q = gmap[(size_t)g2 * nGp + g1];
+                        if (q < 0) return CRT_ERR_STATE;
+                        const size_t ti = (size_t)(g1 / 16) * (g1 / 16 + 1) / 2 + g2 / 16;
+                        bt[ti].push_back({(g2 % 16) * 16 + g1 % 16, q, g1, g2});
+                    }
+            }
+    }
+    std::vector<int> bptr(ntri + 1, 0), bent, bq;
+    std::vector<int2> bg;
+    for (size_t ti = 0; ti < ntri; ++ti) {
+        for (auto &b : bt[ti]) {
+            bent.push_back(b.e);
+            bq.push_back(b.q);
+            bg.push_back(int2{b.g1, b.g2});
+        }
+        bptr[ti + 1] = (int)bent.size();
+    }
+    P.nb = (int)bent.size();
+    if (bent.empty()) { bent.push_back(0); bq.push_back(0); bg.push_back(int2{0, 0}); }
     // sources: U column l of an interior electrode node, b_G entry of a boundary one
     P.E = c->eanz;
     P.Ep = (P.E + 15) / 16 * 16;
@@ -1333,6 +1376,10 @@ static int cond_plan(crt_ctx *c) {
     e = e ? e : P.d_nodeI.upload(P.nodeI.data(), P.nodeI.size(), st);
     e = e ? e : P.d_nodeG.upload(P.nodeG.data(), P.nodeG.size(), st);
     e = e ? e : P.d_eG.upload(eG.data(), eG.size(), st);
+    e = e ? e : P.d_bptr.upload(bptr.data(), bptr.size(), st);
+    e = e ? e : P.d_bent.upload(bent.data(), bent.size(), st);
+    e = e ? e : P.d_bq.upload(bq.data(), bq.size(), st);
+    e = e ? e : P.d_bg.upload(bg.data(), bg.size(), st);
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(st));
     P.ok = true;
@@ -1359,8 +1406,8 @@ static int forward_condensed(crt_ctx *c) {
     const int nGp = P.nGp, nt = P.nt, ldx = P.nblkI * 16, D = P.DI;
     const size_t fstrideI = (size_t)P.nblkI * 16, ntri = (size_t)nt * (nt + 1) / 2;
     const size_t per_k = sizeof(zd) * ((size_t)P.ncol * ldx + (size_t)P.nblkI * (D + 1) * 256 + 2 * fstrideI +
-                                       (size_t)P.nnzI * 2 + 2 + (size_t)nGp * nGp + 2 * (size_t)E * nGp +
-                                       (size_t)E * (ntri + nt) * 256 + P.nent) +
+                                       (size_t)P.nnzI * 2 + 2 + ntri * 256 + 2 * (size_t)E * nGp +
+                                       (size_t)E * (ntri + nt) * 256 + P.nent + (size_t)E * P.nb) +
                          sizeof(double) * (2 * fstrideI + nGp);
     const int kc = (int)std::max<size_t>(1, std::min<size_t>(K, c->panel_budget / per_k));
     e = P.areaI.ensure((size_t)kc * P.nnzI);
@@ -1373,11 +1420,12 @@ static int forward_condensed(crt_ctx *c) {
     e = e ? e : P.fG.ensure((size_t)kc * nGp);
     e = e ? e : P.rv.ensure((size_t)kc * std::max(P.nent, 1));
     e = e ? e : P.X.ensure((size_t)kc * P.ncol * ldx);
-    e = e ? e : P.S0.ensure((size_t)kc * nGp * nGp);
+    e = e ? e : P.S0.ensure((size_t)kc * ntri * 256);
     e = e ? e : P.Gr.ensure((size_t)kc * E * nGp);
     e = e ? e : P.XG.ensure((size_t)kc * E * nGp);
     e = e ? e : P.Lt.ensure((size_t)kc * E * ntri * 256);
     e = e ? e : P.Li.ensure((size_t)kc * E * nt * 256);
+    e = e ? e : P.bval.ensure((size_t)kc * E * std::max(P.nb, 1));
     e = e ? e : P.sysI.ensure(kc);
     e = e ? e : P.stI.ensure(K);
     e = e ? e : P.stD.ensure((size_t)K * E);
@@ -1398,8 +1446,12 @@ static int forward_condensed(crt_ctx *c) {
     gI.tmap = P.d_tmap.p; gI.tmapD = D;
     gI.area = P.areaI.p;
     gI.dmask = nullptr;
-    const int variant = D <= 10 ? 1 : 2;
-    double tdiag = 0, tfac = 0, tsol = 0, tsch = 0;
+    int variant = D <= 10 ? 1 : 2;
+    if (const char *ev = getenv("CRT_COND_VARIANT")) {
+        const int v = atoi(ev);
+        if ((v == 1 && D <= 10) || (v == 2 && D >= 3)) variant = v;
+    }
+    double tdiag = 0, tfac = 0, tsol = 0, tsch = 0, tprod = 0;
     for (int k0 = 0; k0 < K; k0 += kc) {
         const int n = std::min(kc, K - k0);
         c->tic(T_DIAG);
@@ -1428,23 +1480,26 @@ static int forward_condensed(crt_ctx *c) {
         if (e) return e;
         c->toc(T_SOLVE);
         c->tic(T_SCHUR);
-        e = crt_launch_cond_dense(st, g, P.nG, nGp, E, K, k0, n * E, P.S0.p, P.Gr.p, P.d_gmap.p, P.d_qdiag.p,
-                                  P.d_nodeG.p, P.fG.p, P.Lt.p, P.Li.p, P.XG.p, c->d_kpotN.p, P.stD.p + (size_t)k0 * E);
+        e = crt_launch_cond_bval(st, g, n * E, E, k0, nGp, P.nb, P.d_bq.p, P.d_bg.p, P.fG.p, P.bval.p);
+        e = e ? e : crt_launch_cond_dense(st, g, P.nG, nGp, E, K, k0, n * E, P.S0.p, P.Gr.p, P.d_bptr.p, P.d_bent.p,
+                                          P.bval.p, P.nb, P.d_qdiag.p, P.d_nodeG.p, P.fG.p, P.Lt.p, P.Li.p, P.XG.p,
+                                          c->d_kpotN.p, P.stD.p + (size_t)k0 * E);
         if (e) return e;
         c->toc(T_SCHUR);
-        c->tic(T_REFINE);
+        c->tic(T_PRODUCT);
         e = crt_launch_cond_gemm(st, n, k0, P.nI, ldx, P.ntile, nGp, E, K, (int)fstrideI, P.X.p, P.XG.p, P.fak.p,
                                  P.d_nodeI.p, c->d_kpotN.p);
         if (e) return e;
-        c->toc(T_REFINE);
+        c->toc(T_PRODUCT);
         CRT_HIP(hipStreamSynchronize(st));
         c->collect(T_DIAG); tdiag += c->times[T_DIAG];
         c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
         c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
         c->collect(T_SCHUR); tsch += c->times[T_SCHUR];
-        c->collect(T_REFINE); tsol += c->times[T_REFINE];
+        c->collect(T_PRODUCT); tprod += c->times[T_PRODUCT];
     }
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol; c->times[T_SCHUR] = tsch;
+    c->times[T_PRODUCT] = tprod;
     c->times[T_REFINE] = 0.0;
     std::vector<int> sI(K), sD((size_t)K * E);
     CRT_HIP(c->copy_out(sI.data(), P.stI.p, K * sizeof(int)));
@@ -1466,6 +1521,14 @@ int crt_set_forward_method(crt_ctx *c, int method) {
 int crt_get_forward_method(crt_ctx *c, int *method) {
     if (!c || !method) return CRT_ERR_ARG;
     *method = c->last_method;
+    return 0;
+}
+
+int crt_get_condense_info(crt_ctx *c, int *info) {
+    if (!c || !info) return CRT_ERR_ARG;
+    const auto &P = c->cond;
+    const int v[6] = {P.ok ? 1 : 0, P.nI, P.nG, P.mbI, P.DI, P.order};
+    for (int i = 0; i < 6; ++i) info[i] = v[i];
     return 0;
 }
 
@@ -1493,7 +1556,7 @@ int crt_forward(crt_ctx *c) {
         return 0;
     }
     c->last_method = 0;
-    c->times[T_SCHUR] = 0.0;
+    c->times[T_SCHUR] = c->times[T_PRODUCT] = 0.0;
     // 16x16 matrix-core tiles when the band fits (mb <= 256), else 8x8 LDS tiles
     const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
     const int NBt = use16 ? 16 : CRT_NB;

@@ -128,10 +128,12 @@ __global__ void __launch_bounds__(256) crt_cond_rhs_kernel(int kc, int k0, int n
 // dinv the reciprocal pivots.  Per block J the products with the D off-diagonal
 // tiles are split over the waves (wave 0 takes a = 1, waves 1..3 the rest), the
 // partial sums meet in LDS, and wave 0 applies the inverse diagonal tile on the
-// matrix cores; the last D + 1 result tiles stay in an LDS ring.  The forward pass
-// reads its right-hand side from the sparse entry lists (sorted by row) and starts
-// at the tile's first nonzero block j0; X above it is zero and is not written (the
-// backward pass reads zeros there).
+// matrix cores; the last D result tiles stay in an LDS ring (slot J mod D is rewritten
+// after the barrier of block J, when its old tile J - D has been read).  The forward
+// pass takes its right-hand side from the sparse entry lists (sorted by row; each
+// entry is handed to the lane that holds its element) and starts at the tile's first
+// nonzero block j0; X above it is zero and is not written (the backward pass reads
+// zeros there).  At D <= 7 the 52 KB of LDS let three workgroups share a CU.
 struct TrsmArgs {
     int nblk, ldx, ntile, kc, nvirt;
     const zd *panels;   // [kk][nblk][D+1][256]
@@ -146,12 +148,11 @@ struct TrsmArgs {
 };
 
 template <int D, bool BWD>
-__global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(TrsmArgs A) {
-    constexpr int R = D + 1;                       // ring slots
+__global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_trsm_kernel(TrsmArgs A) {
+    constexpr int R = D;                              // ring slots
     constexpr int NA = D >= 2 ? (D - 1 + 2) / 3 : 1;  // tiles per worker wave (a = 2 + w-1 + 3u)
     __shared__ zd ring[R][256];
     __shared__ zd part[2][3][256];
-    __shared__ zd rt[256];
     // XCD-aware order: consecutive virtual ids (tiles of one wavenumber) on one XCD
     const int b = blockIdx.x;
     const int per = A.nvirt / 8;
@@ -165,9 +166,7 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
     const zd *dinv = A.dinv + (size_t)kk * nblk * T16;
     zd *X = A.X + ((size_t)kk * A.ntile + t) * T16 * A.ldx;
     const int J0 = A.j0[t];
-    const zd zero = zd{0.0, 0.0};
-    for (int i = threadIdx.x; i < R * 256; i += 256) (&ring[0][0])[i] = zero;
-    if (threadIdx.x < 256) rt[threadIdx.x] = zero;
+    for (int i = threadIdx.x; i < R * 256; i += 256) (&ring[0][0])[i] = zd{};
     __syncthreads();
     if (J0 >= nblk && !BWD) return;
 
@@ -179,7 +178,7 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
         const zd *tp = panel + ((size_t)(live ? Jt : 0) * (D + 1) + a) * 256;
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
-            o[ks] = live ? (BWD ? ldz(tp + 64 * ks + lane) : ldz(tp + jc * T16 + 4 * ks + rq)) : zero;
+            o[ks] = live ? (BWD ? ldz(tp + 64 * ks + lane) : ldz(tp + jc * T16 + 4 * ks + rq)) : zd{};
     };
     // inverse diagonal tile: forward Linv(i, k), backward Linv(k, i)
     auto load_diag = [&](int J, zd (&o)[4]) {
@@ -188,7 +187,7 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             const int k = 4 * ks + rq;
-            zd e = zero;
+            zd e = zd{};
             if (live) {
                 if (jc == k) e = ldz(dinv + (size_t)J * T16 + jc);
                 else if (BWD ? (k > jc) : (jc > k)) e = BWD ? ldz(tp + jc * T16 + k) : ldz(tp + 64 * ks + lane);
@@ -199,15 +198,24 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
 
     if (wave == 0) {
         zd a1[4], a1n[4], dg[4], dgn[4];
-        zd xin[4] = {zero, zero, zero, zero}, xinn[4];
-        int cur = BWD ? 0 : A.tptr[t];
+        zd xin[4] = {zd{}, zd{}, zd{}, zd{}}, xinn[4];
+        int wb = BWD ? 0 : A.tptr[t], wc = 0;
         const int cend = BWD ? 0 : A.tptr[t + 1];
         const zd *rv = BWD ? nullptr : A.rv + (size_t)kk * A.nent;
+        int wrow = INT32_MAX, wcol = 0;
+        zd wv = zd{};
+        auto load_window = [&]() {
+            const int p = wb + lane;
+            wrow = p < cend ? A.erow[p] : INT32_MAX;
+            wcol = p < cend ? A.ecol[p] : 0;
+            wv = p < cend ? rv[p] : zd{};
+        };
+        if (!BWD) load_window();
         auto load_x = [&](int J, zd (&o)[4]) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 o[u] = (J >= J0 && J >= 0 && J < nblk) ? ldz(X + (size_t)jc * A.ldx + (size_t)J * T16 + 4 * u + rq)
-                                                       : zero;
+                                                       : zd{};
         };
         const int Js = BWD ? nblk - 1 : J0;
         const int step = BWD ? -1 : 1;
@@ -224,38 +232,42 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) cmma(cr, ci, a1[ks], B[64 * ks + lane]);
             }
-            // right-hand-side entries of block J (forward)
-            int nb = 0;
+            // right-hand-side entries of block J (forward): a window of 64 entries sits in
+            // registers (lane i holds entry wb + i, wc of them used), each entry goes to
+            // the lane that holds element (r, c) = (4u + rq, jc)
+            zd T[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) T[u] = BWD ? xin[u] : zd{};
             if (!BWD) {
                 const int rend = (J + 1) * T16;
                 for (;;) {
-                    const int p = cur + lane;
-                    const bool in = p < cend && A.erow[p] < rend;
-                    const unsigned long long m = __ballot(in);
-                    if (in) rt[(A.erow[p] - J * T16) * T16 + A.ecol[p]] = rv[p];
-                    const int n = __popcll(m);
-                    cur += n;
-                    nb += n;
-                    if (n < 64) break;
+                    const bool in = lane >= wc && wrow < rend;
+                    const int n = __popcll(__ballot(in));
+                    for (int i = wc; i < wc + n; ++i) {
+                        const int ri = __shfl(wrow, i, 64) - J * T16, ci_ = __shfl(wcol, i, 64);
+                        const double vr = __shfl(wv.re, i, 64), vi = __shfl(wv.im, i, 64);
+                        const bool mine = jc == ci_ && rq == (ri & 3);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (mine && (ri >> 2) == u) {
+                                T[u].re += vr;
+                                T[u].im += vi;
+                            }
+                    }
+                    wc += n;
+                    if (wc < 64) break;
+                    wb += 64;
+                    wc = 0;
+                    load_window();
                 }
-                __builtin_amdgcn_wave_barrier();
             }
             lds_barrier();  // the worker waves' partial sums of block J
             const int pb = J & 1;
-            zd T[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int o = 64 * u + lane;
-                zd s = BWD ? xin[u] : rt[o];
-                s.re -= cr[u] + part[pb][0][o].re + part[pb][1][o].re + part[pb][2][o].re;
-                s.im -= ci[u] + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
-                T[u] = s;
-            }
-            if (!BWD && nb) {
-                __builtin_amdgcn_wave_barrier();
-                // clear the entries of this block again (rows of block J only)
-                for (int p = cur - nb + lane; p < cur; p += 64) rt[(A.erow[p] - J * T16) * T16 + A.ecol[p]] = zero;
-                __builtin_amdgcn_wave_barrier();
+                T[u].re -= cr[u] + part[pb][0][o].re + part[pb][1][o].re + part[pb][2][o].re;
+                T[u].im -= ci[u] + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
             }
             // X_J = Linv T (forward) or Linv^T T (backward); T in the result layout is
             // the B operand of step ks = element ks
@@ -289,7 +301,7 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
         if (a <= D) load_off(Js, a, cur[u]);
         else
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) cur[u][ks] = zero;
+            for (int ks = 0; ks < 4; ++ks) cur[u][ks] = zd{};
     }
     for (int J = Js; BWD ? J >= 0 : J < nblk; J += step) {
 #pragma unroll
@@ -319,9 +331,11 @@ __global__ void __launch_bounds__(256, D <= 8 ? 2 : 1) crt_cond_trsm_kernel(Trsm
 }
 
 // ------------------------------------------------------ Schur complement
-// S0[kk](cg, c) = A_GG^area(cg, c) - sum_i A_GI(cg, i) Z(i, c)   (c < nGp)
+// S0[kk](cg, c) = A_GG^area(cg, c) - sum_i A_GI(cg, i) Z(i, c)   (lower tiles)
 // G[kk](cg, l)  = b_G(cg, l)       - sum_i A_GI(cg, i) U(i, l)   (column nGp + l of X)
-// scaled; padding rows and columns of S0 are the identity.  One thread per entry.
+// scaled; padding rows and columns of S0 are the identity.  S0 is written in the
+// dense kernel's layout: column-major 16 x 16 tiles packed by tile rows, tile (I, J),
+// I >= J, at I(I+1)/2 + J.  One thread per entry.
 struct SchurArgs {
     int kc, k0, nG, nGp, E, ldx, ntile, nnzF, fstrideI;
     const int *gi_ptr, *gi_row, *gi_q;  // A_GI by boundary row
@@ -334,19 +348,32 @@ struct SchurArgs {
 };
 
 __global__ void __launch_bounds__(256) crt_cond_schur_kernel(SchurArgs A) {
-    const int ncs = A.nGp + A.E;
+    const int nt = A.nGp / T16;
+    const long ns = (long)nt * (nt + 1) / 2 * 256, ng = (long)A.E * A.nGp, per = ns + ng;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)A.kc * ncs * A.nGp) return;
-    const int cg = (int)(t % A.nGp);
-    const long r = t / A.nGp;
-    const int col = (int)(r % ncs), kk = (int)(r / ncs);
+    if (t >= (long)A.kc * per) return;
+    const int kk = (int)(t / per);
+    const long o = t - (long)kk * per;
     const int k = A.k0 + kk;
-    const bool zpart = col < A.nGp;
-    const int xcol = col;  // X columns: Z (nGp), then U
+    const bool zpart = o < ns;
+    int cg, col;
+    if (zpart) {
+        const int tile = (int)(o / 256), e = (int)(o % 256);
+        int I = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+        while ((I + 1) * (I + 2) / 2 <= tile) ++I;
+        while (I * (I + 1) / 2 > tile) --I;
+        const int J = tile - I * (I + 1) / 2;
+        cg = 16 * I + e % 16;
+        col = 16 * J + e / 16;
+    } else {
+        const long og = o - ns;
+        cg = (int)(og % A.nGp);
+        col = A.nGp + (int)(og / A.nGp);  // X column nGp + l
+    }
     zd s = zd{0.0, 0.0};
     if (cg < A.nG) {
         const double fg = A.fG[(size_t)kk * A.nGp + cg];
-        const zd *Xc = A.X + ((size_t)kk * A.ntile * T16 + xcol) * A.ldx;
+        const zd *Xc = A.X + ((size_t)kk * A.ntile * T16 + col) * A.ldx;
         for (int p = A.gi_ptr[cg]; p < A.gi_ptr[cg + 1]; ++p) {
             const int i = A.gi_row[p];
             const zd a = A.area[(size_t)k * A.nnzF + A.gi_q[p]];
@@ -369,13 +396,29 @@ __global__ void __launch_bounds__(256) crt_cond_schur_kernel(SchurArgs A) {
     } else if (zpart && col == cg) {
         s = zd{1.0, 0.0};
     }
-    if (zpart) A.S0[((size_t)kk * A.nGp + col) * A.nGp + cg] = s;
+    if (zpart) A.S0[(size_t)kk * ns + o] = s;
     else A.Gr[((size_t)kk * A.E + (col - A.nGp)) * A.nGp + cg] = s;
+}
+
+// scaled boundary terms of every system: bval[s][p] = (type-11 part of entry bq[p] for
+// source l at wavenumber k) fG(g1[p]) fG(g2[p]), s = kk E + l
+__global__ void __launch_bounds__(256) crt_cond_bval_kernel(DevGrid g, int nsys, int E, int k0, int nGp, int nb,
+                                                            const int *__restrict__ bq, const int2 *__restrict__ bg,
+                                                            const double *__restrict__ fG, zd *__restrict__ bval) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nsys * nb) return;
+    const int s = (int)(t / nb), p = (int)(t - (long)s * nb);
+    const int l = s % E, kk = s / E;
+    const zd bp = boundary_part(g, bq[p], l, k0 + kk);
+    const int2 gg = bg[p];
+    const double f = fG[(size_t)kk * nGp + gg.x] * fG[(size_t)kk * nGp + gg.y];
+    bval[t] = zd{bp.re * f, bp.im * f};
 }
 
 // --------------------------------------------------- dense factor and solve
 // One workgroup (eight waves) per system (l, kk): S(l) = S0 + the boundary terms of source l
-// (scaled), factorised S = L L^T without conjugation by tile columns (left-looking:
+// (scaled: S0's tile plus the tile's list of entries that hold boundary terms),
+// factorised S = L L^T without conjugation by tile columns (left-looking:
 // the tiles of column J are accumulated in registers over the row J tiles staged in
 // LDS, wave 0 factors the diagonal tile, each wave divides its tiles by L_JJ^T), then
 // L L^T x = g by substitution.  Tiles are column-major 16 x 16, packed by tile rows:
@@ -386,7 +429,9 @@ struct DenseArgs {
     DevGrid g;  // full grid: boundary contributions, area sums
     int nG, nGp, nt, E, K, k0;
     const zd *S0, *Gr;
-    const int *gmap, *qdiag, *nodeG;
+    const int *bptr, *bent, *qdiag, *nodeG;  // boundary entries of each tile: element c*16 + r
+    const zd *bval;                          // [s][nb] their values for each system
+    int nb;
     const double *fG;
     zd *Lt, *Li, *XG, *kpotN;
     int *status;  // [kk][E]
@@ -408,6 +453,7 @@ struct alignas(16) DenseLds {
 
 template <int NI>
 __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A) {
+    constexpr int DQ = NI >= 3 ? 4 : 8;  // tiles per batch of loads
     __shared__ DenseLds m;
     extern __shared__ zd yv[];  // nGp
     const int s = blockIdx.x;
@@ -419,10 +465,10 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     const size_t ntri = (size_t)nt * (nt + 1) / 2;
     zd *Ls = A.Lt + (size_t)s * ntri * 256;
     zd *Lis = A.Li + (size_t)s * nt * 256;
-    const zd *S0 = A.S0 + (size_t)kk * nGp * nGp;
+    const zd *S0 = A.S0 + (size_t)kk * ntri * 256;
+    const zd *bval = A.bval + (size_t)s * A.nb;
     const double *fG = A.fG + (size_t)kk * nGp;
     const DevGrid &g = A.g;
-    const zd zero = zd{0.0, 0.0};
     if (threadIdx.x == 0) m.err = 0;
     __syncthreads();
     auto tri = [](int I, int J) { return (size_t)I * (I + 1) / 2 + J; };
@@ -446,24 +492,34 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
 #pragma unroll
         for (int u = 0; u < NI; ++u) {
             const int I = J + wave + DNW * u;
+            const zd *st = S0 + tri(I < nt ? I : J, J) * 256;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                zd e = zero;
-                if (I < nt) {
-                    const int r = 16 * I + 4 * v + rq, c = 16 * J + jc;
-                    e = S0[(size_t)c * nGp + r];
-                    if (r < nG && c < nG && r >= c) {
-                        const int q = A.gmap[(size_t)c * nGp + r];
-                        if (q >= 0) {
-                            const zd bp = boundary_part(g, q, l, k);
-                            const double f = fG[r] * fG[c];
-                            e.re += bp.re * f;
-                            e.im += bp.im * f;
-                        }
-                    }
-                }
+                const zd e = I < nt ? ldz(st + jc * T16 + 4 * v + rq) : zd{};
                 ar[u][v] = e.re;
                 ai[u][v] = e.im;
+            }
+            if (I < nt) {
+                // the tile's boundary terms of source l: lane i reads entry p0 + i, then
+                // each goes to the lane that holds its element (r, c) = (4v + rq, jc)
+                const int ti = (int)tri(I, J);
+                const int pe = A.bptr[ti + 1];
+                for (int p0 = A.bptr[ti]; p0 < pe; p0 += 64) {
+                    const int p = p0 + lane, n = min(64, pe - p0);
+                    const int ev = p < pe ? A.bent[p] : 0;
+                    const zd bv = p < pe ? bval[p] : zd{};
+                    for (int i = 0; i < n; ++i) {
+                        const int e = __shfl(ev, i, 64), r = e % T16, c = e / T16;
+                        const double vr = __shfl(bv.re, i, 64), vi = __shfl(bv.im, i, 64);
+                        const bool mine = lane == c + 16 * (r % 4);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (mine && v == r / 4) {
+                                ar[u][v] += vr;
+                                ai[u][v] += vi;
+                            }
+                    }
+                }
             }
         }
         for (int p0 = 0; p0 < J; p0 += DCH) {
@@ -472,24 +528,26 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             const zd *src = Ls + tri(J, p0) * 256;
             for (int i = threadIdx.x; i < np * 256; i += 64 * DNW) (&m.rowJ[0][0])[i] = ldz(src + i);
             __syncthreads();
-            for (int pp = 0; pp < np; ++pp) {
-                zd bo[4];
+            // per owned tile: the chunk's tiles L(I, p0..) (contiguous) loaded DQ at a time,
+            // all in flight together, then the products
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) bo[ks] = m.rowJ[pp][64 * ks + lane];
-                const int p = p0 + pp;
+            for (int u = 0; u < NI; ++u) {
+                const int I = J + wave + DNW * u;
+                if (I < nt) {
+                    const zd *tp = Ls + tri(I, p0) * 256;
+                    for (int pq = 0; pq < np; pq += DQ) {
+                        zd ao[DQ][4];
 #pragma unroll
-                for (int u = 0; u < NI; ++u) {
-                    const int I = J + wave + DNW * u;
-                    if (I < nt) {
-                        const zd *tp = Ls + tri(I, p) * 256;
-                        zd ao[4];
+                        for (int j = 0; j < DQ; ++j)
 #pragma unroll
-                        for (int ks = 0; ks < 4; ++ks) ao[ks] = ldz(tp + 64 * ks + lane);
+                            for (int ks = 0; ks < 4; ++ks)
+                                ao[j][ks] = pq + j < np ? ldz(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
 #pragma unroll
-                        for (int ks = 0; ks < 4; ++ks) {
-                            const zd na = zd{-ao[ks].re, -ao[ks].im};
-                            cmma(ar[u], ai[u], na, bo[ks]);
-                        }
+                        for (int j = 0; j < DQ; ++j)
+                            if (pq + j < np)
+#pragma unroll
+                                for (int ks = 0; ks < 4; ++ks)
+                                    cmma(ar[u], ai[u], zd{-ao[j][ks].re, -ao[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
                     }
                 }
             }
@@ -528,7 +586,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             // inverse of the lower triangle by rows, X(r, j) = (e_r - sum_{c<r} L(r,c) X(c, j)) / L(r,r):
             // lane (j, q) sums the terms c = q (mod 4)
             for (int r = 0; r < T16; ++r) {
-                zd sacc = zero;
+                zd sacc = zd{};
                 for (int c = rq; c < r; c += 4) sacc = zfma(sacc, m.dt[c * T16 + r], m.li[jc * T16 + c]);
                 sacc.re += __shfl_xor(sacc.re, 16, 64);
                 sacc.im += __shfl_xor(sacc.im, 16, 64);
@@ -543,7 +601,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int e = 64 * u + lane, c = e / T16, r = e % T16;
-                stz(dst + e, r >= c ? m.dt[e] : zero);
+                stz(dst + e, r >= c ? m.dt[e] : zd{});
                 stz(ldst + e, m.li[e]);
             }
         }
@@ -578,7 +636,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     __syncthreads();
     for (int J = 0; J < nt; ++J) {
         // lane (r, q) of step w holds L(r, 4w + q)
-        zd acc = zero;
+        zd acc = zd{};
         for (int p = wave; p < J; p += DNW) {
             const zd *tp = Ls + tri(J, p) * 256;
 #pragma unroll
@@ -600,7 +658,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             }
             __builtin_amdgcn_wave_barrier();
             const zd *lt = Lis + (size_t)J * 256;
-            zd y = zero;
+            zd y = zd{};
 #pragma unroll
             for (int w = 0; w < 4; ++w) y = zfma(y, ldz(lt + 64 * w + lane), m.ws[0][4 * w + rq]);
             y.re += __shfl_xor(y.re, 16, 64);
@@ -613,7 +671,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     }
     for (int J = nt - 1; J >= 0; --J) {
         // out(c) = sum_r L(I,J)(r, c) x(16I + r): lane (r, q) of step w holds L(r, 4w + q)
-        zd acc[4] = {zero, zero, zero, zero};
+        zd acc[4] = {zd{}, zd{}, zd{}, zd{}};
         for (int I = J + 1 + wave; I < nt; I += DNW) {
             const zd *tp = Ls + tri(I, J) * 256;
             const zd xr = yv[16 * I + jc];
@@ -692,7 +750,6 @@ __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
     const int k = A.k0 + kk;
     const zd *X = A.X + (size_t)kk * A.ntile * T16 * A.ldx;
     const zd *XG = A.XG + (size_t)kk * A.E * A.nGp;
-    const zd zero = zd{0.0, 0.0};
     d4 cr[4], ci[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) cr[c] = ci[c] = d4{0.0, 0.0, 0.0, 0.0};
@@ -707,7 +764,7 @@ __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
             if (16 * (4 * cgp + c) < A.E) {
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
-                    const zd bo = lcol < A.E ? ldz(XG + (size_t)lcol * A.nGp + 16 * kt + 4 * ks + rq) : zero;
+                    const zd bo = lcol < A.E ? ldz(XG + (size_t)lcol * A.nGp + 16 * kt + 4 * ks + rq) : zd{};
                     cmma(cr[c], ci[c], ao[ks], bo);
                 }
             }
@@ -792,7 +849,8 @@ int crt_launch_cond_schur(hipStream_t st, int kc, int k0, int nG, int nGp, int E
                           const int *eG, const zd *area, const double *fI, const double *fG, const zd *X, zd *S0,
                           zd *Gr) {
     SchurArgs a{kc, k0, nG, nGp, E, ldx, ntile, nnzF, fstrideI, gi_ptr, gi_row, gi_q, gmap, eG, area, fI, fG, X, S0, Gr};
-    const long n = (long)kc * (nGp + E) * nGp;
+    const long nt = nGp / T16;
+    const long n = (long)kc * (nt * (nt + 1) / 2 * 256 + (long)E * nGp);
     hipLaunchKernelGGL(crt_cond_schur_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
@@ -808,11 +866,13 @@ static int launch_dense(hipStream_t st, const DenseArgs &a, int nsys) {
 }
 
 int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int nG, int nGp, int E, int K, int k0, int nsys,
-                          const zd *S0, const zd *Gr, const int *gmap, const int *qdiag, const int *nodeG,
-                          const double *fG, zd *Lt, zd *Li, zd *XG, zd *kpotN, int *status) {
+                          const zd *S0, const zd *Gr, const int *bptr, const int *bent, const zd *bval, int nb,
+                          const int *qdiag, const int *nodeG, const double *fG, zd *Lt, zd *Li, zd *XG, zd *kpotN,
+                          int *status) {
     const int nt = nGp / T16;
     if (nt < 1 || nt > crt_cond_dense_max_tiles() || nGp % T16) return CRT_ERR_ARG;
-    DenseArgs a{g, nG, nGp, nt, E, K, k0, S0, Gr, gmap, qdiag, nodeG, fG, Lt, Li, XG, kpotN, status};
+    DenseArgs a{g,  nG,    nGp,  nt, E,     K,  k0, S0, Gr, bptr, bent, qdiag, nodeG, bval,
+                nb, fG,    Lt,   Li, XG,    kpotN, status};
     switch ((nt + DNW - 1) / DNW) {
         case 1: return launch_dense<1>(st, a, nsys);
         case 2: return launch_dense<2>(st, a, nsys);
@@ -821,6 +881,15 @@ int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int nG, int nGp, int
         case 5: return launch_dense<5>(st, a, nsys);
         default: return CRT_ERR_UNSUPPORTED;
     }
+}
+
+int crt_launch_cond_bval(hipStream_t st, const DevGrid &g, int nsys, int E, int k0, int nGp, int nb, const int *bq,
+                         const int2 *bg, const double *fG, zd *bval) {
+    const long n = (long)nsys * nb;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(crt_cond_bval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, E, k0, nGp,
+                       nb, bq, bg, fG, bval);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 int crt_launch_cond_gemm(hipStream_t st, int kc, int k0, int nI, int ldx, int ntile, int nGp, int E, int K,

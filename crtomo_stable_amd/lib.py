@@ -352,6 +352,12 @@ class Context:
         _chk(self.lib.crt_get_forward_method(self._h, C.byref(m)), "crt_get_forward_method")
         return m.value
 
+    def condense_info(self) -> dict:
+        """The grid's condensation plan (after a condensed forward)."""
+        v = np.zeros(6, np.int32)
+        _chk(self.lib.crt_get_condense_info(self._h, v.ctypes.data_as(C.POINTER(C.c_int))), "crt_get_condense_info")
+        return dict(zip(("ok", "interior", "boundary", "band", "tiles", "order"), (int(x) for x in v)))
+
     def refine_info(self):
         """(sweeps, final correction ratio, batches redone in FP64) of the last forward."""
         sw, fb = C.c_int(), C.c_int()
@@ -427,10 +433,10 @@ class Context:
         return err, a, p
 
     def times(self):
-        ms = np.zeros(10)
-        self.lib.crt_get_times(self._h, _ptr(ms), 10)
-        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine", "schur"),
-                        ms))
+        ms = np.zeros(11)
+        self.lib.crt_get_times(self._h, _ptr(ms), 11)
+        return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine", "schur",
+                         "product"), ms))
 
 
 class Batch:

@@ -292,14 +292,19 @@ int crt_get_refine_info(crt_ctx *ctx, int *sweeps, double *ratio, int *fallbacks
  * crt_get_forward_method: the method of the last crt_forward. */
 int crt_set_forward_method(crt_ctx *ctx, int method);
 int crt_get_forward_method(crt_ctx *ctx, int *method);
+/* The condensation plan of the grid (made by the first condensed crt_forward):
+ * info[6] = applicable (0/1), interior nodes, boundary nodes, interior band width,
+ * interior band tiles of 16, interior numbering (0 the grid's, 1 by x, 2 by y). */
+int crt_get_condense_info(crt_ctx *ctx, int *info);
 
 /* Device time of the last call of each phase, milliseconds:
  * [0] diag, [1] factor, [2] solve, [3] voltages, [4] sensitivities,
  * [5] precal, [6] cg total, [7] cg matvec (mean per iteration),
  * [8] refinement sweeps of the FP32 forward (total), [9] the dense boundary
- * factorisations of the condensed forward (total).  The condensed forward counts its
- * interior band factorisation as [1] and its substitutions, Schur complements and
- * interior potentials as [2]. */
+ * factorisations and solves of the condensed forward (total), [10] its interior
+ * potentials (the product with the boundary solution).  The condensed forward counts
+ * its interior band factorisation as [1] and its multi-right-hand-side substitutions
+ * and Schur complements as [2]. */
 int crt_get_times(crt_ctx *ctx, double *ms, int n);
 
 /* Batched forward of many models of uneven size (crt_forward of every context,
