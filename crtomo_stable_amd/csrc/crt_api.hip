@@ -45,22 +45,24 @@ int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4
                            const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
 // forward by static condensation onto the mixed-boundary nodes (crt_condense.hip)
 int crt_launch_cond_gather(hipStream_t, int, int, int, int, const int *, const zd *, zd *);
-int crt_launch_cond_fg(hipStream_t, int, int, int, int, int, const int *, const zd *, double *);
-int crt_launch_cond_rhs(hipStream_t, int, int, int, int, int, int, const int *, const int *, const int *, const zd *,
-                        const double *, const double *, zd *);
-int crt_launch_cond_trsm(hipStream_t, int, int, int, int, int, int, const zd *, const zd *, zd *, const int *,
+int crt_launch_cond_fb(hipStream_t, int, int, int, int, int, int, int, const int *, const zd *, double *, int *);
+int crt_launch_cond_rhs(hipStream_t, int, int, int, int, int, int, int, const int *, const int *, const int *,
+                        const zd *, const double *, const double *, zd *);
+int crt_launch_cond_trsm(hipStream_t, int, int, int, int, int, int, int, const zd *, const zd *, zd *, const int *,
                          const int *, const int *, const zd *, int, const int *);
 int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, int, const int *, const int *,
-                          const int *, const int *, const int *, const zd *, const double *, const double *,
-                          const zd *, zd *, zd *);
+                          const int *, const int *, const int *, const int *, const int *, const int *, const int *,
+                          const zd *, const double *, const double *, const zd *, zd *, zd *);
 int crt_cond_dense_max_tiles();
-int crt_launch_cond_dense(hipStream_t, const DevGrid &, int, int, int, int, int, int, const zd *, const zd *,
-                          const int *, const int *, const zd *, int, const int *, const int *, const double *, zd *,
-                          zd *, zd *, zd *, int *);
+int crt_cond_prefix_max_tiles();
 int crt_launch_cond_bval(hipStream_t, const DevGrid &, int, int, int, int, int, const int *, const int2 *,
                          const double *, zd *);
-int crt_launch_cond_gemm(hipStream_t, int, int, int, int, int, int, int, int, int, const zd *, const zd *,
-                         const double *, const int *, zd *);
+int crt_launch_cond_dense(hipStream_t, const DevGrid &, int, int, int, int, int, int, int, int, int, const zd *,
+                          const zd *, const zd *, const int *, const int *, const int *, const int *, const zd *, int,
+                          const double *, zd *, zd *, zd *, zd *, zd *, zd *, zd *, int *, int);
+int crt_launch_cond_gemm(hipStream_t, int, int, int, int, int, int, int, int, int, int, const zd *, const zd *,
+                         const double *, const int *, const int *, const int *, const int *, const int *, const int *,
+                         zd *);
 // model uncertainty (crt_bmcm.hip)
 int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, int, const double *, const int *, double,
                  int, double *, double *, double *, double *, double *, double *, int *, int *);
@@ -282,16 +284,17 @@ struct crt_ctx {
     // static condensation plan of the grid (boundary nodes G, interior nodes I)
     struct Cond {
         bool built = false, ok = false;
-        int nI = 0, nG = 0, nGp = 0, nt = 0, mbI = 0, DI = 0, nblkI = 0, nnzI = 0, E = 0, Ep = 0, ncol = 0;
-        int ntile = 0, nent = 0, order = 0;
-        std::vector<int> nodeI, nodeG;
+        int P = 1;                                    // interior band pieces
+        int nI = 0, nG = 0, nS = 0, nSp = 0, nGp = 0, nBp = 0, JS = 0, nt = 0, ntB = 0;
+        int mbI = 0, DI = 0, nblkP = 0, nnzI = 0, E = 0, ncolP = 0, ntileP = 0, nent = 0, order = 0, nb = 0;
+        std::vector<int> nodeI, nodeB, roff, plen, nbc;
         DBuf<int> d_row_ptr, d_col, d_crow, d_zero, d_tmap, d_qmap, d_enr0;
-        DBuf<int> d_tptr, d_erow, d_ecol, d_ecolg, d_eq, d_j0;
-        DBuf<int> d_gi_ptr, d_gi_row, d_gi_q, d_gmap, d_qdiag, d_nodeI, d_nodeG, d_eG, d_bptr, d_bent, d_bq;
+        DBuf<int> d_tptr, d_erow, d_erowg, d_ecol, d_ebcol, d_eq, d_j0;
+        DBuf<int> d_gi_ptr, d_gi_row, d_gi_piece, d_gi_q, d_gmap, d_qdiag, d_qdiagG, d_nodeI, d_nodeB, d_eB;
+        DBuf<int> d_bcol, d_ucol, d_colmap, d_roff, d_plen, d_nbc, d_bptr, d_bent, d_bq;
         DBuf<int2> d_bg;
-        int nb = 0;
-        DBuf<zd> areaI, vals, panels, yI, dinv, X, S0, Gr, XG, Lt, Li, rv, bval;
-        DBuf<double> fak, dga, fG;
+        DBuf<zd> areaI, vals, panels, yI, dinv, X, T, S0, Gr, XB, Lt, Li, Lpre, Lipre, rv, bval;
+        DBuf<double> fak, dga, fB;
         DBuf<int2> sysI;
         DBuf<int> stI, stD;
     } cond;
@@ -1155,12 +1158,16 @@ static int forward_batch_f32(crt_ctx *c, const DevGrid &g, int nb, size_t s0, in
 
 // ------------------------------------------------ condensed forward: plan
 // The boundary nodes G are the nodes of the type-11 elements (their terms hold the
-// source through kg, src/kompab.f90:111-116), the interior nodes I the rest.  The
-// interior band is numbered in the order of smallest band width among the reference's
-// (CutMcK) numbering, the nodes sorted by x and the nodes sorted by y; the boundary
-// nodes by their first interior neighbour, so the right-hand sides of a 16-column tile
-// of [A_IG | b_I] start close together.  Not applicable (ok false): no boundary
-// nodes, an interior band of one tile or more than 16, more than 640 boundary nodes.
+// source through kg, src/kompab.f90:111-116); the other nodes form the interior band,
+// numbered in the order of smallest band width among the reference's (CutMcK)
+// numbering and the nodes sorted by x or by y.  The band is cut into P pieces by
+// windows of one band width (no entry reaches across a window): the windows are the
+// separators S, the pieces independent band systems, factorised side by side.  The
+// dense block is B = [S, G]: S is eliminated once per wavenumber, G per source.  G
+// is numbered by the first interior neighbour, and so are each piece's coupled B
+// columns, so the right-hand sides of a 16-column tile start close together.  Not
+// applicable (ok false): no boundary nodes, an interior band of one tile or more than
+// 16, more than 640 boundary nodes or 768 in B.
 static int cond_plan(crt_ctx *c) {
     auto &P = c->cond;
     if (P.built) return 0;
@@ -1177,12 +1184,11 @@ static int cond_plan(crt_ctx *c) {
                 for (int m = 0; m < c->selanz[t]; ++m) inG[c->nrel[(size_t)iel * c->maxn + m] - 1] = 1;
             }
     }
-    P.nodeI.clear();
-    P.nodeG.clear();
-    for (int i = 0; i < sanz; ++i) (inG[i] ? P.nodeG : P.nodeI).push_back(i);
-    P.nI = (int)P.nodeI.size();
-    P.nG = (int)P.nodeG.size();
-    if (P.nG == 0 || P.nI == 0) return 0;
+    std::vector<int> I0, G0;
+    for (int i = 0; i < sanz; ++i) (inG[i] ? G0 : I0).push_back(i);
+    const int nI0 = (int)I0.size();
+    P.nG = (int)G0.size();
+    if (P.nG == 0 || nI0 == 0) return 0;
     P.nGp = (P.nG + 15) / 16 * 16;
     P.nt = P.nGp / 16;
     if (P.nt > crt_cond_dense_max_tiles()) return 0;
@@ -1190,7 +1196,7 @@ static int cond_plan(crt_ctx *c) {
     std::vector<int> best, pos(sanz, -1);
     int bestw = INT32_MAX;
     for (int order = 0; order < 3; ++order) {
-        std::vector<int> ord = P.nodeI;
+        std::vector<int> ord = I0;
         if (order == 1)
             std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
                 return c->x[a] != c->x[b] ? c->x[a] < c->x[b] : c->y[a] < c->y[b];
@@ -1200,7 +1206,7 @@ static int cond_plan(crt_ctx *c) {
                 return c->y[a] != c->y[b] ? c->y[a] < c->y[b] : c->x[a] < c->x[b];
             });
         std::vector<int> ps(sanz, -1);
-        for (int i = 0; i < P.nI; ++i) ps[ord[i]] = i;
+        for (int i = 0; i < nI0; ++i) ps[ord[i]] = i;
         int w = 0;
         for (int q = 0; q < nnz; ++q) {
             const int a = ps[crow[q]], b = ps[col[q]];
@@ -1213,18 +1219,62 @@ static int cond_plan(crt_ctx *c) {
             P.order = order;
         }
     }
-    P.nodeI = best;
     P.mbI = bestw;
     P.DI = std::max(1, (bestw + 15) / 16);
     if (P.DI < 2 || P.DI > 16) return 0;
-    P.nblkI = (P.nI + 15) / 16;
-    // interior CSR (lower triangle, diagonal last) and its entries in the full CSR
+    // pieces: [start, start + len) of the interior positions, separators of width mbI between
+    int np = 2;
+    if (const char *ev = getenv("CRT_COND_PIECES")) np = std::max(1, atoi(ev));
+    const int w = bestw;
+    for (; np > 1; --np) {
+        const int L = (nI0 - (np - 1) * w) / np;
+        const int nS = (np - 1) * w, ntB = ((nS + 15) / 16 * 16 + P.nGp) / 16;
+        if (L >= 32 * w && ntB <= crt_cond_prefix_max_tiles()) break;
+    }
+    std::vector<int> pst, plen;
+    {
+        const int L = (nI0 - (np - 1) * w) / np;
+        int a = 0;
+        for (int p = 0; p < np; ++p) {
+            pst.push_back(a);
+            const int len = p == np - 1 ? nI0 - a : L;
+            plen.push_back(len);
+            a += len + w;
+        }
+    }
+    P.P = np;
+    std::vector<int> piece(sanz, -1), prow(sanz, -1);  // piece and combined row of each interior node
+    std::vector<int> S;                                 // separator nodes in position order
+    P.roff.assign(np, 0);
+    P.plen = plen;
+    P.nodeI.clear();
+    for (int p = 0; p < np; ++p) {
+        P.roff[p] = (int)P.nodeI.size();
+        for (int r = 0; r < plen[p]; ++r) {
+            const int nd = best[pst[p] + r];
+            piece[nd] = p;
+            prow[nd] = (int)P.nodeI.size();
+            P.nodeI.push_back(nd);
+        }
+    }
+    for (int r = 0; r < nI0; ++r)
+        if (piece[best[r]] < 0) S.push_back(best[r]);
+    P.nI = (int)P.nodeI.size();
+    P.nS = (int)S.size();
+    P.nSp = (P.nS + 15) / 16 * 16;
+    P.JS = P.nSp / 16;
+    P.nBp = P.nSp + P.nGp;
+    P.ntB = P.nBp / 16;
+    P.nblkP = 0;
+    for (int p = 0; p < np; ++p) P.nblkP = std::max(P.nblkP, (plen[p] + 15) / 16);
+    // combined interior CSR over the pieces (no entry joins two pieces)
     struct E3 { int r, cc, q; };
     std::vector<E3> ie;
-    std::vector<int> gidx(sanz, -1);
     for (int q = 0; q < nnz; ++q) {
-        const int a = pos[crow[q]], b = pos[col[q]];
-        if (a >= 0 && b >= 0) ie.push_back({std::max(a, b), std::min(a, b), q});
+        const int i = crow[q], j = col[q];
+        if (piece[i] < 0 || piece[j] < 0) continue;
+        if (piece[i] != piece[j]) return CRT_ERR_STATE;
+        ie.push_back({std::max(prow[i], prow[j]), std::min(prow[i], prow[j]), q});
     }
     std::sort(ie.begin(), ie.end(), [](const E3 &u, const E3 &v) { return u.r != v.r ? u.r < v.r : u.cc < v.cc; });
     P.nnzI = (int)ie.size();
@@ -1238,51 +1288,69 @@ static int cond_plan(crt_ctx *c) {
     for (int i = 0; i < P.nI; ++i) rp[i + 1] += rp[i];
     for (int i = 0; i < P.nI; ++i)
         if (rp[i + 1] == rp[i] || ci[rp[i + 1] - 1] != i) return CRT_ERR_STATE;  // every row ends on its diagonal
+    // one tile map per piece over its local rows (padding rows: unit diagonal)
     const int D = P.DI;
-    std::vector<int> tmap((size_t)P.nblkI * (D + 1) * 256, P.nnzI);
-    for (int i = 0; i < P.nI; ++i)
-        for (int q = rp[i]; q < rp[i + 1]; ++q) {
-            const int j = ci[q], I = i / 16, d = I - j / 16;
-            tmap[((size_t)(I * (D + 1) + d) * 16 + (i % 16)) * 16 + (j % 16)] = q;
+    const size_t tstride = (size_t)P.nblkP * (D + 1) * 256;
+    std::vector<int> tmap(tstride * np, P.nnzI);
+    for (int p = 0; p < np; ++p) {
+        int *tm = tmap.data() + tstride * p;
+        for (int r = 0; r < plen[p]; ++r) {
+            const int g = P.roff[p] + r;
+            for (int q = rp[g]; q < rp[g + 1]; ++q) {
+                const int cl = ci[q] - P.roff[p], I = r / 16, d = I - cl / 16;
+                if (d < 0 || d > D) return CRT_ERR_STATE;
+                tm[((size_t)(I * (D + 1) + d) * 16 + (r % 16)) * 16 + (cl % 16)] = q;
+            }
         }
-    for (int i = P.nI; i < P.nblkI * 16; ++i)
-        tmap[((size_t)((i / 16) * (D + 1)) * 16 + (i % 16)) * 16 + (i % 16)] = P.nnzI + 1;
-    // boundary numbering: by the first interior neighbour
+        for (int r = plen[p]; r < P.nblkP * 16; ++r)
+            tm[((size_t)((r / 16) * (D + 1)) * 16 + (r % 16)) * 16 + (r % 16)] = P.nnzI + 1;
+    }
+    // B numbering: S in position order, padding, then G by its first interior neighbour
     std::vector<int> key(sanz, INT32_MAX);
     for (int q = 0; q < nnz; ++q) {
         const int i = crow[q], j = col[q];
         if (inG[i] && pos[j] >= 0) key[i] = std::min(key[i], pos[j]);
         if (inG[j] && pos[i] >= 0) key[j] = std::min(key[j], pos[i]);
     }
-    std::stable_sort(P.nodeG.begin(), P.nodeG.end(), [&](int a, int b) { return key[a] < key[b]; });
-    for (int g = 0; g < P.nG; ++g) gidx[P.nodeG[g]] = g;
-    const int nGp = P.nGp;
-    std::vector<int> gmap((size_t)nGp * nGp, -1), qdiag(P.nG, -1);
-    std::vector<std::vector<std::pair<int, int>>> gi(P.nG);
-    struct RE { int row, colx, q; };
-    std::vector<RE> re;
+    std::stable_sort(G0.begin(), G0.end(), [&](int a, int b) { return key[a] < key[b]; });
+    const int nBp = P.nBp, nSp = P.nSp;
+    std::vector<int> bidx(sanz, -1);
+    P.nodeB.assign(nBp, -1);
+    for (int i = 0; i < P.nS; ++i) {
+        bidx[S[i]] = i;
+        P.nodeB[i] = S[i];
+    }
+    for (int g = 0; g < P.nG; ++g) {
+        bidx[G0[g]] = nSp + g;
+        P.nodeB[nSp + g] = G0[g];
+    }
+    // B-B entries, B diagonals, couplings of each B node to the pieces
+    std::vector<int> gmap((size_t)nBp * nBp, -1), qdiagB(nBp, -1);
+    struct Cp { int b, p, r, q; };  // B node b coupled to local row r of piece p by entry q
+    std::vector<Cp> cpl;
     for (int q = 0; q < nnz; ++q) {
         const int i = crow[q], j = col[q];
-        const int gi_ = gidx[i], gj = gidx[j];
-        if (gi_ >= 0 && gj >= 0) {
-            gmap[(size_t)gj * nGp + gi_] = q;
-            gmap[(size_t)gi_ * nGp + gj] = q;
-            if (i == j) qdiag[gi_] = q;
-        } else if (gi_ >= 0 || gj >= 0) {
-            const int g = gi_ >= 0 ? gi_ : gj, ii = gi_ >= 0 ? pos[j] : pos[i];
-            gi[g].push_back({ii, q});
-            re.push_back({ii, g, q});
+        const int bi = bidx[i], bj = bidx[j];
+        if (bi >= 0 && bj >= 0) {
+            gmap[(size_t)bj * nBp + bi] = q;
+            gmap[(size_t)bi * nBp + bj] = q;
+            if (i == j) qdiagB[bi] = q;
+        } else if (bi >= 0 || bj >= 0) {
+            const int b = bi >= 0 ? bi : bj, x = bi >= 0 ? j : i;
+            cpl.push_back({b, piece[x], prow[x] - P.roff[piece[x]], q});
         }
     }
-    for (int g = 0; g < P.nG; ++g)
-        if (qdiag[g] < 0) return CRT_ERR_STATE;
+    for (int b = 0; b < nBp; ++b)
+        if ((P.nodeB[b] >= 0) != (qdiagB[b] >= 0)) return CRT_ERR_STATE;
+    std::vector<int> qdiagG(P.nG);
+    for (int g = 0; g < P.nG; ++g) qdiagG[g] = qdiagB[nSp + g];
     // entries that hold boundary terms (the node pairs of the type-11 elements), per
-    // lower tile of the dense boundary matrix: (element c*16 + r of the tile, entry)
+    // lower tile of the G block: (element c*16 + r of the tile, entry, G indices)
     const size_t ntri = (size_t)P.nt * (P.nt + 1) / 2;
     struct BE { int e, q, g1, g2; };
     std::vector<std::vector<BE>> bt(ntri);
     {
-        std::vector<char> seen((size_t)nGp * nGp, 0);
+        std::vector<char> seen((size_t)P.nGp * P.nGp, 0);
         int iel = 0;
         for (int t = 0; t < c->typanz; ++t)
             for (int j = 0; j < c->nelanz[t]; ++j, ++iel) {
@@ -1290,12 +1358,12 @@ static int cond_plan(crt_ctx *c) {
                 const int ns = c->selanz[t];
                 for (int a = 0; a < ns; ++a)
                     for (int b = 0; b <= a; ++b) {
-                        int g1 = gidx[c->nrel[(size_t)iel * c->maxn + a] - 1];
-                        int g2 = gidx[c->nrel[(size_t)iel * c->maxn + b] - 1];
+                        int g1 = bidx[c->nrel[(size_t)iel * c->maxn + a] - 1] - nSp;
+                        int g2 = bidx[c->nrel[(size_t)iel * c->maxn + b] - 1] - nSp;
                         if (g1 < g2) std::swap(g1, g2);
-                        if (seen[(size_t)g2 * nGp + g1]) continue;
-                        seen[(size_t)g2 * nGp + g1] = 1;
-                        const int q = gmap[(size_t)g2 * nGp + g1];
+                        if (seen[(size_t)g2 * P.nGp + g1]) continue;
+                        seen[(size_t)g2 * P.nGp + g1] = 1;
+                        const int q = gmap[(size_t)(nSp + g2) * nBp + nSp + g1];
                         if (q < 0) return CRT_ERR_STATE;
                         const size_t ti = (size_t)(g1 / 16) * (g1 / 16 + 1) / 2 + g2 / 16;
                         bt[ti].push_back({(g2 % 16) * 16 + g1 % 16, q, g1, g2});
@@ -1314,45 +1382,81 @@ static int cond_plan(crt_ctx *c) {
     }
     P.nb = (int)bent.size();
     if (bent.empty()) { bent.push_back(0); bq.push_back(0); bg.push_back(int2{0, 0}); }
-    // sources: U column l of an interior electrode node, b_G entry of a boundary one
+    // columns of each piece: its coupled B nodes by first coupled row, then its electrodes
     P.E = c->eanz;
-    P.Ep = (P.E + 15) / 16 * 16;
-    std::vector<int> eG(std::max(P.E, 1), -1);
-    for (int l = 0; l < P.E; ++l) {
-        const int nd = c->enr0[l];
-        if (pos[nd] >= 0) re.push_back({pos[nd], nGp + l, -1});
-        else eG[l] = gidx[nd];
+    const int E = P.E;
+    std::vector<int> first((size_t)np * nBp, INT32_MAX);
+    for (auto &cp : cpl) first[(size_t)cp.p * nBp + cp.b] = std::min(first[(size_t)cp.p * nBp + cp.b], cp.r);
+    std::vector<int> bcol((size_t)np * nBp, -1), ucol((size_t)np * std::max(E, 1), -1);
+    std::vector<std::vector<int>> cols(np);  // B index of each coupled column
+    P.nbc.assign(np, 0);
+    int ncmax = 0;
+    std::vector<int> ne(np, 0);
+    for (int p = 0; p < np; ++p) {
+        std::vector<int> bs;
+        for (int b = 0; b < nBp; ++b)
+            if (first[(size_t)p * nBp + b] != INT32_MAX) bs.push_back(b);
+        std::stable_sort(bs.begin(), bs.end(), [&](int a, int b) {
+            return first[(size_t)p * nBp + a] < first[(size_t)p * nBp + b];
+        });
+        for (size_t cix = 0; cix < bs.size(); ++cix) bcol[(size_t)p * nBp + bs[cix]] = (int)cix;
+        cols[p] = bs;
+        P.nbc[p] = (int)bs.size();
+        for (int l = 0; l < E; ++l)
+            if (piece[c->enr0[l]] == p) ucol[(size_t)p * E + l] = P.nbc[p] + ne[p]++;
+        ncmax = std::max(ncmax, P.nbc[p] + ne[p]);
     }
-    P.ncol = nGp + P.Ep;
-    P.ntile = P.ncol / 16;
+    P.ncolP = std::max(16, (ncmax + 15) / 16 * 16);
+    P.ntileP = P.ncolP / 16;
+    std::vector<int> colmap((size_t)np * P.ncolP, -1);
+    for (int p = 0; p < np; ++p)
+        for (int cix = 0; cix < P.nbc[p]; ++cix) colmap[(size_t)p * P.ncolP + cix] = cols[p][cix];
+    // right-hand-side entries per (piece, column tile), by row
+    struct RE { int p, row, colx, rowg, b, q; };
+    std::vector<RE> re;
+    for (auto &cp : cpl) re.push_back({cp.p, cp.r, bcol[(size_t)cp.p * nBp + cp.b], P.roff[cp.p] + cp.r, cp.b, cp.q});
+    std::vector<int> eB(std::max(E, 1), -1);
+    for (int l = 0; l < E; ++l) {
+        const int nd = c->enr0[l];
+        if (piece[nd] >= 0) re.push_back({piece[nd], prow[nd] - P.roff[piece[nd]], ucol[(size_t)piece[nd] * E + l],
+                                          prow[nd], -1, -1});
+        else eB[l] = bidx[nd];
+    }
     std::sort(re.begin(), re.end(), [](const RE &u, const RE &v) {
+        if (u.p != v.p) return u.p < v.p;
         if (u.colx / 16 != v.colx / 16) return u.colx / 16 < v.colx / 16;
         return u.row != v.row ? u.row < v.row : u.colx < v.colx;
     });
     P.nent = (int)re.size();
-    std::vector<int> tptr(P.ntile + 1, 0), erow(std::max(P.nent, 1)), ecol(std::max(P.nent, 1)),
-        ecolg(std::max(P.nent, 1)), eq(std::max(P.nent, 1)), j0(P.ntile, P.nblkI);
+    const int npt = np * P.ntileP;
+    std::vector<int> tptr(npt + 1, 0), erow(std::max(P.nent, 1)), erowg(std::max(P.nent, 1)), ecol(std::max(P.nent, 1)),
+        ebcol(std::max(P.nent, 1)), eq(std::max(P.nent, 1)), j0(npt, P.nblkP);
     for (int e = 0; e < P.nent; ++e) {
-        tptr[re[e].colx / 16 + 1]++;
+        tptr[re[e].p * P.ntileP + re[e].colx / 16 + 1]++;
         erow[e] = re[e].row;
+        erowg[e] = re[e].rowg;
         ecol[e] = re[e].colx % 16;
-        ecolg[e] = re[e].colx;
+        ebcol[e] = re[e].b;
         eq[e] = re[e].q;
     }
-    for (int t = 0; t < P.ntile; ++t) tptr[t + 1] += tptr[t];
-    for (int t = 0; t < P.ntile; ++t)
+    for (int t = 0; t < npt; ++t) tptr[t + 1] += tptr[t];
+    for (int t = 0; t < npt; ++t)
         if (tptr[t] < tptr[t + 1]) j0[t] = erow[tptr[t]] / 16;
-    std::vector<int> gptr(P.nG + 1, 0), grow_, gq;
-    for (int g = 0; g < P.nG; ++g) {
-        std::sort(gi[g].begin(), gi[g].end());
-        for (auto &pr : gi[g]) {
-            grow_.push_back(pr.first);
-            gq.push_back(pr.second);
+    // A_Bi lists by B row: (combined interior row, piece, entry)
+    std::vector<std::vector<Cp>> gl(nBp);
+    for (auto &cp : cpl) gl[cp.b].push_back(cp);
+    std::vector<int> gptr(nBp + 1, 0), grow_, gpc, gq;
+    for (int b = 0; b < nBp; ++b) {
+        std::sort(gl[b].begin(), gl[b].end(), [](const Cp &u, const Cp &v) { return u.p != v.p ? u.p < v.p : u.r < v.r; });
+        for (auto &cp : gl[b]) {
+            grow_.push_back(P.roff[cp.p] + cp.r);
+            gpc.push_back(cp.p);
+            gq.push_back(cp.q);
         }
-        gptr[g + 1] = (int)grow_.size();
+        gptr[b + 1] = (int)grow_.size();
     }
-    if (grow_.empty()) { grow_.push_back(0); gq.push_back(0); }
-    std::vector<int> zero(P.nnzI + 1, 0), enr0(std::max(c->eanz, 1), 0);
+    if (grow_.empty()) { grow_.push_back(0); gpc.push_back(0); gq.push_back(0); }
+    std::vector<int> zero(P.nnzI + 1, 0), enr0(std::max(np, 1), 0);
     hipStream_t st = c->stream;
     int e = 0;
     e = e ? e : P.d_row_ptr.upload(rp.data(), rp.size(), st);
@@ -1364,18 +1468,27 @@ static int cond_plan(crt_ctx *c) {
     e = e ? e : P.d_tmap.upload(tmap.data(), tmap.size(), st);
     e = e ? e : P.d_tptr.upload(tptr.data(), tptr.size(), st);
     e = e ? e : P.d_erow.upload(erow.data(), erow.size(), st);
+    e = e ? e : P.d_erowg.upload(erowg.data(), erowg.size(), st);
     e = e ? e : P.d_ecol.upload(ecol.data(), ecol.size(), st);
-    e = e ? e : P.d_ecolg.upload(ecolg.data(), ecolg.size(), st);
+    e = e ? e : P.d_ebcol.upload(ebcol.data(), ebcol.size(), st);
     e = e ? e : P.d_eq.upload(eq.data(), eq.size(), st);
     e = e ? e : P.d_j0.upload(j0.data(), j0.size(), st);
     e = e ? e : P.d_gi_ptr.upload(gptr.data(), gptr.size(), st);
     e = e ? e : P.d_gi_row.upload(grow_.data(), grow_.size(), st);
+    e = e ? e : P.d_gi_piece.upload(gpc.data(), gpc.size(), st);
     e = e ? e : P.d_gi_q.upload(gq.data(), gq.size(), st);
     e = e ? e : P.d_gmap.upload(gmap.data(), gmap.size(), st);
-    e = e ? e : P.d_qdiag.upload(qdiag.data(), qdiag.size(), st);
+    e = e ? e : P.d_qdiag.upload(qdiagB.data(), qdiagB.size(), st);
+    e = e ? e : P.d_qdiagG.upload(qdiagG.data(), qdiagG.size(), st);
     e = e ? e : P.d_nodeI.upload(P.nodeI.data(), P.nodeI.size(), st);
-    e = e ? e : P.d_nodeG.upload(P.nodeG.data(), P.nodeG.size(), st);
-    e = e ? e : P.d_eG.upload(eG.data(), eG.size(), st);
+    e = e ? e : P.d_nodeB.upload(P.nodeB.data(), P.nodeB.size(), st);
+    e = e ? e : P.d_eB.upload(eB.data(), eB.size(), st);
+    e = e ? e : P.d_bcol.upload(bcol.data(), bcol.size(), st);
+    e = e ? e : P.d_ucol.upload(ucol.data(), ucol.size(), st);
+    e = e ? e : P.d_colmap.upload(colmap.data(), colmap.size(), st);
+    e = e ? e : P.d_roff.upload(P.roff.data(), P.roff.size(), st);
+    e = e ? e : P.d_plen.upload(P.plen.data(), P.plen.size(), st);
+    e = e ? e : P.d_nbc.upload(P.nbc.data(), P.nbc.size(), st);
     e = e ? e : P.d_bptr.upload(bptr.data(), bptr.size(), st);
     e = e ? e : P.d_bent.upload(bent.data(), bent.size(), st);
     e = e ? e : P.d_bq.upload(bq.data(), bq.size(), st);
@@ -1388,64 +1501,75 @@ static int cond_plan(crt_ctx *c) {
 
 // ------------------------------------------------ condensed forward: solve
 // Every wavenumber chunk: the interior area sums, scaling, assembly and band
-// factorisation (the band kernels on the interior grid), the substitutions for
-// [Z | U], the Schur complements and right-hand sides, the dense factorisation and
-// solve of every (l, k), and the interior potentials.  Errors: a zero interior
-// diagonal or pivot names the wavenumber's first source (the interior rows are the
-// same for every source), a boundary one its own source; the first failing system
-// in the reference's loop order (wavenumber, then electrode) is returned.
+// factorisation of every piece (the band kernels on the pieces' grids), the
+// substitutions for each piece's [Z | U], the Schur complement T over B and the
+// right-hand sides, the separators' elimination (prefix), the dense factorisation and
+// solve of every (l, k) over G, and the interior potentials.  Errors: a zero interior
+// or separator diagonal or pivot names the wavenumber's first source (those rows are
+// the same for every source), a boundary one its own source; the first failing
+// system in the reference's loop order (wavenumber, then electrode) is returned.
 static int forward_condensed(crt_ctx *c) {
     auto &P = c->cond;
-    const int K = c->kwnanz, E = c->eanz;
+    const int K = c->kwnanz, E = c->eanz, nP = P.P;
     hipStream_t st = c->stream;
     DevGrid g = c->devgrid();
     int e = c->d_area.ensure((size_t)K * c->nnz);
     e = e ? e : crt_launch_area(st, g, K, c->d_area.p);
     if (e) return e;
     g.area = c->d_area.p;
-    const int nGp = P.nGp, nt = P.nt, ldx = P.nblkI * 16, D = P.DI;
-    const size_t fstrideI = (size_t)P.nblkI * 16, ntri = (size_t)nt * (nt + 1) / 2;
-    const size_t per_k = sizeof(zd) * ((size_t)P.ncol * ldx + (size_t)P.nblkI * (D + 1) * 256 + 2 * fstrideI +
-                                       (size_t)P.nnzI * 2 + 2 + ntri * 256 + 2 * (size_t)E * nGp +
-                                       (size_t)E * (ntri + nt) * 256 + P.nent + (size_t)E * P.nb) +
-                         sizeof(double) * (2 * fstrideI + nGp);
+    const int nGp = P.nGp, nBp = P.nBp, nt = P.nt, ntB = P.ntB, JS = P.JS, ldx = P.nblkP * 16, D = P.DI;
+    const int fstrideC = (P.nI + 15) / 16 * 16;
+    const size_t ntri = (size_t)nt * (nt + 1) / 2, ntriB = (size_t)ntB * (ntB + 1) / 2;
+    const size_t per_k = sizeof(zd) * ((size_t)nP * P.ncolP * ldx + (size_t)nP * P.nblkP * (D + 1) * 256 +
+                                       2 * (size_t)nP * ldx + (size_t)nP * (P.nnzI + 2) + P.nnzI + ntriB * 256 * 2 +
+                                       ntri * 256 + 2 * (size_t)E * nBp + (size_t)E * (ntri + nt) * 256 +
+                                       (size_t)JS * 256 + P.nent + (size_t)E * P.nb) +
+                         sizeof(double) * (2 * (size_t)nP * fstrideC + nBp);
     const int kc = (int)std::max<size_t>(1, std::min<size_t>(K, c->panel_budget / per_k));
+    const int ns = kc * nP;
     e = P.areaI.ensure((size_t)kc * P.nnzI);
-    e = e ? e : P.vals.ensure((size_t)kc * (P.nnzI + 2));
-    e = e ? e : P.panels.ensure((size_t)kc * P.nblkI * (D + 1) * 256);
-    e = e ? e : P.yI.ensure((size_t)kc * fstrideI);
-    e = e ? e : P.dinv.ensure((size_t)kc * fstrideI);
-    e = e ? e : P.fak.ensure((size_t)kc * fstrideI);
-    e = e ? e : P.dga.ensure((size_t)kc * fstrideI);
-    e = e ? e : P.fG.ensure((size_t)kc * nGp);
+    e = e ? e : P.vals.ensure((size_t)ns * (P.nnzI + 2));
+    e = e ? e : P.panels.ensure((size_t)ns * P.nblkP * (D + 1) * 256);
+    e = e ? e : P.yI.ensure((size_t)ns * std::max<size_t>(ldx, fstrideC));
+    e = e ? e : P.dinv.ensure((size_t)ns * ldx);
+    e = e ? e : P.fak.ensure((size_t)ns * fstrideC);
+    e = e ? e : P.dga.ensure((size_t)ns * fstrideC);
+    e = e ? e : P.fB.ensure((size_t)kc * nBp);
     e = e ? e : P.rv.ensure((size_t)kc * std::max(P.nent, 1));
-    e = e ? e : P.X.ensure((size_t)kc * P.ncol * ldx);
+    e = e ? e : P.X.ensure((size_t)ns * P.ncolP * ldx);
+    e = e ? e : P.T.ensure((size_t)kc * ntriB * 256);
     e = e ? e : P.S0.ensure((size_t)kc * ntri * 256);
-    e = e ? e : P.Gr.ensure((size_t)kc * E * nGp);
-    e = e ? e : P.XG.ensure((size_t)kc * E * nGp);
+    e = e ? e : P.Gr.ensure((size_t)kc * E * nBp);
+    e = e ? e : P.XB.ensure((size_t)kc * E * nBp);
     e = e ? e : P.Lt.ensure((size_t)kc * E * ntri * 256);
     e = e ? e : P.Li.ensure((size_t)kc * E * nt * 256);
+    e = e ? e : P.Lpre.ensure((size_t)kc * (JS ? ntriB : 1) * 256);
+    e = e ? e : P.Lipre.ensure((size_t)kc * std::max(JS, 1) * 256);
     e = e ? e : P.bval.ensure((size_t)kc * E * std::max(P.nb, 1));
-    e = e ? e : P.sysI.ensure(kc);
-    e = e ? e : P.stI.ensure(K);
+    e = e ? e : P.sysI.ensure(ns);
+    e = e ? e : P.stI.ensure((size_t)K * nP);
     e = e ? e : P.stD.ensure((size_t)K * E);
     e = e ? e : c->d_kpotN.ensure((size_t)c->sanz * K * E);
     if (e) return e;
-    std::vector<int2> sysI(kc);
-    for (int kk = 0; kk < kc; ++kk) sysI[kk] = int2{0, kk};
-    CRT_HIP(hipMemcpyAsync(P.sysI.p, sysI.data(), kc * sizeof(int2), hipMemcpyHostToDevice, st));
-    CRT_HIP(hipMemsetAsync(P.stI.p, 0, K * sizeof(int), st));
+    std::vector<int2> sysI(ns);
+    for (int kk = 0; kk < kc; ++kk)
+        for (int p = 0; p < nP; ++p) sysI[(size_t)kk * nP + p] = int2{p, kk};
+    CRT_HIP(hipMemcpyAsync(P.sysI.p, sysI.data(), ns * sizeof(int2), hipMemcpyHostToDevice, st));
+    CRT_HIP(hipMemsetAsync(P.stI.p, 0, (size_t)K * nP * sizeof(int), st));
     CRT_HIP(hipMemsetAsync(P.stD.p, 0, (size_t)K * E * sizeof(int), st));
-    // the interior grid: the band kernels' view of A_II (area sums only)
+    // the pieces' grids: rows of all pieces in one CSR for the scaling and assembly
+    // (gI), each piece's own tile map over its padded local rows for the factorisation (gP)
     DevGrid gI = g;
-    gI.sanz = P.nI; gI.mb = P.mbI; gI.nblk = P.nblkI; gI.D = D; gI.nnz = P.nnzI;
-    gI.fstride = (int)fstrideI; gI.vstride = P.nnzI + 2;
+    gI.sanz = P.nI; gI.mb = P.mbI; gI.nnz = P.nnzI; gI.fstride = fstrideC; gI.vstride = P.nnzI + 2;
     gI.row_ptr = P.d_row_ptr.p; gI.col = P.d_col.p; gI.crow = P.d_crow.p;
     gI.cptr = P.d_zero.p; gI.csplit = P.d_zero.p; gI.contrib = P.d_zero.p;
     gI.enr = P.d_enr0.p;
-    gI.tmap = P.d_tmap.p; gI.tmapD = D;
     gI.area = P.areaI.p;
     gI.dmask = nullptr;
+    gI.tmap = nullptr; gI.tmapD = 0; gI.tmap_sys = 0;
+    DevGrid gP = gI;
+    gP.sanz = ldx; gP.nblk = P.nblkP; gP.D = D;
+    gP.tmap = P.d_tmap.p; gP.tmapD = D; gP.tmap_sys = (long)P.nblkP * (D + 1) * 256;
     int variant = D <= 10 ? 1 : 2;
     if (const char *ev = getenv("CRT_COND_VARIANT")) {
         const int v = atoi(ev);
@@ -1453,42 +1577,50 @@ static int forward_condensed(crt_ctx *c) {
     }
     double tdiag = 0, tfac = 0, tsol = 0, tsch = 0, tprod = 0;
     for (int k0 = 0; k0 < K; k0 += kc) {
-        const int n = std::min(kc, K - k0);
+        const int n = std::min(kc, K - k0), nsn = n * nP;
+        int *stI = P.stI.p + (size_t)k0 * nP;
         c->tic(T_DIAG);
         e = crt_launch_cond_gather(st, n, k0, c->nnz, P.nnzI, P.d_qmap.p, c->d_area.p, P.areaI.p);
-        e = e ? e : crt_launch_diag(st, gI, n, P.sysI.p, P.fak.p, P.dga.p, P.stI.p + k0);
-        e = e ? e : crt_launch_assemble(st, gI, n, P.sysI.p, P.fak.p, P.dga.p, P.vals.p);
-        e = e ? e : crt_launch_cond_fg(st, n, k0, P.nG, nGp, c->nnz, P.d_qdiag.p, c->d_area.p, P.fG.p);
-        e = e ? e : crt_launch_cond_rhs(st, n, k0, P.nent, c->nnz, nGp, (int)fstrideI, P.d_erow.p, P.d_ecolg.p,
-                                        P.d_eq.p, c->d_area.p, P.fak.p, P.fG.p, P.rv.p);
+        // the systems (piece, kk) read the area sums of wavenumber kk: sys.y
+        e = e ? e : crt_launch_diag(st, gI, nsn, P.sysI.p, P.fak.p, P.dga.p, stI);
+        e = e ? e : crt_launch_assemble(st, gI, nsn, P.sysI.p, P.fak.p, P.dga.p, P.vals.p);
+        e = e ? e : crt_launch_cond_fb(st, n, k0, P.nS, nBp, c->nnz, g.dc, nP, P.d_qdiag.p, c->d_area.p, P.fB.p, stI);
+        e = e ? e : crt_launch_cond_rhs(st, n, k0, P.nent, c->nnz, nBp, fstrideC, nP, P.d_erowg.p, P.d_ebcol.p,
+                                        P.d_eq.p, c->d_area.p, P.fak.p, P.fB.p, P.rv.p);
         if (e) return e;
         c->toc(T_DIAG);
         c->tic(T_FACTOR);
-        e = crt_launch_factor16(st, gI, n, P.sysI.p, P.fak.p, P.vals.p, P.panels.p, P.yI.p, P.dinv.p, P.stI.p + k0,
-                                variant);
-        if (!e && g.dc) e = crt_launch_dc_pivot(st, n, P.nI, (int)fstrideI, P.dinv.p, P.stI.p + k0);
+        e = crt_launch_factor16(st, gP, nsn, P.sysI.p, P.fak.p, P.vals.p, P.panels.p, P.yI.p, P.dinv.p, stI, variant);
+        if (!e && g.dc) e = crt_launch_dc_pivot(st, nsn, ldx, ldx, P.dinv.p, stI);
         if (e) return e;
         c->toc(T_FACTOR);
         c->tic(T_SOLVE);
-        e = crt_launch_cond_trsm(st, D, 0, P.nblkI, ldx, P.ntile, n, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
+        e = crt_launch_cond_trsm(st, D, 0, P.nblkP, ldx, P.ntileP, nsn, nP, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
                                  P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
-        e = e ? e : crt_launch_cond_trsm(st, D, 1, P.nblkI, ldx, P.ntile, n, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
-                                         P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
-        e = e ? e : crt_launch_cond_schur(st, n, k0, P.nG, nGp, E, ldx, P.ntile, c->nnz, (int)fstrideI, P.d_gi_ptr.p,
-                                          P.d_gi_row.p, P.d_gi_q.p, P.d_gmap.p, P.d_eG.p, c->d_area.p, P.fak.p,
-                                          P.fG.p, P.X.p, P.S0.p, P.Gr.p);
+        e = e ? e : crt_launch_cond_trsm(st, D, 1, P.nblkP, ldx, P.ntileP, nsn, nP, P.panels.p, P.dinv.p, P.X.p,
+                                         P.d_tptr.p, P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
+        e = e ? e : crt_launch_cond_schur(st, n, k0, nBp, E, ldx, P.ntileP, c->nnz, fstrideC, nP, P.d_gi_ptr.p,
+                                          P.d_gi_row.p, P.d_gi_piece.p, P.d_gi_q.p, P.d_roff.p, P.d_bcol.p,
+                                          P.d_ucol.p, P.d_gmap.p, P.d_eB.p, c->d_area.p, P.fak.p, P.fB.p, P.X.p,
+                                          JS ? P.T.p : P.S0.p, P.Gr.p);
         if (e) return e;
         c->toc(T_SOLVE);
         c->tic(T_SCHUR);
-        e = crt_launch_cond_bval(st, g, n * E, E, k0, nGp, P.nb, P.d_bq.p, P.d_bg.p, P.fG.p, P.bval.p);
-        e = e ? e : crt_launch_cond_dense(st, g, P.nG, nGp, E, K, k0, n * E, P.S0.p, P.Gr.p, P.d_bptr.p, P.d_bent.p,
-                                          P.bval.p, P.nb, P.d_qdiag.p, P.d_nodeG.p, P.fG.p, P.Lt.p, P.Li.p, P.XG.p,
-                                          c->d_kpotN.p, P.stD.p + (size_t)k0 * E);
+        if (JS)  // the separators, once per wavenumber: L_SS, L_GS and S0 = T_GG - L_GS L_GS^T
+            e = crt_launch_cond_dense(st, g, 1, P.nG, nt, nBp, JS, E, K, k0, n, nullptr, P.T.p, P.Gr.p, P.d_bptr.p,
+                                      P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p, P.Lt.p, P.Li.p,
+                                      P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p, stI, nP);
+        e = e ? e : crt_launch_cond_bval(st, g, n * E, E, k0, nBp, P.nb, P.d_bq.p, P.d_bg.p, P.fB.p + P.nSp, P.bval.p);
+        e = e ? e : crt_launch_cond_dense(st, g, 0, P.nG, nt, nBp, JS, E, K, k0, n * E, P.S0.p, P.T.p, P.Gr.p,
+                                          P.d_bptr.p, P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p,
+                                          P.Lt.p, P.Li.p, P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p,
+                                          P.stD.p + (size_t)k0 * E, 1);
         if (e) return e;
         c->toc(T_SCHUR);
         c->tic(T_PRODUCT);
-        e = crt_launch_cond_gemm(st, n, k0, P.nI, ldx, P.ntile, nGp, E, K, (int)fstrideI, P.X.p, P.XG.p, P.fak.p,
-                                 P.d_nodeI.p, c->d_kpotN.p);
+        e = crt_launch_cond_gemm(st, nsn, nP, k0, ldx, P.ntileP, P.ncolP, nBp, E, K, fstrideC, P.X.p, P.XB.p, P.fak.p,
+                                 P.d_nodeI.p, P.d_roff.p, P.d_plen.p, P.d_nbc.p, P.d_colmap.p, P.d_ucol.p,
+                                 c->d_kpotN.p);
         if (e) return e;
         c->toc(T_PRODUCT);
         CRT_HIP(hipStreamSynchronize(st));
@@ -1501,11 +1633,12 @@ static int forward_condensed(crt_ctx *c) {
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol; c->times[T_SCHUR] = tsch;
     c->times[T_PRODUCT] = tprod;
     c->times[T_REFINE] = 0.0;
-    std::vector<int> sI(K), sD((size_t)K * E);
-    CRT_HIP(c->copy_out(sI.data(), P.stI.p, K * sizeof(int)));
-    CRT_HIP(c->copy_out(sD.data(), P.stD.p, (size_t)K * E * sizeof(int)));
+    std::vector<int> sI((size_t)K * nP), sD((size_t)K * E);
+    CRT_HIP(c->copy_out(sI.data(), P.stI.p, sI.size() * sizeof(int)));
+    CRT_HIP(c->copy_out(sD.data(), P.stD.p, sD.size() * sizeof(int)));
     for (int k = 0; k < K; ++k) {
-        if (sI[k]) return sI[k];
+        for (int p = 0; p < nP; ++p)
+            if (sI[(size_t)k * nP + p]) return sI[(size_t)k * nP + p];
         for (int l = 0; l < E; ++l)
             if (sD[(size_t)k * E + l]) return sD[(size_t)k * E + l];
     }
@@ -1527,8 +1660,8 @@ int crt_get_forward_method(crt_ctx *c, int *method) {
 int crt_get_condense_info(crt_ctx *c, int *info) {
     if (!c || !info) return CRT_ERR_ARG;
     const auto &P = c->cond;
-    const int v[6] = {P.ok ? 1 : 0, P.nI, P.nG, P.mbI, P.DI, P.order};
-    for (int i = 0; i < 6; ++i) info[i] = v[i];
+    const int v[8] = {P.ok ? 1 : 0, P.nI, P.nG, P.mbI, P.DI, P.order, P.P, P.nS};
+    for (int i = 0; i < 8; ++i) info[i] = v[i];
     return 0;
 }
 

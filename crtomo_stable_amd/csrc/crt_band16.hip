@@ -506,8 +506,8 @@ __device__ __forceinline__ zd *la4_tile(La4Lds<D> &m, int d) { return m.P + (d -
 
 // map entries (CSR indices) of this lane's four elements of tile (I, I-d)
 template <int D>
-__device__ __forceinline__ void la4_map(const DevGrid &g, int I, int d, int jc, int rq, int (&mi)[4]) {
-    const int *mp = g.tmap + ((size_t)(I * (D + 1) + d) * T16) * T16 + jc;
+__device__ __forceinline__ void la4_map(const DevGrid &g, const int *tm, int I, int d, int jc, int rq, int (&mi)[4]) {
+    const int *mp = tm + ((size_t)(I * (D + 1) + d) * T16) * T16 + jc;
 #pragma unroll
     for (int v = 0; v < 4; ++v) mi[v] = I < g.nblk ? mp[(4 * v + rq) * T16] : g.nnz;
 }
@@ -660,6 +660,7 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
     const int s = blockIdx.x;
     const int lane = threadIdx.x & 63, tid = WV * 64 + lane;
     const int enode = g.enr[A.sys[s].x];
+    const int *tm = g.tmap + (size_t)A.sys[s].x * g.tmap_sys;
     const double *fak = A.fak + (size_t)s * nblk * T16;
     const zd *vals = A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
@@ -678,9 +679,9 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
     // map entries of the entering tiles, loaded one block row ahead
     int mA[4], mB[4], mZ[4];
     if constexpr (!isP) {
-        la4_map<D>(g, 0, dA, jc, rq, mA);
-        if constexpr (LB > 0) la4_map<D>(g, 0, dB, jc, rq, mB);
-        if constexpr (topZ) la4_map<D>(g, 0, 0, jc, rq, mZ);
+        la4_map<D>(g, tm, 0, dA, jc, rq, mA);
+        if constexpr (LB > 0) la4_map<D>(g, tm, 0, dB, jc, rq, mB);
+        if constexpr (topZ) la4_map<D>(g, tm, 0, 0, jc, rq, mZ);
     }
 
     for (int J = -(D + 1); J < nblk; ++J) {
@@ -796,9 +797,9 @@ __device__ __forceinline__ void la4_wave(const Factor16Args &A, La4Lds<D> &m) {
             la4_vals(vals, mA, eAr, eAi);
             if constexpr (LB > 0) la4_vals(vals, mB, eBr, eBi);
             if constexpr (topZ) la4_vals(vals, mZ, eZr, eZi);
-            la4_map<D>(g, Ie + 1, dA, jc, rq, mA);
-            if constexpr (LB > 0) la4_map<D>(g, Ie + 1, dB, jc, rq, mB);
-            if constexpr (topZ) la4_map<D>(g, Ie + 1, 0, jc, rq, mZ);
+            la4_map<D>(g, tm, Ie + 1, dA, jc, rq, mA);
+            if constexpr (LB > 0) la4_map<D>(g, tm, Ie + 1, dB, jc, rq, mB);
+            if constexpr (topZ) la4_map<D>(g, tm, Ie + 1, 0, jc, rq, mZ);
             // diagonal 0: slot q holds position ZLO+q; position ZLO goes down to wave WV-1
             {
                 zd *h = m.HO[WV - 1];
@@ -957,15 +958,16 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
     const int nblk = g.nblk, s = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int jc = lane & 15, rq = lane >> 4;
     const int enode = g.enr[A.sys[s].x];
+    const int *tm = g.tmap + (size_t)A.sys[s].x * g.tmap_sys;
     const double *fak = A.fak + (size_t)s * nblk * T16;
     const zd *vals = A.vals + (size_t)s * g.vstride;
     const zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     int err = 0;
     int mi[4];
     d4 dr, di;
-    la4_map<D>(g, 0, 0, jc, rq, mi);
+    la4_map<D>(g, tm, 0, 0, jc, rq, mi);
     la4_vals(vals, mi, dr, di);
-    la4_map<D>(g, 1, 0, jc, rq, mi);
+    la4_map<D>(g, tm, 1, 0, jc, rq, mi);
     for (int J = 0; J < nblk; ++J) {
         st_stage<D>(panel, m, J, tid);
         lds_barrier();
@@ -989,7 +991,7 @@ __device__ __forceinline__ void st_pivot_wave(const Factor16Args &A, StLds<D> &m
         lds_barrier();
         // diagonal tile of column J+1 (map loaded one column ahead)
         la4_vals(vals, mi, dr, di);
-        la4_map<D>(g, J + 2, 0, jc, rq, mi);
+        la4_map<D>(g, tm, J + 2, 0, jc, rq, mi);
         __syncthreads();  // panel column J is read by the next stage
     }
     if (lane == 0 && err && A.status[s] == 0) A.status[s] = err;
@@ -1063,6 +1065,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
     const DevGrid &g = A.g;
     const int nblk = g.nblk, s = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int jc = lane & 15, rq = lane >> 4;
+    const int *tm = g.tmap + (size_t)A.sys[s].x * g.tmap_sys;
     const zd *vals = A.vals + (size_t)s * g.vstride;
     zd *panel = A.panels + (size_t)s * nblk * (D + 1) * 256;
     d4 cr[NA], ci[NA];
@@ -1071,9 +1074,9 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
     for (int u = 0; u < NA; ++u) {
         const int a = C::tile(W, u);
         if (a <= D) {
-            la4_map<D>(g, a, a, jc, rq, mi[u]);
+            la4_map<D>(g, tm, a, a, jc, rq, mi[u]);
             la4_vals(vals, mi[u], cr[u], ci[u]);
-            la4_map<D>(g, 1 + a, a, jc, rq, mi[u]);
+            la4_map<D>(g, tm, 1 + a, a, jc, rq, mi[u]);
         }
     }
     for (int J = 0; J < nblk; ++J) {
@@ -1141,7 +1144,7 @@ __device__ __forceinline__ void st_update_wave(const Factor16Args &A, StLds<D> &
                 }
                 // entering tile of column J+1 (map loaded one column ahead), map of column J+2
                 la4_vals(vals, mi[u], cr[u], ci[u]);
-                la4_map<D>(g, J + 2 + a, a, jc, rq, mi[u]);
+                la4_map<D>(g, tm, J + 2 + a, a, jc, rq, mi[u]);
             }
         }
         __syncthreads();  // panel column J is read by the next stage

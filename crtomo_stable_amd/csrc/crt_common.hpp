@@ -131,6 +131,9 @@ struct DevGrid {
     // tile element is a plain load
     const int *tmap;
     int tmapD;                // D16 the map was built for (0 if none)
+    // systems of different structure in one launch (the condensed forward's interior
+    // pieces): system s reads the map at tmap + sys[s].x * tmap_sys; 0: one map for all
+    long tmap_sys = 0;
     // Every contribution list holds its area elements before its boundary elements
     // (checked on the host), so the area part of an entry is the same sum for every
     // source: area[k*nnz + q] holds it per wavenumber, csplit[q] is where the boundary

@@ -12,6 +12,11 @@
 //     S(l) = A_GG(l) - A_GI Z               dense, complex symmetric
 //     S(l) x_G = b_G - A_GI U(:,l)          dense factorisation per (l, k)
 //     x_I = U(:,l) - Z x_G                  matrix product
+// The interior band is cut into P pieces by windows of one band width, the
+// separators S (no entry reaches across a window): A_II is block diagonal over the
+// pieces, factorised side by side, and S joins the dense block B = [S, G], whose S
+// part is eliminated once per wavenumber (the same for every source) before the
+// per-source factorisation of the G part.
 // This is block Gaussian elimination of the reference's system with the boundary
 // nodes numbered last: the potentials equal those of the band Cholesky of the whole
 // system (src/chol.f90, src/vre.f90) up to rounding.  Rows are scaled symmetrically
@@ -20,7 +25,8 @@
 // sources), so the kernels work on F A F and kpot = f x_hat.
 //
 // Tiles are 16 x 16 complex FP64 on v_mfma_f64_16x16x4f64; a complex product is
-// four real ones.  Lanes: A[i][k] at lane i + 16(k%4) of step k/4, B[k][j] at lane
+// three real ones (Gauss) in the substitution and the interior product, four in the
+// dense factorisation.  Lanes: A[i][k] at lane i + 16(k%4) of step k/4, B[k][j] at lane
 // j + 16(k%4), D[i][j] at lane j + 16(i%4), element i/4.
 #include "crt_common.hpp"
 
@@ -39,6 +45,23 @@ __device__ __forceinline__ void cmma(d4 &cr, d4 &ci, zd a, zd b) {
     cr = mfma4(-a.im, b.im, cr);
     ci = mfma4(a.re, b.im, ci);
     ci = mfma4(a.im, b.re, ci);
+}
+
+// complex products with three real ones (Gauss): p1 = sum ar br, p2 = sum ai bi,
+// p3 = sum (ar + ai)(br + bi); the product is (p1 - p2, p3 - p1 - p2).  The
+// imaginary part loses |re|/|im| ulps to the subtraction (1e-13 relative for the
+// milliradian phases of these potentials).
+struct C3 {
+    d4 p1, p2, p3;
+};
+__device__ __forceinline__ void c3zero(C3 &c) { c.p1 = c.p2 = c.p3 = d4{0.0, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ void c3ma(C3 &c, zd a, zd b) {
+    c.p1 = mfma4(a.re, b.re, c.p1);
+    c.p2 = mfma4(a.im, b.im, c.p2);
+    c.p3 = mfma4(a.re + a.im, b.re + b.im, c.p3);
+}
+__device__ __forceinline__ zd c3get(const C3 &c, int v) {
+    return zd{c.p1[v] - c.p2[v], c.p3[v] - c.p1[v] - c.p2[v]};
 }
 
 __device__ __forceinline__ zd ldz(const zd *p) {
@@ -82,37 +105,45 @@ __global__ void __launch_bounds__(256) crt_cond_gather_kernel(int kc, int k0, in
     areaI[t] = area[(size_t)(k0 + kk) * nnzF + qmap[q]];
 }
 
-// fG[kk][cg] = 1/sqrt|area part of a_gg| (1 where it vanishes and for padding)
-__global__ void __launch_bounds__(256) crt_cond_fg_kernel(int kc, int k0, int nG, int nGp, int nnzF,
+// fB[kk][b] = 1/sqrt|area part of a_bb| for the nodes of B (1 where it vanishes and
+// for padding).  A separator node's diagonal is the whole diagonal: zero (dc: not
+// real and positive) is error 27 of the wavenumber's first system (src/scalab.f90:54-58).
+__global__ void __launch_bounds__(256) crt_cond_fb_kernel(int kc, int k0, int nS, int nBp, int nnzF, int dc, int nP,
                                                           const int *__restrict__ qdiag,
-                                                          const zd *__restrict__ area, double *__restrict__ fG) {
+                                                          const zd *__restrict__ area, double *__restrict__ fB,
+                                                          int *__restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= kc * nGp) return;
-    const int kk = t / nGp, cg = t - kk * nGp;
+    if (t >= kc * nBp) return;
+    const int kk = t / nBp, b = t - kk * nBp;
     double f = 1.0;
-    if (cg < nG) {
-        const double m = zabs(area[(size_t)(k0 + kk) * nnzF + qdiag[cg]]);
+    const int q = qdiag[b];
+    if (q >= 0) {
+        const zd a = area[(size_t)(k0 + kk) * nnzF + q];
+        const double m = zabs(a);
         if (m > 0.0) f = 1.0 / sqrt(m);
+        if (b < nS && (!(m > 0.0) || (dc && !(a.re > 0.0)))) status[(size_t)kk * nP] = CRT_ERR_ZERODIAG;
     }
-    fG[t] = f;
+    fB[t] = f;
 }
 
-// values of the right-hand-side entries of [A_IG | b_I] (scaled): an A_IG entry
-// area(q) fI(i') fG(c), a source entry fI(i')
-__global__ void __launch_bounds__(256) crt_cond_rhs_kernel(int kc, int k0, int nent, int nnzF, int nGp, int fstrideI,
-                                                           const int *__restrict__ erow, const int *__restrict__ ecolg,
-                                                           const int *__restrict__ eq, const zd *__restrict__ area,
-                                                           const double *__restrict__ fI, const double *__restrict__ fG,
-                                                           zd *__restrict__ rv) {
+// values of the right-hand-side entries of each piece's [A_pB | b_p] (scaled): an A_pB
+// entry area(q) fI(row) fB(b), a source entry fI(row).  fI of interior row g at
+// wavenumber kk: fak of the chunk's system kk nP (every piece of a wavenumber scales
+// the interior rows alike).
+__global__ void __launch_bounds__(256) crt_cond_rhs_kernel(int kc, int k0, int nent, int nnzF, int nBp, int fstrideC,
+                                                           int nP, const int *__restrict__ erowg,
+                                                           const int *__restrict__ ebcol, const int *__restrict__ eq,
+                                                           const zd *__restrict__ area, const double *__restrict__ fI,
+                                                           const double *__restrict__ fB, zd *__restrict__ rv) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)kc * nent) return;
     const int kk = (int)(t / nent), e = (int)(t - (long)kk * nent);
-    const double fi = fI[(size_t)kk * fstrideI + erow[e]];
+    const double fi = fI[(size_t)kk * nP * fstrideC + erowg[e]];
     const int q = eq[e];
     zd v;
     if (q >= 0) {
         const zd a = area[(size_t)(k0 + kk) * nnzF + q];
-        const double f = fi * fG[(size_t)kk * nGp + ecolg[e]];
+        const double f = fi * fB[(size_t)kk * nBp + ebcol[e]];
         v = zd{a.re * f, a.im * f};
     } else {
         v = zd{fi, 0.0};
@@ -121,8 +152,8 @@ __global__ void __launch_bounds__(256) crt_cond_rhs_kernel(int kc, int k0, int n
 }
 
 // ------------------------------------------------- multi-RHS band substitution
-// X <- L^-1 X (forward) or X <- L^-T X (backward) for the interior factor of each
-// wavenumber of the chunk, 16 columns per workgroup.  Factor layout of the band
+// X <- L^-1 X (forward) or X <- L^-T X (backward) for the factor of each interior
+// piece of each wavenumber of the chunk (system kk nP + p), 16 columns per workgroup.  Factor layout of the band
 // kernels: panel[J][a] holds the row-major tile L(J+a, J); the diagonal tile holds
 // L on and below its diagonal and L^-1 above it (tile(c, r) = L^-1(r, c), r > c);
 // dinv the reciprocal pivots.  Per block J the products with the D off-diagonal
@@ -135,16 +166,16 @@ __global__ void __launch_bounds__(256) crt_cond_rhs_kernel(int kc, int k0, int n
 // nonzero block j0; X above it is zero and is not written (the backward pass reads
 // zeros there).  At D <= 7 the 52 KB of LDS let three workgroups share a CU.
 struct TrsmArgs {
-    int nblk, ldx, ntile, kc, nvirt;
-    const zd *panels;   // [kk][nblk][D+1][256]
-    const zd *dinv;     // [kk][nblk*16]
-    zd *X;              // [kk][ntile*16][ldx]
-    const int *tptr;    // ntile+1: entries of each column tile
-    const int *erow;    // row (interior index) of each entry
+    int nblk, ldx, ntile, nsys, nP, nvirt;
+    const zd *panels;   // [s][nblk][D+1][256]
+    const zd *dinv;     // [s][nblk*16]
+    zd *X;              // [s][ntile*16][ldx]
+    const int *tptr;    // nP*ntile+1: entries of each (piece, column tile)
+    const int *erow;    // row (within the piece) of each entry
     const int *ecol;    // column within the tile
     const zd *rv;       // [kk][nent]
     int nent;
-    const int *j0;      // first nonzero block of each column tile
+    const int *j0;      // first nonzero block of each (piece, column tile)
 };
 
 template <int D, bool BWD>
@@ -157,15 +188,16 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
     const int b = blockIdx.x;
     const int per = A.nvirt / 8;
     const int v = (b % 8) * per + b / 8;
-    const int kk = v / A.ntile, t = v - kk * A.ntile;
-    if (kk >= A.kc) return;
+    const int sy = v / A.ntile, t = v - sy * A.ntile;
+    if (sy >= A.nsys) return;
+    const int kk = sy / A.nP, pt = (sy - kk * A.nP) * A.ntile + t;  // (piece, tile) of the entry lists
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jc = lane & 15, rq = lane >> 4;
     const int nblk = A.nblk;
-    const zd *panel = A.panels + (size_t)kk * nblk * (D + 1) * 256;
-    const zd *dinv = A.dinv + (size_t)kk * nblk * T16;
-    zd *X = A.X + ((size_t)kk * A.ntile + t) * T16 * A.ldx;
-    const int J0 = A.j0[t];
+    const zd *panel = A.panels + (size_t)sy * nblk * (D + 1) * 256;
+    const zd *dinv = A.dinv + (size_t)sy * nblk * T16;
+    zd *X = A.X + ((size_t)sy * A.ntile + t) * T16 * A.ldx;
+    const int J0 = A.j0[pt];
     for (int i = threadIdx.x; i < R * 256; i += 256) (&ring[0][0])[i] = zd{};
     __syncthreads();
     if (J0 >= nblk && !BWD) return;
@@ -199,8 +231,8 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
     if (wave == 0) {
         zd a1[4], a1n[4], dg[4], dgn[4];
         zd xin[4] = {zd{}, zd{}, zd{}, zd{}}, xinn[4];
-        int wb = BWD ? 0 : A.tptr[t], wc = 0;
-        const int cend = BWD ? 0 : A.tptr[t + 1];
+        int wb = BWD ? 0 : A.tptr[pt], wc = 0;
+        const int cend = BWD ? 0 : A.tptr[pt + 1];
         const zd *rv = BWD ? nullptr : A.rv + (size_t)kk * A.nent;
         int wrow = INT32_MAX, wcol = 0;
         zd wv = zd{};
@@ -226,11 +258,12 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             load_off(J + step, 1, a1n);
             load_diag(J + step, dgn);
             if (BWD) load_x(J + step, xinn);
-            d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+            C3 acc;
+            c3zero(acc);
             {
                 const zd *B = ring[((J - step) % R + R) % R];
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) cmma(cr, ci, a1[ks], B[64 * ks + lane]);
+                for (int ks = 0; ks < 4; ++ks) c3ma(acc, a1[ks], B[64 * ks + lane]);
             }
             // right-hand-side entries of block J (forward): a window of 64 entries sits in
             // registers (lane i holds entry wb + i, wc of them used), each entry goes to
@@ -266,18 +299,20 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int o = 64 * u + lane;
-                T[u].re -= cr[u] + part[pb][0][o].re + part[pb][1][o].re + part[pb][2][o].re;
-                T[u].im -= ci[u] + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
+                const zd a0 = c3get(acc, u);
+                T[u].re -= a0.re + part[pb][0][o].re + part[pb][1][o].re + part[pb][2][o].re;
+                T[u].im -= a0.im + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
             }
             // X_J = Linv T (forward) or Linv^T T (backward); T in the result layout is
             // the B operand of step ks = element ks
-            d4 xr = {0.0, 0.0, 0.0, 0.0}, xi = xr;
+            C3 xa;
+            c3zero(xa);
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) cmma(xr, xi, dg[ks], T[ks]);
+            for (int ks = 0; ks < 4; ++ks) c3ma(xa, dg[ks], T[ks]);
             zd *slot = ring[(J % R + R) % R];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const zd xv = zd{xr[u], xi[u]};
+                const zd xv = c3get(xa, u);
                 slot[64 * u + lane] = xv;
                 stz(X + (size_t)jc * A.ldx + (size_t)J * T16 + 4 * u + rq, xv);
             }
@@ -309,19 +344,20 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             const int a = wave + 1 + 3 * u;
             if (a <= D) load_off(J + step, a, nxt[u]);
         }
-        d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+        C3 acc;
+        c3zero(acc);
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
             const int a = wave + 1 + 3 * u;
             if (a <= D) {
                 const zd *B = ring[((J - step * a) % R + R) % R];
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) cmma(cr, ci, cur[u][ks], B[64 * ks + lane]);
+                for (int ks = 0; ks < 4; ++ks) c3ma(acc, cur[u][ks], B[64 * ks + lane]);
             }
         }
         zd *pw = part[J & 1][wave - 1];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pw[64 * u + lane] = zd{cr[u], ci[u]};
+        for (int u = 0; u < 4; ++u) pw[64 * u + lane] = c3get(acc, u);
         lds_barrier();
 #pragma unroll
         for (int u = 0; u < NA; ++u)
@@ -331,77 +367,85 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
 }
 
 // ------------------------------------------------------ Schur complement
-// S0[kk](cg, c) = A_GG^area(cg, c) - sum_i A_GI(cg, i) Z(i, c)   (lower tiles)
-// G[kk](cg, l)  = b_G(cg, l)       - sum_i A_GI(cg, i) U(i, l)   (column nGp + l of X)
-// scaled; padding rows and columns of S0 are the identity.  S0 is written in the
-// dense kernel's layout: column-major 16 x 16 tiles packed by tile rows, tile (I, J),
+// T[kk](b, c) = A_BB^area(b, c) - sum_i A_Bi(b, i) Z_p(i, c)   (lower tiles over B)
+// G[kk](b, l) = b_B(b, l)      - sum_i A_Bi(b, i) U_p(i, l)
+// scaled, the sums over the interior neighbours i of b in every piece p (Z_p(., c) is
+// zero where c is not coupled to piece p, U_p(., l) where electrode l is not in it);
+// padding rows and columns of T are the identity.  T is written in the dense
+// kernels' layout: column-major 16 x 16 tiles packed by tile rows, tile (I, J),
 // I >= J, at I(I+1)/2 + J.  One thread per entry.
 struct SchurArgs {
-    int kc, k0, nG, nGp, E, ldx, ntile, nnzF, fstrideI;
-    const int *gi_ptr, *gi_row, *gi_q;  // A_GI by boundary row
-    const int *gmap;                   // [nGp][nGp] full-grid entry of (cg, c), -1 none
-    const int *eG;                     // [E] boundary index of the electrode node, -1 interior
-    const zd *area;                    // [K][nnzF]
-    const double *fI, *fG;
-    const zd *X;
-    zd *S0, *Gr;
+    int kc, k0, nBp, E, ldx, ntile, nnzF, fstrideC, nP;
+    const int *gi_ptr, *gi_row, *gi_piece, *gi_q;  // A_Bi by B row: combined interior row, piece, entry
+    const int *roff;                              // first combined row of each piece
+    const int *bcol, *ucol;                       // [p][nBp], [p][E]: column of b / of electrode l in piece p
+    const int *gmap;                              // [nBp][nBp] full-grid entry of (b, c), -1 none
+    const int *eB;                                // [E] B index of the electrode node, -1 interior
+    const zd *area;                               // [K][nnzF]
+    const double *fI, *fB;
+    const zd *X;                                  // [kk nP + p][ntile*16][ldx]
+    zd *T, *Gr;
 };
 
 __global__ void __launch_bounds__(256) crt_cond_schur_kernel(SchurArgs A) {
-    const int nt = A.nGp / T16;
-    const long ns = (long)nt * (nt + 1) / 2 * 256, ng = (long)A.E * A.nGp, per = ns + ng;
+    const int ntb = A.nBp / T16;
+    const long ns = (long)ntb * (ntb + 1) / 2 * 256, ng = (long)A.E * A.nBp, per = ns + ng;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)A.kc * per) return;
     const int kk = (int)(t / per);
     const long o = t - (long)kk * per;
     const int k = A.k0 + kk;
-    const bool zpart = o < ns;
-    int cg, col;
-    if (zpart) {
+    const bool tpart = o < ns;
+    int b, col, l = 0;
+    if (tpart) {
         const int tile = (int)(o / 256), e = (int)(o % 256);
         int I = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
         while ((I + 1) * (I + 2) / 2 <= tile) ++I;
         while (I * (I + 1) / 2 > tile) --I;
         const int J = tile - I * (I + 1) / 2;
-        cg = 16 * I + e % 16;
+        b = 16 * I + e % 16;
         col = 16 * J + e / 16;
     } else {
         const long og = o - ns;
-        cg = (int)(og % A.nGp);
-        col = A.nGp + (int)(og / A.nGp);  // X column nGp + l
+        b = (int)(og % A.nBp);
+        l = (int)(og / A.nBp);
+        col = -1;
     }
     zd s = zd{0.0, 0.0};
-    if (cg < A.nG) {
-        const double fg = A.fG[(size_t)kk * A.nGp + cg];
-        const zd *Xc = A.X + ((size_t)kk * A.ntile * T16 + col) * A.ldx;
-        for (int p = A.gi_ptr[cg]; p < A.gi_ptr[cg + 1]; ++p) {
-            const int i = A.gi_row[p];
+    const double fb = A.fB[(size_t)kk * A.nBp + b];
+    const int q0 = A.gmap[(size_t)b * A.nBp + b];  // >= 0 for a node of B, -1 for padding
+    if (q0 >= 0) {
+        for (int p = A.gi_ptr[b]; p < A.gi_ptr[b + 1]; ++p) {
+            const int g = A.gi_row[p], pc = A.gi_piece[p];
+            const int xc = tpart ? A.bcol[(size_t)pc * A.nBp + col] : A.ucol[(size_t)pc * A.E + l];
+            if (xc < 0) continue;
             const zd a = A.area[(size_t)k * A.nnzF + A.gi_q[p]];
-            const double f = fg * A.fI[(size_t)kk * A.fstrideI + i];
-            const zd x = Xc[i];
+            const double f = fb * A.fI[(size_t)kk * A.nP * A.fstrideC + g];
+            const zd x = A.X[(((size_t)kk * A.nP + pc) * A.ntile * T16 + xc) * A.ldx + (g - A.roff[pc])];
             s.re -= f * (a.re * x.re - a.im * x.im);
             s.im -= f * (a.re * x.im + a.im * x.re);
         }
-        if (zpart) {
-            const int q = col < A.nG ? A.gmap[(size_t)col * A.nGp + cg] : -1;
+        if (tpart) {
+            const int q = A.gmap[(size_t)col * A.nBp + b];
             if (q >= 0) {
                 const zd a = A.area[(size_t)k * A.nnzF + q];
-                const double f = fg * A.fG[(size_t)kk * A.nGp + col];
+                const double f = fb * A.fB[(size_t)kk * A.nBp + col];
                 s.re += a.re * f;
                 s.im += a.im * f;
             }
-        } else if (A.eG[col - A.nGp] == cg) {
-            s.re += fg;
+        } else if (A.eB[l] == b) {
+            s.re += fb;
         }
-    } else if (zpart && col == cg) {
+    } else if (tpart && col == b) {
         s = zd{1.0, 0.0};
     }
-    if (zpart) A.S0[(size_t)kk * ns + o] = s;
-    else A.Gr[((size_t)kk * A.E + (col - A.nGp)) * A.nGp + cg] = s;
+    if (tpart) A.T[(size_t)kk * ns + o] = s;
+    else A.Gr[((size_t)kk * A.E + l) * A.nBp + b] = s;
 }
 
 // scaled boundary terms of every system: bval[s][p] = (type-11 part of entry bq[p] for
-// source l at wavenumber k) fG(g1[p]) fG(g2[p]), s = kk E + l
+// source l at wavenumber k) fG(g1[p]) fG(g2[p]), s = kk E + l, fG of wavenumber kk at
+// fG + kk nGp
 __global__ void __launch_bounds__(256) crt_cond_bval_kernel(DevGrid g, int nsys, int E, int k0, int nGp, int nb,
                                                             const int *__restrict__ bq, const int2 *__restrict__ bg,
                                                             const double *__restrict__ fG, zd *__restrict__ bval) {
@@ -416,25 +460,33 @@ __global__ void __launch_bounds__(256) crt_cond_bval_kernel(DevGrid g, int nsys,
 }
 
 // --------------------------------------------------- dense factor and solve
-// One workgroup (eight waves) per system (l, kk): S(l) = S0 + the boundary terms of source l
-// (scaled: S0's tile plus the tile's list of entries that hold boundary terms),
-// factorised S = L L^T without conjugation by tile columns (left-looking:
-// the tiles of column J are accumulated in registers over the row J tiles staged in
-// LDS, wave 0 factors the diagonal tile, each wave divides its tiles by L_JJ^T), then
-// L L^T x = g by substitution.  Tiles are column-major 16 x 16, packed by tile rows:
-// tile (I, J), I >= J, at I(I+1)/2 + J.  A zero diagonal of S(l) before scaling is
-// error 27 (src/scalab.f90:54-58), a zero pivot 28 (src/chol.f90:66-70); with dc a
-// diagonal or pivot that is not real and positive is (src/scaldc.f90, src/choldc.f90).
+// One workgroup (eight waves) per system.  Source systems (l, kk): S(l) = S0 + the
+// boundary terms of source l over the G block (scaled: S0's tile plus the tile's list
+// of entries that hold boundary terms), factorised S = L L^T without conjugation by
+// tile columns (left-looking: the tiles of column J are accumulated in registers over
+// the row J tiles staged in LDS, wave 0 factors the diagonal tile, each wave divides
+// its tiles by L_JJ^T); then L_B L_B^T x = g over all of B by substitution, the
+// separator columns of L_B from the wavenumber's prefix.  Prefix systems (PRE, one per
+// wavenumber): the same factorisation of T over B for the JS separator tile columns
+// (L_SS, L_GS), and the G block's Schur complement S0 = T_GG - L_GS L_GS^T for the
+// source systems.  Tiles are column-major 16 x 16, packed by tile rows: tile (I, J),
+// I >= J, at I(I+1)/2 + J.  A zero diagonal of S(l) before scaling is error 27
+// (src/scalab.f90:54-58), a zero pivot 28 (src/chol.f90:66-70); with dc a diagonal or
+// pivot that is not real and positive is (src/scaldc.f90, src/choldc.f90).
 struct DenseArgs {
     DevGrid g;  // full grid: boundary contributions, area sums
-    int nG, nGp, nt, E, K, k0;
-    const zd *S0, *Gr;
-    const int *bptr, *bent, *qdiag, *nodeG;  // boundary entries of each tile: element c*16 + r
-    const zd *bval;                          // [s][nb] their values for each system
+    int nG, nt, nBp, JS, E, K, k0;  // nt: G tiles, JS: separator tiles, nBp = 16 (JS + nt)
+    const zd *S0, *T, *Gr;          // S0 [kk][G tiles], T [kk][B tiles], Gr [kk][E][nBp]
+    const int *bptr, *bent, *qdiag, *nodeB;  // boundary entries per G tile (element c*16 + r); G diagonals
+    const zd *bval;                          // [s][nb] values of the boundary entries
     int nb;
-    const double *fG;
-    zd *Lt, *Li, *XG, *kpotN;
-    int *status;  // [kk][E]
+    const double *fB;                        // [kk][nBp]
+    zd *Lt, *Li;                             // [s][G tiles], [s][nt]
+    zd *Lpre, *Lipre;                        // [kk][B tiles] (separator columns), [kk][JS]
+    zd *S0w;                                 // prefix output: S0 [kk][G tiles]
+    zd *XB, *kpotN;                          // XB [kk][E][nBp]
+    int *status;
+    int sstride;                             // status slots per system
 };
 
 namespace {
@@ -451,55 +503,58 @@ struct alignas(16) DenseLds {
 };
 }  // namespace
 
-template <int NI>
+template <int NI, bool PRE>
 __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A) {
-    constexpr int DQ = NI >= 3 ? 4 : 8;  // tiles per batch of loads
     __shared__ DenseLds m;
-    extern __shared__ zd yv[];  // nGp
+    extern __shared__ zd yv[];  // nBp
     const int s = blockIdx.x;
-    const int l = s % A.E, kk = s / A.E;
+    const int l = PRE ? 0 : s % A.E, kk = PRE ? s : s / A.E;
     const int k = A.k0 + kk;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jc = lane & 15, rq = lane >> 4;
-    const int nt = A.nt, nGp = A.nGp, nG = A.nG;
-    const size_t ntri = (size_t)nt * (nt + 1) / 2;
-    zd *Ls = A.Lt + (size_t)s * ntri * 256;
-    zd *Lis = A.Li + (size_t)s * nt * 256;
-    const zd *S0 = A.S0 + (size_t)kk * ntri * 256;
-    const zd *bval = A.bval + (size_t)s * A.nb;
-    const double *fG = A.fG + (size_t)kk * nGp;
+    const int JS = A.JS, ntB = A.nBp / T16, nG = A.nG;
+    const int nt = PRE ? ntB : A.nt;  // tile columns of the matrix factorised here
+    const size_t ntri = (size_t)A.nt * (A.nt + 1) / 2, ntriB = (size_t)ntB * (ntB + 1) / 2;
+    zd *Ls = PRE ? A.Lpre + (size_t)kk * ntriB * 256 : A.Lt + (size_t)s * ntri * 256;
+    zd *Lis = PRE ? A.Lipre + (size_t)kk * JS * 256 : A.Li + (size_t)s * A.nt * 256;
+    const zd *M = PRE ? A.T + (size_t)kk * ntriB * 256 : A.S0 + (size_t)kk * ntri * 256;
+    const zd *bval = A.bval + (size_t)(PRE ? 0 : s) * A.nb;
+    const double *fB = A.fB + (size_t)kk * A.nBp;
     const DevGrid &g = A.g;
     if (threadIdx.x == 0) m.err = 0;
     __syncthreads();
     auto tri = [](int I, int J) { return (size_t)I * (I + 1) / 2 + J; };
-    // scalab's check of every diagonal before the factorisation
-    for (int cg = threadIdx.x; cg < nG; cg += 64 * DNW) {
-        const int q = A.qdiag[cg];
-        const zd bp = boundary_part(g, q, l, k);
-        const zd a = g.area[(size_t)k * g.nnz + q];
-        const zd full = zd{a.re + bp.re, a.im + bp.im};
-        if (!(zabs(full) > 0.0) || (g.dc && !(full.re > 0.0))) m.err = CRT_ERR_ZERODIAG;
-    }
-    __syncthreads();
-    if (m.err) {
-        if (threadIdx.x == 0) A.status[s] = m.err;
-        return;
+    if (!PRE) {
+        // scalab's check of every boundary diagonal before the factorisation
+        for (int cg = threadIdx.x; cg < nG; cg += 64 * DNW) {
+            const int q = A.qdiag[cg];
+            const zd bp = boundary_part(g, q, l, k);
+            const zd a = g.area[(size_t)k * g.nnz + q];
+            const zd full = zd{a.re + bp.re, a.im + bp.im};
+            if (!(zabs(full) > 0.0) || (g.dc && !(full.re > 0.0))) m.err = CRT_ERR_ZERODIAG;
+        }
+        __syncthreads();
+        if (m.err) {
+            if (threadIdx.x == 0) A.status[(size_t)s * A.sstride] = m.err;
+            return;
+        }
     }
 
     for (int J = 0; J < nt; ++J) {
+        const int pend = PRE ? min(J, JS) : J;  // columns of L in this column's update
         d4 ar[NI], ai[NI];
         // S(l) entries of the owned tiles I = J + wave + DNW u, column J
 #pragma unroll
         for (int u = 0; u < NI; ++u) {
             const int I = J + wave + DNW * u;
-            const zd *st = S0 + tri(I < nt ? I : J, J) * 256;
+            const zd *st = M + tri(I < nt ? I : J, J) * 256;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const zd e = I < nt ? ldz(st + jc * T16 + 4 * v + rq) : zd{};
                 ar[u][v] = e.re;
                 ai[u][v] = e.im;
             }
-            if (I < nt) {
+            if (!PRE && I < nt) {
                 // the tile's boundary terms of source l: lane i reads entry p0 + i, then
                 // each goes to the lane that holds its element (r, c) = (4v + rq, jc)
                 const int ti = (int)tri(I, J);
@@ -522,35 +577,59 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
                 }
             }
         }
-        for (int p0 = 0; p0 < J; p0 += DCH) {
-            const int np = min(DCH, J - p0);
+        for (int p0 = 0; p0 < pend; p0 += DCH) {
+            const int np = min(DCH, pend - p0);
             __syncthreads();
             const zd *src = Ls + tri(J, p0) * 256;
             for (int i = threadIdx.x; i < np * 256; i += 64 * DNW) (&m.rowJ[0][0])[i] = ldz(src + i);
             __syncthreads();
-            // per owned tile: the chunk's tiles L(I, p0..) (contiguous) loaded DQ at a time,
-            // all in flight together, then the products
+            // per owned tile: the chunk's tiles L(I, p0..) (contiguous), two at a time, the
+            // next pair in flight while the current one is multiplied (the pipeline runs
+            // on across the owned tiles)
+            const int nv = min(NI, max(0, (nt - (J + wave) + DNW - 1) / DNW));
+            zd bc[2][4], bn[2][4];
+            auto ld2 = [&](int u, int pq, zd (&o)[2][4]) {
+                const zd *tp = Ls + tri(J + wave + DNW * u, p0) * 256;
+                const bool live = u < nv;
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks)
+                        o[j][ks] = live && pq + j < np ? ldz(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
+            };
+            ld2(0, 0, bc);
 #pragma unroll
             for (int u = 0; u < NI; ++u) {
-                const int I = J + wave + DNW * u;
-                if (I < nt) {
-                    const zd *tp = Ls + tri(I, p0) * 256;
-                    for (int pq = 0; pq < np; pq += DQ) {
-                        zd ao[DQ][4];
+                if (u < nv) {
+                    for (int pq = 0; pq < np; pq += 2) {
+                        const bool last = pq + 2 >= np;
+                        ld2(last ? u + 1 : u, last ? 0 : pq + 2, bn);
 #pragma unroll
-                        for (int j = 0; j < DQ; ++j)
-#pragma unroll
-                            for (int ks = 0; ks < 4; ++ks)
-                                ao[j][ks] = pq + j < np ? ldz(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
-#pragma unroll
-                        for (int j = 0; j < DQ; ++j)
+                        for (int j = 0; j < 2; ++j)
                             if (pq + j < np)
 #pragma unroll
                                 for (int ks = 0; ks < 4; ++ks)
-                                    cmma(ar[u], ai[u], zd{-ao[j][ks].re, -ao[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
+                                    cmma(ar[u], ai[u], zd{-bc[j][ks].re, -bc[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int ks = 0; ks < 4; ++ks) bc[j][ks] = bn[j][ks];
                     }
                 }
             }
+        }
+        if (PRE && J >= JS) {
+            // the G block's Schur complement: tile (I - JS, J - JS) of S0
+            zd *o = A.S0w + (size_t)kk * ntri * 256;
+#pragma unroll
+            for (int u = 0; u < NI; ++u) {
+                const int I = J + wave + DNW * u;
+                if (I < nt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        stz(o + tri(I - JS, J - JS) * 256 + jc * T16 + 4 * v + rq, zd{ar[u][v], ai[u][v]});
+            }
+            continue;
         }
         // diagonal tile: wave 0 factors it in LDS
         if (wave == 0) {
@@ -627,18 +706,25 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     }
     __syncthreads();
     if (m.err) {
-        if (threadIdx.x == 0) A.status[s] = m.err;
+        if (threadIdx.x == 0) A.status[(size_t)s * A.sstride] = m.err;
         return;
     }
-    // ---- L L^T x = g: forward over tile rows, backward over tile columns
-    const zd *gr = A.Gr + ((size_t)kk * A.E + l) * nGp;
-    for (int i = threadIdx.x; i < nGp; i += 64 * DNW) yv[i] = gr[i];
+    if constexpr (PRE) return;
+    // ---- L_B L_B^T x = g over B: forward over tile rows, backward over tile columns
+    const zd *Lp = A.Lpre + (size_t)kk * ntriB * 256, *Lip = A.Lipre + (size_t)kk * JS * 256;
+    auto tileL = [&](int I, int J) -> const zd * {
+        return J < JS ? Lp + tri(I, J) * 256 : Ls + tri(I - JS, J - JS) * 256;
+    };
+    auto tileLi = [&](int J) -> const zd * { return J < JS ? Lip + (size_t)J * 256 : Lis + (size_t)(J - JS) * 256; };
+    const int nBp = A.nBp;
+    const zd *gr = A.Gr + ((size_t)kk * A.E + l) * nBp;
+    for (int i = threadIdx.x; i < nBp; i += 64 * DNW) yv[i] = gr[i];
     __syncthreads();
-    for (int J = 0; J < nt; ++J) {
+    for (int J = 0; J < ntB; ++J) {
         // lane (r, q) of step w holds L(r, 4w + q)
         zd acc = zd{};
         for (int p = wave; p < J; p += DNW) {
-            const zd *tp = Ls + tri(J, p) * 256;
+            const zd *tp = tileL(J, p);
 #pragma unroll
             for (int w = 0; w < 4; ++w) acc = zfma(acc, ldz(tp + 64 * w + lane), yv[16 * p + 4 * w + rq]);
         }
@@ -657,7 +743,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
                 m.ws[0][r] = tv;
             }
             __builtin_amdgcn_wave_barrier();
-            const zd *lt = Lis + (size_t)J * 256;
+            const zd *lt = tileLi(J);
             zd y = zd{};
 #pragma unroll
             for (int w = 0; w < 4; ++w) y = zfma(y, ldz(lt + 64 * w + lane), m.ws[0][4 * w + rq]);
@@ -669,11 +755,11 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
         }
         __syncthreads();
     }
-    for (int J = nt - 1; J >= 0; --J) {
+    for (int J = ntB - 1; J >= 0; --J) {
         // out(c) = sum_r L(I,J)(r, c) x(16I + r): lane (r, q) of step w holds L(r, 4w + q)
         zd acc[4] = {zd{}, zd{}, zd{}, zd{}};
-        for (int I = J + 1 + wave; I < nt; I += DNW) {
-            const zd *tp = Ls + tri(I, J) * 256;
+        for (int I = J + 1 + wave; I < ntB; I += DNW) {
+            const zd *tp = tileL(I, J);
             const zd xr = yv[16 * I + jc];
 #pragma unroll
             for (int w = 0; w < 4; ++w) acc[w] = zfma(acc[w], ldz(tp + 64 * w + lane), xr);
@@ -698,7 +784,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             }
             __builtin_amdgcn_wave_barrier();
             // x_J(c) = sum_r Linv(r, c) t(r)
-            const zd *lt = Lis + (size_t)J * 256;
+            const zd *lt = tileLi(J);
             zd xo[4];
             const zd tr = m.ws[0][jc];
 #pragma unroll
@@ -717,70 +803,82 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
         }
         __syncthreads();
     }
-    zd *xg = A.XG + ((size_t)kk * A.E + l) * nGp;
-    for (int i = threadIdx.x; i < nGp; i += 64 * DNW) {
+    zd *xb = A.XB + ((size_t)kk * A.E + l) * nBp;
+    for (int i = threadIdx.x; i < nBp; i += 64 * DNW) {
         const zd x = yv[i];
-        xg[i] = x;
-        if (i < nG) {
-            const double f = fG[i];
-            A.kpotN[((size_t)A.nodeG[i] * A.K + k) * A.E + l] = zd{x.re * f, x.im * f};
+        xb[i] = x;
+        const int nd = A.nodeB[i];
+        if (nd >= 0) {
+            const double f = fB[i];
+            A.kpotN[((size_t)nd * A.K + k) * A.E + l] = zd{x.re * f, x.im * f};
         }
     }
 }
 
 // ------------------------------------------------ interior potentials
-// kpot(node(i), k, l) = fI(i) (U(i, l) - sum_c Z(i, c) x_G(c, l)): each wave one
-// 16-row tile times four 16-column tiles of x_G, K-loop over the boundary tiles.
+// kpot(node(i), k, l) = fI(i) (U_p(i, l) - sum_c Z_p(i, c) x_B(b(c), l)) for the rows i of
+// piece p: each wave one 16-row tile times four 16-column tiles of x_B (gathered
+// through the piece's column map), K-loop over the piece's coupled columns.
 struct GemmArgs {
-    int kc, k0, nI, ldx, ntile, nGp, E, K, fstrideI, rgroups, cgroups;
-    const zd *X, *XG;
+    int nsys, nP, k0, ldx, ntile, ncolP, nBp, E, K, fstrideC, rgroups, cgroups;
+    const zd *X, *XB;
     const double *fI;
-    const int *nodeI;
+    const int *nodeI, *roff, *plen, *nbc, *colmap, *ucol;
     zd *kpotN;
 };
 
 __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
     const int b = blockIdx.x;
-    const int rg = b % A.rgroups, cgp = (b / A.rgroups) % A.cgroups, kk = b / (A.rgroups * A.cgroups);
-    if (kk >= A.kc) return;
+    const int rg = b % A.rgroups, cgp = (b / A.rgroups) % A.cgroups, sy = b / (A.rgroups * A.cgroups);
+    if (sy >= A.nsys) return;
+    const int kk = sy / A.nP, p = sy - kk * A.nP;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jc = lane & 15, rq = lane >> 4;
     const int rt = 4 * rg + wave;  // row tile
-    if (rt * T16 >= A.ldx) return;
+    const int plen = A.plen[p];
+    if (rt * T16 >= plen) return;
     const int k = A.k0 + kk;
-    const zd *X = A.X + (size_t)kk * A.ntile * T16 * A.ldx;
-    const zd *XG = A.XG + (size_t)kk * A.E * A.nGp;
-    d4 cr[4], ci[4];
+    const zd *X = A.X + (size_t)sy * A.ntile * T16 * A.ldx;
+    const zd *XB = A.XB + (size_t)kk * A.E * A.nBp;
+    const int *cmap = A.colmap + (size_t)p * A.ncolP;
+    C3 acc[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) cr[c] = ci[c] = d4{0.0, 0.0, 0.0, 0.0};
-    const int nkt = A.nGp / T16;
+    for (int c = 0; c < 4; ++c) c3zero(acc[c]);
+    const int nkt = (A.nbc[p] + 15) / T16;
     for (int kt = 0; kt < nkt; ++kt) {
         zd ao[4];
+        int bc[4];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) ao[ks] = ldz(X + (size_t)(16 * kt + 4 * ks + rq) * A.ldx + 16 * rt + jc);
+        for (int ks = 0; ks < 4; ++ks) {
+            ao[ks] = ldz(X + (size_t)(16 * kt + 4 * ks + rq) * A.ldx + 16 * rt + jc);
+            bc[ks] = cmap[16 * kt + 4 * ks + rq];
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int lcol = 16 * (4 * cgp + c) + jc;
             if (16 * (4 * cgp + c) < A.E) {
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
-                    const zd bo = lcol < A.E ? ldz(XG + (size_t)lcol * A.nGp + 16 * kt + 4 * ks + rq) : zd{};
-                    cmma(cr[c], ci[c], ao[ks], bo);
+                    const zd bo = (lcol < A.E && bc[ks] >= 0) ? ldz(XB + (size_t)lcol * A.nBp + bc[ks]) : zd{};
+                    c3ma(acc[c], ao[ks], bo);
                 }
             }
         }
     }
+    const int r0 = A.roff[p];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int lcol = 16 * (4 * cgp + c) + jc;
         if (lcol >= A.E) continue;
+        const int uc = A.ucol[(size_t)p * A.E + lcol];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = 16 * rt + 4 * v + rq;
-            if (i >= A.nI) continue;
-            const zd u = X[(size_t)(A.nGp + lcol) * A.ldx + i];
-            const double f = A.fI[(size_t)kk * A.fstrideI + i];
-            A.kpotN[((size_t)A.nodeI[i] * A.K + k) * A.E + lcol] = zd{(u.re - cr[c][v]) * f, (u.im - ci[c][v]) * f};
+            if (i >= plen) continue;
+            const zd u = uc >= 0 ? X[(size_t)uc * A.ldx + i] : zd{};
+            const double f = A.fI[(size_t)kk * A.nP * A.fstrideC + r0 + i];
+            const zd zx = c3get(acc[c], v);
+            A.kpotN[((size_t)A.nodeI[r0 + i] * A.K + k) * A.E + lcol] = zd{(u.re - zx.re) * f, (u.im - zx.im) * f};
         }
     }
 }
@@ -794,36 +892,36 @@ int crt_launch_cond_gather(hipStream_t st, int kc, int k0, int nnzF, int nnzI, c
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-int crt_launch_cond_fg(hipStream_t st, int kc, int k0, int nG, int nGp, int nnzF, const int *qdiag, const zd *area,
-                       double *fG) {
-    const int n = kc * nGp;
-    hipLaunchKernelGGL(crt_cond_fg_kernel, dim3((n + 255) / 256), dim3(256), 0, st, kc, k0, nG, nGp, nnzF, qdiag, area,
-                       fG);
+int crt_launch_cond_fb(hipStream_t st, int kc, int k0, int nS, int nBp, int nnzF, int dc, int nP, const int *qdiag,
+                       const zd *area, double *fB, int *status) {
+    const int n = kc * nBp;
+    hipLaunchKernelGGL(crt_cond_fb_kernel, dim3((n + 255) / 256), dim3(256), 0, st, kc, k0, nS, nBp, nnzF, dc, nP, qdiag,
+                       area, fB, status);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-int crt_launch_cond_rhs(hipStream_t st, int kc, int k0, int nent, int nnzF, int nGp, int fstrideI, const int *erow,
-                        const int *ecolg, const int *eq, const zd *area, const double *fI, const double *fG,
-                        zd *rv) {
+int crt_launch_cond_rhs(hipStream_t st, int kc, int k0, int nent, int nnzF, int nBp, int fstrideC, int nP,
+                        const int *erowg, const int *ebcol, const int *eq, const zd *area, const double *fI,
+                        const double *fB, zd *rv) {
     const long n = (long)kc * nent;
     if (n == 0) return 0;
     hipLaunchKernelGGL(crt_cond_rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kc, k0, nent, nnzF,
-                       nGp, fstrideI, erow, ecolg, eq, area, fI, fG, rv);
+                       nBp, fstrideC, nP, erowg, ebcol, eq, area, fI, fB, rv);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 template <int D>
 static int launch_trsm(hipStream_t st, TrsmArgs a, bool bwd) {
-    a.nvirt = (a.ntile * a.kc + 7) / 8 * 8;
+    a.nvirt = (a.ntile * a.nsys + 7) / 8 * 8;
     if (bwd) hipLaunchKernelGGL((crt_cond_trsm_kernel<D, true>), dim3(a.nvirt), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((crt_cond_trsm_kernel<D, false>), dim3(a.nvirt), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-int crt_launch_cond_trsm(hipStream_t st, int D, int bwd, int nblk, int ldx, int ntile, int kc, const zd *panels,
-                         const zd *dinv, zd *X, const int *tptr, const int *erow, const int *ecol, const zd *rv,
-                         int nent, const int *j0) {
-    TrsmArgs a{nblk, ldx, ntile, kc, 0, panels, dinv, X, tptr, erow, ecol, rv, nent, j0};
+int crt_launch_cond_trsm(hipStream_t st, int D, int bwd, int nblk, int ldx, int ntile, int nsys, int nP,
+                         const zd *panels, const zd *dinv, zd *X, const int *tptr, const int *erow, const int *ecol,
+                         const zd *rv, int nent, const int *j0) {
+    TrsmArgs a{nblk, ldx, ntile, nsys, nP, 0, panels, dinv, X, tptr, erow, ecol, rv, nent, j0};
     switch (D) {
         case 2: return launch_trsm<2>(st, a, bwd);
         case 3: return launch_trsm<3>(st, a, bwd);
@@ -844,59 +942,78 @@ int crt_launch_cond_trsm(hipStream_t st, int D, int bwd, int nblk, int ldx, int 
     }
 }
 
-int crt_launch_cond_schur(hipStream_t st, int kc, int k0, int nG, int nGp, int E, int ldx, int ntile, int nnzF,
-                          int fstrideI, const int *gi_ptr, const int *gi_row, const int *gi_q, const int *gmap,
-                          const int *eG, const zd *area, const double *fI, const double *fG, const zd *X, zd *S0,
-                          zd *Gr) {
-    SchurArgs a{kc, k0, nG, nGp, E, ldx, ntile, nnzF, fstrideI, gi_ptr, gi_row, gi_q, gmap, eG, area, fI, fG, X, S0, Gr};
-    const long nt = nGp / T16;
-    const long n = (long)kc * (nt * (nt + 1) / 2 * 256 + (long)E * nGp);
+int crt_launch_cond_schur(hipStream_t st, int kc, int k0, int nBp, int E, int ldx, int ntile, int nnzF, int fstrideC,
+                          int nP, const int *gi_ptr, const int *gi_row, const int *gi_piece, const int *gi_q,
+                          const int *roff, const int *bcol, const int *ucol, const int *gmap, const int *eB,
+                          const zd *area, const double *fI, const double *fB, const zd *X, zd *T, zd *Gr) {
+    SchurArgs a{kc,   k0,   nBp,  E,     ldx,  ntile,  nnzF, fstrideC, nP, gi_ptr, gi_row, gi_piece, gi_q,
+                roff, bcol, ucol, gmap,  eB,   area,   fI,   fB,       X,  T,      Gr};
+    const long nt = nBp / T16;
+    const long n = (long)kc * (nt * (nt + 1) / 2 * 256 + (long)E * nBp);
     hipLaunchKernelGGL(crt_cond_schur_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-// largest boundary tile count the dense kernel takes (five owned tiles per wave)
+// largest boundary tile count of the source systems and of B in the prefix (five and
+// six owned tiles per wave)
 int crt_cond_dense_max_tiles() { return 40; }
+int crt_cond_prefix_max_tiles() { return 48; }
 
-template <int NI>
-static int launch_dense(hipStream_t st, const DenseArgs &a, int nsys) {
-    const size_t lds = (size_t)a.nGp * sizeof(zd);
-    hipLaunchKernelGGL(crt_cond_dense_kernel<NI>, dim3(nsys), dim3(64 * DNW), lds, st, a);
+int crt_launch_cond_bval(hipStream_t st, const DevGrid &g, int nsys, int E, int k0, int fstride, int nb, const int *bq,
+                         const int2 *bg, const double *fG, zd *bval) {
+    const long n = (long)nsys * nb;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(crt_cond_bval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, E, k0,
+                       fstride, nb, bq, bg, fG, bval);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int nG, int nGp, int E, int K, int k0, int nsys,
-                          const zd *S0, const zd *Gr, const int *bptr, const int *bent, const zd *bval, int nb,
-                          const int *qdiag, const int *nodeG, const double *fG, zd *Lt, zd *Li, zd *XG, zd *kpotN,
-                          int *status) {
-    const int nt = nGp / T16;
-    if (nt < 1 || nt > crt_cond_dense_max_tiles() || nGp % T16) return CRT_ERR_ARG;
-    DenseArgs a{g,  nG,    nGp,  nt, E,     K,  k0, S0, Gr, bptr, bent, qdiag, nodeG, bval,
-                nb, fG,    Lt,   Li, XG,    kpotN, status};
+template <int NI, bool PRE>
+static int launch_dense(hipStream_t st, const DenseArgs &a, int nsys) {
+    const size_t lds = (size_t)a.nBp * sizeof(zd);
+    hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE>), dim3(nsys), dim3(64 * DNW), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+// pre = 0: the source systems (nsys = kc E); pre = 1: the prefixes (nsys = kc)
+int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int pre, int nG, int nt, int nBp, int JS, int E, int K,
+                          int k0, int nsys, const zd *S0, const zd *T, const zd *Gr, const int *bptr, const int *bent,
+                          const int *qdiag, const int *nodeB, const zd *bval, int nb, const double *fB, zd *Lt,
+                          zd *Li, zd *Lpre, zd *Lipre, zd *S0w, zd *XB, zd *kpotN, int *status, int sstride) {
+    const int ntB = nBp / T16;
+    if (nt < 1 || nt > crt_cond_dense_max_tiles() || nBp % T16 || ntB != nt + JS) return CRT_ERR_ARG;
+    if (pre && (JS < 1 || ntB > crt_cond_prefix_max_tiles())) return CRT_ERR_ARG;
+    DenseArgs a{g,     nG,     nt,   nBp,  JS,   E,     K,     k0,   S0,  T,       Gr,      bptr,  bent,
+                qdiag, nodeB, bval, nb,   fB,   Lt,    Li,    Lpre, Lipre, S0w,   XB,      kpotN, status, sstride};
+    if (pre) {
+        switch ((ntB + DNW - 1) / DNW) {
+            case 1: return launch_dense<1, true>(st, a, nsys);
+            case 2: return launch_dense<2, true>(st, a, nsys);
+            case 3: return launch_dense<3, true>(st, a, nsys);
+            case 4: return launch_dense<4, true>(st, a, nsys);
+            case 5: return launch_dense<5, true>(st, a, nsys);
+            case 6: return launch_dense<6, true>(st, a, nsys);
+            default: return CRT_ERR_UNSUPPORTED;
+        }
+    }
     switch ((nt + DNW - 1) / DNW) {
-        case 1: return launch_dense<1>(st, a, nsys);
-        case 2: return launch_dense<2>(st, a, nsys);
-        case 3: return launch_dense<3>(st, a, nsys);
-        case 4: return launch_dense<4>(st, a, nsys);
-        case 5: return launch_dense<5>(st, a, nsys);
+        case 1: return launch_dense<1, false>(st, a, nsys);
+        case 2: return launch_dense<2, false>(st, a, nsys);
+        case 3: return launch_dense<3, false>(st, a, nsys);
+        case 4: return launch_dense<4, false>(st, a, nsys);
+        case 5: return launch_dense<5, false>(st, a, nsys);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }
 
-int crt_launch_cond_bval(hipStream_t st, const DevGrid &g, int nsys, int E, int k0, int nGp, int nb, const int *bq,
-                         const int2 *bg, const double *fG, zd *bval) {
-    const long n = (long)nsys * nb;
-    if (n == 0) return 0;
-    hipLaunchKernelGGL(crt_cond_bval_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nsys, E, k0, nGp,
-                       nb, bq, bg, fG, bval);
-    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
-}
-
-int crt_launch_cond_gemm(hipStream_t st, int kc, int k0, int nI, int ldx, int ntile, int nGp, int E, int K,
-                         int fstrideI, const zd *X, const zd *XG, const double *fI, const int *nodeI, zd *kpotN) {
+int crt_launch_cond_gemm(hipStream_t st, int nsys, int nP, int k0, int ldx, int ntile, int ncolP, int nBp, int E,
+                         int K, int fstrideC, const zd *X, const zd *XB, const double *fI, const int *nodeI,
+                         const int *roff, const int *plen, const int *nbc, const int *colmap, const int *ucol,
+                         zd *kpotN) {
     const int rgroups = (ldx / T16 + 3) / 4, cgroups = ((E + 15) / 16 + 3) / 4;
-    GemmArgs a{kc, k0, nI, ldx, ntile, nGp, E, K, fstrideI, rgroups, cgroups, X, XG, fI, nodeI, kpotN};
-    const long n = (long)rgroups * cgroups * kc;
+    GemmArgs a{nsys, nP,   k0,  ldx,  ntile, ncolP, nBp,    E,    K,  fstrideC, rgroups, cgroups,
+               X,    XB,   fI,  nodeI, roff, plen,  nbc,    colmap, ucol, kpotN};
+    const long n = (long)rgroups * cgroups * nsys;
     hipLaunchKernelGGL(crt_cond_gemm_kernel, dim3((unsigned)n), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

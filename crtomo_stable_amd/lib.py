@@ -354,9 +354,10 @@ class Context:
 
     def condense_info(self) -> dict:
         """The grid's condensation plan (after a condensed forward)."""
-        v = np.zeros(6, np.int32)
+        v = np.zeros(8, np.int32)
         _chk(self.lib.crt_get_condense_info(self._h, v.ctypes.data_as(C.POINTER(C.c_int))), "crt_get_condense_info")
-        return dict(zip(("ok", "interior", "boundary", "band", "tiles", "order"), (int(x) for x in v)))
+        return dict(zip(("ok", "interior", "boundary", "band", "tiles", "order", "pieces", "separator"),
+                        (int(x) for x in v)))
 
     def refine_info(self):
         """(sweeps, final correction ratio, batches redone in FP64) of the last forward."""

@@ -284,7 +284,8 @@ int crt_get_refine_info(crt_ctx *ctx, int *sweeps, double *ratio, int *fallbacks
  * src/fem.f90:328-397; the potentials agree to rounding):
  *   0  the band factorisation of every system (kompab + scalab + chol + vre);
  *   1  static condensation onto the mixed-boundary nodes: the interior block, the same
- *      for every source, factorised once per wavenumber, a dense complex-symmetric
+ *      for every source, factorised once per wavenumber (cut into independent band
+ *      pieces by separators that join the dense block), a dense complex-symmetric
  *      factorisation of the boundary Schur complement per system; 903 where it does
  *      not apply (no type-11 elements, Dirichlet rows, the FP32 forward, an interior
  *      band of one or more than 16 tiles of 16, more than 640 boundary nodes);
@@ -293,8 +294,9 @@ int crt_get_refine_info(crt_ctx *ctx, int *sweeps, double *ratio, int *fallbacks
 int crt_set_forward_method(crt_ctx *ctx, int method);
 int crt_get_forward_method(crt_ctx *ctx, int *method);
 /* The condensation plan of the grid (made by the first condensed crt_forward):
- * info[6] = applicable (0/1), interior nodes, boundary nodes, interior band width,
- * interior band tiles of 16, interior numbering (0 the grid's, 1 by x, 2 by y). */
+ * info[8] = applicable (0/1), interior nodes in the pieces, boundary nodes, interior
+ * band width, interior band tiles of 16, interior numbering (0 the grid's, 1 by x,
+ * 2 by y), interior pieces, separator nodes. */
 int crt_get_condense_info(crt_ctx *ctx, int *info);
 
 /* Device time of the last call of each phase, milliseconds:
