@@ -243,15 +243,25 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jq_kernel(CGArgs A, const JT *J
     A.part_ap[(size_t)cb * A.nanz + i] = zadd(acc0, acc1);
 }
 
-// K_e: ap = sum over chunks, then wap = w * ap
-__global__ void __launch_bounds__(CG_BLK) crt_cg_apred_kernel(CGArgs A) {
+// K_e: ap = sum over chunks, then wap = w * ap.  64 rows per block: wave v of 16 sums
+// the chunks cb = v (mod 16) in order, wave 0 adds the 16 sums in order
+constexpr int CG_RED_WAVES = 16;
+__global__ void __launch_bounds__(64 * CG_RED_WAVES) crt_cg_apred_kernel(CGArgs A) {
+    __shared__ zd red[CG_RED_WAVES][64];
     if (A.scn->done) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.nanz) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
     zd s = zd{0.0, 0.0};
-    for (int cb = 0; cb < A.ncb; ++cb) s = zadd(s, A.part_ap[(size_t)cb * A.nanz + i]);
-    double wi = A.w[i];
-    A.ap[i] = zd{s.re * wi, s.im * wi};
+    if (i < A.nanz)
+        for (int cb = wv; cb < A.ncb; cb += CG_RED_WAVES) s = zadd(s, A.part_ap[(size_t)cb * A.nanz + i]);
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && i < A.nanz) {
+        zd t = red[0][lane];
+        for (int v = 1; v < CG_RED_WAVES; ++v) t = zadd(t, red[v][lane]);
+        const double wi = A.w[i];
+        A.ap[i] = zd{t.re * wi, t.im * wi};
+    }
 }
 
 // K_f: Bp_j = C_j (J^H wap + lam R q)_j, CG_JHW_COLS columns per wave, dr1 partials
@@ -555,7 +565,7 @@ int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
         hipLaunchKernelGGL(crt_cg_jq_kernel<float2>, gq, dim3(CG_BLK), 0, s, a, a.J32);
     else
         hipLaunchKernelGGL(crt_cg_jq_kernel<zd>, gq, dim3(CG_BLK), 0, s, a, a.J);
-    hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + CG_BLK - 1) / CG_BLK), dim3(CG_BLK), 0, s, a);
+    hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + 63) / 64), dim3(64 * CG_RED_WAVES), 0, s, a);
     if (a.Jr)
         hipLaunchKernelGGL(crt_cg_jhw_kernel<double>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.Jr);
     else if (a.J32)
