@@ -49,19 +49,21 @@ def source_hash(kernel: str) -> str:
         return hashlib.sha1(f.read()).hexdigest()[:12]
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+def measured_traffic(kernel: str, workload: str, method: str):
+    """HBM bytes per launch of `kernel` on `workload` in the forward `method` ("condensed"
+    or "band", since the interior factorisation of the condensed forward and the band
+    factorisation of every system share a kernel) from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (profiles/<workload>_pmc_traffic.md, a line
-    `traffic <kernel> <bytes> <source sha1[:12]>`).  None when no measurement of this
-    kernel is committed, or when its source changed since it was measured."""
+    `traffic <kernel> <method> <bytes> <source sha1[:12]>`).  None when no measurement of
+    this kernel is committed, or when its source changed since it was measured."""
     import re
     path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(r"^traffic\s+(\S+)\s+(\d+)\s+([0-9a-f]{12})\s*$", line.strip())
-        if m and m.group(1) == kernel:
-            return int(m.group(2)) if m.group(3) == source_hash(kernel) else None
+        m = re.match(r"^traffic\s+(\S+(?:,\s\S+)*)\s+(condensed|band)\s+(\d+)\s+([0-9a-f]{12})\s*$", line.strip())
+        if m and m.group(1) == kernel and m.group(2) == method:
+            return int(m.group(3)) if m.group(4) == source_hash(kernel) else None
     return None
 
 
@@ -682,7 +684,7 @@ def main():
         ctx.voltages(1)
         ctx.sens(1)
 
-    keys = ("diag", "factor", "solve", "schur", "product", "refine", "volt", "sens")
+    keys = ("diag", "factor", "solve", "prefix", "schur", "product", "refine", "volt", "sens")
 
     def timed(nsteps, nwarm):
         for _ in range(nwarm):
@@ -715,7 +717,7 @@ def main():
         else:
             kernel = f"crt_factor16{'la' if d16 <= 10 else 'st'}_kernel<{d16}>"
         return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
-                    frac=round(achieved / peak, 4), traffic=measured_traffic(kernel, args.workload),
+                    frac=round(achieved / peak, 4), traffic=measured_traffic(kernel, args.workload, "band"),
                     kernel=kernel, flop_per_system=flop_sys, systems_per_launch=nsys, launch_ms=round(fac_ms, 4))
 
     def condensed_rooflines(phase):
@@ -724,17 +726,24 @@ def main():
           interior band factorisation: 8 nI mbI(mbI+1)/2 per wavenumber;
           substitution (forward and back, both passes): 2 x 8 nI mbI per right-hand side,
             nG + E right-hand sides per wavenumber;
+          separator elimination: (4/3) (nB^3 - nG^3) per wavenumber (the first nS columns of
+            the complex factorisation over B = [S, G], nB = nS + nG, with the update of G);
           dense boundary factorisation: (4/3) nG^3 + 16 nG^2 per (electrode, wavenumber);
           interior potentials: 8 nI nG per (electrode, wavenumber)."""
         info = ctx.condense_info()
         nI, nG, mbI, DI = info["interior"], info["boundary"], info["band"], info["tiles"]
-        NI = -(-(-(-nG // 16)) // 8)
+        nS = info["separator"]
+        JS, nt = (nS + 15) // 16, (nG + 15) // 16
+        NW = 4 if nt <= 20 else 8  # waves per source system (the library's choice)
+        NI, NIp = (nt + NW - 1) // NW, (JS + nt + 7) // 8
         ent = {
             "factor": (f"crt_factor16{'la' if DI <= 10 else 'st'}_kernel<{DI}>",
                        K * 8.0 * nI * mbI * (mbI + 1) / 2.0, K),
             "solve": (f"crt_cond_trsm_kernel<{DI}, false> + crt_cond_trsm_kernel<{DI}, true>",
                       K * (nG + E) * 2.0 * 8.0 * nI * mbI, K * (nG + E)),
-            "schur": (f"crt_cond_dense_kernel<{NI}>", K * E * (4.0 / 3.0 * nG ** 3 + 16.0 * nG ** 2), K * E),
+            "prefix": (f"crt_cond_dense_kernel<{NIp}, true, 8>", K * 4.0 / 3.0 * ((nS + nG) ** 3 - nG ** 3), K),
+            "schur": (f"crt_cond_dense_kernel<{NI}, false, {NW}>", K * E * (4.0 / 3.0 * nG ** 3 + 16.0 * nG ** 2),
+                      K * E),
             "product": ("crt_cond_gemm_kernel", K * E * 8.0 * nI * nG, K * E),
         }
         out = {}
@@ -745,7 +754,7 @@ def main():
             a = flop / (ms * 1e-3) / 1e12
             out[key] = dict(bound="mfma", achieved=round(a, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                             frac=round(a / FP64_PEAK_TFLOPS, 4), traffic=measured_traffic(kernel.split(" + ")[0],
-                                                                                          args.workload),
+                                                                                          args.workload, "condensed"),
                             kernel=kernel, flop_per_launch=flop, units_per_launch=units, launch_ms=round(ms, 4))
         return info, out
 

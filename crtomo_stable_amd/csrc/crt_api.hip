@@ -176,7 +176,8 @@ struct DBuf {
     }
 };
 
-enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_SCHUR, T_PRODUCT, T_N };
+enum { T_DIAG = 0, T_FACTOR, T_SOLVE, T_VOLT, T_SENS, T_PRECAL, T_CG, T_CGMV, T_REFINE, T_SCHUR, T_PRODUCT, T_PREFIX,
+       T_N };
 
 }  // namespace
 
@@ -1222,14 +1223,19 @@ static int cond_plan(crt_ctx *c) {
     P.mbI = bestw;
     P.DI = std::max(1, (bestw + 15) / 16);
     if (P.DI < 2 || P.DI > 16) return 0;
-    // pieces: [start, start + len) of the interior positions, separators of width mbI between
-    int np = 2;
+    // pieces: [start, start + len) of the interior positions, separators of width mbI between.
+    // The most pieces (up to 8) of at least 32 band widths each whose separators and
+    // boundary nodes fit the prefix kernel: the pieces' factorisations run side by side,
+    // each a chain of block steps as long as the piece, while the separators' elimination
+    // grows with their count (C1: 2 pieces 8.1 ms per step, 3 pieces 7.3, 4 pieces 7.6).
+    int np = 8, minlen = 32;
     if (const char *ev = getenv("CRT_COND_PIECES")) np = std::max(1, atoi(ev));
+    if (const char *ev = getenv("CRT_COND_MINLEN")) minlen = std::max(1, atoi(ev));
     const int w = bestw;
     for (; np > 1; --np) {
         const int L = (nI0 - (np - 1) * w) / np;
         const int nS = (np - 1) * w, ntB = ((nS + 15) / 16 * 16 + P.nGp) / 16;
-        if (L >= 32 * w && ntB <= crt_cond_prefix_max_tiles()) break;
+        if (L >= minlen * w && ntB <= crt_cond_prefix_max_tiles()) break;
     }
     std::vector<int> pst, plen;
     {
@@ -1575,7 +1581,7 @@ static int forward_condensed(crt_ctx *c) {
         const int v = atoi(ev);
         if ((v == 1 && D <= 10) || (v == 2 && D >= 3)) variant = v;
     }
-    double tdiag = 0, tfac = 0, tsol = 0, tsch = 0, tprod = 0;
+    double tdiag = 0, tfac = 0, tsol = 0, tsch = 0, tprod = 0, tpre = 0;
     for (int k0 = 0; k0 < K; k0 += kc) {
         const int n = std::min(kc, K - k0), nsn = n * nP;
         int *stI = P.stI.p + (size_t)k0 * nP;
@@ -1605,13 +1611,16 @@ static int forward_condensed(crt_ctx *c) {
                                           JS ? P.T.p : P.S0.p, P.Gr.p);
         if (e) return e;
         c->toc(T_SOLVE);
-        c->tic(T_SCHUR);
+        c->tic(T_PREFIX);
         if (JS)  // the separators, once per wavenumber: L_SS, L_GS and S0 = T_GG - L_GS L_GS^T
             e = crt_launch_cond_dense(st, g, 1, P.nG, nt, nBp, JS, E, K, k0, n, nullptr, P.T.p, P.Gr.p, P.d_bptr.p,
                                       P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p, P.Lt.p, P.Li.p,
                                       P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p, stI, nP);
         e = e ? e : crt_launch_cond_bval(st, g, n * E, E, k0, nBp, P.nb, P.d_bq.p, P.d_bg.p, P.fB.p + P.nSp, P.bval.p);
-        e = e ? e : crt_launch_cond_dense(st, g, 0, P.nG, nt, nBp, JS, E, K, k0, n * E, P.S0.p, P.T.p, P.Gr.p,
+        if (e) return e;
+        c->toc(T_PREFIX);
+        c->tic(T_SCHUR);
+        e = crt_launch_cond_dense(st, g, 0, P.nG, nt, nBp, JS, E, K, k0, n * E, P.S0.p, P.T.p, P.Gr.p,
                                           P.d_bptr.p, P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p,
                                           P.Lt.p, P.Li.p, P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p,
                                           P.stD.p + (size_t)k0 * E, 1);
@@ -1627,11 +1636,13 @@ static int forward_condensed(crt_ctx *c) {
         c->collect(T_DIAG); tdiag += c->times[T_DIAG];
         c->collect(T_FACTOR); tfac += c->times[T_FACTOR];
         c->collect(T_SOLVE); tsol += c->times[T_SOLVE];
+        c->collect(T_PREFIX); tpre += c->times[T_PREFIX];
         c->collect(T_SCHUR); tsch += c->times[T_SCHUR];
         c->collect(T_PRODUCT); tprod += c->times[T_PRODUCT];
     }
     c->times[T_DIAG] = tdiag; c->times[T_FACTOR] = tfac; c->times[T_SOLVE] = tsol; c->times[T_SCHUR] = tsch;
     c->times[T_PRODUCT] = tprod;
+    c->times[T_PREFIX] = tpre;
     c->times[T_REFINE] = 0.0;
     std::vector<int> sI((size_t)K * nP), sD((size_t)K * E);
     CRT_HIP(c->copy_out(sI.data(), P.stI.p, sI.size() * sizeof(int)));
@@ -1689,7 +1700,7 @@ int crt_forward(crt_ctx *c) {
         return 0;
     }
     c->last_method = 0;
-    c->times[T_SCHUR] = c->times[T_PRODUCT] = 0.0;
+    c->times[T_SCHUR] = c->times[T_PRODUCT] = c->times[T_PREFIX] = 0.0;
     // 16x16 matrix-core tiles when the band fits (mb <= 256), else 8x8 LDS tiles
     const bool use16 = crt_factor16_supported(c->mb) && !getenv("CRT_FACTOR8");
     const int NBt = use16 ? 16 : CRT_NB;

@@ -54,31 +54,6 @@ __device__ __forceinline__ double readlane_d(double x, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// 1/sqrt(m) for finite m > 0: hardware estimate y0 and one third-order step
-// y = y0 (1 + e/2 + 3e^2/8), e = 1 - m y0^2, which cubes the estimate's error
-__device__ __forceinline__ double rsqrt_f64(double m) {
-    double y0 = __builtin_amdgcn_rsq(m);
-    double e = fma(-m * y0, y0, 1.0);
-    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
-}
-
-// Pivot of column c: L = principal sqrt(s) (branch cut on the negative real
-// axis, as Fortran's CDSQRT) and 1/L = conj(L)/|s|, from two reciprocal square
-// roots.  bad is set for a zero (or non-finite) pivot: src/chol.f90 error 28.
-__device__ __forceinline__ void pivot_fast(zd s, zd &L, zd &inv, bool &bad) {
-    double m2 = fma(s.re, s.re, s.im * s.im);
-    bad = !(m2 > 0.0) || !(m2 < 1.0e300);
-    double rm = rsqrt_f64(m2);  // 1/|s|
-    double mag = m2 * rm;       // |s|
-    double h = 0.5 * (mag + fabs(s.re));
-    double rh = rsqrt_f64(h);   // 1/t, t = sqrt((|s| + |Re s|)/2)
-    double t = h * rh;
-    double q = 0.5 * fabs(s.im) * rh;  // |Im s|/(2t)
-    bool pos = s.re >= 0.0;
-    L = zd{pos ? t : q, copysign(pos ? q : t, s.im)};
-    inv = zd{L.re * rm, -L.im * rm};
-}
-
 // r = 1/p = conj(p)/|p|^2 with the hardware reciprocal and two Newton steps;
 // bad as in pivot_fast
 __device__ __forceinline__ zd crecip(zd p, bool &bad) {

@@ -491,21 +491,28 @@ struct DenseArgs {
 
 namespace {
 constexpr int DCH = 8;  // row-J tiles staged per round
-constexpr int DNW = 8;  // waves per system
+constexpr int DNW = 8;  // waves per system (prefix, and source systems of more than 20 tiles)
 
+// rowJ (the staged row tiles of the column updates) and ws (each wave's panel tile, and
+// wave 0's scratch in the substitution) are live in phases that barriers separate
+template <int NW>
 struct alignas(16) DenseLds {
-    zd rowJ[DCH][256];
+    union {
+        zd rowJ[DCH][256];
+        zd ws[NW][256];
+    };
     zd dt[256], li[256];
-    zd ws[DNW][256];
-    zd pt[DNW][T16];
+    zd pt[NW][T16];
     zd pinv[T16];
     int err;
 };
 }  // namespace
 
-template <int NI, bool PRE>
-__global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A) {
-    __shared__ DenseLds m;
+// NW waves per system.  With four (source systems of up to 20 tiles) two systems share
+// a CU, so one system's diagonal chain and loads overlap the other's products.
+template <int NI, bool PRE, int NW>
+__global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kernel(DenseArgs A) {
+    __shared__ DenseLds<NW> m;
     extern __shared__ zd yv[];  // nBp
     const int s = blockIdx.x;
     const int l = PRE ? 0 : s % A.E, kk = PRE ? s : s / A.E;
@@ -526,7 +533,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     auto tri = [](int I, int J) { return (size_t)I * (I + 1) / 2 + J; };
     if (!PRE) {
         // scalab's check of every boundary diagonal before the factorisation
-        for (int cg = threadIdx.x; cg < nG; cg += 64 * DNW) {
+        for (int cg = threadIdx.x; cg < nG; cg += 64 * NW) {
             const int q = A.qdiag[cg];
             const zd bp = boundary_part(g, q, l, k);
             const zd a = g.area[(size_t)k * g.nnz + q];
@@ -543,10 +550,10 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     for (int J = 0; J < nt; ++J) {
         const int pend = PRE ? min(J, JS) : J;  // columns of L in this column's update
         d4 ar[NI], ai[NI];
-        // S(l) entries of the owned tiles I = J + wave + DNW u, column J
+        // S(l) entries of the owned tiles I = J + wave + NW u, column J
 #pragma unroll
         for (int u = 0; u < NI; ++u) {
-            const int I = J + wave + DNW * u;
+            const int I = J + wave + NW * u;
             const zd *st = M + tri(I < nt ? I : J, J) * 256;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -581,18 +588,19 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             const int np = min(DCH, pend - p0);
             __syncthreads();
             const zd *src = Ls + tri(J, p0) * 256;
-            for (int i = threadIdx.x; i < np * 256; i += 64 * DNW) (&m.rowJ[0][0])[i] = ldz(src + i);
+            for (int i = threadIdx.x; i < np * 256; i += 64 * NW) (&m.rowJ[0][0])[i] = ldz(src + i);
             __syncthreads();
-            // per owned tile: the chunk's tiles L(I, p0..) (contiguous), two at a time, the
-            // next pair in flight while the current one is multiplied (the pipeline runs
+            // per owned tile: the chunk's tiles L(I, p0..) (contiguous), PW at a time, the
+            // next group in flight while the current one is multiplied (the pipeline runs
             // on across the owned tiles)
-            const int nv = min(NI, max(0, (nt - (J + wave) + DNW - 1) / DNW));
-            zd bc[2][4], bn[2][4];
-            auto ld2 = [&](int u, int pq, zd (&o)[2][4]) {
-                const zd *tp = Ls + tri(J + wave + DNW * u, p0) * 256;
+            constexpr int PW = 2;
+            const int nv = min(NI, max(0, (nt - (J + wave) + NW - 1) / NW));
+            zd bc[PW][4], bn[PW][4];
+            auto ld2 = [&](int u, int pq, zd (&o)[PW][4]) {
+                const zd *tp = Ls + tri(J + wave + NW * u, p0) * 256;
                 const bool live = u < nv;
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < PW; ++j)
 #pragma unroll
                     for (int ks = 0; ks < 4; ++ks)
                         o[j][ks] = live && pq + j < np ? ldz(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
@@ -601,18 +609,17 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
 #pragma unroll
             for (int u = 0; u < NI; ++u) {
                 if (u < nv) {
-                    for (int pq = 0; pq < np; pq += 2) {
-                        const bool last = pq + 2 >= np;
-                        ld2(last ? u + 1 : u, last ? 0 : pq + 2, bn);
+                    for (int pq = 0; pq < np; pq += PW) {
+                        const bool last = pq + PW >= np;
+                        ld2(last ? u + 1 : u, last ? 0 : pq + PW, bn);
 #pragma unroll
-                        for (int j = 0; j < 2; ++j)
+                        for (int j = 0; j < PW; ++j)
                             if (pq + j < np)
 #pragma unroll
                                 for (int ks = 0; ks < 4; ++ks)
                                     cmma(ar[u], ai[u], zd{-bc[j][ks].re, -bc[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
 #pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
+                        for (int j = 0; j < PW; ++j)
                             for (int ks = 0; ks < 4; ++ks) bc[j][ks] = bn[j][ks];
                     }
                 }
@@ -623,7 +630,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             zd *o = A.S0w + (size_t)kk * ntri * 256;
 #pragma unroll
             for (int u = 0; u < NI; ++u) {
-                const int I = J + wave + DNW * u;
+                const int I = J + wave + NW * u;
                 if (I < nt)
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
@@ -640,9 +647,10 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
 #pragma unroll
             for (int c = 0; c < T16; ++c) {
                 const zd sv = m.dt[c * T16 + c];
-                const zd L = zsqrt(sv);
-                if (!(zabs(sv) > 0.0) || (g.dc && !(sv.re > 0.0))) err = CRT_ERR_PIVOT;
-                const zd inv = zinv(L);
+                zd L, inv;
+                bool bad;
+                pivot_fast(sv, L, inv, bad);
+                if (bad || (g.dc && !(sv.re > 0.0))) err = CRT_ERR_PIVOT;
                 // lane (r, cq) owns entries (r, cq + 4w), w = 0..3
                 const int r = jc;
                 const zd lr = zmul(m.dt[c * T16 + r], inv);
@@ -688,7 +696,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
         // panel tiles L_IJ = S_IJ L_JJ^-T of the owned I > J
 #pragma unroll
         for (int u = 0; u < NI; ++u) {
-            const int I = J + wave + DNW * u;
+            const int I = J + wave + NW * u;
             if (I > J && I < nt) {
                 zd *w = m.ws[wave];
 #pragma unroll
@@ -718,12 +726,12 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     auto tileLi = [&](int J) -> const zd * { return J < JS ? Lip + (size_t)J * 256 : Lis + (size_t)(J - JS) * 256; };
     const int nBp = A.nBp;
     const zd *gr = A.Gr + ((size_t)kk * A.E + l) * nBp;
-    for (int i = threadIdx.x; i < nBp; i += 64 * DNW) yv[i] = gr[i];
+    for (int i = threadIdx.x; i < nBp; i += 64 * NW) yv[i] = gr[i];
     __syncthreads();
     for (int J = 0; J < ntB; ++J) {
         // lane (r, q) of step w holds L(r, 4w + q)
         zd acc = zd{};
-        for (int p = wave; p < J; p += DNW) {
+        for (int p = wave; p < J; p += NW) {
             const zd *tp = tileL(J, p);
 #pragma unroll
             for (int w = 0; w < 4; ++w) acc = zfma(acc, ldz(tp + 64 * w + lane), yv[16 * p + 4 * w + rq]);
@@ -739,7 +747,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
                 const int r = lane;
                 zd tv = yv[16 * J + r];
 #pragma unroll
-                for (int w = 0; w < DNW; ++w) tv = zsub(tv, m.pt[w][r]);
+                for (int w = 0; w < NW; ++w) tv = zsub(tv, m.pt[w][r]);
                 m.ws[0][r] = tv;
             }
             __builtin_amdgcn_wave_barrier();
@@ -758,7 +766,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
     for (int J = ntB - 1; J >= 0; --J) {
         // out(c) = sum_r L(I,J)(r, c) x(16I + r): lane (r, q) of step w holds L(r, 4w + q)
         zd acc[4] = {zd{}, zd{}, zd{}, zd{}};
-        for (int I = J + 1 + wave; I < ntB; I += DNW) {
+        for (int I = J + 1 + wave; I < ntB; I += NW) {
             const zd *tp = tileL(I, J);
             const zd xr = yv[16 * I + jc];
 #pragma unroll
@@ -779,7 +787,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
             if (lane < T16) {
                 zd tv = yv[16 * J + lane];
 #pragma unroll
-                for (int w = 0; w < DNW; ++w) tv = zsub(tv, m.pt[w][lane]);
+                for (int w = 0; w < NW; ++w) tv = zsub(tv, m.pt[w][lane]);
                 m.ws[0][lane] = tv;
             }
             __builtin_amdgcn_wave_barrier();
@@ -804,7 +812,7 @@ __global__ void __launch_bounds__(64 * DNW, 1) crt_cond_dense_kernel(DenseArgs A
         __syncthreads();
     }
     zd *xb = A.XB + ((size_t)kk * A.E + l) * nBp;
-    for (int i = threadIdx.x; i < nBp; i += 64 * DNW) {
+    for (int i = threadIdx.x; i < nBp; i += 64 * NW) {
         const zd x = yv[i];
         xb[i] = x;
         const int nd = A.nodeB[i];
@@ -968,10 +976,10 @@ int crt_launch_cond_bval(hipStream_t st, const DevGrid &g, int nsys, int E, int 
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-template <int NI, bool PRE>
+template <int NI, bool PRE, int NW = DNW>
 static int launch_dense(hipStream_t st, const DenseArgs &a, int nsys) {
     const size_t lds = (size_t)a.nBp * sizeof(zd);
-    hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE>), dim3(nsys), dim3(64 * DNW), lds, st, a);
+    hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE, NW>), dim3(nsys), dim3(64 * NW), lds, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -996,6 +1004,14 @@ int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int pre, int nG, int
             default: return CRT_ERR_UNSUPPORTED;
         }
     }
+    if (nt <= 20) switch ((nt + 3) / 4) {
+            case 1: return launch_dense<1, false, 4>(st, a, nsys);
+            case 2: return launch_dense<2, false, 4>(st, a, nsys);
+            case 3: return launch_dense<3, false, 4>(st, a, nsys);
+            case 4: return launch_dense<4, false, 4>(st, a, nsys);
+            case 5: return launch_dense<5, false, 4>(st, a, nsys);
+            default: return CRT_ERR_UNSUPPORTED;
+        }
     switch ((nt + DNW - 1) / DNW) {
         case 1: return launch_dense<1, false>(st, a, nsys);
         case 2: return launch_dense<2, false>(st, a, nsys);

@@ -434,10 +434,10 @@ class Context:
         return err, a, p
 
     def times(self):
-        ms = np.zeros(11)
-        self.lib.crt_get_times(self._h, _ptr(ms), 11)
+        ms = np.zeros(12)
+        self.lib.crt_get_times(self._h, _ptr(ms), 12)
         return dict(zip(("diag", "factor", "solve", "volt", "sens", "precal", "cg", "cg_matvec", "refine", "schur",
-                         "product"), ms))
+                         "product", "prefix"), ms))
 
 
 class Batch:

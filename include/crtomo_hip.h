@@ -304,9 +304,10 @@ int crt_get_condense_info(crt_ctx *ctx, int *info);
  * [5] precal, [6] cg total, [7] cg matvec (mean per iteration),
  * [8] refinement sweeps of the FP32 forward (total), [9] the dense boundary
  * factorisations and solves of the condensed forward (total), [10] its interior
- * potentials (the product with the boundary solution).  The condensed forward counts
- * its interior band factorisation as [1] and its multi-right-hand-side substitutions
- * and Schur complements as [2]. */
+ * potentials (the product with the boundary solution), [11] its elimination of the
+ * separators (once per wavenumber) with the boundary terms of every source.  The
+ * condensed forward counts its interior band factorisation as [1] and its
+ * multi-right-hand-side substitutions and Schur complements as [2]. */
 int crt_get_times(crt_ctx *ctx, double *ms, int n);
 
 /* Batched forward of many models of uneven size (crt_forward of every context,

@@ -72,7 +72,7 @@ def test_condensed_equals_band(name):
     ev = float(np.max(np.abs(c["volt"] - b["volt"]) / np.abs(b["volt"])))
     es = _rel_rows(c["sens"].T, b["sens"].T)
     tb = sum(b["times"][p] for p in ("diag", "factor", "solve"))
-    tc = sum(c["times"][p] for p in ("diag", "factor", "solve", "schur", "product"))
+    tc = sum(c["times"][p] for p in ("diag", "factor", "solve", "prefix", "schur", "product"))
     print(f"[condensed] {name}: kpot {ek:.2e} volt {ev:.2e} sens {es:.2e}; forward band {tb:.2f} ms, "
           f"condensed {tc:.2f} ms ({c['times']})", flush=True)
     assert ek <= TOL and ev <= TOL and es <= TOL
@@ -143,7 +143,7 @@ def test_condensed_c3_sampled(oracle_lib):
     ek = _rel_rows(c["kpot"], b["kpot"])
     ev = float(np.max(np.abs(c["volt"] - b["volt"]) / np.abs(b["volt"])))
     tb = sum(b["times"][p] for p in ("diag", "factor", "solve"))
-    tc = sum(c["times"][p] for p in ("diag", "factor", "solve", "schur", "product"))
+    tc = sum(c["times"][p] for p in ("diag", "factor", "solve", "prefix", "schur", "product"))
     print(f"[condensed] c3: kpot {ek:.2e} volt {ev:.2e}; forward band {tb:.1f} ms, condensed {tc:.1f} ms "
           f"({c['times']})", flush=True)
     kwn, kwi, _, _ = O.rwaven(w.grid, w.survey)
