@@ -233,10 +233,12 @@ def gn_step(ctx, w, ltri: int):
     tm = ctx.times()
     s64 = ctx.get_sens()  # host copy for the mixed-precision comparison, outside the step
     # the same update in the other CG forms: complex FP32 copy of J in the iterations,
-    # and the FSAI preconditioner (pattern 2) on top of the Jacobi scaling
+    # the FSAI preconditioner (pattern 2, and the stencil pattern 1) on top of the Jacobi
+    # scaling, and the learned preconditioner's analytic prior alone (kind 4)
     d64 = r["dpar"]
     forms = {}
-    for name, jbits, prec in (("fp32_jacobian", 32, 0), ("fsai", 64, 2), ("fp32_jacobian_fsai", 32, 2)):
+    for name, jbits, prec in (("fp32_jacobian", 32, 0), ("fsai", 64, 2), ("fp32_jacobian_fsai", 32, 2),
+                              ("fp32_jacobian_fsai_stencil", 32, 1), ("fp32_jacobian_learned_prior", 32, 4)):
         ctx.set_cg_precision(jbits)
         ctx.set_cg_preconditioner(prec)
         t1 = time.perf_counter()

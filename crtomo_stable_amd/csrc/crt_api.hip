@@ -95,7 +95,7 @@ int crt_launch_pc_norm(hipStream_t, int, const float *, float *);
 int crt_launch_pc_fwdbwd(hipStream_t, int, const float *, const float *, const float *, const float *, const float *,
                          float *, float *, double *, double *, float *);
 int crt_launch_pc_adam(hipStream_t, float *, float *, float *, const float *, float, int);
-int crt_launch_pc_infer(hipStream_t, int, int, const float *, const float *, const float *, const int *, zd *);
+int crt_launch_pc_infer(hipStream_t, int, int, const float *, const float *, const float *, const int *, zd *, int);
 int crt_launch_real_part(hipStream_t, size_t, const zd *, double *);
 int crt_launch_cg_rhs(hipStream_t, int, int, double, const zd *, const double *, const double *, const zd *,
                       const zd *, const double *, double *, zd *);
@@ -2167,8 +2167,8 @@ static int cg_fsai(crt_ctx *c, CGArgs &a) {
     c->have_cgsys = true;
     if (c->cg_prec == 0) return 0;
     const int manz = a.manz;
-    const bool learned = c->cg_prec == 3;
-    if (learned && (!c->pc_ready || c->ltri != 0)) return CRT_ERR_STATE;
+    const bool learned = c->cg_prec >= 3, prior_only = c->cg_prec == 4;
+    if (learned && ((!prior_only && !c->pc_ready) || c->ltri != 0)) return CRT_ERR_STATE;
     std::vector<int> pat, tp;
     fsai_pattern(c, manz, learned ? 1 : c->cg_prec, pat, tp);
     int e = c->d_fpat.upload(pat.data(), pat.size(), c->stream);
@@ -2183,7 +2183,7 @@ static int cg_fsai(crt_ctx *c, CGArgs &a) {
     if (learned) {
         if ((e = pc_features(c, a))) return e;
         e = crt_launch_pc_infer(c->stream, manz, c->rnx, c->d_pc_par.p, c->d_pc_norm.p, c->d_pc_feat.p, c->d_fpat.p,
-                                c->d_fg.p);
+                                c->d_fg.p, prior_only ? 1 : 0);
     } else {
         e = crt_launch_cg_fsai(c->stream, a, c->d_fg.p);
     }
@@ -2280,7 +2280,7 @@ static int cg_iterate(crt_ctx *c, CGArgs a, const zd *bvec, int *ncg) {
 // the learned preconditioner falls back to the reference's Jacobi-scaled CG when it
 // does not reach the stop test within ncgmax iterations
 static int cg_fallback(crt_ctx *c, const CGArgs &a, int ncgmax, int *n) {
-    if (c->cg_prec != 3 || !a.fg || *n < ncgmax) return 0;
+    if (c->cg_prec < 3 || !a.fg || *n < ncgmax) return 0;
     CGArgs aj = a;
     aj.fg = nullptr;
     ++c->pc_fallbacks;
@@ -2294,7 +2294,7 @@ static int cg_weights(crt_ctx *c, int nanz, const double *wmatd, const int *wdfa
 }
 
 int crt_set_cg_preconditioner(crt_ctx *c, int kind) {
-    if (!c || kind < 0 || kind > 3) return CRT_ERR_ARG;
+    if (!c || kind < 0 || kind > 4) return CRT_ERR_ARG;
     c->cg_prec = kind;
     return 0;
 }
@@ -2435,6 +2435,13 @@ int crt_pc_train(crt_ctx *c, int steps, double lr, double *loss_first, double *l
     return 0;
 }
 
+int crt_pc_reset(crt_ctx *c) {
+    if (!c) return CRT_ERR_ARG;
+    c->pc_tn = c->pc_manz = c->pc_steps = 0;
+    c->pc_ready = c->pc_norm_set = false;
+    return 0;
+}
+
 int crt_pc_grad(crt_ctx *c, double *loss, float *grad) {
     // loss and gradient of the current parameters on the training system of the last
     // crt_pc_train call (no update)
@@ -2458,11 +2465,15 @@ int crt_pc_grad(crt_ctx *c, double *loss, float *grad) {
     return 0;
 }
 
-int crt_pc_data(crt_ctx *c, float *feat, float *tgt, float *mask) {
-    // the standardised network inputs' raw features, targets and mask of the training system
-    if (!c) return CRT_ERR_ARG;
+int crt_pc_data(crt_ctx *c, int *n, float *feat, float *tgt, float *mask) {
+    // the network inputs' raw features, targets and mask of the training set; *n: the
+    // caller's capacity in cells on entry, the set's size on return
+    if (!c || !n) return CRT_ERR_ARG;
     if (c->pc_manz == 0) return CRT_ERR_STATE;
     const size_t manz = c->pc_manz;
+    const int cap = *n;
+    *n = c->pc_manz;
+    if ((feat || tgt || mask) && cap < c->pc_manz) return CRT_ERR_ARG;
     if (feat) CRT_HIP(c->copy_out(feat, c->d_pc_tfeat.p, manz * 8 * sizeof(float)));
     if (tgt) CRT_HIP(c->copy_out(tgt, c->d_pc_ttgt.p, manz * 5 * sizeof(float)));
     if (mask) CRT_HIP(c->copy_out(mask, c->d_pc_tmask.p, manz * 5 * sizeof(float)));

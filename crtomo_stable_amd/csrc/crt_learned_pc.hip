@@ -339,21 +339,25 @@ __global__ void __launch_bounds__(256) crt_pc_adam_kernel(float *params, float *
 }
 
 // ---------------------------------------------------------------- inference
-// G rows from the network (BF16 weights, FP32 accumulation), in the FSAI layout
+// G rows from the network (BF16 weights, FP32 accumulation), in the FSAI layout;
+// prior_only: the analytic estimate alone (the network's output taken as zero)
 __global__ void __launch_bounds__(64) crt_pc_infer_kernel(int manz, int nx, const float *params, const float *norm,
-                                                           const float *feat, const int *pat, zd *g) {
+                                                           const float *feat, const int *pat, zd *g, int prior_only) {
     __shared__ float P[PC_NP];
-    for (int t = threadIdx.x; t < PC_NP; t += blockDim.x) P[t] = bf16_round(params[t]);
+    if (!prior_only)
+        for (int t = threadIdx.x; t < PC_NP; t += blockDim.x) P[t] = bf16_round(params[t]);
     __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= manz) return;
-    float x[PC_NI], h1[PC_NH], h2[PC_NH], y[PC_NO];
+    float x[PC_NI], h1[PC_NH], h2[PC_NH], y[PC_NO] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (!prior_only) {
 #pragma unroll
-    for (int k = 0; k < PC_NI; ++k) x[k] = (feat[(size_t)j * PC_NI + k] - norm[k]) * norm[PC_NI + k];
-    pc_layer<PC_NH, PC_NI, true>(P + PC_W1, P + PC_B1, x, h1);
-    pc_layer<PC_NH, PC_NH, true>(P + PC_W2, P + PC_B2, h1, h2);
-    pc_layer<PC_NH, PC_NH, true>(P + PC_W3, P + PC_B3, h2, h1);
-    pc_layer<PC_NO, PC_NH, false>(P + PC_W4, P + PC_B4, h1, y);
+        for (int k = 0; k < PC_NI; ++k) x[k] = (feat[(size_t)j * PC_NI + k] - norm[k]) * norm[PC_NI + k];
+        pc_layer<PC_NH, PC_NI, true>(P + PC_W1, P + PC_B1, x, h1);
+        pc_layer<PC_NH, PC_NH, true>(P + PC_W2, P + PC_B2, h1, h2);
+        pc_layer<PC_NH, PC_NH, true>(P + PC_W3, P + PC_B3, h2, h1);
+        pc_layer<PC_NO, PC_NH, false>(P + PC_W4, P + PC_B4, h1, y);
+    }
     float b[PC_NO];
     pc_baseline(feat + (size_t)j * PC_NI, b);
 #pragma unroll
@@ -412,8 +416,8 @@ int crt_launch_pc_adam(hipStream_t s, float *params, float *m, float *v, const f
 }
 
 int crt_launch_pc_infer(hipStream_t s, int manz, int nx, const float *params, const float *norm, const float *feat,
-                        const int *pat, zd *g) {
+                        const int *pat, zd *g, int prior_only) {
     hipLaunchKernelGGL(crt_pc_infer_kernel, dim3((manz + 63) / 64), dim3(64), 0, s, manz, nx, params, norm, feat, pat,
-                       g);
+                       g, prior_only);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

@@ -296,7 +296,8 @@ class Context:
         return lam.value
 
     def set_cg_preconditioner(self, kind: int):
-        """0: Jacobi scaling (the reference); 1/2: FSAI on the stencil / wider pattern."""
+        """0: Jacobi scaling (the reference); 1/2: FSAI on the stencil / wider pattern;
+        3: the learned preconditioner; 4: its analytic prior alone."""
         _chk(self.lib.crt_set_cg_preconditioner(self._h, int(kind)), "crt_set_cg_preconditioner")
 
     def pc_train(self, steps: int, lr: float = 1e-3):
@@ -324,10 +325,19 @@ class Context:
         _chk(self.lib.crt_pc_grad(self._h, C.byref(loss), _ptr(g)), "crt_pc_grad")
         return loss.value, g
 
-    def pc_data(self, manz: int):
-        f = np.zeros((manz, 8), np.float32); t = np.zeros((manz, 5), np.float32); m = np.zeros((manz, 5), np.float32)
-        _chk(self.lib.crt_pc_data(self._h, _ptr(f), _ptr(t), _ptr(m)), "crt_pc_data")
+    def pc_data(self):
+        """The training set: raw features (n, 8), targets and mask (n, 5)."""
+        n = C.c_int(0)
+        _chk(self.lib.crt_pc_data(self._h, C.byref(n), None, None, None), "crt_pc_data")
+        f = np.zeros((n.value, 8), np.float32)
+        t = np.zeros((n.value, 5), np.float32)
+        m = np.zeros((n.value, 5), np.float32)
+        _chk(self.lib.crt_pc_data(self._h, C.byref(n), _ptr(f), _ptr(t), _ptr(m)), "crt_pc_data")
         return f, t, m
+
+    def pc_reset(self):
+        """Drop the learned preconditioner's training set and model."""
+        _chk(self.lib.crt_pc_reset(self._h), "crt_pc_reset")
 
     def tomodir_crmod(self, cfg_path: str):
         """Run a CRMod tomodir from its crmod.cfg (grid, electrodes, rho, config read by

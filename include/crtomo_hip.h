@@ -159,7 +159,9 @@ int crt_set_cg_precision(crt_ctx *ctx, int jbits);
  * on the regularisation stencil (row j: its lower neighbours and j), built on the
  * device after cgfac from the Hermitian blocks B(S_j, S_j).  2: the same on a wider
  * lower pattern (regular grids: two cells left, the three cells above, two above).
- * 3: the learned preconditioner below.  The stop test stays on |r|^2 of the scaled
+ * 3: the learned preconditioner below; 4: its analytic prior alone (the FSAI row
+ * on the stencil with the coupling between the two neighbours neglected, no network,
+ * no training).  The stop test stays on |r|^2 of the scaled
  * system, as in the reference; only the iteration count changes.  No counterpart
  * in the reference. */
 int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
@@ -181,12 +183,17 @@ int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
  *     standardisation values, then the parameters).
  *   crt_pc_grad: loss and parameter gradient on the training set (no update).
  *   crt_pc_data: the training set's raw features (n x 8), targets (the corrections
- *     to the analytic estimate) and mask (n x 5); n = cells added so far. */
+ *     to the analytic estimate) and mask (n x 5); *n holds the buffers' capacity in
+ *     cells on entry (900 when it is too small; NULL buffers only query) and the
+ *     number of cells added so far on return.
+ *   crt_pc_reset: drops the training set, the model and its standardisation (the
+ *     next crt_pc_train starts a new network). */
 int crt_pc_train(crt_ctx *ctx, int steps, double lr, double *loss_first, double *loss_last);
 int crt_pc_size(void);
 int crt_pc_weights(crt_ctx *ctx, float *get, const float *set);
 int crt_pc_grad(crt_ctx *ctx, double *loss, float *grad);
-int crt_pc_data(crt_ctx *ctx, float *feat, float *tgt, float *mask);
+int crt_pc_data(crt_ctx *ctx, int *n, float *feat, float *tgt, float *mask);
+int crt_pc_reset(crt_ctx *ctx);
 
 /* Smoothness regularisation built on the device from the grid of crt_set_grid and
  * installed for crt_cjg / crt_update (replaces crt_set_regularization):

@@ -235,11 +235,13 @@ def test_learned_preconditioner(ctx, oracle_lib):
     w = _workload(0)
     manz = w.grid.elanz
     P = _problem(O, w, 0, seed=0)
+    ctx.pc_reset()  # a new network on this grid's systems alone
     _gpu_update(ctx, w, P, 0, 3.0, 1e-12, 400)
     ctx.pc_train(1, 1e-3)
     loss, g = ctx.pc_grad()
     wv = ctx.pc_weights()
-    feat, tgt, mask = ctx.pc_data(manz)
+    feat, tgt, mask = ctx.pc_data()
+    assert feat.shape[0] == manz
     lref, gref = R.loss_grad(wv[16:].astype(np.float64), wv[:16], feat, tgt, mask)
     print(f"pc loss {loss:.6e} ref {lref:.6e}; grad max diff {np.max(np.abs(g - gref)):.3e} of {np.max(np.abs(gref)):.3e}",
           flush=True)

@@ -38,6 +38,7 @@ def test_learned_pc_c3(ctx, oracle_lib):
     par0 = np.full(g.elanz, np.log((1.0 / 100.0) * np.exp(-1j * 5e-3)))
     ctx.set_cg_precision(32)
     ctx.set_cg_preconditioner(2)
+    ctx.pc_reset()  # a new network trained on this test's inversions alone
     losses = []
     try:
         for m in range(2):
@@ -62,6 +63,12 @@ def test_learned_pc_c3(ctx, oracle_lib):
         ctx.set_cg_preconditioner(3)
         lrn_conv = ctx.update(lam, 1e-30, NCG_CONV, wmatd, wdfak, dat, par0)
         lrn = ctx.update(lam, 1e-12, g.elanz // 10, wmatd, wdfak, dat, par0)
+        # the same stop test with the network's analytic prior alone (no network) and with
+        # the exact FSAI rows of the same stencil pattern (the network's teacher)
+        others = {}
+        for name, kind in (("prior", 4), ("fsai_stencil", 1)):
+            ctx.set_cg_preconditioner(kind)
+            others[name] = ctx.update(lam, 1e-12, g.elanz // 10, wmatd, wdfak, dat, par0)
     finally:
         ctx.set_cg_precision(64)
         ctx.set_cg_preconditioner(0)
@@ -80,6 +87,10 @@ def test_learned_pc_c3(ctx, oracle_lib):
         rel = float(np.linalg.norm(res) / np.linalg.norm(bvec))
         print(f"[learned c3] {name}: |B y - b| / |b| = {rel:.2e}", flush=True)
         assert rel <= 1e-12
+    for name, r in others.items():
+        e = _rel(r["dpar"], jac["dpar"])
+        print(f"[learned c3] eps 1e-12, {name}: ncg {r['ncg']}, dpar diff vs Jacobi {e:.2e}", flush=True)
+        assert e <= 1e-4
     assert e_conv <= 1e-10  # measured 5.6e-15
     assert e_stop <= 1e-4
     assert lrn["ncg"] < jac["ncg"]
