@@ -604,6 +604,183 @@ void orc_randdc(const orc_grid *g, double *a, double *b) { RANDB_BODY(double) }
 void orc_randb2(const orc_grid *g, zdbl *a, zdbl *b) { RANDB2_BODY(zdbl, ZVAL) }
 void orc_randbdc2(const orc_grid *g, double *a, double *b) { RANDB2_BODY(double, RVAL) }
 
+/* ------------------------------------------------ singularity removal, sink */
+/* src/refsig.f90:33-70: sigma0 = exp(sum_e area_e log sigma_e / sum_e area_e) over the
+ * area elements (types 3, 5, 8), areas by Gauss' formula */
+void orc_refsig(const orc_grid *g, const zdbl *sigma, zdbl *sigma0) {
+    const int maxn = g->maxn;
+    double area = 0.0;
+    zdbl s0 = 0.0;
+    int iel = 0;
+    for (int t = 0; t < g->typanz; ++t) {
+        for (int j = 0; j < g->nelanz[t]; ++j, ++iel) {
+            const int ty = g->typ[t];
+            if (ty > 10) continue;
+            const int *nd = g->nrel + (size_t)iel * maxn;
+            const double xk = g->x[nd[0] - 1], yk = g->y[nd[0] - 1];
+            const double ax = g->x[nd[1] - 1] - xk, ay = g->y[nd[1] - 1] - yk;
+            const double bx = g->x[nd[2] - 1] - xk, by = g->y[nd[2] - 1] - yk;
+            double dum = 0.0;
+            if (ty == 3) {
+                dum = fabs(ax * by - ay * bx) / 2.0;
+            } else if (ty == 5 || ty == 8) {
+                const double cx = g->x[nd[3] - 1] - xk, cy = g->y[nd[3] - 1] - yk;
+                dum = (fabs(ax * by - ay * bx) + fabs(bx * cy - by * cx)) / 2.0;
+            }
+            s0 += dum * clog(sigma[iel]);
+            area += dum;
+        }
+    }
+    *sigma0 = cexp(s0 / area);
+}
+
+/* src/potana.f90:51-97: the homogeneous half-space potential of a unit current at
+ * electrode l for wavenumber k, K0 of the distances to the source and its mirror image
+ * in y = 0, the source node set to 5 times the largest value, scaled by (1/2pi)/sigma0 */
+void orc_potana(const orc_grid *g, const double *kwn, int l, int k, const zdbl *sigma0p, zdbl *pota) {
+    const double pi = acos(-1.0), kk = kwn[k - 1];
+    const zdbl sigma0 = *sigma0p;
+    const int idum = g->enr[l - 1];
+    const double xk1 = g->x[idum - 1], yk1 = g->y[idum - 1];
+    double potmax = 0.0;
+    for (int j = 1; j <= g->sanz; ++j) {
+        if (j == idum) continue;
+        const double x21 = g->x[j - 1] - xk1, y21m = g->y[j - 1] - yk1, y21p = g->y[j - 1] + yk1;
+        const double rm = sqrt(x21 * x21 + y21m * y21m), rp = sqrt(x21 * x21 + y21p * y21p);
+        const double dum = orc_bessk0(rm * kk) + orc_bessk0(rp * kk);
+        potmax = fmax(potmax, dum);
+        pota[j - 1] = dum;
+    }
+    pota[idum - 1] = 5.0 * potmax;
+    const zdbl dum2 = (0.5 / pi) / sigma0;
+    for (int j = 0; j < g->sanz; ++j) pota[j] *= dum2;
+}
+
+/* src/kompab.f90:80-183 with the singularity removal (lsr: the right-hand side
+ * sum_e elbg (sigma_e - sigma0) pota, no unit source, :129-133,162) and the fictitious
+ * sink (lsink: b(nsink) = 1, :183) */
+static int kompab_sw(const orc_grid *g, const zdbl *sigma, const double *elbg, const double *relbg,
+                     const double *kg, int l, int k, int lsr, zdbl sigma0, const zdbl *pota, int lsink,
+                     int nsink, zdbl *a, zdbl *b) {
+    const int maxn = g->maxn, npk = NPK(maxn), mb = g->mb, sanz = g->sanz;
+    memset(a, 0, sizeof(zdbl) * (size_t)(mb + 1) * sanz);
+    memset(b, 0, sizeof(zdbl) * (size_t)sanz);
+    int iel = 0;
+    for (int t = 0; t < g->typanz; ++t) {
+        int ntyp = g->typ[t], nkel = g->selanz[t];
+        for (int j = 0; j < g->nelanz[t]; ++j, ++iel) {
+            if (ntyp > 11) continue;
+            int ikl = 0;
+            for (int kk = 0; kk < nkel; ++kk) {
+                int nzp = g->nrel[iel * maxn + kk];
+                for (int ll = 0; ll <= kk; ++ll) {
+                    int nnp = g->nrel[iel * maxn + ll];
+                    if (abs(nzp - nnp) > mb) return 19;
+                    int im = imax_(nzp, nnp) * mb + imin_(nzp, nnp);
+                    double dum;
+                    zdbl s;
+                    if (ntyp == 11) {
+                        int rel = iel - g->elanz;
+                        dum = relbg[(size_t)ikl * g->relanz + rel] *
+                              kg[((size_t)(k - 1) * g->eanz + (l - 1)) * g->relanz + rel];
+                        s = sigma[g->rnr[rel] - 1];
+                    } else {
+                        dum = elbg[((size_t)(k - 1) * npk + ikl) * g->elanz + iel];
+                        s = sigma[iel];
+                    }
+                    a[im - 1] += CMPLX(dum * creal(s), dum * cimag(s));
+                    if (lsr) {
+                        const zdbl d2 = dum * (s - sigma0);
+                        b[nzp - 1] += d2 * pota[nnp - 1];
+                        if (nnp != nzp) b[nnp - 1] += d2 * pota[nzp - 1];
+                    }
+                    ++ikl;
+                }
+            }
+        }
+    }
+    if (!lsr) b[g->enr[l - 1] - 1] = CMPLX(-1.0, 0.0);
+    if (lsink) b[nsink - 1] = CMPLX(1.0, 0.0);
+    return 0;
+}
+
+/* The forward loop of src/fem.f90:302-397 with its switches: refsig, lsr forced off
+ * without mixed boundaries (:311); per (l, k) potana when lsr or lana (:350), the
+ * assembly (kompab, randb, randb2, scalab, chol) when lbeta, l = 1 or lsr and not lana
+ * (:355-370), else kompb with the sink (src/kompb.f90:49-58); kpot = pota (lana), or
+ * pot fak (+ pota with lsr) (:383-391).  Errors as orc_forward. */
+int orc_forward_sw(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *kwn, const double *elbg,
+                   const double *relbg, const double *kg, int lbeta, int lsr, int lana, int lsink, int nsink,
+                   zdbl *kpot) {
+    const int sanz = g->sanz, mb = g->mb, eanz = g->eanz;
+    zdbl *a = (zdbl *)malloc(sizeof(zdbl) * (size_t)(mb + 1) * sanz);
+    zdbl *b = (zdbl *)malloc(sizeof(zdbl) * (size_t)sanz);
+    zdbl *pot = (zdbl *)malloc(sizeof(zdbl) * (size_t)sanz);
+    zdbl *pota = (zdbl *)malloc(sizeof(zdbl) * (size_t)sanz);
+    double *fak = (double *)malloc(sizeof(double) * (size_t)sanz);
+    zdbl sigma0;
+    orc_refsig(g, sigma, &sigma0);
+    if (!lbeta) lsr = 0;
+    if (lsr) lana = 0; /* src/fem.f90:198-203 */
+    int err = 0;
+    for (int k = 1; k <= kwnanz && !err; ++k) {
+        for (int l = 1; l <= eanz && !err; ++l) {
+            if (lsr || lana) orc_potana(g, kwn, l, k, &sigma0, pota);
+            if ((lbeta || l == 1 || lsr) && !lana) {
+                err = kompab_sw(g, sigma, elbg, relbg, kg, l, k, lsr, sigma0, pota, lsink, nsink, a, b);
+                if (!err && has_type13(g)) orc_randb(g, a, b);
+                if (!err && g_rwdanz) orc_randb2(g, a, b);
+                if (!err) err = orc_scalab(sanz, mb, a, b, fak);
+                if (!err) err = orc_chol(sanz, mb, a);
+            } else if (!lana) {
+                orc_kompb(sanz, g->enr[l - 1], b, fak);
+                if (lsink) b[nsink - 1] = fak[nsink - 1];
+            }
+            if (err) break;
+            if (!lana) orc_vre(sanz, mb, a, b, pot);
+            zdbl *kp = kpot + ((size_t)(k - 1) * eanz + (l - 1)) * sanz;
+            for (int j = 0; j < sanz; ++j) {
+                if (lana) kp[j] = pota[j];
+                else kp[j] = pot[j] * fak[j] + (lsr ? pota[j] : 0.0);
+            }
+        }
+    }
+    free(a); free(b); free(pot); free(pota); free(fak);
+    return err;
+}
+
+/* src/bkfak.f90:48-136: the half-space geometric factor of each measurement from the
+ * electrode positions and their mirror images in y = 0, kfak = 4 pi / ((r1 - r2) - (r3 -
+ * r4)); errnr 93 when the denominator vanishes */
+int orc_bkfak(const orc_grid *g, int nanz, const int *strnr, const int *vnr, double *kfak) {
+    const double pi = acos(-1.0);
+    double xk[4] = {0, 0, 0, 0}, yk[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nanz; ++i) {
+        const int e[4] = {strnr[i] % 10000, strnr[i] / 10000, vnr[i] % 10000, vnr[i] / 10000};
+        for (int m = 0; m < 4; ++m)
+            if (e[m] > 0) {
+                xk[m] = g->x[g->enr[e[m] - 1] - 1];
+                yk[m] = g->y[g->enr[e[m] - 1] - 1];
+            }
+        double r[4];
+        /* r1: electrodes (4, 2), r2: (4, 1), r3: (3, 2), r4: (3, 1), 0-based below */
+        const int pu[4] = {3, 3, 2, 2}, pv[4] = {1, 0, 1, 0};
+        for (int m = 0; m < 4; ++m) {
+            const int u = pu[m], v = pv[m];
+            if (e[u] > 0 && e[v] > 0) {
+                const double dx = xk[u] - xk[v], dym = yk[u] - yk[v], dyp = yk[u] + yk[v];
+                r[m] = 1.0 / sqrt(dx * dx + dym * dym) + 1.0 / sqrt(dx * dx + dyp * dyp);
+            } else {
+                r[m] = 0.0;
+            }
+        }
+        const double dum = (r[0] - r[1]) - (r[2] - r[3]);
+        if (fabs(dum) == 0.0) return 93;
+        kfak[i] = 4.0 * pi / dum;
+    }
+    return 0;
+}
+
 int orc_forward(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
                 const double *relbg, const double *kg, int lbeta, zdbl *kpot,
                 int nsys, const int *sysl, const int *sysk) {

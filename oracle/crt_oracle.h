@@ -82,6 +82,16 @@ void orc_randb(const orc_grid *g, zdbl *a, zdbl *b);
 void orc_randdc(const orc_grid *g, double *a, double *b);
 void orc_randb2(const orc_grid *g, zdbl *a, zdbl *b);
 void orc_randbdc2(const orc_grid *g, double *a, double *b);
+/* reference conductivity (src/refsig.f90), analytic half-space potential (src/potana.f90) */
+void orc_refsig(const orc_grid *g, const zdbl *sigma, zdbl *sigma0);
+void orc_potana(const orc_grid *g, const double *kwn, int l, int k, const zdbl *sigma0p, zdbl *pota);
+/* the forward loop of src/fem.f90:302-397 with singularity removal (lsr), the analytic
+ * solution alone (lana) and the fictitious sink (lsink, nsink 1-based) */
+int orc_forward_sw(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *kwn, const double *elbg,
+                   const double *relbg, const double *kg, int lbeta, int lsr, int lana, int lsink, int nsink,
+                   zdbl *kpot);
+/* half-space geometric factors (src/bkfak.f90); 93 on a vanishing denominator */
+int orc_bkfak(const orc_grid *g, int nanz, const int *strnr, const int *vnr, double *kfak);
 int orc_forward(const orc_grid *g, const zdbl *sigma, int kwnanz, const double *elbg,
                 const double *relbg, const double *kg, int lbeta, zdbl *kpot,
                 int nsys, const int *sysl, const int *sysk);

@@ -144,6 +144,47 @@ def forward(grid, sigma, elbg, relbg, kg, lbeta, systems=None):
     return kpot
 
 
+def refsig(grid, sigma) -> complex:
+    """sigma0 of src/refsig.f90."""
+    og = OracleGrid(grid)
+    s0 = np.zeros(1, np.complex128)
+    lib().orc_refsig(og.ref, ptr(np.ascontiguousarray(sigma, np.complex128)), ptr(s0))
+    return complex(s0[0])
+
+
+def potana(grid, kwn, l, k, sigma0):
+    """pota (sanz) of src/potana.f90 for electrode l, wavenumber k (1-based)."""
+    og = OracleGrid(grid)
+    pota = np.zeros(grid.sanz, np.complex128)
+    s0 = np.array([sigma0], np.complex128)
+    lib().orc_potana(og.ref, ptr(np.ascontiguousarray(kwn, np.float64)), int(l), int(k), ptr(s0), ptr(pota))
+    return pota
+
+
+def forward_sw(grid, sigma, kwn, elbg, relbg, kg, lbeta, lsr=False, lana=False, lsink=False, nsink=0):
+    """kpot (kwnanz, eanz, sanz) of the forward loop of src/fem.f90:302-397 with the
+    singularity removal (lsr), the analytic solution alone (lana) and the fictitious
+    sink (lsink at node nsink, 1-based)."""
+    og = OracleGrid(grid)
+    sigma = np.ascontiguousarray(sigma, np.complex128)
+    K = elbg.shape[0]
+    kpot = np.zeros((K, grid.eanz, grid.sanz), np.complex128)
+    err = lib().orc_forward_sw(og.ref, ptr(sigma), K, ptr(np.ascontiguousarray(kwn, np.float64)), ptr(elbg),
+                               ptr(relbg), ptr(kg), int(lbeta), int(lsr), int(lana), int(lsink), int(nsink),
+                               ptr(kpot))
+    _chk(err, "forward_sw")
+    return kpot
+
+
+def bkfak(grid, survey):
+    """kfak (nanz) of src/bkfak.f90."""
+    og = OracleGrid(grid)
+    kfak = np.zeros(survey.nanz)
+    err = lib().orc_bkfak(og.ref, survey.nanz, ptr(survey.strnr), ptr(survey.vnr), ptr(kfak))
+    _chk(err, "bkfak")
+    return kfak
+
+
 _boundary_keep = None
 
 
@@ -502,6 +543,30 @@ class RefSession:
         s = np.ascontiguousarray(sigma, np.complex128)
         _chk(self.r.ref_forward(ptr(s), int(nthreads), ptr(kpot)), "ref_forward")
         return kpot
+
+    def refsig(self, sigma):
+        s0 = np.zeros(1, np.complex128)
+        _chk(self.r.ref_refsig(ptr(np.ascontiguousarray(sigma, np.complex128)), ptr(s0)), "ref_refsig")
+        return complex(s0[0])
+
+    def potana(self, l, k):
+        """pota of electrode l, wavenumber k (1-based), with the sigma0 of refsig()."""
+        pota = np.zeros(self.grid.sanz, np.complex128)
+        _chk(self.r.ref_potana(int(l), int(k), ptr(pota)), "ref_potana")
+        return pota
+
+    def forward_sw(self, sigma, lsr=False, lana=False, lsink=False, nsink=0):
+        g = self.grid
+        kpot = np.zeros((self.K, g.eanz, g.sanz), np.complex128)
+        s = np.ascontiguousarray(sigma, np.complex128)
+        _chk(self.r.ref_forward_sw(ptr(s), int(lsr), int(lana), int(lsink), int(nsink), ptr(kpot)), "ref_forward_sw")
+        return kpot
+
+    def bkfak(self):
+        """kfak of src/bkfak.f90 (writes tmp.kfak into the working directory)."""
+        kfak = np.zeros(self.survey.nanz)
+        _chk(self.r.ref_bkfak(ptr(kfak)), "ref_bkfak")
+        return kfak
 
     def voltages(self, complex_inv=True):
         g = self.grid

@@ -208,6 +208,102 @@ contains
     ref_forward = errnr
   end function ref_forward
 
+  ! reference conductivity of sigma_ (src/refsig.f90)
+  integer(c_int) function ref_refsig(sigma_, s0_) bind(C, name='ref_refsig')
+    complex(c_double_complex), intent(in) :: sigma_(*)
+    complex(c_double_complex), intent(out) :: s0_
+    external refsig
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = sigma_(1:elanz)
+    call refsig()
+    s0_ = sigma0
+    ref_refsig = 0
+  end function ref_refsig
+
+  ! analytic half-space potential of electrode l at wavenumber k (src/potana.f90), with
+  ! the sigma0 of the last ref_refsig
+  integer(c_int) function ref_potana(l, k, pota_) bind(C, name='ref_potana')
+    integer(c_int), value :: l, k
+    complex(c_double_complex), intent(out) :: pota_(*)
+    complex(kind(0d0)), allocatable :: lp(:)
+    external potana
+    allocate(lp(sanz))
+    call potana(l, k, lp)
+    pota_(1:sanz) = lp
+    ref_potana = 0
+  end function ref_potana
+
+  ! the forward loop of src/fem.f90:302-397 with its switches (serial): refsig, lsr off
+  ! without mixed boundaries, potana for lsr or lana, the assembly for lbeta, l = 1 or
+  ! lsr (not lana), kompb with the sink otherwise, kpot = pota (lana) or pot fak (+ pota)
+  integer(c_int) function ref_forward_sw(sigma_, lsr_, lana_, lsink_, nsink_, kpot_) &
+       bind(C, name='ref_forward_sw')
+    complex(c_double_complex), intent(in) :: sigma_(*)
+    integer(c_int), value :: lsr_, lana_, lsink_, nsink_
+    complex(c_double_complex), intent(out) :: kpot_(*)
+    complex(kind(0d0)), allocatable :: la(:), lb(:), lpot(:)
+    real(kind(0d0)), allocatable :: lfak(:)
+    logical :: lana
+    integer :: j, l, k, n
+    external kompab, randb, randb2, scalab, chol, kompb, vre, potana, refsig
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = sigma_(1:elanz)
+    lsr = lsr_ /= 0
+    lana = lana_ /= 0
+    if (lsr) lana = .false.
+    lsink = lsink_ /= 0
+    nsink = nsink_
+    errnr = 0
+    call refsig()
+    if (.not. lbeta) lsr = .false.
+    if (allocated(kpot)) deallocate(kpot)
+    allocate(kpot(sanz, eanz, kwnanz))
+    allocate(la((mb + 1) * sanz), lb(sanz), lpot(sanz), lfak(sanz))
+    ! kompab's singularity removal reads femmod's pota, which fem.f90 fills by potana
+    if (allocated(pota)) deallocate(pota)
+    allocate(pota(sanz))
+    do k = 1, kwnanz
+       do l = 1, eanz
+          if (lsr .or. lana) call potana(l, k, pota)
+          if ((lbeta .or. l == 1 .or. lsr) .and. .not. lana) then
+             call kompab(l, k, la, lb)
+             if (lrandb) call randb(la, lb)
+             if (lrandb2) call randb2(la, lb)
+             call scalab(la, lb, lfak)
+             call chol(la)
+          else if (.not. lana) then
+             call kompb(l, lb, lfak)
+          end if
+          if (.not. lana) call vre(la, lb, lpot)
+          do j = 1, sanz
+             if (lana) then
+                kpot(j, l, k) = pota(j)
+             else
+                kpot(j, l, k) = lpot(j) * dcmplx(lfak(j))
+                if (lsr) kpot(j, l, k) = kpot(j, l, k) + pota(j)
+             end if
+          end do
+       end do
+    end do
+    n = size(kpot)
+    kpot_(1:n) = reshape(kpot, (/n/))
+    ref_forward_sw = errnr
+    lsr = .false.; lsink = .false.
+  end function ref_forward_sw
+
+  ! half-space geometric factors of the measurements (src/bkfak.f90; writes tmp.kfak
+  ! into the working directory)
+  integer(c_int) function ref_bkfak(kfak_) bind(C, name='ref_bkfak')
+    real(c_double), intent(out) :: kfak_(*)
+    external bkfak
+    errnr = 0
+    call bkfak()
+    ref_bkfak = errnr
+    if (errnr == 0) kfak_(1:nanz) = kfak(1:nanz)
+  end function ref_bkfak
+
   ! Bounded sample of the same OpenMP loop (k = 1..nk, l = 1..nl) for timing;
   ! returns the wall time in seconds.
   real(c_double) function ref_forward_sample(sigma_, nthreads, nk, nl) bind(C, name='ref_forward_sample')

@@ -89,6 +89,35 @@ def main(names):
         w = configs.small_quad(mixed=False, cutmck=O.cutmck)
         out, R = forward_golden(w)
         np.savez_compressed(os.path.join(HERE, "ref_small_quad_noflow.npz"), **out)
+    if want("switches"):
+        # crmod.cfg's forward switches (src/fem.f90:177-203, 302-397): singularity
+        # removal (refsig, potana, kompab's lsr terms), the analytic solution alone, the
+        # fictitious sink; bkfak's geometric factors.  kpot of the first three sources.
+        import tempfile
+        out = {}
+        for tag, w in (("quad", configs.small_quad(cutmck=O.cutmck)),
+                       ("noflow", configs.small_quad(mixed=False, cutmck=O.cutmck)),
+                       ("tri", configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck))):
+            R = O.RefSession(w.grid, w.survey)
+            R.rwaven()
+            R.precal()
+            nsink = w.grid.sanz // 2 + 3
+            out[tag + "_sigma0"] = R.refsig(w.sigma)
+            out[tag + "_pota_2_3"] = R.potana(2, 3)
+            out[tag + "_nsink"] = nsink
+            runs = {"lsink": dict(lsink=True, nsink=nsink)}
+            if tag != "noflow":
+                runs.update(lsr=dict(lsr=True), lana=dict(lana=True), lsr_lsink=dict(lsr=True, lsink=True, nsink=nsink))
+            for name, sw in runs.items():
+                out[f"{tag}_kpot_{name}"] = R.forward_sw(w.sigma, **sw)[:, :3, :]
+            cwd = os.getcwd()
+            with tempfile.TemporaryDirectory() as d:
+                os.chdir(d)
+                try:
+                    out[tag + "_kfak"] = R.bkfak()
+                finally:
+                    os.chdir(cwd)
+        np.savez_compressed(os.path.join(HERE, "ref_switches.npz"), **out)
     if want("small_tri_sytop"):
         # boundary factors with the topography-aware singularity position (lsytop)
         w = configs.c1(scale=0.2, nelec=12, nmeas=60, seed=7, cutmck=O.cutmck)
