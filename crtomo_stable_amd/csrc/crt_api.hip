@@ -43,6 +43,17 @@ int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const
                             const double *, const zd *, const zd *, zd *);
 int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
                            const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
+// general right-hand sides: singularity removal, analytic solution, fictitious sink
+int crt_launch_refsig(hipStream_t, int, int, const int *, const int *, const double *, const double *, const zd *,
+                      zd *);
+int crt_launch_potana(hipStream_t, int, int, const int4 *, const int *, const double *, const double *,
+                      const double *, const zd *, zd *, size_t);
+int crt_launch_gen_rhs(hipStream_t, const DevGrid &, int, const int4 *, int, const zd *, const zd *, size_t,
+                       const int *, const int *, int, const double *, int, zd *);
+int crt_launch_pota_kpot(hipStream_t, int, int, int, int, const int4 *, const zd *, size_t, int, zd *);
+int crt_launch_solve16_rhs(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
+                           const zd *, const zd *, zd *, zd *, int);
+int crt_launch_bkfak(hipStream_t, int, const int4 *, const int *, const double *, const double *, double *, int *);
 // forward by static condensation onto the mixed-boundary nodes (crt_condense.hip)
 int crt_launch_cond_gather(hipStream_t, int, int, int, int, const int *, const zd *, zd *);
 int crt_launch_cond_fb(hipStream_t, int, int, int, int, int, int, int, const int *, const zd *, double *, int *);
@@ -281,6 +292,12 @@ struct crt_ctx {
     // every system; 0 always the band factorisation (the reference's algorithm);
     // 1 condensed (903 where it does not apply)
     int fwd_method = -1;
+    // crmod.cfg's forward switches (src/fem.f90:177-203): singularity removal, the
+    // analytic solution alone, the fictitious sink (0-based node, -1 none)
+    int sw_lsr = 0, sw_lana = 0, sw_nsink = -1;
+    DBuf<zd> d_pota, d_sigma0;
+    DBuf<int> d_tptr, d_tq;   // per node the strictly lower CSR entries of its column
+    bool tcol_built = false;
     int last_method = 0;
     // static condensation plan of the grid (boundary nodes G, interior nodes I)
     struct Cond {
@@ -683,6 +700,9 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     CRT_HIP(hipSetDevice(c->device));
     c->have_grid = c->have_precal = c->have_kpot = c->have_volt = c->have_sens = false;
     c->cond.built = c->cond.ok = false;
+    c->tcol_built = false;
+    c->sw_lsr = c->sw_lana = 0;
+    c->sw_nsink = -1;
     c->sanz = sanz; c->mb = mb; c->typanz = typanz; c->maxn = maxn; c->eanz = eanz;
     c->npk = maxn * (maxn + 1) / 2;
     c->typ.assign(typ, typ + typanz);
@@ -1676,17 +1696,105 @@ int crt_get_condense_info(crt_ctx *c, int *info) {
     return 0;
 }
 
+// per node the strictly lower CSR entries of its column (the transpose of the row
+// lists), for the right-hand sides that sum over every entry of a row and column
+static int build_tcol(crt_ctx *c) {
+    if (c->tcol_built) return 0;
+    const int sanz = c->sanz, nnz = c->nnz;
+    std::vector<int> cnt(sanz + 1, 0), tq;
+    for (int q = 0; q < nnz; ++q)
+        if (c->h_crow[q] != c->h_col[q]) ++cnt[c->h_col[q] + 1];
+    for (int i = 0; i < sanz; ++i) cnt[i + 1] += cnt[i];
+    tq.assign(std::max(cnt[sanz], 1), 0);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int q = 0; q < nnz; ++q)
+        if (c->h_crow[q] != c->h_col[q]) tq[pos[c->h_col[q]]++] = q;
+    int e = c->d_tptr.upload(cnt.data(), cnt.size(), c->stream);
+    e = e ? e : c->d_tq.upload(tq.data(), tq.size(), c->stream);
+    if (e) return e;
+    c->tcol_built = true;
+    return 0;
+}
+
+// lana (src/fem.f90:350, 384-386): kpot is the analytic half-space potential of each
+// source, no system is assembled or solved
+static int forward_analytic(crt_ctx *c) {
+    const int K = c->kwnanz, E = c->eanz, sanz = c->sanz;
+    const int chunk = std::max(1, std::min(K * E, 1024));
+    int e = c->d_sigma0.ensure(1);
+    e = e ? e : c->d_pota.ensure((size_t)chunk * sanz);
+    e = e ? e : c->d_rhs.ensure(chunk);
+    e = e ? e : c->d_kpotN.ensure((size_t)sanz * K * E);
+    if (e) return e;
+    c->have_kpot = false;
+    c->last_method = 0;
+    if ((e = crt_launch_refsig(c->stream, c->elanz, c->maxn, c->d_etyp.p, c->d_nrel.p, c->d_x.p, c->d_y.p,
+                               c->d_sigma.p, c->d_sigma0.p)))
+        return e;
+    std::vector<int4> r;
+    for (int s0 = 0; s0 < K * E; s0 += chunk) {
+        const int n = std::min(chunk, K * E - s0);
+        r.clear();
+        for (int s = s0; s < s0 + n; ++s) r.push_back(int4{s % E, s / E, 0, 0});
+        CRT_HIP(hipMemcpyAsync(c->d_rhs.p, r.data(), n * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        e = crt_launch_potana(c->stream, sanz, n, c->d_rhs.p, c->d_enr.p, c->d_x.p, c->d_y.p, c->d_kwn.p,
+                              c->d_sigma0.p, c->d_pota.p, sanz);
+        e = e ? e : crt_launch_pota_kpot(c->stream, sanz, K, E, n, c->d_rhs.p, c->d_pota.p, sanz, 0, c->d_kpotN.p);
+        if (e) return e;
+        CRT_HIP(hipStreamSynchronize(c->stream));
+    }
+    for (int t : {T_DIAG, T_FACTOR, T_SOLVE, T_SCHUR, T_PRODUCT, T_PREFIX, T_REFINE}) c->times[t] = 0.0;
+    c->have_kpot = true;
+    c->have_volt = c->have_sens = false;
+    return 0;
+}
+
+int crt_bkfak(crt_ctx *c, double *kfak) {
+    if (!c || !kfak) return CRT_ERR_ARG;
+    if (!c->have_grid || !c->have_survey) return CRT_ERR_STATE;
+    CRT_HIP(hipSetDevice(c->device));
+    int e = c->d_kfak.ensure(c->nanz);
+    e = e ? e : c->d_status.ensure(1);
+    if (e) return e;
+    CRT_HIP(hipMemsetAsync(c->d_status.p, 0, sizeof(int), c->stream));
+    if ((e = crt_launch_bkfak(c->stream, c->nanz, c->d_meas.p, c->d_enr.p, c->d_x.p, c->d_y.p, c->d_kfak.p,
+                              c->d_status.p)))
+        return e;
+    int st = 0;
+    CRT_HIP(c->copy_out(&st, c->d_status.p, sizeof(int)));
+    if (st) return st;
+    CRT_HIP(c->copy_out(kfak, c->d_kfak.p, (size_t)c->nanz * sizeof(double)));
+    return 0;
+}
+
+int crt_set_fem_switches(crt_ctx *c, int lsr, int lana, int nsink) {
+    if (!c) return CRT_ERR_ARG;
+    if (!c->have_grid) return CRT_ERR_STATE;
+    if (nsink != 0 && (nsink < 1 || nsink > c->sanz)) return CRT_ERR_ARG;
+    c->sw_lsr = lsr ? 1 : 0;
+    c->sw_lana = lana ? 1 : 0;
+    c->sw_nsink = nsink - 1;
+    c->have_kpot = c->have_volt = c->have_sens = false;
+    return 0;
+}
+
 int crt_forward(crt_ctx *c) {
     if (!c) return CRT_ERR_ARG;
     if (!c->have_precal || !c->have_sigma) return CRT_ERR_STATE;
     CRT_HIP(hipSetDevice(c->device));
     if (int e0 = build_dmask(c)) return e0;
+    // crmod.cfg's switches (src/fem.f90:191-203, 311): lsr only with mixed boundaries,
+    // lsr switches lana off; both belong to CRMod's complex forward
+    const bool lsr = c->sw_lsr && c->lbeta, lana = c->sw_lana && !c->sw_lsr;
+    const bool gen = lsr || c->sw_nsink >= 0;
+    if ((c->sw_lsr || c->sw_lana) && c->inv_kind != 0) return CRT_ERR_UNSUPPORTED;
+    if (lana) return forward_analytic(c);
     // the condensed solve where the grid has mixed boundaries and no Dirichlet rows
-    if (c->fwd_method != 0 && c->lbeta && !c->dmask_any && c->asplit && c->fwd_bits == 64) {
+    const bool cond_ok = c->fwd_method != 0 && c->lbeta && !c->dmask_any && c->asplit && c->fwd_bits == 64 && !gen;
+    if (cond_ok) {
         if (int e0 = cond_plan(c)) return e0;
     }
-    const bool cond = c->fwd_method != 0 && c->lbeta && !c->dmask_any && c->asplit && c->fwd_bits == 64 &&
-                      c->cond.ok;
+    const bool cond = cond_ok && c->cond.ok;
     if (c->fwd_method == 1 && !cond) return CRT_ERR_UNSUPPORTED;
     if (cond) {
         c->refine_sweeps = 0;
@@ -1709,6 +1817,9 @@ int crt_forward(crt_ctx *c) {
     if (!use16 && crt_factor_lds_bytes(D) > 160 * 1024) return CRT_ERR_UNSUPPORTED;
     const bool fp32 = c->fwd_bits == 32;
     if (fp32 && !(use16 && crt_factor32_supported(c->mb) && c->tmapD == D)) return CRT_ERR_UNSUPPORTED;
+    // general right-hand sides run through the 16-row FP64 substitution, without
+    // Dirichlet rows (the reference applies randb / randb2 to them as well)
+    if (gen && (!use16 || fp32 || c->dmask_any)) return CRT_ERR_UNSUPPORTED;
     // systems to factorise: every (l,k) with mixed boundaries, else l = 1 per k
     // (src/fem.f90:355: lbeta.or.l.eq.1)
     std::vector<int2> sys;
@@ -1751,6 +1862,15 @@ int crt_forward(crt_ctx *c) {
     size_t nyscr = c->lbeta ? 1 : rhs.size();
     // refinement: per right-hand side of a batch the potentials x and the residual
     const size_t rhs_batch = c->lbeta ? batch : std::min(rhs.size(), batch * (size_t)E);
+    if (gen) {
+        nyscr = std::max(nyscr, rhs_batch);
+        if ((e = build_tcol(c))) return e;
+        if ((e = c->d_sigma0.ensure(1))) return e;
+        if (lsr && (e = c->d_pota.ensure(rhs_batch * (size_t)c->sanz))) return e;
+        if (lsr && (e = crt_launch_refsig(c->stream, c->elanz, c->maxn, c->d_etyp.p, c->d_nrel.p, c->d_x.p, c->d_y.p,
+                                          c->d_sigma.p, c->d_sigma0.p)))
+            return e;
+    }
     if (fp32) {
         nyscr = std::max(nyscr, rhs_batch);
         if ((e = c->d_xacc.ensure(rhs_batch * fstride))) return e;
@@ -1816,7 +1936,23 @@ int crt_forward(crt_ctx *c) {
             if (e) return e;
             c->toc(T_FACTOR);
             c->tic(T_SOLVE);
-            if (use16)
+            if (gen) {
+                // the singularity removal's or the sink's right-hand side, the substitution
+                // from it, and the analytic part added (src/fem.f90:350-389)
+                const int nr = (int)r.size();
+                if (lsr)
+                    e = crt_launch_potana(c->stream, c->sanz, nr, c->d_rhs.p, c->d_enr.p, c->d_x.p, c->d_y.p,
+                                          c->d_kwn.p, c->d_sigma0.p, c->d_pota.p, c->sanz);
+                e = e ? e : crt_launch_gen_rhs(c->stream, g, nr, c->d_rhs.p, lsr ? 1 : 0, c->d_sigma0.p, c->d_pota.p,
+                                               c->sanz, c->d_tptr.p, c->d_tq.p, c->sw_nsink, c->d_fak.p,
+                                               (int)fstride, c->d_yscr.p);
+                e = e ? e : crt_launch_solve16_rhs(c->stream, c->sanz, nblk, D, K, E, nr, c->d_rhs.p, c->d_enr.p,
+                                                   c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yscr.p,
+                                                   c->d_kpotN.p, crt_factor16_resolve(D, 0) != 3);
+                if (!e && lsr)
+                    e = crt_launch_pota_kpot(c->stream, c->sanz, K, E, nr, c->d_rhs.p, c->d_pota.p, c->sanz, 1,
+                                             c->d_kpotN.p);
+            } else if (use16)
                 e = crt_launch_solve16(c->stream, c->sanz, nblk, D, K, E, (int)r.size(), c->d_rhs.p, c->d_enr.p,
                                        c->d_fak.p, c->d_panels.p, c->d_dinv.p, c->d_yf.p, c->d_yscr.p, c->d_kpotN.p,
                                        crt_factor16_resolve(D, 0) != 3);
@@ -3023,6 +3159,8 @@ int crt_cholz(crt_ctx *c, int n, double *a, double *p) {
 
 // ------------------------------------------------------------------ batches
 int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int flags, crt_batch **out) {
+    for (int i = 0; ctxs && i < n; ++i)  // unit sources only: no forward switches in a batch
+        if (ctxs[i] && (ctxs[i]->sw_lsr || ctxs[i]->sw_lana || ctxs[i]->sw_nsink >= 0)) return CRT_ERR_UNSUPPORTED;
     if (!ctxs || n <= 0 || !out || (flags & ~1)) return CRT_ERR_ARG;
     *out = nullptr;
     for (int i = 0; i < n; ++i) {

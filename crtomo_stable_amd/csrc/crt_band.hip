@@ -634,3 +634,101 @@ int crt_launch_solve(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int 
     hipLaunchKernelGGL(crt_solve_kernel, dim3((nrhs + waves - 1) / waves), dim3(64 * waves), lds, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
+
+// ------------------------------------------------ general right-hand sides
+// The right-hand side of system r = (l, k, factor fs) when it is not a unit source,
+// scaled as src/scalab.f90:64 scales it (yscr[r][i] = fak[fs][i] b_i, zero on the
+// padding rows):
+//   lsr:   b_i = sum over the entries q of row and column i of the system's matrix of
+//          sum_p dum_p (sigma_p - sigma0) pota_r(other node of q), dum_p and sigma_p of
+//          the entry's contributions (the singularity removal of src/kompab.f90:129-133:
+//          the matrix of sigma - sigma0 times the analytic potential);
+//   else:  b = -e_enr(l) (src/kompab.f90:162, src/kompb.f90:58);
+//   lsink: b_nsink = 1 (src/kompab.f90:183, src/kompb.f90:59), set last.
+// tptr/tq: per node the strictly lower entries whose column it is.
+__global__ void __launch_bounds__(256) crt_gen_rhs_kernel(DevGrid g, int nrhs, const int4 *__restrict__ rhs, int lsr,
+                                                          const zd *__restrict__ sigma0, const zd *__restrict__ pota,
+                                                          size_t pstride, const int *__restrict__ tptr,
+                                                          const int *__restrict__ tq, int nsink0,
+                                                          const double *__restrict__ fak, int ystride,
+                                                          zd *__restrict__ yscr) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nrhs * ystride) return;
+    const int r = (int)(t / ystride), i = (int)(t - (long)r * ystride);
+    zd *yo = yscr + (size_t)r * ystride;
+    if (i >= g.sanz) {
+        yo[i] = zd{0.0, 0.0};
+        return;
+    }
+    const int4 q4 = rhs[r];
+    const int l = q4.x, k = q4.y;
+    zd b = zd{0.0, 0.0};
+    if (lsr) {
+        const zd s0 = *sigma0;
+        const zd *p = pota + (size_t)r * pstride;
+        auto dq = [&](int q) {
+            zd d = zd{0.0, 0.0};
+            for (int pp = g.cptr[q]; pp < g.cptr[q + 1]; ++pp) {
+                const int c = g.contrib[pp];
+                double dum;
+                zd sg;
+                if (c >= 0) {
+                    const int e = c / g.npk, imn = c - e * g.npk;
+                    dum = g.elbg[((size_t)k * g.npk + imn) * g.elanz + e];
+                    sg = g.sigma[e];
+                } else {
+                    const int cc = -c - 1;
+                    const int rel = cc / g.npk, imn = cc - rel * g.npk;
+                    dum = g.relbg[(size_t)imn * g.relanz + rel] * g.kg[((size_t)k * g.eanz + l) * g.relanz + rel];
+                    sg = g.sigma[g.rnr[rel]];
+                }
+                d.re += dum * (sg.re - s0.re);
+                d.im += dum * (sg.im - s0.im);
+            }
+            return d;
+        };
+        for (int q = g.row_ptr[i]; q < g.row_ptr[i + 1]; ++q) b = zfma(b, dq(q), p[g.col[q]]);
+        for (int u = tptr[i]; u < tptr[i + 1]; ++u) {
+            const int q = tq[u];
+            b = zfma(b, dq(q), p[g.crow[q]]);
+        }
+    } else if (i == g.enr[l]) {
+        b = zd{-1.0, 0.0};
+    }
+    if (i == nsink0) b = zd{1.0, 0.0};
+    const double f = fak[(size_t)q4.z * g.fstride + i];
+    yo[i] = zd{b.re * f, b.im * f};
+}
+
+// kpot of the systems r: + pota (the singularity removal's analytic part, src/fem.f90:389)
+// or = pota (the analytic solution alone, src/fem.f90:384-386)
+__global__ void __launch_bounds__(256) crt_pota_kpot_kernel(int sanz, int K, int E, int nrhs,
+                                                            const int4 *__restrict__ rhs, const zd *__restrict__ pota,
+                                                            size_t pstride, int add, zd *__restrict__ kpotN) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nrhs * sanz) return;
+    const int r = (int)(t / sanz), i = (int)(t - (long)r * sanz);
+    const int4 q4 = rhs[r];
+    zd *o = kpotN + ((size_t)i * K + q4.y) * E + q4.x;
+    const zd p = pota[(size_t)r * pstride + i];
+    *o = add ? zd{o->re + p.re, o->im + p.im} : p;
+}
+
+int crt_launch_gen_rhs(hipStream_t st, const DevGrid &g, int nrhs, const int4 *rhs, int lsr, const zd *sigma0,
+                       const zd *pota, size_t pstride, const int *tptr, const int *tq, int nsink0, const double *fak,
+                       int ystride, zd *yscr) {
+    const long n = (long)nrhs * ystride;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(crt_gen_rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nrhs, rhs, lsr,
+                       sigma0, pota, pstride, tptr, tq, nsink0, fak, ystride, yscr);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_pota_kpot(hipStream_t st, int sanz, int K, int E, int nrhs, const int4 *rhs, const zd *pota,
+                         size_t pstride, int add, zd *kpotN) {
+    const long n = (long)nrhs * sanz;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(crt_pota_kpot_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sanz, K, E, nrhs,
+                       rhs, pota, pstride, add, kpotN);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

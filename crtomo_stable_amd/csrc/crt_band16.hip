@@ -1268,7 +1268,8 @@ int crt_launch_factor16(hipStream_t st, const DevGrid &g16, int nsys, const int2
 // skipped when the factorisation already produced y.  Lanes hold tile entries
 // (r, c) = (l/16 + 4v, l%16), v = 0..3.
 // PT is the panel element type: zd, or zf for the FP32 factorisation (read into
-// FP64 arithmetic).  MODE 0: x scaled by fak into kpotN.  MODE 1 (first solve of
+// FP64 arithmetic).  MODE 0: x scaled by fak into kpotN.  MODE 3: as MODE 0 for the
+// scaled right-hand side in yscr (overwritten by y) instead of the unit source.  MODE 1 (first solve of
 // the refinement): also x into xacc.  MODE 2 (refinement sweep): the right-hand
 // side is the negated residual in yscr (overwritten by y), x is added to xacc,
 // kpotN gets fak * xacc, and stats[2 rid + 0/1] the max-norms of fak * x and of
@@ -1345,10 +1346,10 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
     const PT *dinv = A.dinv + (size_t)fs * nblk * T16;
     const int enode = A.enr[l];
     zd *yscr = A.yscr + (size_t)rid * nblk * T16;
-    // a refinement sweep runs the forward pass over every block (its right-hand
-    // side is a full residual); otherwise y is zero above the electrode's block
-    const bool have_y = MODE == 2 ? false : q4.w != 0;
-    const int J0 = (MODE == 2 || have_y) ? 0 : enode / T16;
+    // a refinement sweep (MODE 2) and a general right-hand side (MODE 3) run the forward
+    // pass over every block; otherwise y is zero above the electrode's block
+    const bool have_y = MODE >= 2 ? false : q4.w != 0;
+    const int J0 = (MODE >= 2 || have_y) ? 0 : enode / T16;
 
     // The two wave roles run separate loops with the same barrier sequence, so
     // their registers are allocated independently.
@@ -1364,7 +1365,7 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 const int r = c;  // lane r = lane & 15 carries row r of the block
                 const int i = J * T16 + r;
                 zd yv;
-                if constexpr (MODE == 2) yv = yscr[(size_t)J * T16 + r];
+                if constexpr (MODE >= 2) yv = yscr[(size_t)J * T16 + r];
                 else yv = (i < A.sanz && i == enode) ? zd{-fak[i], 0.0} : zd{0.0, 0.0};
                 __syncthreads();
 #pragma unroll
@@ -1471,7 +1472,7 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
                 if (i < A.sanz) {
                     const double f = fcur;
                     zd xo = xv;
-                    if constexpr (MODE >= 1) {
+                    if constexpr (MODE == 1 || MODE == 2) {
                         zd *xa = A.xacc + (size_t)rid * nblk * T16 + i;
                         if constexpr (MODE == 2) xo = zadd(*xa, xv);
                         *xa = xo;
@@ -1490,7 +1491,7 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
             move(s2, sn);
             __syncthreads();
         }
-        if constexpr (MODE >= 1) {
+        if constexpr (MODE == 1 || MODE == 2) {
 #pragma unroll
             for (int m = 1; m < T16; m <<= 1) {
                 dmax = fmax(dmax, __shfl_xor(dmax, m, 64));
@@ -1642,6 +1643,14 @@ int crt_launch_solve16(hipStream_t st, int sanz, int nblk, int D, int kwnanz, in
                        zd *kpotN, int inv) {
     Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, nullptr, nullptr};
     return inv ? solve16_dispatch<zd, 0, true>(st, D, a) : solve16_dispatch<zd, 0, false>(st, D, a);
+}
+
+// general right-hand sides (yscr[rhs][nblk*16], scaled; overwritten by y)
+int crt_launch_solve16_rhs(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs, const int4 *rhs,
+                           const int *enr, const double *fak, const zd *panels, const zd *dinv, zd *yscr, zd *kpotN,
+                           int inv) {
+    Solve16Args a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, nullptr, yscr, kpotN, nullptr, nullptr};
+    return inv ? solve16_dispatch<zd, 3, true>(st, D, a) : solve16_dispatch<zd, 3, false>(st, D, a);
 }
 
 // substitution through the FP32 panels for the refinement: mode 1 is the first

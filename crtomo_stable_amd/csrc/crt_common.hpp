@@ -7,7 +7,8 @@
 // Error numbers follow the reference's errmod codes (src/error.txt):
 //   19 band width exceeded, 26 element geometry, 27 zero diagonal,
 //   28 non-positive pivot, 35 beta geometry, 73 duplicate electrodes,
-//   82 zero voltage, 94/97 allocation.  Library-only codes start at 900.
+//   82 zero voltage, 93 vanishing geometric factor, 94/97 allocation.  Library-only
+//   codes start at 900.
 enum {
     CRT_OK = 0,
     CRT_ERR_BAND = 19,
@@ -17,6 +18,7 @@ enum {
     CRT_ERR_BETA = 35,
     CRT_ERR_ELEC = 73,
     CRT_ERR_ZEROVOLT = 82,
+    CRT_ERR_KFAK = 93,
     CRT_ERR_BOUNDARY_NODE = 30,
     CRT_ERR_BOUNDARY_MIX = 101,
     CRT_ERR_ALLOC = 97,

@@ -342,14 +342,18 @@ int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path) {
                           g.nrel.data(), g.x.data(), g.y.data(), g.rnr.data(), ne, enr.data())))
         return e;
     if ((e = crt_set_survey(ctx, nc, strnr.data(), vnr.data()))) return e;
-    // switches this forward does not run are refused instead of ignored
-    if (cfg.lagain || cfg.lsink || (cfg.mswitch & 1) || (cfg.mswitch & 4)) {
-        fprintf(stderr, "crt_tomodir_crmod: %s is not supported\n",
-                cfg.lagain ? "another dataset (lagain)" : cfg.lsink ? "the fictitious sink (lsink)" :
-                (cfg.mswitch & 1) ? "the analytic solution (mswitch lana)" : "singularity removal (mswitch lsr)");
+    // another dataset in the same run (lagain) is a host loop this call does not make
+    if (cfg.lagain) {
+        fprintf(stderr, "crt_tomodir_crmod: another dataset (lagain) is not supported\n");
         return CRT_ERR_UNSUPPORTED;
     }
     if (cfg.swrtr != 0 && cfg.swrtr != 1) return 3;
+    // mswitch bits 0 (lana) and 2 (lsr; lsr switches lana off), both off in 2D
+    // (src/fem.f90:191-203, 241-243); the fictitious sink (src/fem.f90:179-180)
+    {
+        const bool lsr = (cfg.mswitch & 4) && cfg.swrtr != 0, lana = (cfg.mswitch & 1) && cfg.swrtr != 0 && !lsr;
+        if ((e = crt_set_fem_switches(ctx, lsr ? 1 : 0, lana ? 1 : 0, cfg.lsink ? cfg.nsink : 0))) return e;
+    }
     if (cfg.lrandb2) {
         std::vector<int> bn;
         std::vector<double> bv;
@@ -369,15 +373,17 @@ int crt_tomodir_crmod(crt_ctx *ctx, const char *cfg_path) {
     } else if ((e = crt_rwaven(ctx, &kwnanz, kwn.data(), kwnwi.data(), &amin, &amax))) return e;
     int lbeta = 0;
     if ((e = crt_precal(ctx, 0, &lbeta))) return e;
+    // wkfak with mixed boundaries: bvolt multiplies by bkfak's geometric factors
+    // (src/fem.f90:417-423), otherwise kfak = 1
+    std::vector<double> kfak;
     if ((cfg.mswitch & 2) && lbeta) {
-        // wkfak with mixed boundaries asks for the numerical k-factors of bkfak
-        fprintf(stderr, "crt_tomodir_crmod: k-factors (mswitch wkfak) are not supported\n");
-        return CRT_ERR_UNSUPPORTED;
+        kfak.resize(nc);
+        if ((e = crt_bkfak(ctx, kfak.data()))) return e;
     }
     if ((e = crt_set_sigma(ctx, sigma.data()))) return e;
     if ((e = crt_forward(ctx))) return e;
     std::vector<double> volt(2 * (size_t)nc), sigmaa(2 * (size_t)nc);
-    if ((e = crt_voltages(ctx, 0, nullptr, volt.data(), sigmaa.data()))) return e;
+    if ((e = crt_voltages(ctx, 0, kfak.empty() ? nullptr : kfak.data(), volt.data(), sigmaa.data()))) return e;
     if (cfg.wvolt && (e = crt_write_volt(cfg.volt.c_str(), nc, strnr.data(), vnr.data(), sigmaa.data()))) return e;
     if (cfg.wpot) {
         // bpot (src/bpot.f90:25-40): per measurement the potential of its current

@@ -262,3 +262,102 @@ int crt_launch_precal(hipStream_t s, int elanz, int relanz, int maxn, int npk, i
     }
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
+
+// ------------------------------------------------ singularity removal, analytic solution
+// Reference conductivity (src/refsig.f90:33-70): sigma0 = exp(sum_e A_e log sigma_e /
+// sum_e A_e) over the area elements, A_e by Gauss' area formula (triangles from nodes
+// 1-3, quadrilaterals as the triangles 1-2-3 and 1-3-4).  One workgroup: each thread sums
+// a stride of the elements, the partial sums meet in a fixed-order tree.
+__global__ void __launch_bounds__(256) crt_refsig_kernel(int elanz, int maxn, const int *__restrict__ etyp,
+                                                         const int *__restrict__ nrel, const double *__restrict__ x,
+                                                         const double *__restrict__ y, const zd *__restrict__ sigma,
+                                                         zd *__restrict__ sigma0) {
+    __shared__ double red[3][256];
+    double sr = 0.0, si = 0.0, ar = 0.0;
+    for (int e = threadIdx.x; e < elanz; e += 256) {
+        const int *nd = nrel + (size_t)e * maxn;
+        const double xk = x[nd[0] - 1], yk = y[nd[0] - 1];
+        const double ax = x[nd[1] - 1] - xk, ay = y[nd[1] - 1] - yk;
+        const double bx = x[nd[2] - 1] - xk, by = y[nd[2] - 1] - yk;
+        double dum = 0.0;
+        if (etyp[e] == 3) {
+            dum = fabs(ax * by - ay * bx) / 2.0;
+        } else {
+            const double cx = x[nd[3] - 1] - xk, cy = y[nd[3] - 1] - yk;
+            dum = (fabs(ax * by - ay * bx) + fabs(bx * cy - by * cx)) / 2.0;
+        }
+        const zd s = sigma[e];
+        sr += dum * log(hypot(s.re, s.im));
+        si += dum * atan2(s.im, s.re);
+        ar += dum;
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    red[2][threadIdx.x] = ar;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int v = 0; v < 3; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = exp(red[0][0] / red[2][0]), ph = red[1][0] / red[2][0];
+        *sigma0 = zd{m * cos(ph), m * sin(ph)};
+    }
+}
+
+// Analytic potential of a unit current in a homogeneous half-space of conductivity
+// sigma0 (src/potana.f90:51-97), one workgroup per (electrode, wavenumber) system r:
+// pota[r][j] = (K0(k rm) + K0(k rp)) / (2 pi sigma0), rm and rp the distances of node j
+// to the electrode and to its mirror image in y = 0; the electrode's own node gets five
+// times the largest value of the other nodes.
+__global__ void __launch_bounds__(256) crt_potana_kernel(int sanz, int nrhs, const int4 *__restrict__ rhs,
+                                                         const int *__restrict__ enr, const double *__restrict__ x,
+                                                         const double *__restrict__ y, const double *__restrict__ kwn,
+                                                         const zd *__restrict__ sigma0, zd *__restrict__ pota,
+                                                         size_t pstride) {
+    __shared__ double red[256];
+    const int r = blockIdx.x;
+    if (r >= nrhs) return;
+    const int4 q = rhs[r];
+    const int idum = enr[q.x];
+    const double kk = kwn[q.y], xk1 = x[idum], yk1 = y[idum];
+    zd *p = pota + (size_t)r * pstride;
+    double pmax = 0.0;
+    for (int j = threadIdx.x; j < sanz; j += 256) {
+        if (j == idum) continue;
+        const double x21 = x[j] - xk1, y21m = y[j] - yk1, y21p = y[j] + yk1;
+        const double rm = sqrt(x21 * x21 + y21m * y21m), rp = sqrt(x21 * x21 + y21p * y21p);
+        const double dum = bessk0(rm * kk) + bessk0(rp * kk);
+        pmax = fmax(pmax, dum);
+        p[j] = zd{dum, 0.0};
+    }
+    red[threadIdx.x] = pmax;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const double pi = acos(-1.0);
+    const zd s0 = *sigma0;
+    const double den = s0.re * s0.re + s0.im * s0.im;
+    const zd sc = zd{0.5 / pi * s0.re / den, -0.5 / pi * s0.im / den};
+    for (int j = threadIdx.x; j < sanz; j += 256) {
+        const double v = j == idum ? 5.0 * red[0] : p[j].re;
+        p[j] = zd{v * sc.re, v * sc.im};
+    }
+}
+
+int crt_launch_refsig(hipStream_t s, int elanz, int maxn, const int *etyp, const int *nrel, const double *x,
+                      const double *y, const zd *sigma, zd *sigma0) {
+    hipLaunchKernelGGL(crt_refsig_kernel, dim3(1), dim3(256), 0, s, elanz, maxn, etyp, nrel, x, y, sigma, sigma0);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_potana(hipStream_t s, int sanz, int nrhs, const int4 *rhs, const int *enr, const double *x,
+                      const double *y, const double *kwn, const zd *sigma0, zd *pota, size_t pstride) {
+    if (nrhs < 1) return 0;
+    hipLaunchKernelGGL(crt_potana_kernel, dim3(nrhs), dim3(256), 0, s, sanz, nrhs, rhs, enr, x, y, kwn, sigma0, pota,
+                       pstride);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

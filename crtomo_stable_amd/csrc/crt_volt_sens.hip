@@ -500,3 +500,43 @@ int crt_launch_kpot_ref(hipStream_t s, int sanz, int eanz, int K, const zd *kpot
                        out);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
+
+// Geometric factors of a homogeneous half-space (src/bkfak.f90:48-136), one thread per
+// measurement: with the electrodes A, B (current) and M, N (potential) and their
+// mirror images in y = 0, r(P, Q) = 1/|P - Q| + 1/|P - Q'|, kfak = 4 pi / ((r(N,B) -
+// r(N,A)) - (r(M,B) - r(M,A))); a vanishing denominator is errnr 93.
+__global__ void __launch_bounds__(256) crt_bkfak_kernel(int nanz, const int4 *__restrict__ meas,
+                                                        const int *__restrict__ enr, const double *__restrict__ x,
+                                                        const double *__restrict__ y, double *__restrict__ kfak,
+                                                        int *__restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nanz) return;
+    const int4 m = meas[i];
+    const int e[4] = {m.x, m.y, m.z, m.w};
+    double xk[4], yk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        xk[j] = e[j] > 0 ? x[enr[e[j] - 1]] : 0.0;
+        yk[j] = e[j] > 0 ? y[enr[e[j] - 1]] : 0.0;
+    }
+    auto rr = [&](int u, int v) {
+        if (e[u] <= 0 || e[v] <= 0) return 0.0;
+        const double dx = xk[u] - xk[v], dym = yk[u] - yk[v], dyp = yk[u] + yk[v];
+        return 1.0 / sqrt(dx * dx + dym * dym) + 1.0 / sqrt(dx * dx + dyp * dyp);
+    };
+    const double dum = (rr(3, 1) - rr(3, 0)) - (rr(2, 1) - rr(2, 0));
+    if (fabs(dum) == 0.0) {
+        *status = CRT_ERR_KFAK;
+        kfak[i] = 0.0;
+        return;
+    }
+    kfak[i] = 4.0 * acos(-1.0) / dum;
+}
+
+int crt_launch_bkfak(hipStream_t st, int nanz, const int4 *meas, const int *enr, const double *x, const double *y,
+                     double *kfak, int *status) {
+    if (nanz < 1) return 0;
+    hipLaunchKernelGGL(crt_bkfak_kernel, dim3((nanz + 255) / 256), dim3(256), 0, st, nanz, meas, enr, x, y, kfak,
+                       status);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}

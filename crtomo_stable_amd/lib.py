@@ -25,7 +25,8 @@ EXPORTS = (
 ERRORS = {
     18: "unknown element type", 19: "band width exceeded", 26: "element geometry (det <= 0)",
     27: "zero diagonal", 28: "non-positive pivot in Cholesky", 35: "boundary geometry in beta",
-    73: "duplicate electrodes", 82: "zero voltage", 97: "allocation failed", 900: "HIP runtime error",
+    73: "duplicate electrodes", 82: "zero voltage", 93: "vanishing geometric factor", 97: "allocation failed",
+    900: "HIP runtime error",
     901: "call order (missing set-up call)", 902: "invalid argument", 903: "unsupported configuration",
 }
 
@@ -334,6 +335,18 @@ class Context:
         m = np.zeros((n.value, 5), np.float32)
         _chk(self.lib.crt_pc_data(self._h, C.byref(n), _ptr(f), _ptr(t), _ptr(m)), "crt_pc_data")
         return f, t, m
+
+    def set_fem_switches(self, lsr: bool = False, lana: bool = False, nsink: int = 0):
+        """crmod.cfg's forward switches: singularity removal, the analytic solution
+        alone, the fictitious sink node (1-based, 0 none)."""
+        _chk(self.lib.crt_set_fem_switches(self._h, int(bool(lsr)), int(bool(lana)), int(nsink)),
+             "crt_set_fem_switches")
+
+    def bkfak(self):
+        """Half-space geometric factors of the survey (bkfak)."""
+        k = np.zeros(self.survey.nanz)
+        _chk(self.lib.crt_bkfak(self._h, _ptr(k)), "crt_bkfak")
+        return k
 
     def pc_reset(self):
         """Drop the learned preconditioner's training set and model."""

@@ -68,6 +68,26 @@ int crt_set_wavenumbers(crt_ctx *ctx, int kwnanz, const double *kwn, const doubl
  * removes them. */
 int crt_set_boundary_values(crt_ctx *ctx, int n, const int *nodes, const double *values);
 
+/* crmod.cfg's forward switches (src/fem.f90:177-203; reset by crt_set_grid):
+ *   lsr:   singularity removal (mswitch bit 2): per (electrode, wavenumber) the analytic
+ *          half-space potential pota of a unit current in the reference conductivity
+ *          sigma0 (src/refsig.f90, src/potana.f90) is removed: the right-hand side is
+ *          (A(sigma) - A(sigma0)) pota (src/kompab.f90:129-133) and kpot = the solution +
+ *          pota (src/fem.f90:389); only with mixed boundaries (otherwise off, :311);
+ *   lana:  the analytic solution alone (mswitch bit 0): kpot = pota, no system is
+ *          solved (src/fem.f90:384-386); off when lsr is on (:198-203);
+ *   nsink: the fictitious sink node (lsink, 1-based; 0 none): b(nsink) = 1 in every
+ *          system (src/kompab.f90:183, src/kompb.f90:59, the DC twins alike).
+ * lsr and lana are CRMod's complex forward only (903 with crt_set_inversion_kind 1/2 at
+ * crt_forward); general right-hand sides run on the band method with 16-row tiles (903
+ * for bands wider than 256, the FP32 forward or Dirichlet rows) and not in crt_batch_*. */
+int crt_set_fem_switches(crt_ctx *ctx, int lsr, int lana, int nsink);
+
+/* Geometric factors of the survey's measurements for a homogeneous half-space
+ * (bkfak, src/bkfak.f90:48-136; wkfak, mswitch bit 1, with mixed boundaries): kfak
+ * (nanz) for crt_voltages mode 0.  Returns 93 when a denominator vanishes. */
+int crt_bkfak(crt_ctx *ctx, double *kfak);
+
 /* Element matrices and mixed-boundary factors on the device (src/precal.f90:13);
  * errnr 101 for a grid with both mixed (type 11) and Dirichlet (type 13) boundaries. */
 int crt_precal(crt_ctx *ctx, int lsytop, int *lbeta);
@@ -184,7 +204,7 @@ int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
  *   crt_pc_grad: loss and parameter gradient on the training set (no update).
  *   crt_pc_data: the training set's raw features (n x 8), targets (the corrections
  *     to the analytic estimate) and mask (n x 5); *n holds the buffers' capacity in
- *     cells on entry (900 when it is too small; NULL buffers only query) and the
+ *     cells on entry (902 when it is too small; NULL buffers only query) and the
  *     number of cells added so far on return.
  *   crt_pc_reset: drops the training set, the model and its standardisation (the
  *     next crt_pc_train starts a new network). */

@@ -236,11 +236,10 @@ def test_tomodir_crmod_2d_dirichlet(ctx, oracle_lib, tmp_path):
     print(f"[tomodir 2D dirichlet] |V| {np.abs(sa)}", flush=True)
 
 
-@pytest.mark.parametrize("line,text", [(13, "T ! fictitious sink ?"), (11, "T ! another dataset ?"),
-                                       (17, "   4")])
+@pytest.mark.parametrize("line,text", [(11, "T ! another dataset ?")])
 def test_tomodir_crmod_refuses_unsupported_switches(ctx, tmp_path, line, text):
-    """The switches this forward does not run (lsink, lagain, mswitch lsr) return 903
-    instead of being ignored."""
+    """The switch this call does not run (lagain: a second dataset in the same run)
+    returns 903 instead of being ignored."""
     from crtomo_stable_amd.lib import CRTError
     from test_lib_cpu import _tomodir
     cfg, _ = _tomodir(str(tmp_path), "minimal")
@@ -252,3 +251,44 @@ def test_tomodir_crmod_refuses_unsupported_switches(ctx, tmp_path, line, text):
     with pytest.raises(CRTError) as ei:
         ctx.tomodir_crmod(cfg)
     assert ei.value.code == 903
+
+
+@pytest.mark.parametrize("mswitch,lsink", [(6, True), (1, False), (2, True)])
+def test_tomodir_crmod_switches(ctx, oracle_lib, tmp_path, mswitch, lsink):
+    """crmod.cfg's mswitch (bit 0 lana, 1 wkfak, 2 lsr) and the fictitious sink run
+    through the tomodir: volt.dat against the oracle's forward with the same switches
+    (src/fem.f90:302-423: forward_sw, rtrafo, bkfak, bvolt), written by the same
+    writer, within the file's REAL*4."""
+    import os
+    from crtomo_stable_amd import grid as G, lib
+    from test_lib_cpu import _tomodir
+    O = oracle_lib
+    cfg, src = _tomodir(str(tmp_path), "minimal")
+    g = G.read_elem(os.path.join(src, "elem.dat"))
+    g = G.replace(g, enr=G.read_elec(os.path.join(src, "elec.dat")))
+    g = G.fix_rnr(g)
+    survey, _, _ = G.read_volt(os.path.join(src, "volt.dat"))
+    sigma = G.read_rho(os.path.join(src, "rho.dat"))
+    nsink = g.sanz // 2
+    lines = open(cfg).read().rstrip("\n").split("\n")
+    lines[13] = ("T" if lsink else "F") + " ! fictitious sink ?"
+    lines[14] = f"{nsink} ! fictitious sink node number"
+    lines.append(f"{mswitch:4d}")
+    open(cfg, "w").write("\n".join(lines) + "\n")
+    print(f"[tomodir switches] mswitch={mswitch} lsink={lsink} nsink={nsink}", flush=True)
+    ctx.tomodir_crmod(cfg)
+    s_out, m_out, p_out = G.read_volt(os.path.join(str(tmp_path), "mod", "volt.dat"))
+    kwn, kwi, _, _ = O.rwaven(g, survey)
+    elbg, relbg, kg, lb = O.precal(g, kwn)
+    lsr, lana = bool(mswitch & 4), bool(mswitch & 1) and not (mswitch & 4)
+    kp = O.forward_sw(g, sigma, kwn, elbg, relbg, kg, lb, lsr=lsr, lana=lana, lsink=lsink, nsink=nsink)
+    kfak = O.bkfak(g, survey) if (mswitch & 2) and lb else None
+    _, sa = O.bvolt(g, survey, O.rtrafo(kp, kwi), kfak)
+    ref = os.path.join(str(tmp_path), "ref_volt.dat")
+    lib.write_volt(ref, survey.strnr, survey.vnr, sa)
+    s_ref, m_ref, p_ref = G.read_volt(ref)
+    assert np.array_equal(s_out.strnr, s_ref.strnr) and np.array_equal(s_out.vnr, s_ref.vnr)
+    em = float(np.max(np.abs(m_out - m_ref) / m_ref))
+    ep = float(np.max(np.abs(p_out - p_ref)))
+    print(f"[tomodir switches] |V| {em:.2e}, phase {ep:.2e} mrad", flush=True)
+    assert em < 1e-6 and ep < 1e-3
