@@ -51,6 +51,16 @@ module crt_hip_mod
       type(c_ptr), value :: ctx
       complex(c_double_complex) :: sigma(*)
     end function
+    integer(c_int) function crt_set_fem_switches(ctx, lsr, lana, nsink) bind(C, name='crt_set_fem_switches')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lsr, lana, nsink
+    end function
+    integer(c_int) function crt_bkfak(ctx, kfak) bind(C, name='crt_bkfak')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: kfak(*)
+    end function
     integer(c_int) function crt_set_inversion_kind(ctx, kind) bind(C, name='crt_set_inversion_kind')
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
@@ -176,6 +186,7 @@ module crt_hip_mod
     end function
   end interface
   public :: crt_create, crt_set_grid, crt_set_survey, crt_set_wavenumbers, crt_precal, crt_set_sigma, &
+       crt_set_fem_switches, crt_bkfak, &
        crt_set_inversion_kind, crt_forward, crt_get_kpot, crt_voltages, crt_get_hpot, crt_sens, crt_get_sens, &
        crt_set_sens, crt_set_regularization, crt_bsmatm, crt_bnachbar, crt_blam0, crt_cjg, crt_cjg_noalloc, &
        crt_chol, crt_vre, crt_choldc, crt_vredc, crt_cholz

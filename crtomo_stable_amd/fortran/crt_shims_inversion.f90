@@ -25,6 +25,8 @@ subroutine crt_hip_setup(lsytop, lbeta_out)
   use electrmod, only: eanz, enr
   use datmod, only: nanz, strnr, vnr
   use wavenmod, only: kwnanz, kwn, kwnwi
+  use femmod, only: lsr
+  use elemmod, only: lsink, nsink
   use errmod, only: errnr, fetxt
   implicit none
   logical, intent(in) :: lsytop
@@ -49,9 +51,35 @@ subroutine crt_hip_setup(lsytop, lbeta_out)
   if (errnr == 0) errnr = crt_set_wavenumbers(ctx, kwnanz, kwn, kwnwi)
   lb = 0
   if (errnr == 0) errnr = crt_precal(ctx, merge(1_c_int, 0_c_int, lsytop), lb)
+  ! the singularity removal and the fictitious sink of femmod / elemmod (lsr is off in
+  ! the inversion, src/rall.f90:153; the device applies it only with mixed boundaries)
+  if (errnr == 0) errnr = crt_set_fem_switches(ctx, merge(1_c_int, 0_c_int, lsr), 0_c_int, &
+       merge(int(nsink, c_int), 0_c_int, lsink))
   lbeta_out = lb /= 0
   if (errnr /= 0) fetxt = 'crt_hip_setup (libcrtomo_hip)'
 end subroutine crt_hip_setup
+
+! bkfak (src/bkfak.f90:48-136): the half-space geometric factors into datmod's kfak on
+! the device, and tmp.kfak as the reference writes it
+subroutine bkfak()
+  use iso_c_binding
+  use crt_hip_mod
+  use datmod, only: nanz, kfak
+  use errmod, only: errnr, fetxt
+  implicit none
+  real(c_double), allocatable :: kf(:)
+  integer :: fp, i
+  allocate(kf(nanz))
+  errnr = crt_bkfak(crt_hip_ctx(), kf)
+  if (errnr /= 0) then
+     fetxt = 'bkfak (libcrtomo_hip)'
+     return
+  end if
+  kfak(1:nanz) = kf
+  open (newunit=fp, file='tmp.kfak', status='replace')
+  write (fp, '(I8,2X,G12.4)') (i, kfak(i), i = 1, nanz)
+  close (fp)
+end subroutine bkfak
 
 subroutine crt_hip_forward()
   use iso_c_binding

@@ -67,7 +67,7 @@ def test_binding_compiles_and_links():
                             "-Wl,--no-undefined"], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
         defined, undefined = _nm(so)
-        shims = {"chol_", "vre_", "choldc_", "vredc_", "cholz_", "bvolt_", "bvolti_", "bsens_", "bsensi_", "bsendc_",
+        shims = {"chol_", "vre_", "choldc_", "vredc_", "cholz_", "bvolt_", "bvolti_", "bsens_", "bsensi_", "bsendc_", "bkfak_",
                  "bnachbar_", "blam0_", "crt_hip_setup_", "crt_hip_forward_", "crt_hip_cjg_", "crt_hip_bsmatm_"}
         assert shims <= defined, shims - defined
         calls = {u for u in undefined if u.startswith("crt_")}
