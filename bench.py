@@ -37,7 +37,8 @@ KERNEL_SOURCES = {  # the source file each benched kernel is compiled from
     "crt_factor16la_kernel": "crt_band16.hip", "crt_factor16st_kernel": "crt_band16.hip",
     "crt_factor16_kernel": "crt_band16.hip", "crt_solve16_kernel": "crt_band16.hip",
     "crt_factor32st_kernel": "crt_band32.hip", "crt_factor_kernel": "crt_band.hip",
-    "crt_sens_gm_kernel": "crt_volt_sens.hip", "crt_cg_jq_kernel": "crt_cg.hip", "crt_cg_jhw_kernel": "crt_cg.hip",
+    "crt_sens_gm_kernel": "crt_volt_sens.hip", "crt_sens_gp_kernel": "crt_volt_sens.hip",
+    "crt_cg_jq_kernel": "crt_cg.hip", "crt_cg_jhw_kernel": "crt_cg.hip",
 }
 
 
@@ -174,7 +175,7 @@ def other_rooflines(g, s, K, E, nsys, phase, bits):
     out = {}
     for name, key, b in (("substitution (crt_solve16_kernel, first solve)", "solve", solve_b),
                          ("assembly (crt_diag_kernel + crt_assemble_kernel)", "diag", asm_b),
-                         ("Jacobian (crt_sens_gm_kernel)", "sens", sens_b)):
+                         ("Jacobian (crt_sens_gp_kernel)", "sens", sens_b)):
         ms = phase.get(key, 0.0)
         if ms > 0:
             gbs = b / (ms * 1e-3) / 1e9

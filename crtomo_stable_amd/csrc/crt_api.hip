@@ -81,7 +81,7 @@ int crt_launch_hpot_elec(hipStream_t, int, int, const int *, const double *, con
 int crt_launch_hpot_full(hipStream_t, int, int, int, const double *, const zd *, zd *);
 int crt_launch_volt(hipStream_t, int, int, int, const int4 *, const zd *, const double *, zd *, zd *, int *);
 int crt_launch_sens(hipStream_t, int, int, int, int, int, int, int, const int *, const int *, const int4 *,
-                    const double *, const zd *, const double *, const zd *, const zd *, const double *, zd *);
+                    const double *, const zd *, const double *, const zd *, const zd *, const double *, zd *, zd *);
 int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
 
 // CG (crt_cg.hip)
@@ -296,6 +296,7 @@ struct crt_ctx {
     // analytic solution alone, the fictitious sink (0-based node, -1 none)
     int sw_lsr = 0, sw_lana = 0, sw_nsink = -1;
     DBuf<zd> d_pota, d_sigma0;
+    DBuf<zd> d_vfac;          // per-measurement factor of the Jacobian's lookups
     DBuf<int> d_tptr, d_tq;   // per node the strictly lower CSR entries of its column
     bool tcol_built = false;
     int last_method = 0;
@@ -2063,6 +2064,7 @@ int crt_sens(crt_ctx *c, int mode, const double *strom) {
     if (c->inv_kind == 1 && mode == 0) return CRT_ERR_UNSUPPORTED;  // the DC twin is bsendc (of bsensi)
     CRT_HIP(hipSetDevice(c->device));
     int e = c->d_sens.ensure((size_t)c->nanz * c->elanz);
+    e = e ? e : c->d_vfac.ensure(c->nanz);
     if (e) return e;
     const double *ds = nullptr;
     if (strom && mode == 0) {
@@ -2072,7 +2074,7 @@ int crt_sens(crt_ctx *c, int mode, const double *strom) {
     c->tic(T_SENS);
     e = crt_launch_sens(c->stream, c->elanz, c->nanz, c->eanz, c->kwnanz, c->maxn, c->npk, mode, c->d_nrel.p,
                         c->d_nkel.p, c->d_meas.p, c->d_kwnwi.p, c->d_kpotN.p, c->d_elbg.p, c->d_sigma.p,
-                        c->d_volt.p ? c->d_volt.p : nullptr, ds, c->d_sens.p);
+                        c->d_volt.p ? c->d_volt.p : nullptr, ds, c->d_sens.p, c->d_vfac.p);
     c->toc(T_SENS);
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(c->stream));

@@ -388,6 +388,191 @@ __global__ void __launch_bounds__(SGW * 64) crt_sens_gm_kernel(SensArgs A, int K
     }
 }
 
+namespace {
+__device__ __forceinline__ zd ldz(const zd *p) {
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    return zd{v.x, v.y};
+}
+__device__ __forceinline__ void stz(zd *p, zd z) { *reinterpret_cast<double2 *>(p) = make_double2(z.re, z.im); }
+}  // namespace
+
+// potentials per thread staged in registers for the next stage (fewer where two
+// workgroups share a CU's registers)
+constexpr int sens_pf(int nt) { return nt <= 4 ? 4 : 6; }
+
+// The same G tiles, but each workgroup runs through a contiguous range of elements
+// (neighbouring elements share nodes: their potential rows stay in the XCD's L2) and
+// the potentials and element matrix of the next (element, wavenumber chunk) are loaded
+// into registers while the current one is multiplied and looked up, so the global
+// loads do not stand between the stages.  The per-measurement factor is precomputed
+// (vfac: sigma-free part of dum sigma / (pi V) for bsensi, strom / pi for bsens), so
+// the lookups multiply instead of dividing.
+template <int NT>
+__global__ void __launch_bounds__(SGW * 64, NT <= 4 ? 4 : 2) crt_sens_gp_kernel(SensArgs A, int KC,
+                                                                   const zd *__restrict__ vfac) {
+    constexpr int EP = 16 * NT;
+    constexpr int NUT = NT * (NT + 1) / 2;
+    constexpr int TPW = (NUT + SGW - 1) / SGW;
+    extern __shared__ double smem_gm[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const int K = A.K, E = A.eanz, maxn = A.maxn, npk = A.npk;
+    const int KAm = (KC * maxn + 3) & ~3;        // LDS rows of the largest element
+    zd *Phi = reinterpret_cast<zd *>(smem_gm);    // [KAc][EP]
+    zd *T = Phi + (size_t)KAm * EP;               // [KAc][EP]
+    zd *G = Phi;                                  // [NUT][256], after the last chunk
+    const size_t gz = (size_t)NUT * 256 > (size_t)2 * KAm * EP ? (size_t)NUT * 256 : (size_t)2 * KAm * EP;
+    double *S = reinterpret_cast<double *>(Phi + gz);  // [KC][npk]
+    const int nch = (K + KC - 1) / KC;
+    const int e0 = (int)((long)A.elanz * blockIdx.x / gridDim.x);
+    const int e1 = (int)((long)A.elanz * (blockIdx.x + 1) / gridDim.x);
+    const int nst = (e1 - e0) * nch;
+    constexpr int SENS_PF = sens_pf(NT);
+    const int nslot = (KAm * EP + SGW * 64 - 1) / (SGW * 64);  // <= SENS_PF (checked at launch)
+    zd pre[SENS_PF];
+    double spre = 0.0;
+    auto fetch = [&](int st) {
+        const int e = e0 + st / nch, k0 = (st % nch) * KC;
+        const int kc = min(KC, K - k0), nk = A.nkel[e];
+        const int KAc = (kc * nk + 3) & ~3;
+#pragma unroll
+        for (int j = 0; j < SENS_PF; ++j) {
+            const int t = tid + j * SGW * 64;
+            zd v = zd{0.0, 0.0};
+            if (j < nslot && t < KAc * EP) {
+                const int ka = t / EP, p = t - ka * EP;
+                const int kl = ka / nk, a = ka - kl * nk;
+                if (kl < kc && p < E) {
+                    const int node = A.nrel[(size_t)e * maxn + a] - 1;
+                    v = ldz(A.kpotN + ((size_t)node * K + k0 + kl) * E + p);
+                }
+            }
+            pre[j] = v;
+        }
+        spre = tid < kc * npk ? A.elbg[((size_t)(k0 + tid / npk) * npk + tid % npk) * A.elanz + e] : 0.0;
+    };
+    sd4 gr[TPW], gi[TPW];
+    if (nst > 0) fetch(0);
+    for (int st = 0; st < nst; ++st) {
+        const int e = e0 + st / nch, ch = st % nch, k0 = ch * KC;
+        const int kc = min(KC, K - k0), nk = A.nkel[e];
+        const int KAc = (kc * nk + 3) & ~3;
+        __syncthreads();  // the previous stage is done with Phi, T, G and S
+#pragma unroll
+        for (int j = 0; j < SENS_PF; ++j) {
+            const int t = tid + j * SGW * 64;
+            if (j < nslot && t < KAc * EP) stz(Phi + t, pre[j]);
+        }
+        if (tid < kc * npk) S[tid] = spre;
+        if (ch == 0)
+#pragma unroll
+            for (int u = 0; u < TPW; ++u) gr[u] = gi[u] = sd4{0.0, 0.0, 0.0, 0.0};
+        __syncthreads();
+        if (st + 1 < nst) fetch(st + 1);  // in flight during this stage
+        // T = w S Phi, rows (k, a)
+        for (int t = tid; t < KAc * EP; t += SGW * 64) {
+            const int ka = t / EP, q = t - ka * EP;
+            const int kl = ka / nk, a = ka - kl * nk;
+            zd acc = zd{0.0, 0.0};
+            if (kl < kc) {
+                const double w = A.kwnwi[k0 + kl];
+                for (int b = 0; b < nk; ++b) {
+                    const int hi = a > b ? a : b, lo = a > b ? b : a;
+                    const double sab = S[kl * npk + hi * (hi + 1) / 2 + lo];
+                    const zd ph = Phi[(kl * nk + b) * EP + q];
+                    acc.re = fma(sab, ph.re, acc.re);
+                    acc.im = fma(sab, ph.im, acc.im);
+                }
+                acc = zd{w * acc.re, w * acc.im};
+            }
+            stz(T + t, acc);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            if (wave + SGW * u < NUT) {
+                int t = wave + SGW * u, P = 0;
+                while (t >= NT - P) {
+                    t -= NT - P;
+                    ++P;
+                }
+                const zd *pa = Phi + rq * EP + 16 * P + jc;
+                const zd *pb = T + rq * EP + 16 * (P + t) + jc;
+                for (int ks = 0; ks < KAc; ks += 4) {
+                    const zd a = ldz(pa + ks * EP), b = ldz(pb + ks * EP);
+                    gr[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, gr[u], 0, 0, 0);
+                    gi[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, gi[u], 0, 0, 0);
+                    gr[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, gr[u], 0, 0, 0);
+                    gi[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, gi[u], 0, 0, 0);
+                }
+            }
+        }
+        if (ch != nch - 1) continue;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            const int t = wave + SGW * u;
+            if (t < NUT)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) G[t * 256 + (4 * v + rq) * 16 + jc] = zd{gr[u][v], gi[u][v]};
+        }
+        __syncthreads();
+        auto gv = [&](int p, int q) -> zd {
+            if (p <= 0 || q <= 0) return zd{0.0, 0.0};
+            const int a = min(p, q) - 1, b = max(p, q) - 1;
+            return ldz(G + sens_upper_tile(a >> 4, b >> 4, NT) * 256 + (a & 15) * 16 + (b & 15));
+        };
+        const zd sg = A.sigma[e];
+        zd *out = A.sens + (size_t)e * A.nanz;
+        // four measurements per thread and round, their loads issued together
+        constexpr int LB = 4;
+        for (int i0 = tid; i0 < A.nanz; i0 += LB * SGW * 64) {
+            int4 m[LB];
+            zd f[LB];
+#pragma unroll
+            for (int j = 0; j < LB; ++j) {
+                const int i = i0 + j * SGW * 64;
+                m[j] = i < A.nanz ? A.meas[i] : int4{0, 0, 0, 0};  // x=e1 (A), y=e2 (B), z=e3 (M), w=e4 (N)
+                f[j] = i < A.nanz ? ldz(vfac + i) : zd{0.0, 0.0};
+            }
+#pragma unroll
+            for (int j = 0; j < LB; ++j) {
+                const int i = i0 + j * SGW * 64;
+                const zd dum = zadd(zsub(gv(m[j].w, m[j].y), gv(m[j].w, m[j].x)), zsub(gv(m[j].z, m[j].x), gv(m[j].z, m[j].y)));
+                if (i < A.nanz)
+                    stz(out + i, A.mode == 1 ? zmul(zmul(dum, sg), f[j]) : zd{-(dum.re * f[j].re), -(dum.im * f[j].re)});
+            }
+        }
+    }
+}
+
+// per-measurement factor of the pipelined form: 1 / (pi V) (bsensi), strom / pi (bsens)
+__global__ void __launch_bounds__(256) crt_sens_vfac_kernel(int nanz, int mode, const zd *__restrict__ volt,
+                                                            const double *__restrict__ strom, zd *__restrict__ vfac) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nanz) return;
+    const double pi = acos(-1.0);
+    if (mode == 1) {
+        vfac[i] = zdiv(zd{1.0, 0.0}, zd{pi * volt[i].re, pi * volt[i].im});
+    } else {
+        vfac[i] = zd{(strom ? strom[i] : 1.0) / pi, 0.0};
+    }
+}
+
+template <int NT>
+static int launch_sens_gp(hipStream_t s, const SensArgs &a, int KC, size_t lds, zd *vfac) {
+    (void)hipFuncSetAttribute((const void *)crt_sens_gp_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    hipLaunchKernelGGL(crt_sens_vfac_kernel, dim3((a.nanz + 255) / 256), dim3(256), 0, s, a.nanz, a.mode, a.volt,
+                       a.strom, vfac);
+    // resident workgroups: two per CU when the LDS allows, each a contiguous element range
+    const int per_cu = lds <= 80 * 1024 ? 2 : 1;  // registers hold two (NT <= 4) when the LDS does
+    static const int mult = getenv("CRT_SENS_WG_MULT") ? std::max(1, atoi(getenv("CRT_SENS_WG_MULT"))) : 4;
+    const int nwg = std::max(1, std::min(a.elanz, 256 * per_cu * mult));
+    hipLaunchKernelGGL(crt_sens_gp_kernel<NT>, dim3(nwg), dim3(SGW * 64), lds, s, a, KC, vfac);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
 template <int NT>
 static int launch_sens_gm(hipStream_t s, const SensArgs &a, int KC, size_t lds) {
     (void)hipFuncSetAttribute((const void *)crt_sens_gm_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -434,11 +619,13 @@ size_t crt_sens_lds_bytes(int maxn, int K, int eanz) {
 
 int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int maxn, int npk, int mode,
                     const int *nrel, const int *nkel, const int4 *meas, const double *kwnwi, const zd *kpotN,
-                    const double *elbg, const zd *sigma, const zd *volt, const double *strom, zd *sens) {
+                    const double *elbg, const zd *sigma, const zd *volt, const double *strom, zd *sens,
+                    zd *vfac) {
     SensArgs a{elanz, nanz, eanz, K, maxn, npk, mode, nrel, nkel, meas, kwnwi, kpotN, elbg, sigma, volt, strom, sens};
-    static const int form = getenv("CRT_SENS_FORM") ? atoi(getenv("CRT_SENS_FORM")) : 2;
-    // electrode-pair form on the matrix cores: up to 128 electrodes
-    if (form == 2 && eanz <= 128) {
+    static const int form = getenv("CRT_SENS_FORM") ? atoi(getenv("CRT_SENS_FORM")) : 3;
+    // electrode-pair form on the matrix cores: up to 128 electrodes; 3 (default) the
+    // pipelined workgroups over element ranges, 2 one workgroup per element
+    if (form >= 2 && eanz <= 128) {
         const int nt = (eanz + 15) / 16, nut = nt * (nt + 1) / 2;
         const size_t gbytes = (size_t)nut * 256 * sizeof(zd);
         // wavenumber chunk: Phi and T rows within max(G, 80 KB) of LDS (two workgroups per CU)
@@ -452,6 +639,36 @@ int crt_launch_sens(hipStream_t s, int elanz, int nanz, int eanz, int K, int max
             if (f >= 1 && f < KC) KC = f;
         }
         const size_t lds = std::max(gbytes, (size_t)2 * ((KC * maxn + 3) & ~3) * 16 * nt * sizeof(zd));
+        // the pipelined form: chunks of equal size whose Phi and T rows and element
+        // matrices fit the same budget (C1: two chunks of 7 and 6 wavenumbers in 57 KB,
+        // so two workgroups share a CU)
+        int KP = K;
+        for (int nch = 1; nch <= K; ++nch) {
+            KP = (K + nch - 1) / nch;
+            if ((size_t)2 * ((KP * maxn + 3) & ~3) * 16 * nt * sizeof(zd) + (size_t)KP * npk * sizeof(double) <=
+                budget)
+                break;
+        }
+        if (const char *ev = getenv("CRT_SENS_KC")) {
+            const int f = atoi(ev);
+            if (f >= 1 && f < KP) KP = f;
+        }
+        const size_t ldsp = std::max(gbytes, (size_t)2 * ((KP * maxn + 3) & ~3) * 16 * nt * sizeof(zd)) +
+                            (size_t)KP * npk * sizeof(double);
+        const int nslot = (((KP * maxn + 3) & ~3) * 16 * nt + SGW * 64 - 1) / (SGW * 64);
+        if (form == 3 && vfac && ldsp <= 160 * 1024 && nslot <= sens_pf(nt)) {
+            switch (nt) {
+                case 1: return launch_sens_gp<1>(s, a, KP, ldsp, vfac);
+                case 2: return launch_sens_gp<2>(s, a, KP, ldsp, vfac);
+                case 3: return launch_sens_gp<3>(s, a, KP, ldsp, vfac);
+                case 4: return launch_sens_gp<4>(s, a, KP, ldsp, vfac);
+                case 5: return launch_sens_gp<5>(s, a, KP, ldsp, vfac);
+                case 6: return launch_sens_gp<6>(s, a, KP, ldsp, vfac);
+                case 7: return launch_sens_gp<7>(s, a, KP, ldsp, vfac);
+                case 8: return launch_sens_gp<8>(s, a, KP, ldsp, vfac);
+                default: break;
+            }
+        }
         if (lds <= 160 * 1024) {
             switch (nt) {
                 case 1: return launch_sens_gm<1>(s, a, KC, lds);
