@@ -65,6 +65,7 @@ int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, i
                           const int *, const int *, const int *, const int *, const int *, const int *, const int *,
                           const zd *, const double *, const double *, const zd *, zd *, zd *);
 int crt_cond_dense_max_tiles();
+int crt_launch_cond_s0(hipStream_t, int, int, int, int, const zd *, const zd *, zd *);
 int crt_cond_prefix_max_tiles();
 int crt_launch_cond_bval(hipStream_t, const DevGrid &, int, int, int, int, int, const int *, const int2 *,
                          const double *, zd *);
@@ -1637,6 +1638,7 @@ static int forward_condensed(crt_ctx *c) {
             e = crt_launch_cond_dense(st, g, 1, P.nG, nt, nBp, JS, E, K, k0, n, nullptr, P.T.p, P.Gr.p, P.d_bptr.p,
                                       P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p, P.Lt.p, P.Li.p,
                                       P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p, stI, nP);
+        if (JS) e = e ? e : crt_launch_cond_s0(st, n, nt, ntB, JS, P.T.p, P.Lpre.p, P.S0.p);
         e = e ? e : crt_launch_cond_bval(st, g, n * E, E, k0, nBp, P.nb, P.d_bq.p, P.d_bg.p, P.fB.p + P.nSp, P.bval.p);
         if (e) return e;
         c->toc(T_PREFIX);

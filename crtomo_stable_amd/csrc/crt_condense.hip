@@ -547,8 +547,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
         }
     }
 
-    for (int J = 0; J < nt; ++J) {
-        const int pend = PRE ? min(J, JS) : J;  // columns of L in this column's update
+    // the prefix factorises the separator columns only; S0 = T_GG - L_GS L_GS^T is formed
+    // by crt_cond_s0_kernel over every tile at once
+    for (int J = 0; J < (PRE ? JS : nt); ++J) {
+        const int pend = J;  // columns of L in this column's update
         d4 ar[NI], ai[NI];
         // S(l) entries of the owned tiles I = J + wave + NW u, column J
 #pragma unroll
@@ -624,19 +626,6 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
                     }
                 }
             }
-        }
-        if (PRE && J >= JS) {
-            // the G block's Schur complement: tile (I - JS, J - JS) of S0
-            zd *o = A.S0w + (size_t)kk * ntri * 256;
-#pragma unroll
-            for (int u = 0; u < NI; ++u) {
-                const int I = J + wave + NW * u;
-                if (I < nt)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        stz(o + tri(I - JS, J - JS) * 256 + jc * T16 + 4 * v + rq, zd{ar[u][v], ai[u][v]});
-            }
-            continue;
         }
         // diagonal tile: wave 0 factors it in LDS
         if (wave == 0) {
@@ -821,6 +810,63 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             A.kpotN[((size_t)nd * A.K + k) * A.E + l] = zd{x.re * f, x.im * f};
         }
     }
+}
+
+// ------------------------------------------------ S0 of the prefix
+// S0 = T_GG - L_GS L_GS^T per wavenumber, one wave per tile (I, J), I >= J, of the G
+// block: the T tile minus its JS products L(JS + I, p) L(JS + J, p)^T over the separator
+// columns the prefix factorised (tiles in the packed column-major layout of the dense
+// kernels, operands in the MFMA lanes as there).
+struct S0Args {
+    int kc, nt, ntB, JS;
+    const zd *T, *Lpre;  // [kk][B tiles]
+    zd *S0w;             // [kk][G tiles]
+};
+
+__global__ void __launch_bounds__(256) crt_cond_s0_kernel(S0Args A) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const long ntri = (long)A.nt * (A.nt + 1) / 2, ntriB = (long)A.ntB * (A.ntB + 1) / 2;
+    const long w = (long)blockIdx.x * 4 + wave;
+    if (w >= A.kc * ntri) return;
+    const int kk = (int)(w / ntri);
+    int t = (int)(w - kk * ntri), I = 0;
+    while (t > I) {  // tile t = I (I + 1) / 2 + J
+        t -= I + 1;
+        ++I;
+    }
+    const int J = t;
+    auto tri = [](long a, long b) { return a * (a + 1) / 2 + b; };
+    const zd *Tk = A.T + kk * ntriB * 256, *Lk = A.Lpre + kk * ntriB * 256;
+    const zd *src = Tk + tri(I + A.JS, J + A.JS) * 256;
+    d4 ar, ai;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const zd e = ldz(src + jc * T16 + 4 * v + rq);
+        ar[v] = e.re;
+        ai[v] = e.im;
+    }
+    const zd *la = Lk + tri(I + A.JS, 0) * 256, *lb = Lk + tri(J + A.JS, 0) * 256;
+    for (int p = 0; p < A.JS; ++p) {
+        zd a4[4], b4[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            a4[ks] = ldz(la + (size_t)p * 256 + 64 * ks + lane);
+            b4[ks] = ldz(lb + (size_t)p * 256 + 64 * ks + lane);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) cmma(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
+    }
+    zd *o = A.S0w + kk * ntri * 256 + tri(I, J) * 256;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) stz(o + jc * T16 + 4 * v + rq, zd{ar[v], ai[v]});
+}
+
+int crt_launch_cond_s0(hipStream_t st, int kc, int nt, int ntB, int JS, const zd *T, const zd *Lpre, zd *S0w) {
+    S0Args a{kc, nt, ntB, JS, T, Lpre, S0w};
+    const long n = (long)kc * nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(crt_cond_s0_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 // ------------------------------------------------ interior potentials
