@@ -66,6 +66,7 @@ int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, i
                           const zd *, const double *, const double *, const zd *, zd *, zd *);
 int crt_cond_dense_max_tiles();
 int crt_launch_cond_s0(hipStream_t, int, int, int, int, const zd *, const zd *, zd *);
+int crt_launch_cond_lgs(hipStream_t, int, int, int, const zd *, zd *, const zd *);
 int crt_cond_prefix_max_tiles();
 int crt_launch_cond_bval(hipStream_t, const DevGrid &, int, int, int, int, int, const int *, const int2 *,
                          const double *, zd *);
@@ -304,6 +305,8 @@ struct crt_ctx {
     // static condensation plan of the grid (boundary nodes G, interior nodes I)
     struct Cond {
         bool built = false, ok = false;
+        bool tuned = false;                           // pieces chosen by timing (first forward)
+        int force_np = 0;                             // pieces the plan starts from (0: 8)
         int P = 1;                                    // interior band pieces
         int nI = 0, nG = 0, nS = 0, nSp = 0, nGp = 0, nBp = 0, JS = 0, nt = 0, ntB = 0;
         int mbI = 0, DI = 0, nblkP = 0, nnzI = 0, E = 0, ncolP = 0, ntileP = 0, nent = 0, order = 0, nb = 0;
@@ -701,7 +704,8 @@ int crt_set_grid(crt_ctx *c, int sanz, int mb, int typanz, const int *typ, const
     if (!c || sanz <= 0 || mb < 0 || typanz <= 0 || maxn <= 0 || maxn > 4) return CRT_ERR_ARG;
     CRT_HIP(hipSetDevice(c->device));
     c->have_grid = c->have_precal = c->have_kpot = c->have_volt = c->have_sens = false;
-    c->cond.built = c->cond.ok = false;
+    c->cond.built = c->cond.ok = c->cond.tuned = false;
+    c->cond.force_np = 0;
     c->tcol_built = false;
     c->sw_lsr = c->sw_lana = 0;
     c->sw_nsink = -1;
@@ -1246,18 +1250,19 @@ static int cond_plan(crt_ctx *c) {
     P.DI = std::max(1, (bestw + 15) / 16);
     if (P.DI < 2 || P.DI > 16) return 0;
     // pieces: [start, start + len) of the interior positions, separators of width mbI between.
-    // The most pieces (up to 8) of at least 32 band widths each whose separators and
-    // boundary nodes fit the prefix kernel: the pieces' factorisations run side by side,
-    // each a chain of block steps as long as the piece, while the separators' elimination
-    // grows with their count (C1: 2 pieces 8.1 ms per step, 3 pieces 7.3, 4 pieces 7.6).
-    int np = 8, minlen = 32;
+    // The most pieces (up to force_np, else 8) of at least 32 band widths each whose
+    // separators fit the prefix kernel: the pieces' factorisations run side by side, each a
+    // chain of block steps as long as the piece, while the separators' elimination grows
+    // with their count; the first condensed forward of a grid times the candidates
+    // (crt_forward) and keeps the fastest.
+    int np = P.force_np > 0 ? P.force_np : 8, minlen = 32;
     if (const char *ev = getenv("CRT_COND_PIECES")) np = std::max(1, atoi(ev));
     if (const char *ev = getenv("CRT_COND_MINLEN")) minlen = std::max(1, atoi(ev));
     const int w = bestw;
     for (; np > 1; --np) {
         const int L = (nI0 - (np - 1) * w) / np;
-        const int nS = (np - 1) * w, ntB = ((nS + 15) / 16 * 16 + P.nGp) / 16;
-        if (L >= minlen * w && ntB <= crt_cond_prefix_max_tiles()) break;
+        const int nS = (np - 1) * w;
+        if (L >= minlen * w && (nS + 15) / 16 <= crt_cond_prefix_max_tiles()) break;
     }
     std::vector<int> pst, plen;
     {
@@ -1638,6 +1643,7 @@ static int forward_condensed(crt_ctx *c) {
             e = crt_launch_cond_dense(st, g, 1, P.nG, nt, nBp, JS, E, K, k0, n, nullptr, P.T.p, P.Gr.p, P.d_bptr.p,
                                       P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p, P.Lt.p, P.Li.p,
                                       P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p, stI, nP);
+        if (JS) e = e ? e : crt_launch_cond_lgs(st, n, ntB, JS, P.T.p, P.Lpre.p, P.Lipre.p);
         if (JS) e = e ? e : crt_launch_cond_s0(st, n, nt, ntB, JS, P.T.p, P.Lpre.p, P.S0.p);
         e = e ? e : crt_launch_cond_bval(st, g, n * E, E, k0, nBp, P.nb, P.d_bq.p, P.d_bg.p, P.fB.p + P.nSp, P.bval.p);
         if (e) return e;
@@ -1677,6 +1683,40 @@ static int forward_condensed(crt_ctx *c) {
             if (sD[(size_t)k * E + l]) return sD[(size_t)k * E + l];
     }
     return 0;
+}
+
+// The interior pieces of the condensed forward, chosen by timing one forward of each
+// candidate count on the grid's first condensed forward (the latency of the pieces'
+// factorisations against the separators' elimination and the longer boundary
+// substitutions: C1 3 or 4, C3 6 of 2..8).  CRT_COND_PIECES fixes the count instead,
+// CRT_COND_TUNE=0 keeps the plan's own choice.
+static int tune_pieces(crt_ctx *c) {
+    auto &P = c->cond;
+    if (P.tuned) return 0;
+    P.tuned = true;
+    const char *tv = getenv("CRT_COND_TUNE");
+    if (getenv("CRT_COND_PIECES") || (tv && atoi(tv) == 0)) return 0;
+    int best = P.P;
+    double best_ms = 0.0;
+    std::vector<int> seen;
+    for (int cand : {2, 3, 4, 5, 6, 8}) {
+        P.built = false;
+        P.force_np = cand;
+        if (int e = cond_plan(c)) return e;
+        if (!P.ok) return 0;
+        if (std::find(seen.begin(), seen.end(), P.P) != seen.end()) continue;
+        seen.push_back(P.P);
+        if (int e = forward_condensed(c)) return e;
+        double ms = 0.0;
+        for (int t : {T_DIAG, T_FACTOR, T_SOLVE, T_PREFIX, T_SCHUR, T_PRODUCT}) ms += c->times[t];
+        if (best_ms == 0.0 || ms < best_ms) {
+            best_ms = ms;
+            best = P.P;
+        }
+    }
+    P.built = false;
+    P.force_np = best;
+    return cond_plan(c);
 }
 
 int crt_set_forward_method(crt_ctx *c, int method) {
@@ -1805,6 +1845,7 @@ int crt_forward(crt_ctx *c) {
         c->refine_fallbacks = 0;
         c->last_method = 1;
         c->have_kpot = false;
+        if (int e0 = tune_pieces(c)) return e0;
         if (int e0 = forward_condensed(c)) return e0;
         c->have_kpot = true;
         c->have_volt = c->have_sens = false;

@@ -520,7 +520,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jc = lane & 15, rq = lane >> 4;
     const int JS = A.JS, ntB = A.nBp / T16, nG = A.nG;
-    const int nt = PRE ? ntB : A.nt;  // tile columns of the matrix factorised here
+    // tile rows and columns of the matrix factorised here: the prefix factorises the
+    // separator block T_SS alone (L_GS and S0 follow in crt_cond_lgs_kernel and
+    // crt_cond_s0_kernel, row by row and tile by tile)
+    const int nt = PRE ? JS : A.nt;
     const size_t ntri = (size_t)A.nt * (A.nt + 1) / 2, ntriB = (size_t)ntB * (ntB + 1) / 2;
     zd *Ls = PRE ? A.Lpre + (size_t)kk * ntriB * 256 : A.Lt + (size_t)s * ntri * 256;
     zd *Lis = PRE ? A.Lipre + (size_t)kk * JS * 256 : A.Li + (size_t)s * A.nt * 256;
@@ -812,6 +815,76 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
     }
 }
 
+// ------------------------------------------------ L_GS of the prefix
+// L(I, p) = (T(I, p) - sum_{q < p} L(I, q) L(p, q)^T) L(p, p)^-T for the tile rows I of
+// the G block (JS <= I < ntB) and the separator columns p < JS, one wave per (wavenumber,
+// row): the separator block's factor L(p, q) and the inverses of its diagonal tiles come
+// from the prefix kernel; the row's own earlier tiles are read back from global memory
+// after the wave's stores are complete.
+struct LgsArgs {
+    int kc, ntB, JS;
+    const zd *T;          // [kk][B tiles]
+    zd *Lpre;             // [kk][B tiles]
+    const zd *Lipre;      // [kk][JS]
+};
+
+__global__ void __launch_bounds__(256) crt_cond_lgs_kernel(LgsArgs A) {
+    __shared__ zd ws[4][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jc = lane & 15, rq = lane >> 4;
+    const int nr = A.ntB - A.JS;
+    const long w = (long)blockIdx.x * 4 + wave;
+    if (w >= (long)A.kc * nr) return;
+    const int kk = (int)(w / nr), I = A.JS + (int)(w - (long)kk * nr);
+    const long ntriB = (long)A.ntB * (A.ntB + 1) / 2;
+    auto tri = [](long a, long b) { return a * (a + 1) / 2 + b; };
+    const zd *Tk = A.T + kk * ntriB * 256, *Lik = A.Lipre + (long)kk * A.JS * 256;
+    zd *Lk = A.Lpre + kk * ntriB * 256;
+    zd *wsm = ws[wave];
+    for (int p = 0; p < A.JS; ++p) {
+        d4 ar, ai;
+        const zd *src = Tk + tri(I, p) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const zd e = ldz(src + jc * T16 + 4 * v + rq);
+            ar[v] = e.re;
+            ai[v] = e.im;
+        }
+        for (int q = 0; q < p; ++q) {
+            const zd *la = Lk + tri(I, q) * 256, *lb = Lk + tri(p, q) * 256;
+            zd a4[4], b4[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                a4[ks] = ldz(la + 64 * ks + lane);
+                b4[ks] = ldz(lb + 64 * ks + lane);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) cmma(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
+        }
+        // X = C L(p, p)^-T with C staged column-major in the wave's LDS tile
+#pragma unroll
+        for (int v = 0; v < 4; ++v) wsm[jc * T16 + 4 * v + rq] = zd{ar[v], ai[v]};
+        __builtin_amdgcn_wave_barrier();
+        d4 xr = {0.0, 0.0, 0.0, 0.0}, xi = xr;
+        const zd *li = Lik + (long)p * 256;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) cmma(xr, xi, wsm[64 * ks + lane], ldz(li + 64 * ks + lane));
+        __builtin_amdgcn_wave_barrier();
+        zd *dst = Lk + tri(I, p) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) stz(dst + jc * T16 + 4 * v + rq, zd{xr[v], xi[v]});
+        __threadfence_block();  // the row's later columns read this tile back
+    }
+}
+
+int crt_launch_cond_lgs(hipStream_t st, int kc, int ntB, int JS, const zd *T, zd *Lpre, const zd *Lipre) {
+    LgsArgs a{kc, ntB, JS, T, Lpre, Lipre};
+    const long n = (long)kc * (ntB - JS);
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(crt_cond_lgs_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
 // ------------------------------------------------ S0 of the prefix
 // S0 = T_GG - L_GS L_GS^T per wavenumber, one wave per tile (I, J), I >= J, of the G
 // block: the T tile minus its JS products L(JS + I, p) L(JS + J, p)^T over the separator
@@ -1008,8 +1081,8 @@ int crt_launch_cond_schur(hipStream_t st, int kc, int k0, int nBp, int E, int ld
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
-// largest boundary tile count of the source systems and of B in the prefix (five and
-// six owned tiles per wave)
+// largest boundary tile count of the source systems and separator tile count of the
+// prefix (five and six owned tiles per wave)
 int crt_cond_dense_max_tiles() { return 40; }
 int crt_cond_prefix_max_tiles() { return 48; }
 
@@ -1036,11 +1109,11 @@ int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int pre, int nG, int
                           zd *Li, zd *Lpre, zd *Lipre, zd *S0w, zd *XB, zd *kpotN, int *status, int sstride) {
     const int ntB = nBp / T16;
     if (nt < 1 || nt > crt_cond_dense_max_tiles() || nBp % T16 || ntB != nt + JS) return CRT_ERR_ARG;
-    if (pre && (JS < 1 || ntB > crt_cond_prefix_max_tiles())) return CRT_ERR_ARG;
+    if (pre && (JS < 1 || JS > crt_cond_prefix_max_tiles())) return CRT_ERR_ARG;
     DenseArgs a{g,     nG,     nt,   nBp,  JS,   E,     K,     k0,   S0,  T,       Gr,      bptr,  bent,
                 qdiag, nodeB, bval, nb,   fB,   Lt,    Li,    Lpre, Lipre, S0w,   XB,      kpotN, status, sstride};
     if (pre) {
-        switch ((ntB + DNW - 1) / DNW) {
+        switch ((JS + DNW - 1) / DNW) {
             case 1: return launch_dense<1, true>(st, a, nsys);
             case 2: return launch_dense<2, true>(st, a, nsys);
             case 3: return launch_dense<3, true>(st, a, nsys);
