@@ -1055,7 +1055,7 @@ int crt_set_sigma(crt_ctx *c, const double *sigma) {
     if (e) return e;
     CRT_HIP(hipStreamSynchronize(c->stream));
     c->have_sigma = true;
-    c->have_kpot = c->have_volt = c->have_sens = false;
+    c->have_kpot = c->have_volt = false;
     return 0;
 }
 
@@ -1788,7 +1788,7 @@ static int forward_analytic(crt_ctx *c) {
     }
     for (int t : {T_DIAG, T_FACTOR, T_SOLVE, T_SCHUR, T_PRODUCT, T_PREFIX, T_REFINE}) c->times[t] = 0.0;
     c->have_kpot = true;
-    c->have_volt = c->have_sens = false;
+    c->have_volt = false;
     return 0;
 }
 
@@ -1817,7 +1817,7 @@ int crt_set_fem_switches(crt_ctx *c, int lsr, int lana, int nsink) {
     c->sw_lsr = lsr ? 1 : 0;
     c->sw_lana = lana ? 1 : 0;
     c->sw_nsink = nsink - 1;
-    c->have_kpot = c->have_volt = c->have_sens = false;
+    c->have_kpot = c->have_volt = false;
     return 0;
 }
 
@@ -1848,7 +1848,7 @@ int crt_forward(crt_ctx *c) {
         if (int e0 = tune_pieces(c)) return e0;
         if (int e0 = forward_condensed(c)) return e0;
         c->have_kpot = true;
-        c->have_volt = c->have_sens = false;
+        c->have_volt = false;
         return 0;
     }
     c->last_method = 0;
@@ -2041,7 +2041,7 @@ int crt_forward(crt_ctx *c) {
     if (worst) return worst;
     if ((e = zero_cleared_sources(c))) return e;
     c->have_kpot = true;
-    c->have_volt = c->have_sens = false;
+    c->have_volt = false;
     return 0;
 }
 

@@ -25,7 +25,12 @@ EXPORTS = (
 ERRORS = {
     18: "unknown element type", 19: "band width exceeded", 26: "element geometry (det <= 0)",
     27: "zero diagonal", 28: "non-positive pivot in Cholesky", 35: "boundary geometry in beta",
-    73: "duplicate electrodes", 82: "zero voltage", 93: "vanishing geometric factor", 97: "allocation failed",
+    46: "electrode number out of range", 50: "model cells differ from grid cells",
+    61: "max. iterations out of range", 73: "duplicate electrodes", 82: "zero voltage", 88: "non-positive data error",
+    89: "regular grid needs nx, nz >= 2", 91: "invalid background model", 95: "phase beyond 1000 pi",
+    99: "no valid data", 100: "no starting model", 102: "no mixed boundary and no sink", 104: "invalid error model",
+    105: "invalid phase error model", 106: "mixed boundary in 2D", 108: "factorisation rejected ata_reg",
+    110: "mixed boundary elements in 2D", 93: "vanishing geometric factor", 97: "allocation failed",
     900: "HIP runtime error",
     901: "call order (missing set-up call)", 902: "invalid argument", 903: "unsupported configuration",
 }
@@ -96,6 +101,37 @@ def write_volt(path: str, strnr, vnr, sigmaa):
     st = np.ascontiguousarray(strnr, np.int32); vn = np.ascontiguousarray(vnr, np.int32)
     sa = np.ascontiguousarray(sigmaa, np.complex128)
     _chk(lib.crt_write_volt(path.encode(), int(st.size), _ptr(st), _ptr(vn), _ptr(sa)), "crt_write_volt")
+
+
+class CrtomoCfg(C.Structure):
+    """crt_crtomo_cfg (include/crtomo_hip.h): crtomo.cfg as src/rall.f90:141-593 reads it."""
+    _P = C.c_char * 81
+    _fields_ = [("mswitch", C.c_int), ("elem", _P), ("elec", _P), ("volt", _P), ("ramd", _P), ("ldiff", C.c_int),
+                ("dd0", _P), ("dm0", _P), ("dfm0", _P), ("dstart", _P), ("iseedpri", C.c_int),
+                ("modl_stdn", C.c_double), ("nx", C.c_int), ("nz", C.c_int), ("alfx", C.c_double),
+                ("alfz", C.c_double), ("itmax", C.c_int), ("ldc", C.c_int), ("lrobust", C.c_int),
+                ("lfphai", C.c_int), ("stabw0", C.c_double), ("stabm0", C.c_double), ("stabpA1", C.c_double),
+                ("stabpB", C.c_double), ("stabpA2", C.c_double), ("stabp0", C.c_double), ("lrho0", C.c_int),
+                ("bet0", C.c_double), ("pha0", C.c_double), ("lagain", C.c_int), ("swrtr", C.c_int),
+                ("lsink", C.c_int), ("nsink", C.c_int), ("lrandb2", C.c_int), ("drandb", _P), ("ltri", C.c_int),
+                ("llamf", C.c_int), ("lamfix", C.c_double), ("betamgs", C.c_double), ("iseed", C.c_int),
+                ("lffhom", C.c_int), ("lnse", C.c_int), ("lnse2", C.c_int), ("lnsepri", C.c_int),
+                ("lstart", C.c_int), ("lprior", C.c_int), ("lsens", C.c_int), ("lcov1", C.c_int), ("lres", C.c_int),
+                ("lcov2", C.c_int), ("lgauss", C.c_int), ("lelerr", C.c_int), ("lphi0", C.c_int),
+                ("lsytop", C.c_int), ("lvario", C.c_int), ("lverb", C.c_int)]
+
+
+def read_crtomo_cfg(cfg_path: str) -> dict:
+    """crtomo.cfg read by the library's host code (``src/rall.f90:141-593``); raises
+    CRTError with the reference's errnr for a file the reference rejects."""
+    lib = load()
+    c = CrtomoCfg()
+    _chk(lib.crt_read_crtomo_cfg(cfg_path.encode(), C.byref(c)), "crt_read_crtomo_cfg")
+    out = {}
+    for name, _ in CrtomoCfg._fields_:
+        v = getattr(c, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
 
 
 BESP_NAMES = ("esp_min", "esp_max", "esp_mit", "esp_med", "esp_std", "grid_min", "grid_max", "grid_minx",
@@ -356,6 +392,15 @@ class Context:
         """Run a CRMod tomodir from its crmod.cfg (grid, electrodes, rho, config read by
         the library) and write volt.dat when the measurements switch is set."""
         _chk(self.lib.crt_tomodir_crmod(self._h, cfg_path.encode()), "crt_tomodir_crmod")
+
+    def tomodir_crtomo(self, cfg_path: str):
+        """Run a CRTomo tomodir (the inversion) from its crtomo.cfg; returns the last
+        iteration index and its data RMS."""
+        it = C.c_int()
+        rms = C.c_double()
+        self.lib.crt_tomodir_crtomo.argtypes = [_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        _chk(self.lib.crt_tomodir_crtomo(self._h, cfg_path.encode(), C.byref(it), C.byref(rms)), "crt_tomodir_crtomo")
+        return it.value, rms.value
 
     def set_forward_precision(self, bits: int, tol: float = 1e-14, max_sweeps: int = 12):
         """64: FP64 band factorisation (default); 32: FP32 factorisation on the matrix

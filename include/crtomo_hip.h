@@ -124,7 +124,10 @@ int crt_voltages(crt_ctx *ctx, int mode, const double *kfak, double *volt, doubl
 int crt_get_hpot(crt_ctx *ctx, double *hpot);
 
 /* Sensitivities: bsensi (mode 1, src/bsensi.f90:23) or bsens (mode 0,
- * src/bsens.f90:23, strom may be NULL = 1).  J stays on the device. */
+ * src/bsens.f90:23, strom may be NULL = 1).  J stays on the device until the next
+ * crt_sens / crt_set_sens or a new grid, survey, boundary values, forward switches or
+ * inversion kind; forwards of other models keep it, as the reference's lambda search
+ * updates with the Jacobian of the iteration's start model (src/inv.f90:704-736). */
 int crt_sens(crt_ctx *ctx, int mode, const double *strom);
 int crt_get_sens(crt_ctx *ctx, double *sens);
 
@@ -259,6 +262,53 @@ int crt_tomodir_sizes(const char *cfg_path, int *sanz, int *mb, int *elanz, int 
  * and 1000 * phase; a phase near +-pi is shifted by pi and swaps the potential
  * electrodes of vnr.  Host only. */
 int crt_write_volt(const char *path, int nanz, const int *strnr, const int *vnr, const double *sigmaa);
+
+/* crtomo.cfg as src/rall.f90:141-593 reads it: defaults, '#' comment lines
+ * (src/get_unit.f90 read_comments), paths cut at their first blank (clear_string), the
+ * optional lines with their fallbacks (mswitch, seed/variance of the prior, ltri read
+ * as I2 with the fixed-lambda bits 32/64 and its value, the MGS beta, the seed of coupled
+ * noise), the switches derived from mswitch and the validation errors (61 itmax, 89
+ * regular grid, 91 background, 104 error model, 105 phase error model).  Logical
+ * switches are 0/1; paths hold at most 80 characters. */
+typedef struct crt_crtomo_cfg {
+    int mswitch;
+    char elem[81], elec[81], volt[81], ramd[81];
+    int ldiff;
+    char dd0[81], dm0[81], dfm0[81], dstart[81];
+    int iseedpri;
+    double modl_stdn;
+    int nx, nz;
+    double alfx, alfz;
+    int itmax, ldc, lrobust, lfphai;
+    double stabw0, stabm0, stabpA1, stabpB, stabpA2, stabp0;
+    int lrho0;
+    double bet0, pha0;
+    int lagain, swrtr, lsink, nsink, lrandb2;
+    char drandb[81];
+    int ltri, llamf;
+    double lamfix, betamgs;
+    int iseed;
+    int lffhom, lnse, lnse2, lnsepri, lstart, lprior;
+    int lsens, lcov1, lres, lcov2, lgauss, lelerr, lphi0, lsytop, lvario, lverb;
+} crt_crtomo_cfg;
+int crt_read_crtomo_cfg(const char *cfg_path, crt_crtomo_cfg *cfg);
+
+/* CRTomo tomodir without the Fortran host (the inversion of src/inv.f90:136-1100):
+ * crtomo.cfg (crt_read_crtomo_cfg; relative paths from its directory, where crt.lamnull
+ * and inv.lastmod are read and written as the reference does in its working directory),
+ * elem.dat, elec.dat and the measurements with their error model (src/rdati.f90); the
+ * starting model (src/bsigm0.f90); then the Gauss-Newton loop with the reference's
+ * polarity check, misfit, convergence tests, lambda search, step-length control,
+ * parabolic step fit and final phase improvement, every forward, Jacobian,
+ * regularisation, blam0 and update through this library; writes inv.ctr, run.ctr,
+ * eps.ctr, cjg.ctr, rhoNN.mag/.modl/.pha/.sig and voltNN.dat per iteration
+ * (src/wout.f90), coverage.mag (src/bbsens.f90) and, with mswitch bits 1-3, the .diag
+ * files of buncert.  Refused with 903: another dataset (lagain), data noise (a negative
+ * error level or crt.noisemod), difference and prior inversions, robust inversion,
+ * variograms, regularisations other than ltri 0, 1, 2, decouplings.dat and
+ * electrode_capacitances.dat.  iterations / final_rms (may be NULL) receive the last
+ * iteration index and its data RMS. */
+int crt_tomodir_crtomo(crt_ctx *ctx, const char *cfg_path, int *iterations, double *final_rms);
 
 /* Cuthill-McKee renumbering (cutmckee/cutmck.f:126-337): perm[old-1] = new
  * number; returns the band width in *mb. */
