@@ -602,6 +602,34 @@ def bench_c2(args):
         "tune_ms": {int(D): [round(float(x), 3) for x in tune[D]] for D in range(17) if tune[D].any()},
         "setup_s": round(t1 - t0, 1),
     }
+    # the mixed-precision variant (FP32 factorisation on the matrix cores, FP64 refinement
+    # in the graph) on the first 32 models, against the FP64 batch of the same models
+    b.close()
+    nm = min(32, len(ctxs))
+    sub = ctxs[:nm]
+    nsub = sum(c.kwnanz * w.grid.eanz for c, w in zip(sub, works[:nm]))
+    try:
+        res = {}
+        for label, mixed in (("fp64", False), ("mixed", True)):
+            bx = lib.Batch(sub, mixed=mixed)
+            for it in range(1 + args.steps):
+                print(f"[c2] {label} batch of {nm} models: launch {it}", file=sys.stderr, flush=True)
+                t2 = time.perf_counter()
+                bx.forward()
+                device_sync()
+                ms = 1e3 * (time.perf_counter() - t2)
+            res[label] = (ms, bx.info()[0], bx.refine_info() if mixed else None,
+                          [c.voltages(1)[0] for c in sub])
+            bx.close()
+        (ms64, dev64, _, v64), (msm, devm, (sweeps, ratio, fallbacks), vm) = res["fp64"], res["mixed"]
+        err = max(float(np.abs(a / r - 1).max()) for a, r in zip(vm, v64))
+        out["mixed"] = dict(models=nm, systems=nsub, ms_per_launch=round(msm, 3), graph_ms=round(devm, 3),
+                            fp64_ms_per_launch=round(ms64, 3), speedup_vs_fp64=round(ms64 / msm, 3),
+                            value=round(nsub * 1e3 / msm, 1), unit="solves/s", max_sweeps=sweeps,
+                            max_accepted_ratio=ratio, models_redone_fp64=fallbacks,
+                            max_rel_voltage_err_vs_fp64=err)
+    except Exception as e:
+        out["mixed"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             w0 = works[0]
@@ -620,7 +648,6 @@ def bench_c2(args):
         except Exception as e:
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
-    b.close()
     for c in ctxs:
         c.close()
 

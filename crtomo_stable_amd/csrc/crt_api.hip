@@ -40,9 +40,12 @@ int crt_factor32_supported(int mb);
 int crt_launch_factor32(hipStream_t, const DevGrid &, int, const int2 *, const double *, const zd *, zf *, zf *, zf *,
                         int *);
 int crt_launch_refine_resid(hipStream_t, int, int, int, int, const int4 *, const int *, const int *, const int2 *,
-                            const double *, const zd *, const zd *, zd *);
+                            const double *, const zd *, const zd *, zd *, const int *skip = nullptr);
 int crt_launch_solve16_f32(hipStream_t, int, int, int, int, int, int, const int4 *, const int *, const double *,
-                           const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int);
+                           const zf *, const zf *, const zf *, zd *, zd *, zd *, double *, int,
+                           const int *skip = nullptr);
+int crt_launch_refine_init(hipStream_t, int *, double *);
+int crt_launch_refine_check(hipStream_t, int, const double *, double, double, int *, double *);
 // general right-hand sides: singularity removal, analytic solution, fictitious sink
 int crt_launch_refsig(hipStream_t, int, int, const int *, const int *, const double *, const double *, const zd *,
                       zd *);
@@ -468,6 +471,14 @@ struct crt_batch {
     struct Model {
         crt_ctx *c = nullptr;
         bool use16 = true;
+        // mixed precision (flags bit 1): FP32 factorisation on the matrix cores, refine_max
+        // FP64 refinement sweeps per chunk in the graph, each skipped once the chunk's
+        // device check has accepted it; cstate/crat per chunk (crt_refine_check_kernel)
+        bool mixed = false;
+        DBuf<int> cstate;
+        DBuf<double> crat;
+        size_t max_nr = 0;
+        int fallback = 0;  // the last launch redid this model in FP64
         int nblk = 0, D = 0, NBt = 16, variant = 0;
         size_t fstride = 0, panel_elems = 0, nsys = 0, nrhs = 0, batch = 0, nyscr = 0;
         std::vector<Chunk> chunks;
@@ -484,14 +495,18 @@ struct crt_batch {
         hipEvent_t join = nullptr;
         zd *panels = nullptr, *yf = nullptr, *dinv = nullptr, *vals = nullptr, *yscr = nullptr;
         double *fak = nullptr, *dga = nullptr;
+        zd *xacc = nullptr;        // mixed: FP64 potentials per right-hand side of a chunk
+        double *rstat = nullptr;   // mixed: |fak d|, |fak x| per right-hand side
     };
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr;
     std::vector<std::unique_ptr<Model>> m;
     std::vector<Slot> slots;
-    DBuf<zd> panels, yf, dinv, vals, yscr;
-    DBuf<double> fak, dga;
+    DBuf<zd> panels, yf, dinv, vals, yscr, xacc;
+    DBuf<double> fak, dga, rstat;
+    int mixed_sweeps = 0, mixed_fallbacks = 0;  // last launch: most sweeps of a chunk, models redone
+    double mixed_ratio = 0.0;                   // last launch: largest accepted ratio
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     float ms = 0.f;
@@ -551,6 +566,9 @@ int batch_plan(crt_batch::Model &M, size_t ws_bytes) {
         if (res > 0 && M.batch >= res) M.batch = M.batch / res * res;
     }
     M.nyscr = c->lbeta ? 1 : M.nrhs;
+    if (M.mixed) {
+        if (!M.use16 || !crt_factor32_supported(c->mb) || c->tmapD != M.D) return CRT_ERR_UNSUPPORTED;
+    }
     std::vector<int4> rall;
     M.chunks.clear();
     for (size_t s0 = 0; s0 < M.nsys; s0 += M.batch) {
@@ -559,7 +577,14 @@ int batch_plan(crt_batch::Model &M, size_t ws_bytes) {
         for (auto q : rhs)
             if ((size_t)q.z >= s0 && (size_t)q.z < s0 + nb) rall.push_back(int4{q.x, q.y, q.z - (int)s0, q.w});
         ch.nr = (int)(rall.size() - ch.r0);
+        M.max_nr = std::max(M.max_nr, (size_t)ch.nr);
         M.chunks.push_back(ch);
+    }
+    if (M.mixed) {
+        M.nyscr = std::max(M.nyscr, M.max_nr);
+        int e = M.cstate.ensure(2 * M.chunks.size());
+        e = e ? e : M.crat.ensure(2 * M.chunks.size());
+        if (e) return e;
     }
     int e = M.sys.upload(sys.data(), sys.size(), c->stream);
     e = e ? e : M.rhs.upload(rall.data(), rall.size(), c->stream);
@@ -598,6 +623,15 @@ int batch_assemble(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch
 int batch_factor(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch, int variant) {
     const DevGrid g = batch_grid(M);
     const crt_batch::Slot &w = b->slots[M.slot];
+    if (M.mixed) {
+        int e = crt_launch_factor32(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.vals, reinterpret_cast<zf *>(w.panels),
+                                    reinterpret_cast<zf *>(w.yf), reinterpret_cast<zf *>(w.dinv),
+                                    M.status.p + 1 + ch.s0);
+        if (!e && g.dc)
+            e = crt_launch_dc_pivot32(w.st, ch.nb, M.c->sanz, g.fstride, reinterpret_cast<zf *>(w.dinv),
+                                      M.status.p + 1 + ch.s0);
+        return e;
+    }
     if (M.use16) {
         int e = crt_launch_factor16(w.st, g, ch.nb, M.sys.p + ch.s0, w.fak, w.vals, w.panels, w.yf, w.dinv,
                                     M.status.p + 1 + ch.s0, variant);
@@ -612,6 +646,27 @@ int batch_solve(crt_batch *b, crt_batch::Model &M, const crt_batch::Chunk &ch) {
     crt_ctx *c = M.c;
     const int K = c->kwnanz, E = c->eanz;
     const crt_batch::Slot &w = b->slots[M.slot];
+    if (M.mixed) {
+        // FP32 substitution, then refine_max sweeps (FP64 residual, FP32 correction), each
+        // skipped once the chunk's check has accepted it
+        const int ci = (int)(&ch - M.chunks.data());
+        int *state = M.cstate.p + 2 * ci;
+        double *rat = M.crat.p + 2 * ci;
+        const zf *pan = reinterpret_cast<const zf *>(w.panels), *di = reinterpret_cast<const zf *>(w.dinv),
+                 *y32 = reinterpret_cast<const zf *>(w.yf);
+        const DevGrid g = batch_grid(M);
+        int e = crt_launch_refine_init(w.st, state, rat);
+        e = e ? e : crt_launch_solve16_f32(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak,
+                                           pan, di, y32, w.yscr, c->d_kpotN.p, w.xacc, w.rstat, 1);
+        for (int sw = 0; sw < c->refine_max && !e; ++sw) {
+            e = crt_launch_refine_resid(w.st, c->sanz, g.fstride, g.vstride, ch.nr, M.rhs.p + ch.r0, c->d_enr.p,
+                                        c->d_frow.p, c->d_fent.p, w.fak, w.vals, w.xacc, w.yscr, state);
+            e = e ? e : crt_launch_solve16_f32(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p,
+                                               w.fak, pan, di, y32, w.yscr, c->d_kpotN.p, w.xacc, w.rstat, 2, state);
+            e = e ? e : crt_launch_refine_check(w.st, ch.nr, w.rstat, c->refine_tol, c->refine_accept, state, rat);
+        }
+        return e;
+    }
     if (M.use16)
         return crt_launch_solve16(w.st, c->sanz, M.nblk, M.D, K, E, ch.nr, M.rhs.p + ch.r0, c->d_enr.p, w.fak,
                                   w.panels, w.dinv, w.yf, w.yscr, c->d_kpotN.p,
@@ -3206,7 +3261,7 @@ int crt_cholz(crt_ctx *c, int n, double *a, double *p) {
 int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int flags, crt_batch **out) {
     for (int i = 0; ctxs && i < n; ++i)  // unit sources only: no forward switches in a batch
         if (ctxs[i] && (ctxs[i]->sw_lsr || ctxs[i]->sw_lana || ctxs[i]->sw_nsink >= 0)) return CRT_ERR_UNSUPPORTED;
-    if (!ctxs || n <= 0 || !out || (flags & ~1)) return CRT_ERR_ARG;
+    if (!ctxs || n <= 0 || !out || (flags & ~3)) return CRT_ERR_ARG;
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return CRT_ERR_ARG;
@@ -3232,17 +3287,22 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
         else CRT_HIP(hipStreamCreateWithFlags(&b->slots[q].st, hipStreamNonBlocking));
         CRT_HIP(hipEventCreateWithFlags(&b->slots[q].join, hipEventDisableTiming));
     }
-    size_t np = 0, nf = 0, nv = 0, ny = 0;
+    size_t np = 0, nf = 0, nv = 0, ny = 0, nx = 0, nr = 0;
     for (int i = 0; i < n; ++i) {
         auto M = std::make_unique<crt_batch::Model>();
         M->c = ctxs[i];
         M->slot = i % nslot;
+        M->mixed = (flags & 2) != 0;
         int e = batch_plan(*M, workspace_bytes / nslot);
         if (e) return e;
         np = std::max(np, M->batch * M->panel_elems);
         nf = std::max(nf, M->batch * M->fstride);
         nv = std::max(nv, M->batch * ((size_t)M->c->nnz + 2));
         ny = std::max(ny, M->nyscr * M->fstride);
+        if (M->mixed) {
+            nx = std::max(nx, M->max_nr * M->fstride);
+            nr = std::max(nr, 2 * M->max_nr);
+        }
         b->nsys_total += M->nsys;
         b->m.push_back(std::move(M));
     }
@@ -3253,6 +3313,8 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
     e = e ? e : b->dga.ensure(nf * nslot);
     e = e ? e : b->vals.ensure(nv * nslot);
     e = e ? e : b->yscr.ensure(ny * nslot);
+    e = e ? e : (nx ? b->xacc.ensure(nx * nslot) : 0);
+    e = e ? e : (nr ? b->rstat.ensure(nr * nslot) : 0);
     if (e) return e;
     for (int q = 0; q < nslot; ++q) {
         crt_batch::Slot &w = b->slots[q];
@@ -3263,6 +3325,8 @@ int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int fl
         w.dga = b->dga.p + q * nf;
         w.vals = b->vals.p + q * nv;
         w.yscr = b->yscr.p + q * ny;
+        w.xacc = nx ? b->xacc.p + q * nx : nullptr;
+        w.rstat = nr ? b->rstat.p + q * nr : nullptr;
     }
     if (flags & 1) {
         for (auto &M : b->m) CRT_HIP(hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), b->stream));
@@ -3323,6 +3387,8 @@ int crt_batch_forward(crt_batch *b) {
     CRT_HIP(hipEventSynchronize(b->e1));
     CRT_HIP(hipEventElapsedTime(&b->ms, b->e0, b->e1));
     int worst = 0;
+    b->mixed_sweeps = b->mixed_fallbacks = 0;
+    b->mixed_ratio = 0.0;
     for (auto &M : b->m) {
         std::vector<int> st(M->nsys + 1);
         CRT_HIP(hipMemcpyAsync(st.data(), M->status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -3331,11 +3397,57 @@ int crt_batch_forward(crt_batch *b) {
         // failing model the batch's
         int w = st[0];
         for (size_t q = 1; q < st.size() && !w; ++q) w = st[q];
+        M->fallback = 0;
+        if (M->mixed) {
+            // a chunk whose refinement did not converge, or an FP32 pivot failure: the
+            // model again in FP64 through its own forward, which also names a genuine error
+            const size_t nc = M->chunks.size();
+            std::vector<int> cs(2 * nc);
+            std::vector<double> cr(2 * nc);
+            CRT_HIP(hipMemcpy(cs.data(), M->cstate.p, cs.size() * sizeof(int), hipMemcpyDeviceToHost));
+            CRT_HIP(hipMemcpy(cr.data(), M->crat.p, cr.size() * sizeof(double), hipMemcpyDeviceToHost));
+            bool ok = w == 0;
+            for (size_t q = 0; q < nc; ++q) {
+                b->mixed_sweeps = std::max(b->mixed_sweeps, cs[2 * q + 1]);
+                ok = ok && (cs[2 * q] == 1 || cs[2 * q] == 2);
+                if (cs[2 * q] == 1 || cs[2 * q] == 2) b->mixed_ratio = std::max(b->mixed_ratio, cr[2 * q + 1]);
+            }
+            if (!ok) {
+                // the model's chunks again with the FP64 kernels, directly on the batch's
+                // workspace (no allocation); their status names a genuine error
+                M->fallback = 1;
+                b->mixed_fallbacks++;
+                M->mixed = false;
+                const hipStream_t sst = b->slots[M->slot].st;
+                int e = hipMemsetAsync(M->status.p, 0, (M->nsys + 1) * sizeof(int), sst) == hipSuccess ? 0
+                                                                                          : CRT_ERR_HIP;
+                e = e ? e : batch_area(b, *M);
+                for (const auto &ch : M->chunks) {
+                    e = e ? e : batch_assemble(b, *M, ch);
+                    e = e ? e : batch_factor(b, *M, ch, M->variant);
+                    e = e ? e : batch_solve(b, *M, ch);
+                }
+                M->mixed = true;
+                if (e) return e;
+                CRT_HIP(hipStreamSynchronize(sst));
+                CRT_HIP(hipMemcpy(st.data(), M->status.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+                w = st[0];
+                for (size_t q = 1; q < st.size() && !w; ++q) w = st[q];
+            }
+        }
         if (!worst) worst = w;
         M->c->have_kpot = w == 0;
-        M->c->have_volt = M->c->have_sens = false;
+        M->c->have_volt = false;
     }
     return worst;
+}
+
+int crt_batch_refine_info(crt_batch *b, int *sweeps, double *ratio, int *fallbacks) {
+    if (!b) return CRT_ERR_ARG;
+    if (sweeps) *sweeps = b->mixed_sweeps;
+    if (ratio) *ratio = b->mixed_ratio;
+    if (fallbacks) *fallbacks = b->mixed_fallbacks;
+    return 0;
 }
 
 int crt_batch_info(crt_batch *b, double *ms, long long *nsys, int *variant, float *tune_ms) {

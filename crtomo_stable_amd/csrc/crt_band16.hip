@@ -1287,6 +1287,7 @@ struct SolveArgs {
     zd *kpotN;
     zd *xacc;            // MODE 1/2: [rhs][nblk*16]
     double *stats;       // MODE 2: [rhs][2]
+    const int *skip;     // non-zero *skip: the launch does nothing (a converged refinement)
 };
 using Solve16Args = SolveArgs<zd>;
 
@@ -1336,6 +1337,7 @@ __global__ void __launch_bounds__(64 * SW, D <= 6 ? 4 : 2) crt_solve16_kernel(So
     __shared__ zd dtile[T16 * PS];  // diagonal tile of the current block
     const int rid = blockIdx.x;
     if (rid >= A.nrhs) return;
+    if (A.skip && *A.skip) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int c = lane & 15, rq = lane >> 4;
     const int nblk = A.nblk;
@@ -1658,8 +1660,8 @@ int crt_launch_solve16_rhs(hipStream_t st, int sanz, int nblk, int D, int kwnanz
 // the max-norms of the correction and of the sum into stats)
 int crt_launch_solve16_f32(hipStream_t st, int sanz, int nblk, int D, int kwnanz, int eanz, int nrhs,
                            const int4 *rhs, const int *enr, const double *fak, const zf *panels, const zf *dinv,
-                           const zf *y, zd *yscr, zd *kpotN, zd *xacc, double *stats, int mode) {
-    SolveArgs<zf> a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, xacc, stats};
+                           const zf *y, zd *yscr, zd *kpotN, zd *xacc, double *stats, int mode, const int *skip) {
+    SolveArgs<zf> a{sanz, nblk, kwnanz, eanz, nrhs, rhs, enr, fak, panels, dinv, y, yscr, kpotN, xacc, stats, skip};
     if (mode == 1) return solve16_dispatch<zf, 1, true>(st, D, a);
     if (mode == 2) return solve16_dispatch<zf, 2, true>(st, D, a);
     return CRT_ERR_ARG;

@@ -442,10 +442,11 @@ __global__ void __launch_bounds__(SWV * 64, 2) crt_factor32st_kernel(Factor32Arg
 // (frow: row pointers, fent: (column, CSR index) per entry of the full row).
 __global__ void crt_refine_resid_kernel(int sanz, int fstride, int vstride, int nrhs, const int4 *rhs, const int *enr,
                                         const int *frow, const int2 *fent, const double *fak, const zd *vals,
-                                        const zd *x, zd *nres) {
+                                        const zd *x, zd *nres, const int *skip) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int rid = blockIdx.y;
     if (i >= fstride || rid >= nrhs) return;
+    if (skip && *skip) return;
     const int4 q = rhs[rid];
     zd acc = zd{0.0, 0.0};
     if (i < sanz) {
@@ -502,11 +503,57 @@ int crt_launch_factor32(hipStream_t st, const DevGrid &g16, int nsys, const int2
 
 int crt_launch_refine_resid(hipStream_t st, int sanz, int fstride, int vstride, int nrhs, const int4 *rhs,
                             const int *enr, const int *frow, const int2 *fent, const double *fak, const zd *vals,
-                            const zd *x, zd *nres) {
+                            const zd *x, zd *nres, const int *skip) {
     if (nrhs <= 0) return 0;
     if (nrhs > 65535) return CRT_ERR_UNSUPPORTED;
     dim3 grid((fstride + 255) / 256, nrhs);
     hipLaunchKernelGGL(crt_refine_resid_kernel, grid, dim3(256), 0, st, sanz, fstride, vstride, nrhs, rhs, enr, frow,
-                       fent, fak, vals, x, nres);
+                       fent, fak, vals, x, nres, skip);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+// Convergence of a refinement on the device, for sweeps captured in a graph: state[0]
+// 0 running, 1 converged (max over the right-hand sides of |fak d| / |fak x| at most
+// tol), 2 stalled at most accept (the FP64 rounding floor), 3 failed (stalled above
+// accept, or not finite); state[1] counts the sweeps that ran; rat[0] the previous
+// ratio, rat[1] the last.  The same rule as the host loop of the per-context forward.
+__global__ void __launch_bounds__(64) crt_refine_check_kernel(int nrhs, const double *stats, double tol,
+                                                              double accept, int *state, double *rat) {
+    if (state[0] != 0) return;
+    const int lane = threadIdx.x;
+    double r = 0.0;
+    for (int q = lane; q < nrhs; q += 64) {
+        const double d = stats[2 * q], x = stats[2 * q + 1];
+        const double rq = x > 0.0 ? d / x : (d > 0.0 ? HUGE_VAL : 0.0);
+        r = (rq <= r) ? r : rq;  // NaN propagates as not converged
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(r, o, 64);
+        r = (t <= r) ? r : t;
+    }
+    if (lane == 0) {
+        state[1] += 1;
+        rat[1] = r;
+        if (r <= tol) state[0] = 1;
+        else if (!(r < 0.5 * rat[0])) state[0] = r <= accept ? 2 : 3;
+        else rat[0] = r;
+    }
+}
+
+__global__ void crt_refine_init_kernel(int *state, double *rat) {
+    state[0] = 0;
+    state[1] = 0;
+    rat[0] = HUGE_VAL;
+    rat[1] = 0.0;
+}
+
+int crt_launch_refine_init(hipStream_t st, int *state, double *rat) {
+    hipLaunchKernelGGL(crt_refine_init_kernel, dim3(1), dim3(1), 0, st, state, rat);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
+}
+
+int crt_launch_refine_check(hipStream_t st, int nrhs, const double *stats, double tol, double accept, int *state,
+                            double *rat) {
+    hipLaunchKernelGGL(crt_refine_check_kernel, dim3(1), dim3(64), 0, st, nrhs, stats, tol, accept, state, rat);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

@@ -502,6 +502,7 @@ struct alignas(16) DenseLds {
         zd ws[NW][256];
     };
     zd dt[256], li[256];
+    zd lip[T16][T16 + 1];  // the inverse while it is formed, columns padded against bank conflicts
     zd pt[NW][T16];
     zd pinv[T16];
     int err;
@@ -666,15 +667,21 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             // lane (j, q) sums the terms c = q (mod 4)
             for (int r = 0; r < T16; ++r) {
                 zd sacc = zd{};
-                for (int c = rq; c < r; c += 4) sacc = zfma(sacc, m.dt[c * T16 + r], m.li[jc * T16 + c]);
+                for (int c = rq; c < r; c += 4) sacc = zfma(sacc, m.dt[c * T16 + r], m.lip[jc][c]);
                 sacc.re += __shfl_xor(sacc.re, 16, 64);
                 sacc.im += __shfl_xor(sacc.im, 16, 64);
                 sacc.re += __shfl_xor(sacc.re, 32, 64);
                 sacc.im += __shfl_xor(sacc.im, 32, 64);
                 const zd xv = zmul(zd{(r == jc ? 1.0 : 0.0) - sacc.re, -sacc.im}, m.pinv[r]);
-                if (rq == 0) m.li[jc * T16 + r] = xv;
+                if (rq == 0) m.lip[jc][r] = xv;
                 __builtin_amdgcn_wave_barrier();
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = 64 * u + lane;
+                m.li[e] = m.lip[e / T16][e % T16];
+            }
+            __builtin_amdgcn_wave_barrier();
             zd *dst = Ls + tri(J, J) * 256;
             zd *ldst = Lis + (size_t)J * 256;
 #pragma unroll

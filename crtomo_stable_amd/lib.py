@@ -514,15 +514,16 @@ class Batch:
     Every context needs its grid, survey, wavenumbers, precal and sigma; after
     ``forward()`` each holds kpot as after ``Context.forward()``.  ``autotune``
     times the factor kernel variants per band width on the first chunk and keeps
-    the fastest."""
+    the fastest.  ``mixed`` runs the mixed-precision variant: FP32 factorisation and
+    FP64 refinement inside the graph, models that do not converge redone in FP64."""
 
-    def __init__(self, contexts, workspace_bytes: int = 0, autotune: bool = False):
+    def __init__(self, contexts, workspace_bytes: int = 0, autotune: bool = False, mixed: bool = False):
         lib = load()
         lib.crt_batch_create.argtypes = [C.POINTER(_p), C.c_int, C.c_size_t, C.c_int, C.POINTER(_p)]
         arr = (_p * len(contexts))(*[c._h.value for c in contexts])
         h = _p()
-        _chk(lib.crt_batch_create(arr, len(contexts), workspace_bytes, 1 if autotune else 0, C.byref(h)),
-             "crt_batch_create")
+        flags = (1 if autotune else 0) | (2 if mixed else 0)
+        _chk(lib.crt_batch_create(arr, len(contexts), workspace_bytes, flags, C.byref(h)), "crt_batch_create")
         self._h = h
         self.lib = lib
         self.contexts = list(contexts)
@@ -539,6 +540,15 @@ class Batch:
         tune = np.zeros(68, np.float32)
         _chk(self.lib.crt_batch_info(self._h, C.byref(ms), C.byref(n), _ptr(var), _ptr(tune)), "crt_batch_info")
         return ms.value, n.value, var, tune.reshape(17, 4)
+
+    def refine_info(self):
+        """Mixed variant, last launch: (most sweeps of a chunk, largest accepted
+        correction ratio, models redone in FP64)."""
+        sw = C.c_int()
+        r = C.c_double()
+        fb = C.c_int()
+        _chk(self.lib.crt_batch_refine_info(self._h, C.byref(sw), C.byref(r), C.byref(fb)), "crt_batch_refine_info")
+        return sw.value, r.value, fb.value
 
     def close(self):
         if getattr(self, "_h", None):

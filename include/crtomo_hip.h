@@ -402,11 +402,21 @@ int crt_get_times(crt_ctx *ctx, double *ms, int n);
  *     crt_forward does (crt_voltages, crt_sens follow per context).
  *   crt_batch_info: device time of the last graph launch (ms), systems per
  *     launch, variant[17] per band width (0 default), tune_ms[17*4] timings.
+ *   flags bit 1: the mixed-precision variant: each chunk's systems factorised in
+ *     complex FP32 on the matrix cores, solved in FP32 and refined in FP64 (residual
+ *     from the FP64 matrix, correction through the FP32 factor) for up to each
+ *     context's refine_max sweeps inside the graph, a sweep skipped once the device
+ *     check has accepted the chunk (crt_set_forward_precision's tolerance and rule);
+ *     a model with a chunk that did not converge, or an FP32 pivot failure, is redone
+ *     with the FP64 kernels on the batch's workspace after the graph.  Band widths of 3..16 tiles (903
+ *     otherwise).  crt_batch_refine_info: the most sweeps a chunk ran, the largest
+ *     accepted correction ratio and the number of models redone, of the last launch.
  * A batch owns its stream; calls on a batch and on its contexts must not overlap. */
 typedef struct crt_batch crt_batch;
 int crt_batch_create(crt_ctx *const *ctxs, int n, size_t workspace_bytes, int flags, crt_batch **out);
 int crt_batch_forward(crt_batch *b);
 int crt_batch_info(crt_batch *b, double *ms, long long *nsys, int *variant, float *tune_ms);
+int crt_batch_refine_info(crt_batch *b, int *sweeps, double *ratio, int *fallbacks);
 int crt_batch_destroy(crt_batch *b);
 
 #ifdef __cplusplus

@@ -130,3 +130,29 @@ def test_batch_autotune(models):
             assert _rel_rows(c.get_kpot(), kp) <= TOL, w.name
             got_v = c.voltages(1)[0]
             assert float(np.max(np.abs(got_v - v) / np.abs(v))) <= TOL, w.name
+
+
+def test_batch_mixed_precision(models):
+    """The mixed-precision variant (FP32 factorisation on the matrix cores, FP64
+    refinement captured in the graph, a sweep skipped once the device check accepts the
+    chunk): the refined voltages within 1e-10 of each context's FP64 forward, every
+    model refined rather than redone in FP64, the same again on a replay."""
+    ws, cs = models
+    pick = [i for i, w in enumerate(ws) if 3 <= -(-w.grid.mb // 16) <= 16]
+    assert len(pick) >= 3
+    ref = []
+    for i in pick:
+        cs[i].forward()
+        ref.append(cs[i].voltages(1)[0])
+    with lib.Batch([cs[i] for i in pick], mixed=True) as b:
+        for rep in range(2):
+            b.forward()
+            sweeps, ratio, fallbacks = b.refine_info()
+            print(f"[batch mixed] replay {rep}: {b.info()[0]:.2f} ms, sweeps {sweeps}, ratio {ratio:.2e}, "
+                  f"fallbacks {fallbacks}", flush=True)
+            assert fallbacks == 0 and 1 <= sweeps <= 12 and ratio <= 1e-12
+            for i, r in zip(pick, ref):
+                v = cs[i].voltages(1)[0]
+                err = float(np.abs(v / r - 1).max())
+                print(f"[batch mixed]   {ws[i].name}: max relative voltage error {err:.2e}", flush=True)
+                assert err < TOL
