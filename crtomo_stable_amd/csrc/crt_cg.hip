@@ -264,13 +264,19 @@ __global__ void __launch_bounds__(64 * CG_RED_WAVES) crt_cg_apred_kernel(CGArgs 
     }
 }
 
-// K_f: Bp_j = C_j (J^H wap + lam R q)_j, CG_JHW_COLS columns per wave, dr1 partials
+// K_f: Bp_j = C_j (J^H wap + lam R q)_j, CG_JHW_COLS columns per wave, dr1 partials.
+// The column groups run from the last to the first: K_d streams J from its first column to
+// its last, so this pass starts on the columns K_d read last, which the memory-side cache
+// may still hold, and the next K_d on the ones this pass read last (C3: 0.978 -> 0.959 ms
+// per FP64 iteration, the FP32 copy unchanged).  Each group keeps its own partial slot, so
+// the sums are the same in either order.
 template <class JT>
 __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A, const JT *J) {
     __shared__ double red[8];
     if (A.scn->done) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int jbase = (blockIdx.x * (blockDim.x >> 6) + wv) * CG_JHW_COLS;
+    const int bx = gridDim.x - 1 - blockIdx.x;
+    const int jbase = (bx * (blockDim.x >> 6) + wv) * CG_JHW_COLS;
     double dpart = 0.0;
     if (jbase < A.manz) {
         const int nc = min(CG_JHW_COLS, A.manz - jbase);
@@ -329,7 +335,7 @@ __global__ void __launch_bounds__(CG_BLK) crt_cg_jhw_kernel(CGArgs A, const JT *
         }
     }
     double s = block_sum(dpart, red);
-    if (threadIdx.x == 0) A.part_dr1[blockIdx.x] = s;
+    if (threadIdx.x == 0) A.part_dr1[bx] = s;
 }
 
 // K_h: alpha; dpar += alpha p; r -= alpha Bp; |r|^2 partials
