@@ -5,11 +5,13 @@ crtomo_cfg_echo.json: per tomodir the text of crtomo.cfg, which of its side file
 existed at run time (the prior model, crt.noisemod) and the configuration the reference
 echoed into inv.ctr (src/kont1.f90) after reading the file (src/rall.f90:141-593).
 
-crtomo_runs.json: the committed runs of the reference's test tomodirs (tests/minimal,
-tests/2.5D_mixed_surf, tests/DC_Problem: crtomo.cfg, the inv.ctr iteration lines of
-src/kont2.f90, the REAL*8 misfit of rho00.mag, the forward voltages of volt00.dat, and
-error.dat where the run stopped) for the inversion tests; their grid, electrodes and
-measurements are copied to tests/golden/reference_fixtures/<inputs>.
+crtomo_runs.json: the committed runs of the reference's tomodirs with noise-free data
+(tests/minimal, tests/2.5D_mixed_surf, tests/DC_Problem, tests/non-ordered_boundary_conditions,
+frontcover/inversion, examples/uncert/test_dev: crtomo.cfg, the inv.ctr iteration lines of
+src/kont2.f90, the REAL*8 misfit of rho00.mag, the forward voltages of volt00.dat, error.dat
+where the run stopped, and for the uncertainty run its coverage and .diag files) for the
+inversion tests; their grid, electrodes and measurements are copied to
+tests/golden/reference_fixtures/<inputs>.
 
     python tests/golden/make_crtomo_golden.py
 """
@@ -106,7 +108,22 @@ RUNS = {
     "mixed_surf": ("tests/2.5D_mixed_surf/Reference", "mixed_surf"),
     "dc_problem_complex": ("tests/DC_Problem/Complex", "dc_problem"),
     "dc_problem_dc": ("tests/DC_Problem/DC", "dc_problem"),
+    "frontcover": ("frontcover/inversion", "frontcover"),
+    "non_ordered_bc": ("tests/non-ordered_boundary_conditions/Reference", "non_ordered_bc"),
+    "uncert_dc": ("examples/uncert/test_dev", "uncert_dc"),
 }
+
+
+def numeric_rows(path):
+    """The numeric rows of a list-directed output file (header and Max/Min lines dropped)."""
+    out = []
+    for l in open(path).read().splitlines()[1:]:
+        t = l.split()
+        try:
+            out.append([float(x) for x in t])
+        except ValueError:
+            continue
+    return out
 
 
 def committed_run(tomodir, inputs):
@@ -127,6 +144,10 @@ def committed_run(tomodir, inputs):
         out["error_dat"] = open(err).read()
     # the run ended normally when inv.ctr ends with ***finished***
     out["finished"] = open(f"{d}/inv/inv.ctr").read().rstrip().endswith("***finished***")
+    # the uncertainty run's final outputs: coverage and the five main diagonals
+    if tomodir.endswith("uncert/test_dev"):
+        out["final"] = {f: numeric_rows(f"{d}/inv/{f}") for f in
+                        ("coverage.mag", "ata.diag", "ata_reg.diag", "cov1_m.diag", "res_m.diag", "cov2_m.diag")}
     return out
 
 

@@ -11,7 +11,11 @@ tests/golden/reference_fixtures):
 * dc_problem_complex: 947 triangles, 2816 measurements of which hundreds change
   polarity (the "- # data" column), a zero phase error model (NaN phase RMS);
 * dc_problem_dc: the same data inverted DC: the reference stops with errnr 28 ("FE
-  matrix indefinite", its error.dat) in the forward after the first update.
+  matrix indefinite", its error.dat) in the forward after the first update;
+* frontcover: the reference's cover inversion (1411 nodes, complex, five iterations);
+* non_ordered_bc: the mixed_surf grid with its boundary elements listed out of order;
+* uncert_dc: DC, no iterations (itmax 0), fixed lambda 100, mswitch 8: the coverage and the
+  five .diag files of buncert.
 
 Pinned per run: the starting model's forward and misfit (volt00.dat REAL*4 voltages and
 the REAL*8 magnitude misfit of rho00.mag where committed), the IT 0 line of inv.ctr
@@ -25,8 +29,14 @@ rounding edge, as the reference's own runs of one tomodir from two versions do
 mixed_surf's CG stops at ncgmax (96 of 960 cells) on ever smaller lambdas, where the
 summation order moves the step size by a few 1e-3 from the third trial on; dc_problem_complex's
 CG stop test (eps 1e-4) lands on 27 against 28 iterations in the second trial of
-iteration 2.  After the common prefix the run must still end normally, mixed_surf with
-as many iterations and a final misfit within 1 % of the committed one.  The committed
+iteration 2; non_ordered_bc parts where mixed_surf does (the same grid, its boundary elements
+listed in another order).  frontcover parts at the first trial of iteration 2 with the same
+model step size and roughness but a 7 % larger misfit and 223 against 182 data excluded by
+the polarity check at iteration 2 (both 247 at iteration 1): the 2014 run handles the data
+whose sign changed during iteration 1 differently from src/chkpol.f90 as the driver follows
+it (called after the forward of a new iteration and again in its convergence test).  After
+the common prefix the run must still end normally, mixed_surf with as many iterations and a
+final misfit within 1 % of the committed one.  The committed
 runs are from the reference's 2014 master, whose column layout of the CG steps differs
 from src/kont2.f90's (i6): values, not lines, are compared after IT 0.
 """
@@ -46,7 +56,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FIX = os.path.join(HERE, "golden", "reference_fixtures")
 RUNS = json.load(open(os.path.join(HERE, "golden", "crtomo_runs.json")))
 # leading inv.ctr iteration lines compared value by value (None: all)
-STRICT = {"minimal": None, "mixed_surf": 3, "dc_problem_complex": 6, "dc_problem_dc": None}
+STRICT = {"minimal": None, "mixed_surf": 3, "dc_problem_complex": 6, "dc_problem_dc": None, "frontcover": 7,
+          "non_ordered_bc": 3, "uncert_dc": None}
+# committed files that an earlier run of the same directory left (not compared): the
+# uncertainty example's inv/ also holds rho01..06 of a run with iterations, and its
+# rho00.mag carries that run's misfit
+STALE = {"uncert_dc": {"betrms0", "volt00"}}
 
 
 def _tomodir(tmp_path, run):
@@ -107,13 +122,13 @@ def test_tomodir_against_committed_run(tmp_path, name):
         ctx.close()
     inv = root / "inv"
     print(open(inv / "inv.ctr").read().split("steplength", 1)[1], flush=True)
-    if "volt00" in run:
+    if "volt00" in run and "volt00" not in STALE.get(name, set()):
         v = np.array([[float(x) for x in l.split()[2:4]] for l in open(inv / "volt00.dat").read().splitlines()[1:]
                       if l.split()])
         ref = np.array(run["volt00"])
         assert np.allclose(v[:, 0], ref[:, 0], rtol=2e-6, atol=0), np.abs(v[:, 0] / ref[:, 0] - 1).max()
         assert np.allclose(v[:, 1], ref[:, 1], atol=1e-4)
-    if "betrms0" in run:
+    if "betrms0" in run and "betrms0" not in STALE.get(name, set()):
         betrms0 = float(open(inv / "rho00.mag").readline().split()[-1])
         assert abs(betrms0 / run["betrms0"] - 1) < 1e-9, (betrms0, run["betrms0"])
     rows, ref_rows = _rows(inv / "inv.ctr"), run["rows"]
@@ -139,6 +154,21 @@ def test_tomodir_against_committed_run(tmp_path, name):
         for f in ("run.ctr", "eps.ctr", "cjg.ctr", "rho00.modl", "coverage.mag"):
             assert (inv / f).exists(), f
         assert (root / "exe" / "crt.lamnull").exists() and (root / "exe" / "inv.lastmod").exists()
+    if "final" in run:
+        # the uncertainty run (DC, no iterations, fixed lambda 100, mswitch 8): coverage
+        # (REAL*8 log10 of the normalised column sums) and the five main diagonals of
+        # buncert (ata.diag REAL*4 in that version)
+        for f, ref_rows in run["final"].items():
+            got = np.array([[float(x) for x in l.split()] for l in open(inv / f).read().splitlines()[1:]
+                            if l.split() and not l.split()[0].startswith("Max")])
+            ref = np.array(ref_rows)
+            assert got.shape == ref.shape, (f, got.shape, ref.shape)
+            if f == "coverage.mag":
+                assert np.abs(got[:, 2] - ref[:, 2]).max() < 1e-6, f
+            else:
+                rel = np.abs(got[:, 0] / ref[:, 0] - 1).max()
+                print(f"[crtomo {name}] {f}: max relative difference of the diagonal {rel:.2e}", flush=True)
+                assert rel < 1e-5, (f, rel)
     if name == "minimal":
         assert "Resetting phase model" in text
         for f in ("rho00.pha", "rho00.sig", "ata.diag", "ata_reg.diag", "cov1_m.diag", "res_m.diag", "cov2_m.diag"):
