@@ -63,13 +63,13 @@ int crt_launch_cond_fb(hipStream_t, int, int, int, int, int, int, int, const int
 int crt_launch_cond_rhs(hipStream_t, int, int, int, int, int, int, int, const int *, const int *, const int *,
                         const zd *, const double *, const double *, zd *);
 int crt_launch_cond_trsm(hipStream_t, int, int, int, int, int, int, int, const zd *, const zd *, zd *, const int *,
-                         const int *, const int *, const zd *, int, const int *);
+                         const int *, const int *, const zd *, int, const int *, int);
 int crt_launch_cond_schur(hipStream_t, int, int, int, int, int, int, int, int, int, const int *, const int *,
                           const int *, const int *, const int *, const int *, const int *, const int *, const int *,
                           const zd *, const double *, const double *, const zd *, zd *, zd *);
 int crt_cond_dense_max_tiles();
-int crt_launch_cond_s0(hipStream_t, int, int, int, int, const zd *, const zd *, zd *);
-int crt_launch_cond_lgs(hipStream_t, int, int, int, const zd *, zd *, const zd *);
+int crt_launch_cond_s0(hipStream_t, int, int, int, int, const zd *, const zd *, zd *, int);
+int crt_launch_cond_lgs(hipStream_t, int, int, int, const zd *, zd *, const zd *, int);
 int crt_cond_prefix_max_tiles();
 int crt_launch_cond_bval(hipStream_t, const DevGrid &, int, int, int, int, int, const int *, const int2 *,
                          const double *, zd *);
@@ -78,7 +78,7 @@ int crt_launch_cond_dense(hipStream_t, const DevGrid &, int, int, int, int, int,
                           const double *, zd *, zd *, zd *, zd *, zd *, zd *, zd *, int *, int);
 int crt_launch_cond_gemm(hipStream_t, int, int, int, int, int, int, int, int, int, int, const zd *, const zd *,
                          const double *, const int *, const int *, const int *, const int *, const int *, const int *,
-                         zd *);
+                         zd *, int);
 // model uncertainty (crt_bmcm.hip)
 int crt_bmcm_run(hipStream_t, int, int, const zd *, const double *, int, int, int, int, const double *, const int *, double,
                  int, double *, double *, double *, double *, double *, double *, int *, int *);
@@ -1684,9 +1684,9 @@ static int forward_condensed(crt_ctx *c) {
         c->toc(T_FACTOR);
         c->tic(T_SOLVE);
         e = crt_launch_cond_trsm(st, D, 0, P.nblkP, ldx, P.ntileP, nsn, nP, P.panels.p, P.dinv.p, P.X.p, P.d_tptr.p,
-                                 P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
+                                 P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p, g.dc);
         e = e ? e : crt_launch_cond_trsm(st, D, 1, P.nblkP, ldx, P.ntileP, nsn, nP, P.panels.p, P.dinv.p, P.X.p,
-                                         P.d_tptr.p, P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p);
+                                         P.d_tptr.p, P.d_erow.p, P.d_ecol.p, P.rv.p, P.nent, P.d_j0.p, g.dc);
         e = e ? e : crt_launch_cond_schur(st, n, k0, nBp, E, ldx, P.ntileP, c->nnz, fstrideC, nP, P.d_gi_ptr.p,
                                           P.d_gi_row.p, P.d_gi_piece.p, P.d_gi_q.p, P.d_roff.p, P.d_bcol.p,
                                           P.d_ucol.p, P.d_gmap.p, P.d_eB.p, c->d_area.p, P.fak.p, P.fB.p, P.X.p,
@@ -1698,8 +1698,8 @@ static int forward_condensed(crt_ctx *c) {
             e = crt_launch_cond_dense(st, g, 1, P.nG, nt, nBp, JS, E, K, k0, n, nullptr, P.T.p, P.Gr.p, P.d_bptr.p,
                                       P.d_bent.p, P.d_qdiagG.p, P.d_nodeB.p, P.bval.p, P.nb, P.fB.p, P.Lt.p, P.Li.p,
                                       P.Lpre.p, P.Lipre.p, P.S0.p, P.XB.p, c->d_kpotN.p, stI, nP);
-        if (JS) e = e ? e : crt_launch_cond_lgs(st, n, ntB, JS, P.T.p, P.Lpre.p, P.Lipre.p);
-        if (JS) e = e ? e : crt_launch_cond_s0(st, n, nt, ntB, JS, P.T.p, P.Lpre.p, P.S0.p);
+        if (JS) e = e ? e : crt_launch_cond_lgs(st, n, ntB, JS, P.T.p, P.Lpre.p, P.Lipre.p, g.dc);
+        if (JS) e = e ? e : crt_launch_cond_s0(st, n, nt, ntB, JS, P.T.p, P.Lpre.p, P.S0.p, g.dc);
         e = e ? e : crt_launch_cond_bval(st, g, n * E, E, k0, nBp, P.nb, P.d_bq.p, P.d_bg.p, P.fB.p + P.nSp, P.bval.p);
         if (e) return e;
         c->toc(T_PREFIX);
@@ -1713,7 +1713,7 @@ static int forward_condensed(crt_ctx *c) {
         c->tic(T_PRODUCT);
         e = crt_launch_cond_gemm(st, nsn, nP, k0, ldx, P.ntileP, P.ncolP, nBp, E, K, fstrideC, P.X.p, P.XB.p, P.fak.p,
                                  P.d_nodeI.p, P.d_roff.p, P.d_plen.p, P.d_nbc.p, P.d_colmap.p, P.d_ucol.p,
-                                 c->d_kpotN.p);
+                                 c->d_kpotN.p, g.dc);
         if (e) return e;
         c->toc(T_PRODUCT);
         CRT_HIP(hipStreamSynchronize(st));

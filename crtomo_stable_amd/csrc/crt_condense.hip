@@ -39,12 +39,20 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// RE: the real arithmetic of the DC inversion (src/choldc.f90, src/vredc.f90) on the
+// same complex storage.  The DC model and every matrix built from it have zero
+// imaginary parts, so the kernels read only the real parts, run one real MFMA chain
+// per product and store +0 imaginary parts.
+
 // (cr, ci) += a b for one k-step of complex operands
+template <bool RE>
 __device__ __forceinline__ void cmma(d4 &cr, d4 &ci, zd a, zd b) {
     cr = mfma4(a.re, b.re, cr);
-    cr = mfma4(-a.im, b.im, cr);
-    ci = mfma4(a.re, b.im, ci);
-    ci = mfma4(a.im, b.re, ci);
+    if constexpr (!RE) {
+        cr = mfma4(-a.im, b.im, cr);
+        ci = mfma4(a.re, b.im, ci);
+        ci = mfma4(a.im, b.re, ci);
+    }
 }
 
 // complex products with three real ones (Gauss): p1 = sum ar br, p2 = sum ai bi,
@@ -55,18 +63,59 @@ struct C3 {
     d4 p1, p2, p3;
 };
 __device__ __forceinline__ void c3zero(C3 &c) { c.p1 = c.p2 = c.p3 = d4{0.0, 0.0, 0.0, 0.0}; }
+template <bool RE>
 __device__ __forceinline__ void c3ma(C3 &c, zd a, zd b) {
     c.p1 = mfma4(a.re, b.re, c.p1);
-    c.p2 = mfma4(a.im, b.im, c.p2);
-    c.p3 = mfma4(a.re + a.im, b.re + b.im, c.p3);
+    if constexpr (!RE) {
+        c.p2 = mfma4(a.im, b.im, c.p2);
+        c.p3 = mfma4(a.re + a.im, b.re + b.im, c.p3);
+    }
 }
+template <bool RE>
 __device__ __forceinline__ zd c3get(const C3 &c, int v) {
-    return zd{c.p1[v] - c.p2[v], c.p3[v] - c.p1[v] - c.p2[v]};
+    if constexpr (RE) return zd{c.p1[v], 0.0};
+    else return zd{c.p1[v] - c.p2[v], c.p3[v] - c.p1[v] - c.p2[v]};
 }
 
 __device__ __forceinline__ zd ldz(const zd *p) {
     const double2 v = *reinterpret_cast<const double2 *>(p);
     return zd{v.x, v.y};
+}
+// the real part alone under RE
+template <bool RE>
+__device__ __forceinline__ zd ldzr(const zd *p) {
+    if constexpr (RE) return zd{p->re, 0.0};
+    else return ldz(p);
+}
+template <bool RE>
+__device__ __forceinline__ zd zmul_r(zd a, zd b) {
+    if constexpr (RE) return zd{a.re * b.re, 0.0};
+    else return zmul(a, b);
+}
+// a + b c and a - b c
+template <bool RE>
+__device__ __forceinline__ zd zfma_r(zd a, zd b, zd c) {
+    if constexpr (RE) return zd{a.re + b.re * c.re, 0.0};
+    else return zfma(a, b, c);
+}
+template <bool RE>
+__device__ __forceinline__ zd zfms_r(zd a, zd b, zd c) {
+    if constexpr (RE) return zd{a.re - b.re * c.re, 0.0};
+    else return zfms(a, b, c);
+}
+// one butterfly step of a wave sum: a += the value of lane (lane ^ m)
+template <bool RE>
+__device__ __forceinline__ void zxor(zd &a, int m) {
+    a.re += __shfl_xor(a.re, m, 64);
+    if constexpr (!RE) a.im += __shfl_xor(a.im, m, 64);
+}
+// pivot of a DC column: L = sqrt(s), 1/L; bad for a pivot that is not positive and
+// finite (src/choldc.f90 error 28)
+__device__ __forceinline__ void pivot_real(double s, zd &L, zd &inv, bool &bad) {
+    bad = !(s > 0.0) || !(s < 1.0e300);
+    const double rh = rsqrt_f64(bad ? 1.0 : s);
+    L = zd{s * rh, 0.0};
+    inv = zd{rh, 0.0};
 }
 __device__ __forceinline__ void stz(zd *p, zd z) { *reinterpret_cast<double2 *>(p) = make_double2(z.re, z.im); }
 
@@ -178,7 +227,7 @@ struct TrsmArgs {
     const int *j0;      // first nonzero block of each (piece, column tile)
 };
 
-template <int D, bool BWD>
+template <int D, bool BWD, bool RE>
 __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_trsm_kernel(TrsmArgs A) {
     constexpr int R = D;                              // ring slots
     constexpr int NA = D >= 2 ? (D - 1 + 2) / 3 : 1;  // tiles per worker wave (a = 2 + w-1 + 3u)
@@ -210,7 +259,7 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
         const zd *tp = panel + ((size_t)(live ? Jt : 0) * (D + 1) + a) * 256;
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
-            o[ks] = live ? (BWD ? ldz(tp + 64 * ks + lane) : ldz(tp + jc * T16 + 4 * ks + rq)) : zd{};
+            o[ks] = live ? (BWD ? ldzr<RE>(tp + 64 * ks + lane) : ldzr<RE>(tp + jc * T16 + 4 * ks + rq)) : zd{};
     };
     // inverse diagonal tile: forward Linv(i, k), backward Linv(k, i)
     auto load_diag = [&](int J, zd (&o)[4]) {
@@ -221,8 +270,8 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             const int k = 4 * ks + rq;
             zd e = zd{};
             if (live) {
-                if (jc == k) e = ldz(dinv + (size_t)J * T16 + jc);
-                else if (BWD ? (k > jc) : (jc > k)) e = BWD ? ldz(tp + jc * T16 + k) : ldz(tp + 64 * ks + lane);
+                if (jc == k) e = ldzr<RE>(dinv + (size_t)J * T16 + jc);
+                else if (BWD ? (k > jc) : (jc > k)) e = BWD ? ldzr<RE>(tp + jc * T16 + k) : ldzr<RE>(tp + 64 * ks + lane);
             }
             o[ks] = e;
         }
@@ -240,13 +289,13 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             const int p = wb + lane;
             wrow = p < cend ? A.erow[p] : INT32_MAX;
             wcol = p < cend ? A.ecol[p] : 0;
-            wv = p < cend ? rv[p] : zd{};
+            wv = p < cend ? ldzr<RE>(rv + p) : zd{};
         };
         if (!BWD) load_window();
         auto load_x = [&](int J, zd (&o)[4]) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                o[u] = (J >= J0 && J >= 0 && J < nblk) ? ldz(X + (size_t)jc * A.ldx + (size_t)J * T16 + 4 * u + rq)
+                o[u] = (J >= J0 && J >= 0 && J < nblk) ? ldzr<RE>(X + (size_t)jc * A.ldx + (size_t)J * T16 + 4 * u + rq)
                                                        : zd{};
         };
         const int Js = BWD ? nblk - 1 : J0;
@@ -263,7 +312,7 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             {
                 const zd *B = ring[((J - step) % R + R) % R];
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) c3ma(acc, a1[ks], B[64 * ks + lane]);
+                for (int ks = 0; ks < 4; ++ks) c3ma<RE>(acc, a1[ks], B[64 * ks + lane]);
             }
             // right-hand-side entries of block J (forward): a window of 64 entries sits in
             // registers (lane i holds entry wb + i, wc of them used), each entry goes to
@@ -278,13 +327,13 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
                     const int n = __popcll(__ballot(in));
                     for (int i = wc; i < wc + n; ++i) {
                         const int ri = __shfl(wrow, i, 64) - J * T16, ci_ = __shfl(wcol, i, 64);
-                        const double vr = __shfl(wv.re, i, 64), vi = __shfl(wv.im, i, 64);
+                        const double vr = __shfl(wv.re, i, 64), vi = RE ? 0.0 : __shfl(wv.im, i, 64);
                         const bool mine = jc == ci_ && rq == (ri & 3);
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
                             if (mine && (ri >> 2) == u) {
                                 T[u].re += vr;
-                                T[u].im += vi;
+                                if constexpr (!RE) T[u].im += vi;
                             }
                     }
                     wc += n;
@@ -299,20 +348,20 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int o = 64 * u + lane;
-                const zd a0 = c3get(acc, u);
+                const zd a0 = c3get<RE>(acc, u);
                 T[u].re -= a0.re + part[pb][0][o].re + part[pb][1][o].re + part[pb][2][o].re;
-                T[u].im -= a0.im + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
+                if constexpr (!RE) T[u].im -= a0.im + part[pb][0][o].im + part[pb][1][o].im + part[pb][2][o].im;
             }
             // X_J = Linv T (forward) or Linv^T T (backward); T in the result layout is
             // the B operand of step ks = element ks
             C3 xa;
             c3zero(xa);
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) c3ma(xa, dg[ks], T[ks]);
+            for (int ks = 0; ks < 4; ++ks) c3ma<RE>(xa, dg[ks], T[ks]);
             zd *slot = ring[(J % R + R) % R];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const zd xv = c3get(xa, u);
+                const zd xv = c3get<RE>(xa, u);
                 slot[64 * u + lane] = xv;
                 stz(X + (size_t)jc * A.ldx + (size_t)J * T16 + 4 * u + rq, xv);
             }
@@ -352,12 +401,12 @@ __global__ void __launch_bounds__(256, D <= 7 ? 3 : (D <= 14 ? 2 : 1)) crt_cond_
             if (a <= D) {
                 const zd *B = ring[((J - step * a) % R + R) % R];
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) c3ma(acc, cur[u][ks], B[64 * ks + lane]);
+                for (int ks = 0; ks < 4; ++ks) c3ma<RE>(acc, cur[u][ks], B[64 * ks + lane]);
             }
         }
         zd *pw = part[J & 1][wave - 1];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pw[64 * u + lane] = c3get(acc, u);
+        for (int u = 0; u < 4; ++u) pw[64 * u + lane] = c3get<RE>(acc, u);
         lds_barrier();
 #pragma unroll
         for (int u = 0; u < NA; ++u)
@@ -511,7 +560,7 @@ struct alignas(16) DenseLds {
 
 // NW waves per system.  With four (source systems of up to 20 tiles) two systems share
 // a CU, so one system's diagonal chain and loads overlap the other's products.
-template <int NI, bool PRE, int NW>
+template <int NI, bool PRE, int NW, bool RE>
 __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kernel(DenseArgs A) {
     __shared__ DenseLds<NW> m;
     extern __shared__ zd yv[];  // nBp
@@ -563,7 +612,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             const zd *st = M + tri(I < nt ? I : J, J) * 256;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const zd e = I < nt ? ldz(st + jc * T16 + 4 * v + rq) : zd{};
+                const zd e = I < nt ? ldzr<RE>(st + jc * T16 + 4 * v + rq) : zd{};
                 ar[u][v] = e.re;
                 ai[u][v] = e.im;
             }
@@ -575,16 +624,16 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
                 for (int p0 = A.bptr[ti]; p0 < pe; p0 += 64) {
                     const int p = p0 + lane, n = min(64, pe - p0);
                     const int ev = p < pe ? A.bent[p] : 0;
-                    const zd bv = p < pe ? bval[p] : zd{};
+                    const zd bv = p < pe ? ldzr<RE>(bval + p) : zd{};
                     for (int i = 0; i < n; ++i) {
                         const int e = __shfl(ev, i, 64), r = e % T16, c = e / T16;
-                        const double vr = __shfl(bv.re, i, 64), vi = __shfl(bv.im, i, 64);
+                        const double vr = __shfl(bv.re, i, 64), vi = RE ? 0.0 : __shfl(bv.im, i, 64);
                         const bool mine = lane == c + 16 * (r % 4);
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
                             if (mine && v == r / 4) {
                                 ar[u][v] += vr;
-                                ai[u][v] += vi;
+                                if constexpr (!RE) ai[u][v] += vi;
                             }
                     }
                 }
@@ -594,7 +643,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             const int np = min(DCH, pend - p0);
             __syncthreads();
             const zd *src = Ls + tri(J, p0) * 256;
-            for (int i = threadIdx.x; i < np * 256; i += 64 * NW) (&m.rowJ[0][0])[i] = ldz(src + i);
+            for (int i = threadIdx.x; i < np * 256; i += 64 * NW) {
+                if constexpr (RE) (&m.rowJ[0][0])[i].re = src[i].re;
+                else (&m.rowJ[0][0])[i] = ldz(src + i);
+            }
             __syncthreads();
             // per owned tile: the chunk's tiles L(I, p0..) (contiguous), PW at a time, the
             // next group in flight while the current one is multiplied (the pipeline runs
@@ -609,7 +661,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
                 for (int j = 0; j < PW; ++j)
 #pragma unroll
                     for (int ks = 0; ks < 4; ++ks)
-                        o[j][ks] = live && pq + j < np ? ldz(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
+                        o[j][ks] = live && pq + j < np ? ldzr<RE>(tp + (size_t)(pq + j) * 256 + 64 * ks + lane) : zd{};
             };
             ld2(0, 0, bc);
 #pragma unroll
@@ -623,7 +675,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
                             if (pq + j < np)
 #pragma unroll
                                 for (int ks = 0; ks < 4; ++ks)
-                                    cmma(ar[u], ai[u], zd{-bc[j][ks].re, -bc[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
+                                    cmma<RE>(ar[u], ai[u], zd{-bc[j][ks].re, -bc[j][ks].im}, m.rowJ[pq + j][64 * ks + lane]);
 #pragma unroll
                         for (int j = 0; j < PW; ++j)
                             for (int ks = 0; ks < 4; ++ks) bc[j][ks] = bn[j][ks];
@@ -634,7 +686,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
         // diagonal tile: wave 0 factors it in LDS
         if (wave == 0) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) m.dt[jc * T16 + 4 * v + rq] = zd{ar[0][v], ai[0][v]};
+            for (int v = 0; v < 4; ++v) m.dt[jc * T16 + 4 * v + rq] = zd{ar[0][v], RE ? 0.0 : ai[0][v]};
             __builtin_amdgcn_wave_barrier();
             int err = 0;
 #pragma unroll
@@ -642,20 +694,21 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
                 const zd sv = m.dt[c * T16 + c];
                 zd L, inv;
                 bool bad;
-                pivot_fast(sv, L, inv, bad);
+                if constexpr (RE) pivot_real(sv.re, L, inv, bad);
+                else pivot_fast(sv, L, inv, bad);
                 if (bad || (g.dc && !(sv.re > 0.0))) err = CRT_ERR_PIVOT;
                 // lane (r, cq) owns entries (r, cq + 4w), w = 0..3
                 const int r = jc;
-                const zd lr = zmul(m.dt[c * T16 + r], inv);
+                const zd lr = zmul_r<RE>(m.dt[c * T16 + r], inv);
                 zd lc[4];
 #pragma unroll
-                for (int w = 0; w < 4; ++w) lc[w] = zmul(m.dt[c * T16 + rq + 4 * w], inv);
+                for (int w = 0; w < 4; ++w) lc[w] = zmul_r<RE>(m.dt[c * T16 + rq + 4 * w], inv);
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
                     const int cc = rq + 4 * w;
                     zd *e = &m.dt[cc * T16 + r];
-                    if (cc > c && r >= cc) *e = zfms(*e, lr, lc[w]);
+                    if (cc > c && r >= cc) *e = zfms_r<RE>(*e, lr, lc[w]);
                     else if (cc == c && r > c) *e = lr;
                     else if (cc == c && r == c) *e = L;
                 }
@@ -667,12 +720,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             // lane (j, q) sums the terms c = q (mod 4)
             for (int r = 0; r < T16; ++r) {
                 zd sacc = zd{};
-                for (int c = rq; c < r; c += 4) sacc = zfma(sacc, m.dt[c * T16 + r], m.lip[jc][c]);
-                sacc.re += __shfl_xor(sacc.re, 16, 64);
-                sacc.im += __shfl_xor(sacc.im, 16, 64);
-                sacc.re += __shfl_xor(sacc.re, 32, 64);
-                sacc.im += __shfl_xor(sacc.im, 32, 64);
-                const zd xv = zmul(zd{(r == jc ? 1.0 : 0.0) - sacc.re, -sacc.im}, m.pinv[r]);
+                for (int c = rq; c < r; c += 4) sacc = zfma_r<RE>(sacc, m.dt[c * T16 + r], m.lip[jc][c]);
+                zxor<RE>(sacc, 16);
+                zxor<RE>(sacc, 32);
+                const zd xv = zmul_r<RE>(zd{(r == jc ? 1.0 : 0.0) - sacc.re, -sacc.im}, m.pinv[r]);
                 if (rq == 0) m.lip[jc][r] = xv;
                 __builtin_amdgcn_wave_barrier();
             }
@@ -699,15 +750,15 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             if (I > J && I < nt) {
                 zd *w = m.ws[wave];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) w[jc * T16 + 4 * v + rq] = zd{ar[u][v], ai[u][v]};
+                for (int v = 0; v < 4; ++v) w[jc * T16 + 4 * v + rq] = zd{ar[u][v], RE ? 0.0 : ai[u][v]};
                 __builtin_amdgcn_wave_barrier();
                 d4 xr = {0.0, 0.0, 0.0, 0.0}, xi = xr;
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) cmma(xr, xi, w[64 * ks + lane], m.li[64 * ks + lane]);
+                for (int ks = 0; ks < 4; ++ks) cmma<RE>(xr, xi, w[64 * ks + lane], m.li[64 * ks + lane]);
                 __builtin_amdgcn_wave_barrier();
                 zd *dst = Ls + tri(I, J) * 256;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) stz(dst + jc * T16 + 4 * v + rq, zd{xr[v], xi[v]});
+                for (int v = 0; v < 4; ++v) stz(dst + jc * T16 + 4 * v + rq, zd{xr[v], RE ? 0.0 : xi[v]});
             }
         }
     }
@@ -733,12 +784,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
         for (int p = wave; p < J; p += NW) {
             const zd *tp = tileL(J, p);
 #pragma unroll
-            for (int w = 0; w < 4; ++w) acc = zfma(acc, ldz(tp + 64 * w + lane), yv[16 * p + 4 * w + rq]);
+            for (int w = 0; w < 4; ++w) acc = zfma_r<RE>(acc, ldzr<RE>(tp + 64 * w + lane), yv[16 * p + 4 * w + rq]);
         }
-        acc.re += __shfl_xor(acc.re, 16, 64);
-        acc.im += __shfl_xor(acc.im, 16, 64);
-        acc.re += __shfl_xor(acc.re, 32, 64);
-        acc.im += __shfl_xor(acc.im, 32, 64);
+        zxor<RE>(acc, 16);
+        zxor<RE>(acc, 32);
         if (lane < T16) m.pt[wave][lane] = acc;
         __syncthreads();
         if (wave == 0) {
@@ -753,11 +802,9 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             const zd *lt = tileLi(J);
             zd y = zd{};
 #pragma unroll
-            for (int w = 0; w < 4; ++w) y = zfma(y, ldz(lt + 64 * w + lane), m.ws[0][4 * w + rq]);
-            y.re += __shfl_xor(y.re, 16, 64);
-            y.im += __shfl_xor(y.im, 16, 64);
-            y.re += __shfl_xor(y.re, 32, 64);
-            y.im += __shfl_xor(y.im, 32, 64);
+            for (int w = 0; w < 4; ++w) y = zfma_r<RE>(y, ldzr<RE>(lt + 64 * w + lane), m.ws[0][4 * w + rq]);
+            zxor<RE>(y, 16);
+            zxor<RE>(y, 32);
             if (lane < T16) yv[16 * J + lane] = y;
         }
         __syncthreads();
@@ -769,15 +816,12 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             const zd *tp = tileL(I, J);
             const zd xr = yv[16 * I + jc];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) acc[w] = zfma(acc[w], ldz(tp + 64 * w + lane), xr);
+            for (int w = 0; w < 4; ++w) acc[w] = zfma_r<RE>(acc[w], ldzr<RE>(tp + 64 * w + lane), xr);
         }
 #pragma unroll
         for (int w = 0; w < 4; ++w)
 #pragma unroll
-            for (int o = 1; o < T16; o <<= 1) {
-                acc[w].re += __shfl_xor(acc[w].re, o, 64);
-                acc[w].im += __shfl_xor(acc[w].im, o, 64);
-            }
+            for (int o = 1; o < T16; o <<= 1) zxor<RE>(acc[w], o);
         if (jc == 0)
 #pragma unroll
             for (int w = 0; w < 4; ++w) m.pt[wave][4 * w + rq] = acc[w];
@@ -795,14 +839,11 @@ __global__ void __launch_bounds__(64 * NW, NW == 4 ? 2 : 1) crt_cond_dense_kerne
             zd xo[4];
             const zd tr = m.ws[0][jc];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) xo[w] = zmul(ldz(lt + 64 * w + lane), tr);
+            for (int w = 0; w < 4; ++w) xo[w] = zmul_r<RE>(ldzr<RE>(lt + 64 * w + lane), tr);
 #pragma unroll
             for (int w = 0; w < 4; ++w)
 #pragma unroll
-                for (int o = 1; o < T16; o <<= 1) {
-                    xo[w].re += __shfl_xor(xo[w].re, o, 64);
-                    xo[w].im += __shfl_xor(xo[w].im, o, 64);
-                }
+                for (int o = 1; o < T16; o <<= 1) zxor<RE>(xo[w], o);
             __builtin_amdgcn_wave_barrier();
             if (jc == 0)
 #pragma unroll
@@ -835,6 +876,7 @@ struct LgsArgs {
     const zd *Lipre;      // [kk][JS]
 };
 
+template <bool RE>
 __global__ void __launch_bounds__(256) crt_cond_lgs_kernel(LgsArgs A) {
     __shared__ zd ws[4][256];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -853,7 +895,7 @@ __global__ void __launch_bounds__(256) crt_cond_lgs_kernel(LgsArgs A) {
         const zd *src = Tk + tri(I, p) * 256;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const zd e = ldz(src + jc * T16 + 4 * v + rq);
+            const zd e = ldzr<RE>(src + jc * T16 + 4 * v + rq);
             ar[v] = e.re;
             ai[v] = e.im;
         }
@@ -862,33 +904,34 @@ __global__ void __launch_bounds__(256) crt_cond_lgs_kernel(LgsArgs A) {
             zd a4[4], b4[4];
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
-                a4[ks] = ldz(la + 64 * ks + lane);
-                b4[ks] = ldz(lb + 64 * ks + lane);
+                a4[ks] = ldzr<RE>(la + 64 * ks + lane);
+                b4[ks] = ldzr<RE>(lb + 64 * ks + lane);
             }
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) cmma(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
+            for (int ks = 0; ks < 4; ++ks) cmma<RE>(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
         }
         // X = C L(p, p)^-T with C staged column-major in the wave's LDS tile
 #pragma unroll
-        for (int v = 0; v < 4; ++v) wsm[jc * T16 + 4 * v + rq] = zd{ar[v], ai[v]};
+        for (int v = 0; v < 4; ++v) wsm[jc * T16 + 4 * v + rq] = zd{ar[v], RE ? 0.0 : ai[v]};
         __builtin_amdgcn_wave_barrier();
         d4 xr = {0.0, 0.0, 0.0, 0.0}, xi = xr;
         const zd *li = Lik + (long)p * 256;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) cmma(xr, xi, wsm[64 * ks + lane], ldz(li + 64 * ks + lane));
+        for (int ks = 0; ks < 4; ++ks) cmma<RE>(xr, xi, wsm[64 * ks + lane], ldzr<RE>(li + 64 * ks + lane));
         __builtin_amdgcn_wave_barrier();
         zd *dst = Lk + tri(I, p) * 256;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) stz(dst + jc * T16 + 4 * v + rq, zd{xr[v], xi[v]});
+        for (int v = 0; v < 4; ++v) stz(dst + jc * T16 + 4 * v + rq, zd{xr[v], RE ? 0.0 : xi[v]});
         __threadfence_block();  // the row's later columns read this tile back
     }
 }
 
-int crt_launch_cond_lgs(hipStream_t st, int kc, int ntB, int JS, const zd *T, zd *Lpre, const zd *Lipre) {
+int crt_launch_cond_lgs(hipStream_t st, int kc, int ntB, int JS, const zd *T, zd *Lpre, const zd *Lipre, int re) {
     LgsArgs a{kc, ntB, JS, T, Lpre, Lipre};
     const long n = (long)kc * (ntB - JS);
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(crt_cond_lgs_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    if (re) hipLaunchKernelGGL(crt_cond_lgs_kernel<true>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(crt_cond_lgs_kernel<false>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -903,6 +946,7 @@ struct S0Args {
     zd *S0w;             // [kk][G tiles]
 };
 
+template <bool RE>
 __global__ void __launch_bounds__(256) crt_cond_s0_kernel(S0Args A) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int jc = lane & 15, rq = lane >> 4;
@@ -922,7 +966,7 @@ __global__ void __launch_bounds__(256) crt_cond_s0_kernel(S0Args A) {
     d4 ar, ai;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const zd e = ldz(src + jc * T16 + 4 * v + rq);
+        const zd e = ldzr<RE>(src + jc * T16 + 4 * v + rq);
         ar[v] = e.re;
         ai[v] = e.im;
     }
@@ -931,21 +975,23 @@ __global__ void __launch_bounds__(256) crt_cond_s0_kernel(S0Args A) {
         zd a4[4], b4[4];
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-            a4[ks] = ldz(la + (size_t)p * 256 + 64 * ks + lane);
-            b4[ks] = ldz(lb + (size_t)p * 256 + 64 * ks + lane);
+            a4[ks] = ldzr<RE>(la + (size_t)p * 256 + 64 * ks + lane);
+            b4[ks] = ldzr<RE>(lb + (size_t)p * 256 + 64 * ks + lane);
         }
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) cmma(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
+        for (int ks = 0; ks < 4; ++ks) cmma<RE>(ar, ai, zd{-a4[ks].re, -a4[ks].im}, b4[ks]);
     }
     zd *o = A.S0w + kk * ntri * 256 + tri(I, J) * 256;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) stz(o + jc * T16 + 4 * v + rq, zd{ar[v], ai[v]});
+    for (int v = 0; v < 4; ++v) stz(o + jc * T16 + 4 * v + rq, zd{ar[v], RE ? 0.0 : ai[v]});
 }
 
-int crt_launch_cond_s0(hipStream_t st, int kc, int nt, int ntB, int JS, const zd *T, const zd *Lpre, zd *S0w) {
+int crt_launch_cond_s0(hipStream_t st, int kc, int nt, int ntB, int JS, const zd *T, const zd *Lpre, zd *S0w,
+                       int re) {
     S0Args a{kc, nt, ntB, JS, T, Lpre, S0w};
     const long n = (long)kc * nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(crt_cond_s0_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    if (re) hipLaunchKernelGGL(crt_cond_s0_kernel<true>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(crt_cond_s0_kernel<false>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -961,6 +1007,7 @@ struct GemmArgs {
     zd *kpotN;
 };
 
+template <bool RE>
 __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
     const int b = blockIdx.x;
     const int rg = b % A.rgroups, cgp = (b / A.rgroups) % A.cgroups, sy = b / (A.rgroups * A.cgroups);
@@ -984,7 +1031,7 @@ __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
         int bc[4];
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-            ao[ks] = ldz(X + (size_t)(16 * kt + 4 * ks + rq) * A.ldx + 16 * rt + jc);
+            ao[ks] = ldzr<RE>(X + (size_t)(16 * kt + 4 * ks + rq) * A.ldx + 16 * rt + jc);
             bc[ks] = cmap[16 * kt + 4 * ks + rq];
         }
 #pragma unroll
@@ -993,8 +1040,8 @@ __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
             if (16 * (4 * cgp + c) < A.E) {
 #pragma unroll
                 for (int ks = 0; ks < 4; ++ks) {
-                    const zd bo = (lcol < A.E && bc[ks] >= 0) ? ldz(XB + (size_t)lcol * A.nBp + bc[ks]) : zd{};
-                    c3ma(acc[c], ao[ks], bo);
+                    const zd bo = (lcol < A.E && bc[ks] >= 0) ? ldzr<RE>(XB + (size_t)lcol * A.nBp + bc[ks]) : zd{};
+                    c3ma<RE>(acc[c], ao[ks], bo);
                 }
             }
         }
@@ -1009,9 +1056,9 @@ __global__ void __launch_bounds__(256) crt_cond_gemm_kernel(GemmArgs A) {
         for (int v = 0; v < 4; ++v) {
             const int i = 16 * rt + 4 * v + rq;
             if (i >= plen) continue;
-            const zd u = uc >= 0 ? X[(size_t)uc * A.ldx + i] : zd{};
+            const zd u = uc >= 0 ? ldzr<RE>(X + (size_t)uc * A.ldx + i) : zd{};
             const double f = A.fI[(size_t)kk * A.nP * A.fstrideC + r0 + i];
-            const zd zx = c3get(acc[c], v);
+            const zd zx = c3get<RE>(acc[c], v);
             A.kpotN[((size_t)A.nodeI[r0 + i] * A.K + k) * A.E + lcol] = zd{(u.re - zx.re) * f, (u.im - zx.im) * f};
         }
     }
@@ -1045,33 +1092,39 @@ int crt_launch_cond_rhs(hipStream_t st, int kc, int k0, int nent, int nnzF, int 
 }
 
 template <int D>
-static int launch_trsm(hipStream_t st, TrsmArgs a, bool bwd) {
+static int launch_trsm(hipStream_t st, TrsmArgs a, bool bwd, bool re) {
     a.nvirt = (a.ntile * a.nsys + 7) / 8 * 8;
-    if (bwd) hipLaunchKernelGGL((crt_cond_trsm_kernel<D, true>), dim3(a.nvirt), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((crt_cond_trsm_kernel<D, false>), dim3(a.nvirt), dim3(256), 0, st, a);
+    const dim3 gr(a.nvirt), bl(256);
+    if (re) {
+        if (bwd) hipLaunchKernelGGL((crt_cond_trsm_kernel<D, true, true>), gr, bl, 0, st, a);
+        else hipLaunchKernelGGL((crt_cond_trsm_kernel<D, false, true>), gr, bl, 0, st, a);
+    } else {
+        if (bwd) hipLaunchKernelGGL((crt_cond_trsm_kernel<D, true, false>), gr, bl, 0, st, a);
+        else hipLaunchKernelGGL((crt_cond_trsm_kernel<D, false, false>), gr, bl, 0, st, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 int crt_launch_cond_trsm(hipStream_t st, int D, int bwd, int nblk, int ldx, int ntile, int nsys, int nP,
                          const zd *panels, const zd *dinv, zd *X, const int *tptr, const int *erow, const int *ecol,
-                         const zd *rv, int nent, const int *j0) {
+                         const zd *rv, int nent, const int *j0, int re) {
     TrsmArgs a{nblk, ldx, ntile, nsys, nP, 0, panels, dinv, X, tptr, erow, ecol, rv, nent, j0};
     switch (D) {
-        case 2: return launch_trsm<2>(st, a, bwd);
-        case 3: return launch_trsm<3>(st, a, bwd);
-        case 4: return launch_trsm<4>(st, a, bwd);
-        case 5: return launch_trsm<5>(st, a, bwd);
-        case 6: return launch_trsm<6>(st, a, bwd);
-        case 7: return launch_trsm<7>(st, a, bwd);
-        case 8: return launch_trsm<8>(st, a, bwd);
-        case 9: return launch_trsm<9>(st, a, bwd);
-        case 10: return launch_trsm<10>(st, a, bwd);
-        case 11: return launch_trsm<11>(st, a, bwd);
-        case 12: return launch_trsm<12>(st, a, bwd);
-        case 13: return launch_trsm<13>(st, a, bwd);
-        case 14: return launch_trsm<14>(st, a, bwd);
-        case 15: return launch_trsm<15>(st, a, bwd);
-        case 16: return launch_trsm<16>(st, a, bwd);
+        case 2: return launch_trsm<2>(st, a, bwd, re);
+        case 3: return launch_trsm<3>(st, a, bwd, re);
+        case 4: return launch_trsm<4>(st, a, bwd, re);
+        case 5: return launch_trsm<5>(st, a, bwd, re);
+        case 6: return launch_trsm<6>(st, a, bwd, re);
+        case 7: return launch_trsm<7>(st, a, bwd, re);
+        case 8: return launch_trsm<8>(st, a, bwd, re);
+        case 9: return launch_trsm<9>(st, a, bwd, re);
+        case 10: return launch_trsm<10>(st, a, bwd, re);
+        case 11: return launch_trsm<11>(st, a, bwd, re);
+        case 12: return launch_trsm<12>(st, a, bwd, re);
+        case 13: return launch_trsm<13>(st, a, bwd, re);
+        case 14: return launch_trsm<14>(st, a, bwd, re);
+        case 15: return launch_trsm<15>(st, a, bwd, re);
+        case 16: return launch_trsm<16>(st, a, bwd, re);
         default: return CRT_ERR_UNSUPPORTED;
     }
 }
@@ -1105,7 +1158,8 @@ int crt_launch_cond_bval(hipStream_t st, const DevGrid &g, int nsys, int E, int 
 template <int NI, bool PRE, int NW = DNW>
 static int launch_dense(hipStream_t st, const DenseArgs &a, int nsys) {
     const size_t lds = (size_t)a.nBp * sizeof(zd);
-    hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE, NW>), dim3(nsys), dim3(64 * NW), lds, st, a);
+    if (a.g.dc) hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE, NW, true>), dim3(nsys), dim3(64 * NW), lds, st, a);
+    else hipLaunchKernelGGL((crt_cond_dense_kernel<NI, PRE, NW, false>), dim3(nsys), dim3(64 * NW), lds, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
@@ -1151,11 +1205,12 @@ int crt_launch_cond_dense(hipStream_t st, const DevGrid &g, int pre, int nG, int
 int crt_launch_cond_gemm(hipStream_t st, int nsys, int nP, int k0, int ldx, int ntile, int ncolP, int nBp, int E,
                          int K, int fstrideC, const zd *X, const zd *XB, const double *fI, const int *nodeI,
                          const int *roff, const int *plen, const int *nbc, const int *colmap, const int *ucol,
-                         zd *kpotN) {
+                         zd *kpotN, int re) {
     const int rgroups = (ldx / T16 + 3) / 4, cgroups = ((E + 15) / 16 + 3) / 4;
     GemmArgs a{nsys, nP,   k0,  ldx,  ntile, ncolP, nBp,    E,    K,  fstrideC, rgroups, cgroups,
                X,    XB,   fI,  nodeI, roff, plen,  nbc,    colmap, ucol, kpotN};
     const long n = (long)rgroups * cgroups * nsys;
-    hipLaunchKernelGGL(crt_cond_gemm_kernel, dim3((unsigned)n), dim3(256), 0, st, a);
+    if (re) hipLaunchKernelGGL(crt_cond_gemm_kernel<true>, dim3((unsigned)n), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(crt_cond_gemm_kernel<false>, dim3((unsigned)n), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }

@@ -95,13 +95,21 @@ def test_condensed_oracle_c0(oracle_lib):
     assert e <= TOL
 
 
-def test_condensed_dc():
-    w = configs.c0()
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_condensed_dc(name):
+    """The DC kind: the condensed kernels run real arithmetic on the real parts (one MFMA
+    chain per product), the band method the complex kernels on (Re sigma, 0); c1 has
+    separators, so the prefix kernels run too.  The imaginary parts stay exactly 0."""
+    w = configs.c0() if name == "c0" else configs.c1()
     b, c = _pair(w, kind=1, sens=False)
     ek = _rel_rows(c["kpot"], b["kpot"])
     ev = float(np.max(np.abs(c["volt"] - b["volt"]) / np.abs(b["volt"])))
-    print(f"[condensed] dc: kpot {ek:.2e} volt {ev:.2e}", flush=True)
+    tb = sum(b["times"][p] for p in ("diag", "factor", "solve"))
+    tc = sum(c["times"][p] for p in ("diag", "factor", "solve", "prefix", "schur", "product"))
+    print(f"[condensed] dc {name}: kpot {ek:.2e} volt {ev:.2e}; forward band {tb:.2f} ms, condensed {tc:.2f} ms "
+          f"({c['times']})", flush=True)
     assert ek <= TOL and ev <= TOL
+    assert not np.any(c["kpot"].imag)
 
 
 def test_condensed_boundary_electrode(oracle_lib):
