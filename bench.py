@@ -765,16 +765,18 @@ def main():
         nS = info["separator"]
         JS, nt = (nS + 15) // 16, (nG + 15) // 16
         NW = 4 if nt <= 20 else 8  # waves per source system (the library's choice)
-        NI, NIp = (nt + NW - 1) // NW, (JS + nt + 7) // 8
+        NI, NIp = (nt + NW - 1) // NW, (JS + 7) // 8  # the prefix kernel holds the separator block alone
+        re = "false"  # the complex kind (the DC kind runs the real instantiations)
         ent = {
             "factor": (f"crt_factor16{'la' if DI <= 10 else 'st'}_kernel<{DI}>",
                        K * 8.0 * nI * mbI * (mbI + 1) / 2.0, K),
-            "solve": (f"crt_cond_trsm_kernel<{DI}, false> + crt_cond_trsm_kernel<{DI}, true>",
+            "solve": (f"crt_cond_trsm_kernel<{DI}, false, {re}> + crt_cond_trsm_kernel<{DI}, true, {re}>",
                       K * (nG + E) * 2.0 * 8.0 * nI * mbI, K * (nG + E)),
-            "prefix": (f"crt_cond_dense_kernel<{NIp}, true, 8>", K * 4.0 / 3.0 * ((nS + nG) ** 3 - nG ** 3), K),
-            "schur": (f"crt_cond_dense_kernel<{NI}, false, {NW}>", K * E * (4.0 / 3.0 * nG ** 3 + 16.0 * nG ** 2),
-                      K * E),
-            "product": ("crt_cond_gemm_kernel", K * E * 8.0 * nI * nG, K * E),
+            "prefix": (f"crt_cond_dense_kernel<{NIp}, true, 8, {re}>", K * 4.0 / 3.0 * ((nS + nG) ** 3 - nG ** 3),
+                       K),
+            "schur": (f"crt_cond_dense_kernel<{NI}, false, {NW}, {re}>",
+                      K * E * (4.0 / 3.0 * nG ** 3 + 16.0 * nG ** 2), K * E),
+            "product": (f"crt_cond_gemm_kernel<{re}>", K * E * 8.0 * nI * nG, K * E),
         }
         out = {}
         for key, (kernel, flop, units) in ent.items():
