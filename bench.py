@@ -346,6 +346,22 @@ def gn_step(ctx, w, ltri: int):
                          "ncg": rd["ncg"], "cg_ms_per_iteration": round(itd, 4),
                          "cg_matvec_GBps": round(2.0 * 8.0 * v_true.size * manz / (itd * 1e-3) / 1e9, 1)
                          if itd > 0 else None}
+        # the same DC update with the FP32 copy of the real J and FSAI on the wider pattern
+        ctx.set_cg_precision(32)
+        ctx.set_cg_preconditioner(2)
+        try:
+            t1 = time.perf_counter()
+            rdf = ctx.update(lam_dc, 1e-12, ncgmax, wmatd, wdfak, dat, par)
+            device_sync()
+            tuf = time.perf_counter() - t1
+            itf = ctx.times()["cg_matvec"]
+        finally:
+            ctx.set_cg_preconditioner(0)
+            ctx.set_cg_precision(64)
+        dc["dc_step_fp32j_fsai"] = {
+            "update_ms": round(1e3 * tuf, 3), "ncg": rdf["ncg"], "cg_ms_per_iteration": round(itf, 4),
+            "step_ms": round(1e3 * (td["forward"] + td["sens"] + td["bsmatm_blam0"] + tuf), 3),
+            "dpar_rel_diff_vs_jacobi": float(np.max(np.abs(rdf["dpar"] - rd["dpar"])) / np.max(np.abs(rd["dpar"])))}
         ctx.set_inversion_kind(2)
         ctx.set_sigma(sig0)
         ctx.forward()

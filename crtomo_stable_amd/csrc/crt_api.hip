@@ -92,6 +92,7 @@ int crt_launch_kpot_ref(hipStream_t, int, int, int, const zd *, zd *);
 // CG (crt_cg.hip)
 int crt_cg_geometry(int, int, int *, int *, int *, int *);
 int crt_launch_cg_j32(hipStream_t, size_t, const zd *, float2 *);
+int crt_launch_cg_jr32(hipStream_t, size_t, const double *, float *);
 int crt_launch_cg_fsai(hipStream_t, const CGArgs &, zd *);
 // learned preconditioner (crt_learned_pc.hip)
 struct PcCell {
@@ -271,6 +272,8 @@ struct crt_ctx {
     int inv_kind = 0;
     DBuf<double> d_sensr;   // real J of the DC / FPI CG (sensdc, or DBLE(sens))
     bool sensr_ok = false;
+    DBuf<float> d_sensr32;  // its FP32 copy for the iterations
+    bool sensr32_ok = false;
     DBuf<double> d_re;      // real parts for the DC getters
     int cg_prec = 0;                      // 0 Jacobi scaling only, 1/2 FSAI patterns
     std::vector<int> h_nachbar;           // host copy of the neighbour table (ltri 1/2)
@@ -1119,7 +1122,7 @@ int crt_set_inversion_kind(crt_ctx *c, int kind) {
     if (kind != c->inv_kind) {
         // the conductivity and everything derived from it belong to the previous kind
         c->have_sigma = c->have_kpot = c->have_volt = c->have_sens = false;
-        c->sens32_ok = c->sensr_ok = false;
+        c->sens32_ok = c->sensr_ok = c->sensr32_ok = false;
     }
     c->inv_kind = kind;
     return 0;
@@ -2180,6 +2183,7 @@ int crt_sens(crt_ctx *c, int mode, const double *strom) {
     c->have_sens = true;
     c->sens32_ok = false;
     c->sensr_ok = false;
+    c->sensr32_ok = false;
     c->sens_nanz = c->nanz;
     c->sens_manz = c->elanz;
     return 0;
@@ -2214,6 +2218,7 @@ int crt_set_sens(crt_ctx *c, int nanz, int manz, const double *sens) {
     c->have_sens = true;
     c->sens32_ok = false;
     c->sensr_ok = false;
+    c->sensr32_ok = false;
     return 0;
 }
 
@@ -2326,17 +2331,29 @@ static int cg_prepare(crt_ctx *c, int nanz, int manz, int ncgmax, double lam, do
     a.J = c->d_sens.p; a.w = c->d_w.p; a.smatm = c->d_smatm.p; a.nachbar = c->d_nachbar.p;
     a.J32 = nullptr;
     a.Jr = nullptr;
+    a.Jr32 = nullptr;
     a.part = c->inv_kind;
     if (c->inv_kind != 0) {
-        // DC / FPI: the reference's Jacobi-scaled CG on a real J (src/cg_mod.f90:106-196)
-        if (c->cg_jbits != 64 || c->cg_prec != 0) return CRT_ERR_UNSUPPORTED;
+        // DC / FPI: the reference's Jacobi-scaled CG on a real J (src/cg_mod.f90:106-196);
+        // the FP32 copy and the FSAI patterns as for the complex inversion (the learned
+        // preconditioner is trained on complex systems only)
+        if (c->cg_prec >= 3) return CRT_ERR_UNSUPPORTED;
         const size_t n = (size_t)nanz * manz;
         if (!c->sensr_ok) {
             if ((e = c->d_sensr.ensure(n))) return e;
             if ((e = crt_launch_real_part(c->stream, n, c->d_sens.p, c->d_sensr.p))) return e;
             c->sensr_ok = true;
+            c->sensr32_ok = false;
         }
         a.Jr = c->d_sensr.p;
+        if (c->cg_jbits == 32) {
+            if (!c->sensr32_ok) {
+                if ((e = c->d_sensr32.ensure(n))) return e;
+                if ((e = crt_launch_cg_jr32(c->stream, n, c->d_sensr.p, c->d_sensr32.p))) return e;
+                c->sensr32_ok = true;
+            }
+            a.Jr32 = c->d_sensr32.p;
+        }
     } else if (c->cg_jbits == 32) {
         const size_t n = (size_t)nanz * manz;
         if (!c->sens32_ok) {
@@ -2563,6 +2580,7 @@ int crt_pc_weights(crt_ctx *c, float *get, const float *set) {
 int crt_pc_train(crt_ctx *c, int steps, double lr, double *loss_first, double *loss_last) {
     if (!c || steps < 1 || !(lr > 0.0)) return CRT_ERR_ARG;
     if (!c->have_cgsys || !c->have_sens || c->ltri != 0) return CRT_ERR_STATE;
+    if (c->inv_kind != 0) return CRT_ERR_UNSUPPORTED;  // trained on complex systems only
     CRT_HIP(hipSetDevice(c->device));
     int nanz = 0, manz = 0;
     if (int e0 = jsizes(c, true, &nanz, &manz)) return e0;
@@ -2719,9 +2737,13 @@ int crt_pc_data(crt_ctx *c, int *n, float *feat, float *tgt, float *mask) {
 int crt_set_cg_precision(crt_ctx *c, int jbits) {
     if (!c || (jbits != 64 && jbits != 32)) return CRT_ERR_ARG;
     c->cg_jbits = jbits;
-    if (jbits == 64) c->d_sens32.release();
+    if (jbits == 64) {
+        c->d_sens32.release();
+        c->d_sensr32.release();
+    }
     c->sens32_ok = false;
     c->sensr_ok = false;
+    c->sensr32_ok = false;
     return 0;
 }
 

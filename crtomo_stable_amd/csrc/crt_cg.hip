@@ -26,6 +26,7 @@ namespace {
 // Jacobian entry as complex FP64 from the FP64, the FP32 or the real copy
 __device__ __forceinline__ zd ldj(const zd *p) { return *p; }
 __device__ __forceinline__ zd ldj(const double *p) { return zd{*p, 0.0}; }
+__device__ __forceinline__ zd ldj(const float *p) { return zd{(double)*p, 0.0}; }
 __device__ __forceinline__ zd ldj(const float2 *p) {
     const float2 v = *p;
     return zd{(double)v.x, (double)v.y};
@@ -455,7 +456,9 @@ __global__ void __launch_bounds__(256) crt_cg_fsai_kernel(CGArgs A, zd *g) {
         const double wi = A.w[i];
         zd v[NP];
 #pragma unroll
-        for (int s = 0; s < NP; ++s) v[s] = pc[s] >= 0 ? A.J[(size_t)pc[s] * A.nanz + i] : zd{0.0, 0.0};
+        for (int s = 0; s < NP; ++s)
+            v[s] = pc[s] < 0 ? zd{0.0, 0.0}
+                             : (A.Jr ? zd{A.Jr[(size_t)pc[s] * A.nanz + i], 0.0} : A.J[(size_t)pc[s] * A.nanz + i]);
         int q = 0;
 #pragma unroll
         for (int a = 0; a < NP; ++a)
@@ -565,14 +568,18 @@ int crt_launch_cg_init(hipStream_t s, const CGArgs &a, const zd *bvec) {
 int crt_launch_cg_iter(hipStream_t s, const CGArgs &a) {
     hipLaunchKernelGGL(crt_cg_dir_kernel, dim3(a.n_rr), dim3(CG_BLK), 0, s, a);
     const dim3 gq((a.nanz + CG_BLK - 1) / CG_BLK, a.ncb);
-    if (a.Jr)
+    if (a.Jr32)
+        hipLaunchKernelGGL(crt_cg_jq_kernel<float>, gq, dim3(CG_BLK), 0, s, a, a.Jr32);
+    else if (a.Jr)
         hipLaunchKernelGGL(crt_cg_jq_kernel<double>, gq, dim3(CG_BLK), 0, s, a, a.Jr);
     else if (a.J32)
         hipLaunchKernelGGL(crt_cg_jq_kernel<float2>, gq, dim3(CG_BLK), 0, s, a, a.J32);
     else
         hipLaunchKernelGGL(crt_cg_jq_kernel<zd>, gq, dim3(CG_BLK), 0, s, a, a.J);
     hipLaunchKernelGGL(crt_cg_apred_kernel, dim3((a.nanz + 63) / 64), dim3(64 * CG_RED_WAVES), 0, s, a);
-    if (a.Jr)
+    if (a.Jr32)
+        hipLaunchKernelGGL(crt_cg_jhw_kernel<float>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.Jr32);
+    else if (a.Jr)
         hipLaunchKernelGGL(crt_cg_jhw_kernel<double>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.Jr);
     else if (a.J32)
         hipLaunchKernelGGL(crt_cg_jhw_kernel<float2>, dim3(a.n_dr1), dim3(CG_BLK), 0, s, a, a.J32);
@@ -610,6 +617,16 @@ __global__ void __launch_bounds__(256) crt_cg_j32_kernel(size_t n, const zd *J, 
     if (t >= n) return;
     const zd v = J[t];
     J32[t] = make_float2((float)v.re, (float)v.im);
+}
+
+__global__ void __launch_bounds__(256) crt_cg_jr32_kernel(size_t n, const double *Jr, float *Jr32) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) Jr32[t] = (float)Jr[t];
+}
+
+int crt_launch_cg_jr32(hipStream_t s, size_t n, const double *Jr, float *Jr32) {
+    hipLaunchKernelGGL(crt_cg_jr32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Jr, Jr32);
+    return hipGetLastError() == hipSuccess ? 0 : CRT_ERR_HIP;
 }
 
 int crt_launch_cg_j32(hipStream_t s, size_t n, const zd *J, float2 *J32) {

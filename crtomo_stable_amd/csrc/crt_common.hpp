@@ -195,6 +195,7 @@ struct CGArgs {
     const zd *J;            // (nanz, manz) column-major
     const float2 *J32;      // the same in complex FP32 for the iterations, or null (FP64)
     const double *Jr;       // real J of the DC / FPI branches (sensdc or DBLE(sens)), or null
+    const float *Jr32;      // the real J in FP32 for the iterations of those branches, or null
     int part;               // CG right-hand side: 0 bvec, 1 its real part (DC), 2 its imaginary part (FPI)
     const double *w;        // nanz: wmatd * wdfak
     const double *smatm;    // (manz, ncol) column-major

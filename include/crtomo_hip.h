@@ -173,7 +173,8 @@ int crt_update(crt_ctx *ctx, double lam, double eps, int ncgmax, double step, co
  * the reference's arithmetic) or 32 (complex FP32 copy of J, made once per Jacobian;
  * products accumulate and all inner products run in FP64; the RHS and cgfac pass
  * keeps FP64 J).  No counterpart in the reference: the mixed-precision form halves
- * the bytes of each iteration. */
+ * the bytes of each iteration.  The DC and final-phase-improvement kinds keep a real
+ * FP32 copy of their real J (4 bytes per entry). */
 int crt_set_cg_precision(crt_ctx *ctx, int jbits);
 
 /* Preconditioner of the CG iterations of crt_cjg / crt_update.  0 (default): the
@@ -186,7 +187,8 @@ int crt_set_cg_precision(crt_ctx *ctx, int jbits);
  * on the stencil with the coupling between the two neighbours neglected, no network,
  * no training).  The stop test stays on |r|^2 of the scaled
  * system, as in the reference; only the iteration count changes.  No counterpart
- * in the reference. */
+ * in the reference.  The DC and final-phase-improvement kinds take kinds 0-2 (their FSAI
+ * is built on the real J); 3 and 4 return 903 there. */
 int crt_set_cg_preconditioner(crt_ctx *ctx, int kind);
 
 /* Learned preconditioner (kind 3 of crt_set_cg_preconditioner, regular grids): a

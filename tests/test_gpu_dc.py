@@ -162,6 +162,51 @@ def test_dc_fpi_update(ctx, oracle_lib, kind, ltri, ncgmax, eps):
         ctx.set_inversion_kind(0)
 
 
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("ltri,prec", [(0, 2), (1, 1)])
+def test_dc_fpi_fast_cg_forms(ctx, oracle_lib, kind, ltri, prec):
+    """The CG forms without a counterpart in the reference on the real J of the DC and
+    final-phase-improvement branches: FSAI preconditioning built on the real J (pattern
+    2 on the regular grid, 1 on the triangle mesh) alone and with the FP32 copy of the
+    real J.  The converged update (eps = 1e-30) equals the oracle's cjggdc to 1e-10 with
+    FP64 J and to 1e-6 with the FP32 copy (the FP32 J products, as for the complex
+    inversion); FSAI needs fewer iterations than the Jacobi scaling at eps 1e-12; the
+    learned preconditioner is refused (903)."""
+    O = oracle_lib
+    w = _workload(ltri)
+    lam = 3.0
+    P = _cg_problem(O, w, ltri, kind)
+    cp = O.CGProblem(P["sens"], P["wmatd"], P["wdfak"], P["dat"], P["sa"], P["par"], P["smatm"], ltri, w.grid.nx,
+                     lam, 1e-30, 3000, nachbar=P["nachbar"], maxn=w.grid.maxn)
+    _, _, dpar_o, _, ncg_o = O.update_dc(cp, kind, P["sensdc"])
+    try:
+        ctx.set_inversion_kind(kind)
+        ctx.set_grid(w.grid)
+        ctx.set_sens(P["sensdc"] if kind == 1 else P["sens"])
+        ctx.set_regularization(ltri, P["smatm"], nx=w.grid.nx, nachbar=P["nachbar"])
+        args = (P["wmatd"], P["wdfak"], P["dat"], P["par"])
+        jac = ctx.update(lam, 1e-12, 400, *args, sigmaa=P["sa"])
+        ctx.set_cg_preconditioner(prec)
+        conv = ctx.update(lam, 1e-30, 3000, *args, sigmaa=P["sa"])
+        stop = ctx.update(lam, 1e-12, 400, *args, sigmaa=P["sa"])
+        ctx.set_cg_precision(32)
+        conv32 = ctx.update(lam, 1e-30, 3000, *args, sigmaa=P["sa"])
+        ctx.set_cg_preconditioner(3)
+        with pytest.raises(CRTError) as ei:
+            ctx.update(lam, 1e-12, 400, *args, sigmaa=P["sa"])
+        assert ei.value.code == 903
+    finally:
+        ctx.set_cg_preconditioner(0)
+        ctx.set_cg_precision(64)
+        ctx.set_inversion_kind(0)
+    e64, e32 = _rel(conv["dpar"], dpar_o), _rel(conv32["dpar"], dpar_o)
+    print(f"[dc/fpi fast cg] kind={kind} ltri={ltri} fsai {prec}: ncg {stop['ncg']} vs Jacobi {jac['ncg']}; "
+          f"converged {conv['ncg']} (FP64 J {e64:.2e}), {conv32['ncg']} (FP32 J {e32:.2e})", flush=True)
+    assert e64 <= 1e-10 and e32 <= 1e-6
+    assert np.all((conv32["dpar"].imag if kind == 1 else conv32["dpar"].real) == 0.0)
+    assert stop["ncg"] < jac["ncg"]
+
+
 @pytest.mark.parametrize("ltri", [0, 1])
 def test_fpi_zero_rhs(ctx, oracle_lib, ltri):
     """cjggdc tests its stop before the first step as well (src/cg_mod.f90:143-150):
