@@ -31,10 +31,14 @@ summation order moves the step size by a few 1e-3 from the third trial on; dc_pr
 CG stop test (eps 1e-4) lands on 27 against 28 iterations in the second trial of
 iteration 2; non_ordered_bc parts where mixed_surf does (the same grid, its boundary elements
 listed in another order).  frontcover parts at the first trial of iteration 2 with the same
-model step size and roughness but a 7 % larger misfit and 223 against 182 data excluded by
-the polarity check at iteration 2 (both 247 at iteration 1): the 2014 run handles the data
-whose sign changed during iteration 1 differently from src/chkpol.f90 as the driver follows
-it (called after the forward of a new iteration and again in its convergence test).  After
+model step size, roughness and CG count but a 7 % larger misfit and 223 against 182 data
+excluded by the polarity check at iteration 2.  Up to there the runs agree datum by datum:
+the polarity check after iteration 1 swaps and excludes the same 247 measurements (the
+committed run.ctr lists them), and the misfit table of iteration 1 in eps.ctr excludes the
+same ones.  The committed run is of an older version: its eps.ctr has five columns where
+src/dmisft.f90 writes eight, and its run.ctr prints step-length blocks src/ does not, so the
+treatment of the swapped data in its misfit (src/dmisft.f90's polarity test per datum) is
+the likely difference.  After
 the common prefix the run must still end normally, mixed_surf with as many iterations and a
 final misfit within 1 % of the committed one.  The committed
 runs are from the reference's 2014 master, whose column layout of the CG steps differs
