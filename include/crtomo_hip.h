@@ -107,9 +107,10 @@ int crt_set_sigma(crt_ctx *ctx, const double *sigma);
  * dpar = i*dpar for FPI (src/update.f90:302-304); crt_blam0 takes the sensdc branch for 1.
  * crt_buncert runs bata on the real Jacobian for 1 (src/bmcm_mod.f90:197-206) and the
  * complex branches for 2 (the caller passes lam_cri and wmatdr, :56-83).
- * Not for 1/2: bvolt/bsens (mode 0, DC), the FP32 and FSAI/learned CG
- * forms (903).  A change of kind discards sigma and its results:
- * call crt_set_sigma after it. */
+ * The condensed forward runs its kernels in real arithmetic for 1.  The FP32 copy of
+ * J and the FSAI preconditioners work on the real J for 1 and 2.  Not for 1/2:
+ * bvolt/bsens (mode 0, DC) and the learned preconditioner (903).  A change of kind
+ * discards sigma and its results: call crt_set_sigma after it. */
 int crt_set_inversion_kind(crt_ctx *ctx, int kind);
 
 /* The forward loop of src/fem.f90:328-397 / src/inv.f90:415-484 for every
