@@ -152,7 +152,7 @@ def cpu_baseline(w, threads: int, sens_meas: int, fwd_electrodes: int):
                 forward_s=fwd_full, sens_s=None, cpu_model=cpu_model())
 
 
-def other_rooflines(g, s, K, E, nsys, phase, bits):
+def other_rooflines(g, s, K, E, nsys, phase, bits, workload):
     """Algorithmic HBM bytes over the measured phase time (HIP events on the library's
     stream) for the HBM-bound kernels of one step: substitution, assembly (diagonal
     and value passes) and the Jacobian.  Bytes per unit as in the design notes:
@@ -173,14 +173,17 @@ def other_rooflines(g, s, K, E, nsys, phase, bits):
     asm_b = nsys * (8 * g.elanz * g.npk + 16 * g.elanz + 16 * nnz)
     sens_b = 16.0 * s.nanz * g.elanz + 16.0 * g.sanz * E * K + 8.0 * g.elanz * g.npk * K
     out = {}
-    for name, key, b in (("substitution (crt_solve16_kernel, first solve)", "solve", solve_b),
-                         ("assembly (crt_diag_kernel + crt_assemble_kernel)", "diag", asm_b),
-                         ("Jacobian (crt_sens_gp_kernel)", "sens", sens_b)):
+    nt = (E + 15) // 16  # the Jacobian kernel's electrode tiles
+    for name, key, b, kern in (("substitution (crt_solve16_kernel, first solve)", "solve", solve_b, None),
+                               ("assembly (crt_diag_kernel + crt_assemble_kernel)", "diag", asm_b, None),
+                               ("Jacobian (crt_sens_gp_kernel)", "sens", sens_b, f"crt_sens_gp_kernel<{nt}>")):
         ms = phase.get(key, 0.0)
         if ms > 0:
             gbs = b / (ms * 1e-3) / 1e9
             out[name] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=round(gbs / HBM_PEAK_GBS, 4), bytes=b, ms=round(ms, 4))
+            if kern:
+                out[name].update(kernel=kern, traffic=measured_traffic(kern, workload, "condensed"))
     return out
 
 
@@ -814,12 +817,12 @@ def main():
         top = max(rl, key=lambda k: rl[k]["launch_ms"])
         roofline = rl[top]
         others = {k: v for k, v in rl.items() if k != top}
-        others.update({k: v for k, v in other_rooflines(g, s, K, E, nsys, phase, args.forward_precision).items()
+        others.update({k: v for k, v in other_rooflines(g, s, K, E, nsys, phase, args.forward_precision, args.workload).items()
                        if k.startswith("Jacobian")})
     else:
         info = None
         roofline = band_roofline(phase)
-        others = other_rooflines(g, s, K, E, nsys, phase, args.forward_precision)
+        others = other_rooflines(g, s, K, E, nsys, phase, args.forward_precision, args.workload)
 
     out = {
         "metric": "forward+Jacobian solves/s",
