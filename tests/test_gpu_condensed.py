@@ -7,7 +7,8 @@ Jacobian agree to rounding: relative 1e-10 here (the FP64 tolerance of the forwa
 tests; measured ~1e-13).  Workloads: the 60x20 quad grid (30 electrodes), a
 triangle mesh of a quarter the cells, the 20k-triangle mesh with 64 electrodes, and the 400x100 quad grid
 with 96 electrodes (sampled systems against the oracle there), the DC kind on the
-quad grid, and a source electrode on a mixed-boundary node.
+60x20 and 400x100 quad grids and the triangle mesh, and a source electrode on a
+mixed-boundary node.
 """
 import numpy as np
 import pytest
@@ -95,12 +96,13 @@ def test_condensed_oracle_c0(oracle_lib):
     assert e <= TOL
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
+@pytest.mark.parametrize("name", ["c0", "c1", "c3"])
 def test_condensed_dc(name):
     """The DC kind: the condensed kernels run real arithmetic on the real parts (one MFMA
     chain per product), the band method the complex kernels on (Re sigma, 0); c1 has
-    separators, so the prefix kernels run too.  The imaginary parts stay exactly 0."""
-    w = configs.c0() if name == "c0" else configs.c1()
+    separators, so the prefix kernels run too; c3 (38 boundary tiles) runs the dense
+    kernel on eight waves per system.  The imaginary parts stay exactly 0."""
+    w = {"c0": configs.c0, "c1": configs.c1, "c3": configs.c3}[name]()
     b, c = _pair(w, kind=1, sens=False)
     ek = _rel_rows(c["kpot"], b["kpot"])
     ev = float(np.max(np.abs(c["volt"] - b["volt"]) / np.abs(b["volt"])))
